@@ -1,0 +1,212 @@
+/* so3g_hip.h -- C ABI of the MI355X projection engine (libso3g_hip.so).
+ *
+ * This is the drop-in boundary behind so3g_amd.ProjEng_* (the Python
+ * mirror of so3g's ProjectionEngine classes).  Each entry point replaces
+ * one method of the reference's boost-python engine; the method it
+ * replaces is cited on each declaration.
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers (hipMalloc'd or torch-ROCm
+ *     tensors), C-contiguous, unless stated otherwise.  Segment lists are
+ *     HOST pointers.
+ *   - bore:  (n_t, 4) float64 boresight quaternions (a,b,c,d) = (1,i,j,k)
+ *   - dets:  (n_det, 4) float64 detector offset quaternions
+ *   - resp:  (n_det, 2) float32 (T, P) responses
+ *   - signal:(n_det, n_t) float32
+ *   - det_weights: (n_det) float32 or NULL
+ *   - maps: float64.  Non-tiled flat (lead..., ny, nx); non-tiled HEALPix
+ *     (lead..., npix); lead = (ncomp) for maps, (ncomp, ncomp) for weight
+ *     maps.  Tiled maps are given as a device array `tiles` of n_tile
+ *     device pointers (NULL = inactive tile) plus a device array
+ *     `tile_dims` of n_tile (th, tw) pairs (HEALPix tiles: (1, npix/tile)).
+ *   - segs: host array of n_seg (det, t_start, t_end) int32 triples; every
+ *     listed segment is processed (the flattened form of so3g's
+ *     [bunch][thread][det] RangesInt32 lists, Projection.cxx:1856-1922).
+ *     segs == NULL means every sample of every detector.
+ *   - stream: a hipStream_t (NULL = default stream).  Calls are ordered
+ *     on that stream; calls that can raise an in-kernel condition (inactive
+ *     tile hit) synchronise the stream before returning.
+ *   - Return value: SO3G_OK or an SO3G_ERR_* code; so3g_last_error() then
+ *     holds a message (thread-local).
+ */
+#ifndef SO3G_HIP_H
+#define SO3G_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Coordinate systems: the CoordSys template options of
+ * ProjectionEngine<C,P,S> (Projection.cxx:152-159, 238-507). */
+enum {
+    SO3G_PROJ_FLAT = 0,
+    SO3G_PROJ_QUAT = 1,
+    SO3G_PROJ_CAR = 2,
+    SO3G_PROJ_CEA = 3,
+    SO3G_PROJ_ARC = 4,
+    SO3G_PROJ_SIN = 5,
+    SO3G_PROJ_TAN = 6,
+    SO3G_PROJ_ZEA = 7
+};
+
+/* Pixelizors: Pixelizor2_Flat (Projection.cxx:802-1166) and
+ * Pixelizor_Healpix (:510-800). */
+enum { SO3G_PIX_FLAT = 0, SO3G_PIX_HEALPIX = 1 };
+
+/* Status codes; the Python layer maps them onto the exception classes
+ * that so3g's exceptions.cxx:10-32 translates to. */
+enum {
+    SO3G_OK = 0,
+    SO3G_ERR_VALUE = 1,   /* ValueError  */
+    SO3G_ERR_RUNTIME = 2, /* RuntimeError (shape / tiling problems) */
+    SO3G_ERR_TYPE = 3,    /* TypeError */
+    SO3G_ERR_HIP = 4      /* HIP runtime failure (RuntimeError) */
+};
+
+/* Sample-domain modes for so3g_sample_domains. */
+enum {
+    SO3G_DOMAIN_STRIPE = 0, /* pixelizor stripe() (pixel_ranges, map=None) */
+    SO3G_DOMAIN_MAP = 1,    /* thread map value   (pixel_ranges, map given) */
+    SO3G_DOMAIN_TILE = 2    /* tile -> domain table (tile_ranges) */
+};
+
+/* Precision modes for the so3g_to_map_prec / so3g_to_weights_prec study. */
+enum {
+    SO3G_PREC_REF = 0,  /* float32 products, float64 accumulation (so3g) */
+    SO3G_PREC_F32 = 1,  /* float32 products, float32 accumulation (map is float32) */
+    SO3G_PREC_F64 = 2   /* float64 products, float64 accumulation */
+};
+
+/* The template parameters of one ProjectionEngine<C,P,S> plus its
+ * pixelizor constructor arguments (Projection.cxx:821-829, 971-994,
+ * 534-542, 640-675). */
+typedef struct so3g_geom {
+    int32_t proj;          /* SO3G_PROJ_* (HEALPix engines use CAR) */
+    int32_t pixelizor;     /* SO3G_PIX_* */
+    int32_t tiled;         /* 0 / 1 */
+    int32_t interp;        /* 0 nearest, 1 bilinear (flat only) */
+    int32_t ncomp;         /* 1 (T), 2 (QU), 3 (TQU) */
+    int32_t naxis[2];      /* flat: ny, nx */
+    double cdelt[2];       /* flat: dy, dx (radians) */
+    double crpix[2];       /* flat: iy0, ix0 */
+    int32_t tile_shape[2]; /* flat tiled: ty, tx */
+    int32_t nside;         /* HEALPix */
+    int32_t nest;          /* HEALPix: 1 NEST, 0 RING */
+    int32_t nside_tile;    /* HEALPix tiled */
+    int32_t allow_nside_max; /* opt-in: accept nside == 8192 */
+    /* Derived by so3g_geom_init: */
+    int32_t n_tile;        /* tiles (0 if not tiled) */
+    int32_t index_count;   /* P::index_count */
+    int64_t npix;          /* pixels per map plane (non-tiled) */
+} so3g_geom;
+
+/* Library set-up: selects the device, builds the 16384-entry asin/atan2
+ * interpolation tables on the host with the C library (as
+ * so_linterp.h:35-56 does at static init) and uploads them.  Idempotent. */
+int so3g_init(int device);
+const char *so3g_last_error(void);
+int so3g_abi_version(void);
+/* Copies the host-built tables (2 x 16384 float64) for verification. */
+int so3g_get_tables(double *asin_vals, double *atan2_vals);
+
+/* Validates the geometry (check_nside, Projection.cxx:607-614, 785-792;
+ * nside_tile <= nside, :670-674; RING + tiles, :646-650) and fills the
+ * derived fields.  Replaces the engine constructors. */
+int so3g_geom_init(so3g_geom *g);
+
+/* ProjectionEngine::coords (Projection.cxx:1316-1349).
+ * coords: (n_det, n_t, 4) float64. */
+int so3g_coords(const so3g_geom *g, const double *bore, int64_t n_t,
+                const double *dets, int64_t n_det, double *coords, void *stream);
+
+/* ProjectionEngine::pixels (Projection.cxx:1351-1385).
+ * pixel: (n_det, n_t, index_count) int32.  Off-map samples get -1 in slot
+ * 0 and keep the previous sample's other slots (initially 0). */
+int so3g_pixels(const so3g_geom *g, const double *bore, int64_t n_t,
+                const double *dets, int64_t n_det, int32_t *pixel, void *stream);
+
+/* ProjectionEngine::pointing_matrix (Projection.cxx:1391-1438).
+ * pixel as above; spin_proj: (n_det, n_t, ncomp) float32. */
+int so3g_pointing_matrix(const so3g_geom *g, const double *bore, int64_t n_t,
+                         const double *dets, int64_t n_det, const float *resp,
+                         int32_t *pixel, float *spin_proj, void *stream);
+
+/* ProjectionEngine::from_map (Projection.cxx:1728-1770).  Accumulates into
+ * signal; with accumulate == 0 the signal is overwritten instead (the
+ * result equals accumulating into zeros). */
+int so3g_from_map(const so3g_geom *g, const double *map, const double *const *tiles,
+                  const int32_t *tile_dims, const double *bore, int64_t n_t,
+                  const double *dets, int64_t n_det, const float *resp,
+                  float *signal, int accumulate, void *stream);
+
+/* ProjectionEngine::to_map (Projection.cxx:1924-1966, 1772-1811). */
+int so3g_to_map(const so3g_geom *g, double *map, double *const *tiles,
+                const int32_t *tile_dims, const double *bore, int64_t n_t,
+                const double *dets, int64_t n_det, const float *resp,
+                const float *signal, const float *det_weights,
+                const int32_t *segs, int64_t n_seg, void *stream);
+
+/* ProjectionEngine::to_weight_map (Projection.cxx:1968-2010, 1813-1854).
+ * Only the upper triangle (j >= i) of the (ncomp, ncomp) block is filled. */
+int so3g_to_weights(const so3g_geom *g, double *map, double *const *tiles,
+                    const int32_t *tile_dims, const double *bore, int64_t n_t,
+                    const double *dets, int64_t n_det, const float *resp,
+                    const float *det_weights, const int32_t *segs, int64_t n_seg,
+                    void *stream);
+
+/* Precision-study variants of to_map / to_weights (no counterpart in the
+ * reference, whose arithmetic is SO3G_PREC_REF).  For SO3G_PREC_F32 the
+ * map (and tiles) are float32 arrays passed through the same pointers. */
+int so3g_to_map_prec(const so3g_geom *g, void *map, void *const *tiles,
+                     const int32_t *tile_dims, const double *bore, int64_t n_t,
+                     const double *dets, int64_t n_det, const float *resp,
+                     const float *signal, const float *det_weights, int prec,
+                     void *stream);
+int so3g_to_weights_prec(const so3g_geom *g, void *map, void *const *tiles,
+                         const int32_t *tile_dims, const double *bore, int64_t n_t,
+                         const double *dets, int64_t n_det, const float *resp,
+                         const float *det_weights, int prec, void *stream);
+
+/* ProjectionEngine::tile_hits (Projection.cxx:1543-1601).  hits: n_tile
+ * int32, overwritten. */
+int so3g_tile_hits(const so3g_geom *g, const double *bore, int64_t n_t,
+                   const double *dets, int64_t n_det, int32_t *hits, void *stream);
+
+/* Per-sample domain for pixel_ranges / tile_ranges (Projection.cxx:
+ * 1440-1541, 1606-1705): out (n_det, n_t) int32 = the common domain of the
+ * sample's points, n_domain if they disagree, -1 if the sample has none.
+ * The run-length grouping into ranges is done by the caller. */
+int so3g_sample_domains(const so3g_geom *g, const double *bore, int64_t n_t,
+                        const double *dets, int64_t n_det, int mode, int n_domain,
+                        const double *tmap, const double *const *ttiles,
+                        const int32_t *tile_dims, const int32_t *tile_dom,
+                        int32_t *out, void *stream);
+
+/* ProjEng_Precomp (Projection.cxx:2012-2253): projection with a cached
+ * pointing matrix.  pixel: (n_det, n_t, k) int32 (k = 2 non-tiled flat,
+ * 3 tiled flat, 1 for a 1-D (ncomp, npix) map); spin: (n_det, n_t, nc).
+ * Non-tiled maps: map_shape = (ny, nx) (k=2) or (npix, 1) (k=1). */
+int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
+                          const double *const *tiles, const int32_t *tile_dims,
+                          const int32_t *pixel, int32_t k, const float *spin,
+                          int32_t nc, int64_t n_det, int64_t n_t, float *signal,
+                          int accumulate, void *stream);
+int so3g_precomp_to_map(double *map, const int64_t *map_shape, double *const *tiles,
+                        const int32_t *tile_dims, const int32_t *pixel, int32_t k,
+                        const float *spin, int32_t nc, int64_t n_det, int64_t n_t,
+                        const float *signal, const float *det_weights,
+                        const int32_t *segs, int64_t n_seg, void *stream);
+int so3g_precomp_to_weights(double *map, const int64_t *map_shape,
+                            double *const *tiles, const int32_t *tile_dims,
+                            const int32_t *pixel, int32_t k, const float *spin,
+                            int32_t nc, int64_t n_det, int64_t n_t,
+                            const float *det_weights, const int32_t *segs,
+                            int64_t n_seg, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SO3G_HIP_H */

@@ -1,0 +1,593 @@
+// capi.hip -- the extern "C" boundary of libso3g_hip.so (declared in
+// include/so3g_hip.h): argument validation with the reference's error
+// classes, engine-geometry set-up, trig-table construction, segment
+// chunking, kernel dispatch and the device error flags.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/so3g_hip.h"
+#include "launch.h"
+
+namespace so3g {
+extern template int dispatch<PROJ_FLAT>(const OpArgs &);
+extern template int dispatch<PROJ_QUAT>(const OpArgs &);
+extern template int dispatch<PROJ_CAR>(const OpArgs &);
+extern template int dispatch<PROJ_CEA>(const OpArgs &);
+extern template int dispatch<PROJ_ARC>(const OpArgs &);
+extern template int dispatch<PROJ_SIN>(const OpArgs &);
+extern template int dispatch<PROJ_TAN>(const OpArgs &);
+extern template int dispatch<PROJ_ZEA>(const OpArgs &);
+}  // namespace so3g
+
+using namespace so3g;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                          \
+    do {                                                                       \
+        hipError_t _e = (expr);                                                \
+        if (_e != hipSuccess)                                                  \
+            return fail(SO3G_ERR_HIP, std::string("HIP error in ") + #expr +   \
+                                          ": " + hipGetErrorString(_e));       \
+    } while (0)
+
+struct State {
+    int device = -1;
+    std::vector<double> asin_h, atan2_h;
+    double *asin_d = nullptr, *atan2_d = nullptr;
+    int *flags_d = nullptr;
+    int *flags_h = nullptr;     // pinned
+    Chunk *chunks_d = nullptr;
+    size_t chunk_cap = 0;
+};
+State g_state;
+std::mutex g_mutex;
+
+// The 16384-entry tables of so_linterp.h:35-39 (asin(i*dx)) and :51-56
+// (atan2(i*dx, 1.)), dx = 1./(n-1), evaluated with the host C library.
+void build_tables(std::vector<double> &as, std::vector<double> &at) {
+    const int n = kTrigN;
+    as.resize(n);
+    at.resize(n);
+    const double dx = 1. / (n - 1);
+    for (int i = 0; i < n; i++) as[i] = asin(i * dx);
+    for (int i = 0; i < n; i++) at[i] = atan2(i * dx, 1.);
+}
+
+__global__ void k_zero_flags(int *flags) {
+    if (threadIdx.x < FLAG_COUNT) flags[threadIdx.x] = 0;
+}
+
+int ensure_init() {
+    if (g_state.device >= 0) return SO3G_OK;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    return so3g_init(dev);
+}
+
+int geom_to_launch(const so3g_geom *g, Geo &geo, int &pk) {
+    if (g == nullptr) return fail(SO3G_ERR_VALUE, "geometry is NULL");
+    if (g->index_count <= 0)
+        return fail(SO3G_ERR_VALUE, "geometry was not initialised (call so3g_geom_init)");
+    memset(&geo, 0, sizeof(geo));
+    geo.ncomp = g->ncomp;
+    geo.interp = g->interp;
+    if (g->pixelizor == SO3G_PIX_FLAT) {
+        geo.ny = g->naxis[0];
+        geo.nx = g->naxis[1];
+        geo.cdelt_y = g->cdelt[0];
+        geo.cdelt_x = g->cdelt[1];
+        geo.crpix_y = g->crpix[0];
+        geo.crpix_x = g->crpix[1];
+        geo.npix = (int64_t)geo.ny * geo.nx;
+        if (g->tiled) {
+            geo.ty = g->tile_shape[0];
+            geo.tx = g->tile_shape[1];
+            geo.n_tx = (geo.nx + geo.tx - 1) / geo.tx;
+        }
+        pk = g->tiled ? PK_FLAT_TILED : PK_FLAT;
+    } else {
+        geo.nside = g->nside;
+        geo.nest = g->tiled ? 1 : g->nest;
+        geo.npix = 12LL * g->nside * g->nside;
+        if (g->tiled) geo.npt = (int32_t)(geo.npix / (12LL * g->nside_tile * g->nside_tile));
+        pk = g->tiled ? PK_HP_TILED : PK_HP;
+    }
+    return SO3G_OK;
+}
+
+int run_dispatch(int proj, const OpArgs &a) {
+    int r = -1;
+    switch (proj) {
+    case PROJ_FLAT: r = dispatch<PROJ_FLAT>(a); break;
+    case PROJ_QUAT: r = dispatch<PROJ_QUAT>(a); break;
+    case PROJ_CAR: r = dispatch<PROJ_CAR>(a); break;
+    case PROJ_CEA: r = dispatch<PROJ_CEA>(a); break;
+    case PROJ_ARC: r = dispatch<PROJ_ARC>(a); break;
+    case PROJ_SIN: r = dispatch<PROJ_SIN>(a); break;
+    case PROJ_TAN: r = dispatch<PROJ_TAN>(a); break;
+    case PROJ_ZEA: r = dispatch<PROJ_ZEA>(a); break;
+    }
+    if (r != 0) return fail(SO3G_ERR_VALUE, "no kernel for this engine configuration");
+    HIP_TRY(hipGetLastError());
+    return SO3G_OK;
+}
+
+// Splits host segments (det, t0, t1) into chunks of at most kBlock samples
+// and uploads them.  Returns the chunk count in *n_chunks.
+int upload_chunks(const int32_t *segs, int64_t n_seg, int64_t n_det, int64_t n_t,
+                  hipStream_t stream, int64_t *n_chunks) {
+    std::vector<Chunk> ch;
+    for (int64_t i = 0; i < n_seg; i++) {
+        const int32_t d = segs[3 * i], t0 = segs[3 * i + 1], t1 = segs[3 * i + 2];
+        if (d < 0 || d >= n_det)
+            return fail(SO3G_ERR_RUNTIME, "segment detector index out of range");
+        if (t0 < 0 || t1 > n_t || t0 > t1)
+            return fail(SO3G_ERR_RUNTIME, "segment sample range out of bounds");
+        for (int32_t s = t0; s < t1; s += kBlock)
+            ch.push_back(Chunk{d, s, std::min<int32_t>(s + kBlock, t1)});
+    }
+    *n_chunks = (int64_t)ch.size();
+    if (ch.empty()) return SO3G_OK;
+    if (ch.size() > g_state.chunk_cap) {
+        if (g_state.chunks_d) HIP_TRY(hipFree(g_state.chunks_d));
+        g_state.chunks_d = nullptr;
+        size_t cap = std::max<size_t>(ch.size(), 1 << 16);
+        HIP_TRY(hipMalloc(&g_state.chunks_d, cap * sizeof(Chunk)));
+        g_state.chunk_cap = cap;
+    }
+    HIP_TRY(hipMemcpyAsync(g_state.chunks_d, ch.data(), ch.size() * sizeof(Chunk),
+                           hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return SO3G_OK;
+}
+
+int zero_flags(hipStream_t stream) {
+    hipLaunchKernelGGL(k_zero_flags, dim3(1), dim3(64), 0, stream, g_state.flags_d);
+    HIP_TRY(hipGetLastError());
+    return SO3G_OK;
+}
+
+// Synchronises and reports an inactive-tile hit the way the reference's
+// tiling_exception does (a RuntimeError).
+int check_tile_flag(hipStream_t stream) {
+    HIP_TRY(hipMemcpyAsync(g_state.flags_h, g_state.flags_d, FLAG_COUNT * sizeof(int),
+                           hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (g_state.flags_h[FLAG_TILE])
+        return fail(SO3G_ERR_RUNTIME,
+                    "Tiling problem: Attempted pointing operation on non-instantiated tile.");
+    return SO3G_OK;
+}
+
+int base_args(const so3g_geom *g, const double *bore, int64_t n_t, const double *dets,
+              int64_t n_det, void *stream, OpArgs &a) {
+    int r = ensure_init();
+    if (r) return r;
+    memset(&a, 0, sizeof(a));
+    r = geom_to_launch(g, a.g, a.pk);
+    if (r) return r;
+    if (n_t < 0 || n_det < 0) return fail(SO3G_ERR_RUNTIME, "negative sample or detector count");
+    if (n_t > INT32_MAX) return fail(SO3G_ERR_RUNTIME, "n_t exceeds int32 range");
+    a.in.bore = bore;
+    a.in.dets = dets;
+    a.in.n_t = n_t;
+    a.in.n_det = n_det;
+    a.in.tabs = Tabs{g_state.asin_d, g_state.atan2_d};
+    // Detectors per block: at least 4 (boresight reuse), more when the
+    // detector count would overflow gridDim.y.
+    a.in.dpb = (int32_t)std::max<int64_t>(4, (n_det + 65534) / 65535);
+    a.interp = g->interp;
+    a.nc = g->ncomp;
+    a.stream = (hipStream_t)stream;
+    a.flags = g_state.flags_d;
+    return SO3G_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int so3g_abi_version(void) { return 1; }
+
+const char *so3g_last_error(void) { return g_err.c_str(); }
+
+int so3g_init(int device) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (g_state.device >= 0) {
+        if (device != g_state.device)
+            return fail(SO3G_ERR_RUNTIME, "library already initialised on another device");
+        return SO3G_OK;
+    }
+    HIP_TRY(hipSetDevice(device));
+    build_tables(g_state.asin_h, g_state.atan2_h);
+    HIP_TRY(hipMalloc(&g_state.asin_d, kTrigN * sizeof(double)));
+    HIP_TRY(hipMalloc(&g_state.atan2_d, kTrigN * sizeof(double)));
+    HIP_TRY(hipMemcpy(g_state.asin_d, g_state.asin_h.data(), kTrigN * sizeof(double),
+                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g_state.atan2_d, g_state.atan2_h.data(), kTrigN * sizeof(double),
+                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&g_state.flags_d, FLAG_COUNT * sizeof(int)));
+    HIP_TRY(hipMemset(g_state.flags_d, 0, FLAG_COUNT * sizeof(int)));
+    HIP_TRY(hipHostMalloc(&g_state.flags_h, FLAG_COUNT * sizeof(int), hipHostMallocDefault));
+    g_state.device = device;
+    return SO3G_OK;
+}
+
+int so3g_get_tables(double *asin_vals, double *atan2_vals) {
+    std::vector<double> as, at;
+    build_tables(as, at);
+    memcpy(asin_vals, as.data(), kTrigN * sizeof(double));
+    memcpy(atan2_vals, at.data(), kTrigN * sizeof(double));
+    return SO3G_OK;
+}
+
+int so3g_geom_init(so3g_geom *g) {
+    if (g == nullptr) return fail(SO3G_ERR_VALUE, "geometry is NULL");
+    if (g->proj < SO3G_PROJ_FLAT || g->proj > SO3G_PROJ_ZEA)
+        return fail(SO3G_ERR_VALUE, "unknown coordinate system");
+    if (g->ncomp < 1 || g->ncomp > 3) return fail(SO3G_ERR_VALUE, "ncomp must be 1, 2 or 3");
+    if (g->interp < 0 || g->interp > 1) return fail(SO3G_ERR_VALUE, "unknown interpolation");
+    g->n_tile = 0;
+    if (g->pixelizor == SO3G_PIX_FLAT) {
+        g->index_count = g->tiled ? 3 : 2;
+        g->npix = (int64_t)g->naxis[0] * g->naxis[1];
+        if (g->tiled) {
+            if (g->tile_shape[0] <= 0 || g->tile_shape[1] <= 0)
+                return fail(SO3G_ERR_VALUE, "tile shape must be positive");
+            const int n_ty = (g->naxis[0] + g->tile_shape[0] - 1) / g->tile_shape[0];
+            const int n_tx = (g->naxis[1] + g->tile_shape[1] - 1) / g->tile_shape[1];
+            g->n_tile = n_ty * n_tx;
+        }
+        return SO3G_OK;
+    }
+    if (g->pixelizor != SO3G_PIX_HEALPIX) return fail(SO3G_ERR_VALUE, "unknown pixelizor");
+    if (g->proj != SO3G_PROJ_CAR)
+        return fail(SO3G_ERR_VALUE, "HEALPix engines use the CAR pointer");
+    if (g->interp != 0)
+        return fail(SO3G_ERR_VALUE, "Only 'nearest' interpolation is supported for Healpix");
+    // check_nside, Projection.cxx:607-614: 0 < nside < 8192, power of 2.
+    auto nside_ok = [&](int n) {
+        const int lim = g->allow_nside_max ? 8192 : 8191;
+        return n > 0 && n <= lim && (n & (n - 1)) == 0;
+    };
+    if (g->tiled && !g->nest) return fail(SO3G_ERR_VALUE, "RING not supported for tiled maps");
+    if (!nside_ok(g->nside))
+        return fail(SO3G_ERR_VALUE, "Invalid nside " + std::to_string(g->nside));
+    g->npix = 12LL * g->nside * g->nside;
+    if (g->tiled) {
+        if (!nside_ok(g->nside_tile))
+            return fail(SO3G_ERR_VALUE, "Invalid nside " + std::to_string(g->nside_tile));
+        if (g->nside_tile > g->nside)
+            return fail(SO3G_ERR_VALUE, "Invalid nside_tile " + std::to_string(g->nside_tile) +
+                                            " > nside " + std::to_string(g->nside));
+        g->n_tile = 12 * g->nside_tile * g->nside_tile;
+        g->index_count = 2;
+    } else {
+        g->index_count = 1;
+    }
+    return SO3G_OK;
+}
+
+int so3g_coords(const so3g_geom *g, const double *bore, int64_t n_t, const double *dets,
+                int64_t n_det, double *coords, void *stream) {
+    OpArgs a;
+    int r = base_args(g, bore, n_t, dets, n_det, stream, a);
+    if (r) return r;
+    if (n_t == 0 || n_det == 0) return SO3G_OK;
+    a.op = OP_COORDS;
+    a.coords_out = coords;
+    return run_dispatch(g->proj, a);
+}
+
+static int pixels_common(const so3g_geom *g, const double *bore, int64_t n_t,
+                         const double *dets, int64_t n_det, const float *resp,
+                         int32_t *pixel, float *spin, void *stream) {
+    OpArgs a;
+    int r = base_args(g, bore, n_t, dets, n_det, stream, a);
+    if (r) return r;
+    if (n_t == 0 || n_det == 0) return SO3G_OK;
+    r = zero_flags(a.stream);
+    if (r) return r;
+    a.op = spin ? OP_PMAT : OP_PIXELS;
+    a.resp = resp;
+    a.pix_out = pixel;
+    a.spin_out = spin;
+    return run_dispatch(g->proj, a);
+}
+
+int so3g_pixels(const so3g_geom *g, const double *bore, int64_t n_t, const double *dets,
+                int64_t n_det, int32_t *pixel, void *stream) {
+    return pixels_common(g, bore, n_t, dets, n_det, nullptr, pixel, nullptr, stream);
+}
+
+int so3g_pointing_matrix(const so3g_geom *g, const double *bore, int64_t n_t,
+                         const double *dets, int64_t n_det, const float *resp,
+                         int32_t *pixel, float *spin_proj, void *stream) {
+    if (spin_proj == nullptr) return fail(SO3G_ERR_VALUE, "spin_proj is NULL");
+    return pixels_common(g, bore, n_t, dets, n_det, resp, pixel, spin_proj, stream);
+}
+
+int so3g_from_map(const so3g_geom *g, const double *map, const double *const *tiles,
+                  const int32_t *tile_dims, const double *bore, int64_t n_t,
+                  const double *dets, int64_t n_det, const float *resp, float *signal,
+                  int accumulate, void *stream) {
+    OpArgs a;
+    int r = base_args(g, bore, n_t, dets, n_det, stream, a);
+    if (r) return r;
+    if (n_t == 0 || n_det == 0) return SO3G_OK;
+    if (g->tiled && (tiles == nullptr || tile_dims == nullptr))
+        return fail(SO3G_ERR_RUNTIME, "tiled map requires tile pointers");
+    if (g->tiled) {
+        r = zero_flags(a.stream);
+        if (r) return r;
+    }
+    a.op = OP_FROM_MAP;
+    a.map = (void *)map;
+    a.tiles = (void *const *)tiles;
+    a.tile_dims = tile_dims;
+    a.resp = resp;
+    a.sig_out = signal;
+    a.accumulate = accumulate;
+    r = run_dispatch(g->proj, a);
+    if (r) return r;
+    return g->tiled ? check_tile_flag(a.stream) : SO3G_OK;
+}
+
+static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
+                         const int32_t *tile_dims, const double *bore, int64_t n_t,
+                         const double *dets, int64_t n_det, const float *resp,
+                         const float *signal, const float *det_weights, const int32_t *segs,
+                         int64_t n_seg, int prec, int op, void *stream) {
+    OpArgs a;
+    int r = base_args(g, bore, n_t, dets, n_det, stream, a);
+    if (r) return r;
+    if (n_t == 0 || n_det == 0) return SO3G_OK;
+    if (g->tiled && (tiles == nullptr || tile_dims == nullptr))
+        return fail(SO3G_ERR_RUNTIME, "tiled map requires tile pointers");
+    if (segs != nullptr) {
+        r = upload_chunks(segs, n_seg, n_det, n_t, a.stream, &a.n_chunks);
+        if (r) return r;
+        if (a.n_chunks == 0) return SO3G_OK;
+        a.in.chunks = g_state.chunks_d;
+    }
+    if (g->tiled) {
+        r = zero_flags(a.stream);
+        if (r) return r;
+    }
+    a.op = op;
+    a.prec = prec;
+    a.map = map;
+    a.tiles = tiles;
+    a.tile_dims = tile_dims;
+    a.resp = resp;
+    a.sig_in = signal;
+    a.det_w = det_weights;
+    r = run_dispatch(g->proj, a);
+    if (r) return r;
+    return g->tiled ? check_tile_flag(a.stream) : SO3G_OK;
+}
+
+int so3g_to_map(const so3g_geom *g, double *map, double *const *tiles,
+                const int32_t *tile_dims, const double *bore, int64_t n_t, const double *dets,
+                int64_t n_det, const float *resp, const float *signal,
+                const float *det_weights, const int32_t *segs, int64_t n_seg, void *stream) {
+    return to_map_common(g, map, (void *const *)tiles, tile_dims, bore, n_t, dets, n_det,
+                         resp, signal, det_weights, segs, n_seg, SO3G_PREC_REF, OP_TO_MAP,
+                         stream);
+}
+
+int so3g_to_weights(const so3g_geom *g, double *map, double *const *tiles,
+                    const int32_t *tile_dims, const double *bore, int64_t n_t,
+                    const double *dets, int64_t n_det, const float *resp,
+                    const float *det_weights, const int32_t *segs, int64_t n_seg,
+                    void *stream) {
+    return to_map_common(g, map, (void *const *)tiles, tile_dims, bore, n_t, dets, n_det,
+                         resp, nullptr, det_weights, segs, n_seg, SO3G_PREC_REF,
+                         OP_TO_WEIGHTS, stream);
+}
+
+int so3g_to_map_prec(const so3g_geom *g, void *map, void *const *tiles,
+                     const int32_t *tile_dims, const double *bore, int64_t n_t,
+                     const double *dets, int64_t n_det, const float *resp,
+                     const float *signal, const float *det_weights, int prec, void *stream) {
+    if (prec < 0 || prec > 2) return fail(SO3G_ERR_VALUE, "unknown precision mode");
+    return to_map_common(g, map, tiles, tile_dims, bore, n_t, dets, n_det, resp, signal,
+                         det_weights, nullptr, 0, prec, OP_TO_MAP, stream);
+}
+
+int so3g_to_weights_prec(const so3g_geom *g, void *map, void *const *tiles,
+                         const int32_t *tile_dims, const double *bore, int64_t n_t,
+                         const double *dets, int64_t n_det, const float *resp,
+                         const float *det_weights, int prec, void *stream) {
+    if (prec < 0 || prec > 2) return fail(SO3G_ERR_VALUE, "unknown precision mode");
+    return to_map_common(g, map, tiles, tile_dims, bore, n_t, dets, n_det, resp, nullptr,
+                         det_weights, nullptr, 0, prec, OP_TO_WEIGHTS, stream);
+}
+
+int so3g_tile_hits(const so3g_geom *g, const double *bore, int64_t n_t, const double *dets,
+                   int64_t n_det, int32_t *hits, void *stream) {
+    if (g != nullptr && !g->tiled)
+        return fail(SO3G_ERR_RUNTIME, "No tiles in this pixelization.");
+    OpArgs a;
+    int r = base_args(g, bore, n_t, dets, n_det, stream, a);
+    if (r) return r;
+    HIP_TRY(hipMemsetAsync(hits, 0, sizeof(int32_t) * g->n_tile, a.stream));
+    if (n_t == 0 || n_det == 0) return SO3G_OK;
+    a.op = OP_TILE_HITS;
+    a.hits = hits;
+    return run_dispatch(g->proj, a);
+}
+
+int so3g_sample_domains(const so3g_geom *g, const double *bore, int64_t n_t,
+                        const double *dets, int64_t n_det, int mode, int n_domain,
+                        const double *tmap, const double *const *ttiles,
+                        const int32_t *tile_dims, const int32_t *tile_dom, int32_t *out,
+                        void *stream) {
+    OpArgs a;
+    int r = base_args(g, bore, n_t, dets, n_det, stream, a);
+    if (r) return r;
+    if (n_t == 0 || n_det == 0) return SO3G_OK;
+    if (mode < 0 || mode > 2) return fail(SO3G_ERR_VALUE, "unknown domain mode");
+    if (mode == SO3G_DOMAIN_TILE && !g->tiled)
+        return fail(SO3G_ERR_RUNTIME, "No tiles in this pixelization.");
+    if (n_domain <= 0) return fail(SO3G_ERR_VALUE, "n_domain must be positive");
+    r = zero_flags(a.stream);
+    if (r) return r;
+    a.op = OP_DOMAINS;
+    a.mode = mode;
+    a.n_domain = n_domain;
+    a.map = (void *)tmap;
+    a.tiles = (void *const *)ttiles;
+    a.tile_dims = tile_dims;
+    a.tile_dom = tile_dom;
+    a.dom_out = out;
+    r = run_dispatch(g->proj, a);
+    if (r) return r;
+    return check_tile_flag(a.stream);
+}
+
+// ---------------------------------------------------------------------------
+// ProjEng_Precomp.
+// ---------------------------------------------------------------------------
+static int pc_setup(const int64_t *map_shape, const int32_t *tile_dims, int32_t k,
+                    int32_t nc, int64_t n_det, int64_t n_t, void *stream, PcIn &in,
+                    PcMap &pm) {
+    int r = ensure_init();
+    if (r) return r;
+    if (k < 1 || k > 3) return fail(SO3G_ERR_RUNTIME, "pixel index count must be 1, 2 or 3");
+    if (nc < 1 || nc > 3) return fail(SO3G_ERR_RUNTIME, "spin component count must be 1..3");
+    if (n_t < 0 || n_det < 0 || n_t > INT32_MAX)
+        return fail(SO3G_ERR_RUNTIME, "bad sample or detector count");
+    memset(&in, 0, sizeof(in));
+    memset(&pm, 0, sizeof(pm));
+    in.n_t = n_t;
+    in.n_det = n_det;
+    in.dpb = (int32_t)std::max<int64_t>(1, (n_det + 65534) / 65535);
+    if (k < 3) {
+        if (map_shape == nullptr) return fail(SO3G_ERR_RUNTIME, "map shape required");
+        pm.ny = map_shape[0];
+        pm.nx = map_shape[1];
+    } else {
+        if (tile_dims == nullptr) return fail(SO3G_ERR_RUNTIME, "tile dims required");
+        pm.tile_dims = tile_dims;
+    }
+    (void)stream;
+    return SO3G_OK;
+}
+
+static dim3 pc_grid(const PcIn &in, int64_t n_chunks) {
+    if (in.chunks) return dim3((unsigned)n_chunks);
+    return dim3((unsigned)((in.n_t + kBlock - 1) / kBlock),
+                (unsigned)((in.n_det + in.dpb - 1) / in.dpb));
+}
+
+#define PC_SWITCH(K, NC, KERNEL, ...)                                                   \
+    do {                                                                               \
+        int _ok = 1;                                                                   \
+        switch ((K) * 4 + (NC)) {                                                      \
+        case 5: hipLaunchKernelGGL((KERNEL<1, 1>), __VA_ARGS__); break;                \
+        case 6: hipLaunchKernelGGL((KERNEL<1, 2>), __VA_ARGS__); break;                \
+        case 7: hipLaunchKernelGGL((KERNEL<1, 3>), __VA_ARGS__); break;                \
+        case 9: hipLaunchKernelGGL((KERNEL<2, 1>), __VA_ARGS__); break;                \
+        case 10: hipLaunchKernelGGL((KERNEL<2, 2>), __VA_ARGS__); break;               \
+        case 11: hipLaunchKernelGGL((KERNEL<2, 3>), __VA_ARGS__); break;               \
+        case 13: hipLaunchKernelGGL((KERNEL<3, 1>), __VA_ARGS__); break;               \
+        case 14: hipLaunchKernelGGL((KERNEL<3, 2>), __VA_ARGS__); break;               \
+        case 15: hipLaunchKernelGGL((KERNEL<3, 3>), __VA_ARGS__); break;               \
+        default: _ok = 0;                                                              \
+        }                                                                              \
+        if (!_ok) return fail(SO3G_ERR_RUNTIME, "unsupported pixel/spin layout");       \
+    } while (0)
+
+int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
+                          const double *const *tiles, const int32_t *tile_dims,
+                          const int32_t *pixel, int32_t k, const float *spin, int32_t nc,
+                          int64_t n_det, int64_t n_t, float *signal, int accumulate,
+                          void *stream) {
+    PcIn in;
+    PcMap pm;
+    int r = pc_setup(map_shape, tile_dims, k, nc, n_det, n_t, stream, in, pm);
+    if (r) return r;
+    if (n_t == 0 || n_det == 0) return SO3G_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (k == 3) {
+        r = zero_flags(s);
+        if (r) return r;
+    }
+    PC_SWITCH(k, nc, k_pc_from_map, pc_grid(in, 0), dim3(kBlock), 0, s, in, pm, map, tiles,
+              pixel, spin, signal, accumulate, g_state.flags_d);
+    HIP_TRY(hipGetLastError());
+    return k == 3 ? check_tile_flag(s) : SO3G_OK;
+}
+
+int so3g_precomp_to_map(double *map, const int64_t *map_shape, double *const *tiles,
+                        const int32_t *tile_dims, const int32_t *pixel, int32_t k,
+                        const float *spin, int32_t nc, int64_t n_det, int64_t n_t,
+                        const float *signal, const float *det_weights, const int32_t *segs,
+                        int64_t n_seg, void *stream) {
+    PcIn in;
+    PcMap pm;
+    int r = pc_setup(map_shape, tile_dims, k, nc, n_det, n_t, stream, in, pm);
+    if (r) return r;
+    if (n_t == 0 || n_det == 0) return SO3G_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int64_t n_chunks = 0;
+    if (segs != nullptr) {
+        r = upload_chunks(segs, n_seg, n_det, n_t, s, &n_chunks);
+        if (r) return r;
+        if (n_chunks == 0) return SO3G_OK;
+        in.chunks = g_state.chunks_d;
+    }
+    if (k == 3) {
+        r = zero_flags(s);
+        if (r) return r;
+    }
+    PC_SWITCH(k, nc, k_pc_to_map, pc_grid(in, n_chunks), dim3(kBlock), 0, s, in, pm, map,
+              tiles, pixel, spin, signal, det_weights, g_state.flags_d);
+    HIP_TRY(hipGetLastError());
+    return k == 3 ? check_tile_flag(s) : SO3G_OK;
+}
+
+int so3g_precomp_to_weights(double *map, const int64_t *map_shape, double *const *tiles,
+                            const int32_t *tile_dims, const int32_t *pixel, int32_t k,
+                            const float *spin, int32_t nc, int64_t n_det, int64_t n_t,
+                            const float *det_weights, const int32_t *segs, int64_t n_seg,
+                            void *stream) {
+    PcIn in;
+    PcMap pm;
+    int r = pc_setup(map_shape, tile_dims, k, nc, n_det, n_t, stream, in, pm);
+    if (r) return r;
+    if (n_t == 0 || n_det == 0) return SO3G_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int64_t n_chunks = 0;
+    if (segs != nullptr) {
+        r = upload_chunks(segs, n_seg, n_det, n_t, s, &n_chunks);
+        if (r) return r;
+        if (n_chunks == 0) return SO3G_OK;
+        in.chunks = g_state.chunks_d;
+    }
+    if (k == 3) {
+        r = zero_flags(s);
+        if (r) return r;
+    }
+    PC_SWITCH(k, nc, k_pc_to_weights, pc_grid(in, n_chunks), dim3(kBlock), 0, s, in, pm,
+              map, tiles, pixel, spin, det_weights, g_state.flags_d);
+    HIP_TRY(hipGetLastError());
+    return k == 3 ? check_tile_flag(s) : SO3G_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,331 @@
+// kernels.h -- on-the-fly projection kernels (pointing fused with the
+// pixel/spin computation and the map gather or scatter).
+//
+// Work mapping.  A block owns 256 consecutive time samples; each thread
+// keeps its boresight quaternion in registers and walks a group of
+// detectors, so the boresight stream (32 B/sample) is read once per
+// detector group instead of once per detector, the detector quaternion
+// and response are wave-uniform (scalar loads), and every signal access
+// is a coalesced 1 KiB row segment of the det-major float32 array.
+// With an explicit segment list (so3g's thread_intervals) a block owns
+// one chunk (det, t0, t1) of at most 256 samples instead.
+#pragma once
+
+#include "ptg.h"
+
+namespace so3g {
+
+constexpr int kBlock = 256;
+
+struct Chunk { int32_t det, t0, t1; };
+
+struct PtgIn {
+    const double *bore;
+    const double *dets;
+    int64_t n_t, n_det;
+    Tabs tabs;
+    int32_t dpb;               // detectors per block (dense mode)
+    const Chunk *chunks;       // segment mode when non-null
+};
+
+// Calls f(det, t, qbore) for the samples this thread owns.
+template <class F>
+__device__ __forceinline__ void for_samples(const PtgIn &in, F &&f) {
+    if (in.chunks != nullptr) {
+        const Chunk c = in.chunks[blockIdx.x];
+        const int t = c.t0 + (int)threadIdx.x;
+        if (t < c.t1) f(c.det, (int64_t)t, load_q(in.bore + 4 * (int64_t)t));
+        return;
+    }
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= in.n_t) return;
+    const Q4 qb = load_q(in.bore + 4 * t);
+    const int64_t d0 = (int64_t)blockIdx.y * in.dpb;
+    const int64_t d1 = min(d0 + in.dpb, in.n_det);
+    for (int64_t d = d0; d < d1; d++) f(d, t, qb);
+}
+
+// Library-owned device flags.
+enum { FLAG_TILE = 0, FLAG_OFFMAP = 1, FLAG_COUNT = 4 };
+
+__device__ __forceinline__ void raise_flag(int *flags, int which) {
+    __hip_atomic_fetch_or(flags + which, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// --------------------------------------------------------------------------
+// coords / pixels / pointing_matrix
+// --------------------------------------------------------------------------
+template <int PROJ>
+__global__ __launch_bounds__(kBlock) void k_coords(PtgIn in, double *out) {
+    for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
+        const Q4 qd = load_q(in.dets + 4 * d);
+        double co[4];
+        get_coords<PROJ>(qb, qd, in.tabs, co);
+        double *o = out + 4 * (d * in.n_t + t);
+        *reinterpret_cast<double2 *>(o) = double2{co[0], co[1]};
+        *reinterpret_cast<double2 *>(o + 2) = double2{co[2], co[3]};
+    });
+}
+
+// pixels() and pointing_matrix() in one kernel (spin == nullptr for
+// pixels()).  Off-map samples write slot 0 = -1 and raise FLAG_OFFMAP;
+// their other slots are filled afterwards by k_fill_stale.
+template <int PROJ, int PK, int NC>
+__global__ __launch_bounds__(kBlock) void k_pixels(PtgIn in, Geo g, const float *resp,
+                                                   int32_t *pix, float *spin, int *flags) {
+    constexpr int K = index_count<PK>();
+    for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
+        const Q4 qd = load_q(in.dets + 4 * d);
+        double co[4];
+        get_coords<PROJ>(qb, qd, in.tabs, co);
+        int32_t pi[3] = {0, 0, 0};
+        const bool on = get_pixel<PK>(g, co, pi);
+        int32_t *po = pix + (d * in.n_t + t) * K;
+        if (on) {
+#pragma unroll
+            for (int j = 0; j < K; j++) po[j] = pi[j];
+        } else {
+            po[0] = -1;
+            raise_flag(flags, FLAG_OFFMAP);
+        }
+        if (spin != nullptr) {
+            float pf[3];
+            spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
+            float *so = spin + (d * in.n_t + t) * NC;
+#pragma unroll
+            for (int j = 0; j < NC; j++) so[j] = pf[j];
+        }
+    });
+}
+
+// Forward-fill of the non-leading index slots of off-map samples: the
+// reference keeps one index buffer per detector, initialised {-1, 0, ...},
+// and an off-map sample only overwrites slot 0 (Projection.cxx:1372,
+// 1418, 848-859), so slots 1.. repeat the last on-map sample's values.
+// One block per detector; a block-wide max-scan carries the index of the
+// last on-map sample across 1024-sample chunks.  Exits at once when no
+// sample was off-map.
+constexpr int kFillBlock = 1024;
+
+template <int K>
+__global__ __launch_bounds__(kFillBlock) void k_fill_stale(int32_t *pix, int64_t n_t,
+                                                           const int *flags) {
+    if (flags[FLAG_OFFMAP] == 0) return;
+    __shared__ int64_t scan[kFillBlock];
+    const int64_t d = blockIdx.x;
+    int32_t *row = pix + d * n_t * K;
+    int64_t carry = -1;
+    for (int64_t s = 0; s < n_t; s += kFillBlock) {
+        const int64_t t = s + threadIdx.x;
+        const bool in_range = t < n_t;
+        const bool on = in_range && row[t * K] >= 0;
+        scan[threadIdx.x] = on ? t : -1;
+        __syncthreads();
+        for (int off = 1; off < kFillBlock; off <<= 1) {
+            int64_t v = scan[threadIdx.x];
+            if ((int)threadIdx.x >= off) v = max(v, scan[threadIdx.x - off]);
+            __syncthreads();
+            scan[threadIdx.x] = v;
+            __syncthreads();
+        }
+        const int64_t last = max(carry, scan[threadIdx.x]);
+        if (in_range && !on) {
+#pragma unroll
+            for (int j = 1; j < K; j++) row[t * K + j] = (last >= 0) ? row[last * K + j] : 0;
+        }
+        carry = max(carry, scan[kFillBlock - 1]);
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------------------
+// from_map: sig = (float)((double)sig + map * pf * w) per point and
+// component (Projection.cxx:1761-1765); deterministic per sample.
+// --------------------------------------------------------------------------
+template <int PROJ, int PK, int INTERP, int NC>
+__global__ __launch_bounds__(kBlock) void k_from_map(PtgIn in, Geo g, MapView<const double> m,
+                                                     const float *resp, float *sig,
+                                                     int accumulate, int *flags) {
+    for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
+        const Q4 qd = load_q(in.dets + 4 * d);
+        double co[4];
+        get_coords<PROJ>(qb, qd, in.tabs, co);
+        float pf[3];
+        spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
+        Pt p[4];
+        const int np = get_pixels<PK, INTERP>(g, co, p);
+        float *sp = sig + d * in.n_t + t;
+        float s = accumulate ? *sp : 0.f;
+        for (int ip = 0; ip < np; ip++) {
+            const double *e = elem<PK>(g, m, p[ip], 0);
+            if (e == nullptr) { raise_flag(flags, FLAG_TILE); continue; }
+            const int64_t plane = (PK == PK_FLAT_TILED)
+                ? (int64_t)m.tile_dims[2 * p[ip].i0] * m.tile_dims[2 * p[ip].i0 + 1]
+                : (PK == PK_HP_TILED ? (int64_t)g.npt : g.npix);
+#pragma unroll
+            for (int c = 0; c < NC; c++) s = s + e[c * plane] * pf[c] * p[ip].w;
+        }
+        *sp = s;
+    });
+}
+
+// --------------------------------------------------------------------------
+// to_map / to_weights: the per-sample product is formed in float32 exactly
+// as the reference does (Projection.cxx:1807, 1850) and added to the
+// float64 map with a hardware FP64 atomic.  PREC selects the precision
+// study variants (float32 map; float64 products).
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void atomic_add(T *a, T v) {
+    unsafeAtomicAdd(a, v);
+}
+
+template <int PREC>
+struct PrecT { using map_t = double; };
+template <>
+struct PrecT<1> { using map_t = float; };
+
+template <int PROJ, int PK, int INTERP, int NC, int PREC>
+__global__ __launch_bounds__(kBlock) void k_to_map(PtgIn in, Geo g,
+                                                   MapView<typename PrecT<PREC>::map_t> m,
+                                                   const float *resp, const float *sig,
+                                                   const float *det_weights, int *flags) {
+    using MT = typename PrecT<PREC>::map_t;
+    for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
+        float det_wt = 1.f;
+        if (det_weights != nullptr) {
+            det_wt = det_weights[d];
+            if (det_wt == 0.f) return;
+        }
+        const Q4 qd = load_q(in.dets + 4 * d);
+        double co[4];
+        get_coords<PROJ>(qb, qd, in.tabs, co);
+        float pf[3];
+        spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
+        const float s = sig[d * in.n_t + t];
+        Pt p[4];
+        const int np = get_pixels<PK, INTERP>(g, co, p);
+        for (int ip = 0; ip < np; ip++) {
+            MT *e = elem<PK>(g, m, p[ip], 0);
+            if (e == nullptr) { raise_flag(flags, FLAG_TILE); continue; }
+            const int64_t plane = (PK == PK_FLAT_TILED)
+                ? (int64_t)m.tile_dims[2 * p[ip].i0] * m.tile_dims[2 * p[ip].i0 + 1]
+                : (PK == PK_HP_TILED ? (int64_t)g.npt : g.npix);
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                if constexpr (PREC == 2) {
+                    const double v = (double)s * (double)pf[c] * (double)p[ip].w * (double)det_wt;
+                    atomic_add(e + c * plane, v);
+                } else {
+                    const float v = s * pf[c] * p[ip].w * det_wt;
+                    atomic_add(e + c * plane, (MT)v);
+                }
+            }
+        }
+    });
+}
+
+template <int PROJ, int PK, int INTERP, int NC, int PREC>
+__global__ __launch_bounds__(kBlock) void k_to_weights(PtgIn in, Geo g,
+                                                       MapView<typename PrecT<PREC>::map_t> m,
+                                                       const float *resp,
+                                                       const float *det_weights, int *flags) {
+    using MT = typename PrecT<PREC>::map_t;
+    for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
+        float det_wt = 1.f;
+        if (det_weights != nullptr) {
+            det_wt = det_weights[d];
+            if (det_wt == 0.f) return;
+        }
+        const Q4 qd = load_q(in.dets + 4 * d);
+        double co[4];
+        get_coords<PROJ>(qb, qd, in.tabs, co);
+        float pf[3];
+        spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
+        Pt p[4];
+        const int np = get_pixels<PK, INTERP>(g, co, p);
+        for (int ip = 0; ip < np; ip++) {
+            MT *e = elem<PK>(g, m, p[ip], 0);
+            if (e == nullptr) { raise_flag(flags, FLAG_TILE); continue; }
+            const int64_t plane = (PK == PK_FLAT_TILED)
+                ? (int64_t)m.tile_dims[2 * p[ip].i0] * m.tile_dims[2 * p[ip].i0 + 1]
+                : (PK == PK_HP_TILED ? (int64_t)g.npt : g.npix);
+            const float w = p[ip].w;
+#pragma unroll
+            for (int i = 0; i < NC; i++)
+#pragma unroll
+                for (int j = i; j < NC; j++) {
+                    if constexpr (PREC == 2) {
+                        const double v = (double)pf[i] * (double)pf[j] * (double)w *
+                                         (double)w * (double)det_wt;
+                        atomic_add(e + (i * NC + j) * plane, v);
+                    } else {
+                        const float v = pf[i] * pf[j] * w * w * det_wt;
+                        atomic_add(e + (i * NC + j) * plane, (MT)v);
+                    }
+                }
+        }
+    });
+}
+
+// --------------------------------------------------------------------------
+// tile_hits and per-sample domains (thread-assignment helpers).
+// --------------------------------------------------------------------------
+template <int PROJ, int PK, int INTERP>
+__global__ __launch_bounds__(kBlock) void k_tile_hits(PtgIn in, Geo g, int32_t *hits) {
+    for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
+        const Q4 qd = load_q(in.dets + 4 * d);
+        double co[4];
+        get_coords<PROJ>(qb, qd, in.tabs, co);
+        Pt p[4];
+        const int np = get_pixels<PK, INTERP>(g, co, p);
+        for (int ip = 0; ip < np; ip++) atomicAdd(hits + p[ip].i0, 1);
+    });
+}
+
+__device__ __forceinline__ int wrap_mul(int a, int b) {
+    return (int)((uint32_t)a * (uint32_t)b);
+}
+
+template <int PROJ, int PK, int INTERP>
+__global__ __launch_bounds__(kBlock) void k_domains(PtgIn in, Geo g, int mode, int n_domain,
+                                                    MapView<const double> tm,
+                                                    const int32_t *tile_dom, int32_t *out,
+                                                    int *flags) {
+    for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
+        const Q4 qd = load_q(in.dets + 4 * d);
+        double co[4];
+        get_coords<PROJ>(qb, qd, in.tabs, co);
+        Pt p[4];
+        const int np = get_pixels<PK, INTERP>(g, co, p);
+        int dom = -1;
+        for (int ip = 0; ip < np; ip++) {
+            int pd;
+            if (mode == 1) {
+                const double *e = elem<PK>(g, tm, p[ip], 0);
+                if (e == nullptr) { raise_flag(flags, FLAG_TILE); dom = -2; break; }
+                pd = (int)*e;
+            } else if (mode == 2) {
+                pd = tile_dom[p[ip].i0];
+            } else {
+                // stripe(), in the reference's 32-bit int arithmetic.
+                if constexpr (PK == PK_HP) {
+                    pd = wrap_mul(p[ip].i0, n_domain) / (int)g.npix;
+                } else if constexpr (PK == PK_HP_TILED) {
+                    const int ntiles = (int)(g.npix / g.npt);
+                    const int ipix = (int)((uint32_t)wrap_mul(p[ip].i0, g.npt) + (uint32_t)p[ip].i1);
+                    pd = wrap_mul(ipix, n_domain) / wrap_mul(g.npt, ntiles);
+                } else if constexpr (PK == PK_FLAT) {
+                    pd = wrap_mul(p[ip].i1, n_domain) / g.nx;
+                } else {
+                    pd = p[ip].i0 % n_domain;
+                }
+            }
+            if (ip > 0 && pd != dom) { dom = n_domain; break; }
+            dom = pd;
+        }
+        out[d * in.n_t + t] = dom;
+    });
+}
+
+}  // namespace so3g

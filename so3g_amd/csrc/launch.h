@@ -1,0 +1,171 @@
+// launch.h -- template dispatch from a runtime engine description to the
+// kernel instantiations of one coordinate system.  Each k_<proj>.hip
+// translation unit instantiates dispatch<PROJ> for its projection, so the
+// kernel variants of the eight coordinate systems compile in parallel.
+#pragma once
+
+#include "kernels.h"
+#include "precomp.h"
+
+namespace so3g {
+
+enum { OP_COORDS = 0, OP_PIXELS, OP_PMAT, OP_FROM_MAP, OP_TO_MAP, OP_TO_WEIGHTS,
+       OP_TILE_HITS, OP_DOMAINS };
+
+struct OpArgs {
+    int op;
+    PtgIn in;
+    Geo g;
+    int pk, interp, nc, prec;
+    int64_t n_chunks;          // segment mode when in.chunks != nullptr
+    hipStream_t stream;
+    int *flags;
+    const float *resp;
+    const float *sig_in;
+    float *sig_out;
+    const float *det_w;
+    void *map;
+    void *const *tiles;
+    const int32_t *tile_dims;
+    double *coords_out;
+    int32_t *pix_out;
+    float *spin_out;
+    int accumulate;
+    int mode, n_domain;
+    const int32_t *tile_dom;
+    int32_t *dom_out;
+    int32_t *hits;
+};
+
+inline dim3 grid_for(const OpArgs &a) {
+    if (a.in.chunks != nullptr) return dim3((unsigned)a.n_chunks);
+    const int64_t bx = (a.in.n_t + kBlock - 1) / kBlock;
+    const int64_t by = (a.in.n_det + a.in.dpb - 1) / a.in.dpb;
+    return dim3((unsigned)bx, (unsigned)by);
+}
+
+template <int PROJ, int PK, int INTERP, int NC>
+int launch_map_ops(const OpArgs &a) {
+    const dim3 grid = grid_for(a);
+    const dim3 block(kBlock);
+    switch (a.op) {
+    case OP_FROM_MAP: {
+        MapView<const double> m{(const double *)a.map, (const double *const *)a.tiles,
+                                a.tile_dims};
+        hipLaunchKernelGGL((k_from_map<PROJ, PK, INTERP, NC>), grid, block, 0, a.stream,
+                           a.in, a.g, m, a.resp, a.sig_out, a.accumulate, a.flags);
+        return 0;
+    }
+    case OP_TO_MAP:
+        if (a.prec == 1) {
+            MapView<float> m{(float *)a.map, (float *const *)a.tiles, a.tile_dims};
+            hipLaunchKernelGGL((k_to_map<PROJ, PK, INTERP, NC, 1>), grid, block, 0,
+                               a.stream, a.in, a.g, m, a.resp, a.sig_in, a.det_w, a.flags);
+        } else {
+            MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
+            if (a.prec == 2)
+                hipLaunchKernelGGL((k_to_map<PROJ, PK, INTERP, NC, 2>), grid, block, 0,
+                                   a.stream, a.in, a.g, m, a.resp, a.sig_in, a.det_w,
+                                   a.flags);
+            else
+                hipLaunchKernelGGL((k_to_map<PROJ, PK, INTERP, NC, 0>), grid, block, 0,
+                                   a.stream, a.in, a.g, m, a.resp, a.sig_in, a.det_w,
+                                   a.flags);
+        }
+        return 0;
+    case OP_TO_WEIGHTS:
+        if (a.prec == 1) {
+            MapView<float> m{(float *)a.map, (float *const *)a.tiles, a.tile_dims};
+            hipLaunchKernelGGL((k_to_weights<PROJ, PK, INTERP, NC, 1>), grid, block, 0,
+                               a.stream, a.in, a.g, m, a.resp, a.det_w, a.flags);
+        } else {
+            MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
+            if (a.prec == 2)
+                hipLaunchKernelGGL((k_to_weights<PROJ, PK, INTERP, NC, 2>), grid, block, 0,
+                                   a.stream, a.in, a.g, m, a.resp, a.det_w, a.flags);
+            else
+                hipLaunchKernelGGL((k_to_weights<PROJ, PK, INTERP, NC, 0>), grid, block, 0,
+                                   a.stream, a.in, a.g, m, a.resp, a.det_w, a.flags);
+        }
+        return 0;
+    }
+    return -1;
+}
+
+template <int PROJ, int PK, int INTERP>
+int launch_interp(const OpArgs &a) {
+    const dim3 grid = grid_for(a);
+    const dim3 block(kBlock);
+    if (a.op == OP_TILE_HITS) {
+        hipLaunchKernelGGL((k_tile_hits<PROJ, PK, INTERP>), grid, block, 0, a.stream, a.in,
+                           a.g, a.hits);
+        return 0;
+    }
+    if (a.op == OP_DOMAINS) {
+        MapView<const double> m{(const double *)a.map, (const double *const *)a.tiles,
+                                a.tile_dims};
+        hipLaunchKernelGGL((k_domains<PROJ, PK, INTERP>), grid, block, 0, a.stream, a.in,
+                           a.g, a.mode, a.n_domain, m, a.tile_dom, a.dom_out, a.flags);
+        return 0;
+    }
+    switch (a.nc) {
+    case 1: return launch_map_ops<PROJ, PK, INTERP, 1>(a);
+    case 2: return launch_map_ops<PROJ, PK, INTERP, 2>(a);
+    case 3: return launch_map_ops<PROJ, PK, INTERP, 3>(a);
+    }
+    return -1;
+}
+
+template <int PROJ, int PK>
+int launch_pk(const OpArgs &a) {
+    const dim3 grid = grid_for(a);
+    const dim3 block(kBlock);
+    if (a.op == OP_PIXELS || a.op == OP_PMAT) {
+        float *spin = (a.op == OP_PMAT) ? a.spin_out : nullptr;
+        switch (a.nc) {
+        case 1:
+            hipLaunchKernelGGL((k_pixels<PROJ, PK, 1>), grid, block, 0, a.stream, a.in, a.g,
+                               a.resp, a.pix_out, spin, a.flags);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_pixels<PROJ, PK, 2>), grid, block, 0, a.stream, a.in, a.g,
+                               a.resp, a.pix_out, spin, a.flags);
+            break;
+        default:
+            hipLaunchKernelGGL((k_pixels<PROJ, PK, 3>), grid, block, 0, a.stream, a.in, a.g,
+                               a.resp, a.pix_out, spin, a.flags);
+            break;
+        }
+        constexpr int K = index_count<PK>();
+        if constexpr (K > 1)
+            hipLaunchKernelGGL((k_fill_stale<K>), dim3((unsigned)a.in.n_det), dim3(kFillBlock),
+                               0, a.stream, a.pix_out, a.in.n_t, a.flags);
+        return 0;
+    }
+    if constexpr (PK == PK_FLAT || PK == PK_FLAT_TILED) {
+        if (a.interp) return launch_interp<PROJ, PK, 1>(a);
+    }
+    return launch_interp<PROJ, PK, 0>(a);
+}
+
+template <int PROJ>
+int dispatch(const OpArgs &a) {
+    if (a.op == OP_COORDS) {
+        hipLaunchKernelGGL((k_coords<PROJ>), grid_for(a), dim3(kBlock), 0, a.stream, a.in,
+                           a.coords_out);
+        return 0;
+    }
+    switch (a.pk) {
+    case PK_FLAT: return launch_pk<PROJ, PK_FLAT>(a);
+    case PK_FLAT_TILED: return launch_pk<PROJ, PK_FLAT_TILED>(a);
+    }
+    if constexpr (PROJ == PROJ_CAR) {
+        switch (a.pk) {
+        case PK_HP: return launch_pk<PROJ, PK_HP>(a);
+        case PK_HP_TILED: return launch_pk<PROJ, PK_HP_TILED>(a);
+        }
+    }
+    return -1;
+}
+
+}  // namespace so3g
