@@ -1,0 +1,250 @@
+"""Benchmark: det*samples/s of one to_map + from_map pass over one large
+observation (1000 detectors x 720000 samples = 7.2e8 det*samples; constant-
+elevation scan, el 50 deg, az +-10 deg at 1 deg/s, 200 Hz, detectors in a
+1.5 deg disk; CAR 0.5 arcmin TQU float64 map, 188 x 2689 pixels).
+
+One "step" = from_map (map -> fresh float32 signal) + to_map (signal ->
+float64 TQU map, zeroed first) over the whole observation, with every input
+resident in HBM.  The headline path projects through the cached pointing
+matrix (get_pointing_matrix once, then ProjEng_Precomp from_map / to_map,
+the operator pair of every CG iteration); the on-the-fly path (pointing
+recomputed in both kernels, the Projectionist's default) is timed beside it.
+
+Prints ONE JSON line.  `python bench.py [--steps K] [--warmup W] [--gpus 1]`.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--n-det', type=int, default=1000)
+    ap.add_argument('--n-samp', type=int, default=720000)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--no-otf', action='store_true', help='skip the on-the-fly leg')
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get('WORLD_SIZE', '1'))
+    if args.gpus != 1 or world != 1:
+        print('bench.py measures one GPU only: this build has no multi-GPU path '
+              '(run with --gpus 1).', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import so3g_amd
+    from so3g_amd import _native as N, workloads
+
+    dev = torch.device('cuda', 0)
+    torch.cuda.set_device(dev)
+    N.init(0)
+    lib = N.load()
+
+    t0 = time.time()
+    obs = workloads.Observation(args.n_det, args.n_samp, 1.5, 0.5, seed=0)
+    p = obs.projectionist()
+    pixargs = p._get_pixelizor_args()
+    eng = so3g_amd.ProjEng_CAR_TQU_NonTiled(pixargs)
+    g = eng._g
+    ny, nx = obs.shape
+    n_det, n_t = obs.n_det, obs.n_samp
+    N_s = n_det * n_t
+    bore = torch.from_numpy(obs.bore).to(dev)
+    dets = torch.from_numpy(obs.dets).to(dev)
+    resp = torch.from_numpy(obs.resps).to(dev)
+    log(f'workload built in {time.time() - t0:.1f}s: {n_det} x {n_t}, map {ny}x{nx}')
+
+    stream = torch.cuda.current_stream().cuda_stream or None
+
+    def chk(rc):
+        N.check(rc)
+
+    # Signal = from_map(seeded Gaussian random TQU map) + white noise.
+    gen = torch.Generator(device=dev).manual_seed(2)
+    sky = torch.randn((3, ny, nx), dtype=torch.float64, device=dev, generator=gen)
+    sig = torch.empty((n_det, n_t), dtype=torch.float32, device=dev)
+    chk(lib.so3g_from_map(g, sky.data_ptr(), None, None, bore.data_ptr(), n_t, dets.data_ptr(),
+                          n_det, resp.data_ptr(), sig.data_ptr(), 0, stream))
+    sig += torch.randn((n_det, n_t), dtype=torch.float32, device=dev, generator=gen)
+
+    # Cached pointing matrix (outside the timed region, as in a CG solve).
+    torch.cuda.synchronize()
+    t0 = time.time()
+    pix = torch.empty((n_det, n_t, 2), dtype=torch.int32, device=dev)
+    spin = torch.empty((n_det, n_t, 3), dtype=torch.float32, device=dev)
+    chk(lib.so3g_pointing_matrix(g, bore.data_ptr(), n_t, dets.data_ptr(), n_det,
+                                 resp.data_ptr(), pix.data_ptr(), spin.data_ptr(), stream))
+    torch.cuda.synchronize()
+    t_pmat = time.time() - t0
+    log(f'pointing matrix: {t_pmat * 1e3:.1f} ms')
+
+    shape = np.array([ny, nx], np.int64)
+    sig_out = torch.empty_like(sig)
+    mapo = torch.zeros((3, ny, nx), dtype=torch.float64, device=dev)
+
+    def pc_from():
+        chk(lib.so3g_precomp_from_map(sky.data_ptr(), shape.ctypes.data, None, None,
+                                      pix.data_ptr(), 2, spin.data_ptr(), 3, n_det, n_t,
+                                      sig_out.data_ptr(), 0, stream))
+
+    def pc_to():
+        mapo.zero_()
+        chk(lib.so3g_precomp_to_map(mapo.data_ptr(), shape.ctypes.data, None, None,
+                                    pix.data_ptr(), 2, spin.data_ptr(), 3, n_det, n_t,
+                                    sig.data_ptr(), None, None, 0, stream))
+
+    def otf_from():
+        chk(lib.so3g_from_map(g, sky.data_ptr(), None, None, bore.data_ptr(), n_t,
+                              dets.data_ptr(), n_det, resp.data_ptr(), sig_out.data_ptr(), 0,
+                              stream))
+
+    def otf_to():
+        mapo.zero_()
+        chk(lib.so3g_to_map(g, mapo.data_ptr(), None, None, bore.data_ptr(), n_t,
+                            dets.data_ptr(), n_det, resp.data_ptr(), sig.data_ptr(), None,
+                            None, 0, stream))
+
+    def run(ops, steps, warmup):
+        for _ in range(warmup):
+            for f in ops:
+                f()
+        torch.cuda.synchronize()
+        ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in ops] for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            for i, f in enumerate(ops):
+                ev[s][i][0].record()
+                f()
+                ev[s][i][1].record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        per_op = [float(np.mean([ev[s][i][0].elapsed_time(ev[s][i][1]) for s in range(steps)]))
+                  for i in range(len(ops))]
+        return wall, per_op
+
+    log('timing cached-pointing path ...')
+    wall, (ms_from, ms_to) = run([pc_from, pc_to], args.steps, args.warmup)
+    ms_step = wall / args.steps * 1e3
+    value = N_s / (wall / args.steps)
+
+    # Algorithmic bytes per launch (pixel indices k=2, spin c=3, float32
+    # signal, float64 TQU map of Npix pixels):
+    npix = ny * nx
+    b_from = N_s * (4 * 2 + 4 * 3 + 4) + 8 * 3 * npix
+    b_to = N_s * (4 * 2 + 4 * 3 + 4) + 2 * 8 * 3 * npix
+    gbs_from = b_from / (ms_from * 1e-3)
+    gbs_to = b_to / (ms_to * 1e-3)
+    if ms_to >= ms_from:
+        dom = ('k_pc_to_map<2,3>', gbs_to, b_to)
+    else:
+        dom = ('k_pc_from_map<2,3>', gbs_from, b_from)
+
+    otf = None
+    if not args.no_otf:
+        log('timing on-the-fly path ...')
+        steps_otf = max(1, min(args.steps, 3))
+        wall_o, (ms_of, ms_ot) = run([otf_from, otf_to], steps_otf, 1)
+        otf = {
+            'value': N_s / (wall_o / steps_otf),
+            'ms_per_step': wall_o / steps_otf * 1e3,
+            'ms_from_map': ms_of, 'ms_to_map': ms_ot,
+            'hbm_frac_from_map': (N_s * 4 + 32 * n_t + 40 * n_det + 8 * 3 * npix) /
+            (ms_of * 1e-3) / HBM_PEAK,
+            'hbm_frac_to_map': (N_s * 4 + 32 * n_t + 40 * n_det + 16 * 3 * npix) /
+            (ms_ot * 1e-3) / HBM_PEAK,
+        }
+
+    cpu = None if args.no_cpu else cpu_baseline(obs, pix, spin, sig, sky, pixargs)
+
+    out = {
+        'metric': 'det*samples/s for to_map+from_map (% HBM roofline)',
+        'value': value,
+        'unit': 'det*samples/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32 products, f64 map accumulation',
+        'data': 'synthetic (seeded scan, focal plane, random TQU sky + white noise)',
+        'config': {
+            'workload': f'single observation {n_det} det x {n_t} samples, CAR 0.5 arcmin '
+                        f'TQU {ny}x{nx} non-tiled; step = from_map + to_map via cached '
+                        f'pointing matrix (ProjEng_Precomp)',
+            'n_det': n_det, 'n_samp': n_t, 'map_shape': [3, ny, nx],
+            'pointing': 'pointing_matrix computed once before timing '
+                        f'({t_pmat * 1e3:.1f} ms)',
+        },
+        'ms_from_map': ms_from,
+        'ms_to_map': ms_to,
+        'roofline': {
+            'bound': 'hbm', 'kernel': dom[0], 'achieved': dom[1] / 1e9,
+            'peak': HBM_PEAK / 1e9, 'unit': 'GB/s', 'frac': dom[1] / HBM_PEAK,
+            'traffic': None, 'algorithmic_bytes': dom[2],
+            'from_map_frac': gbs_from / HBM_PEAK, 'to_map_frac': gbs_to / HBM_PEAK,
+        },
+        'on_the_fly': otf,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(obs, pix, spin, sig, sky, pixargs):
+    """The oracle (a C++ port of so3g's loops, g++ -O3, no -march) on the
+    host cores, on a bounded sample of the same workload: 200 detectors of
+    the cached-pointing pair (from_map OpenMP over detectors, to_map single
+    threaded as so3g runs it without a thread assignment)."""
+    try:
+        import oracle
+    except Exception as e:  # pragma: no cover
+        log(f'cpu baseline unavailable: {e}')
+        return None
+    log('timing CPU baseline (oracle) ...')
+    n = min(200, obs.n_det)
+    p = pix[:n].cpu().numpy()
+    s = spin[:n].cpu().numpy()
+    sg = sig[:n].cpu().numpy()
+    m = sky.cpu().numpy()
+    og = oracle.geom_from_args('CAR', 'TQU', pixargs)
+    oracle.precomp_from_map(og, m, p[:2], s[:2])  # warm the library
+    t0 = time.perf_counter()
+    oracle.precomp_from_map(og, m, p, s)
+    t1 = time.perf_counter()
+    out = np.zeros_like(m)
+    oracle.precomp_to_map(og, out, p, s, sg)
+    t2 = time.perf_counter()
+    n_s = n * obs.n_samp
+    return {
+        'value': n_s / (t2 - t0), 'unit': 'det*samples/s', 'cores': oracle.num_threads(),
+        'kind': 'port',
+        'sample': f'{n} of {obs.n_det} detectors x {obs.n_samp} samples, cached-pointing '
+                  f'from_map (OpenMP, {oracle.num_threads()} threads: {t1 - t0:.2f} s) + '
+                  f'to_map (1 thread: {t2 - t1:.2f} s)',
+    }
+
+
+if __name__ == '__main__':
+    main()

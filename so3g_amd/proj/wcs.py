@@ -69,7 +69,9 @@ class _ProjectionistBase:
 
     def _cache_q_fp_to_native(self, q_fp_to_celestial):
         if q_fp_to_celestial is not self._q_fp_to_celestial:
-            self._q_fp_to_native = self.q_celestial_to_native * q_fp_to_celestial
+            # Hamilton product whatever array type the caller holds.
+            self._q_fp_to_native = quat.Quat(quat.qmul(self.q_celestial_to_native,
+                                                       q_fp_to_celestial))
             self._q_fp_to_celestial = q_fp_to_celestial
         return self._q_fp_to_native
 

@@ -1,0 +1,306 @@
+"""GPU parity: every engine method of every engine configuration, run by the
+HIP kernels through the ProjEng_* classes (C ABI underneath), against the
+CPU oracle on the same inputs.
+
+Bar: pixel indices, spin factors, coordinates and from_map signals are
+bit-exact; to_map / to_weights maps (float64 sums of the reference's float32
+products, accumulated with atomics in a different order) agree to
+1e-10 absolute on O(1)-valued maps, far inside the 1e-6 relative target.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import Case, FLAT_PROJS, all_flat_cases, copy_map, flat_map
+
+pytestmark = pytest.mark.gpu
+
+MAP_ATOL = 1e-10
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.int64 if a.dtype == np.float64 else np.int32)
+
+
+def assert_bit_equal(gpu, ref, what, max_ulp=0):
+    gpu = np.asarray(gpu)
+    ref = np.asarray(ref)
+    assert gpu.shape == ref.shape, (what, gpu.shape, ref.shape)
+    if max_ulp == 0:
+        bad = bits(gpu) != bits(ref)
+        # NaN payloads: equal if both NaN.
+        if gpu.dtype.kind == 'f':
+            bad &= ~(np.isnan(gpu) & np.isnan(ref))
+        n = int(bad.sum())
+        assert n == 0, f'{what}: {n} of {bad.size} values differ, e.g. ' \
+            f'{gpu[bad][:4]} vs {ref[bad][:4]}'
+    else:
+        d = np.abs(bits(gpu).astype(np.int64) - bits(ref).astype(np.int64))
+        assert int(d.max()) <= max_ulp, f'{what}: max ulp diff {d.max()}'
+
+
+def maps_close(gpu, ref, what, atol=MAP_ATOL):
+    g, r = flat_map(gpu), flat_map(ref)
+    assert g.shape == r.shape
+    err = np.abs(g - r).max() if g.size else 0.
+    assert err <= atol, f'{what}: max abs err {err}'
+
+
+CASES = [(p, t, i) for (p, t, i) in all_flat_cases()]
+HP_CASES = [('HP', False, True), ('HP', False, False), ('HP', True, True)]
+
+
+def case_id(c):
+    p, t, i = c
+    return f'{p}-{"tiled" if t else "flat"}-{"bilin" if i else "nn"}'
+
+
+def hp_id(c):
+    return f'HP-{"tiled" if c[1] else "flat"}-{"nest" if c[2] else "ring"}'
+
+
+@pytest.mark.parametrize('proj', FLAT_PROJS)
+def test_coords(gpu, proj):
+    c = Case(proj)
+    out = np.array(c.engine.coords(c.bore, c.dets))
+    ref = oracle.coords(c.og, c.bore, c.dets)
+    if proj == 'CEA':
+        # CEA longitude comes from the math library's atan2 (glibc on the
+        # CPU, ROCm's ocml on the GPU); allow a 1-ulp difference there only.
+        assert_bit_equal(out[..., 1:], ref[..., 1:], 'coords[1:]')
+        assert_bit_equal(out[..., 0], ref[..., 0], 'CEA lon', max_ulp=1)
+    else:
+        assert_bit_equal(out, ref, 'coords')
+
+
+@pytest.mark.parametrize('clip', [False, True])
+@pytest.mark.parametrize('cfg', CASES, ids=case_id)
+def test_pixels_and_pmat(gpu, cfg, clip):
+    proj, tiled, interp = cfg
+    if interp:
+        pytest.skip('pixels()/pointing_matrix() do not depend on interpolation')
+    c = Case(proj, tiled=tiled, interp=interp, clip=clip)
+    pix = np.array(c.engine.pixels(c.bore, c.dets))
+    assert_bit_equal(pix, oracle.pixels(c.og, c.bore, c.dets), 'pixels')
+    gp, gs = c.engine.pointing_matrix(c.bore, c.dets, c.resps)
+    rp, rs = oracle.pointing_matrix(c.og, c.bore, c.dets, c.resps)
+    assert_bit_equal(np.array(gp), rp, 'pmat pixels')
+    assert_bit_equal(np.array(gs), rs, 'pmat spin')
+
+
+@pytest.mark.parametrize('comps', ['T', 'QU', 'TQU'])
+@pytest.mark.parametrize('cfg', CASES, ids=case_id)
+def test_map_ops(gpu, cfg, comps):
+    proj, tiled, interp = cfg
+    c = Case(proj, comps=comps, tiled=tiled, interp=interp, clip=(proj == 'CAR'))
+    nc = c.ncomp
+    # from_map into a fresh signal and accumulated onto an existing one.
+    m = c.map_like([nc])
+    sig = np.array(c.engine.from_map(m, c.bore, c.dets, c.resps))
+    assert_bit_equal(sig, oracle.from_map(c.og, m, c.bore, c.dets, c.resps), 'from_map')
+    base = c.signal.copy()
+    out = c.engine.from_map(m, c.bore, c.dets, c.resps, base)
+    assert_bit_equal(np.array(out), oracle.from_map(c.og, m, c.bore, c.dets, c.resps,
+                                                    c.signal), 'from_map accumulate')
+    assert_bit_equal(base, np.array(out), 'from_map in place')
+    # to_map with and without det_weights, into zeros and into a given map.
+    for dw in (None, c.det_weights):
+        g = c.engine.to_map(None, c.bore, c.dets, c.resps, c.signal, dw)
+        r = oracle.to_map(c.og, c.zeros([nc]), c.bore, c.dets, c.resps, c.signal, dw)
+        maps_close(g, r, 'to_map')
+    m0 = c.map_like([nc], seed=5)
+    g = c.engine.to_map(copy_map(m0), c.bore, c.dets, c.resps, c.signal)
+    r = oracle.to_map(c.og, copy_map(m0), c.bore, c.dets, c.resps, c.signal)
+    maps_close(g, r, 'to_map accumulate')
+    # to_weights.
+    for dw in (None, c.det_weights):
+        g = c.engine.to_weight_map(None, c.bore, c.dets, c.resps, dw)
+        r = oracle.to_weights(c.og, c.zeros([nc, nc]), c.bore, c.dets, c.resps, dw)
+        maps_close(g, r, 'to_weights')
+
+
+@pytest.mark.parametrize('cfg', HP_CASES, ids=hp_id)
+def test_healpix(gpu, cfg):
+    _, tiled, nest = cfg
+    for nside in ([128, 4096] if not tiled else [128]):
+        c = Case('HP', comps='TQU', tiled=tiled, nest=nest, nside=nside, n_det=4,
+                 n_samp=4000)
+        pix = np.array(c.engine.pixels(c.bore, c.dets))
+        assert_bit_equal(pix, oracle.pixels(c.og, c.bore, c.dets), f'HP pixels {nside}')
+        m = c.map_like([3])
+        sig = np.array(c.engine.from_map(m, c.bore, c.dets, c.resps))
+        assert_bit_equal(sig, oracle.from_map(c.og, m, c.bore, c.dets, c.resps), 'HP from_map')
+        g = c.engine.to_map(None, c.bore, c.dets, c.resps, c.signal)
+        r = oracle.to_map(c.og, c.zeros([3]), c.bore, c.dets, c.resps, c.signal)
+        maps_close(g, r, 'HP to_map')
+        g = c.engine.to_weight_map(None, c.bore, c.dets, c.resps)
+        r = oracle.to_weights(c.og, c.zeros([3, 3]), c.bore, c.dets, c.resps)
+        maps_close(g, r, 'HP to_weights')
+
+
+def _segments(n_det, n_t, seed):
+    """Overlapping, repeated and empty segments, as thread_intervals."""
+    from so3g_amd.proj.ranges import Ranges, RangesMatrix
+    rng = np.random.default_rng(seed)
+    threads = []
+    for th in range(3):
+        rows = []
+        for d in range(n_det):
+            r = Ranges(n_t)
+            for _ in range(rng.integers(0, 4)):
+                a = int(rng.integers(0, n_t))
+                r.add_interval(a, min(n_t, a + int(rng.integers(1, 700))))
+            rows.append(r)
+        threads.append(RangesMatrix(rows))
+    return [RangesMatrix(threads)]
+
+
+def _flat_segs(ti):
+    out = []
+    for bunch in ti:
+        for thread in bunch.ranges:
+            for d, r in enumerate(thread.ranges):
+                for lo, hi in r.ranges():
+                    out.append((d, lo, hi))
+    return np.array(out, np.int32).reshape(-1, 3)
+
+
+@pytest.mark.parametrize('proj,tiled', [('CAR', False), ('CAR', True), ('TAN', False)])
+def test_thread_intervals(gpu, proj, tiled):
+    c = Case(proj, comps='TQU', tiled=tiled)
+    ti = _segments(c.dets.shape[0], c.bore.shape[0], seed=11)
+    segs = _flat_segs(ti)
+    g = c.engine.to_map(None, c.bore, c.dets, c.resps, c.signal, None, ti)
+    r = oracle.to_map(c.og, c.zeros([3]), c.bore, c.dets, c.resps, c.signal, segs=segs)
+    maps_close(g, r, 'to_map threads')
+    g = c.engine.to_weight_map(None, c.bore, c.dets, c.resps, c.det_weights, ti)
+    r = oracle.to_weights(c.og, c.zeros([3, 3]), c.bore, c.dets, c.resps, c.det_weights,
+                          segs=segs)
+    maps_close(g, r, 'to_weights threads')
+
+
+@pytest.mark.parametrize('tiled', [False, True])
+def test_precomp(gpu, tiled):
+    import so3g_amd
+    c = Case('CAR', comps='TQU', tiled=tiled, clip=True)
+    pix, spin = c.engine.pointing_matrix(c.bore, c.dets, c.resps)
+    pix, spin = np.array(pix), np.array(spin)
+    pe = so3g_amd.ProjEng_Precomp_Tiled() if tiled else so3g_amd.ProjEng_Precomp_NonTiled()
+    m = c.map_like([3])
+    g = np.array(pe.from_map(m, pix, spin))
+    assert_bit_equal(g, oracle.precomp_from_map(c.og, m, pix, spin), 'precomp from_map')
+    # Precomputed and on-the-fly from_map agree within 1e-6 std(map)
+    # (demos/demo_proj1.py:214-219).
+    otf = np.array(c.engine.from_map(m, c.bore, c.dets, c.resps))
+    assert np.abs(g - otf).max() <= 1e-6 * np.std(flat_map(m)) * 10
+    for dw in (None, c.det_weights):
+        g = pe.to_map(c.zeros([3]), pix, spin, c.signal, dw)
+        r = oracle.precomp_to_map(c.og, c.zeros([3]), pix, spin, c.signal, dw)
+        maps_close(g, r, 'precomp to_map')
+        # Precomputed == on-the-fly to_map (demo_proj1.py:208-212).
+        o = c.engine.to_map(None, c.bore, c.dets, c.resps, c.signal, dw)
+        maps_close(g, o, 'precomp vs on-the-fly to_map')
+        g = pe.to_weight_map(c.zeros([3, 3]), pix, spin, dw)
+        r = oracle.precomp_to_weights(c.og, c.zeros([3, 3]), pix, spin, dw)
+        maps_close(g, r, 'precomp to_weights')
+
+
+@pytest.mark.parametrize('proj,tiled,interp', [('CAR', True, 0), ('CAR', True, 1),
+                                               ('TAN', True, 0)])
+def test_tile_hits_and_ranges(gpu, proj, tiled, interp):
+    c = Case(proj, comps='T', tiled=tiled, interp=interp)
+    hits = c.engine.tile_hits(c.bore, c.dets)
+    ref = oracle.tile_hits(c.og, c.bore, c.dets, len(hits))
+    assert list(ref) == list(hits)
+    n_dom = 3
+    ti = c.engine.pixel_ranges(c.bore, c.dets, None, n_dom)
+    dom = oracle.sample_domains(c.og, c.bore, c.dets, 0, n_dom)
+    _check_ranges(ti, dom, n_dom)
+    tile_lists = [[t for t in range(len(hits)) if t % 2 == 0],
+                  [t for t in range(len(hits)) if t % 2 == 1]]
+    ti = c.engine.tile_ranges(c.bore, c.dets, tile_lists)
+    tdom = np.full(len(hits), -1, np.int32)
+    for i, tl in enumerate(tile_lists):
+        tdom[tl] = i
+    dom = oracle.sample_domains(c.og, c.bore, c.dets, 2, 2, tile_dom=tdom)
+    _check_ranges(ti, dom, 2)
+
+
+@pytest.mark.parametrize('interp', [0, 1])
+def test_pixel_ranges_nontiled(gpu, interp):
+    c = Case('CAR', comps='T', interp=interp, clip=True)
+    ti = c.engine.pixel_ranges(c.bore, c.dets, None, 4)
+    _check_ranges(ti, oracle.sample_domains(c.og, c.bore, c.dets, 0, 4), 4)
+    # Domain from a thread map.
+    tmap = np.floor(np.random.default_rng(3).uniform(0, 3, (1,) + c.zeros([1]).shape[1:]))
+    ti = c.engine.pixel_ranges(c.bore, c.dets, tmap, 3)
+    _check_ranges(ti, oracle.sample_domains(c.og, c.bore, c.dets, 1, 3, tmap=tmap), 3)
+
+
+def _check_ranges(ti, dom, n_dom):
+    """Re-paint the returned [bunch][domain][det] Ranges as a per-sample
+    domain array and compare with the oracle's."""
+    assert len(ti) == 2 and len(ti[0]) == n_dom and len(ti[1]) == 1
+    n_det, n_t = dom.shape
+    paint = np.full((n_det, n_t), -1, np.int32)
+    for k in range(n_dom):
+        for d in range(n_det):
+            for lo, hi in ti[0][k][d].ranges():
+                assert np.all(paint[d, lo:hi] == -1)
+                paint[d, lo:hi] = k
+    for d in range(n_det):
+        for lo, hi in ti[1][0][d].ranges():
+            assert np.all(paint[d, lo:hi] == -1)
+            paint[d, lo:hi] = n_dom
+    expect = np.where(dom >= 0, dom, -1)
+    assert np.array_equal(paint, expect)
+
+
+def test_small_config_full_size(gpu):
+    """The small workload at full size (100 det x 40000 samples, CAR 1'
+    TQU): pixels, from_map, to_map and to_weights against the oracle."""
+    from so3g_amd import workloads
+    import so3g_amd
+    obs = workloads.make('small')
+    p = obs.projectionist()
+    eng = so3g_amd.ProjEng_CAR_TQU_NonTiled(p._get_pixelizor_args())
+    og = oracle.geom_from_args('CAR', 'TQU', p._get_pixelizor_args())
+    bore, dets, resp = obs.bore, obs.dets, obs.resps
+    sig = workloads.signal(obs.n_det, obs.n_samp)
+    pix = np.array(eng.pixels(bore, dets))
+    assert_bit_equal(pix, oracle.pixels(og, bore, dets), 'pixels')
+    assert (pix[..., 0] >= 0).all()
+    m = workloads.random_map(3, obs.shape)
+    assert_bit_equal(np.array(eng.from_map(m, bore, dets, resp)),
+                     oracle.from_map(og, m, bore, dets, resp), 'from_map')
+    maps_close(eng.to_map(None, bore, dets, resp, sig),
+               oracle.to_map(og, np.zeros((3,) + obs.shape), bore, dets, resp, sig),
+               'to_map', atol=1e-9)
+    maps_close(eng.to_weight_map(None, bore, dets, resp),
+               oracle.to_weights(og, np.zeros((3, 3) + obs.shape), bore, dets, resp),
+               'to_weights', atol=1e-9)
+
+
+def test_projectionist_torch_path(gpu):
+    """Projectionist with torch device tensors (zero-copy) matches numpy."""
+    import torch
+    from so3g_amd import workloads
+    obs = workloads.make('small', n_det=20, n_samp=8000)
+    p = obs.projectionist()
+    sig = workloads.signal(20, 8000)
+    m_np = p.to_map(sig, obs.asm, comps='TQU')
+    eng = p.get_ProjEng('TQU')
+    q = p._cache_q_fp_to_native(obs.asm.Q)
+    dev = torch.device('cuda')
+    m_t = torch.zeros((3,) + obs.shape, dtype=torch.float64, device=dev)
+    eng.to_map(m_t, torch.from_numpy(np.asarray(q, np.float64)).to(dev),
+               torch.from_numpy(obs.dets).to(dev), torch.from_numpy(obs.resps).to(dev),
+               torch.from_numpy(sig).to(dev))
+    assert np.abs(m_t.cpu().numpy() - m_np).max() < 1e-9
+    s_t = eng.from_map(m_t, torch.from_numpy(np.asarray(q, np.float64)).to(dev),
+                       torch.from_numpy(obs.dets).to(dev),
+                       torch.from_numpy(obs.resps).to(dev))
+    assert isinstance(s_t, torch.Tensor)
+    s_np = np.array(p.from_map(m_np, obs.asm))
+    assert np.abs(s_t.cpu().numpy() - s_np).max() < 1e-3
