@@ -205,6 +205,15 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape,
                             const float *det_weights, const int32_t *segs,
                             int64_t n_seg, void *stream);
 
+/* Implementation tunables (no counterpart in the reference):
+ *   "scatter_agg"   1: sum runs of equal pixels within a wave before the
+ *                   map atomics (default 1)
+ *   "scatter_xcd"   1: XCD-private partial maps for non-tiled scatters,
+ *                   reduced after the kernel (default 0)
+ *   "xcd_budget_mb" device memory allowed for those copies (default 4096) */
+int so3g_set_option(const char *name, int64_t value);
+int so3g_get_option(const char *name, int64_t *value);
+
 #ifdef __cplusplus
 }
 #endif

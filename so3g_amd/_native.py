@@ -59,6 +59,8 @@ _SIGS = {
     'so3g_tile_hits': (c_int, [ctypes.POINTER(Geom), c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
     'so3g_sample_domains': (c_int, [ctypes.POINTER(Geom), c_vp, c_i64, c_vp, c_i64, c_int,
                                     c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    'so3g_set_option': (c_int, [ctypes.c_char_p, c_i64]),
+    'so3g_get_option': (c_int, [ctypes.c_char_p, ctypes.POINTER(c_i64)]),
     'so3g_precomp_from_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_i64,
                                       c_i64, c_vp, c_int, c_vp]),
     'so3g_precomp_to_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_i64,
@@ -130,3 +132,13 @@ def get_tables():
     b = np.zeros(1 << 14)
     check(load().so3g_get_tables(a.ctypes.data, b.ctypes.data))
     return a, b
+
+
+def set_option(name, value):
+    check(load().so3g_set_option(name.encode(), int(value)))
+
+
+def get_option(name):
+    v = c_i64(0)
+    check(load().so3g_get_option(name.encode(), ctypes.byref(v)))
+    return int(v.value)

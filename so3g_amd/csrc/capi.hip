@@ -52,7 +52,19 @@ struct State {
     int *flags_h = nullptr;     // pinned
     Chunk *chunks_d = nullptr;
     size_t chunk_cap = 0;
+    double *copies_d = nullptr;   // XCD-private partial maps
+    size_t copies_cap = 0;        // in doubles
 };
+
+// Tunables (so3g_set_option): wave-level run aggregation of scatter-adds,
+// XCD-private partial maps, and the memory budget for the latter.
+struct Options {
+    int scatter_agg = 1;
+    int scatter_xcd = 0;
+    int64_t xcd_budget_mb = 4096;
+};
+Options g_opt;
+constexpr int kNumXcd = 8;
 State g_state;
 std::mutex g_mutex;
 
@@ -190,6 +202,7 @@ int base_args(const so3g_geom *g, const double *bore, int64_t n_t, const double 
     // Detectors per block: at least 4 (boresight reuse), more when the
     // detector count would overflow gridDim.y.
     a.in.dpb = (int32_t)std::max<int64_t>(4, (n_det + 65534) / 65535);
+    a.in.agg = g_opt.scatter_agg;
     a.interp = g->interp;
     a.nc = g->ncomp;
     a.stream = (hipStream_t)stream;
@@ -534,6 +547,72 @@ int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
     return k == 3 ? check_tile_flag(s) : SO3G_OK;
 }
 
+// Launches the precomp to_map variant selected by the options.
+extern "C++" template <int K, int NC>
+static int pc_to_map_launch(dim3 grid, hipStream_t s, const PcIn &in, const PcMap &pm,
+                            double *map, double *const *tiles, const int32_t *pixel,
+                            const float *spin, const float *signal,
+                            const float *det_weights) {
+    const int64_t n = (int64_t)NC * pm.ny * pm.nx;
+    bool xcd = g_opt.scatter_xcd && K < 3 &&
+               n * kNumXcd * 8 <= g_opt.xcd_budget_mb * (int64_t)(1 << 20);
+    if (xcd && (size_t)(n * kNumXcd) > g_state.copies_cap) {
+        if (g_state.copies_d) HIP_TRY(hipFree(g_state.copies_d));
+        g_state.copies_d = nullptr;
+        g_state.copies_cap = 0;
+        if (hipMalloc(&g_state.copies_d, n * kNumXcd * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            xcd = false;
+        } else {
+            g_state.copies_cap = n * kNumXcd;
+        }
+    }
+    const dim3 block(kBlock);
+    double *cp = xcd ? g_state.copies_d : nullptr;
+    if (xcd)
+        hipLaunchKernelGGL(k_fill_zero<double>, dim3(2048), block, 0, s, cp, n * kNumXcd);
+    if (g_opt.scatter_agg) {
+        if (xcd)
+            hipLaunchKernelGGL((k_pc_to_map<K, NC, true, true>), grid, block, 0, s, in, pm, map,
+                               tiles, pixel, spin, signal, det_weights, cp, n, g_state.flags_d);
+        else
+            hipLaunchKernelGGL((k_pc_to_map<K, NC, true, false>), grid, block, 0, s, in, pm,
+                               map, tiles, pixel, spin, signal, det_weights, cp, n,
+                               g_state.flags_d);
+    } else {
+        if (xcd)
+            hipLaunchKernelGGL((k_pc_to_map<K, NC, false, true>), grid, block, 0, s, in, pm,
+                               map, tiles, pixel, spin, signal, det_weights, cp, n,
+                               g_state.flags_d);
+        else
+            hipLaunchKernelGGL((k_pc_to_map<K, NC, false, false>), grid, block, 0, s, in, pm,
+                               map, tiles, pixel, spin, signal, det_weights, cp, n,
+                               g_state.flags_d);
+    }
+    if (xcd)
+        hipLaunchKernelGGL(k_reduce_copies<double>, dim3(2048), block, 0, s, map, cp, n, n, kNumXcd);
+    HIP_TRY(hipGetLastError());
+    return SO3G_OK;
+}
+
+int so3g_set_option(const char *name, int64_t value) {
+    std::string n(name ? name : "");
+    if (n == "scatter_agg") g_opt.scatter_agg = (int)(value != 0);
+    else if (n == "scatter_xcd") g_opt.scatter_xcd = (int)(value != 0);
+    else if (n == "xcd_budget_mb") g_opt.xcd_budget_mb = value;
+    else return fail(SO3G_ERR_VALUE, "unknown option " + n);
+    return SO3G_OK;
+}
+
+int so3g_get_option(const char *name, int64_t *value) {
+    std::string n(name ? name : "");
+    if (n == "scatter_agg") *value = g_opt.scatter_agg;
+    else if (n == "scatter_xcd") *value = g_opt.scatter_xcd;
+    else if (n == "xcd_budget_mb") *value = g_opt.xcd_budget_mb;
+    else return fail(SO3G_ERR_VALUE, "unknown option " + n);
+    return SO3G_OK;
+}
+
 int so3g_precomp_to_map(double *map, const int64_t *map_shape, double *const *tiles,
                         const int32_t *tile_dims, const int32_t *pixel, int32_t k,
                         const float *spin, int32_t nc, int64_t n_det, int64_t n_t,
@@ -556,9 +635,22 @@ int so3g_precomp_to_map(double *map, const int64_t *map_shape, double *const *ti
         r = zero_flags(s);
         if (r) return r;
     }
-    PC_SWITCH(k, nc, k_pc_to_map, pc_grid(in, n_chunks), dim3(kBlock), 0, s, in, pm, map,
-              tiles, pixel, spin, signal, det_weights, g_state.flags_d);
-    HIP_TRY(hipGetLastError());
+    {
+        const dim3 grid = pc_grid(in, n_chunks);
+        switch (k * 4 + nc) {
+        case 5: r = pc_to_map_launch<1, 1>(grid, s, in, pm, map, tiles, pixel, spin, signal, det_weights); break;
+        case 6: r = pc_to_map_launch<1, 2>(grid, s, in, pm, map, tiles, pixel, spin, signal, det_weights); break;
+        case 7: r = pc_to_map_launch<1, 3>(grid, s, in, pm, map, tiles, pixel, spin, signal, det_weights); break;
+        case 9: r = pc_to_map_launch<2, 1>(grid, s, in, pm, map, tiles, pixel, spin, signal, det_weights); break;
+        case 10: r = pc_to_map_launch<2, 2>(grid, s, in, pm, map, tiles, pixel, spin, signal, det_weights); break;
+        case 11: r = pc_to_map_launch<2, 3>(grid, s, in, pm, map, tiles, pixel, spin, signal, det_weights); break;
+        case 13: r = pc_to_map_launch<3, 1>(grid, s, in, pm, map, tiles, pixel, spin, signal, det_weights); break;
+        case 14: r = pc_to_map_launch<3, 2>(grid, s, in, pm, map, tiles, pixel, spin, signal, det_weights); break;
+        case 15: r = pc_to_map_launch<3, 3>(grid, s, in, pm, map, tiles, pixel, spin, signal, det_weights); break;
+        default: return fail(SO3G_ERR_RUNTIME, "unsupported pixel/spin layout");
+        }
+        if (r) return r;
+    }
     return k == 3 ? check_tile_flag(s) : SO3G_OK;
 }
 
@@ -585,7 +677,7 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape, double *const
         if (r) return r;
     }
     PC_SWITCH(k, nc, k_pc_to_weights, pc_grid(in, n_chunks), dim3(kBlock), 0, s, in, pm,
-              map, tiles, pixel, spin, det_weights, g_state.flags_d);
+              map, tiles, pixel, spin, det_weights, g_opt.scatter_agg, g_state.flags_d);
     HIP_TRY(hipGetLastError());
     return k == 3 ? check_tile_flag(s) : SO3G_OK;
 }

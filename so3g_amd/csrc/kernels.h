@@ -11,6 +11,8 @@
 // one chunk (det, t0, t1) of at most 256 samples instead.
 #pragma once
 
+#include <type_traits>
+
 #include "ptg.h"
 
 namespace so3g {
@@ -25,6 +27,7 @@ struct PtgIn {
     int64_t n_t, n_det;
     Tabs tabs;
     int32_t dpb;               // detectors per block (dense mode)
+    int32_t agg;               // wave-level run aggregation of scatter-adds
     const Chunk *chunks;       // segment mode when non-null
 };
 
@@ -43,6 +46,43 @@ __device__ __forceinline__ void for_samples(const PtgIn &in, F &&f) {
     const int64_t d0 = (int64_t)blockIdx.y * in.dpb;
     const int64_t d1 = min(d0 + in.dpb, in.n_det);
     for (int64_t d = d0; d < d1; d++) f(d, t, qb);
+}
+
+// Hardware id (0..7 on MI355X) of the XCD -- the chiplet with its own L2 --
+// that runs the calling wave (HW_REG_XCC_ID, gfx940 and later).
+__device__ __forceinline__ int xcc_id() {
+    unsigned x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+    return (int)(x & 7);
+}
+
+// Wave-level run aggregation for scatter-adds.  The 64 lanes of a wave hold
+// (key, v[NC]) for consecutive samples; a key < 0 marks a lane with nothing
+// to add.  Lanes in a run of equal keys sum their values (segmented
+// inclusive scan) so that the run's last lane holds the run total; returns
+// true for that lane.  A detector crossing a map pixel stays in it for a
+// few consecutive samples, so this divides the number of memory atomics by
+// the mean run length.  All 64 lanes must call this together.
+template <int NC, typename VT>
+__device__ __forceinline__ bool wave_runs(int64_t key, VT (&v)[NC]) {
+    const int lane = (int)__lane_id();
+    const int64_t kprev = __shfl_up(key, 1);
+    int head = (lane == 0 || kprev != key) ? lane : 0;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int h = __shfl_up(head, off);
+        if (lane >= off) head = max(head, h);
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const VT u = __shfl_up(v[c], off);
+            if (lane - off >= head) v[c] += u;
+        }
+    }
+    const int64_t knext = __shfl_down(key, 1);
+    return lane == 63 || knext != key;
 }
 
 // Library-owned device flags.
@@ -185,43 +225,104 @@ struct PrecT { using map_t = double; };
 template <>
 struct PrecT<1> { using map_t = float; };
 
+// Calls f(det, t, qbore, valid) on every lane of the block for every
+// detector of the block's group (segment mode: the chunk's detector);
+// lanes past the end of the samples get valid == false, so that wave-level
+// collectives see all 64 lanes.
+template <class F>
+__device__ __forceinline__ void for_samples_all(const PtgIn &in, F &&f) {
+    int64_t d0, d1, t, t_end;
+    if (in.chunks != nullptr) {
+        const Chunk c = in.chunks[blockIdx.x];
+        d0 = c.det;
+        d1 = d0 + 1;
+        t = c.t0 + (int64_t)threadIdx.x;
+        t_end = c.t1;
+    } else {
+        t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        t_end = in.n_t;
+        d0 = (int64_t)blockIdx.y * in.dpb;
+        d1 = min(d0 + in.dpb, in.n_det);
+    }
+    const bool valid = t < t_end;
+    const Q4 qb = valid ? load_q(in.bore + 4 * t) : Q4{1., 0., 0., 0.};
+    for (int64_t d = d0; d < d1; d++) f(d, t, qb, valid);
+}
+
+// Scatter-add of NV values per point to e + off[k] for the NP interpolation
+// points of each lane's sample (key < 0: no point).  With agg, runs of equal
+// keys within the wave are summed first (wave_runs) and only each run's
+// last lane issues the atomics.
+template <int NP, int NV, typename MT, typename VT>
+__device__ __forceinline__ void scatter_points(bool agg, MT *const (&e)[NP],
+                                               const int64_t (&key)[NP],
+                                               const int64_t (&off)[NP][NV],
+                                               VT (&v)[NP][NV]) {
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+        bool emit = key[j] >= 0;
+        if (agg) emit = wave_runs<NV>(key[j], v[j]) && key[j] >= 0;
+        if (emit) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) atomic_add(e[j] + off[j][k], (MT)v[j][k]);
+        }
+    }
+}
+
 template <int PROJ, int PK, int INTERP, int NC, int PREC>
 __global__ __launch_bounds__(kBlock) void k_to_map(PtgIn in, Geo g,
                                                    MapView<typename PrecT<PREC>::map_t> m,
                                                    const float *resp, const float *sig,
                                                    const float *det_weights, int *flags) {
     using MT = typename PrecT<PREC>::map_t;
-    for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
+    using VT = typename std::conditional<PREC == 1, float, double>::type;
+    constexpr int NP = INTERP ? 4 : 1;
+    for_samples_all(in, [&](int64_t d, int64_t t, const Q4 &qb, bool valid) {
         float det_wt = 1.f;
         if (det_weights != nullptr) {
             det_wt = det_weights[d];
             if (det_wt == 0.f) return;
         }
-        const Q4 qd = load_q(in.dets + 4 * d);
-        double co[4];
-        get_coords<PROJ>(qb, qd, in.tabs, co);
-        float pf[3];
-        spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
-        const float s = sig[d * in.n_t + t];
-        Pt p[4];
-        const int np = get_pixels<PK, INTERP>(g, co, p);
-        for (int ip = 0; ip < np; ip++) {
-            MT *e = elem<PK>(g, m, p[ip], 0);
-            if (e == nullptr) { raise_flag(flags, FLAG_TILE); continue; }
-            const int64_t plane = (PK == PK_FLAT_TILED)
-                ? (int64_t)m.tile_dims[2 * p[ip].i0] * m.tile_dims[2 * p[ip].i0 + 1]
-                : (PK == PK_HP_TILED ? (int64_t)g.npt : g.npix);
+        MT *e[NP];
+        int64_t key[NP], off[NP][NC];
+        VT v[NP][NC];
 #pragma unroll
-            for (int c = 0; c < NC; c++) {
-                if constexpr (PREC == 2) {
-                    const double v = (double)s * (double)pf[c] * (double)p[ip].w * (double)det_wt;
-                    atomic_add(e + c * plane, v);
-                } else {
-                    const float v = s * pf[c] * p[ip].w * det_wt;
-                    atomic_add(e + c * plane, (MT)v);
+        for (int j = 0; j < NP; j++) {
+            key[j] = -1;
+            e[j] = nullptr;
+#pragma unroll
+            for (int c = 0; c < NC; c++) { v[j][c] = 0; off[j][c] = 0; }
+        }
+        if (valid) {
+            const Q4 qd = load_q(in.dets + 4 * d);
+            double co[4];
+            get_coords<PROJ>(qb, qd, in.tabs, co);
+            float pf[3];
+            spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
+            const float s = sig[d * in.n_t + t];
+            Pt p[4];
+            const int np = get_pixels<PK, INTERP>(g, co, p);
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                if (j >= np) break;
+                MT *ej = elem<PK>(g, m, p[j], 0);
+                if (ej == nullptr) { raise_flag(flags, FLAG_TILE); continue; }
+                const int64_t plane = (PK == PK_FLAT_TILED)
+                    ? (int64_t)m.tile_dims[2 * p[j].i0] * m.tile_dims[2 * p[j].i0 + 1]
+                    : (PK == PK_HP_TILED ? (int64_t)g.npt : g.npix);
+                e[j] = ej;
+                key[j] = (int64_t)(uintptr_t)ej;
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    off[j][c] = c * plane;
+                    if constexpr (PREC == 2)
+                        v[j][c] = (double)s * (double)pf[c] * (double)p[j].w * (double)det_wt;
+                    else
+                        v[j][c] = (VT)(s * pf[c] * p[j].w * det_wt);
                 }
             }
         }
+        scatter_points<NP, NC, MT, VT>(in.agg, e, key, off, v);
     });
 }
 
@@ -231,40 +332,59 @@ __global__ __launch_bounds__(kBlock) void k_to_weights(PtgIn in, Geo g,
                                                        const float *resp,
                                                        const float *det_weights, int *flags) {
     using MT = typename PrecT<PREC>::map_t;
-    for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
+    using VT = typename std::conditional<PREC == 1, float, double>::type;
+    constexpr int NP = INTERP ? 4 : 1;
+    constexpr int NV = NC * (NC + 1) / 2;
+    for_samples_all(in, [&](int64_t d, int64_t t, const Q4 &qb, bool valid) {
         float det_wt = 1.f;
         if (det_weights != nullptr) {
             det_wt = det_weights[d];
             if (det_wt == 0.f) return;
         }
-        const Q4 qd = load_q(in.dets + 4 * d);
-        double co[4];
-        get_coords<PROJ>(qb, qd, in.tabs, co);
-        float pf[3];
-        spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
-        Pt p[4];
-        const int np = get_pixels<PK, INTERP>(g, co, p);
-        for (int ip = 0; ip < np; ip++) {
-            MT *e = elem<PK>(g, m, p[ip], 0);
-            if (e == nullptr) { raise_flag(flags, FLAG_TILE); continue; }
-            const int64_t plane = (PK == PK_FLAT_TILED)
-                ? (int64_t)m.tile_dims[2 * p[ip].i0] * m.tile_dims[2 * p[ip].i0 + 1]
-                : (PK == PK_HP_TILED ? (int64_t)g.npt : g.npix);
-            const float w = p[ip].w;
+        MT *e[NP];
+        int64_t key[NP], off[NP][NV];
+        VT v[NP][NV];
 #pragma unroll
-            for (int i = 0; i < NC; i++)
+        for (int j = 0; j < NP; j++) {
+            key[j] = -1;
+            e[j] = nullptr;
 #pragma unroll
-                for (int j = i; j < NC; j++) {
-                    if constexpr (PREC == 2) {
-                        const double v = (double)pf[i] * (double)pf[j] * (double)w *
-                                         (double)w * (double)det_wt;
-                        atomic_add(e + (i * NC + j) * plane, v);
-                    } else {
-                        const float v = pf[i] * pf[j] * w * w * det_wt;
-                        atomic_add(e + (i * NC + j) * plane, (MT)v);
-                    }
-                }
+            for (int k = 0; k < NV; k++) { v[j][k] = 0; off[j][k] = 0; }
         }
+        if (valid) {
+            const Q4 qd = load_q(in.dets + 4 * d);
+            double co[4];
+            get_coords<PROJ>(qb, qd, in.tabs, co);
+            float pf[3];
+            spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
+            Pt p[4];
+            const int np = get_pixels<PK, INTERP>(g, co, p);
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                if (j >= np) break;
+                MT *ej = elem<PK>(g, m, p[j], 0);
+                if (ej == nullptr) { raise_flag(flags, FLAG_TILE); continue; }
+                const int64_t plane = (PK == PK_FLAT_TILED)
+                    ? (int64_t)m.tile_dims[2 * p[j].i0] * m.tile_dims[2 * p[j].i0 + 1]
+                    : (PK == PK_HP_TILED ? (int64_t)g.npt : g.npix);
+                e[j] = ej;
+                key[j] = (int64_t)(uintptr_t)ej;
+                const float w = p[j].w;
+                int k = 0;
+#pragma unroll
+                for (int a = 0; a < NC; a++)
+#pragma unroll
+                    for (int b = a; b < NC; b++, k++) {
+                        off[j][k] = (a * NC + b) * plane;
+                        if constexpr (PREC == 2)
+                            v[j][k] = (double)pf[a] * (double)pf[b] * (double)w * (double)w *
+                                      (double)det_wt;
+                        else
+                            v[j][k] = (VT)(pf[a] * pf[b] * w * w * det_wt);
+                    }
+            }
+        }
+        scatter_points<NP, NV, MT, VT>(in.agg, e, key, off, v);
     });
 }
 

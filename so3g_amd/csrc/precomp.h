@@ -86,66 +86,158 @@ __global__ __launch_bounds__(kBlock) void k_pc_from_map(PcIn in, PcMap pm, const
     });
 }
 
-// Precomp to_map, Projection.cxx:2100-2121: (det_wt * sig) * pf in float32.
-template <int K, int NC>
+// Precomp to_map, Projection.cxx:2100-2121: (det_wt * sig) * pf in float32,
+// added to the float64 map.  AGG sums runs of equal pixels within a wave
+// before the atomics (wave_runs); XCD sends each XCD's atomics to its own
+// private copy of the map (copies + xcc * copy_stride), summed afterwards
+// by k_reduce_copies.  All lanes of a block stay alive to the end (the
+// wave-level scan needs them); lanes without a sample add nothing.
+template <int K, int NC, bool AGG, bool XCD>
 __global__ __launch_bounds__(kBlock) void k_pc_to_map(PcIn in, PcMap pm, double *map,
                                                       double *const *tiles, const int32_t *pix,
                                                       const float *spin, const float *sig,
-                                                      const float *det_weights, int *flags) {
-    for_idx(in, [&](int64_t d, int64_t t) {
+                                                      const float *det_weights, double *copies,
+                                                      int64_t copy_stride, int *flags) {
+    double *mbase = map;
+    if constexpr (XCD) mbase = copies + (int64_t)xcc_id() * copy_stride;
+    int64_t d0, d1, t;
+    int t_end;
+    if (in.chunks != nullptr) {
+        const Chunk c = in.chunks[blockIdx.x];
+        d0 = c.det;
+        d1 = d0 + 1;
+        t = c.t0 + (int64_t)threadIdx.x;
+        t_end = c.t1;
+    } else {
+        t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        t_end = (int)in.n_t;
+        d0 = (int64_t)blockIdx.y * in.dpb;
+        d1 = min(d0 + in.dpb, in.n_det);
+    }
+    const bool valid = t < t_end;
+    for (int64_t d = d0; d < d1; d++) {
         float det_wt = 1.f;
         if (det_weights != nullptr) {
             det_wt = det_weights[d];
-            if (det_wt == 0.f) return;
+            if (det_wt == 0.f) continue;
         }
-        const int64_t i = d * in.n_t + t;
-        int32_t pi[K];
+        int64_t key = -1;
+        double v[NC];
+        int64_t plane = 0;
+        double *e = nullptr;
 #pragma unroll
-        for (int j = 0; j < K; j++) pi[j] = pix[i * K + j];
-        if (pi[0] < 0) return;
-        int64_t plane;
-        double *e = pc_elem<K>(map, tiles, pm, pi, &plane);
-        if (e == nullptr) { raise_flag(flags, FLAG_TILE); return; }
-        const float s = sig[i];
+        for (int c = 0; c < NC; c++) v[c] = 0.;
+        if (valid) {
+            const int64_t i = d * in.n_t + t;
+            int32_t pi[K];
 #pragma unroll
-        for (int c = 0; c < NC; c++) {
-            const float v = det_wt * s * spin[i * NC + c];
-            unsafeAtomicAdd(e + c * plane, (double)v);
+            for (int j = 0; j < K; j++) pi[j] = pix[i * K + j];
+            if (pi[0] >= 0) {
+                e = pc_elem<K>(mbase, tiles, pm, pi, &plane);
+                if (e == nullptr) {
+                    raise_flag(flags, FLAG_TILE);
+                } else {
+                    key = (int64_t)(uintptr_t)e;
+                    const float s = sig[i];
+#pragma unroll
+                    for (int c = 0; c < NC; c++)
+                        v[c] = (double)(det_wt * s * spin[i * NC + c]);
+                }
+            }
         }
-    });
+        bool emit = key >= 0;
+        if constexpr (AGG) emit = wave_runs<NC>(key, v) && key >= 0;
+        if (emit) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) unsafeAtomicAdd(e + c * plane, v[c]);
+        }
+    }
 }
 
-// Precomp to_weight_map, Projection.cxx:2135-2154: pf_a * pf_b * det_wt.
+// map[i] += sum over the n_copy XCD-private partial maps, fixed order.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reduce_copies(T *map, const T *copies, int64_t n,
+                                                          int64_t stride, int n_copy) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        T acc = map[i];
+        for (int c = 0; c < n_copy; c++) acc += copies[c * stride + i];
+        map[i] = acc;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill_zero(T *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        p[i] = T(0);
+}
+
+// Precomp to_weight_map, Projection.cxx:2135-2154: pf_a * pf_b * det_wt,
+// upper triangle; runs of equal pixels summed per wave when agg is set.
 template <int K, int NC>
 __global__ __launch_bounds__(kBlock) void k_pc_to_weights(PcIn in, PcMap pm, double *map,
                                                           double *const *tiles,
                                                           const int32_t *pix,
                                                           const float *spin,
-                                                          const float *det_weights,
+                                                          const float *det_weights, int agg,
                                                           int *flags) {
-    for_idx(in, [&](int64_t d, int64_t t) {
+    constexpr int NV = NC * (NC + 1) / 2;
+    int64_t d0, d1, t, t_end;
+    if (in.chunks != nullptr) {
+        const Chunk c = in.chunks[blockIdx.x];
+        d0 = c.det;
+        d1 = d0 + 1;
+        t = c.t0 + (int64_t)threadIdx.x;
+        t_end = c.t1;
+    } else {
+        t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        t_end = in.n_t;
+        d0 = (int64_t)blockIdx.y * in.dpb;
+        d1 = min(d0 + in.dpb, in.n_det);
+    }
+    const bool valid = t < t_end;
+    for (int64_t d = d0; d < d1; d++) {
         float det_wt = 1.f;
         if (det_weights != nullptr) {
             det_wt = det_weights[d];
-            if (det_wt == 0.f) return;
+            if (det_wt == 0.f) continue;
         }
-        const int64_t i = d * in.n_t + t;
-        int32_t pi[K];
+        double *e[1] = {nullptr};
+        int64_t key[1] = {-1};
+        int64_t off[1][NV];
+        double v[1][NV];
 #pragma unroll
-        for (int j = 0; j < K; j++) pi[j] = pix[i * K + j];
-        if (pi[0] < 0) return;
-        int64_t plane;
-        double *e = pc_elem<K>(map, tiles, pm, pi, &plane);
-        if (e == nullptr) { raise_flag(flags, FLAG_TILE); return; }
-        float pf[NC];
+        for (int k = 0; k < NV; k++) { v[0][k] = 0.; off[0][k] = 0; }
+        if (valid) {
+            const int64_t i = d * in.n_t + t;
+            int32_t pi[K];
 #pragma unroll
-        for (int c = 0; c < NC; c++) pf[c] = spin[i * NC + c];
+            for (int j = 0; j < K; j++) pi[j] = pix[i * K + j];
+            if (pi[0] >= 0) {
+                int64_t plane;
+                double *ej = pc_elem<K>(map, tiles, pm, pi, &plane);
+                if (ej == nullptr) {
+                    raise_flag(flags, FLAG_TILE);
+                } else {
+                    e[0] = ej;
+                    key[0] = (int64_t)(uintptr_t)ej;
+                    float pf[NC];
 #pragma unroll
-        for (int a = 0; a < NC; a++)
+                    for (int c = 0; c < NC; c++) pf[c] = spin[i * NC + c];
+                    int k = 0;
 #pragma unroll
-            for (int b = a; b < NC; b++)
-                unsafeAtomicAdd(e + (a * NC + b) * plane, (double)(pf[a] * pf[b] * det_wt));
-    });
+                    for (int a = 0; a < NC; a++)
+#pragma unroll
+                        for (int b = a; b < NC; b++, k++) {
+                            off[0][k] = (a * NC + b) * plane;
+                            v[0][k] = (double)(pf[a] * pf[b] * det_wt);
+                        }
+                }
+            }
+        }
+        scatter_points<1, NV, double, double>(agg != 0, e, key, off, v);
+    }
 }
 
 }  // namespace so3g
