@@ -97,6 +97,14 @@ def main():
     t_pmat = time.time() - t0
     log(f'pointing matrix: {t_pmat * 1e3:.1f} ms')
 
+    # Pixel-sorted plan for the scatter (built once per pointing matrix).
+    torch.cuda.synchronize()
+    t0 = time.time()
+    plan = so3g_amd.PrecompPlan(pix, spin, (ny, nx))
+    torch.cuda.synchronize()
+    t_plan = time.time() - t0
+    log(f'plan: {t_plan * 1e3:.1f} ms')
+
     shape = np.array([ny, nx], np.int64)
     sig_out = torch.empty_like(sig)
     mapo = torch.zeros((3, ny, nx), dtype=torch.float64, device=dev)
@@ -107,6 +115,10 @@ def main():
                                       sig_out.data_ptr(), 0, stream))
 
     def pc_to():
+        mapo.zero_()
+        chk(lib.so3g_pc_plan_to_map(plan._h, mapo.data_ptr(), sig.data_ptr(), None, stream))
+
+    def pc_to_direct():
         mapo.zero_()
         chk(lib.so3g_precomp_to_map(mapo.data_ptr(), shape.ctypes.data, None, None,
                                     pix.data_ptr(), 2, spin.data_ptr(), 3, n_det, n_t,
@@ -156,9 +168,11 @@ def main():
     gbs_from = b_from / (ms_from * 1e-3)
     gbs_to = b_to / (ms_to * 1e-3)
     if ms_to >= ms_from:
-        dom = ('k_pc_to_map<2,3>', gbs_to, b_to)
+        dom = ('k_plan_to_map_r<3,8>', gbs_to, b_to)
     else:
-        dom = ('k_pc_from_map<2,3>', gbs_from, b_from)
+        dom = ('k_pc_from_map_v4<3>', gbs_from, b_from)
+    # The plain ProjEng_Precomp scatter (no plan), for reference.
+    _, (ms_to_direct,) = run([pc_to_direct], max(1, min(args.steps, 3)), 1)
 
     otf = None
     if not args.no_otf:
@@ -193,13 +207,14 @@ def main():
         'config': {
             'workload': f'single observation {n_det} det x {n_t} samples, CAR 0.5 arcmin '
                         f'TQU {ny}x{nx} non-tiled; step = from_map + to_map via cached '
-                        f'pointing matrix (ProjEng_Precomp)',
+                        f'pointing matrix (ProjEng_Precomp from_map, PrecompPlan to_map)',
             'n_det': n_det, 'n_samp': n_t, 'map_shape': [3, ny, nx],
-            'pointing': 'pointing_matrix computed once before timing '
-                        f'({t_pmat * 1e3:.1f} ms)',
+            'pointing': 'pointing_matrix and pixel-sorted plan computed once before '
+                        f'timing ({t_pmat * 1e3:.1f} ms + {t_plan * 1e3:.1f} ms)',
         },
         'ms_from_map': ms_from,
         'ms_to_map': ms_to,
+        'ms_to_map_without_plan': ms_to_direct,
         'roofline': {
             'bound': 'hbm', 'kernel': dom[0], 'achieved': dom[1] / 1e9,
             'peak': HBM_PEAK / 1e9, 'unit': 'GB/s', 'frac': dom[1] / HBM_PEAK,

@@ -205,11 +205,32 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape,
                             const float *det_weights, const int32_t *segs,
                             int64_t n_seg, void *stream);
 
+/* Projection plan for repeated cached-pointing to_map calls with one
+ * pointing matrix (no counterpart in the reference; for CG-style loops).
+ * Built once from a Precomp pixel array (k = 2, non-tiled flat map of
+ * map_shape = (ny, nx), host array) and spin array; holds the on-map samples
+ * sorted by pixel and the spin factors in that order (12 + 4*nc bytes per
+ * on-map sample of device memory).  Samples are grouped in time blocks
+ * whose signal slice is about t_block_mb MiB (sized to stay in the
+ * Infinity Cache; <= 0: one block), then sorted by pixel within each block.
+ * so3g_pc_plan_to_map(plan, ...) adds the same contributions as
+ * so3g_precomp_to_map over every sample. */
+typedef struct so3g_pc_plan so3g_pc_plan;
+int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int32_t nc,
+                        int64_t n_det, int64_t n_t, const int64_t *map_shape,
+                        int64_t t_block_mb, so3g_pc_plan **out, void *stream);
+int so3g_pc_plan_to_map(const so3g_pc_plan *plan, double *map, const float *signal,
+                        const float *det_weights, void *stream);
+int so3g_pc_plan_info(const so3g_pc_plan *plan, int64_t *n_on, int64_t *bytes);
+int so3g_pc_plan_destroy(so3g_pc_plan *plan);
+
 /* Implementation tunables (no counterpart in the reference):
  *   "scatter_agg"   1: sum runs of equal pixels within a wave before the
  *                   map atomics (default 1)
  *   "scatter_xcd"   1: XCD-private partial maps for non-tiled scatters,
  *                   reduced after the kernel (default 0)
+ *   "scatter_lds"   1: LDS-privatised partial maps for the cached (Precomp)
+ *                   non-tiled scatter (default 0: measured slower, 12.8 vs 10.9 ms)
  *   "xcd_budget_mb" device memory allowed for those copies (default 4096) */
 int so3g_set_option(const char *name, int64_t value);
 int so3g_get_option(const char *name, int64_t *value);

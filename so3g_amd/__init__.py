@@ -10,6 +10,7 @@ so3g_amd/lib/libso3g_hip.so (build with ``python -m so3g_amd.build``).
 import os
 
 from .engine import make_engine_classes as _make_engine_classes
+from .engine import PrecompPlan  # noqa: F401
 from .proj.ranges import Ranges, RangesInt32, RangesMatrix  # noqa: F401
 
 globals().update(_make_engine_classes())

@@ -59,6 +59,11 @@ _SIGS = {
     'so3g_tile_hits': (c_int, [ctypes.POINTER(Geom), c_vp, c_i64, c_vp, c_i64, c_vp, c_vp]),
     'so3g_sample_domains': (c_int, [ctypes.POINTER(Geom), c_vp, c_i64, c_vp, c_i64, c_int,
                                     c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    'so3g_pc_plan_create': (c_int, [c_vp, c_i32, c_vp, c_i32, c_i64, c_i64, c_vp, c_i64,
+                                    ctypes.POINTER(c_vp), c_vp]),
+    'so3g_pc_plan_to_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    'so3g_pc_plan_info': (c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
+    'so3g_pc_plan_destroy': (c_int, [c_vp]),
     'so3g_set_option': (c_int, [ctypes.c_char_p, c_i64]),
     'so3g_get_option': (c_int, [ctypes.c_char_p, ctypes.POINTER(c_i64)]),
     'so3g_precomp_from_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_i64,

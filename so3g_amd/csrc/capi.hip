@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -54,6 +55,12 @@ struct State {
     size_t chunk_cap = 0;
     double *copies_d = nullptr;   // XCD-private partial maps
     size_t copies_cap = 0;        // in doubles
+    // Detector order for the LDS-privatised scatter, cached per pixel array.
+    int64_t *keys_d = nullptr;
+    int32_t *order_d = nullptr;
+    int64_t order_cap = 0;
+    const void *order_pix = nullptr;
+    int64_t order_ndet = -1, order_nt = -1;
 };
 
 // Tunables (so3g_set_option): wave-level run aggregation of scatter-adds,
@@ -61,6 +68,7 @@ struct State {
 struct Options {
     int scatter_agg = 1;
     int scatter_xcd = 0;
+    int scatter_lds = 0;
     int64_t xcd_budget_mb = 4096;
 };
 Options g_opt;
@@ -211,6 +219,10 @@ int base_args(const so3g_geom *g, const double *bore, int64_t n_t, const double 
 }
 
 }  // namespace
+
+namespace so3g {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace so3g
 
 extern "C" {
 
@@ -541,10 +553,60 @@ int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
         r = zero_flags(s);
         if (r) return r;
     }
+    if (k == 2 && n_t % 4 == 0) {
+        const dim3 g4((unsigned)((n_t / 4 + kBlock - 1) / kBlock),
+                      (unsigned)((n_det + in.dpb - 1) / in.dpb));
+        switch (nc) {
+        case 1: hipLaunchKernelGGL((k_pc_from_map_v4<1>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
+        case 2: hipLaunchKernelGGL((k_pc_from_map_v4<2>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
+        default: hipLaunchKernelGGL((k_pc_from_map_v4<3>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
+        }
+        HIP_TRY(hipGetLastError());
+        return SO3G_OK;
+    }
     PC_SWITCH(k, nc, k_pc_from_map, pc_grid(in, 0), dim3(kBlock), 0, s, in, pm, map, tiles,
               pixel, spin, signal, accumulate, g_state.flags_d);
     HIP_TRY(hipGetLastError());
     return k == 3 ? check_tile_flag(s) : SO3G_OK;
+}
+
+// Detector order for k_pc_to_map_lds: detectors sorted by their pixel at
+// the middle sample (row-major), so each block's detectors sweep the same
+// map rows.  Cached while the same pixel array is passed again.
+static int det_order(const int32_t *pixel, int64_t n_det, int64_t n_t, int64_t nx,
+                     hipStream_t s, const int32_t **order) {
+    if (g_state.order_pix == pixel && g_state.order_ndet == n_det && g_state.order_nt == n_t) {
+        *order = g_state.order_d;
+        return SO3G_OK;
+    }
+    if (n_det > g_state.order_cap) {
+        if (g_state.keys_d) HIP_TRY(hipFree(g_state.keys_d));
+        if (g_state.order_d) HIP_TRY(hipFree(g_state.order_d));
+        g_state.keys_d = nullptr;
+        g_state.order_d = nullptr;
+        HIP_TRY(hipMalloc(&g_state.keys_d, n_det * sizeof(int64_t)));
+        HIP_TRY(hipMalloc(&g_state.order_d, n_det * sizeof(int32_t)));
+        g_state.order_cap = n_det;
+    }
+    hipLaunchKernelGGL(k_det_keys<int32_t>, dim3((unsigned)((n_det + 255) / 256)), dim3(256),
+                       0, s, pixel, n_det, n_t, nx, g_state.keys_d);
+    HIP_TRY(hipGetLastError());
+    std::vector<int64_t> keys(n_det);
+    HIP_TRY(hipMemcpyAsync(keys.data(), g_state.keys_d, n_det * sizeof(int64_t),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<int32_t> idx(n_det);
+    for (int64_t i = 0; i < n_det; i++) idx[i] = (int32_t)i;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int32_t a, int32_t b) { return keys[a] < keys[b]; });
+    HIP_TRY(hipMemcpyAsync(g_state.order_d, idx.data(), n_det * sizeof(int32_t),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    g_state.order_pix = pixel;
+    g_state.order_ndet = n_det;
+    g_state.order_nt = n_t;
+    *order = g_state.order_d;
+    return SO3G_OK;
 }
 
 // Launches the precomp to_map variant selected by the options.
@@ -554,6 +616,19 @@ static int pc_to_map_launch(dim3 grid, hipStream_t s, const PcIn &in, const PcMa
                             const float *spin, const float *signal,
                             const float *det_weights) {
     const int64_t n = (int64_t)NC * pm.ny * pm.nx;
+    if constexpr (K == 2) {
+        if (g_opt.scatter_lds && in.chunks == nullptr && !g_opt.scatter_xcd) {
+            const int32_t *order = nullptr;
+            int r = det_order(pixel, in.n_det, in.n_t, pm.nx, s, &order);
+            if (r) return r;
+            const dim3 g2((unsigned)((in.n_t + kLdsT - 1) / kLdsT),
+                          (unsigned)((in.n_det + kLdsDets - 1) / kLdsDets));
+            hipLaunchKernelGGL((k_pc_to_map_lds<NC>), g2, dim3(kBlock), 0, s, in, pm, map,
+                               pixel, spin, signal, det_weights, order);
+            HIP_TRY(hipGetLastError());
+            return SO3G_OK;
+        }
+    }
     bool xcd = g_opt.scatter_xcd && K < 3 &&
                n * kNumXcd * 8 <= g_opt.xcd_budget_mb * (int64_t)(1 << 20);
     if (xcd && (size_t)(n * kNumXcd) > g_state.copies_cap) {
@@ -599,6 +674,7 @@ int so3g_set_option(const char *name, int64_t value) {
     std::string n(name ? name : "");
     if (n == "scatter_agg") g_opt.scatter_agg = (int)(value != 0);
     else if (n == "scatter_xcd") g_opt.scatter_xcd = (int)(value != 0);
+    else if (n == "scatter_lds") g_opt.scatter_lds = (int)(value != 0);
     else if (n == "xcd_budget_mb") g_opt.xcd_budget_mb = value;
     else return fail(SO3G_ERR_VALUE, "unknown option " + n);
     return SO3G_OK;
@@ -608,6 +684,7 @@ int so3g_get_option(const char *name, int64_t *value) {
     std::string n(name ? name : "");
     if (n == "scatter_agg") *value = g_opt.scatter_agg;
     else if (n == "scatter_xcd") *value = g_opt.scatter_xcd;
+    else if (n == "scatter_lds") *value = g_opt.scatter_lds;
     else if (n == "xcd_budget_mb") *value = g_opt.xcd_budget_mb;
     else return fail(SO3G_ERR_VALUE, "unknown option " + n);
     return SO3G_OK;

@@ -86,6 +86,47 @@ __global__ __launch_bounds__(kBlock) void k_pc_from_map(PcIn in, PcMap pm, const
     });
 }
 
+// Vectorised precomp from_map for k = 2 maps and n_t % 4 == 0: four
+// consecutive samples per lane with 16-byte loads of the pixel indices, spin
+// factors and signal.  Same arithmetic as k_pc_from_map.
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_pc_from_map_v4(PcIn in, PcMap pm, const double *map,
+                                                           const int32_t *pix,
+                                                           const float *spin, float *sig,
+                                                           int accumulate) {
+    const int64_t t = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 4;
+    if (t >= in.n_t) return;
+    const int64_t plane = pm.ny * pm.nx;
+    const int64_t d0 = (int64_t)blockIdx.y * in.dpb;
+    const int64_t d1 = min(d0 + in.dpb, in.n_det);
+    for (int64_t d = d0; d < d1; d++) {
+        const int64_t i = d * in.n_t + t;
+        const int4 pa = *reinterpret_cast<const int4 *>(pix + 2 * i);
+        const int4 pb = *reinterpret_cast<const int4 *>(pix + 2 * i + 4);
+        float sp[4 * NC];
+#pragma unroll
+        for (int q = 0; q < NC; q++) {
+            const float4 f = reinterpret_cast<const float4 *>(spin + i * NC)[q];
+            sp[4 * q] = f.x; sp[4 * q + 1] = f.y; sp[4 * q + 2] = f.z; sp[4 * q + 3] = f.w;
+        }
+        const int py[4] = {pa.x, pa.z, pb.x, pb.z};
+        const int px[4] = {pa.y, pa.w, pb.y, pb.w};
+        float4 out = accumulate ? *reinterpret_cast<const float4 *>(sig + i)
+                                : float4{0.f, 0.f, 0.f, 0.f};
+        float o[4] = {out.x, out.y, out.z, out.w};
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (py[r] < 0) continue;
+            const double *e = map + ((int64_t)py[r] * pm.nx + px[r]);
+            float sr = 0;
+#pragma unroll
+            for (int c = 0; c < NC; c++) sr = sr + sp[r * NC + c] * e[c * plane];
+            o[r] = accumulate ? o[r] + sr : sr;
+        }
+        *reinterpret_cast<float4 *>(sig + i) = float4{o[0], o[1], o[2], o[3]};
+    }
+}
+
 // Precomp to_map, Projection.cxx:2100-2121: (det_wt * sig) * pf in float32,
 // added to the float64 map.  AGG sums runs of equal pixels within a wave
 // before the atomics (wave_runs); XCD sends each XCD's atomics to its own
@@ -152,6 +193,143 @@ __global__ __launch_bounds__(kBlock) void k_pc_to_map(PcIn in, PcMap pm, double 
             for (int c = 0; c < NC; c++) unsafeAtomicAdd(e + c * plane, v[c]);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-privatised cached to_map for non-tiled flat maps (k = 2).
+//
+// A block owns kLdsDets detectors (taken in det_order, which groups
+// detectors that sweep the same map rows) x kLdsT consecutive samples.  It
+// estimates the block's pixel bounding box from every kLdsSub-th sample,
+// keeps a float64 partial map of that box in LDS, adds the run-aggregated
+// sums of its samples there with ds_add_f64, and finally adds the non-zero
+// box entries to the global map with coalesced FP64 atomics.  Samples that
+// fall outside the estimated box, and whole blocks whose box exceeds the LDS
+// capacity, use the direct run-aggregated global atomics instead, so the
+// result never depends on the estimate.
+// ---------------------------------------------------------------------------
+constexpr int kLdsDets = 16;
+constexpr int kLdsT = 1024;
+constexpr int kLdsSub = 8;
+constexpr int kLdsBytes = 64 * 1024;
+
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_pc_to_map_lds(PcIn in, PcMap pm, double *map,
+                                                          const int32_t *pix,
+                                                          const float *spin,
+                                                          const float *sig,
+                                                          const float *det_weights,
+                                                          const int32_t *det_order) {
+    constexpr int CAP = kLdsBytes / (8 * NC);     // pixels per component
+    __shared__ double part[NC * CAP];
+    __shared__ int red[4][kBlock / 64];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int64_t t0 = (int64_t)blockIdx.x * kLdsT;
+    const int64_t g0 = (int64_t)blockIdx.y * kLdsDets;
+    const int nd = (int)min((int64_t)kLdsDets, in.n_det - g0);
+    const int nt = (int)min((int64_t)kLdsT, in.n_t - t0);
+
+    // 1. Bounding box of a subsample of the block's on-map samples.
+    int ymin = INT32_MAX, xmin = INT32_MAX, ymax = -1, xmax = -1;
+    const int nsub = nd * ((nt + kLdsSub - 1) / kLdsSub);
+    const int per_det = (nt + kLdsSub - 1) / kLdsSub;
+    for (int k = tid; k < nsub; k += kBlock) {
+        const int64_t d = det_order[g0 + k / per_det];
+        const int64_t t = t0 + (int64_t)(k % per_det) * kLdsSub;
+        const int2 pi = *reinterpret_cast<const int2 *>(pix + 2 * (d * in.n_t + t));
+        if (pi.x >= 0) {
+            ymin = min(ymin, pi.x); ymax = max(ymax, pi.x);
+            xmin = min(xmin, pi.y); xmax = max(xmax, pi.y);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ymin = min(ymin, __shfl_xor(ymin, off)); ymax = max(ymax, __shfl_xor(ymax, off));
+        xmin = min(xmin, __shfl_xor(xmin, off)); xmax = max(xmax, __shfl_xor(xmax, off));
+    }
+    if (lane == 0) { red[0][wid] = ymin; red[1][wid] = ymax; red[2][wid] = xmin; red[3][wid] = xmax; }
+    __syncthreads();
+    ymin = red[0][0]; ymax = red[1][0]; xmin = red[2][0]; xmax = red[3][0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; w++) {
+        ymin = min(ymin, red[0][w]); ymax = max(ymax, red[1][w]);
+        xmin = min(xmin, red[2][w]); xmax = max(xmax, red[3][w]);
+    }
+    // One pixel of margin for the samples between the subsampled ones.
+    ymin = max(ymin - 1, 0); xmin = max(xmin - 1, 0);
+    ymax = min(ymax + 1, (int)pm.ny - 1); xmax = min(xmax + 1, (int)pm.nx - 1);
+    const int bh = ymax - ymin + 1, bw = xmax - xmin + 1;
+    const bool use_lds = ymax >= ymin && xmax >= xmin && (int64_t)bh * bw <= CAP;
+    const int area = use_lds ? bh * bw : 0;
+    for (int k = tid; k < NC * area; k += kBlock) part[k] = 0.;
+    __syncthreads();
+
+    // 2. Accumulate.
+    const int64_t plane = pm.ny * pm.nx;
+    for (int j = 0; j < nd; j++) {
+        const int64_t d = det_order[g0 + j];
+        float det_wt = 1.f;
+        if (det_weights != nullptr) {
+            det_wt = det_weights[d];
+            if (det_wt == 0.f) continue;
+        }
+        for (int s0 = 0; s0 < nt; s0 += kBlock) {
+            const int64_t t = t0 + s0 + tid;
+            int64_t key = -1;
+            double v[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c++) v[c] = 0.;
+            if (s0 + tid < nt) {
+                const int64_t i = d * in.n_t + t;
+                const int2 pi = *reinterpret_cast<const int2 *>(pix + 2 * i);
+                if (pi.x >= 0) {
+                    key = (int64_t)pi.x * pm.nx + pi.y;
+                    const float sg = sig[i];
+#pragma unroll
+                    for (int c = 0; c < NC; c++)
+                        v[c] = (double)(det_wt * sg * spin[i * NC + c]);
+                }
+            }
+            if (wave_runs<NC>(key, v) && key >= 0) {
+                const int y = (int)(key / pm.nx), x = (int)(key - (int64_t)y * pm.nx);
+                if (use_lds && y >= ymin && y <= ymax && x >= xmin && x <= xmax) {
+                    const int o = (y - ymin) * bw + (x - xmin);
+#pragma unroll
+                    for (int c = 0; c < NC; c++)
+                        __hip_atomic_fetch_add(part + c * area + o, v[c], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < NC; c++) unsafeAtomicAdd(map + c * plane + key, v[c]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // 3. Flush the box.
+    for (int k = tid; k < NC * area; k += kBlock) {
+        const double v = part[k];
+        if (v != 0.) {
+            const int c = k / area, o = k - c * area;
+            const int y = ymin + o / bw, x = xmin + o % bw;
+            unsafeAtomicAdd(map + c * plane + (int64_t)y * pm.nx + x, v);
+        }
+    }
+}
+
+// det_key[d] = row-major pixel of detector d at the middle sample (or a
+// large value when off the map): sorting detectors by it groups those that
+// sweep the same rows.
+template <typename T>
+__global__ void k_det_keys(const T *pix, int64_t n_det, int64_t n_t, int64_t nx,
+                           int64_t *keys) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n_det) return;
+    const int64_t t = n_t / 2;
+    const int2 pi = *reinterpret_cast<const int2 *>(pix + 2 * (d * n_t + t));
+    keys[d] = pi.x >= 0 ? (int64_t)pi.x * nx + pi.y : INT64_MAX;
 }
 
 // map[i] += sum over the n_copy XCD-private partial maps, fixed order.

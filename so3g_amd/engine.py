@@ -709,3 +709,67 @@ def make_engine_classes():
 
 
 ENGINE_NAMES = sorted(make_engine_classes())
+
+
+class PrecompPlan:
+    """Pixel-sorted plan for repeated cached-pointing projection with one
+    pointing matrix (CG-style loops).  No counterpart in the reference: built
+    once from a ProjEng_Precomp pixel_index (k = 2, non-tiled flat map of
+    `map_shape` = (ny, nx)) and spin_proj, it sorts the on-map samples by
+    pixel so that to_map sums each pixel's contributions with a wave-level
+    segmented reduction instead of per-sample atomics.  to_map adds what
+    ProjEng_Precomp_NonTiled.to_map adds over every sample (float64 sums of
+    the same float32 products, in a different order); from_map is
+    ProjEng_Precomp_NonTiled.from_map on the stored pointing matrix."""
+
+    def __init__(self, pixel_index, spin_proj, map_shape, t_block_mb=64):
+        import ctypes
+        call = _Call()
+        self._pix = call.array(pixel_index, 'int32', 'pixel_index', spec=[-1, -1, 2])
+        n_det, n_t, _ = self._pix.shape
+        self._spin = call.array(spin_proj, 'float32', 'spin_proj', spec=[n_det, n_t, -1])
+        self.n_det, self.n_t = int(n_det), int(n_t)
+        self.nc = int(self._spin.shape[2])
+        self.map_shape = (int(map_shape[0]), int(map_shape[1]))
+        self._shape = np.array(self.map_shape, np.int64)
+        h = ctypes.c_void_p()
+        N.check(N.load().so3g_pc_plan_create(self._pix.data_ptr(), 2, self._spin.data_ptr(),
+                                             self.nc, self.n_det, self.n_t,
+                                             self._shape.ctypes.data, int(t_block_mb),
+                                             ctypes.byref(h), _stream()))
+        self._h = h
+
+    def info(self):
+        import ctypes
+        n_on, nbytes = ctypes.c_int64(), ctypes.c_int64()
+        N.check(N.load().so3g_pc_plan_info(self._h, ctypes.byref(n_on), ctypes.byref(nbytes)))
+        return {'n_on': n_on.value, 'bytes': nbytes.value}
+
+    def to_map(self, map, signal, det_weights=None):
+        call = _Call()
+        m = call.array(map, 'float64', 'map', spec=[self.nc] + list(self.map_shape), out=True)
+        sig = call.array(signal, 'float32', 'signal', spec=[self.n_det, self.n_t])
+        dw = call.array(det_weights, 'float32', 'det_weights', spec=[self.n_det],
+                        optional=True)
+        N.check(N.load().so3g_pc_plan_to_map(self._h, m.data_ptr(), sig.data_ptr(), _ptr(dw),
+                                             _stream()))
+        call.finish()
+        return map
+
+    def from_map(self, map, signal=None):
+        pe = ProjEngPrecompBase()
+        out = pe.from_map(map, self._pix, self._spin, signal)
+        if _is_torch(out) and not _is_torch(map) and not _is_torch(signal):
+            if signal is not None and not isinstance(signal, (list, tuple)):
+                return list(signal)
+            return list(out.cpu().numpy())
+        return out
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            try:
+                N.load().so3g_pc_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None

@@ -180,10 +180,10 @@ def test_thread_intervals(gpu, proj, tiled):
     maps_close(g, r, 'to_weights threads')
 
 
-@pytest.mark.parametrize('tiled', [False, True])
-def test_precomp(gpu, tiled):
+@pytest.mark.parametrize('tiled,n_samp', [(False, 3000), (False, 3001), (True, 3000)])
+def test_precomp(gpu, tiled, n_samp):
     import so3g_amd
-    c = Case('CAR', comps='TQU', tiled=tiled, clip=True)
+    c = Case('CAR', comps='TQU', tiled=tiled, clip=True, n_samp=n_samp)
     pix, spin = c.engine.pointing_matrix(c.bore, c.dets, c.resps)
     pix, spin = np.array(pix), np.array(spin)
     pe = so3g_amd.ProjEng_Precomp_Tiled() if tiled else so3g_amd.ProjEng_Precomp_NonTiled()
@@ -310,15 +310,16 @@ def test_projectionist_torch_path(gpu):
     assert np.abs(s_t.cpu().numpy() - s_np).max() < 1e-3
 
 
-@pytest.mark.parametrize('agg,xcd', [(0, 0), (1, 1), (0, 1)])
-def test_scatter_variants(gpu, agg, xcd):
+@pytest.mark.parametrize('agg,xcd,lds', [(0, 0, 0), (1, 1, 0), (0, 1, 0), (1, 0, 0), (1, 0, 1)])
+def test_scatter_variants(gpu, agg, xcd, lds):
     """Every scatter variant (wave-level run aggregation on/off, XCD-private
     partial maps on/off) gives the oracle's maps."""
     import so3g_amd
     from so3g_amd import _native
-    old = (_native.get_option('scatter_agg'), _native.get_option('scatter_xcd'))
-    _native.set_option('scatter_agg', agg)
-    _native.set_option('scatter_xcd', xcd)
+    names = ('scatter_agg', 'scatter_xcd', 'scatter_lds')
+    old = [_native.get_option(n) for n in names]
+    for n, v in zip(names, (agg, xcd, lds)):
+        _native.set_option(n, v)
     try:
         for proj, tiled, interp in [('CAR', False, 0), ('CAR', True, 1), ('ZEA', False, 1)]:
             c = Case(proj, comps='TQU', tiled=tiled, interp=interp)
@@ -338,6 +339,35 @@ def test_scatter_variants(gpu, agg, xcd):
         g = pe.to_weight_map(c.zeros([3, 3]), pix, spin)
         r = oracle.precomp_to_weights(c.og, c.zeros([3, 3]), pix, spin)
         maps_close(g, r, 'precomp to_weights')
+        # A larger case with many blocks and row-grouped detectors.
+        c = Case('CAR', comps='TQU', n_det=40, n_samp=9000, ny=120, nx=400)
+        pix, spin = [np.array(x) for x in c.engine.pointing_matrix(c.bore, c.dets, c.resps)]
+        for comps, sl in (('TQU', slice(None)), ('T', slice(0, 1))):
+            g = pe.to_map(np.zeros((len(comps), 120, 400)), pix, spin[..., sl], c.signal,
+                          c.det_weights)
+            og = oracle.geom_from_args('CAR', comps, c.pixargs)
+            r = oracle.precomp_to_map(og, np.zeros((len(comps), 120, 400)), pix,
+                                      np.ascontiguousarray(spin[..., sl]), c.signal,
+                                      c.det_weights)
+            maps_close(g, r, f'precomp to_map large {comps}', atol=1e-9)
     finally:
-        _native.set_option('scatter_agg', old[0])
-        _native.set_option('scatter_xcd', old[1])
+        for n, v in zip(names, old):
+            _native.set_option(n, v)
+
+
+def test_precomp_plan(gpu):
+    """PrecompPlan.to_map == Precomp to_map over all samples (oracle)."""
+    import so3g_amd
+    c = Case('CAR', comps='TQU', n_det=30, n_samp=7000, ny=100, nx=300, clip=True)
+    pix, spin = [np.array(x) for x in c.engine.pointing_matrix(c.bore, c.dets, c.resps)]
+    plan = so3g_amd.PrecompPlan(pix, spin, (100, 300))
+    assert plan.info()['n_on'] == int((pix[..., 0] >= 0).sum())
+    dw = c.det_weights.copy()
+    dw[3] = 0.
+    for w in (None, dw):
+        g = plan.to_map(np.zeros((3, 100, 300)), c.signal, w)
+        r = oracle.precomp_to_map(c.og, np.zeros((3, 100, 300)), pix, spin, c.signal, w)
+        maps_close(g, r, 'plan to_map', atol=1e-9)
+    m = c.map_like([3])
+    assert_bit_equal(np.array(plan.from_map(m)), oracle.precomp_from_map(c.og, m, pix, spin),
+                     'plan from_map')
