@@ -61,6 +61,8 @@ struct State {
     int64_t order_cap = 0;
     const void *order_pix = nullptr;
     int64_t order_ndet = -1, order_nt = -1;
+    double *packed_d = nullptr;   // pixel-major map copy for gathers
+    size_t packed_cap = 0;
 };
 
 // Tunables (so3g_set_option): wave-level run aggregation of scatter-adds,
@@ -70,6 +72,8 @@ struct Options {
     int scatter_xcd = 0;
     int scatter_lds = 0;
     int64_t xcd_budget_mb = 4096;
+    int pc_from_variant = 2;   // 0: 4 samples/lane, 1: + 2 detectors per iteration, 2: + pixel-major map copy
+    int pc_from_dpb = 2;       // detectors per block of the vectorised from_map
 };
 Options g_opt;
 constexpr int kNumXcd = 8;
@@ -553,7 +557,50 @@ int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
         r = zero_flags(s);
         if (r) return r;
     }
+    if (k == 2 && n_t % 4 == 0 && g_opt.pc_from_variant == 2) {
+        in.dpb = std::max<int32_t>(in.dpb, (int32_t)g_opt.pc_from_dpb);
+        const int64_t plane = pm.ny * pm.nx;
+        if ((size_t)(plane * 4) > g_state.packed_cap) {
+            if (g_state.packed_d) HIP_TRY(hipFree(g_state.packed_d));
+            g_state.packed_d = nullptr;
+            g_state.packed_cap = 0;
+            HIP_TRY(hipMalloc(&g_state.packed_d, plane * 4 * sizeof(double)));
+            g_state.packed_cap = plane * 4;
+        }
+        double *pk = g_state.packed_d;
+        const dim3 gp(1024), g4((unsigned)((n_t / 4 + kBlock - 1) / kBlock),
+                                (unsigned)((n_det + in.dpb - 1) / in.dpb));
+        switch (nc) {
+        case 1:
+            hipLaunchKernelGGL((k_pack_map<1>), gp, dim3(kBlock), 0, s, map, plane, pk);
+            hipLaunchKernelGGL((k_pc_from_map_packed<1>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_pack_map<2>), gp, dim3(kBlock), 0, s, map, plane, pk);
+            hipLaunchKernelGGL((k_pc_from_map_packed<2>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate);
+            break;
+        default:
+            hipLaunchKernelGGL((k_pack_map<3>), gp, dim3(kBlock), 0, s, map, plane, pk);
+            hipLaunchKernelGGL((k_pc_from_map_packed<3>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate);
+            break;
+        }
+        HIP_TRY(hipGetLastError());
+        return SO3G_OK;
+    }
+    if (k == 2 && n_t % 4 == 0 && g_opt.pc_from_variant == 1) {
+        in.dpb = std::max<int32_t>(in.dpb, (int32_t)g_opt.pc_from_dpb);
+        const dim3 g4((unsigned)((n_t / 4 + kBlock - 1) / kBlock),
+                      (unsigned)((n_det + in.dpb - 1) / in.dpb));
+        switch (nc) {
+        case 1: hipLaunchKernelGGL((k_pc_from_map_v4x2<1>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
+        case 2: hipLaunchKernelGGL((k_pc_from_map_v4x2<2>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
+        default: hipLaunchKernelGGL((k_pc_from_map_v4x2<3>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
+        }
+        HIP_TRY(hipGetLastError());
+        return SO3G_OK;
+    }
     if (k == 2 && n_t % 4 == 0) {
+        in.dpb = std::max<int32_t>(in.dpb, (int32_t)g_opt.pc_from_dpb);
         const dim3 g4((unsigned)((n_t / 4 + kBlock - 1) / kBlock),
                       (unsigned)((n_det + in.dpb - 1) / in.dpb));
         switch (nc) {
@@ -675,6 +722,8 @@ int so3g_set_option(const char *name, int64_t value) {
     if (n == "scatter_agg") g_opt.scatter_agg = (int)(value != 0);
     else if (n == "scatter_xcd") g_opt.scatter_xcd = (int)(value != 0);
     else if (n == "scatter_lds") g_opt.scatter_lds = (int)(value != 0);
+    else if (n == "pc_from_variant") g_opt.pc_from_variant = (int)value;
+    else if (n == "pc_from_dpb") g_opt.pc_from_dpb = (int)std::max<int64_t>(1, value);
     else if (n == "xcd_budget_mb") g_opt.xcd_budget_mb = value;
     else return fail(SO3G_ERR_VALUE, "unknown option " + n);
     return SO3G_OK;
@@ -685,6 +734,8 @@ int so3g_get_option(const char *name, int64_t *value) {
     if (n == "scatter_agg") *value = g_opt.scatter_agg;
     else if (n == "scatter_xcd") *value = g_opt.scatter_xcd;
     else if (n == "scatter_lds") *value = g_opt.scatter_lds;
+    else if (n == "pc_from_variant") *value = g_opt.pc_from_variant;
+    else if (n == "pc_from_dpb") *value = g_opt.pc_from_dpb;
     else if (n == "xcd_budget_mb") *value = g_opt.xcd_budget_mb;
     else return fail(SO3G_ERR_VALUE, "unknown option " + n);
     return SO3G_OK;

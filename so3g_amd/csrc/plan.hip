@@ -40,7 +40,7 @@ struct so3g_pc_plan {
     int64_t bytes;
 };
 
-namespace {
+namespace so3g {
 
 #define PLAN_TRY(expr)                                                               \
     do {                                                                             \
@@ -57,22 +57,25 @@ constexpr int kPlanBlock = 256;
 __global__ __launch_bounds__(kPlanBlock) void k_plan_keys(const int32_t *pix, int64_t n,
                                                           int64_t nx, int32_t npix, int64_t n_t,
                                                           int64_t t_block, int32_t off_key,
-                                                          int32_t *keys, uint32_t *vals,
-                                                          unsigned long long *n_on) {
-    __shared__ unsigned int cnt;
-    if (threadIdx.x == 0) cnt = 0;
-    __syncthreads();
+                                                          int32_t *keys, uint32_t *vals) {
     const int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
-    if (i < n) {
-        const int2 p = *reinterpret_cast<const int2 *>(pix + 2 * i);
-        const bool on = p.x >= 0;
-        const int64_t tb = (i % n_t) / t_block;
-        keys[i] = on ? (int32_t)(tb * npix + p.x * nx + p.y) : off_key;
-        vals[i] = (uint32_t)i;
-        if (on) atomicAdd(&cnt, 1u);
+    if (i >= n) return;
+    const int2 p = *reinterpret_cast<const int2 *>(pix + 2 * i);
+    const int64_t tb = (i % n_t) / t_block;
+    keys[i] = p.x >= 0 ? (int32_t)(tb * npix + p.x * nx + p.y) : off_key;
+    vals[i] = (uint32_t)i;
+}
+
+// Number of sorted keys below off_key (the on-map samples): binary search.
+__global__ void k_plan_count(const int32_t *keys, int64_t n, int32_t off_key,
+                             unsigned long long *n_on) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (keys[mid] < off_key) lo = mid + 1;
+        else hi = mid;
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && cnt) atomicAdd(n_on, (unsigned long long)cnt);
+    *n_on = (unsigned long long)lo;
 }
 
 template <int NC>
@@ -221,7 +224,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_to_map_r(const int32_t *key
 
 constexpr int kPlanR = 8;
 
-}  // namespace
+}  // namespace so3g
 
 extern "C" {
 
@@ -267,16 +270,17 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         (void)hipGetLastError();
         return set_error(SO3G_ERR_HIP, "out of device memory building the plan");
     }
-    PLAN_TRY(hipMemsetAsync(d_on, 0, sizeof(unsigned long long), s));
     hipLaunchKernelGGL(k_plan_keys, dim3((unsigned)((n + kPlanBlock - 1) / kPlanBlock)),
                        dim3(kPlanBlock), 0, s, pixel, n, nx, npix, n_t, t_block, off_key,
-                       keys_in, vals_in, d_on);
+                       keys_in, vals_in);
     PLAN_TRY(hipGetLastError());
     PLAN_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, vals_in, vals_out,
                                        (size_t)n, 0, end_bit, s));
     PLAN_TRY(hipMalloc(&tmp, tmp_bytes));
     PLAN_TRY(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, vals_out,
                                        (size_t)n, 0, end_bit, s));
+    hipLaunchKernelGGL(k_plan_count, dim3(1), dim3(1), 0, s, keys_out, n, off_key, d_on);
+    PLAN_TRY(hipGetLastError());
     unsigned long long n_on = 0;
     PLAN_TRY(hipMemcpyAsync(&n_on, d_on, sizeof(n_on), hipMemcpyDeviceToHost, s));
     PLAN_TRY(hipStreamSynchronize(s));

@@ -127,6 +127,149 @@ __global__ __launch_bounds__(kBlock) void k_pc_from_map_v4(PcIn in, PcMap pm, co
     }
 }
 
+// As k_pc_from_map_v4, two detectors per iteration with every independent
+// load issued before the map gathers (more bytes in flight per lane).
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_pc_from_map_v4x2(PcIn in, PcMap pm,
+                                                             const double *map,
+                                                             const int32_t *pix,
+                                                             const float *spin, float *sig,
+                                                             int accumulate) {
+    const int64_t t = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 4;
+    if (t >= in.n_t) return;
+    const int64_t plane = pm.ny * pm.nx;
+    const int64_t d0 = (int64_t)blockIdx.y * in.dpb;
+    const int64_t d1 = min(d0 + in.dpb, in.n_det);
+    for (int64_t d = d0; d < d1; d += 2) {
+        const bool two = d + 1 < d1;
+        int4 pa[2], pb[2];
+        float sp[2][4 * NC];
+        float o[2][4];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int64_t i = (d + (two ? u : 0)) * in.n_t + t;
+            pa[u] = *reinterpret_cast<const int4 *>(pix + 2 * i);
+            pb[u] = *reinterpret_cast<const int4 *>(pix + 2 * i + 4);
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                const float4 f = reinterpret_cast<const float4 *>(spin + i * NC)[q];
+                sp[u][4 * q] = f.x; sp[u][4 * q + 1] = f.y;
+                sp[u][4 * q + 2] = f.z; sp[u][4 * q + 3] = f.w;
+            }
+            const float4 a = accumulate ? *reinterpret_cast<const float4 *>(sig + i)
+                                        : float4{0.f, 0.f, 0.f, 0.f};
+            o[u][0] = a.x; o[u][1] = a.y; o[u][2] = a.z; o[u][3] = a.w;
+        }
+        double mv[2][4][NC];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int py[4] = {pa[u].x, pa[u].z, pb[u].x, pb[u].z};
+            const int px[4] = {pa[u].y, pa[u].w, pb[u].y, pb[u].w};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const double *e = map + (py[r] >= 0 ? ((int64_t)py[r] * pm.nx + px[r]) : 0);
+#pragma unroll
+                for (int c = 0; c < NC; c++) mv[u][r][c] = py[r] >= 0 ? e[c * plane] : 0.;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            if (u == 1 && !two) break;
+            const int py[4] = {pa[u].x, pa[u].z, pb[u].x, pb[u].z};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (py[r] < 0) continue;
+                float sr = 0;
+#pragma unroll
+                for (int c = 0; c < NC; c++) sr = sr + sp[u][r * NC + c] * mv[u][r][c];
+                o[u][r] = accumulate ? o[u][r] + sr : sr;
+            }
+            const int64_t i = (d + u) * in.n_t + t;
+            *reinterpret_cast<float4 *>(sig + i) = float4{o[u][0], o[u][1], o[u][2], o[u][3]};
+        }
+    }
+}
+
+// Pixel-major copy of a (NC, npix) map, padded to 4 components, so that the
+// from_map gather of one sample reads one 32-byte span instead of NC lines.
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_pack_map(const double *map, int64_t plane,
+                                                     double *packed) {
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < plane;
+         p += (int64_t)gridDim.x * kBlock) {
+        double v[4] = {0., 0., 0., 0.};
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] = map[c * plane + p];
+        reinterpret_cast<double2 *>(packed + 4 * p)[0] = double2{v[0], v[1]};
+        reinterpret_cast<double2 *>(packed + 4 * p)[1] = double2{v[2], v[3]};
+    }
+}
+
+// k_pc_from_map_v4x2 reading the packed map.
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_pc_from_map_packed(PcIn in, PcMap pm,
+                                                               const double *packed,
+                                                               const int32_t *pix,
+                                                               const float *spin, float *sig,
+                                                               int accumulate) {
+    const int64_t t = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 4;
+    if (t >= in.n_t) return;
+    const int64_t d0 = (int64_t)blockIdx.y * in.dpb;
+    const int64_t d1 = min(d0 + in.dpb, in.n_det);
+    for (int64_t d = d0; d < d1; d += 2) {
+        const bool two = d + 1 < d1;
+        int4 pa[2], pb[2];
+        float sp[2][4 * NC];
+        float o[2][4];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int64_t i = (d + (two ? u : 0)) * in.n_t + t;
+            pa[u] = *reinterpret_cast<const int4 *>(pix + 2 * i);
+            pb[u] = *reinterpret_cast<const int4 *>(pix + 2 * i + 4);
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                const float4 f = reinterpret_cast<const float4 *>(spin + i * NC)[q];
+                sp[u][4 * q] = f.x; sp[u][4 * q + 1] = f.y;
+                sp[u][4 * q + 2] = f.z; sp[u][4 * q + 3] = f.w;
+            }
+            const float4 a = accumulate ? *reinterpret_cast<const float4 *>(sig + i)
+                                        : float4{0.f, 0.f, 0.f, 0.f};
+            o[u][0] = a.x; o[u][1] = a.y; o[u][2] = a.z; o[u][3] = a.w;
+        }
+        double mv[2][4][4];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int py[4] = {pa[u].x, pa[u].z, pb[u].x, pb[u].z};
+            const int px[4] = {pa[u].y, pa[u].w, pb[u].y, pb[u].w};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t p = py[r] >= 0 ? ((int64_t)py[r] * pm.nx + px[r]) : 0;
+                const double2 lo = reinterpret_cast<const double2 *>(packed + 4 * p)[0];
+                mv[u][r][0] = lo.x; mv[u][r][1] = lo.y;
+                if (NC > 2) {
+                    const double2 hi = reinterpret_cast<const double2 *>(packed + 4 * p)[1];
+                    mv[u][r][2] = hi.x; mv[u][r][3] = hi.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            if (u == 1 && !two) break;
+            const int py[4] = {pa[u].x, pa[u].z, pb[u].x, pb[u].z};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (py[r] < 0) continue;
+                float sr = 0;
+#pragma unroll
+                for (int c = 0; c < NC; c++) sr = sr + sp[u][r * NC + c] * mv[u][r][c];
+                o[u][r] = accumulate ? o[u][r] + sr : sr;
+            }
+            const int64_t i = (d + u) * in.n_t + t;
+            *reinterpret_cast<float4 *>(sig + i) = float4{o[u][0], o[u][1], o[u][2], o[u][3]};
+        }
+    }
+}
+
 // Precomp to_map, Projection.cxx:2100-2121: (det_wt * sig) * pf in float32,
 // added to the float64 map.  AGG sums runs of equal pixels within a wave
 // before the atomics (wave_runs); XCD sends each XCD's atomics to its own
