@@ -1,7 +1,7 @@
 """Benchmark: det*samples/s of one to_map + from_map pass over one large
 observation (1000 detectors x 720000 samples = 7.2e8 det*samples; constant-
 elevation scan, el 50 deg, az +-10 deg at 1 deg/s, 200 Hz, detectors in a
-1.5 deg disk; CAR 0.5 arcmin TQU float64 map, 188 x 2689 pixels).
+1.5 deg disk; CAR 0.5 arcmin TQU float64 map, 191 x 2701 pixels).
 
 One "step" = from_map (map -> fresh float32 signal) + to_map (signal ->
 float64 TQU map, zeroed first) over the whole observation, with every input
@@ -11,6 +11,11 @@ the operator pair of every CG iteration); the on-the-fly path (pointing
 recomputed in both kernels, the Projectionist's default) is timed beside it.
 
 Prints ONE JSON line.  `python bench.py [--steps K] [--warmup W] [--gpus 1]`.
+
+`--pmc-pass` runs, instead of the timing, one plain copy of 1 GiB (the
+calibration for rocprofv3's FETCH_SIZE / WRITE_SIZE counters) and one launch
+of each headline kernel; the HBM traffic measured that way is read back from
+profiles/traffic_large_car_tqu.md into roofline.traffic.
 """
 import argparse
 import json
@@ -24,6 +29,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+COPY_BYTES = 1 << 30  # calibration copy of the --pmc-pass run
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'traffic_large_car_tqu.md')
+
+
+def measured_traffic():
+    """HBM bytes per launch of the headline kernels, from the JSON block of
+    the committed rocprofv3 --pmc summary (None when absent)."""
+    try:
+        txt = open(TRAFFIC_FILE).read()
+        block = txt.split('```json', 1)[1].split('```', 1)[0]
+        return json.loads(block)
+    except (OSError, IndexError, ValueError):
+        return None
 
 
 def log(*a):
@@ -39,6 +57,8 @@ def parse():
     ap.add_argument('--n-samp', type=int, default=720000)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-otf', action='store_true', help='skip the on-the-fly leg')
+    ap.add_argument('--pmc-pass', action='store_true',
+                    help='one calibration copy + one launch of each headline kernel')
     return ap.parse_args()
 
 
@@ -155,6 +175,19 @@ def main():
                   for i in range(len(ops))]
         return wall, per_op
 
+    if args.pmc_pass:
+        a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        pc_from()
+        torch.cuda.synchronize()
+        pc_to()
+        torch.cuda.synchronize()
+        print(json.dumps({'pmc_pass': True, 'copy_bytes': COPY_BYTES}), flush=True)
+        return
+
     log('timing cached-pointing path ...')
     wall, (ms_from, ms_to) = run([pc_from, pc_to], args.steps, args.warmup)
     ms_step = wall / args.steps * 1e3
@@ -167,10 +200,11 @@ def main():
     b_to = N_s * (4 * 2 + 4 * 3 + 4) + 2 * 8 * 3 * npix
     gbs_from = b_from / (ms_from * 1e-3)
     gbs_to = b_to / (ms_to * 1e-3)
+    tr = measured_traffic() or {}
     if ms_to >= ms_from:
-        dom = ('k_plan_to_map_r<3,8>', gbs_to, b_to)
+        dom = ('k_plan_to_map_r<3,8>', gbs_to, b_to, tr.get('to_map_bytes'))
     else:
-        dom = ('k_pc_from_map_v4<3>', gbs_from, b_from)
+        dom = ('k_pc_from_map_packed<3>', gbs_from, b_from, tr.get('from_map_bytes'))
     # The plain ProjEng_Precomp scatter (no plan), for reference.
     _, (ms_to_direct,) = run([pc_to_direct], max(1, min(args.steps, 3)), 1)
 
@@ -218,7 +252,7 @@ def main():
         'roofline': {
             'bound': 'hbm', 'kernel': dom[0], 'achieved': dom[1] / 1e9,
             'peak': HBM_PEAK / 1e9, 'unit': 'GB/s', 'frac': dom[1] / HBM_PEAK,
-            'traffic': None, 'algorithmic_bytes': dom[2],
+            'traffic': dom[3], 'algorithmic_bytes': dom[2],
             'from_map_frac': gbs_from / HBM_PEAK, 'to_map_frac': gbs_to / HBM_PEAK,
         },
         'on_the_fly': otf,

@@ -531,6 +531,49 @@ class ProjEngBase:
         call.finish()
         return map
 
+    def to_map_prec(self, map, pbore, pofs, response, signal, det_weights=None, prec=0):
+        """to_map in one of three arithmetics (no counterpart in the
+        reference, whose arithmetic is prec=0): 0 float32 products added to a
+        float64 map (so3g's), 1 float32 products and a float32 map, 2 float64
+        products and a float64 map.  `map` may be None."""
+        dt = 'float32' if prec == 1 else 'float64'
+        call = _Call()
+        bore, dets, n_t, n_det = self._pointing(call, pbore, pofs)
+        if map is None:
+            map = self._zeros_dtype((self.comp_count,), dt)
+        mp, tp, dp = self._map(call, map, [self.comp_count], dtype=dt)
+        sig, _, _ = _signal_space(call, signal, 'signal', 'float32', n_det, n_t)
+        dw = call.array(det_weights, 'float32', 'det_weights', spec=[n_det], optional=True)
+        resp = call.array(response, 'float32', 'response', spec=[n_det, 2])
+        N.check(N.load().so3g_to_map_prec(self._g, mp, tp, dp, bore.data_ptr(), n_t,
+                                          dets.data_ptr(), n_det, resp.data_ptr(),
+                                          sig.data_ptr(), _ptr(dw), int(prec), _stream()))
+        call.finish()
+        return map
+
+    def to_weight_map_prec(self, map, pbore, pofs, response, det_weights=None, prec=0):
+        """to_weight_map in the arithmetic selected by prec (see to_map_prec)."""
+        dt = 'float32' if prec == 1 else 'float64'
+        call = _Call()
+        bore, dets, n_t, n_det = self._pointing(call, pbore, pofs)
+        nc = self.comp_count
+        if map is None:
+            map = self._zeros_dtype((nc, nc), dt)
+        mp, tp, dp = self._map(call, map, [nc, nc], dtype=dt)
+        dw = call.array(det_weights, 'float32', 'det_weights', spec=[n_det], optional=True)
+        resp = call.array(response, 'float32', 'response', spec=[n_det, 2])
+        N.check(N.load().so3g_to_weights_prec(self._g, mp, tp, dp, bore.data_ptr(), n_t,
+                                              dets.data_ptr(), n_det, resp.data_ptr(), _ptr(dw),
+                                              int(prec), _stream()))
+        call.finish()
+        return map
+
+    def _zeros_dtype(self, lead, dtype):
+        z = self.zeros(lead)
+        if isinstance(z, list):
+            return [None if t is None else t.astype(dtype) for t in z]
+        return z.astype(dtype)
+
     def tile_hits(self, pbore, pofs):
         """Projection.cxx:1543-1601; returns a list of per-tile hit counts."""
         if not self._tiled:

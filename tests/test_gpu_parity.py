@@ -371,3 +371,21 @@ def test_precomp_plan(gpu):
     m = c.map_like([3])
     assert_bit_equal(np.array(plan.from_map(m)), oracle.precomp_from_map(c.og, m, pix, spin),
                      'plan from_map')
+
+
+def test_precision_modes(gpu):
+    """to_map / to_weights in the reference arithmetic agree with float64
+    products to 1e-6 relative (the precision target); the all-float32 variant
+    is reported, and is looser."""
+    from so3g_amd.studies import precision
+    r = precision.run(n_det=20, n_samp=8000, res_arcmin=1.0, nside=512)
+    for cname, c in r['cases'].items():
+        for kind in ('map', 'weights'):
+            for k, e in c['ref'].get(kind, {}).items():
+                assert e['max_rel'] < 1e-6, (cname, k, e)
+        assert c['hit_pixels'] > 100
+    # The f64 comparator itself against the oracle-consistent ref path.
+    c = Case('ZEA', comps='TQU')
+    g = c.engine.to_map_prec(None, c.bore, c.dets, c.resps, c.signal, None, 2)
+    r = oracle.to_map(c.og, c.zeros([3]), c.bore, c.dets, c.resps, c.signal)
+    maps_close(g, r, 'f64 vs ref', atol=1e-5)
