@@ -224,6 +224,48 @@ int so3g_pc_plan_to_map(const so3g_pc_plan *plan, double *map, const float *sign
 int so3g_pc_plan_info(const so3g_pc_plan *plan, int64_t *n_on, int64_t *bytes);
 int so3g_pc_plan_destroy(so3g_pc_plan *plan);
 
+/* Batched projection of many observations into one map (no counterpart in
+ * the reference, which projects one observation per call,
+ * python/proj/wcs.py:415-476 looped by the caller).  An observation list is
+ * turned into one table of equal work items (256 samples x dpb detectors);
+ * so3g_batch_to_map / so3g_batch_from_map then cover the whole batch in a
+ * single persistent launch.  mode SO3G_BATCH_EAGER launches directly,
+ * SO3G_BATCH_GRAPH captures the launch in a HIP graph (re-captured when the
+ * map pointers, shape or op change) and replays it, SO3G_BATCH_PER_OBS
+ * launches the same kernel once per observation (for comparison).  Results
+ * equal so3g_to_map / so3g_from_map called per observation (to_map up to
+ * the order of the float64 additions).  The observation arrays must stay
+ * allocated while the batch is used.  so3g_batch_tune times every launch
+ * shape (dpb in {1,2,4,8,16} x blocks per CU in {2,4,8,16, one per item})
+ * and keeps the fastest for the op (0 from_map, 1 to_map); results holds
+ * (dpb, blocks per CU, ms) triples.  Tuning to_map adds into `map`: pass a
+ * scratch map. */
+typedef struct so3g_batch so3g_batch;
+typedef struct {
+    const double *bore;        /* (n_t, 4) native-frame boresight */
+    int64_t n_t;
+    const double *dets;        /* (n_det, 4) */
+    int64_t n_det;
+    const float *resp;         /* (n_det, 2) */
+    const float *signal;       /* (n_det, n_t) to_map input, or NULL */
+    float *signal_out;         /* (n_det, n_t) from_map output, or NULL */
+    const float *det_weights;  /* (n_det) or NULL */
+} so3g_obs;
+enum { SO3G_BATCH_EAGER = 0, SO3G_BATCH_GRAPH = 1, SO3G_BATCH_PER_OBS = 2 };
+int so3g_batch_create(const so3g_geom *g, const so3g_obs *obs, int64_t n_obs,
+                      so3g_batch **out);
+int so3g_batch_to_map(so3g_batch *b, double *map, double *const *tiles,
+                      const int32_t *tile_dims, int mode, void *stream);
+int so3g_batch_from_map(so3g_batch *b, const double *map, const double *const *tiles,
+                        const int32_t *tile_dims, int accumulate, int mode, void *stream);
+int so3g_batch_set_shape(so3g_batch *b, int op, int dpb, int grid_mult);
+int so3g_batch_tune(so3g_batch *b, int op, double *map, double *const *tiles,
+                    const int32_t *tile_dims, int reps, double *results, int max_results,
+                    int *n_results, void *stream);
+/* info: n_obs, det*samples, from (dpb, blocks/CU), to (dpb, blocks/CU), CUs */
+int so3g_batch_info(const so3g_batch *b, int64_t *info);
+int so3g_batch_destroy(so3g_batch *b);
+
 /* Implementation tunables (no counterpart in the reference):
  *   "scatter_agg"   1: sum runs of equal pixels within a wave before the
  *                   map atomics (default 1)

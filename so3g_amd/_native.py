@@ -35,6 +35,16 @@ class Geom(ctypes.Structure):
     ]
 
 
+class Obs(ctypes.Structure):
+    """so3g_obs: one observation of a batch (device pointers)."""
+    _fields_ = [
+        ('bore', c_vp), ('n_t', c_i64), ('dets', c_vp), ('n_det', c_i64), ('resp', c_vp),
+        ('signal', c_vp), ('signal_out', c_vp), ('det_weights', c_vp),
+    ]
+
+
+BATCH_EAGER, BATCH_GRAPH, BATCH_PER_OBS = 0, 1, 2
+
 # name -> (restype, argtypes)
 _SIGS = {
     'so3g_init': (c_int, [c_int]),
@@ -64,6 +74,15 @@ _SIGS = {
     'so3g_pc_plan_to_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     'so3g_pc_plan_info': (c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
     'so3g_pc_plan_destroy': (c_int, [c_vp]),
+    'so3g_batch_create': (c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Obs), c_i64,
+                                  ctypes.POINTER(c_vp)]),
+    'so3g_batch_to_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_vp]),
+    'so3g_batch_from_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp]),
+    'so3g_batch_set_shape': (c_int, [c_vp, c_int, c_int, c_int]),
+    'so3g_batch_tune': (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_vp, c_int,
+                                ctypes.POINTER(c_int), c_vp]),
+    'so3g_batch_info': (c_int, [c_vp, c_vp]),
+    'so3g_batch_destroy': (c_int, [c_vp]),
     'so3g_set_option': (c_int, [ctypes.c_char_p, c_i64]),
     'so3g_get_option': (c_int, [ctypes.c_char_p, ctypes.POINTER(c_i64)]),
     'so3g_precomp_from_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_i64,

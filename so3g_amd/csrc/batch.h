@@ -1,0 +1,84 @@
+// batch.h -- one launch over many observations projected into one map.
+//
+// A batch of observations (each its own boresight, focal plane, responses
+// and signal) is cut into equal work items of kBlock time samples x dpb
+// detectors.  Items are numbered observation by observation (obs.item0 is
+// the first item of each, time tiles fastest) and a persistent grid walks
+// them with a block stride, so that a single launch -- one node of a
+// captured graph -- covers observations of any mix of sizes without a
+// host round trip between them.  Each item runs the same per-sample body
+// as the single-observation kernels (from_map_sample, to_map_sample), so
+// results match one launch per observation.
+#pragma once
+
+#include "kernels.h"
+
+namespace so3g {
+
+struct ObsDev {
+    const double *bore;        // (n_t, 4) native-frame boresight
+    const double *dets;        // (n_det, 4)
+    const float *resp;         // (n_det, 2)
+    const float *sig_in;       // to_map input (n_det, n_t)
+    float *sig_out;            // from_map output (n_det, n_t)
+    const float *det_w;        // (n_det) or nullptr
+    int64_t n_t, n_det;
+    int64_t item0;             // first work item of this observation
+    int64_t n_tt;              // time tiles, ceil(n_t / kBlock)
+};
+
+// Calls f(obs, in, d, t, qbore, valid) for every detector of the block's
+// current item; all lanes take part (valid == false past the end of the
+// observation) so that the wave-level aggregation sees 64 lanes.
+template <class F>
+__device__ __forceinline__ void for_batch_items(const ObsDev *obs, int n_obs, int64_t n_items,
+                                                int dpb, const Tabs &tabs, int agg, F &&f) {
+    for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        int lo = 0, hi = n_obs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (obs[mid].item0 <= it) lo = mid;
+            else hi = mid - 1;
+        }
+        const ObsDev &o = obs[lo];
+        const int64_t local = it - o.item0;
+        const int64_t dg = local / o.n_tt;
+        const int64_t t = (local - dg * o.n_tt) * kBlock + threadIdx.x;
+        const bool valid = t < o.n_t;
+        const Q4 qb = valid ? load_q(o.bore + 4 * t) : Q4{1., 0., 0., 0.};
+        const PtgIn in{o.bore, o.dets, o.n_t, o.n_det, tabs, dpb, agg, nullptr};
+        const int64_t d0 = dg * dpb;
+        const int64_t d1 = min(d0 + (int64_t)dpb, o.n_det);
+        for (int64_t d = d0; d < d1; d++) f(o, in, d, t, qb, valid);
+    }
+}
+
+template <int PROJ, int PK, int INTERP, int NC>
+__global__ __launch_bounds__(kBlock) void k_batch_to_map(const ObsDev *obs, int n_obs,
+                                                         int64_t n_items, int dpb, Tabs tabs,
+                                                         int agg, Geo g, MapView<double> m,
+                                                         int *flags) {
+    for_batch_items(obs, n_obs, n_items, dpb, tabs, agg,
+                    [&](const ObsDev &o, const PtgIn &in, int64_t d, int64_t t, const Q4 &qb,
+                        bool valid) {
+                        to_map_sample<PROJ, PK, INTERP, NC, 0>(in, g, m, o.resp, o.sig_in,
+                                                               o.det_w, flags, d, t, qb,
+                                                               valid);
+                    });
+}
+
+template <int PROJ, int PK, int INTERP, int NC>
+__global__ __launch_bounds__(kBlock) void k_batch_from_map(const ObsDev *obs, int n_obs,
+                                                           int64_t n_items, int dpb, Tabs tabs,
+                                                           Geo g, MapView<const double> m,
+                                                           int accumulate, int *flags) {
+    for_batch_items(obs, n_obs, n_items, dpb, tabs, 0,
+                    [&](const ObsDev &o, const PtgIn &in, int64_t d, int64_t t, const Q4 &qb,
+                        bool valid) {
+                        if (valid)
+                            from_map_sample<PROJ, PK, INTERP, NC>(in, g, m, o.resp, o.sig_out,
+                                                                  accumulate, flags, d, t, qb);
+                    });
+}
+
+}  // namespace so3g

@@ -226,6 +226,14 @@ int base_args(const so3g_geom *g, const double *bore, int64_t n_t, const double 
 
 namespace so3g {
 int set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+// Entry points for the other translation units of the library (batch.hip).
+int op_args(const so3g_geom *g, void *stream, OpArgs &a) {
+    return base_args(g, nullptr, 1, nullptr, 1, stream, a);
+}
+int launch(int proj, const OpArgs &a) { return run_dispatch(proj, a); }
+int flags_zero(hipStream_t stream) { return zero_flags(stream); }
+int flags_check_tile(hipStream_t stream) { return check_tile_flag(stream); }
 }  // namespace so3g
 
 extern "C" {

@@ -183,29 +183,38 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_stale(int32_t *pix, int64_t
 // component (Projection.cxx:1761-1765); deterministic per sample.
 // --------------------------------------------------------------------------
 template <int PROJ, int PK, int INTERP, int NC>
+__device__ __forceinline__ void from_map_sample(const PtgIn &in, const Geo &g,
+                                                const MapView<const double> &m,
+                                                const float *resp, float *sig, int accumulate,
+                                                int *flags, int64_t d, int64_t t,
+                                                const Q4 &qb) {
+    const Q4 qd = load_q(in.dets + 4 * d);
+    double co[4];
+    get_coords<PROJ>(qb, qd, in.tabs, co);
+    float pf[3];
+    spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
+    Pt p[4];
+    const int np = get_pixels<PK, INTERP>(g, co, p);
+    float *sp = sig + d * in.n_t + t;
+    float s = accumulate ? *sp : 0.f;
+    for (int ip = 0; ip < np; ip++) {
+        const double *e = elem<PK>(g, m, p[ip], 0);
+        if (e == nullptr) { raise_flag(flags, FLAG_TILE); continue; }
+        const int64_t plane = (PK == PK_FLAT_TILED)
+            ? (int64_t)m.tile_dims[2 * p[ip].i0] * m.tile_dims[2 * p[ip].i0 + 1]
+            : (PK == PK_HP_TILED ? (int64_t)g.npt : g.npix);
+#pragma unroll
+        for (int c = 0; c < NC; c++) s = s + e[c * plane] * pf[c] * p[ip].w;
+    }
+    *sp = s;
+}
+
+template <int PROJ, int PK, int INTERP, int NC>
 __global__ __launch_bounds__(kBlock) void k_from_map(PtgIn in, Geo g, MapView<const double> m,
                                                      const float *resp, float *sig,
                                                      int accumulate, int *flags) {
     for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
-        const Q4 qd = load_q(in.dets + 4 * d);
-        double co[4];
-        get_coords<PROJ>(qb, qd, in.tabs, co);
-        float pf[3];
-        spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
-        Pt p[4];
-        const int np = get_pixels<PK, INTERP>(g, co, p);
-        float *sp = sig + d * in.n_t + t;
-        float s = accumulate ? *sp : 0.f;
-        for (int ip = 0; ip < np; ip++) {
-            const double *e = elem<PK>(g, m, p[ip], 0);
-            if (e == nullptr) { raise_flag(flags, FLAG_TILE); continue; }
-            const int64_t plane = (PK == PK_FLAT_TILED)
-                ? (int64_t)m.tile_dims[2 * p[ip].i0] * m.tile_dims[2 * p[ip].i0 + 1]
-                : (PK == PK_HP_TILED ? (int64_t)g.npt : g.npix);
-#pragma unroll
-            for (int c = 0; c < NC; c++) s = s + e[c * plane] * pf[c] * p[ip].w;
-        }
-        *sp = s;
+        from_map_sample<PROJ, PK, INTERP, NC>(in, g, m, resp, sig, accumulate, flags, d, t, qb);
     });
 }
 
@@ -270,59 +279,69 @@ __device__ __forceinline__ void scatter_points(bool agg, MT *const (&e)[NP],
 }
 
 template <int PROJ, int PK, int INTERP, int NC, int PREC>
+__device__ __forceinline__ void to_map_sample(const PtgIn &in, const Geo &g,
+                                              const MapView<typename PrecT<PREC>::map_t> &m,
+                                              const float *resp, const float *sig,
+                                              const float *det_weights, int *flags, int64_t d,
+                                              int64_t t, const Q4 &qb, bool valid) {
+    using MT = typename PrecT<PREC>::map_t;
+    using VT = typename std::conditional<PREC == 1, float, double>::type;
+    constexpr int NP = INTERP ? 4 : 1;
+    float det_wt = 1.f;
+    if (det_weights != nullptr) {
+        det_wt = det_weights[d];
+        if (det_wt == 0.f) return;
+    }
+    MT *e[NP];
+    int64_t key[NP], off[NP][NC];
+    VT v[NP][NC];
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+        key[j] = -1;
+        e[j] = nullptr;
+#pragma unroll
+        for (int c = 0; c < NC; c++) { v[j][c] = 0; off[j][c] = 0; }
+    }
+    if (valid) {
+        const Q4 qd = load_q(in.dets + 4 * d);
+        double co[4];
+        get_coords<PROJ>(qb, qd, in.tabs, co);
+        float pf[3];
+        spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
+        const float s = sig[d * in.n_t + t];
+        Pt p[4];
+        const int np = get_pixels<PK, INTERP>(g, co, p);
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            if (j >= np) break;
+            MT *ej = elem<PK>(g, m, p[j], 0);
+            if (ej == nullptr) { raise_flag(flags, FLAG_TILE); continue; }
+            const int64_t plane = (PK == PK_FLAT_TILED)
+                ? (int64_t)m.tile_dims[2 * p[j].i0] * m.tile_dims[2 * p[j].i0 + 1]
+                : (PK == PK_HP_TILED ? (int64_t)g.npt : g.npix);
+            e[j] = ej;
+            key[j] = (int64_t)(uintptr_t)ej;
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                off[j][c] = c * plane;
+                if constexpr (PREC == 2)
+                    v[j][c] = (double)s * (double)pf[c] * (double)p[j].w * (double)det_wt;
+                else
+                    v[j][c] = (VT)(s * pf[c] * p[j].w * det_wt);
+            }
+        }
+    }
+    scatter_points<NP, NC, MT, VT>(in.agg, e, key, off, v);
+}
+
+template <int PROJ, int PK, int INTERP, int NC, int PREC>
 __global__ __launch_bounds__(kBlock) void k_to_map(PtgIn in, Geo g,
                                                    MapView<typename PrecT<PREC>::map_t> m,
                                                    const float *resp, const float *sig,
                                                    const float *det_weights, int *flags) {
-    using MT = typename PrecT<PREC>::map_t;
-    using VT = typename std::conditional<PREC == 1, float, double>::type;
-    constexpr int NP = INTERP ? 4 : 1;
     for_samples_all(in, [&](int64_t d, int64_t t, const Q4 &qb, bool valid) {
-        float det_wt = 1.f;
-        if (det_weights != nullptr) {
-            det_wt = det_weights[d];
-            if (det_wt == 0.f) return;
-        }
-        MT *e[NP];
-        int64_t key[NP], off[NP][NC];
-        VT v[NP][NC];
-#pragma unroll
-        for (int j = 0; j < NP; j++) {
-            key[j] = -1;
-            e[j] = nullptr;
-#pragma unroll
-            for (int c = 0; c < NC; c++) { v[j][c] = 0; off[j][c] = 0; }
-        }
-        if (valid) {
-            const Q4 qd = load_q(in.dets + 4 * d);
-            double co[4];
-            get_coords<PROJ>(qb, qd, in.tabs, co);
-            float pf[3];
-            spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
-            const float s = sig[d * in.n_t + t];
-            Pt p[4];
-            const int np = get_pixels<PK, INTERP>(g, co, p);
-#pragma unroll
-            for (int j = 0; j < NP; j++) {
-                if (j >= np) break;
-                MT *ej = elem<PK>(g, m, p[j], 0);
-                if (ej == nullptr) { raise_flag(flags, FLAG_TILE); continue; }
-                const int64_t plane = (PK == PK_FLAT_TILED)
-                    ? (int64_t)m.tile_dims[2 * p[j].i0] * m.tile_dims[2 * p[j].i0 + 1]
-                    : (PK == PK_HP_TILED ? (int64_t)g.npt : g.npix);
-                e[j] = ej;
-                key[j] = (int64_t)(uintptr_t)ej;
-#pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    off[j][c] = c * plane;
-                    if constexpr (PREC == 2)
-                        v[j][c] = (double)s * (double)pf[c] * (double)p[j].w * (double)det_wt;
-                    else
-                        v[j][c] = (VT)(s * pf[c] * p[j].w * det_wt);
-                }
-            }
-        }
-        scatter_points<NP, NC, MT, VT>(in.agg, e, key, off, v);
+        to_map_sample<PROJ, PK, INTERP, NC, PREC>(in, g, m, resp, sig, det_weights, flags, d,
+                                                  t, qb, valid);
     });
 }
 

@@ -4,13 +4,14 @@
 // kernel variants of the eight coordinate systems compile in parallel.
 #pragma once
 
+#include "batch.h"
 #include "kernels.h"
 #include "precomp.h"
 
 namespace so3g {
 
 enum { OP_COORDS = 0, OP_PIXELS, OP_PMAT, OP_FROM_MAP, OP_TO_MAP, OP_TO_WEIGHTS,
-       OP_TILE_HITS, OP_DOMAINS };
+       OP_TILE_HITS, OP_DOMAINS, OP_BATCH_FROM_MAP, OP_BATCH_TO_MAP };
 
 struct OpArgs {
     int op;
@@ -35,6 +36,11 @@ struct OpArgs {
     const int32_t *tile_dom;
     int32_t *dom_out;
     int32_t *hits;
+    // batch ops
+    const ObsDev *obs;
+    int n_obs;
+    int64_t n_items;
+    int grid_blocks;
 };
 
 inline dim3 grid_for(const OpArgs &a) {
@@ -46,8 +52,24 @@ inline dim3 grid_for(const OpArgs &a) {
 
 template <int PROJ, int PK, int INTERP, int NC>
 int launch_map_ops(const OpArgs &a) {
-    const dim3 grid = grid_for(a);
     const dim3 block(kBlock);
+    if (a.op == OP_BATCH_TO_MAP || a.op == OP_BATCH_FROM_MAP) {
+        const dim3 bgrid((unsigned)a.grid_blocks);
+        if (a.op == OP_BATCH_TO_MAP) {
+            MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
+            hipLaunchKernelGGL((k_batch_to_map<PROJ, PK, INTERP, NC>), bgrid, block, 0,
+                               a.stream, a.obs, a.n_obs, a.n_items, a.in.dpb, a.in.tabs,
+                               a.in.agg, a.g, m, a.flags);
+        } else {
+            MapView<const double> m{(const double *)a.map, (const double *const *)a.tiles,
+                                    a.tile_dims};
+            hipLaunchKernelGGL((k_batch_from_map<PROJ, PK, INTERP, NC>), bgrid, block, 0,
+                               a.stream, a.obs, a.n_obs, a.n_items, a.in.dpb, a.in.tabs, a.g,
+                               m, a.accumulate, a.flags);
+        }
+        return 0;
+    }
+    const dim3 grid = grid_for(a);
     switch (a.op) {
     case OP_FROM_MAP: {
         MapView<const double> m{(const double *)a.map, (const double *const *)a.tiles,
