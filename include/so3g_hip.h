@@ -221,6 +221,17 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
                         int64_t t_block_mb, so3g_pc_plan **out, void *stream);
 int so3g_pc_plan_to_map(const so3g_pc_plan *plan, double *map, const float *signal,
                         const float *det_weights, void *stream);
+/* Float64-signal variants for map-making solvers (no counterpart in the
+ * reference, whose signals are float32): products and sums in float64, so
+ * that P^T N^-1 P stays symmetric to float64 rounding when N^-1 has a large
+ * dynamic range (1/f noise).  so3g_precomp_from_map_f64 takes a non-tiled
+ * (ny, nx) map and a k = 2 pixel array; samples off the map add 0. */
+int so3g_pc_plan_to_map_f64(const so3g_pc_plan *plan, double *map, const double *signal,
+                            const float *det_weights, void *stream);
+int so3g_precomp_from_map_f64(const double *map, const int64_t *map_shape,
+                              const int32_t *pixel, const float *spin, int32_t nc,
+                              int64_t n_det, int64_t n_t, double *signal, int accumulate,
+                              void *stream);
 int so3g_pc_plan_info(const so3g_pc_plan *plan, int64_t *n_on, int64_t *bytes);
 int so3g_pc_plan_destroy(so3g_pc_plan *plan);
 
@@ -265,6 +276,39 @@ int so3g_batch_tune(so3g_batch *b, int op, double *map, double *const *tiles,
 /* info: n_obs, det*samples, from (dpb, blocks/CU), to (dpb, blocks/CU), CUs */
 int so3g_batch_info(const so3g_batch *b, int64_t *info);
 int so3g_batch_destroy(so3g_batch *b);
+
+/* Layers of the learned map-making preconditioner (no counterpart in the
+ * reference).  Activations are bf16 NHWC (n, h, w, c) with c in {16, 32, 48,
+ * 64}; conv weights bf16 (9, co, ci) (tap = 3 * (dy + 1) + (dx + 1)), bias
+ * float32 (co).  so3g_nn_conv3x3: out = act(bias + conv), zero padding,
+ * act = ReLU if relu, float32 accumulation on the matrix cores.
+ * so3g_nn_conv3x3_wgrad ADDS dW (float32, (9, co, ci)) and db (float32, co;
+ * may be NULL) of sum(dy * conv(in)).  so3g_nn_weight_flip makes the (9, ci,
+ * co) weights whose conv is the data gradient.  so3g_nn_relu_mask: dz = dy *
+ * (act > 0).  pool2/up2: 2x2 average pooling and nearest 2x upsampling of an
+ * (n, h, w, c) input (h, w: the input's for the forward, the forward
+ * input's for backward = 1).  from/to_planes convert float32 (n, cp, hy, wx)
+ * planes to/from the zero-padded bf16 layout.  so3g_nn_adam: one Adam step
+ * (step counts from 1) on float32 weights, optionally refreshing a bf16
+ * copy. */
+int so3g_nn_conv3x3(const void *in, int n, int h, int w, int ci, const void *wt,
+                    const float *bias, int co, int relu, void *out, void *stream);
+int so3g_nn_conv3x3_wgrad(const void *in, const void *dy, int n, int h, int w, int ci, int co,
+                          float *dw, float *db, void *stream);
+int so3g_nn_weight_flip(const void *wt, int ci, int co, void *wt_t, void *stream);
+int so3g_nn_relu_mask(const void *dy, const void *act, int64_t n, void *dz, void *stream);
+int so3g_nn_add(const void *a, const void *b, int64_t n, void *out, void *stream);
+int so3g_nn_pool2(const void *in, int n, int h, int w, int c, void *out, int backward,
+                  void *stream);
+int so3g_nn_up2(const void *in, int n, int h, int w, int c, void *out, int backward,
+                void *stream);
+int so3g_nn_from_planes(const float *x, int n, int cp, int hy, int wx, int h, int w, int c,
+                        void *out, void *stream);
+int so3g_nn_to_planes(const void *in, int n, int h, int w, int c, int cp, int hy, int wx,
+                      float *x, void *stream);
+int so3g_nn_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, float b1,
+                 float b2, float eps, int step, void *w_bf16, void *stream);
+int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
 
 /* Implementation tunables (no counterpart in the reference):
  *   "scatter_agg"   1: sum runs of equal pixels within a wave before the

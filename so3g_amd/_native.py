@@ -72,6 +72,9 @@ _SIGS = {
     'so3g_pc_plan_create': (c_int, [c_vp, c_i32, c_vp, c_i32, c_i64, c_i64, c_vp, c_i64,
                                     ctypes.POINTER(c_vp), c_vp]),
     'so3g_pc_plan_to_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    'so3g_pc_plan_to_map_f64': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    'so3g_precomp_from_map_f64': (c_int, [c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_i64, c_vp,
+                                          c_int, c_vp]),
     'so3g_pc_plan_info': (c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
     'so3g_pc_plan_destroy': (c_int, [c_vp]),
     'so3g_batch_create': (c_int, [ctypes.POINTER(Geom), ctypes.POINTER(Obs), c_i64,
@@ -83,6 +86,22 @@ _SIGS = {
                                 ctypes.POINTER(c_int), c_vp]),
     'so3g_batch_info': (c_int, [c_vp, c_vp]),
     'so3g_batch_destroy': (c_int, [c_vp]),
+    'so3g_nn_conv3x3': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_int, c_int,
+                                c_vp, c_vp]),
+    'so3g_nn_conv3x3_wgrad': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp,
+                                      c_vp, c_vp]),
+    'so3g_nn_weight_flip': (c_int, [c_vp, c_int, c_int, c_vp, c_vp]),
+    'so3g_nn_relu_mask': (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
+    'so3g_nn_add': (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
+    'so3g_nn_pool2': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_int, c_vp]),
+    'so3g_nn_up2': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_int, c_vp]),
+    'so3g_nn_from_planes': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                    c_vp, c_vp]),
+    'so3g_nn_to_planes': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                  c_vp, c_vp]),
+    'so3g_nn_adam': (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, ctypes.c_float, ctypes.c_float,
+                             ctypes.c_float, ctypes.c_float, c_int, c_vp, c_vp]),
+    'so3g_nn_to_bf16': (c_int, [c_vp, c_i64, c_vp, c_vp]),
     'so3g_set_option': (c_int, [ctypes.c_char_p, c_i64]),
     'so3g_get_option': (c_int, [ctypes.c_char_p, ctypes.POINTER(c_i64)]),
     'so3g_precomp_from_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_i64,

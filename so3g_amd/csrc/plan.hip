@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 
 #include "../../include/so3g_hip.h"
 #include "kernels.h"
@@ -127,12 +128,12 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_to_map(const int32_t *key,
 // wave-level run aggregation; segments that start and end inside the lane,
 // and the lane's last segment when the lane holds more than one, are added
 // to the map directly.
-template <int NC, int R>
+template <int NC, int R, typename ST = float>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_to_map_r(const int32_t *key,
                                                               const uint32_t *idx,
                                                               const float *spin, int64_t n_on,
                                                               uint32_t n_t, uint32_t plane,
-                                                              const float *sig,
+                                                              const ST *sig,
                                                               const float *det_weights,
                                                               double *map) {
     static_assert(R % 4 == 0 && (R * NC) % 4 == 0, "vector loads");
@@ -176,9 +177,14 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_to_map_r(const int32_t *key
         if (det_weights != nullptr) w = det_weights[si / n_t];
         double x[NC];
         if (w != 0.f) {  // a zero weight skips the detector (Projection.cxx:2104)
-            const float sg = sig[si];
+            const ST sg = sig[si];
 #pragma unroll
-            for (int c = 0; c < NC; c++) x[c] = (double)(w * sg * sp[r * NC + c]);
+            for (int c = 0; c < NC; c++) {
+                if constexpr (std::is_same<ST, double>::value)
+                    x[c] = (double)w * sg * (double)sp[r * NC + c];
+                else
+                    x[c] = (double)(w * sg * sp[r * NC + c]);
+            }
         } else {
 #pragma unroll
             for (int c = 0; c < NC; c++) x[c] = 0.;
@@ -311,8 +317,12 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
     return SO3G_OK;
 }
 
-int so3g_pc_plan_to_map(const so3g_pc_plan *p, double *map, const float *signal,
-                        const float *det_weights, void *stream) {
+}  // extern "C"
+
+namespace {
+template <typename ST>
+int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
+                const float *det_weights, void *stream) {
     if (p == nullptr) return set_error(SO3G_ERR_VALUE, "plan is NULL");
     if (p->n_on == 0) return SO3G_OK;
     hipStream_t s = (hipStream_t)stream;
@@ -321,9 +331,69 @@ int so3g_pc_plan_to_map(const so3g_pc_plan *p, double *map, const float *signal,
     const uint32_t plane = (uint32_t)(p->ny * p->nx);
     const uint32_t nt = (uint32_t)p->n_t;
     switch (p->nc) {
-    case 1: hipLaunchKernelGGL((k_plan_to_map_r<1, kPlanR>), g, dim3(kPlanBlock), 0, s, p->key, p->idx, p->spin, p->n_on, nt, plane, signal, det_weights, map); break;
-    case 2: hipLaunchKernelGGL((k_plan_to_map_r<2, kPlanR>), g, dim3(kPlanBlock), 0, s, p->key, p->idx, p->spin, p->n_on, nt, plane, signal, det_weights, map); break;
-    default: hipLaunchKernelGGL((k_plan_to_map_r<3, kPlanR>), g, dim3(kPlanBlock), 0, s, p->key, p->idx, p->spin, p->n_on, nt, plane, signal, det_weights, map); break;
+    case 1: hipLaunchKernelGGL((k_plan_to_map_r<1, kPlanR, ST>), g, dim3(kPlanBlock), 0, s, p->key, p->idx, p->spin, p->n_on, nt, plane, signal, det_weights, map); break;
+    case 2: hipLaunchKernelGGL((k_plan_to_map_r<2, kPlanR, ST>), g, dim3(kPlanBlock), 0, s, p->key, p->idx, p->spin, p->n_on, nt, plane, signal, det_weights, map); break;
+    default: hipLaunchKernelGGL((k_plan_to_map_r<3, kPlanR, ST>), g, dim3(kPlanBlock), 0, s, p->key, p->idx, p->spin, p->n_on, nt, plane, signal, det_weights, map); break;
+    }
+    PLAN_TRY(hipGetLastError());
+    return SO3G_OK;
+}
+
+// Float64 gather for the map-maker: sig[i] (+)= sum_c map[c, pix_i] * spin[i, c]
+// in float64; samples off the map (pixel row < 0 or outside the map) add 0.
+template <int NC>
+__global__ __launch_bounds__(kPlanBlock) void k_pc_from_map_f64(int64_t n, int32_t ny,
+                                                                int32_t nx,
+                                                                const double *map,
+                                                                const int32_t *pix,
+                                                                const float *spin,
+                                                                double *sig,
+                                                                int accumulate) {
+    const int64_t plane = (int64_t)ny * nx;
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kPlanBlock) {
+        const int2 q = reinterpret_cast<const int2 *>(pix)[i];
+        double v = accumulate ? sig[i] : 0.;
+        if (q.x >= 0 && q.x < ny && q.y >= 0 && q.y < nx) {
+            const int64_t px = (int64_t)q.x * nx + q.y;
+#pragma unroll
+            for (int c = 0; c < NC; c++) v += map[c * plane + px] * (double)spin[i * NC + c];
+        }
+        sig[i] = v;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int so3g_pc_plan_to_map(const so3g_pc_plan *p, double *map, const float *signal,
+                        const float *det_weights, void *stream) {
+    return plan_to_map<float>(p, map, signal, det_weights, stream);
+}
+
+int so3g_pc_plan_to_map_f64(const so3g_pc_plan *p, double *map, const double *signal,
+                            const float *det_weights, void *stream) {
+    return plan_to_map<double>(p, map, signal, det_weights, stream);
+}
+
+int so3g_precomp_from_map_f64(const double *map, const int64_t *map_shape,
+                              const int32_t *pixel, const float *spin, int32_t nc,
+                              int64_t n_det, int64_t n_t, double *signal, int accumulate,
+                              void *stream) {
+    if (nc < 1 || nc > 3) return set_error(SO3G_ERR_RUNTIME, "spin component count must be 1..3");
+    if (n_det < 0 || n_t < 0) return set_error(SO3G_ERR_RUNTIME, "negative sample count");
+    const int64_t n = n_det * n_t;
+    if (n == 0) return SO3G_OK;
+    const int64_t ny = map_shape[0], nx = map_shape[1];
+    if (ny <= 0 || nx <= 0 || ny > INT32_MAX || nx > INT32_MAX)
+        return set_error(SO3G_ERR_RUNTIME, "bad map shape");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nb = std::min<int64_t>((n + kPlanBlock - 1) / kPlanBlock, 1 << 20);
+    const dim3 g((unsigned)nb);
+    switch (nc) {
+    case 1: hipLaunchKernelGGL(k_pc_from_map_f64<1>, g, dim3(kPlanBlock), 0, s, n, (int32_t)ny, (int32_t)nx, map, pixel, spin, signal, accumulate); break;
+    case 2: hipLaunchKernelGGL(k_pc_from_map_f64<2>, g, dim3(kPlanBlock), 0, s, n, (int32_t)ny, (int32_t)nx, map, pixel, spin, signal, accumulate); break;
+    default: hipLaunchKernelGGL(k_pc_from_map_f64<3>, g, dim3(kPlanBlock), 0, s, n, (int32_t)ny, (int32_t)nx, map, pixel, spin, signal, accumulate); break;
     }
     PLAN_TRY(hipGetLastError());
     return SO3G_OK;

@@ -1,0 +1,248 @@
+"""Maximum-likelihood map-making on the projection kernels.
+
+Solves  P^T N^-1 P m = P^T N^-1 d  by preconditioned conjugate gradients,
+where P is the cached pointing matrix (ProjEng_Precomp pixel_index /
+spin_proj, applied with the HIP from_map / pixel-sorted to_map kernels) and
+N the per-detector stationary noise covariance, applied in Fourier space.
+The reference has no map-maker; this is the caller the projection
+operators exist for (sotodlib's ML mapmaker uses so3g's ProjEng_Precomp the
+same way: Projection.cxx:2012-2253 for P and P^T).
+
+Preconditioners: the binned (per-pixel nc x nc block-diagonal) inverse of
+P^T diag(1/sigma^2) P, and a learned correction on top of it (see unet.py),
+used with flexible CG.
+"""
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .engine import _stream
+
+
+class NoiseModel:
+    """Stationary noise with PSD_d(f) = sigma_d^2 (1 + (f / fknee)^alpha)
+    per detector (white level sigma_d^2 per sample).  N^-1 is applied as
+    irfft(rfft(x) / PSD) in float64; the f = 0 bin is projected out."""
+
+    def __init__(self, sigma, n_t, fsample=200., fknee=1., alpha=-3., device=None):
+        self.n_t = int(n_t)
+        self.sigma = np.asarray(sigma, np.float64)
+        f = np.fft.rfftfreq(self.n_t, 1. / fsample)
+        with np.errstate(divide='ignore'):
+            shape = 1. + (f / fknee) ** alpha
+        shape[0] = np.inf
+        psd = self.sigma[:, None] ** 2 * shape[None, :]
+        dev = device or torch.device('cuda')
+        self.inv_psd = torch.from_numpy(1. / psd).to(dev)
+        amp = np.sqrt(psd)
+        amp[:, 0] = 0.
+        self.amp = torch.from_numpy(amp).to(dev)
+
+    def apply_inverse(self, sig, out=None):
+        """N^-1 sig, returned in sig's dtype (or written to out)."""
+        x = torch.fft.rfft(sig.to(torch.float64), dim=1)
+        x *= self.inv_psd
+        y = torch.fft.irfft(x, n=self.n_t, dim=1)
+        if out is None:
+            return y.to(sig.dtype)
+        out.copy_(y)
+        return out
+
+    def simulate(self, generator):
+        w = torch.randn(self.inv_psd.shape[0], self.n_t, dtype=torch.float64,
+                        device=self.inv_psd.device, generator=generator)
+        x = torch.fft.rfft(w, dim=1) * self.amp
+        return torch.fft.irfft(x, n=self.n_t, dim=1).to(torch.float32)
+
+
+class PointingOperator:
+    """P (map -> signal) and P^T (signal -> map) for a non-tiled flat map of
+    shape (nc, ny, nx), through the cached pointing matrix; P^T runs on the
+    pixel-sorted plan."""
+
+    def __init__(self, pixel_index, spin_proj, map_shape, use_plan=True):
+        from .engine import PrecompPlan
+        self.pix = pixel_index.contiguous()
+        self.spin = spin_proj.contiguous()
+        self.n_det, self.n_t, _ = self.pix.shape
+        self.nc = int(self.spin.shape[2])
+        self.map_shape = (self.nc, int(map_shape[0]), int(map_shape[1]))
+        self._shape = np.array(map_shape[:2], np.int64)
+        self.plan = PrecompPlan(self.pix, self.spin, map_shape[:2]) if use_plan else None
+        self.lib = N.load()
+
+    def P(self, m, out):
+        """out = P m (fresh)."""
+        N.check(self.lib.so3g_precomp_from_map(m.data_ptr(), self._shape.ctypes.data, None,
+                                               None, self.pix.data_ptr(), 2,
+                                               self.spin.data_ptr(), self.nc, self.n_det,
+                                               self.n_t, out.data_ptr(), 0, _stream()))
+        return out
+
+    def PT(self, sig, out, det_weights=None):
+        """out = P^T sig (out is overwritten)."""
+        out.zero_()
+        dw = None if det_weights is None else det_weights.data_ptr()
+        if self.plan is not None:
+            N.check(self.lib.so3g_pc_plan_to_map(self.plan._h, out.data_ptr(), sig.data_ptr(),
+                                                 dw, _stream()))
+        else:
+            N.check(self.lib.so3g_precomp_to_map(out.data_ptr(), self._shape.ctypes.data,
+                                                 None, None, self.pix.data_ptr(), 2,
+                                                 self.spin.data_ptr(), self.nc, self.n_det,
+                                                 self.n_t, sig.data_ptr(), dw, None, 0,
+                                                 _stream()))
+        return out
+
+    def P64(self, m, out):
+        """out = P m in float64 (out: float64 (n_det, n_t))."""
+        N.check(self.lib.so3g_precomp_from_map_f64(m.data_ptr(), self._shape.ctypes.data,
+                                                   self.pix.data_ptr(), self.spin.data_ptr(),
+                                                   self.nc, self.n_det, self.n_t,
+                                                   out.data_ptr(), 0, _stream()))
+        return out
+
+    def PT64(self, sig, out, det_weights=None):
+        """out = P^T sig with float64 products (sig: float64)."""
+        out.zero_()
+        dw = None if det_weights is None else det_weights.data_ptr()
+        N.check(self.lib.so3g_pc_plan_to_map_f64(self.plan._h, out.data_ptr(), sig.data_ptr(),
+                                                 dw, _stream()))
+        return out
+
+    def weights(self, det_weights=None):
+        """P^T diag(det_weights) P as (nc, nc, ny, nx), upper triangle
+        filled (so3g's to_weights) and mirrored here."""
+        nc = self.nc
+        w = torch.zeros((nc, nc) + self.map_shape[1:], dtype=torch.float64,
+                        device=self.pix.device)
+        dw = None if det_weights is None else det_weights.data_ptr()
+        N.check(self.lib.so3g_precomp_to_weights(w.data_ptr(), self._shape.ctypes.data, None,
+                                                 None, self.pix.data_ptr(), 2,
+                                                 self.spin.data_ptr(), nc, self.n_det,
+                                                 self.n_t, dw, None, 0, _stream()))
+        for a in range(nc):
+            for b in range(a):
+                w[a, b] = w[b, a]
+        return w
+
+
+class BinnedPreconditioner:
+    """M^-1 = per-pixel inverse of the nc x nc block of P^T diag(w) P.
+    Pixels whose block is ill-conditioned (eigenvalue ratio below rcond: too
+    few hits or angles) get its diagonal inverse instead, so that M stays
+    positive definite on every hit pixel; unhit pixels get 0.  `mask` marks
+    the well-conditioned pixels."""
+
+    def __init__(self, weights, rcond=1e-3):
+        nc = weights.shape[0]
+        w = weights.reshape(nc, nc, -1).permute(2, 0, 1)          # (npix, nc, nc)
+        ev = torch.linalg.eigvalsh(w)
+        ok = (ev[:, 0] > rcond * ev[:, -1]) & (ev[:, -1] > 0)
+        eye = torch.eye(nc, dtype=w.dtype, device=w.device)
+        inv = torch.linalg.inv(torch.where(ok[:, None, None], w, eye))
+        diag = torch.diagonal(w, dim1=1, dim2=2)
+        dinv = torch.where(diag > 0, 1. / torch.where(diag > 0, diag, 1.), 0.)
+        inv = torch.where(ok[:, None, None], inv, torch.diag_embed(dinv))
+        self.inv = inv.permute(1, 2, 0).reshape(weights.shape).contiguous()
+        self.mask = ok.reshape(weights.shape[2:])
+        self.nc = nc
+
+    def __call__(self, r):
+        nc = self.nc
+        out = torch.zeros_like(r)
+        for a in range(nc):
+            for b in range(nc):
+                out[a] += self.inv[a, b] * r[b]
+        return out
+
+
+def _dot(a, b):
+    return float(torch.sum(a * b))
+
+
+def pcg(apply_A, b, precond, tol=1e-6, maxiter=2000, x0=None, flexible=False,
+        callback=None):
+    """(Flexible) preconditioned CG.  Stops when ||r|| / ||b|| < tol.
+    flexible: Polak-Ribiere beta (Notay's FCG(1)), for preconditioners that
+    are not fixed linear maps.  Returns (x, info) with info = dict(iters,
+    residuals, seconds)."""
+    t0 = time.perf_counter()
+    x = torch.zeros_like(b) if x0 is None else x0.clone()
+    r = b - apply_A(x) if x0 is not None else b.clone()
+    bnorm = np.sqrt(_dot(b, b))
+    res = [np.sqrt(_dot(r, r)) / bnorm]
+    z = precond(r)
+    p = z.clone()
+    rz = _dot(r, z)
+    it = 0
+    while res[-1] >= tol and it < maxiter:
+        Ap = apply_A(p)
+        pAp = _dot(p, Ap)
+        if not pAp > 0:
+            break       # breakdown: p is in the null space of A (or M is not SPD)
+        alpha = rz / pAp
+        x.add_(p, alpha=alpha)
+        r_old = r.clone() if flexible else None
+        r.sub_(Ap, alpha=alpha)
+        it += 1
+        res.append(np.sqrt(_dot(r, r)) / bnorm)
+        if callback is not None:
+            callback(it, x, r)
+        if res[-1] < tol:
+            break
+        z = precond(r)
+        if flexible:
+            beta = _dot(z, r - r_old) / rz
+            rz = _dot(r, z)
+        else:
+            rz_new = _dot(r, z)
+            beta = rz_new / rz
+            rz = rz_new
+        p.mul_(beta).add_(z)
+    torch.cuda.synchronize()
+    return x, {'iters': it, 'residuals': res, 'seconds': time.perf_counter() - t0}
+
+
+class MapMaker:
+    """A = P^T N^-1 P on one observation; rhs(d) = P^T N^-1 d.
+
+    precise (default): the time-ordered data inside A are float64 (the
+    float64 gather and plan scatter), so A is symmetric to float64 rounding.
+    With float32 time streams (so3g's own arithmetic) the rounding of P m,
+    of order 1e-7 of the signal and white, passes N^-1 at full weight while
+    the 1/f-dominated signal modes are suppressed by up to 1e-7, and CG
+    stalls near ||r||/||b|| ~ 1e-4."""
+
+    def __init__(self, pointing, noise, precise=True):
+        self.P = pointing
+        self.noise = noise
+        self.precise = precise
+        if precise and pointing.plan is None:
+            raise ValueError('precise map-making needs the pointing plan')
+        self._sig = torch.empty((pointing.n_det, pointing.n_t),
+                                dtype=torch.float64 if precise else torch.float32,
+                                device=pointing.pix.device)
+
+    def zeros(self):
+        return torch.zeros(self.P.map_shape, dtype=torch.float64, device=self.P.pix.device)
+
+    def apply_A(self, m):
+        if self.precise:
+            self.P.P64(m, self._sig)
+            return self.P.PT64(self.noise.apply_inverse(self._sig), self.zeros())
+        self.P.P(m, self._sig)
+        return self.P.PT(self.noise.apply_inverse(self._sig), self.zeros())
+
+    def rhs(self, d):
+        if self.precise:
+            return self.P.PT64(self.noise.apply_inverse(d.to(torch.float64)), self.zeros())
+        return self.P.PT(self.noise.apply_inverse(d), self.zeros())
+
+    def binned(self, rcond=1e-3):
+        w = torch.from_numpy(1. / self.noise.sigma ** 2).to(torch.float32).to(
+            self.P.pix.device)
+        return BinnedPreconditioner(self.P.weights(w), rcond=rcond)
