@@ -285,7 +285,7 @@ int so3g_batch_destroy(so3g_batch *b);
  * so3g_nn_conv3x3_wgrad ADDS dW (float32, (9, co, ci)) and db (float32, co;
  * may be NULL) of sum(dy * conv(in)).  so3g_nn_weight_flip makes the (9, ci,
  * co) weights whose conv is the data gradient.  so3g_nn_relu_mask: dz = dy *
- * (act > 0).  pool2/up2: 2x2 average pooling and nearest 2x upsampling of an
+ * (act > 0).  pool/up: (fy, fx) average pooling and nearest upsampling of an
  * (n, h, w, c) input (h, w: the input's for the forward, the forward
  * input's for backward = 1).  from/to_planes convert float32 (n, cp, hy, wx)
  * planes to/from the zero-padded bf16 layout.  so3g_nn_adam: one Adam step
@@ -298,10 +298,10 @@ int so3g_nn_conv3x3_wgrad(const void *in, const void *dy, int n, int h, int w, i
 int so3g_nn_weight_flip(const void *wt, int ci, int co, void *wt_t, void *stream);
 int so3g_nn_relu_mask(const void *dy, const void *act, int64_t n, void *dz, void *stream);
 int so3g_nn_add(const void *a, const void *b, int64_t n, void *out, void *stream);
-int so3g_nn_pool2(const void *in, int n, int h, int w, int c, void *out, int backward,
-                  void *stream);
-int so3g_nn_up2(const void *in, int n, int h, int w, int c, void *out, int backward,
-                void *stream);
+int so3g_nn_pool(const void *in, int n, int h, int w, int c, int fy, int fx, void *out,
+                 int backward, void *stream);
+int so3g_nn_up(const void *in, int n, int h, int w, int c, int fy, int fx, void *out,
+               int backward, void *stream);
 int so3g_nn_from_planes(const float *x, int n, int cp, int hy, int wx, int h, int w, int c,
                         void *out, void *stream);
 int so3g_nn_to_planes(const void *in, int n, int h, int w, int c, int cp, int hy, int wx,

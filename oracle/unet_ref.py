@@ -63,25 +63,28 @@ def conv_grads(x, gz, w):
     return dw, db, bf16(gx)
 
 
-def pool(x):
+def pool(x, f=(2, 2)):
     n, h, w, c = x.shape
-    return bf16(0.25 * x.reshape(n, h // 2, 2, w // 2, 2, c).sum(axis=(2, 4)))
+    fy, fx = f
+    return bf16(x.reshape(n, h // fy, fy, w // fx, fx, c).sum(axis=(2, 4)) / (fy * fx))
 
 
-def pool_bwd(g):
-    return bf16(0.25 * np.repeat(np.repeat(g, 2, axis=1), 2, axis=2))
+def pool_bwd(g, f=(2, 2)):
+    fy, fx = f
+    return bf16(np.repeat(np.repeat(g, fy, axis=1), fx, axis=2) / (fy * fx))
 
 
-def up(x):
-    return np.repeat(np.repeat(x, 2, axis=1), 2, axis=2)
+def up(x, f=(2, 2)):
+    return np.repeat(np.repeat(x, f[0], axis=1), f[1], axis=2)
 
 
-def up_bwd(g):
+def up_bwd(g, f=(2, 2)):
     n, h, w, c = g.shape
-    return bf16(g.reshape(n, h // 2, 2, w // 2, 2, c).sum(axis=(2, 4)))
+    fy, fx = f
+    return bf16(g.reshape(n, h // fy, fy, w // fx, fx, c).sum(axis=(2, 4)))
 
 
-def forward(prog, specs, weights, x):
+def forward(prog, specs, weights, x, f=(2, 2)):
     """weights: name -> (w (9,co,ci) float, b (co)).  Returns the tape."""
     t = {'x': bf16(x)}
     for op, o, ins, layer in prog:
@@ -89,15 +92,15 @@ def forward(prog, specs, weights, x):
             w, b = weights[layer]
             t[o] = conv(t[ins[0]], bf16(w), np.asarray(b, np.float64), specs[layer][2])
         elif op == 'pool':
-            t[o] = pool(t[ins[0]])
+            t[o] = pool(t[ins[0]], f)
         elif op == 'up':
-            t[o] = up(t[ins[0]])
+            t[o] = up(t[ins[0]], f)
         else:
             t[o] = bf16(t[ins[0]] + t[ins[1]])
     return t
 
 
-def backward(prog, specs, weights, tape, gy):
+def backward(prog, specs, weights, tape, gy, f=(2, 2)):
     dw, db = {}, {}
     g = {'y': bf16(gy)}
     def acc(name, v):
@@ -112,9 +115,9 @@ def backward(prog, specs, weights, tape, gy):
             if ins[0] != 'x':
                 acc(ins[0], gx)
         elif op == 'pool':
-            acc(ins[0], pool_bwd(go))
+            acc(ins[0], pool_bwd(go, f))
         elif op == 'up':
-            acc(ins[0], up_bwd(go))
+            acc(ins[0], up_bwd(go, f))
         else:
             acc(ins[0], go)
             acc(ins[1], go)

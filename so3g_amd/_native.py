@@ -93,8 +93,10 @@ _SIGS = {
     'so3g_nn_weight_flip': (c_int, [c_vp, c_int, c_int, c_vp, c_vp]),
     'so3g_nn_relu_mask': (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
     'so3g_nn_add': (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp]),
-    'so3g_nn_pool2': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_int, c_vp]),
-    'so3g_nn_up2': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_int, c_vp]),
+    'so3g_nn_pool': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_vp, c_int,
+                             c_vp]),
+    'so3g_nn_up': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_vp, c_int,
+                           c_vp]),
     'so3g_nn_from_planes': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                     c_vp, c_vp]),
     'so3g_nn_to_planes': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int,

@@ -1,7 +1,7 @@
 // unet.hip -- the layers of the learned CG preconditioner: 3x3 convolutions
 // on bf16 NHWC activations as implicit GEMMs on the CDNA matrix cores
 // (v_mfma_f32_16x16x16_bf16, float32 accumulation), their weight and data
-// gradients, 2x2 average pooling, nearest 2x upsampling, and an Adam step.
+// gradients, average pooling, nearest upsampling, and an Adam step.
 //
 // MFMA 16x16x16 bf16 operand layout (one wave, lane l = 0..63):
 //   A (16 x 16, row i, col k): lane l holds A[l % 16][4 * (l / 16) + v], v = 0..3
@@ -168,9 +168,13 @@ __global__ void k_add(const uint16_t *a, const uint16_t *b, int64_t n, uint16_t 
         o[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
 }
 
-// 2x2 average pooling of (N, H, W, C) -> (N, H/2, W/2, C) and its gradient.
-__global__ void k_pool2(const uint16_t *in, int N, int H, int W, int C, uint16_t *out) {
-    const int Ho = H / 2, Wo = W / 2;
+// fy x fx average pooling of (N, H, W, C) -> (N, H/fy, W/fx, C) and its
+// gradient; nearest (fy, fx) upsampling of (N, H, W, C) -> (N, H fy, W fx, C)
+// and its gradient (the sum over each block).
+__global__ void k_pool(const uint16_t *in, int N, int H, int W, int C, int fy, int fx,
+                       uint16_t *out) {
+    const int Ho = H / fy, Wo = W / fx;
+    const float inv = 1.f / (float)(fy * fx);
     const int64_t n_out = (int64_t)N * Ho * Wo * C;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_out;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -178,15 +182,18 @@ __global__ void k_pool2(const uint16_t *in, int N, int H, int W, int C, uint16_t
         const int64_t p = i / C;
         const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho);
         const int64_t n = p / ((int64_t)Wo * Ho);
-        const int64_t b = ((n * H + 2 * y) * W + 2 * x) * C + c;
-        const float s = bf2f(in[b]) + bf2f(in[b + C]) + bf2f(in[b + (int64_t)W * C]) +
-                        bf2f(in[b + (int64_t)W * C + C]);
-        out[i] = f2bf(0.25f * s);
+        float s = 0.f;
+        for (int a = 0; a < fy; a++)
+            for (int b = 0; b < fx; b++)
+                s += bf2f(in[((n * H + (int64_t)y * fy + a) * W + (int64_t)x * fx + b) * C + c]);
+        out[i] = f2bf(inv * s);
     }
 }
 
-__global__ void k_pool2_bwd(const uint16_t *dy, int N, int H, int W, int C, uint16_t *dx) {
-    const int Ho = H / 2, Wo = W / 2;
+__global__ void k_pool_bwd(const uint16_t *dy, int N, int H, int W, int C, int fy, int fx,
+                           uint16_t *dx) {
+    const int Ho = H / fy, Wo = W / fx;
+    const float inv = 1.f / (float)(fy * fx);
     const int64_t n_in = (int64_t)N * H * W * C;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_in;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -194,24 +201,27 @@ __global__ void k_pool2_bwd(const uint16_t *dy, int N, int H, int W, int C, uint
         const int64_t p = i / C;
         const int x = (int)(p % W), y = (int)((p / W) % H);
         const int64_t n = p / ((int64_t)W * H);
-        dx[i] = f2bf(0.25f * bf2f(dy[((n * Ho + y / 2) * Wo + x / 2) * C + c]));
+        dx[i] = f2bf(inv * bf2f(dy[((n * Ho + y / fy) * Wo + x / fx) * C + c]));
     }
 }
 
-// Nearest 2x upsampling of (N, H, W, C) -> (N, 2H, 2W, C) and its gradient.
-__global__ void k_up2(const uint16_t *in, int N, int H, int W, int C, uint16_t *out) {
-    const int64_t n_out = (int64_t)N * 4 * H * W * C;
+__global__ void k_up(const uint16_t *in, int N, int H, int W, int C, int fy, int fx,
+                     uint16_t *out) {
+    const int Hu = H * fy, Wu = W * fx;
+    const int64_t n_out = (int64_t)N * Hu * Wu * C;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_out;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int c = (int)(i % C);
         const int64_t p = i / C;
-        const int x = (int)(p % (2 * W)), y = (int)((p / (2 * W)) % (2 * H));
-        const int64_t n = p / ((int64_t)4 * W * H);
-        out[i] = in[((n * H + y / 2) * W + x / 2) * C + c];
+        const int x = (int)(p % Wu), y = (int)((p / Wu) % Hu);
+        const int64_t n = p / ((int64_t)Wu * Hu);
+        out[i] = in[((n * H + y / fy) * W + x / fx) * C + c];
     }
 }
 
-__global__ void k_up2_bwd(const uint16_t *dy, int N, int H, int W, int C, uint16_t *dx) {
+__global__ void k_up_bwd(const uint16_t *dy, int N, int H, int W, int C, int fy, int fx,
+                         uint16_t *dx) {
+    const int Hu = H * fy, Wu = W * fx;
     const int64_t n_in = (int64_t)N * H * W * C;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_in;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -219,9 +229,11 @@ __global__ void k_up2_bwd(const uint16_t *dy, int N, int H, int W, int C, uint16
         const int64_t p = i / C;
         const int x = (int)(p % W), y = (int)((p / W) % H);
         const int64_t n = p / ((int64_t)W * H);
-        const int64_t b = ((n * 2 * H + 2 * y) * 2 * W + 2 * x) * C + c;
-        const int64_t r = (int64_t)2 * W * C;
-        dx[i] = f2bf(bf2f(dy[b]) + bf2f(dy[b + C]) + bf2f(dy[b + r]) + bf2f(dy[b + r + C]));
+        float s = 0.f;
+        for (int a = 0; a < fy; a++)
+            for (int b = 0; b < fx; b++)
+                s += bf2f(dy[((n * Hu + (int64_t)y * fy + a) * Wu + (int64_t)x * fx + b) * C + c]);
+        dx[i] = f2bf(s);
     }
 }
 
@@ -351,29 +363,33 @@ int so3g_nn_add(const void *a, const void *b, int64_t n, void *out, void *stream
     return SO3G_OK;
 }
 
-int so3g_nn_pool2(const void *in, int n, int h, int w, int c, void *out, int backward,
-                  void *stream) {
-    if ((h | w) & 1) return set_error(SO3G_ERR_VALUE, "pooling needs even height and width");
-    const int64_t cnt = (int64_t)n * h * w * c / (backward ? 1 : 4);
+int so3g_nn_pool(const void *in, int n, int h, int w, int c, int fy, int fx, void *out,
+                 int backward, void *stream) {
+    if (fy < 1 || fx < 1 || fy > 8 || fx > 8)
+        return set_error(SO3G_ERR_VALUE, "pooling factors must be 1..8");
+    if (h % fy || w % fx) return set_error(SO3G_ERR_VALUE, "image not divisible by the pooling factors");
+    const int64_t cnt = (int64_t)n * h * w * c / (backward ? 1 : fy * fx);
     if (backward)
-        hipLaunchKernelGGL(k_pool2_bwd, ew_grid(cnt), dim3(kNB), 0, (hipStream_t)stream,
-                           (const uint16_t *)in, n, h, w, c, (uint16_t *)out);
+        hipLaunchKernelGGL(k_pool_bwd, ew_grid(cnt), dim3(kNB), 0, (hipStream_t)stream,
+                           (const uint16_t *)in, n, h, w, c, fy, fx, (uint16_t *)out);
     else
-        hipLaunchKernelGGL(k_pool2, ew_grid(cnt), dim3(kNB), 0, (hipStream_t)stream,
-                           (const uint16_t *)in, n, h, w, c, (uint16_t *)out);
+        hipLaunchKernelGGL(k_pool, ew_grid(cnt), dim3(kNB), 0, (hipStream_t)stream,
+                           (const uint16_t *)in, n, h, w, c, fy, fx, (uint16_t *)out);
     NN_CHECK();
     return SO3G_OK;
 }
 
-int so3g_nn_up2(const void *in, int n, int h, int w, int c, void *out, int backward,
-                void *stream) {
-    const int64_t cnt = (int64_t)n * h * w * c * (backward ? 1 : 4);
+int so3g_nn_up(const void *in, int n, int h, int w, int c, int fy, int fx, void *out,
+               int backward, void *stream) {
+    if (fy < 1 || fx < 1 || fy > 8 || fx > 8)
+        return set_error(SO3G_ERR_VALUE, "upsampling factors must be 1..8");
+    const int64_t cnt = (int64_t)n * h * w * c * (backward ? 1 : fy * fx);
     if (backward)
-        hipLaunchKernelGGL(k_up2_bwd, ew_grid(cnt), dim3(kNB), 0, (hipStream_t)stream,
-                           (const uint16_t *)in, n, h, w, c, (uint16_t *)out);
+        hipLaunchKernelGGL(k_up_bwd, ew_grid(cnt), dim3(kNB), 0, (hipStream_t)stream,
+                           (const uint16_t *)in, n, h, w, c, fy, fx, (uint16_t *)out);
     else
-        hipLaunchKernelGGL(k_up2, ew_grid(cnt), dim3(kNB), 0, (hipStream_t)stream,
-                           (const uint16_t *)in, n, h, w, c, (uint16_t *)out);
+        hipLaunchKernelGGL(k_up, ew_grid(cnt), dim3(kNB), 0, (hipStream_t)stream,
+                           (const uint16_t *)in, n, h, w, c, fy, fx, (uint16_t *)out);
     NN_CHECK();
     return SO3G_OK;
 }

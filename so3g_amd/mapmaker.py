@@ -165,11 +165,14 @@ def _dot(a, b):
 
 
 def pcg(apply_A, b, precond, tol=1e-6, maxiter=2000, x0=None, flexible=False,
-        callback=None):
+        callback=None, fallback=None, stall=(200, 0.9)):
     """(Flexible) preconditioned CG.  Stops when ||r|| / ||b|| < tol.
     flexible: Polak-Ribiere beta (Notay's FCG(1)), for preconditioners that
-    are not fixed linear maps.  Returns (x, info) with info = dict(iters,
-    residuals, seconds)."""
+    are not fixed linear maps.  fallback: a second preconditioner; when the
+    residual has not fallen below stall[1] x its value stall[0] iterations
+    earlier, CG restarts from the current x with the fallback (plain PCG).
+    callback(it, x, r, p, Ap) after each iteration.  Returns (x, info) with
+    info = dict(iters, residuals, seconds, fallback_at)."""
     t0 = time.perf_counter()
     x = torch.zeros_like(b) if x0 is None else x0.clone()
     r = b - apply_A(x) if x0 is not None else b.clone()
@@ -179,6 +182,7 @@ def pcg(apply_A, b, precond, tol=1e-6, maxiter=2000, x0=None, flexible=False,
     p = z.clone()
     rz = _dot(r, z)
     it = 0
+    fallback_at = None
     while res[-1] >= tol and it < maxiter:
         Ap = apply_A(p)
         pAp = _dot(p, Ap)
@@ -191,9 +195,18 @@ def pcg(apply_A, b, precond, tol=1e-6, maxiter=2000, x0=None, flexible=False,
         it += 1
         res.append(np.sqrt(_dot(r, r)) / bnorm)
         if callback is not None:
-            callback(it, x, r)
+            callback(it, x, r, p, Ap)
         if res[-1] < tol:
             break
+        if (fallback is not None and fallback_at is None and it > stall[0] and
+                res[-1] > stall[1] * res[-1 - stall[0]]):
+            # stalled: restart as plain PCG with the fallback preconditioner
+            fallback_at = it
+            precond, flexible = fallback, False
+            z = precond(r)
+            p = z.clone()
+            rz = _dot(r, z)
+            continue
         z = precond(r)
         if flexible:
             beta = _dot(z, r - r_old) / rz
@@ -204,7 +217,8 @@ def pcg(apply_A, b, precond, tol=1e-6, maxiter=2000, x0=None, flexible=False,
             rz = rz_new
         p.mul_(beta).add_(z)
     torch.cuda.synchronize()
-    return x, {'iters': it, 'residuals': res, 'seconds': time.perf_counter() - t0}
+    return x, {'iters': it, 'residuals': res, 'seconds': time.perf_counter() - t0,
+               'fallback_at': fallback_at}
 
 
 class MapMaker:
@@ -246,3 +260,86 @@ class MapMaker:
         w = torch.from_numpy(1. / self.noise.sigma ** 2).to(torch.float32).to(
             self.P.pix.device)
         return BinnedPreconditioner(self.P.weights(w), rcond=rcond)
+
+
+SMOOTH = (15, 63, 255)
+
+
+def features(mr):
+    """Network input for binned-preconditioned residuals mr (n, nc, ny, nx):
+    mr / s and its running means along the scan direction (map rows) over
+    SMOOTH pixels, each scaled to unit rms, so that the large-scale part --
+    the part the binned preconditioner leaves unresolved -- is not lost to
+    bf16 rounding next to the pixel-scale part.  Returns (features
+    (n, nc * (1 + len(SMOOTH)), ny, nx) float32, s (n,))."""
+    mr = mr.to(torch.float32)
+    s = mr.pow(2).mean(dim=(1, 2, 3)).sqrt().clamp_min(1e-30)
+    out = [mr / s[:, None, None, None]]
+    n, nc, ny, nx = mr.shape
+    for w in SMOOTH:
+        sm = torch.nn.functional.avg_pool2d(mr, (1, w), stride=1, padding=(0, w // 2),
+                                            count_include_pad=False)
+        ss = sm.pow(2).mean(dim=(1, 2, 3)).sqrt().clamp_min(1e-30)
+        out.append(sm / ss[:, None, None, None])
+    return torch.cat(out, dim=1), s
+
+
+class LearnedPreconditioner:
+    """z = M_b r + s * U(features(M_b r)), s = rms(M_b r): the binned
+    preconditioner plus a U-Net correction trained on (A e, e) pairs.  Not a
+    fixed linear map: use with flexible CG."""
+
+    def __init__(self, binned, net, map_shape):
+        self.M = binned
+        self.net = net
+        self.shape = map_shape      # (nc, ny, nx)
+
+    def __call__(self, r):
+        mr = self.M(r)
+        x, s = features(mr[None])
+        nc, ny, nx = self.shape
+        y, _ = self.net.forward(self.net.to_input(x))
+        c = self.net.from_output(y, nc, ny, nx)[0].to(torch.float64)
+        return mr + float(s[0]) * c
+
+
+def collect_pairs(mm, M, rhs_list, per_rhs, tol=1e-12):
+    """(A p, p) pairs from the search directions of binned PCG runs on each
+    right-hand side: r = A e with e = p, the directions CG still has to
+    resolve.  Returns two float32 tensors (n, nc, ny, nx)."""
+    R, E = [], []
+    def cb(it, x, r, p, Ap):
+        R.append(Ap.to(torch.float32))
+        E.append(p.to(torch.float32))
+    for b in rhs_list:
+        pcg(mm.apply_A, b, M, tol=tol, maxiter=per_rhs, callback=cb)
+    return torch.stack(R), torch.stack(E)
+
+
+def train_correction(net, M, R, E, steps=1500, batch=8, lr=1e-3, seed=0, log=None):
+    """Fit the U-Net so that M r + s U(M r / s) ~ e on the pairs (R, E):
+    relative squared error of the corrected estimate per pair, Adam.
+    Returns the loss history."""
+    g = torch.Generator(device=R.device).manual_seed(seed)
+    n = R.shape[0]
+    nc, ny, nx = R.shape[1:]
+    # binned estimate and scale per pair
+    MR = torch.stack([M(R[i].to(torch.float64)).to(torch.float32) for i in range(n)])
+    X, S = features(MR)
+    T = (E - MR) / S[:, None, None, None]          # target correction
+    TN = (E / S[:, None, None, None]).pow(2).sum(dim=(1, 2, 3))
+    hist = []
+    for step in range(steps):
+        idx = torch.randint(0, n, (batch,), generator=g, device=R.device)
+        x = net.to_input(X[idx])
+        y, tape = net.forward(x)
+        out = net.from_output(y, nc, ny, nx)
+        d = out - T[idx]
+        per = d.pow(2).sum(dim=(1, 2, 3)) / TN[idx]
+        hist.append(float(per.mean()))
+        gy = net.to_input(2 * d / (TN[idx][:, None, None, None] * batch))
+        dw, db = net.backward(tape, gy)
+        net.adam_step(dw, db, lr=lr)
+        if log is not None and step % 100 == 0:
+            log(step, hist[-1])
+    return hist

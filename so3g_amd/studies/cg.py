@@ -6,7 +6,7 @@ Noise: PSD_d(f) = sigma_d^2 (1 + (f / 1 Hz)^-3), sigma_d ~ lognormal(0, 0.3).
 Sky: flat-sky Gaussian random field on the map grid, C_l ~ l^-2.5 for T and
 0.1 x that for E and B, rotated to Q/U.
 
-Run: python -m so3g_amd.studies.cg [--learned] [--pairs 2000]
+Run: python -m so3g_amd.studies.cg [--no-learned] [--steps N] [--n-rhs 4 --per-rhs 500]
 """
 import argparse
 import json
@@ -60,12 +60,23 @@ def setup(n_det=100, n_samp=40000, seed=5):
     sigma = np.exp(rng.normal(0., 0.3, n_det))
     noise = NoiseModel(sigma, n_samp, device=dev)
     mm = MapMaker(pointing, noise)
-    sky = torch.from_numpy(grf_tqu((ny, nx), obs.res_arcmin / 60 * np.pi / 180,
-                                   seed + 1)).to(dev)
+    ctx = dict(mm=mm, shape=(ny, nx), obs=obs)
+    ctx['sky'], ctx['d'] = make_data(ctx, seed)
+    return ctx
+
+
+def make_data(ctx, seed):
+    """(sky, d = P sky + noise) for one realisation."""
+    import torch
+    mm = ctx['mm']
+    P, noise = mm.P, mm.noise
+    dev = P.pix.device
+    res = ctx['obs'].res_arcmin / 60 * np.pi / 180
+    sky = torch.from_numpy(grf_tqu(ctx['shape'], res, seed + 1)).to(dev)
     gen = torch.Generator(device=dev).manual_seed(seed + 2)
-    d = pointing.P(sky, torch.empty((n_det, n_samp), dtype=torch.float32, device=dev))
+    d = P.P(sky, torch.empty((P.n_det, P.n_t), dtype=torch.float32, device=dev))
     d += noise.simulate(gen)
-    return dict(mm=mm, sky=sky, d=d, shape=(ny, nx), obs=obs)
+    return sky, d
 
 
 def solve_binned(ctx, tol=1e-6, maxiter=3000):
@@ -85,30 +96,75 @@ def map_error(x, sky, mask):
     return [float(v) for v in d.pow(2).mean(dim=1).sqrt()]
 
 
-def run(tol=1e-6, maxiter=3000):
+def summary(x, info, ctx, M):
+    return {'iters': info['iters'], 'seconds': info['seconds'],
+            'final_residual': info['residuals'][-1],
+            'residual_every_100': [info['residuals'][i]
+                                   for i in range(0, len(info['residuals']), 100)],
+            'fallback_at': info.get('fallback_at'),
+            'map_rms_error_TQU': map_error(x, ctx['sky'], M.mask),
+            'ms_per_iter': info['seconds'] / max(info['iters'], 1) * 1e3}
+
+
+def run(tol=1e-6, maxiter=40000, learned=True, n_rhs=4, per_rhs=500, steps=3000,
+        batch=8, lr=3e-4, factor=(2, 2)):
     import torch
+    from ..mapmaker import LearnedPreconditioner, collect_pairs, pcg, train_correction
+    from ..unet import UNet
     ctx = setup()
     mm = ctx['mm']
-    # warm-up of every kernel on the path
-    mm.apply_A(mm.zeros())
+    mm.apply_A(mm.zeros())     # warm-up of every kernel on the path
     torch.cuda.synchronize()
     x, info, M = solve_binned(ctx, tol, maxiter)
     out = {'workload': '100 det x 40000 samples, CAR 1 arcmin TQU %dx%d, 1/f noise '
-                       '(fknee 1 Hz, alpha -3)' % ctx['shape'],
-           'binned': {'iters': info['iters'], 'seconds': info['seconds'],
-                      'setup_s': info['setup_s'], 'final_residual': info['residuals'][-1],
-                      'map_rms_error_TQU': map_error(x, ctx['sky'], M.mask),
-                      'ms_per_iter': info['seconds'] / max(info['iters'], 1) * 1e3}}
-    print(json.dumps(out['binned']), file=sys.stderr, flush=True)
+                       '(fknee 1 Hz, alpha -3), float64 time streams in A' % ctx['shape'],
+           'binned': summary(x, info, ctx, M)}
+    out['binned']['setup_s'] = info['setup_s']
+    print('binned', json.dumps(out['binned']), file=sys.stderr, flush=True)
+    if not learned:
+        return out
+    # training pairs from binned PCG directions on independent realisations
+    t0 = time.perf_counter()
+    rhs = [mm.rhs(make_data(ctx, 100 + j)[1]) for j in range(n_rhs)]
+    R, E = collect_pairs(mm, M, rhs, per_rhs)
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    net = UNet(seed=0, factor=factor)
+    t0 = time.perf_counter()
+    hist = train_correction(net, M, R, E, steps=steps, batch=batch, lr=lr,
+                            log=lambda s, l: print(f'step {s}: loss {l:.4f}', file=sys.stderr,
+                                                   flush=True))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del R, E
+    L = LearnedPreconditioner(M, net, mm.P.map_shape)
+    L(mm.zeros() + 1.)       # warm-up
+    b = mm.rhs(ctx['d'])
+    x2, info2 = pcg(mm.apply_A, b, L, tol=tol, maxiter=maxiter, flexible=True, fallback=M)
+    out['learned'] = summary(x2, info2, ctx, M)
+    out['learned'].update({'pairs': int(n_rhs * per_rhs), 'pairs_s': t_pairs,
+                           'pool_factor': list(factor),
+                           'train_steps': steps, 'train_s': t_train,
+                           'loss_first_last': [float(np.mean(hist[:20])),
+                                               float(np.mean(hist[-20:]))]})
+    out['speedup_wall'] = out['binned']['seconds'] / out['learned']['seconds']
+    print('learned', json.dumps(out['learned']), file=sys.stderr, flush=True)
     return out
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--tol', type=float, default=1e-6)
-    ap.add_argument('--maxiter', type=int, default=3000)
+    ap.add_argument('--maxiter', type=int, default=40000)
+    ap.add_argument('--no-learned', action='store_true')
+    ap.add_argument('--steps', type=int, default=3000)
+    ap.add_argument('--per-rhs', type=int, default=500)
+    ap.add_argument('--n-rhs', type=int, default=4)
+    ap.add_argument('--lr', type=float, default=3e-4)
+    ap.add_argument('--factor', type=int, nargs=2, default=(2, 2))
     a = ap.parse_args(argv)
-    print(json.dumps(run(a.tol, a.maxiter)), flush=True)
+    print(json.dumps(run(a.tol, a.maxiter, not a.no_learned, a.n_rhs, a.per_rhs, a.steps,
+                         lr=a.lr, factor=tuple(a.factor))), flush=True)
 
 
 if __name__ == '__main__':

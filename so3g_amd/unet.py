@@ -12,7 +12,10 @@ Architecture (every conv 3x3, ReLU unless noted; '+' adds a skip):
     d1 = conv(conv(up(d2), 32) + e1, 32)
     d0 = conv(conv(up(d1), 16) + e0, 16)
     out = conv(d0, 16)  (no ReLU; the first 3 channels are T, Q, U)
-The 3 input planes are zero-padded to 16 channels and to H, W divisible by 8.
+pool / up change the resolution by `factor` = (rows, columns) per level,
+(2, 2) by default; (1, 4) pools only along map rows, the scan direction.
+The input planes are zero-padded to 16 channels and to H, W divisible by
+factor^3.
 
 No counterpart in the reference (which has no map-maker).
 """
@@ -65,15 +68,16 @@ def program():
 
 
 class UNet:
-    def __init__(self, ch=CH, seed=0, device=None, init_scale=1.0):
+    def __init__(self, ch=CH, seed=0, device=None, init_scale=1.0, factor=(2, 2)):
         self.dev = device or torch.device('cuda')
+        self.factor = tuple(int(f) for f in factor)   # pooling (rows, columns) per level
         self.specs = {s[0]: s[1:] for s in layer_specs(ch)}
         self.prog = program()
         self.lib = N.load()
         g = torch.Generator().manual_seed(seed)
         self.w, self.b = {}, {}
         for name, (ci, co, relu) in self.specs.items():
-            fan_in = 9 * ci if name != 'e0a' else 9 * 3
+            fan_in = 9 * ci
             std = np.sqrt(2. / fan_in) * (init_scale if name != 'out' else 0.1)
             w = torch.randn((9, co, ci), generator=g, dtype=torch.float32) * std
             self.w[name] = w.to(self.dev)
@@ -107,20 +111,22 @@ class UNet:
 
     def _pool(self, x, backward=False):
         n, h, w, c = x.shape
-        shape = (n, 2 * h, 2 * w, c) if backward else (n, h // 2, w // 2, c)
+        fy, fx = self.factor
+        shape = (n, fy * h, fx * w, c) if backward else (n, h // fy, w // fx, c)
         y = torch.empty(shape, dtype=torch.bfloat16, device=self.dev)
-        hh, ww = (2 * h, 2 * w) if backward else (h, w)
-        N.check(self.lib.so3g_nn_pool2(x.data_ptr(), n, hh, ww, c, y.data_ptr(), int(backward),
-                                       _stream()))
+        hh, ww = (fy * h, fx * w) if backward else (h, w)
+        N.check(self.lib.so3g_nn_pool(x.data_ptr(), n, hh, ww, c, fy, fx, y.data_ptr(),
+                                      int(backward), _stream()))
         return y
 
     def _up(self, x, backward=False):
         n, h, w, c = x.shape
-        shape = (n, h // 2, w // 2, c) if backward else (n, 2 * h, 2 * w, c)
+        fy, fx = self.factor
+        shape = (n, h // fy, w // fx, c) if backward else (n, fy * h, fx * w, c)
         y = torch.empty(shape, dtype=torch.bfloat16, device=self.dev)
-        hh, ww = (h // 2, w // 2) if backward else (h, w)
-        N.check(self.lib.so3g_nn_up2(x.data_ptr(), n, hh, ww, c, y.data_ptr(), int(backward),
-                                     _stream()))
+        hh, ww = (h // fy, w // fx) if backward else (h, w)
+        N.check(self.lib.so3g_nn_up(x.data_ptr(), n, hh, ww, c, fy, fx, y.data_ptr(),
+                                    int(backward), _stream()))
         return y
 
     def _add(self, a, b):
@@ -131,7 +137,7 @@ class UNet:
 
     # -- forward / backward --------------------------------------------------
     def forward(self, x):
-        """x: bf16 (n, H, W, 16) with H, W divisible by 8.  Returns (y, tape)."""
+        """x: bf16 (n, H, W, 16) with H, W divisible by factor^3.  Returns (y, tape)."""
         t = {'x': x}
         for op, o, ins, layer in self.prog:
             if op == 'conv':
@@ -195,12 +201,12 @@ class UNet:
                                                  self.wflip[k].data_ptr(), _stream()))
 
     # -- planes <-> activations ---------------------------------------------
-    @staticmethod
-    def padded(h, w):
-        return (h + 7) // 8 * 8, (w + 7) // 8 * 8
+    def padded(self, h, w):
+        my, mx = self.factor[0] ** 3, self.factor[1] ** 3
+        return (h + my - 1) // my * my, (w + mx - 1) // mx * mx
 
     def to_input(self, planes):
-        """float32 (n, 3, hy, wx) -> bf16 (n, H, W, 16)."""
+        """float32 (n, cp <= 16, hy, wx) -> bf16 (n, H, W, 16), zero-padded."""
         n, cp, hy, wx = planes.shape
         H, W = self.padded(hy, wx)
         x = torch.empty((n, H, W, NIN), dtype=torch.bfloat16, device=self.dev)

@@ -123,6 +123,14 @@ def test_pool_up_relu_add(gpu):
     np.testing.assert_array_equal(np64(net._up(ad)), R.up(a))
     np.testing.assert_array_equal(np64(net._up(net._up(ad), backward=True)), R.up_bwd(R.up(a)))
     np.testing.assert_array_equal(np64(net._add(ad, bd)), R.bf16(a + b))
+    net.factor = (1, 4)
+    f = (1, 4)
+    np.testing.assert_array_equal(np64(net._pool(ad)), R.pool(a, f))
+    np.testing.assert_array_equal(np64(net._pool(net._pool(ad), backward=True)),
+                                  R.pool_bwd(R.pool(a, f), f))
+    np.testing.assert_array_equal(np64(net._up(ad)), R.up(a, f))
+    np.testing.assert_array_equal(np64(net._up(net._up(ad), backward=True)),
+                                  R.up_bwd(R.up(a, f), f))
 
 
 def rel_err(g, r):
@@ -130,26 +138,27 @@ def rel_err(g, r):
 
 
 @pytest.mark.gpu
-def test_unet_gradients_against_cpu(gpu):
+@pytest.mark.parametrize('factor,h,w', [((2, 2), 16, 40), ((1, 4), 6, 128)])
+def test_unet_gradients_against_cpu(gpu, factor, h, w):
     """Full forward/backward of the 4-level network against the numpy
     restatement with the same bf16 rounding points."""
     from so3g_amd.unet import UNet, layer_specs, program
-    net = UNet(device=dev(), seed=2)
+    net = UNet(device=dev(), seed=2, factor=factor)
     rng = np.random.default_rng(5)
-    n, h, w = 2, 16, 40
+    n = 2
     planes = rng.standard_normal((n, 3, h - 2, w - 3)).astype(np.float32)
     x = net.to_input(torch.from_numpy(planes).to(dev()))
     y, tape = net.forward(x)
     weights = {k: (net.w[k].cpu().numpy().astype(np.float64), net.b[k].cpu().numpy())
                for k in net.specs}
-    xr = np.zeros((n, h, w, 16))
+    xr = np.zeros((n,) + net.padded(h - 2, w - 3) + (16,))
     xr[:, :h - 2, :w - 3, :3] = planes.transpose(0, 2, 3, 1)
     specs = {s[0]: s[1:] for s in layer_specs()}
-    t = R.forward(program(), specs, weights, xr)
+    t = R.forward(program(), specs, weights, xr, factor)
     assert rel_err(np64(y), t['y']) < 2e-2
     gy = R.bf16(rng.standard_normal(t['y'].shape))
     dw, db = net.backward(tape, bf(gy))
-    rdw, rdb = R.backward(program(), specs, weights, t, gy)
+    rdw, rdb = R.backward(program(), specs, weights, t, gy, factor)
     for k in specs:
         e = rel_err(dw[k].cpu().numpy(), rdw[k])
         assert e < 3e-2, (k, e)
