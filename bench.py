@@ -6,9 +6,11 @@ elevation scan, el 50 deg, az +-10 deg at 1 deg/s, 200 Hz, detectors in a
 One "step" = from_map (map -> fresh float32 signal) + to_map (signal ->
 float64 TQU map, zeroed first) over the whole observation, with every input
 resident in HBM.  The headline path projects through the cached pointing
-matrix (get_pointing_matrix once, then ProjEng_Precomp from_map / to_map,
-the operator pair of every CG iteration); the on-the-fly path (pointing
-recomputed in both kernels, the Projectionist's default) is timed beside it.
+matrix: get_pointing_matrix once, a PrecompPlan built from it once, then the
+plan's from_map / to_map -- the operator pair of every CG iteration, with
+the results of ProjEng_Precomp's from_map / to_map.  The plain
+ProjEng_Precomp kernels and the on-the-fly path (pointing recomputed in both
+kernels, the Projectionist's default) are timed beside it.
 
 Prints ONE JSON line.  `python bench.py [--steps K] [--warmup W] [--gpus 1]`.
 
@@ -130,6 +132,9 @@ def main():
     mapo = torch.zeros((3, ny, nx), dtype=torch.float64, device=dev)
 
     def pc_from():
+        chk(lib.so3g_pc_plan_from_map(plan._h, sky.data_ptr(), sig_out.data_ptr(), 0, stream))
+
+    def pc_from_direct():
         chk(lib.so3g_precomp_from_map(sky.data_ptr(), shape.ctypes.data, None, None,
                                       pix.data_ptr(), 2, spin.data_ptr(), 3, n_det, n_t,
                                       sig_out.data_ptr(), 0, stream))
@@ -193,20 +198,34 @@ def main():
     ms_step = wall / args.steps * 1e3
     value = N_s / (wall / args.steps)
 
-    # Algorithmic bytes per launch (pixel indices k=2, spin c=3, float32
-    # signal, float64 TQU map of Npix pixels):
+    # Algorithmic bytes per launch of the planned kernels: what each must
+    # move at least from its own storage (see the plan's layout):
+    #   from_map: per sample 4 B flat pixel + 8 B (Q, U) factors + 4 B signal
+    #             written (the T factor is one float per detector), plus the
+    #             float64 TQU map read once;
+    #   to_map:   per on-map sample 4 B sample index + 8 B (Q, U) factors +
+    #             4 B signal gathered + 1 bit run start, plus 4 B per pixel run
+    #             and per 512-sample group, plus the map read and written once.
+    # The ProjEng_Precomp interface itself streams 24 B per sample (int32
+    # pixel pair, 3 float32 factors, float32 signal): `precomp_equivalent`
+    # reports the rates in those bytes.
     npix = ny * nx
-    b_from = N_s * (4 * 2 + 4 * 3 + 4) + 8 * 3 * npix
-    b_to = N_s * (4 * 2 + 4 * 3 + 4) + 2 * 8 * 3 * npix
+    pinfo = plan.info()
+    n_on, n_runs, ns = pinfo['n_on'], pinfo['runs'], pinfo['stored_spin']
+    b_from = N_s * (4 + 4 * ns + 4) + 8 * 3 * npix
+    b_to = n_on * (4 + 4 * ns + 4) + n_on // 8 + 4 * n_runs + 4 * (n_on // 512) + \
+        2 * 8 * 3 * npix
+    b_pc = N_s * (4 * 2 + 4 * 3 + 4)
     gbs_from = b_from / (ms_from * 1e-3)
     gbs_to = b_to / (ms_to * 1e-3)
     tr = measured_traffic() or {}
     if ms_to >= ms_from:
-        dom = ('k_plan_to_map_r<3,8>', gbs_to, b_to, tr.get('to_map_bytes'))
+        dom = ('k_plan_to_map_c<3,true,float,1>', gbs_to, b_to, tr.get('to_map_bytes'))
     else:
-        dom = ('k_pc_from_map_packed<3>', gbs_from, b_from, tr.get('from_map_bytes'))
-    # The plain ProjEng_Precomp scatter (no plan), for reference.
-    _, (ms_to_direct,) = run([pc_to_direct], max(1, min(args.steps, 3)), 1)
+        dom = ('k_plan_from_map<3,true,4,1>', gbs_from, b_from, tr.get('from_map_bytes'))
+    # The plain ProjEng_Precomp kernels (no plan), for reference.
+    _, (ms_from_direct, ms_to_direct) = run([pc_from_direct, pc_to_direct],
+                                            max(1, min(args.steps, 3)), 1)
 
     otf = None
     if not args.no_otf:
@@ -240,20 +259,27 @@ def main():
         'data': 'synthetic (seeded scan, focal plane, random TQU sky + white noise)',
         'config': {
             'workload': f'single observation {n_det} det x {n_t} samples, CAR 0.5 arcmin '
-                        f'TQU {ny}x{nx} non-tiled; step = from_map + to_map via cached '
-                        f'pointing matrix (ProjEng_Precomp from_map, PrecompPlan to_map)',
+                        f'TQU {ny}x{nx} non-tiled; step = from_map + to_map through the '
+                        f'cached pointing matrix (PrecompPlan from_map and to_map)',
             'n_det': n_det, 'n_samp': n_t, 'map_shape': [3, ny, nx],
-            'pointing': 'pointing_matrix and pixel-sorted plan computed once before '
+            'pointing': 'pointing_matrix and its plan computed once before '
                         f'timing ({t_pmat * 1e3:.1f} ms + {t_plan * 1e3:.1f} ms)',
         },
         'ms_from_map': ms_from,
         'ms_to_map': ms_to,
+        'ms_from_map_without_plan': ms_from_direct,
         'ms_to_map_without_plan': ms_to_direct,
+        'plan': {'n_on': n_on, 'runs': n_runs, 'stored_spin_per_sample': ns,
+                 't_block': pinfo['t_block'], 'bytes': pinfo['bytes']},
         'roofline': {
             'bound': 'hbm', 'kernel': dom[0], 'achieved': dom[1] / 1e9,
             'peak': HBM_PEAK / 1e9, 'unit': 'GB/s', 'frac': dom[1] / HBM_PEAK,
             'traffic': dom[3], 'algorithmic_bytes': dom[2],
             'from_map_frac': gbs_from / HBM_PEAK, 'to_map_frac': gbs_to / HBM_PEAK,
+            'precomp_equivalent': {
+                'bytes_per_launch': b_pc + 8 * 3 * npix,
+                'from_map_frac': (b_pc + 8 * 3 * npix) / (ms_from * 1e-3) / HBM_PEAK,
+                'to_map_frac': (b_pc + 16 * 3 * npix) / (ms_to * 1e-3) / HBM_PEAK},
         },
         'on_the_fly': otf,
         'cpu_baseline': cpu,

@@ -205,17 +205,23 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape,
                             const float *det_weights, const int32_t *segs,
                             int64_t n_seg, void *stream);
 
-/* Projection plan for repeated cached-pointing to_map calls with one
+/* Projection plan for repeated cached-pointing projections with one
  * pointing matrix (no counterpart in the reference; for CG-style loops).
  * Built once from a Precomp pixel array (k = 2, non-tiled flat map of
- * map_shape = (ny, nx), host array) and spin array; holds the on-map samples
- * sorted by pixel and the spin factors in that order (12 + 4*nc bytes per
- * on-map sample of device memory).  Samples are grouped in time blocks
- * whose signal slice is about t_block_mb MiB (sized to stay in the
- * Infinity Cache; <= 0: one block), then sorted by pixel within each block.
- * so3g_pc_plan_to_map(plan, ...) adds the same contributions as
- * so3g_precomp_to_map over every sample. */
+ * map_shape = (ny, nx), host array) and spin array; the arrays are copied,
+ * so they may be freed afterwards.  Samples are grouped in time blocks whose
+ * signal slice is about t_block_mb MiB (sized to stay in the Infinity Cache;
+ * <= 0: one block), then sorted by pixel within each block.  Stored: per
+ * on-map sample the sample index and the spin factors (the T factor once per
+ * detector when it is constant over the detector, as in so3g's pointing
+ * matrices), run-start bits and run pixels; per sample a time-ordered flat
+ * pixel and the same spin factors.  so3g_pc_plan_to_map adds the same
+ * contributions as so3g_precomp_to_map over every sample (float64 sums in
+ * another order); so3g_pc_plan_from_map writes what so3g_precomp_from_map
+ * writes, bit for bit. */
 typedef struct so3g_pc_plan so3g_pc_plan;
+int so3g_pc_plan_from_map(const so3g_pc_plan *plan, const double *map, float *signal,
+                          int accumulate, void *stream);
 int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int32_t nc,
                         int64_t n_det, int64_t n_t, const int64_t *map_shape,
                         int64_t t_block_mb, so3g_pc_plan **out, void *stream);
@@ -233,6 +239,9 @@ int so3g_precomp_from_map_f64(const double *map, const int64_t *map_shape,
                               int64_t n_det, int64_t n_t, double *signal, int accumulate,
                               void *stream);
 int so3g_pc_plan_info(const so3g_pc_plan *plan, int64_t *n_on, int64_t *bytes);
+/* stats: n_on, pixel runs, T factor constant per detector (0/1), stored spin
+ * factors per sample, samples per time block, device bytes */
+int so3g_pc_plan_stats(const so3g_pc_plan *plan, int64_t *stats);
 int so3g_pc_plan_destroy(so3g_pc_plan *plan);
 
 /* Batched projection of many observations into one map (no counterpart in

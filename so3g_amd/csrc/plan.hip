@@ -1,26 +1,42 @@
-// plan.hip -- pixel-sorted projection plans for repeated to_map calls with a
-// fixed pointing matrix (the CG hot loop).
+// plan.hip -- pixel-sorted projection plans for repeated cached-pointing
+// projections with a fixed pointing matrix (the CG hot loop).
 //
 // ProjEng_Precomp's to_map scatters every sample's float32 contributions into
 // the float64 map with atomics.  Over a whole observation each map pixel is
 // hit by hundreds of detector passes, but passes that share a pixel are far
 // apart in time, so no time-local block sees much of that reuse.  A plan
-// sorts the on-map samples by pixel once (rocPRIM radix sort, stable, so each
-// detector pass stays contiguous) and stores the spin factors in that order.
-// to_map then streams (pixel, sample index, spin) in sorted order, gathers
-// the signal through the sample index, and sums each pixel's contributions
-// with a wave-level segmented scan: one atomic per pixel per wave instead of
-// one per sample.
+// sorts the on-map samples by (time block, pixel) once (rocPRIM radix sort,
+// stable, so each detector pass stays contiguous) and to_map then streams
+// the samples in that order, gathers the signal through the stored sample
+// index, and sums each run of equal pixels with a wave-level segmented scan:
+// one atomic per pixel per wave instead of one per sample.  Time blocks keep
+// the signal slice a block gathers from (t_block_mb) inside the Infinity
+// Cache.
 //
-// Arithmetic per sample is the Precomp one, (det_wt * sig) * pf in float32
-// (Projection.cxx:2117), summed in float64.
+// Compact storage (bytes per on-map sample in the sorted stream):
+//   idx    4     sample index d * n_t + t
+//   spin   4 ns  spin factors in sorted order; the T factor is not stored
+//                when it is constant over each detector's samples (so3g's
+//                pointing matrices: T = the detector's T response), ns = nc - 1
+//   heads  1/8   bit per sample: a new pixel run starts here
+//   runs         pixel of each run (one per distinct (time block, pixel))
+//                and the run count before each 512-sample group
+// so that a sample's pixel is run_pix[run index], the run index being a
+// popcount over the head bits.  from_map reads a time-ordered copy: flat
+// pixel (4 B, -1 off the map) and the ns stored spin factors.
+//
+// Arithmetic per sample is the Precomp one (Projection.cxx:2100-2121 for
+// to_map, (det_wt * sig) * pf in float32 summed in float64; :2060-2085 for
+// from_map, sig += pf * map per component into a float32 partial sum).
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "../../include/so3g_hip.h"
 #include "kernels.h"
@@ -32,12 +48,19 @@ using namespace so3g;
 
 struct so3g_pc_plan {
     int64_t n_det, n_t, ny, nx;
-    int32_t nc;
+    int32_t nc, ns, tconst;
     int64_t n_on;          // on-map samples
+    int64_t n_runs;        // pixel runs in sorted order
     int64_t t_block;       // samples per time block
-    int32_t *key;          // [n_on] tb * npix + iy * nx + ix, ascending
-    uint32_t *idx;         // [n_on] sample index d * n_t + t
-    float *spin;           // [n_on * nc] spin factors in sorted order
+    uint32_t *idx;         // [n_on] sample index, sorted by (time block, pixel)
+    float *sspin;          // [n_on * ns] stored spin factors, sorted order
+    float *tval;           // [n_det] T factor per detector (tconst)
+    uint8_t *heads;        // [n_groups * 64] run-start bits
+    uint32_t *wbase;       // [n_groups] runs before each 512-sample group
+    uint32_t *run_pix;     // [n_runs] flat pixel iy * nx + ix of each run
+    int32_t *tpix;         // [n] time-ordered flat pixel, -1 off the map
+    float *tspin;          // [n * ns] time-ordered stored spin factors
+    double *packed;        // [ny * nx * 4] pixel-major map copy for from_map
     int64_t bytes;
 };
 
@@ -52,6 +75,14 @@ namespace so3g {
     } while (0)
 
 constexpr int kPlanBlock = 256;
+
+// Tunables (so3g_set_option "plan_div", "plan_from_v"): detector index by
+// float reciprocal (1) or integer division (0); samples per lane of the
+// planned from_map (4 or 8, when the sample count allows).
+int g_plan_div = 1;
+int g_plan_from_v = 4;
+constexpr int kPlanR = 8;                 // sorted samples per lane
+constexpr int kGroup = 64 * kPlanR;       // samples per wave = per head group
 
 // Sort keys: (time block, row-major pixel) as tb * npix + pixel, or the
 // largest key for an off-map sample (sorted last).
@@ -79,118 +110,213 @@ __global__ void k_plan_count(const int32_t *keys, int64_t n, int32_t off_key,
     *n_on = (unsigned long long)lo;
 }
 
-template <int NC>
-__global__ __launch_bounds__(kPlanBlock) void k_plan_spin(const float *spin, const uint32_t *idx,
-                                                          int64_t n_on, float *out) {
-    const int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
-    if (i >= n_on) return;
-    const int64_t s = idx[i];
-#pragma unroll
-    for (int c = 0; c < NC; c++) out[i * NC + c] = spin[s * NC + c];
+// bad[d] = 1 if the T factor (component 0) of detector d is not the same
+// float (bit for bit) over all its samples.
+__global__ __launch_bounds__(kPlanBlock) void k_plan_tcheck(const float *spin, int nc,
+                                                            int64_t n_t, int64_t n,
+                                                            int *bad) {
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kPlanBlock) {
+        const int64_t d = i / n_t;
+        if (__float_as_uint(spin[i * nc]) != __float_as_uint(spin[d * n_t * nc])) bad[d] = 1;
+    }
 }
 
-// Planned to_map.  Every lane of the grid handles one sorted sample (lanes
-// past the end add nothing but still take part in the wave scan).
-template <int NC>
-__global__ __launch_bounds__(kPlanBlock) void k_plan_to_map(const int32_t *key,
-                                                            const uint32_t *idx,
-                                                            const float *spin, int64_t n_on,
-                                                            uint32_t n_t, uint32_t plane,
-                                                            const float *sig,
-                                                            const float *det_weights,
-                                                            double *map) {
-    const int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
-    int64_t k = -1;
-    double v[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) v[c] = 0.;
-    if (i < n_on) {
-        k = key[i];
-        const uint32_t s = idx[i];
-        float w = 1.f;
-        if (det_weights != nullptr) w = det_weights[s / n_t];
-        if (w != 0.f) {  // a zero weight skips the detector (Projection.cxx:2104)
-            const float sg = sig[s];
-#pragma unroll
-            for (int c = 0; c < NC; c++) v[c] = (double)(w * sg * spin[i * NC + c]);
+__global__ void k_plan_tval(const float *spin, int nc, int64_t n_t, int64_t n_det,
+                            float *tval) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < n_det) tval[d] = spin[d * n_t * nc];
+}
+
+// Stored spin factors (components c0.. of nc) in sorted order.
+__global__ __launch_bounds__(kPlanBlock) void k_plan_sspin(const float *spin, int nc, int c0,
+                                                           const uint32_t *idx, int64_t n_on,
+                                                           float *out) {
+    const int ns = nc - c0;
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n_on;
+         i += (int64_t)gridDim.x * kPlanBlock) {
+        const int64_t s = idx[i];
+        for (int c = 0; c < ns; c++) out[i * ns + c] = spin[s * nc + c0 + c];
+    }
+}
+
+// Time-ordered copy: flat pixel (or -1) and the stored spin factors.
+__global__ __launch_bounds__(kPlanBlock) void k_plan_tcopy(const int32_t *pix,
+                                                           const float *spin, int nc, int c0,
+                                                           int64_t n, int64_t ny, int64_t nx,
+                                                           int32_t *tpix, float *tspin) {
+    const int ns = nc - c0;
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kPlanBlock) {
+        const int2 p = *reinterpret_cast<const int2 *>(pix + 2 * i);
+        const bool on = p.x >= 0 && p.x < ny && p.y >= 0 && p.y < nx;
+        tpix[i] = on ? (int32_t)(p.x * nx + p.y) : -1;
+        for (int c = 0; c < ns; c++) tspin[i * ns + c] = spin[i * nc + c0 + c];
+    }
+}
+
+// Head bits of the sorted keys, one byte per 8 samples (bit r: sample
+// 8 j + r starts a run), and the head count of each 512-sample group.  One
+// thread per byte; a wave covers one group.  Grid covers n_groups * 64
+// threads exactly.
+__global__ __launch_bounds__(kPlanBlock) void k_plan_heads(const int32_t *key, int64_t n_on,
+                                                           int64_t n_bytes, uint8_t *heads,
+                                                           uint32_t *gcount) {
+    const int64_t j = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
+    if (j >= n_bytes) return;   // whole waves: n_bytes is a multiple of 64
+    const int64_t s0 = j * 8;
+    uint32_t bits = 0;
+    if (s0 < n_on) {
+        int32_t prev = s0 > 0 ? key[s0 - 1] : -1;
+        for (int r = 0; r < 8 && s0 + r < n_on; r++) {
+            const int32_t k = key[s0 + r];
+            if (s0 + r == 0 || k != prev) bits |= 1u << r;
+            prev = k;
         }
     }
-    if (wave_runs<NC>(k, v) && k >= 0) {
-        const uint32_t px = (uint32_t)k % plane;
+    heads[j] = (uint8_t)bits;
+    uint32_t c = __popc(bits);
 #pragma unroll
-        for (int c = 0; c < NC; c++) unsafeAtomicAdd(map + (int64_t)c * plane + px, v[c]);
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
+    if ((threadIdx.x & 63) == 0) gcount[j / 64] = c;
+}
+
+// x / d for x < 2^32, d > 0, via a float reciprocal and one correction
+// step: while x / d < 2^22 the float estimate (relative error below 2e-7)
+// is off by less than 1, so one step fixes it.  Plans hold fewer than 2^22
+// detectors.
+struct FastDiv {
+    uint32_t d;
+    float inv;
+};
+
+template <int DIV>
+__device__ __forceinline__ uint32_t fast_div(uint32_t x, const FastDiv &f) {
+    if constexpr (DIV == 0) {
+        return x / f.d;
+    } else {
+        uint32_t q = (uint32_t)((float)x * f.inv);
+        const int32_t r = (int32_t)(x - q * f.d);   // |r| < 2 d < 2^31
+        if (r < 0) q--;
+        else if (r >= (int32_t)f.d) q++;
+        return q;
     }
 }
 
-// Planned to_map, R sorted samples per lane.  Each lane sums its samples
-// segment by segment (keys are sorted, so a segment is a run of equal keys):
-// its first segment may continue from earlier lanes and goes through the
-// wave-level run aggregation; segments that start and end inside the lane,
-// and the lane's last segment when the lane holds more than one, are added
-// to the map directly.
-template <int NC, int R, typename ST = float>
-__global__ __launch_bounds__(kPlanBlock) void k_plan_to_map_r(const int32_t *key,
-                                                              const uint32_t *idx,
-                                                              const float *spin, int64_t n_on,
-                                                              uint32_t n_t, uint32_t plane,
-                                                              const ST *sig,
-                                                              const float *det_weights,
-                                                              double *map) {
-    static_assert(R % 4 == 0 && (R * NC) % 4 == 0, "vector loads");
+// Exclusive prefix of v over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
+    const int lane = (int)__lane_id();
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    return x - v;
+}
+
+// run_pix[run] = pixel of the run's first sample.
+__global__ __launch_bounds__(kPlanBlock) void k_plan_runs(const int32_t *key, int64_t n_bytes,
+                                                          int32_t npix, const uint8_t *heads,
+                                                          const uint32_t *wbase,
+                                                          uint32_t *run_pix) {
+    const int64_t j = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
+    if (j >= n_bytes) return;   // whole waves: n_bytes is a multiple of 64
+    const uint32_t bits = heads[j];
+    const uint32_t e = wave_excl_scan(__popc(bits));
+    const uint32_t base = wbase[j / 64] + e;
+    uint32_t k = 0;
+    for (int r = 0; r < 8; r++) {
+        if (bits & (1u << r)) {
+            run_pix[base + k] = (uint32_t)(key[j * 8 + r] % npix);
+            k++;
+        }
+    }
+}
+
+// Planned to_map over the compact sorted stream, kPlanR samples per lane.
+// Each lane sums its samples run by run: its first run may continue from
+// earlier lanes and goes through the wave-level run aggregation; runs that
+// start inside the lane and end inside it, and its last run when it holds
+// more than one, are added to the map directly.
+template <int NC, bool TC, typename ST, int DIV>
+__global__ __launch_bounds__(kPlanBlock) void k_plan_to_map_c(
+    const uint32_t *__restrict__ idx, const float *__restrict__ sspin,
+    const float *__restrict__ tval, const uint8_t *__restrict__ heads,
+    const uint32_t *__restrict__ wbase, const uint32_t *__restrict__ run_pix, int64_t n_on,
+    FastDiv n_t, uint32_t plane, const ST *__restrict__ sig,
+    const float *__restrict__ det_weights, double *map) {
+    constexpr int NS = TC ? NC - 1 : NC;
+    constexpr int R = kPlanR;
     const int64_t base = ((int64_t)blockIdx.x * kPlanBlock + threadIdx.x) * R;
-    int32_t kk[R];
-    uint32_t ii[R];
-    float sp[R * NC];
     const int nr = (int)max((int64_t)0, min((int64_t)R, n_on - base));
+    uint32_t ii[R];
+    float sp[R * (NS > 0 ? NS : 1)];
     if (nr == R) {
 #pragma unroll
         for (int q = 0; q < R / 4; q++) {
-            const int4 a = reinterpret_cast<const int4 *>(key + base)[q];
-            kk[4 * q] = a.x; kk[4 * q + 1] = a.y; kk[4 * q + 2] = a.z; kk[4 * q + 3] = a.w;
             const uint4 b = reinterpret_cast<const uint4 *>(idx + base)[q];
             ii[4 * q] = b.x; ii[4 * q + 1] = b.y; ii[4 * q + 2] = b.z; ii[4 * q + 3] = b.w;
         }
+        if constexpr (NS > 0) {
 #pragma unroll
-        for (int q = 0; q < R * NC / 4; q++) {
-            const float4 f = reinterpret_cast<const float4 *>(spin + base * NC)[q];
-            sp[4 * q] = f.x; sp[4 * q + 1] = f.y; sp[4 * q + 2] = f.z; sp[4 * q + 3] = f.w;
+            for (int q = 0; q < R * NS / 4; q++) {
+                const float4 f = reinterpret_cast<const float4 *>(sspin + base * NS)[q];
+                sp[4 * q] = f.x; sp[4 * q + 1] = f.y; sp[4 * q + 2] = f.z; sp[4 * q + 3] = f.w;
+            }
         }
     } else {
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            kk[r] = r < nr ? key[base + r] : -1;
             ii[r] = r < nr ? idx[base + r] : 0u;
 #pragma unroll
-            for (int c = 0; c < NC; c++) sp[r * NC + c] = r < nr ? spin[(base + r) * NC + c] : 0.f;
+            for (int c = 0; c < NS; c++) sp[r * NS + c] = r < nr ? sspin[(base + r) * NS + c] : 0.f;
         }
     }
+    // Run index of each sample: the runs before the wave's group, the heads
+    // of the earlier lanes of the wave, and the heads up to it in this lane.
+    const uint32_t hb = nr > 0 ? heads[base / 8] : 0u;
+    const uint32_t wb = nr > 0 ? wbase[base / kGroup] : 0u;
+    const uint32_t lane_base = wb + wave_excl_scan(__popc(hb));
     int64_t hk = -1, ck = -1;
+    uint32_t cpix = 0;
     double hv[NC], cv[NC];
-    bool closed = false;   // has the lane's first segment ended?
+    bool closed = false;   // has the lane's first run ended?
 #pragma unroll
     for (int c = 0; c < NC; c++) { hv[c] = 0.; cv[c] = 0.; }
 #pragma unroll
     for (int r = 0; r < R; r++) {
         if (r >= nr) break;
+        const int64_t rid = (int64_t)lane_base + __popc(hb & ((2u << r) - 1u)) - 1;
         const uint32_t si = ii[r];
+        uint32_t det = 0;
+        if (TC || det_weights != nullptr) det = fast_div<DIV>(si, n_t);
+        float pf[NC];
+        if constexpr (TC) {
+            pf[0] = tval[det];
+#pragma unroll
+            for (int c = 1; c < NC; c++) pf[c] = sp[r * NS + c - 1];
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; c++) pf[c] = sp[r * NS + c];
+        }
         float w = 1.f;
-        if (det_weights != nullptr) w = det_weights[si / n_t];
+        if (det_weights != nullptr) w = det_weights[det];
         double x[NC];
         if (w != 0.f) {  // a zero weight skips the detector (Projection.cxx:2104)
             const ST sg = sig[si];
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 if constexpr (std::is_same<ST, double>::value)
-                    x[c] = (double)w * sg * (double)sp[r * NC + c];
+                    x[c] = (double)w * sg * (double)pf[c];
                 else
-                    x[c] = (double)(w * sg * sp[r * NC + c]);
+                    x[c] = (double)(w * sg * pf[c]);
             }
         } else {
 #pragma unroll
             for (int c = 0; c < NC; c++) x[c] = 0.;
         }
-        const int64_t k = kk[r];
-        if (k != ck) {
+        if (rid != ck) {
             if (ck >= 0) {
                 if (!closed) {
                     hk = ck;
@@ -198,13 +324,13 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_to_map_r(const int32_t *key
                     for (int c = 0; c < NC; c++) hv[c] = cv[c];
                     closed = true;
                 } else {
-                    const uint32_t px = (uint32_t)ck % plane;
 #pragma unroll
                     for (int c = 0; c < NC; c++)
-                        unsafeAtomicAdd(map + (int64_t)c * plane + px, cv[c]);
+                        unsafeAtomicAdd(map + (int64_t)c * plane + cpix, cv[c]);
                 }
             }
-            ck = k;
+            ck = rid;
+            cpix = run_pix[rid];
 #pragma unroll
             for (int c = 0; c < NC; c++) cv[c] = x[c];
         } else {
@@ -212,129 +338,168 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_to_map_r(const int32_t *key
             for (int c = 0; c < NC; c++) cv[c] += x[c];
         }
     }
+    uint32_t hpix = cpix;
     if (!closed) {
         hk = ck;
 #pragma unroll
         for (int c = 0; c < NC; c++) hv[c] = cv[c];
     } else if (ck >= 0) {
-        const uint32_t px = (uint32_t)ck % plane;
 #pragma unroll
-        for (int c = 0; c < NC; c++) unsafeAtomicAdd(map + (int64_t)c * plane + px, cv[c]);
+        for (int c = 0; c < NC; c++) unsafeAtomicAdd(map + (int64_t)c * plane + cpix, cv[c]);
+        hpix = hk >= 0 ? run_pix[hk] : 0u;
     }
     if (wave_runs<NC>(hk, hv) && hk >= 0) {
-        const uint32_t px = (uint32_t)hk % plane;
 #pragma unroll
-        for (int c = 0; c < NC; c++) unsafeAtomicAdd(map + (int64_t)c * plane + px, hv[c]);
+        for (int c = 0; c < NC; c++) unsafeAtomicAdd(map + (int64_t)c * plane + hpix, hv[c]);
     }
 }
 
-constexpr int kPlanR = 8;
+// Pixel-major copy of an (NC, plane) map, 4 doubles per pixel.
+template <int NC>
+__global__ __launch_bounds__(kPlanBlock) void k_plan_pack(const double *map, int64_t plane,
+                                                          double *packed) {
+    for (int64_t p = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; p < plane;
+         p += (int64_t)gridDim.x * kPlanBlock) {
+        double v[4] = {0., 0., 0., 0.};
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] = map[c * plane + p];
+        reinterpret_cast<double2 *>(packed + 4 * p)[0] = double2{v[0], v[1]};
+        reinterpret_cast<double2 *>(packed + 4 * p)[1] = double2{v[2], v[3]};
+    }
+}
+
+// Planned from_map over the time-ordered copy: V consecutive samples per
+// lane (V = 8 with vector loads when n % 8 == 0, else V = 1).  sr = sum_c
+// pf_c * map_c accumulated into a float (each addition rounded to float),
+// then sig = accumulate ? sig + sr : sr, as ProjEng_Precomp's from_map does;
+// off-map samples keep sig (or 0).
+template <int NC, bool TC, int V, int DIV>
+__global__ __launch_bounds__(kPlanBlock) void k_plan_from_map(
+    const int32_t *__restrict__ tpix, const float *__restrict__ tspin,
+    const float *__restrict__ tval, const double *__restrict__ packed, int64_t n,
+    FastDiv n_t, float *__restrict__ sig, int accumulate) {
+    constexpr int NS = TC ? NC - 1 : NC;
+    for (int64_t i0 = ((int64_t)blockIdx.x * kPlanBlock + threadIdx.x) * V; i0 < n;
+         i0 += (int64_t)gridDim.x * kPlanBlock * V) {
+        int32_t px[V];
+        float sp[V * (NS > 0 ? NS : 1)];
+        float o[V];
+        if constexpr (V > 1) {
+#pragma unroll
+            for (int q = 0; q < V / 4; q++) {
+                const int4 a = reinterpret_cast<const int4 *>(tpix + i0)[q];
+                px[4 * q] = a.x; px[4 * q + 1] = a.y; px[4 * q + 2] = a.z; px[4 * q + 3] = a.w;
+            }
+            if constexpr (NS > 0) {
+#pragma unroll
+                for (int q = 0; q < V * NS / 4; q++) {
+                    const float4 f = reinterpret_cast<const float4 *>(tspin + i0 * NS)[q];
+                    sp[4 * q] = f.x; sp[4 * q + 1] = f.y; sp[4 * q + 2] = f.z; sp[4 * q + 3] = f.w;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < V / 4; q++) {
+                const float4 s = accumulate ? reinterpret_cast<const float4 *>(sig + i0)[q]
+                                            : float4{0.f, 0.f, 0.f, 0.f};
+                o[4 * q] = s.x; o[4 * q + 1] = s.y; o[4 * q + 2] = s.z; o[4 * q + 3] = s.w;
+            }
+        } else {
+            px[0] = tpix[i0];
+#pragma unroll
+            for (int c = 0; c < NS; c++) sp[c] = tspin[i0 * NS + c];
+            o[0] = accumulate ? sig[i0] : 0.f;
+        }
+        // Detector of the first sample; a later sample of the lane belongs to
+        // the next detector once it passes the row end (V <= n_t is not
+        // assumed: the step is recomputed per sample from t0 + r).
+        uint32_t d0 = 0, t0 = 0;
+        if constexpr (TC) {
+            d0 = fast_div<DIV>((uint32_t)i0, n_t);
+            t0 = (uint32_t)i0 - d0 * n_t.d;
+        }
+        double mv[V][4];
+#pragma unroll
+        for (int r = 0; r < V; r++) {
+            const int64_t p = px[r] >= 0 ? px[r] : 0;
+            const double2 lo = reinterpret_cast<const double2 *>(packed + 4 * p)[0];
+            mv[r][0] = lo.x; mv[r][1] = lo.y;
+            if (NC > 2) {
+                const double2 hi = reinterpret_cast<const double2 *>(packed + 4 * p)[1];
+                mv[r][2] = hi.x; mv[r][3] = hi.y;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < V; r++) {
+            if (px[r] < 0) continue;
+            float pf[NC];
+            if constexpr (TC) {
+                const uint32_t k = t0 + r;
+                pf[0] = tval[d0 + (k < n_t.d ? 0u : k / n_t.d)];
+#pragma unroll
+                for (int c = 1; c < NC; c++) pf[c] = sp[r * NS + c - 1];
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; c++) pf[c] = sp[r * NS + c];
+            }
+            float sr = 0;
+#pragma unroll
+            for (int c = 0; c < NC; c++) sr = sr + pf[c] * mv[r][c];
+            o[r] = accumulate ? o[r] + sr : sr;
+        }
+        if constexpr (V > 1) {
+#pragma unroll
+            for (int q = 0; q < V / 4; q++)
+                reinterpret_cast<float4 *>(sig + i0)[q] =
+                    float4{o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]};
+        } else {
+            sig[i0] = o[0];
+        }
+    }
+}
 
 }  // namespace so3g
 
-extern "C" {
+namespace {
 
-int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int32_t nc,
-                        int64_t n_det, int64_t n_t, const int64_t *map_shape,
-                        int64_t t_block_mb, so3g_pc_plan **out, void *stream) {
-    *out = nullptr;
-    if (k != 2) return set_error(SO3G_ERR_RUNTIME, "plans support non-tiled flat maps (k = 2)");
-    if (nc < 1 || nc > 3) return set_error(SO3G_ERR_RUNTIME, "spin component count must be 1..3");
-    const int64_t n = n_det * n_t;
-    if (n <= 0 || n > (int64_t)UINT32_MAX)
-        return set_error(SO3G_ERR_RUNTIME, "plan needs 0 < n_det * n_t < 2^32");
-    const int64_t ny = map_shape[0], nx = map_shape[1];
-    if (ny * nx >= INT32_MAX) return set_error(SO3G_ERR_RUNTIME, "map too large for a plan");
-    hipStream_t s = (hipStream_t)stream;
-    const int32_t npix = (int32_t)(ny * nx);
-    // Time blocks whose signal slice (n_det x t_block float32) stays resident
-    // in the Infinity Cache while the block's samples are gathered.
-    int64_t t_block = std::max<int64_t>(1, (t_block_mb << 20) / (4 * n_det));
-    if (t_block_mb <= 0 || t_block > n_t) t_block = n_t;
-    int64_t n_tb = (n_t + t_block - 1) / t_block;
-    while (n_tb * npix >= INT32_MAX) {  // keys must fit int32
-        t_block *= 2;
-        n_tb = (n_t + t_block - 1) / t_block;
-    }
-    const int32_t off_key = (int32_t)(n_tb * npix);
-    int end_bit = 1;
-    while ((1LL << end_bit) <= off_key) end_bit++;
-
-    int32_t *keys_in = nullptr, *keys_out = nullptr;
-    uint32_t *vals_in = nullptr, *vals_out = nullptr;
-    unsigned long long *d_on = nullptr;
-    void *tmp = nullptr;
-    size_t tmp_bytes = 0;
-    auto cleanup = [&]() {
-        (void)hipFree(keys_in); (void)hipFree(vals_in); (void)hipFree(d_on); (void)hipFree(tmp);
-    };
-    if (hipMalloc(&keys_in, n * 4) != hipSuccess || hipMalloc(&vals_in, n * 4) != hipSuccess ||
-        hipMalloc(&keys_out, n * 4) != hipSuccess || hipMalloc(&vals_out, n * 4) != hipSuccess ||
-        hipMalloc(&d_on, sizeof(unsigned long long)) != hipSuccess) {
-        cleanup();
-        (void)hipFree(keys_out); (void)hipFree(vals_out);
-        (void)hipGetLastError();
-        return set_error(SO3G_ERR_HIP, "out of device memory building the plan");
-    }
-    hipLaunchKernelGGL(k_plan_keys, dim3((unsigned)((n + kPlanBlock - 1) / kPlanBlock)),
-                       dim3(kPlanBlock), 0, s, pixel, n, nx, npix, n_t, t_block, off_key,
-                       keys_in, vals_in);
-    PLAN_TRY(hipGetLastError());
-    PLAN_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, vals_in, vals_out,
-                                       (size_t)n, 0, end_bit, s));
-    PLAN_TRY(hipMalloc(&tmp, tmp_bytes));
-    PLAN_TRY(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, vals_out,
-                                       (size_t)n, 0, end_bit, s));
-    hipLaunchKernelGGL(k_plan_count, dim3(1), dim3(1), 0, s, keys_out, n, off_key, d_on);
-    PLAN_TRY(hipGetLastError());
-    unsigned long long n_on = 0;
-    PLAN_TRY(hipMemcpyAsync(&n_on, d_on, sizeof(n_on), hipMemcpyDeviceToHost, s));
-    PLAN_TRY(hipStreamSynchronize(s));
-    cleanup();
-
-    so3g_pc_plan *p = new so3g_pc_plan();
-    p->n_det = n_det; p->n_t = n_t; p->ny = ny; p->nx = nx; p->nc = nc;
-    p->n_on = (int64_t)n_on;
-    p->t_block = t_block;
-    p->key = keys_out;
-    p->idx = vals_out;
-    p->spin = nullptr;
-    if (hipMalloc(&p->spin, std::max<int64_t>(p->n_on, 1) * nc * sizeof(float)) != hipSuccess) {
-        (void)hipGetLastError();
-        so3g_pc_plan_destroy(p);
-        return set_error(SO3G_ERR_HIP, "out of device memory building the plan");
-    }
-    const dim3 g((unsigned)((p->n_on + kPlanBlock - 1) / kPlanBlock));
-    if (p->n_on > 0) {
-        if (nc == 1) hipLaunchKernelGGL(k_plan_spin<1>, g, dim3(kPlanBlock), 0, s, spin, p->idx, p->n_on, p->spin);
-        else if (nc == 2) hipLaunchKernelGGL(k_plan_spin<2>, g, dim3(kPlanBlock), 0, s, spin, p->idx, p->n_on, p->spin);
-        else hipLaunchKernelGGL(k_plan_spin<3>, g, dim3(kPlanBlock), 0, s, spin, p->idx, p->n_on, p->spin);
-        PLAN_TRY(hipGetLastError());
-    }
-    PLAN_TRY(hipStreamSynchronize(s));
-    p->bytes = n * 8 + p->n_on * nc * 4;
-    *out = p;
-    return SO3G_OK;
+unsigned grid_of(int64_t n, int64_t per = kPlanBlock, int64_t cap = 1 << 20) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + per - 1) / per, cap));
 }
 
-}  // extern "C"
+void free_plan_arrays(so3g_pc_plan *p) {
+    (void)hipFree(p->idx); (void)hipFree(p->sspin); (void)hipFree(p->tval);
+    (void)hipFree(p->heads); (void)hipFree(p->wbase); (void)hipFree(p->run_pix);
+    (void)hipFree(p->tpix); (void)hipFree(p->tspin); (void)hipFree(p->packed);
+}
 
-namespace {
 template <typename ST>
 int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
                 const float *det_weights, void *stream) {
     if (p == nullptr) return set_error(SO3G_ERR_VALUE, "plan is NULL");
     if (p->n_on == 0) return SO3G_OK;
     hipStream_t s = (hipStream_t)stream;
-    const int64_t per_block = (int64_t)kPlanBlock * kPlanR;
-    const dim3 g((unsigned)((p->n_on + per_block - 1) / per_block));
+    const dim3 g(grid_of(p->n_on, (int64_t)kPlanBlock * kPlanR, INT32_MAX));
     const uint32_t plane = (uint32_t)(p->ny * p->nx);
-    const uint32_t nt = (uint32_t)p->n_t;
-    switch (p->nc) {
-    case 1: hipLaunchKernelGGL((k_plan_to_map_r<1, kPlanR, ST>), g, dim3(kPlanBlock), 0, s, p->key, p->idx, p->spin, p->n_on, nt, plane, signal, det_weights, map); break;
-    case 2: hipLaunchKernelGGL((k_plan_to_map_r<2, kPlanR, ST>), g, dim3(kPlanBlock), 0, s, p->key, p->idx, p->spin, p->n_on, nt, plane, signal, det_weights, map); break;
-    default: hipLaunchKernelGGL((k_plan_to_map_r<3, kPlanR, ST>), g, dim3(kPlanBlock), 0, s, p->key, p->idx, p->spin, p->n_on, nt, plane, signal, det_weights, map); break;
+    const FastDiv nt{(uint32_t)p->n_t, 1.f / (float)p->n_t};
+#define PLAN_TO(NC, TC)                                                                     \
+    do {                                                                                    \
+        if (g_plan_div)                                                                     \
+            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 1>), g, dim3(kPlanBlock), 0, s, \
+                               p->idx, p->sspin, p->tval, p->heads, p->wbase, p->run_pix,  \
+                               p->n_on, nt, plane, signal, det_weights, map);              \
+        else                                                                                \
+            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 0>), g, dim3(kPlanBlock), 0, s, \
+                               p->idx, p->sspin, p->tval, p->heads, p->wbase, p->run_pix,  \
+                               p->n_on, nt, plane, signal, det_weights, map);              \
+    } while (0)
+    if (p->tconst) {
+        if (p->nc == 1) PLAN_TO(1, true);
+        else PLAN_TO(3, true);
+    } else {
+        if (p->nc == 1) PLAN_TO(1, false);
+        else if (p->nc == 2) PLAN_TO(2, false);
+        else PLAN_TO(3, false);
     }
+#undef PLAN_TO
     PLAN_TRY(hipGetLastError());
     return SO3G_OK;
 }
@@ -362,9 +527,171 @@ __global__ __launch_bounds__(kPlanBlock) void k_pc_from_map_f64(int64_t n, int32
         sig[i] = v;
     }
 }
+
 }  // namespace
 
 extern "C" {
+
+int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int32_t nc,
+                        int64_t n_det, int64_t n_t, const int64_t *map_shape,
+                        int64_t t_block_mb, so3g_pc_plan **out, void *stream) {
+    if (out == nullptr) return set_error(SO3G_ERR_VALUE, "out is NULL");
+    *out = nullptr;
+    if (k != 2) return set_error(SO3G_ERR_RUNTIME, "plans support non-tiled flat maps (k = 2)");
+    if (nc < 1 || nc > 3) return set_error(SO3G_ERR_RUNTIME, "spin component count must be 1..3");
+    if (n_det <= 0 || n_t <= 0)
+        return set_error(SO3G_ERR_RUNTIME, "plan needs 0 < n_det * n_t < 2^32");
+    if (n_det >= (1 << 22)) return set_error(SO3G_ERR_RUNTIME, "plan needs n_det < 2^22");
+    const int64_t n = n_det * n_t;
+    if (n > (int64_t)UINT32_MAX)
+        return set_error(SO3G_ERR_RUNTIME, "plan needs 0 < n_det * n_t < 2^32");
+    const int64_t ny = map_shape[0], nx = map_shape[1];
+    if (ny <= 0 || nx <= 0 || ny * nx >= INT32_MAX)
+        return set_error(SO3G_ERR_RUNTIME, "map too large for a plan");
+    hipStream_t s = (hipStream_t)stream;
+    const int32_t npix = (int32_t)(ny * nx);
+    // Time blocks whose signal slice (n_det x t_block float32) stays resident
+    // in the Infinity Cache while the block's samples are gathered.
+    int64_t t_block = std::max<int64_t>(1, (t_block_mb << 20) / (4 * n_det));
+    if (t_block_mb <= 0 || t_block > n_t) t_block = n_t;
+    int64_t n_tb = (n_t + t_block - 1) / t_block;
+    while (n_tb * npix >= INT32_MAX) {  // keys must fit int32
+        t_block *= 2;
+        n_tb = (n_t + t_block - 1) / t_block;
+    }
+    const int32_t off_key = (int32_t)(n_tb * npix);
+    int end_bit = 1;
+    while ((1LL << end_bit) <= off_key) end_bit++;
+
+    so3g_pc_plan *p = new so3g_pc_plan();
+    p->n_det = n_det; p->n_t = n_t; p->ny = ny; p->nx = nx; p->nc = nc;
+    p->t_block = t_block;
+    int32_t *keys_in = nullptr, *keys_out = nullptr;
+    uint32_t *vals_in = nullptr, *gcount = nullptr;
+    unsigned long long *d_on = nullptr;
+    int *bad = nullptr;
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    auto cleanup = [&]() {
+        (void)hipFree(keys_in); (void)hipFree(keys_out); (void)hipFree(vals_in);
+        (void)hipFree(gcount); (void)hipFree(d_on); (void)hipFree(bad); (void)hipFree(tmp);
+    };
+    auto oom = [&]() {
+        cleanup();
+        free_plan_arrays(p);
+        delete p;
+        (void)hipGetLastError();
+        return set_error(SO3G_ERR_HIP, "out of device memory building the plan");
+    };
+#define PLAN_ALLOC(ptr, bytes) \
+    if (hipMalloc(&(ptr), std::max<int64_t>((int64_t)(bytes), 1)) != hipSuccess) return oom()
+#define PLAN_STEP(expr)                                                              \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess) {                                                      \
+            cleanup();                                                               \
+            free_plan_arrays(p);                                                     \
+            delete p;                                                                \
+            return set_error(SO3G_ERR_HIP, std::string("HIP error in ") + #expr + ": " + \
+                                               hipGetErrorString(_e));               \
+        }                                                                            \
+    } while (0)
+
+    // 1. sort the samples by (time block, pixel)
+    PLAN_ALLOC(keys_in, n * 4);
+    PLAN_ALLOC(vals_in, n * 4);
+    PLAN_ALLOC(keys_out, n * 4);
+    PLAN_ALLOC(p->idx, n * 4);
+    PLAN_ALLOC(d_on, sizeof(unsigned long long));
+    hipLaunchKernelGGL(k_plan_keys, dim3(grid_of(n, kPlanBlock, INT32_MAX)), dim3(kPlanBlock), 0,
+                       s, pixel, n, nx, npix, n_t, t_block, off_key, keys_in, vals_in);
+    PLAN_STEP(hipGetLastError());
+    PLAN_STEP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, vals_in, p->idx,
+                                        (size_t)n, 0, end_bit, s));
+    PLAN_ALLOC(tmp, tmp_bytes);
+    PLAN_STEP(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, p->idx,
+                                        (size_t)n, 0, end_bit, s));
+    hipLaunchKernelGGL(k_plan_count, dim3(1), dim3(1), 0, s, keys_out, n, off_key, d_on);
+    PLAN_STEP(hipGetLastError());
+    unsigned long long n_on = 0;
+    PLAN_STEP(hipMemcpyAsync(&n_on, d_on, sizeof(n_on), hipMemcpyDeviceToHost, s));
+    PLAN_STEP(hipStreamSynchronize(s));
+    p->n_on = (int64_t)n_on;
+    (void)hipFree(keys_in); keys_in = nullptr;
+    (void)hipFree(vals_in); vals_in = nullptr;
+    (void)hipFree(tmp); tmp = nullptr;
+
+    // 2. is the T factor constant per detector?
+    p->tconst = 0;
+    if (nc != 2) {
+        PLAN_ALLOC(bad, n_det * sizeof(int));
+        PLAN_STEP(hipMemsetAsync(bad, 0, n_det * sizeof(int), s));
+        hipLaunchKernelGGL(k_plan_tcheck, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, spin, nc, n_t,
+                           n, bad);
+        PLAN_STEP(hipGetLastError());
+        std::vector<int> hbad(n_det);
+        PLAN_STEP(hipMemcpyAsync(hbad.data(), bad, n_det * sizeof(int), hipMemcpyDeviceToHost, s));
+        PLAN_STEP(hipStreamSynchronize(s));
+        p->tconst = std::all_of(hbad.begin(), hbad.end(), [](int b) { return b == 0; });
+    }
+    const int c0 = p->tconst ? 1 : 0;
+    p->ns = nc - c0;
+    if (p->tconst) {
+        PLAN_ALLOC(p->tval, n_det * sizeof(float));
+        hipLaunchKernelGGL(k_plan_tval, dim3(grid_of(n_det)), dim3(kPlanBlock), 0, s, spin, nc, n_t,
+                           n_det, p->tval);
+        PLAN_STEP(hipGetLastError());
+    }
+
+    // 3. stored spin factors in sorted order
+    PLAN_ALLOC(p->sspin, p->n_on * p->ns * sizeof(float));
+    if (p->n_on > 0 && p->ns > 0) {
+        hipLaunchKernelGGL(k_plan_sspin, dim3(grid_of(p->n_on)), dim3(kPlanBlock), 0, s, spin, nc,
+                           c0, p->idx, p->n_on, p->sspin);
+        PLAN_STEP(hipGetLastError());
+    }
+
+    // 4. head bits, run counts per group, run pixels
+    const int64_t n_groups = std::max<int64_t>(1, (p->n_on + kGroup - 1) / kGroup);
+    PLAN_ALLOC(p->heads, n_groups * (kGroup / 8));
+    PLAN_ALLOC(gcount, n_groups * sizeof(uint32_t));
+    PLAN_ALLOC(p->wbase, n_groups * sizeof(uint32_t));
+    const int64_t n_bytes = n_groups * (kGroup / 8);
+    const dim3 gb(grid_of(n_bytes, kPlanBlock, INT32_MAX));
+    hipLaunchKernelGGL(k_plan_heads, gb, dim3(kPlanBlock), 0, s, keys_out, p->n_on, n_bytes,
+                       p->heads, gcount);
+    PLAN_STEP(hipGetLastError());
+    PLAN_STEP(rocprim::exclusive_scan(nullptr, tmp_bytes, gcount, p->wbase, 0u,
+                                      (size_t)n_groups, rocprim::plus<uint32_t>(), s));
+    PLAN_ALLOC(tmp, tmp_bytes);
+    PLAN_STEP(rocprim::exclusive_scan(tmp, tmp_bytes, gcount, p->wbase, 0u, (size_t)n_groups,
+                                      rocprim::plus<uint32_t>(), s));
+    uint32_t last_base = 0, last_count = 0;
+    PLAN_STEP(hipMemcpyAsync(&last_base, p->wbase + n_groups - 1, 4, hipMemcpyDeviceToHost, s));
+    PLAN_STEP(hipMemcpyAsync(&last_count, gcount + n_groups - 1, 4, hipMemcpyDeviceToHost, s));
+    PLAN_STEP(hipStreamSynchronize(s));
+    p->n_runs = (int64_t)last_base + last_count;
+    PLAN_ALLOC(p->run_pix, p->n_runs * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_plan_runs, gb, dim3(kPlanBlock), 0, s, keys_out, n_bytes, npix,
+                       p->heads, p->wbase, p->run_pix);
+    PLAN_STEP(hipGetLastError());
+
+    // 5. time-ordered copy for from_map, and the packed map buffer
+    PLAN_ALLOC(p->tpix, n * sizeof(int32_t));
+    PLAN_ALLOC(p->tspin, n * p->ns * sizeof(float));
+    hipLaunchKernelGGL(k_plan_tcopy, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, spin, nc, c0,
+                       n, ny, nx, p->tpix, p->tspin);
+    PLAN_STEP(hipGetLastError());
+    PLAN_ALLOC(p->packed, (int64_t)npix * 4 * sizeof(double));
+    PLAN_STEP(hipStreamSynchronize(s));
+    cleanup();
+#undef PLAN_ALLOC
+#undef PLAN_STEP
+    p->bytes = p->n_on * (4 + 4 * p->ns) + n_groups * (kGroup / 8 + 4) + p->n_runs * 4 +
+               n * (4 + 4 * p->ns) + (int64_t)npix * 32 + (p->tconst ? n_det * 4 : 0);
+    *out = p;
+    return SO3G_OK;
+}
 
 int so3g_pc_plan_to_map(const so3g_pc_plan *p, double *map, const float *signal,
                         const float *det_weights, void *stream) {
@@ -374,6 +701,49 @@ int so3g_pc_plan_to_map(const so3g_pc_plan *p, double *map, const float *signal,
 int so3g_pc_plan_to_map_f64(const so3g_pc_plan *p, double *map, const double *signal,
                             const float *det_weights, void *stream) {
     return plan_to_map<double>(p, map, signal, det_weights, stream);
+}
+
+int so3g_pc_plan_from_map(const so3g_pc_plan *p, const double *map, float *signal,
+                          int accumulate, void *stream) {
+    if (p == nullptr) return set_error(SO3G_ERR_VALUE, "plan is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t plane = p->ny * p->nx;
+    const int64_t n = p->n_det * p->n_t;
+    switch (p->nc) {
+    case 1: hipLaunchKernelGGL(k_plan_pack<1>, dim3(grid_of(plane)), dim3(kPlanBlock), 0, s, map, plane, p->packed); break;
+    case 2: hipLaunchKernelGGL(k_plan_pack<2>, dim3(grid_of(plane)), dim3(kPlanBlock), 0, s, map, plane, p->packed); break;
+    default: hipLaunchKernelGGL(k_plan_pack<3>, dim3(grid_of(plane)), dim3(kPlanBlock), 0, s, map, plane, p->packed); break;
+    }
+    PLAN_TRY(hipGetLastError());
+    const int vv = n % 8 == 0 ? g_plan_from_v : (n % 4 == 0 ? 4 : 1);
+    const dim3 g(grid_of(n, (int64_t)kPlanBlock * vv, INT32_MAX));
+    const FastDiv nt{(uint32_t)p->n_t, 1.f / (float)p->n_t};
+#define PLAN_FROM(NC, TC)                                                                   \
+    do {                                                                                    \
+        if (vv == 8)                                                                        \
+            hipLaunchKernelGGL((k_plan_from_map<NC, TC, 8, 1>), g, dim3(kPlanBlock), 0, s,   \
+                               p->tpix, p->tspin, p->tval, p->packed, n, nt, signal,        \
+                               accumulate);                                                 \
+        else if (vv == 4)                                                                   \
+            hipLaunchKernelGGL((k_plan_from_map<NC, TC, 4, 1>), g, dim3(kPlanBlock), 0, s,   \
+                               p->tpix, p->tspin, p->tval, p->packed, n, nt, signal,        \
+                               accumulate);                                                 \
+        else                                                                                \
+            hipLaunchKernelGGL((k_plan_from_map<NC, TC, 1, 1>), g, dim3(kPlanBlock), 0, s,   \
+                               p->tpix, p->tspin, p->tval, p->packed, n, nt, signal,        \
+                               accumulate);                                                 \
+    } while (0)
+    if (p->tconst) {
+        if (p->nc == 1) PLAN_FROM(1, true);
+        else PLAN_FROM(3, true);
+    } else {
+        if (p->nc == 1) PLAN_FROM(1, false);
+        else if (p->nc == 2) PLAN_FROM(2, false);
+        else PLAN_FROM(3, false);
+    }
+#undef PLAN_FROM
+    PLAN_TRY(hipGetLastError());
+    return SO3G_OK;
 }
 
 int so3g_precomp_from_map_f64(const double *map, const int64_t *map_shape,
@@ -388,8 +758,7 @@ int so3g_precomp_from_map_f64(const double *map, const int64_t *map_shape,
     if (ny <= 0 || nx <= 0 || ny > INT32_MAX || nx > INT32_MAX)
         return set_error(SO3G_ERR_RUNTIME, "bad map shape");
     hipStream_t s = (hipStream_t)stream;
-    const int64_t nb = std::min<int64_t>((n + kPlanBlock - 1) / kPlanBlock, 1 << 20);
-    const dim3 g((unsigned)nb);
+    const dim3 g(grid_of(n));
     switch (nc) {
     case 1: hipLaunchKernelGGL(k_pc_from_map_f64<1>, g, dim3(kPlanBlock), 0, s, n, (int32_t)ny, (int32_t)nx, map, pixel, spin, signal, accumulate); break;
     case 2: hipLaunchKernelGGL(k_pc_from_map_f64<2>, g, dim3(kPlanBlock), 0, s, n, (int32_t)ny, (int32_t)nx, map, pixel, spin, signal, accumulate); break;
@@ -406,11 +775,20 @@ int so3g_pc_plan_info(const so3g_pc_plan *p, int64_t *n_on, int64_t *bytes) {
     return SO3G_OK;
 }
 
+int so3g_pc_plan_stats(const so3g_pc_plan *p, int64_t *stats) {
+    if (p == nullptr || stats == nullptr) return set_error(SO3G_ERR_VALUE, "NULL argument");
+    stats[0] = p->n_on;
+    stats[1] = p->n_runs;
+    stats[2] = p->tconst;
+    stats[3] = p->ns;
+    stats[4] = p->t_block;
+    stats[5] = p->bytes;
+    return SO3G_OK;
+}
+
 int so3g_pc_plan_destroy(so3g_pc_plan *p) {
     if (p == nullptr) return SO3G_OK;
-    (void)hipFree(p->key);
-    (void)hipFree(p->idx);
-    (void)hipFree(p->spin);
+    free_plan_arrays(p);
     delete p;
     return SO3G_OK;
 }

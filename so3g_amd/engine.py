@@ -15,6 +15,8 @@ ROCm device (used in place, zero-copy).  Signal-like outputs are returned
 as lists of per-detector rows for numpy inputs, as the reference does;
 when the caller passes torch tensors, the tensor itself is returned.
 """
+import ctypes
+
 import numpy as np
 
 from . import _native as N
@@ -758,35 +760,39 @@ class PrecompPlan:
     """Pixel-sorted plan for repeated cached-pointing projection with one
     pointing matrix (CG-style loops).  No counterpart in the reference: built
     once from a ProjEng_Precomp pixel_index (k = 2, non-tiled flat map of
-    `map_shape` = (ny, nx)) and spin_proj, it sorts the on-map samples by
-    pixel so that to_map sums each pixel's contributions with a wave-level
-    segmented reduction instead of per-sample atomics.  to_map adds what
-    ProjEng_Precomp_NonTiled.to_map adds over every sample (float64 sums of
-    the same float32 products, in a different order); from_map is
-    ProjEng_Precomp_NonTiled.from_map on the stored pointing matrix."""
+    `map_shape` = (ny, nx)) and spin_proj (both copied into the plan), it
+    sorts the on-map samples by pixel so that to_map sums each pixel's
+    contributions with a wave-level segmented reduction instead of
+    per-sample atomics, and keeps a compact time-ordered copy for from_map.
+    to_map adds what ProjEng_Precomp_NonTiled.to_map adds over every sample
+    (float64 sums of the same float32 products, in a different order);
+    from_map returns what ProjEng_Precomp_NonTiled.from_map returns, bit for
+    bit."""
 
     def __init__(self, pixel_index, spin_proj, map_shape, t_block_mb=64):
         import ctypes
         call = _Call()
-        self._pix = call.array(pixel_index, 'int32', 'pixel_index', spec=[-1, -1, 2])
-        n_det, n_t, _ = self._pix.shape
-        self._spin = call.array(spin_proj, 'float32', 'spin_proj', spec=[n_det, n_t, -1])
+        pix = call.array(pixel_index, 'int32', 'pixel_index', spec=[-1, -1, 2])
+        n_det, n_t, _ = pix.shape
+        spin = call.array(spin_proj, 'float32', 'spin_proj', spec=[n_det, n_t, -1])
         self.n_det, self.n_t = int(n_det), int(n_t)
-        self.nc = int(self._spin.shape[2])
+        self.nc = int(spin.shape[2])
         self.map_shape = (int(map_shape[0]), int(map_shape[1]))
         self._shape = np.array(self.map_shape, np.int64)
         h = ctypes.c_void_p()
-        N.check(N.load().so3g_pc_plan_create(self._pix.data_ptr(), 2, self._spin.data_ptr(),
-                                             self.nc, self.n_det, self.n_t,
-                                             self._shape.ctypes.data, int(t_block_mb),
-                                             ctypes.byref(h), _stream()))
+        N.check(N.load().so3g_pc_plan_create(pix.data_ptr(), 2, spin.data_ptr(), self.nc,
+                                             self.n_det, self.n_t, self._shape.ctypes.data,
+                                             int(t_block_mb), ctypes.byref(h), _stream()))
         self._h = h
 
     def info(self):
-        import ctypes
-        n_on, nbytes = ctypes.c_int64(), ctypes.c_int64()
-        N.check(N.load().so3g_pc_plan_info(self._h, ctypes.byref(n_on), ctypes.byref(nbytes)))
-        return {'n_on': n_on.value, 'bytes': nbytes.value}
+        """n_on (on-map samples), runs (pixel runs in sorted order),
+        t_const (T factor stored once per detector), stored_spin (factors
+        stored per sample), t_block (samples per time block), bytes."""
+        a = (ctypes.c_int64 * 6)()
+        N.check(N.load().so3g_pc_plan_stats(self._h, a))
+        return {'n_on': a[0], 'runs': a[1], 't_const': bool(a[2]), 'stored_spin': a[3],
+                't_block': a[4], 'bytes': a[5]}
 
     def to_map(self, map, signal, det_weights=None):
         call = _Call()
@@ -800,13 +806,16 @@ class PrecompPlan:
         return map
 
     def from_map(self, map, signal=None):
-        pe = ProjEngPrecompBase()
-        out = pe.from_map(map, self._pix, self._spin, signal)
-        if _is_torch(out) and not _is_torch(map) and not _is_torch(signal):
-            if signal is not None and not isinstance(signal, (list, tuple)):
-                return list(signal)
-            return list(out.cpu().numpy())
-        return out
+        """signal (+)= P map; a given signal is accumulated into, as
+        ProjEng_Precomp.from_map does; None returns fresh rows."""
+        call = _Call()
+        m = call.array(map, 'float64', 'map', spec=[self.nc] + list(self.map_shape))
+        sig, obj, fresh = _signal_space(call, signal, 'signal', 'float32', self.n_det,
+                                        self.n_t)
+        N.check(N.load().so3g_pc_plan_from_map(self._h, m.data_ptr(), sig.data_ptr(),
+                                               0 if fresh else 1, _stream()))
+        call.finish()
+        return _signal_result(call, sig, obj)
 
     def __del__(self):
         h = getattr(self, '_h', None)

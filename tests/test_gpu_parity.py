@@ -355,22 +355,55 @@ def test_scatter_variants(gpu, agg, xcd, lds):
             _native.set_option(n, v)
 
 
-def test_precomp_plan(gpu):
-    """PrecompPlan.to_map == Precomp to_map over all samples (oracle)."""
+@pytest.mark.parametrize('n_det,n_samp,comps,tconst,t_block_mb', [
+    (30, 7000, 'TQU', True, 64),      # one time block, T factor per detector
+    (201, 3001, 'TQU', True, 1),      # several time blocks, odd sample count
+    (8, 1001, 'TQU', True, 64),       # 8-sample lanes crossing detector rows
+    (40, 2000, 'TQU', False, 64),     # T factor varies: all factors stored
+    (40, 2000, 'T', True, 64),        # T only: nothing stored per sample
+    (40, 2000, 'QU', False, 1)])
+def test_precomp_plan(gpu, n_det, n_samp, comps, tconst, t_block_mb):
+    """PrecompPlan.to_map == Precomp to_map over all samples (oracle),
+    PrecompPlan.from_map == Precomp from_map bit for bit."""
     import so3g_amd
-    c = Case('CAR', comps='TQU', n_det=30, n_samp=7000, ny=100, nx=300, clip=True)
+    c = Case('CAR', comps=comps, n_det=n_det, n_samp=n_samp, ny=100, nx=300, clip=True)
     pix, spin = [np.array(x) for x in c.engine.pointing_matrix(c.bore, c.dets, c.resps)]
-    plan = so3g_amd.PrecompPlan(pix, spin, (100, 300))
-    assert plan.info()['n_on'] == int((pix[..., 0] >= 0).sum())
+    nc = spin.shape[2]
+    if not tconst and comps != 'QU':
+        spin[..., 0] *= np.linspace(0.5, 1.5, n_samp, dtype=np.float32)
+    plan = so3g_amd.PrecompPlan(pix, spin, (100, 300), t_block_mb=t_block_mb)
+    info = plan.info()
+    on = pix[..., 0] >= 0
+    assert info['n_on'] == int(on.sum())
+    assert info['t_const'] == (tconst and comps != 'QU')
+    assert info['stored_spin'] == nc - int(info['t_const'])
+    # runs = distinct (time block, pixel) among on-map samples
+    tb = (np.arange(n_samp) // info['t_block'])[None, :].repeat(n_det, 0)
+    keys = (tb * 30000 + pix[..., 0] * 300 + pix[..., 1])[on]
+    assert info['runs'] == len(np.unique(keys))
     dw = c.det_weights.copy()
     dw[3] = 0.
     for w in (None, dw):
-        g = plan.to_map(np.zeros((3, 100, 300)), c.signal, w)
-        r = oracle.precomp_to_map(c.og, np.zeros((3, 100, 300)), pix, spin, c.signal, w)
+        g = plan.to_map(np.zeros((nc, 100, 300)), c.signal, w)
+        r = oracle.precomp_to_map(c.og, np.zeros((nc, 100, 300)), pix, spin, c.signal, w)
         maps_close(g, r, 'plan to_map', atol=1e-9)
-    m = c.map_like([3])
-    assert_bit_equal(np.array(plan.from_map(m)), oracle.precomp_from_map(c.og, m, pix, spin),
-                     'plan from_map')
+    m = c.map_like([nc])
+    ref = oracle.precomp_from_map(c.og, m, pix, spin)
+    assert_bit_equal(np.array(plan.from_map(m)), ref, 'plan from_map')
+    s0 = np.array(c.signal)
+    acc = plan.from_map(m, s0.copy())
+    assert_bit_equal(np.array(acc), oracle.precomp_from_map(c.og, m, pix, spin, s0.copy()),
+                     'plan from_map accumulate')
+    # float64 signal variant: the same sums with float64 products
+    import torch
+    dev = torch.device('cuda')
+    mf = torch.zeros((nc, 100, 300), dtype=torch.float64, device=dev)
+    sig64 = torch.from_numpy(c.signal.astype(np.float64)).to(dev)
+    from so3g_amd import _native as N
+    N.check(N.load().so3g_pc_plan_to_map_f64(plan._h, mf.data_ptr(), sig64.data_ptr(), None,
+                                             None))
+    r = oracle.precomp_to_map(c.og, np.zeros((nc, 100, 300)), pix, spin, c.signal)
+    maps_close(mf.cpu().numpy(), r, 'plan to_map f64', atol=1e-5)
 
 
 def test_precision_modes(gpu):
