@@ -33,17 +33,24 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 COPY_BYTES = 1 << 30  # calibration copy of the --pmc-pass run
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'traffic_large_car_tqu.md')
+FP64_FILE = os.path.join(ROOT, 'profiles', 'fp64_onthefly_car_tqu.md')
+FP64_PEAK_OPS = 39.3e12  # MI355X FP64 vector: 78.6 TFLOP/s = 39.3e12 lane ops/s (FMA = 1 op)
 
 
-def measured_traffic():
-    """HBM bytes per launch of the headline kernels, from the JSON block of
-    the committed rocprofv3 --pmc summary (None when absent)."""
+def profile_json(path):
+    """The JSON block of a committed rocprofv3 --pmc summary (None when
+    absent): HBM bytes per launch of the headline kernels, or FP64 lane
+    operations per sample of the on-the-fly kernels."""
     try:
-        txt = open(TRAFFIC_FILE).read()
+        txt = open(path).read()
         block = txt.split('```json', 1)[1].split('```', 1)[0]
         return json.loads(block)
     except (OSError, IndexError, ValueError):
         return None
+
+
+def measured_traffic():
+    return profile_json(TRAFFIC_FILE)
 
 
 def log(*a):
@@ -61,6 +68,10 @@ def parse():
     ap.add_argument('--no-otf', action='store_true', help='skip the on-the-fly leg')
     ap.add_argument('--pmc-pass', action='store_true',
                     help='one calibration copy + one launch of each headline kernel')
+    ap.add_argument('--pmc-otf', action='store_true',
+                    help='with --pmc-pass: also one launch of each on-the-fly kernel')
+    ap.add_argument('--cpu-full', action='store_true',
+                    help='print the full CPU baseline table (oracle, median of 5) and exit')
     return ap.parse_args()
 
 
@@ -190,7 +201,16 @@ def main():
         torch.cuda.synchronize()
         pc_to()
         torch.cuda.synchronize()
+        if args.pmc_otf:
+            otf_from()
+            torch.cuda.synchronize()
+            otf_to()
+            torch.cuda.synchronize()
         print(json.dumps({'pmc_pass': True, 'copy_bytes': COPY_BYTES}), flush=True)
+        return
+
+    if args.cpu_full:
+        print(json.dumps(cpu_full(obs, pix, spin, sig, sky, pixargs)), flush=True)
         return
 
     log('timing cached-pointing path ...')
@@ -241,6 +261,11 @@ def main():
             'hbm_frac_to_map': (N_s * 4 + 32 * n_t + 40 * n_det + 16 * 3 * npix) /
             (ms_ot * 1e-3) / HBM_PEAK,
         }
+        f64 = (profile_json(FP64_FILE) or {}).get('fp64_ops_per_sample')
+        if f64:
+            otf['fp64_ops_per_sample'] = f64
+            otf['fp64_frac_from_map'] = f64['from_map'] * N_s / (ms_of * 1e-3) / FP64_PEAK_OPS
+            otf['fp64_frac_to_map'] = f64['to_map'] * N_s / (ms_ot * 1e-3) / FP64_PEAK_OPS
 
     cpu = None if args.no_cpu else cpu_baseline(obs, pix, spin, sig, sky, pixargs)
 
@@ -319,6 +344,78 @@ def cpu_baseline(obs, pix, spin, sig, sky, pixargs):
                   f'from_map (OpenMP, {oracle.num_threads()} threads: {t1 - t0:.2f} s) + '
                   f'to_map (1 thread: {t2 - t1:.2f} s)',
     }
+
+
+def _stripe_segments(dom, n_thread):
+    """Per-thread (det, t0, t1) segment lists from a per-sample domain array."""
+    segs = [[] for _ in range(n_thread)]
+    for d, row in enumerate(dom):
+        edges = np.flatnonzero(np.diff(row)) + 1
+        starts = np.concatenate([[0], edges])
+        ends = np.concatenate([edges, [len(row)]])
+        for a, b in zip(starts, ends):
+            k = int(row[a])
+            if 0 <= k < n_thread:
+                segs[k].append((d, int(a), int(b)))
+    return [np.asarray(x, np.int32).reshape(-1, 3) for x in segs]
+
+
+def cpu_full(obs, pix, spin, sig, sky, pixargs, n_det=50, reps=5):
+    """The oracle on the host cores for every operation of the path, on the
+    first n_det detectors of the workload: 1 warm-up, median of reps runs.
+    'N threads' = OpenMP over detectors (from_map) or over pixel-stripe thread
+    domains (to_map, so3g's 'simple' assignment); to_map without a thread
+    assignment and to_weights run single-threaded, as so3g does."""
+    import platform
+    import oracle
+    n = min(n_det, obs.n_det)
+    bore = obs.bore
+    d, r = obs.dets[:n], obs.resps[:n]
+    p = pix[:n].cpu().numpy()
+    sp = spin[:n].cpu().numpy()
+    sg = sig[:n].cpu().numpy()
+    m = sky.cpu().numpy()
+    og = oracle.geom_from_args('CAR', 'TQU', pixargs)
+    nthr = oracle.num_threads()
+    dom = oracle.sample_domains(og, bore, d, 0, nthr)
+    segs = _stripe_segments(dom, nthr)
+
+    def med(f):
+        f()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    z = lambda *lead: np.zeros(lead + m.shape[1:])
+    ops = {
+        'on-the-fly from_map (N threads)': lambda: oracle.from_map(og, m, bore, d, r),
+        'on-the-fly to_map (1 thread)': lambda: oracle.to_map(og, z(3), bore, d, r, sg),
+        'on-the-fly to_map (N threads, pixel stripes)':
+            lambda: oracle.to_map_threaded(og, z(3), bore, d, r, sg, segs),
+        'on-the-fly to_weights (1 thread)': lambda: oracle.to_weights(og, z(3, 3), bore, d, r),
+        'precomp from_map (N threads)': lambda: oracle.precomp_from_map(og, m, p, sp),
+        'precomp to_map (1 thread)': lambda: oracle.precomp_to_map(og, z(3), p, sp, sg),
+        'precomp to_weights (1 thread)': lambda: oracle.precomp_to_weights(og, z(3, 3), p, sp),
+    }
+    n_s = n * obs.n_samp
+    rates = {}
+    for k, f in ops.items():
+        t = med(f)
+        rates[k] = n_s / t
+        log(f'{k}: {t:.3f} s, {n_s / t:.3g} det*samples/s')
+    model = ''
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        model = platform.processor()
+    return {'cpu': model, 'threads': nthr, 'sample': f'{n} detectors x {obs.n_samp} samples',
+            'runs': f'1 warm-up, median of {reps}', 'unit': 'det*samples/s', 'rates': rates}
 
 
 if __name__ == '__main__':
