@@ -70,6 +70,7 @@ def parse():
                     help='one calibration copy + one launch of each headline kernel')
     ap.add_argument('--pmc-otf', action='store_true',
                     help='with --pmc-pass: also one launch of each on-the-fly kernel')
+    ap.add_argument('--no-small', action='store_true', help='skip the small-observation leg')
     ap.add_argument('--cpu-full', action='store_true',
                     help='print the full CPU baseline table (oracle, median of 5) and exit')
     return ap.parse_args()
@@ -268,6 +269,8 @@ def main():
             otf['fp64_frac_to_map'] = f64['to_map'] * N_s / (ms_ot * 1e-3) / FP64_PEAK_OPS
 
     cpu = None if args.no_cpu else cpu_baseline(obs, pix, spin, sig, sky, pixargs)
+    small = None if args.no_small else small_config(torch, so3g_amd, N, workloads, dev, stream)
+    nmat = None if args.no_small else nmat_timing(torch, so3g_amd, dev, n_det, n_t)
 
     out = {
         'metric': 'det*samples/s for to_map+from_map (% HBM roofline)',
@@ -307,6 +310,8 @@ def main():
                 'to_map_frac': (b_pc + 16 * 3 * npix) / (ms_to * 1e-3) / HBM_PEAK},
         },
         'on_the_fly': otf,
+        'small_config': small,
+        'nmat_detvecs_apply': nmat,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)
@@ -344,6 +349,98 @@ def cpu_baseline(obs, pix, spin, sig, sky, pixargs):
                   f'from_map (OpenMP, {oracle.num_threads()} threads: {t1 - t0:.2f} s) + '
                   f'to_map (1 thread: {t2 - t1:.2f} s)',
     }
+
+
+def small_config(torch, so3g_amd, N, workloads, dev, stream, reps=20):
+    """The small observation (100 det x 40000 samples, CAR 1 arcmin TQU): the
+    planned from_map + to_map pair, warm (everything resident in the 256 MB
+    Infinity Cache) and cold (a 1 GiB buffer streamed before each pass to
+    flush it).  Times are HIP-event times of the pair on the stream."""
+    lib = N.load()
+    obs = workloads.make('small')
+    eng = so3g_amd.ProjEng_CAR_TQU_NonTiled(obs.pixelizor_args())
+    ny, nx = obs.shape
+    n_det, n_t = obs.n_det, obs.n_samp
+    bore = torch.from_numpy(obs.bore).to(dev)
+    dets = torch.from_numpy(obs.dets).to(dev)
+    resp = torch.from_numpy(obs.resps).to(dev)
+    pix, spin = eng.pointing_matrix(bore, dets, resp)
+    plan = so3g_amd.PrecompPlan(pix, spin, (ny, nx))
+    gen = torch.Generator(device=dev).manual_seed(3)
+    sky = torch.randn((3, ny, nx), dtype=torch.float64, device=dev, generator=gen)
+    sig = torch.empty((n_det, n_t), dtype=torch.float32, device=dev)
+    m = torch.zeros((3, ny, nx), dtype=torch.float64, device=dev)
+    flush_a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    flush_b = torch.empty_like(flush_a)
+
+    def pair():
+        N.check(lib.so3g_pc_plan_from_map(plan._h, sky.data_ptr(), sig.data_ptr(), 0, stream))
+        m.zero_()
+        N.check(lib.so3g_pc_plan_to_map(plan._h, m.data_ptr(), sig.data_ptr(), None, stream))
+
+    def timed(cold):
+        ts = []
+        for _ in range(reps):
+            if cold:
+                flush_b.copy_(flush_a)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            pair()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+
+    pair()
+    torch.cuda.synchronize()
+    warm, cold = timed(False), timed(True)
+    n_s = n_det * n_t
+    npix = ny * nx
+    info = plan.info()
+    b = n_s * 16 + info['n_on'] * 16 + info['n_on'] // 8 + 4 * info['runs'] + 3 * 8 * npix * 3
+    b_pc = n_s * 24 * 2 + 3 * 8 * npix * 3
+    out = {'workload': f'{n_det} det x {n_t} samples, CAR 1 arcmin TQU {ny}x{nx}, planned '
+                       'from_map + to_map',
+           'ms_pair_warm': warm, 'ms_pair_cold': cold,
+           'det_samples_per_s_warm': n_s / (warm * 1e-3),
+           'det_samples_per_s_cold': n_s / (cold * 1e-3),
+           'hbm_frac_cold': b / (cold * 1e-3) / HBM_PEAK,
+           'hbm_frac_cold_precomp_equivalent': b_pc / (cold * 1e-3) / HBM_PEAK,
+           'median_of': reps}
+    del flush_a, flush_b
+    log(f'small config: warm {warm:.3f} ms, cold {cold:.3f} ms per pair')
+    return out
+
+
+def nmat_timing(torch, so3g_amd, dev, n_det, n_t, nvec=10, nbin=20, reps=5):
+    """N^-1 of the detector-vector noise model (nmat_detvecs_apply) on the
+    Fourier transform of the benchmark's time streams: ft (n_det, 2 (n_t/2+1))
+    float32, nbin log-spaced bins, nvec vectors.  Bytes: ft read twice and
+    written once."""
+    nfreq = n_t // 2 + 1
+    nmode = 2 * nfreq
+    edges = np.unique(np.round(np.geomspace(1, nfreq, nbin + 1)).astype(int))
+    edges[0] = 0
+    bins = np.stack([edges[:-1], edges[1:]], 1).astype(np.int32)
+    g = torch.Generator(device=dev).manual_seed(4)
+    ft = torch.randn((n_det, nmode), dtype=torch.float32, device=dev, generator=g)
+    iD = torch.rand((len(bins), n_det), device=dev, generator=g) + 0.5
+    iV = torch.randn((len(bins), n_det, nvec), device=dev, generator=g) / np.sqrt(n_det)
+    so3g_amd.nmat_detvecs_apply(ft, bins, iD, iV, 1.0, 1.0)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        so3g_amd.nmat_detvecs_apply(ft, bins, iD, iV, 1.0, 1.0)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    b = 3 * 4 * n_det * nmode
+    del ft
+    log(f'nmat_detvecs_apply: {ms:.3f} ms')
+    return {'shape': [n_det, nmode], 'nvec': nvec, 'nbin': len(bins), 'ms': ms,
+            'hbm_frac': b / (ms * 1e-3) / HBM_PEAK}
 
 
 def _stripe_segments(dom, n_thread):

@@ -286,6 +286,16 @@ int so3g_batch_tune(so3g_batch *b, int op, double *map, double *const *tiles,
 int so3g_batch_info(const so3g_batch *b, int64_t *info);
 int so3g_batch_destroy(so3g_batch *b);
 
+/* nmat_detvecs_apply (src/array_ops.cxx:39-88): in place on ft (ndet,
+ * nmode) float32 (device), the real view of a Fourier transform unless
+ * dct_binning; bins (nbin, 2) int32 on the HOST; iD (nbin, ndet) and iV
+ * (nbin, ndet, nvec) float32 on the device, nvec <= 64.  For each bin in
+ * order, columns [min(k b0, nmode - 1), min(k b1, nmode)) (k = 2, or 1 with
+ * dct_binning) become ft * (iD / norm) + (s / norm) iV (iV^T ft), float32. */
+int so3g_nmat_detvecs_apply(float *ft, int64_t ndet, int64_t nmode, const int32_t *bins,
+                            int64_t nbin, const float *iD, const float *iV, int64_t nvec,
+                            float s, float norm, int dct_binning, void *stream);
+
 /* Layers of the learned map-making preconditioner (no counterpart in the
  * reference).  Activations are bf16 NHWC (n, h, w, c) with c in {16, 32, 48,
  * 64}; conv weights bf16 (9, co, ci) (tap = 3 * (dy + 1) + (dx + 1)), bias

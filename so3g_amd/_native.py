@@ -88,6 +88,8 @@ _SIGS = {
                                 ctypes.POINTER(c_int), c_vp]),
     'so3g_batch_info': (c_int, [c_vp, c_vp]),
     'so3g_batch_destroy': (c_int, [c_vp]),
+    'so3g_nmat_detvecs_apply': (c_int, [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64,
+                                        ctypes.c_float, ctypes.c_float, c_int, c_vp]),
     'so3g_nn_conv3x3': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_int, c_int,
                                 c_vp, c_vp]),
     'so3g_nn_conv3x3_wgrad': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp,

@@ -1,0 +1,333 @@
+// nmat.hip -- the detector-vector noise model's inverse-covariance
+// application in Fourier space (nmat_detvecs_apply, src/array_ops.cxx:39-88),
+// the other operator of every map-making CG iteration.
+//
+// For each frequency bin b with (real-view) column range [c1, c2):
+//     Q  = iV_b^T ft[:, c1:c2]                       (nvec x nm)
+//     ft[:, c1:c2] = ft * (iD_b / norm) + (s / norm) * iV_b Q
+// in float32, bins in order (a column in two bins is transformed twice, as
+// in the reference when bin ends are clamped).
+//
+// ft is read twice and written once (3 x 4 B per element); Q, nvec rows of
+// nmode floats, and its per-chunk partials live in a library buffer that is
+// reused across calls (calls on one device share it: do not overlap calls on
+// different streams).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/so3g_hip.h"
+
+namespace so3g {
+int set_error(int code, const std::string &msg);
+}
+using namespace so3g;
+
+namespace {
+
+constexpr int kNT = 256;        // threads per block
+constexpr int kMaxVec = 64;
+
+// Two streaming passes, each thread owning C consecutive columns and
+// walking all detector rows, so that a block reads 256 C contiguous floats
+// of every row (whole DRAM pages; a narrow column strip over 1000 rows 2.9 MB
+// apart would open a page per 64 B):
+//   pass 1   Q[v][col] = sum_d iV[bin(col)][d][v] * ft[d][col]
+//   pass 2   ft[d][col] = ft * (iD[bin][d] / norm) + (s / norm) sum_v iV[bin][d][v] Q[v][col]
+// cbin[col] is the column's bin (-1: untouched).  NV >= nvec accumulators
+// per column stay in registers (NV x C <= 64).
+template <int NV, int C>
+__device__ __forceinline__ void load_cols(const float *row, int64_t col0, int nc, bool vec,
+                                          float (&x)[C]) {
+    if constexpr (C == 4) {
+        if (vec && nc == 4) {
+            const float4 f = *reinterpret_cast<const float4 *>(row + col0);
+            x[0] = f.x; x[1] = f.y; x[2] = f.z; x[3] = f.w;
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < C; j++) x[j] = j < nc ? row[col0 + j] : 0.f;
+}
+
+constexpr int kDChunk = 64;     // detectors per block (blockIdx.y)
+
+template <int NV, int C>
+__global__ __launch_bounds__(kNT) void k_nmat_q(const float *__restrict__ ft, int ndet,
+                                                int64_t nmode, const int32_t *__restrict__ cbin,
+                                                const float *__restrict__ iV, int nvec,
+                                                float *__restrict__ Qp) {
+    const int d0 = blockIdx.y * kDChunk, d1 = min(ndet, d0 + kDChunk);
+    float *Q = Qp + (int64_t)blockIdx.y * nvec * nmode;   // this chunk's partial
+    const int64_t col0 = ((int64_t)blockIdx.x * kNT + threadIdx.x) * C;
+    if (col0 >= nmode) return;
+    const int nc = (int)min((int64_t)C, nmode - col0);
+    int b[C];
+    bool same = true;
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+        b[j] = j < nc ? cbin[col0 + j] : -1;
+        same = same && b[j] == b[0];
+    }
+    if (same && b[0] < 0) return;
+    const bool vec = (nmode % 4) == 0;
+    // When the whole wave's columns share one bin (the common case), the
+    // iV row address is wave-uniform and its loads go through the scalar unit.
+    const int bw = __builtin_amdgcn_readfirstlane(b[0]);
+    const bool uni = __all(same && b[0] == bw);
+    float acc[NV][C];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+#pragma unroll
+        for (int j = 0; j < C; j++) acc[v][j] = 0.f;
+#pragma unroll 4
+    for (int d = d0; d < d1; d++) {
+        float x[C];
+        load_cols<NV, C>(ft + (int64_t)d * nmode, col0, nc, vec, x);
+        if (uni) {
+            const float *row = iV + ((int64_t)bw * ndet + d) * nvec;
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                if (v < nvec) {
+                    const float r = row[v];
+#pragma unroll
+                    for (int j = 0; j < C; j++) acc[v][j] += r * x[j];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < C; j++) {
+                if (b[j] < 0) continue;
+                const float *row = iV + ((int64_t)b[j] * ndet + d) * nvec;
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+                    if (v < nvec) acc[v][j] += row[v] * x[j];
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        if (v < nvec) {
+#pragma unroll
+            for (int j = 0; j < C; j++)
+                if (j < nc) Q[(int64_t)v * nmode + col0 + j] = acc[v][j];
+        }
+    }
+}
+
+// Q = sum over detector chunks of the partials, in chunk order.
+__global__ __launch_bounds__(kNT) void k_nmat_reduce(const float *__restrict__ Qp, int nchunk,
+                                                     int64_t n, const int32_t *__restrict__ cbin,
+                                                     int64_t nmode, float *__restrict__ Q) {
+    for (int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kNT) {
+        if (cbin[i % nmode] < 0) continue;
+        float acc = 0.f;
+        for (int k = 0; k < nchunk; k++) acc += Qp[(int64_t)k * n + i];
+        Q[i] = acc;
+    }
+}
+
+template <int NV, int C>
+__global__ __launch_bounds__(kNT) void k_nmat_update(float *__restrict__ ft, int ndet,
+                                                     int64_t nmode,
+                                                     const int32_t *__restrict__ cbin,
+                                                     const float *__restrict__ iD,
+                                                     const float *__restrict__ iV, int nvec,
+                                                     float s, float norm,
+                                                     const float *__restrict__ Q) {
+    const int64_t col0 = ((int64_t)blockIdx.x * kNT + threadIdx.x) * C;
+    if (col0 >= nmode) return;
+    const int nc = (int)min((int64_t)C, nmode - col0);
+    int b[C];
+    bool any = false, same = true;
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+        b[j] = j < nc ? cbin[col0 + j] : -1;
+        any = any || b[j] >= 0;
+        same = same && b[j] == b[0];
+    }
+    if (!any) return;
+    const int bw = __builtin_amdgcn_readfirstlane(b[0]);
+    const bool uni = __all(same && b[0] == bw && bw >= 0);
+    const bool vec = (nmode % 4) == 0;
+    float q[NV][C];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+#pragma unroll
+        for (int j = 0; j < C; j++)
+            q[v][j] = (v < nvec && j < nc) ? Q[(int64_t)v * nmode + col0 + j] : 0.f;
+    const float a = s / norm;
+    const int d0 = blockIdx.y * kDChunk, d1 = min(ndet, d0 + kDChunk);
+#pragma unroll 4
+    for (int d = d0; d < d1; d++) {
+        float *row = ft + (int64_t)d * nmode;
+        float x[C];
+        load_cols<NV, C>(row, col0, nc, vec, x);
+        float o[C];
+        if (uni) {
+            const float *vr = iV + ((int64_t)bw * ndet + d) * nvec;
+            const float dn = iD[(int64_t)bw * ndet + d] / norm;
+            float acc[C];
+#pragma unroll
+            for (int j = 0; j < C; j++) acc[j] = 0.f;
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                if (v < nvec) {
+                    const float r = vr[v];
+#pragma unroll
+                    for (int j = 0; j < C; j++) acc[j] += r * q[v][j];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < C; j++) o[j] = x[j] * dn + a * acc[j];
+        } else {
+#pragma unroll
+        for (int j = 0; j < C; j++) {
+            if (b[j] < 0) { o[j] = x[j]; continue; }
+            const float *vr = iV + ((int64_t)b[j] * ndet + d) * nvec;
+            float acc = 0.f;
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                if (v < nvec) acc += vr[v] * q[v][j];
+            o[j] = x[j] * (iD[(int64_t)b[j] * ndet + d] / norm) + a * acc;
+        }
+        }
+        if constexpr (C == 4) {
+            if (vec && nc == 4) {
+                *reinterpret_cast<float4 *>(row + col0) = float4{o[0], o[1], o[2], o[3]};
+                continue;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < C; j++)
+            if (j < nc && b[j] >= 0) row[col0 + j] = o[j];
+    }
+}
+
+struct NmCache {
+    std::vector<int32_t> key;
+    int64_t nmode = -1;
+    std::vector<int32_t *> dev;    // one column -> bin map per group
+};
+// Q and its per-chunk partials, grown as needed and kept (a CG loop calls
+// this every iteration with the same shapes).
+float *g_nm_q = nullptr;
+size_t g_nm_q_cap = 0;
+NmCache g_nm_cache;
+std::mutex g_nm_mutex;
+
+#define NM_TRY(expr)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return set_error(SO3G_ERR_HIP, std::string("HIP error in ") + #expr + ": " + \
+                                               hipGetErrorString(_e));                 \
+    } while (0)
+
+template <int NV, int C>
+void launch_pair(float *ft, int ndet, int64_t nmode, const int32_t *cbin, const float *iD,
+                 const float *iV, int nvec, float s, float norm, float *Qp, float *Q,
+                 hipStream_t st) {
+    const int nchunk = (ndet + kDChunk - 1) / kDChunk;
+    const dim3 g((unsigned)((nmode + (int64_t)kNT * C - 1) / ((int64_t)kNT * C)),
+                 (unsigned)nchunk);
+    hipLaunchKernelGGL((k_nmat_q<NV, C>), g, dim3(kNT), 0, st, ft, ndet, nmode, cbin, iV, nvec,
+                       Qp);
+    const int64_t n = (int64_t)nvec * nmode;
+    if (n > 0)
+        hipLaunchKernelGGL(k_nmat_reduce, dim3((unsigned)std::min<int64_t>((n + kNT - 1) / kNT, 65536)),
+                           dim3(kNT), 0, st, Qp, nchunk, n, cbin, nmode, Q);
+    hipLaunchKernelGGL((k_nmat_update<NV, C>), g, dim3(kNT), 0, st, ft, ndet, nmode, cbin, iD,
+                       iV, nvec, s, norm, Q);
+}
+
+}  // namespace
+
+extern "C" {
+
+int so3g_nmat_detvecs_apply(float *ft, int64_t ndet, int64_t nmode, const int32_t *bins,
+                            int64_t nbin, const float *iD, const float *iV, int64_t nvec,
+                            float s, float norm, int dct_binning, void *stream) {
+    if (ndet < 0 || nmode < 0 || nbin < 0 || nvec < 0)
+        return set_error(SO3G_ERR_RUNTIME, "negative dimension");
+    if (ndet == 0 || nmode == 0 || nbin == 0) return SO3G_OK;
+    if (nvec > kMaxVec) return set_error(SO3G_ERR_RUNTIME, "nvec above 64 is not supported");
+    if (ndet > INT32_MAX || nbin > INT32_MAX)
+        return set_error(SO3G_ERR_RUNTIME, "too many detectors or bins");
+    if (bins == nullptr) return set_error(SO3G_ERR_VALUE, "bins is NULL");
+    const int64_t scale = dct_binning ? 1 : 2;
+    hipStream_t st = (hipStream_t)stream;
+    // Column ranges as the reference clamps them (array_ops.cxx:65-66).  Bins
+    // run in order; consecutive bins whose ranges do not overlap form one
+    // group (one pass pair), a column's bin recorded in cbin.
+    std::vector<std::vector<int32_t>> groups;
+    std::vector<std::pair<int64_t, int64_t>> used;
+    for (int64_t b = 0; b < nbin; b++) {
+        const int64_t b1 = std::min(scale * (int64_t)bins[2 * b], nmode - 1);
+        const int64_t b2 = std::min(scale * (int64_t)bins[2 * b + 1], nmode);
+        if (b2 <= b1) continue;
+        bool clash = groups.empty();
+        for (auto &r : used)
+            if (b1 < r.second && r.first < b2) clash = true;
+        if (clash) {
+            groups.emplace_back(nmode, -1);
+            used.clear();
+        }
+        used.emplace_back(b1, b2);
+        std::fill(groups.back().begin() + b1, groups.back().begin() + b2, (int32_t)b);
+    }
+    if (groups.empty()) return SO3G_OK;
+    // The bin maps are uploaded once per (bins, nmode, binning): a CG loop
+    // calls this with the same bins every iteration.
+    std::lock_guard<std::mutex> lock(g_nm_mutex);
+    std::vector<int32_t> key(bins, bins + 2 * nbin);
+    key.push_back((int32_t)scale);
+    if (!(g_nm_cache.key == key && g_nm_cache.nmode == nmode)) {
+        for (int32_t *p : g_nm_cache.dev) (void)hipFree(p);
+        g_nm_cache = NmCache{};
+        for (auto &g : groups) {
+            int32_t *d = nullptr;
+            NM_TRY(hipMalloc((void **)&d, nmode * sizeof(int32_t)));
+            g_nm_cache.dev.push_back(d);
+            NM_TRY(hipMemcpy(d, g.data(), nmode * sizeof(int32_t), hipMemcpyHostToDevice));
+        }
+        g_nm_cache.key = key;
+        g_nm_cache.nmode = nmode;
+    }
+    const int64_t nchunk = (ndet + kDChunk - 1) / kDChunk;
+    const size_t need = (size_t)std::max<int64_t>(1, (nchunk + 1) * nvec * nmode);
+    if (need > g_nm_q_cap) {
+        // earlier calls on other streams may still read the old buffer
+        NM_TRY(hipDeviceSynchronize());
+        if (g_nm_q) (void)hipFree(g_nm_q);
+        g_nm_q = nullptr;
+        g_nm_q_cap = 0;
+        NM_TRY(hipMalloc((void **)&g_nm_q, need * sizeof(float)));
+        g_nm_q_cap = need;
+    }
+    float *Q = g_nm_q, *Qp = g_nm_q + nvec * nmode;
+    int rc = SO3G_OK;
+    const int nv = nvec <= 4 ? 4 : nvec <= 8 ? 8 : nvec <= 16 ? 16 : nvec <= 32 ? 32 : 64;
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        const int32_t *cbin = g_nm_cache.dev[gi];
+        const int nd = (int)ndet, nvi = (int)nvec;
+        switch (nv) {
+        case 4: launch_pair<4, 4>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, st); break;
+        case 8: launch_pair<8, 4>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, st); break;
+        case 16: launch_pair<16, 4>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, st); break;
+        case 32: launch_pair<32, 2>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, st); break;
+        default: launch_pair<64, 1>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, st); break;
+        }
+        if (hipGetLastError() != hipSuccess) {
+            rc = set_error(SO3G_ERR_HIP, "nmat kernel launch failed");
+            break;
+        }
+    }
+    return rc;
+}
+
+}  // extern "C"
