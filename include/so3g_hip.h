@@ -296,6 +296,38 @@ int so3g_nmat_detvecs_apply(float *ft, int64_t ndet, int64_t nmode, const int32_
                             int64_t nbin, const float *iD, const float *iV, int64_t nvec,
                             float s, float norm, int dct_binning, void *stream);
 
+/* ML cut handling: process_cuts (src/array_ops.cxx:130-183) and
+ * translate_cuts (:185-217).  A RangesMatrix is passed as a HOST segment
+ * list segs[n_seg][3] = (det, start, end) in its order (detector by
+ * detector, ranges ascending); tod (ndet, nsamp) and vals are device arrays
+ * of float32 (f64 = 0) or float64 (f64 = 1).  op: 0 insert (vals -> tod),
+ * 1 extract (tod -> vals), 2 clear; model: 0 "full" (one value per cut
+ * sample), 1 "poly" (Legendre coefficients per range, count from
+ * resolution and nmax, array_ops.cxx:233-240).  so3g_cuts_measure gives the
+ * value count ("measure").  Poly extract adds into vals for one-coefficient
+ * ranges, as the reference does.  translate maps values between sample
+ * rates (full: nearest / average; poly: copy, zero-pad, truncate). */
+int so3g_cuts_measure(const int32_t *segs, int64_t n_seg, int model, int resolution, int nmax,
+                      int64_t *n_vals);
+int so3g_process_cuts(const int32_t *segs, int64_t n_seg, int op, int model, int resolution,
+                      int nmax, void *tod, int64_t ndet, int64_t nsamp, void *vals,
+                      int64_t n_vals, int f64, void *stream);
+int so3g_translate_cuts(const int32_t *isegs, const int32_t *osegs, int64_t n_seg, int model,
+                        int resolution, int nmax, int64_t insamp, int64_t onsamp,
+                        const void *ivals, int64_t n_ivals, void *ovals, int64_t n_ovals,
+                        int f64, void *stream);
+
+/* Boresight quaternions on the device (no counterpart in the reference's
+ * C++; CelestialSightLine.for_horizon, python/proj/coords.py:190-209, for
+ * mode 0 and naive_az_el, :99-125, for mode 1).  az, el, t and roll are
+ * device arrays of n float64 (roll may be NULL: roll0 for every sample);
+ * site (host, mode 1): {era slope, era offset, era epoch (unix s), site lon
+ * (rad), site lat (rad)}; pre (host, 4 doubles, may be NULL) premultiplies
+ * every quaternion; out (device) is (n, 4). */
+int so3g_boresight(int mode, const double *t, const double *az, const double *el,
+                   const double *roll, double roll0, int64_t n, const double *site,
+                   const double *pre, double *out, void *stream);
+
 /* Layers of the learned map-making preconditioner (no counterpart in the
  * reference).  Activations are bf16 NHWC (n, h, w, c) with c in {16, 32, 48,
  * 64}; conv weights bf16 (9, co, ci) (tap = 3 * (dy + 1) + (dx + 1)), bias

@@ -343,3 +343,46 @@ def qmul(l, r):
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+# -- ML cut handling (cuts_oracle.cpp) -----------------------------------------
+
+CUT_OPS = {'insert': 0, 'extract': 1, 'clear': 2}
+CUT_MODELS = {'full': 0, 'poly': 1}
+
+
+def cut_segments(ranges):
+    """(n, 3) int32 (det, start, end) of a list of Ranges objects, in the
+    order the reference walks them."""
+    segs = []
+    for d, r in enumerate(ranges):
+        for a, b in np.asarray(r.ranges()).reshape(-1, 2):
+            segs.append((d, a, b))
+    return np.asarray(segs, np.int32).reshape(-1, 3)
+
+
+def cuts_measure(segs, model, resolution=0, nmax=0):
+    f = lib().oracle_cuts_measure
+    f.restype = ctypes.c_int64
+    s = _c(segs, np.int32)
+    return int(f(_p(s), ctypes.c_int64(len(s)), ctypes.c_int(CUT_MODELS[model]),
+                 ctypes.c_int(resolution), ctypes.c_int(nmax)))
+
+
+def process_cuts(segs, operation, model, tod, vals, resolution=0, nmax=0):
+    """In place on tod (insert, clear) or vals (extract); float32 or float64."""
+    s = _c(segs, np.int32)
+    assert tod.flags['C_CONTIGUOUS'] and vals.flags['C_CONTIGUOUS']
+    lib().oracle_process_cuts(_p(s), ctypes.c_int64(len(s)), ctypes.c_int(CUT_OPS[operation]),
+                              ctypes.c_int(CUT_MODELS[model]), ctypes.c_int(resolution),
+                              ctypes.c_int(nmax), _p(tod), ctypes.c_int64(tod.shape[1]),
+                              _p(vals), ctypes.c_int(int(tod.dtype == np.float64)))
+
+
+def translate_cuts(isegs, osegs, model, insamp, onsamp, ivals, ovals, resolution=0, nmax=0):
+    i, o = _c(isegs, np.int32), _c(osegs, np.int32)
+    lib().oracle_translate_cuts(_p(i), _p(o), ctypes.c_int64(len(i)),
+                                ctypes.c_int(CUT_MODELS[model]), ctypes.c_int(resolution),
+                                ctypes.c_int(nmax), ctypes.c_int64(insamp),
+                                ctypes.c_int64(onsamp), _p(ivals), _p(ovals),
+                                ctypes.c_int(int(ivals.dtype == np.float64)))

@@ -90,6 +90,13 @@ _SIGS = {
     'so3g_batch_destroy': (c_int, [c_vp]),
     'so3g_nmat_detvecs_apply': (c_int, [c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64,
                                         ctypes.c_float, ctypes.c_float, c_int, c_vp]),
+    'so3g_boresight': (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, c_i64, c_vp,
+                               c_vp, c_vp, c_vp]),
+    'so3g_cuts_measure': (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_vp]),
+    'so3g_process_cuts': (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_i64, c_i64,
+                                  c_vp, c_i64, c_int, c_vp]),
+    'so3g_translate_cuts': (c_int, [c_vp, c_vp, c_i64, c_int, c_int, c_int, c_i64, c_i64, c_vp,
+                                    c_i64, c_vp, c_i64, c_int, c_vp]),
     'so3g_nn_conv3x3': (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_int, c_int,
                                 c_vp, c_vp]),
     'so3g_nn_conv3x3_wgrad': (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp,
