@@ -40,7 +40,6 @@ def draw(n_obs, seed):
 def build(n_obs=64, seed=4, res_arcmin=0.5, tile=100, disk_deg=1.5, throw=10.):
     import torch
     from .. import workloads
-    from ..proj.coords import CelestialSightLine, Assembly
     from ..proj.wcs import Projectionist
     from ..proj.wcslite import car_geometry
     n_det, n_samp, az0, el = draw(n_obs, seed)
@@ -55,14 +54,14 @@ def build(n_obs=64, seed=4, res_arcmin=0.5, tile=100, disk_deg=1.5, throw=10.):
     gen = torch.Generator(device=dev).manual_seed(seed)
     items = []
     hits = np.zeros(p.tiling.tile_count, np.int64)
+    from ..boresight import horizon
+    qc2n = getattr(p, 'q_celestial_to_native', None)
     for i in range(n_obs):
         t, az, e = workloads.scan(int(n_samp[i]), el_deg=el[i], az0_deg=az0[i],
                                   throw_deg=throw)
-        sight = CelestialSightLine.for_horizon(t, az, e)
         fp = workloads.focal_plane(int(n_det[i]), disk_deg, seed=seed * 1000 + i)
-        asm = Assembly.attach(sight, fp)
-        q = torch.from_numpy(np.ascontiguousarray(
-            np.asarray(p._cache_q_fp_to_native(asm.Q)), np.float64)).to(dev)
+        # native-frame boresight made on the GPU (for_horizon, premultiplied)
+        q = horizon(az, e, q_pre=None if qc2n is None else np.asarray(qc2n))
         dets = torch.from_numpy(np.ascontiguousarray(np.asarray(fp.quats), np.float64)).to(dev)
         resp = torch.from_numpy(np.ascontiguousarray(fp.resps, np.float32)).to(dev)
         hits += np.asarray(eng_all.tile_hits(q, dets), np.int64)
