@@ -368,7 +368,15 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   reduced after the kernel (default 0)
  *   "scatter_lds"   1: LDS-privatised partial maps for the cached (Precomp)
  *                   non-tiled scatter (default 0: measured slower, 12.8 vs 10.9 ms)
- *   "xcd_budget_mb" device memory allowed for those copies (default 4096) */
+ *   "xcd_budget_mb" device memory allowed for those copies (default 4096)
+ *   "pc_from_variant", "pc_from_dpb"  Precomp gather variant (default 2)
+ *                   and detectors per block (default 2)
+ *   "plan_div"      plan kernels' detector index by float reciprocal (1,
+ *                   default) or integer division (0)
+ *   "plan_from_v"   samples per lane of the planned gather (4 default, 8)
+ *   "plan_packed"   planned scatter into a pixel-major scratch map, added to
+ *                   the map afterwards (default 0: measured slower, 3.79 vs
+ *                   3.51 ms) */
 int so3g_set_option(const char *name, int64_t value);
 int so3g_get_option(const char *name, int64_t *value);
 

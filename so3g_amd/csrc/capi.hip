@@ -18,6 +18,7 @@
 namespace so3g {
 extern int g_plan_div;
 extern int g_plan_from_v;
+extern int g_plan_packed;
 extern template int dispatch<PROJ_FLAT>(const OpArgs &);
 extern template int dispatch<PROJ_QUAT>(const OpArgs &);
 extern template int dispatch<PROJ_CAR>(const OpArgs &);
@@ -736,6 +737,7 @@ int so3g_set_option(const char *name, int64_t value) {
     else if (n == "pc_from_dpb") g_opt.pc_from_dpb = (int)std::max<int64_t>(1, value);
     else if (n == "xcd_budget_mb") g_opt.xcd_budget_mb = value;
     else if (n == "plan_div") so3g::g_plan_div = (int)(value != 0);
+    else if (n == "plan_packed") so3g::g_plan_packed = (int)(value != 0);
     else if (n == "plan_from_v") {
         if (value != 4 && value != 8) return fail(SO3G_ERR_VALUE, "plan_from_v must be 4 or 8");
         so3g::g_plan_from_v = (int)value;
@@ -752,6 +754,7 @@ int so3g_get_option(const char *name, int64_t *value) {
     else if (n == "pc_from_dpb") *value = g_opt.pc_from_dpb;
     else if (n == "xcd_budget_mb") *value = g_opt.xcd_budget_mb;
     else if (n == "plan_div") *value = so3g::g_plan_div;
+    else if (n == "plan_packed") *value = so3g::g_plan_packed;
     else if (n == "plan_from_v") *value = so3g::g_plan_from_v;
     else return fail(SO3G_ERR_VALUE, "unknown option " + n);
     return SO3G_OK;

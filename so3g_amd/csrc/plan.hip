@@ -81,6 +81,7 @@ constexpr int kPlanBlock = 256;
 // planned from_map (4 or 8, when the sample count allows).
 int g_plan_div = 1;
 int g_plan_from_v = 4;
+int g_plan_packed = 0;   // "plan_packed": scatter into a pixel-major scratch map
 constexpr int kPlanR = 8;                 // sorted samples per lane
 constexpr int kGroup = 64 * kPlanR;       // samples per wave = per head group
 
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_runs(const int32_t *key, in
 // earlier lanes and goes through the wave-level run aggregation; runs that
 // start inside the lane and end inside it, and its last run when it holds
 // more than one, are added to the map directly.
-template <int NC, bool TC, typename ST, int DIV>
+template <int NC, bool TC, typename ST, int DIV, bool PKD>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_to_map_c(
     const uint32_t *__restrict__ idx, const float *__restrict__ sspin,
     const float *__restrict__ tval, const uint8_t *__restrict__ heads,
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_to_map_c(
                 } else {
 #pragma unroll
                     for (int c = 0; c < NC; c++)
-                        unsafeAtomicAdd(map + (int64_t)c * plane + cpix, cv[c]);
+                        unsafeAtomicAdd(map + (PKD ? 4 * (int64_t)cpix + c : (int64_t)c * plane + cpix), cv[c]);
                 }
             }
             ck = rid;
@@ -345,12 +346,32 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_to_map_c(
         for (int c = 0; c < NC; c++) hv[c] = cv[c];
     } else if (ck >= 0) {
 #pragma unroll
-        for (int c = 0; c < NC; c++) unsafeAtomicAdd(map + (int64_t)c * plane + cpix, cv[c]);
+        for (int c = 0; c < NC; c++)
+            unsafeAtomicAdd(map + (PKD ? 4 * (int64_t)cpix + c : (int64_t)c * plane + cpix), cv[c]);
         hpix = hk >= 0 ? run_pix[hk] : 0u;
     }
     if (wave_runs<NC>(hk, hv) && hk >= 0) {
 #pragma unroll
-        for (int c = 0; c < NC; c++) unsafeAtomicAdd(map + (int64_t)c * plane + hpix, hv[c]);
+        for (int c = 0; c < NC; c++)
+            unsafeAtomicAdd(map + (PKD ? 4 * (int64_t)hpix + c : (int64_t)c * plane + hpix), hv[c]);
+    }
+}
+
+// Pixel-major scatter target: zeroed before, added into the (NC, plane) map
+// after (one pass over the map each).
+__global__ __launch_bounds__(kPlanBlock) void k_plan_zero(double *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kPlanBlock)
+        p[i] = 0.;
+}
+
+template <int NC>
+__global__ __launch_bounds__(kPlanBlock) void k_plan_unpack_add(const double *packed,
+                                                                int64_t plane, double *map) {
+    for (int64_t p = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; p < plane;
+         p += (int64_t)gridDim.x * kPlanBlock) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) map[c * plane + p] += packed[4 * p + c];
     }
 }
 
@@ -480,16 +501,24 @@ int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
     const dim3 g(grid_of(p->n_on, (int64_t)kPlanBlock * kPlanR, INT32_MAX));
     const uint32_t plane = (uint32_t)(p->ny * p->nx);
     const FastDiv nt{(uint32_t)p->n_t, 1.f / (float)p->n_t};
+    const bool pkd = g_plan_packed != 0;
+    double *target = pkd ? p->packed : map;
+    if (pkd) hipLaunchKernelGGL(k_plan_zero, dim3(grid_of(4 * (int64_t)plane)), dim3(kPlanBlock),
+                                0, s, p->packed, 4 * (int64_t)plane);
 #define PLAN_TO(NC, TC)                                                                     \
     do {                                                                                    \
-        if (g_plan_div)                                                                     \
-            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 1>), g, dim3(kPlanBlock), 0, s, \
-                               p->idx, p->sspin, p->tval, p->heads, p->wbase, p->run_pix,  \
-                               p->n_on, nt, plane, signal, det_weights, map);              \
+        if (pkd)                                                                            \
+            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 1, true>), g, dim3(kPlanBlock), \
+                               0, s, p->idx, p->sspin, p->tval, p->heads, p->wbase,         \
+                               p->run_pix, p->n_on, nt, plane, signal, det_weights, target); \
+        else if (g_plan_div)                                                                \
+            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 1, false>), g, dim3(kPlanBlock),\
+                               0, s, p->idx, p->sspin, p->tval, p->heads, p->wbase,         \
+                               p->run_pix, p->n_on, nt, plane, signal, det_weights, map);   \
         else                                                                                \
-            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 0>), g, dim3(kPlanBlock), 0, s, \
-                               p->idx, p->sspin, p->tval, p->heads, p->wbase, p->run_pix,  \
-                               p->n_on, nt, plane, signal, det_weights, map);              \
+            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 0, false>), g, dim3(kPlanBlock),\
+                               0, s, p->idx, p->sspin, p->tval, p->heads, p->wbase,         \
+                               p->run_pix, p->n_on, nt, plane, signal, det_weights, map);   \
     } while (0)
     if (p->tconst) {
         if (p->nc == 1) PLAN_TO(1, true);
@@ -500,6 +529,12 @@ int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
         else PLAN_TO(3, false);
     }
 #undef PLAN_TO
+    if (pkd) {
+        const dim3 gu(grid_of(plane));
+        if (p->nc == 1) hipLaunchKernelGGL(k_plan_unpack_add<1>, gu, dim3(kPlanBlock), 0, s, p->packed, (int64_t)plane, map);
+        else if (p->nc == 2) hipLaunchKernelGGL(k_plan_unpack_add<2>, gu, dim3(kPlanBlock), 0, s, p->packed, (int64_t)plane, map);
+        else hipLaunchKernelGGL(k_plan_unpack_add<3>, gu, dim3(kPlanBlock), 0, s, p->packed, (int64_t)plane, map);
+    }
     PLAN_TRY(hipGetLastError());
     return SO3G_OK;
 }
