@@ -160,6 +160,70 @@ class BinnedPreconditioner:
         return out
 
 
+class TwoLevelPreconditioner:
+    """Additive two-level preconditioner: M^-1 r = M_b r + Z A_c^-1 Z^T r.
+
+    Z spans, on every map row and component, hat functions along the row
+    (the scan direction of a constant-elevation scan) with knots every h
+    pixels: the large-scale along-scan structure that 1/f weighting leaves
+    poorly conditioned and a per-pixel preconditioner cannot reach.  The
+    coarse operator A_c = Z^T A Z is probed: one application of A per
+    (component, knot, row colour), on every row of the colour at once (rows
+    k = colour mod colors); it is exact when rows within colors - 1 of each
+    other do not couple (one row per detector for a constant-elevation scan
+    in a CAR map).  A_c is block-diagonal by row, each block regularised by
+    ridge x its diagonal and inverted on the host in float64."""
+
+    def __init__(self, apply_A, binned, map_shape, h=16, colors=1, ridge=1e-8, device=None):
+        nc, ny, nx = map_shape
+        dev = device or torch.device('cuda')
+        knots = np.arange(0, nx - 1 + h, h, dtype=np.float64)
+        knots[-1] = min(knots[-1], nx - 1)
+        knots = np.unique(knots)
+        nk = len(knots)
+        x = np.arange(nx, dtype=np.float64)
+        H = np.zeros((nx, nk))
+        for k in range(nk):
+            lo = knots[k - 1] if k > 0 else knots[k]
+            hi = knots[k + 1] if k < nk - 1 else knots[k]
+            left = (x >= lo) & (x <= knots[k])
+            right = (x >= knots[k]) & (x <= hi)
+            if knots[k] > lo:
+                H[left, k] = (x[left] - lo) / (knots[k] - lo)
+            if hi > knots[k]:
+                H[right, k] = (hi - x[right]) / (hi - knots[k])
+            H[int(knots[k]), k] = 1.
+        self.H = torch.from_numpy(H).to(dev)                       # (nx, nk)
+        m = nc * nk
+        Ac = np.zeros((ny, m, m))
+        e = torch.zeros(map_shape, dtype=torch.float64, device=dev)
+        for col in range(colors):
+            rows = torch.arange(col, ny, colors, device=dev)
+            rows_h = rows.cpu().numpy()
+            for c in range(nc):
+                for k in range(nk):
+                    e.zero_()
+                    e[c, rows, :] = self.H[:, k]
+                    y = apply_A(e)                                   # (nc, ny, nx)
+                    proj = torch.einsum('cyx,xk->yck', y[:, rows, :], self.H)  # (nr, nc, nk)
+                    Ac[rows_h, :, c * nk + k] = proj.reshape(len(rows_h), m).cpu().numpy()
+        Ac = 0.5 * (Ac + Ac.transpose(0, 2, 1))
+        d = np.diagonal(Ac, axis1=1, axis2=2)
+        reg = np.where(d > 0, ridge * d, 1.)
+        inv = np.linalg.inv(Ac + np.einsum('ij,jk->ijk', reg, np.eye(m)))
+        self.inv = torch.from_numpy(inv).to(dev)                   # (ny, m, m)
+        self.M = binned
+        self.shape = map_shape
+        self.nk = nk
+
+    def __call__(self, r):
+        nc, ny, nx = self.shape
+        zc = torch.einsum('cyx,xk->yck', r, self.H).reshape(ny, nc * self.nk, 1)
+        vc = torch.bmm(self.inv, zc).reshape(ny, nc, self.nk)
+        coarse = torch.einsum('yck,xk->cyx', vc, self.H)
+        return self.M(r) + coarse
+
+
 def _dot(a, b):
     return float(torch.sum(a * b))
 
@@ -285,12 +349,12 @@ def features(mr):
 
 
 class LearnedPreconditioner:
-    """z = M_b r + s * U(features(M_b r)), s = rms(M_b r): the binned
-    preconditioner plus a U-Net correction trained on (A e, e) pairs.  Not a
-    fixed linear map: use with flexible CG."""
+    """z = M r + s * U(features(M r)), s = rms(M r): a base preconditioner M
+    (binned or two-level) plus a U-Net correction trained on (A e, e) pairs.
+    Not a fixed linear map: use with flexible CG."""
 
-    def __init__(self, binned, net, map_shape):
-        self.M = binned
+    def __init__(self, base, net, map_shape):
+        self.M = base
         self.net = net
         self.shape = map_shape      # (nc, ny, nx)
 
