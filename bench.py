@@ -241,9 +241,9 @@ def main():
     gbs_to = b_to / (ms_to * 1e-3)
     tr = measured_traffic() or {}
     if ms_to >= ms_from:
-        dom = ('k_plan_to_map_c<3,true,float,1>', gbs_to, b_to, tr.get('to_map_bytes'))
+        dom = ('k_plan_to_map_c<3,true,float,1,false,true>', gbs_to, b_to, tr.get('to_map_bytes'))
     else:
-        dom = ('k_plan_from_map<3,true,4,1>', gbs_from, b_from, tr.get('from_map_bytes'))
+        dom = ('k_plan_from_map_s<3,true,4,1>', gbs_from, b_from, tr.get('from_map_bytes'))
     # The plain ProjEng_Precomp kernels (no plan), for reference.
     _, (ms_from_direct, ms_to_direct) = run([pc_from_direct, pc_to_direct],
                                             max(1, min(args.steps, 3)), 1)

@@ -374,6 +374,14 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *   "plan_div"      plan kernels' detector index by float reciprocal (1,
  *                   default) or integer division (0)
  *   "plan_from_v"   samples per lane of the planned gather (4 default, 8)
+ *   "plan_to_lds"   1 (default): the planned scatter stages the per-detector
+ *                   tables (T factor, weights) in LDS when they fit in 32 KB
+ *   "plan_to_cpb"   2048-sample chunks per block of the planned scatter
+ *                   (default 2, 1..64)
+ *   "plan_from_layout"  1 (default): a lane's gather samples are 64 apart, so
+ *                   one gather instruction covers 64 consecutive samples;
+ *                   0: consecutive samples per lane (measured slower, 3.51 vs
+ *                   3.05 ms for pack + gather)
  *   "plan_packed"   planned scatter into a pixel-major scratch map, added to
  *                   the map afterwards (default 0: measured slower, 3.79 vs
  *                   3.51 ms) */

@@ -19,6 +19,9 @@ namespace so3g {
 extern int g_plan_div;
 extern int g_plan_from_v;
 extern int g_plan_packed;
+extern int g_plan_from_layout;
+extern int g_plan_to_lds;
+extern int g_plan_to_cpb;
 extern template int dispatch<PROJ_FLAT>(const OpArgs &);
 extern template int dispatch<PROJ_QUAT>(const OpArgs &);
 extern template int dispatch<PROJ_CAR>(const OpArgs &);
@@ -738,6 +741,12 @@ int so3g_set_option(const char *name, int64_t value) {
     else if (n == "xcd_budget_mb") g_opt.xcd_budget_mb = value;
     else if (n == "plan_div") so3g::g_plan_div = (int)(value != 0);
     else if (n == "plan_packed") so3g::g_plan_packed = (int)(value != 0);
+    else if (n == "plan_from_layout") so3g::g_plan_from_layout = (int)(value != 0);
+    else if (n == "plan_to_lds") so3g::g_plan_to_lds = (int)(value != 0);
+    else if (n == "plan_to_cpb") {
+        if (value < 1 || value > 64) return fail(SO3G_ERR_VALUE, "plan_to_cpb must be 1..64");
+        so3g::g_plan_to_cpb = (int)value;
+    }
     else if (n == "plan_from_v") {
         if (value != 4 && value != 8) return fail(SO3G_ERR_VALUE, "plan_from_v must be 4 or 8");
         so3g::g_plan_from_v = (int)value;
@@ -755,6 +764,9 @@ int so3g_get_option(const char *name, int64_t *value) {
     else if (n == "xcd_budget_mb") *value = g_opt.xcd_budget_mb;
     else if (n == "plan_div") *value = so3g::g_plan_div;
     else if (n == "plan_packed") *value = so3g::g_plan_packed;
+    else if (n == "plan_from_layout") *value = so3g::g_plan_from_layout;
+    else if (n == "plan_to_lds") *value = so3g::g_plan_to_lds;
+    else if (n == "plan_to_cpb") *value = so3g::g_plan_to_cpb;
     else if (n == "plan_from_v") *value = so3g::g_plan_from_v;
     else return fail(SO3G_ERR_VALUE, "unknown option " + n);
     return SO3G_OK;

@@ -383,17 +383,39 @@ def test_precomp_plan(gpu, n_det, n_samp, comps, tconst, t_block_mb):
     assert info['runs'] == len(np.unique(keys))
     dw = c.det_weights.copy()
     dw[3] = 0.
-    for w in (None, dw):
-        g = plan.to_map(np.zeros((nc, 100, 300)), c.signal, w)
-        r = oracle.precomp_to_map(c.og, np.zeros((nc, 100, 300)), pix, spin, c.signal, w)
-        maps_close(g, r, 'plan to_map', atol=1e-9)
+    from so3g_amd import _native
+    names = ('plan_to_lds', 'plan_to_cpb')
+    old = [_native.get_option(n) for n in names]
+    try:
+        # detector tables in LDS (default) or read from memory; 1..3 chunks per block
+        for lds, cpb in ((1, 2), (0, 1), (1, 3)):
+            _native.set_option('plan_to_lds', lds)
+            _native.set_option('plan_to_cpb', cpb)
+            for w in (None, dw):
+                g = plan.to_map(np.zeros((nc, 100, 300)), c.signal, w)
+                r = oracle.precomp_to_map(c.og, np.zeros((nc, 100, 300)), pix, spin, c.signal,
+                                          w)
+                maps_close(g, r, f'plan to_map lds={lds} cpb={cpb}', atol=1e-9)
+    finally:
+        for n, v in zip(names, old):
+            _native.set_option(n, v)
     m = c.map_like([nc])
     ref = oracle.precomp_from_map(c.og, m, pix, spin)
-    assert_bit_equal(np.array(plan.from_map(m)), ref, 'plan from_map')
     s0 = np.array(c.signal)
-    acc = plan.from_map(m, s0.copy())
-    assert_bit_equal(np.array(acc), oracle.precomp_from_map(c.og, m, pix, spin, s0.copy()),
-                     'plan from_map accumulate')
+    ref_acc = oracle.precomp_from_map(c.og, m, pix, spin, s0.copy())
+    names = ('plan_from_layout', 'plan_from_v')
+    old = [_native.get_option(n) for n in names]
+    try:
+        # lane-strided (default) and consecutive gathers, 4 and 8 samples per lane
+        for lay, v in ((1, 4), (1, 8), (0, 4), (0, 8)):
+            _native.set_option('plan_from_layout', lay)
+            _native.set_option('plan_from_v', v)
+            tag = f'plan from_map layout={lay} v={v}'
+            assert_bit_equal(np.array(plan.from_map(m)), ref, tag)
+            assert_bit_equal(np.array(plan.from_map(m, s0.copy())), ref_acc, tag + ' accumulate')
+    finally:
+        for n, v in zip(names, old):
+            _native.set_option(n, v)
     # float64 signal variant: the same sums with float64 products
     import torch
     dev = torch.device('cuda')
