@@ -328,6 +328,60 @@ int so3g_boresight(int mode, const double *t, const double *az, const double *el
                    const double *roll, double roll0, int64_t n, const double *site,
                    const double *pre, double *out, void *stream);
 
+/* G3SuperTimestream array blobs (src/G3SuperTimestream.cxx).
+ *
+ * so3g_g3s_split replaces the quantise-and-rebranch half of encode_array
+ * (G3SuperTimestream.cxx:72-92 rebranch, :218-231): per channel, float data
+ * are divided by the quantum and rounded (roundf of the float-converted
+ * quotient for float32, round for float64), then each value v is split as
+ * v = r + y, r a multiple of 2^24 changing only when v leaves its multiple,
+ * y in [-2^23, 2^23).  type: 0 int32, 1 int64, 2 float32, 3 float64
+ * (host arrays, (n_det, n)); y int32, r int64 (n_det, n); changes[d] = 1
+ * when r is constant over the channel.
+ *
+ * so3g_flac_encode replaces the libFLAC stream encoder call of encode_array
+ * (:236-260: mono, bps bits per sample, compression level 0..8): writes a
+ * FLAC stream of the n host samples x into out (capacity bytes; out NULL
+ * or capacity < 0 to size only) and its length into n_bytes.  Returns
+ * SO3G_ERR_VALUE if the stream would exceed capacity, as libFLAC's write
+ * callback refuses (:99-104).
+ *
+ * so3g_g3s_decode replaces Extract (:664-812) and Decode (:636-662): the
+ * host has walked the blob's per-channel blocks into chans (one entry per
+ * extracted channel); payload (device, 16-byte aligned) holds the blob;
+ * offsets (device, n_offset_rows x n_samp, full-width integers of the
+ * array type) the inflated bz2 blocks; dest (device) dest_rows rows of
+ * dest_stride elements receives samples [start, stop) of each channel in
+ * row dest_row.  Channels with neither FLAC nor raw data add to dest's
+ * contents, as Extract does.  Synchronises the stream. */
+typedef struct {
+    int64_t flac_pos, flac_end;  /* first FLAC frame and end of the FLAC block; -1: none */
+    int64_t raw_pos;             /* ALGO_NONE: payload offset of the raw data; -1: none */
+    int64_t off_row;             /* row of offsets added to the channel; -1: none */
+    int64_t const_bits;          /* CONST datum (full-width two's complement) */
+    double quantum;              /* float types: the channel's quantum */
+    int64_t dest_row;
+    int32_t block;               /* FLAC: STREAMINFO maximum block size */
+    int32_t bps;                 /* FLAC: STREAMINFO bits per sample */
+    int32_t has_const;
+    int32_t pad_;
+} so3g_g3s_chan;
+int so3g_g3s_split(const void *data, int32_t type, int64_t n_det, int64_t n,
+                   const double *quanta, int32_t *y, int64_t *r, int32_t *changes);
+int so3g_flac_encode(const int32_t *x, int64_t n, int32_t bps, int32_t level, uint8_t *out,
+                     int64_t capacity, int64_t *n_bytes);
+int so3g_g3s_decode(const uint8_t *payload, int64_t payload_bytes, const so3g_g3s_chan *chans,
+                    int64_t n_chan, int32_t type, int64_t n_samp, int64_t start, int64_t stop,
+                    const void *offsets, int64_t n_offset_rows, void *dest, int64_t dest_rows,
+                    int64_t dest_stride, void *stream);
+/* Diagnostics of the last so3g_g3s_decode call (no counterpart in the
+ * reference): out[0] = FLAC frames decoded frame-parallel, out[1] = channels
+ * decoded again frame after frame, out[2] = why, OR of 1 variable block
+ * size, 2 a frame without sync, 4 a malformed frame, 8 a frame not ending
+ * where the next begins (false sync), 16 an unexpected frame number or
+ * block size. */
+int so3g_g3s_stats(int64_t *out);
+
 /* Layers of the learned map-making preconditioner (no counterpart in the
  * reference).  Activations are bf16 NHWC (n, h, w, c) with c in {16, 32, 48,
  * 64}; conv weights bf16 (9, co, ci) (tap = 3 * (dy + 1) + (dx + 1)), bias

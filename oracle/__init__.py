@@ -386,3 +386,102 @@ def translate_cuts(isegs, osegs, model, insamp, onsamp, ivals, ovals, resolution
                                 ctypes.c_int(nmax), ctypes.c_int64(insamp),
                                 ctypes.c_int64(onsamp), _p(ivals), _p(ovals),
                                 ctypes.c_int(int(ivals.dtype == np.float64)))
+
+
+# -- G3SuperTimestream decoding (g3super_oracle.cpp) ---------------------------
+
+G3S_TYPES = {'int32': 0, 'int64': 1, 'float32': 2, 'float64': 3}
+G3S_FLAC, G3S_BZ, G3S_CONST = 1, 2, 4
+
+
+def crc8(data):
+    b = np.frombuffer(bytes(data), np.uint8)
+    f = lib().oracle_crc8
+    f.restype = ctypes.c_uint8
+    return int(f(_p(b), ctypes.c_int64(len(b))))
+
+
+def crc16(data):
+    b = np.frombuffer(bytes(data), np.uint8)
+    f = lib().oracle_crc16
+    f.restype = ctypes.c_uint16
+    return int(f(_p(b), ctypes.c_int64(len(b))))
+
+
+def flac_decode(stream, cap):
+    """int32 samples of one mono FLAC stream (bytes); raises ValueError with
+    the oracle's status on a malformed stream."""
+    b = np.frombuffer(bytes(stream), np.uint8)
+    out = np.zeros(max(int(cap), 1), np.int32)
+    f = lib().oracle_flac_decode
+    f.restype = ctypes.c_int64
+    n = int(f(_p(b), ctypes.c_int64(len(b)), _p(out), ctypes.c_int64(cap)))
+    if n < 0:
+        raise ValueError(f'FLAC stream rejected by the oracle (status {n})')
+    return out[:n]
+
+
+def g3s_split(data, quanta=None):
+    """(y int32, r int64, branches int32) per channel: the quantised data
+    (float types) split into a 24-bit part and a multiple of 2^24, as
+    G3SuperTimestream.cxx:72-92, 218-231."""
+    data = np.ascontiguousarray(data)
+    n_det, n = data.shape
+    q = np.ascontiguousarray(np.ones(n_det) if quanta is None else quanta, np.float64)
+    y = np.zeros((n_det, n), np.int32)
+    r = np.zeros((n_det, n), np.int64)
+    br = np.zeros(n_det, np.int32)
+    lib().oracle_g3s_split(_p(data), ctypes.c_int(G3S_TYPES[data.dtype.name]),
+                           ctypes.c_int64(n_det), ctypes.c_int64(n), _p(q), _p(y), _p(r),
+                           _p(br))
+    return y, r, br
+
+
+def g3s_extract(payload, offsets, dtype, n_samp, quanta, dest, dest_indices=None,
+                src_indices=None, start=0, stop=None):
+    """Restatement of G3SuperTimestream::Extract (G3SuperTimestream.cxx:664-812)
+    over an encoded array blob: per extracted channel the algo byte, FLAC
+    block (oracle decoder), bz2 offsets (Python's bz2 = libbz2), CONST datum,
+    then the float conversion.  Writes dest in place (numpy)."""
+    import bz2
+    dtype = np.dtype(dtype)
+    payload = bytes(payload)
+    n_det = len(offsets) - 1
+    stop = n_samp if stop is None or stop < 0 else stop
+    wide = dtype.itemsize == 8
+    itype = np.int64 if wide else np.int32
+    if src_indices is not None:
+        n_ex = len(src_indices)
+    else:
+        n_ex = n_det
+    t = G3S_TYPES[dtype.name]
+    for i in range(n_ex):
+        si = int(src_indices[i]) if src_indices is not None else i
+        if si < 0 or si >= n_det:
+            continue
+        di = int(dest_indices[i]) if dest_indices is not None else i
+        if di < 0 or di >= dest.shape[0]:
+            continue
+        at = offsets[si]
+        algo = np.frombuffer(payload[at:at + 1], np.int8)[0]
+        at += 1
+        flac = raw = offs = cval = None
+        if algo == 0:
+            raw = np.frombuffer(payload[at:at + n_samp * dtype.itemsize], itype).copy()
+        if algo & G3S_FLAC:
+            n = int(np.frombuffer(payload[at:at + 4], np.int32)[0])
+            flac = flac_decode(payload[at + 4:at + 4 + n], n_samp)
+            at += 4 + n
+        if algo & G3S_BZ:
+            n = int(np.frombuffer(payload[at:at + 4], np.int32)[0])
+            offs = np.frombuffer(bz2.decompress(payload[at + 4:at + 4 + n]), itype).copy()
+            at += 4 + n
+        if algo & G3S_CONST:
+            cval = np.frombuffer(payload[at:at + dtype.itemsize], itype).copy()
+        row = np.ascontiguousarray(dest[di])
+        lib().oracle_g3s_combine(ctypes.c_int(t), ctypes.c_int64(start), ctypes.c_int64(stop),
+                                 _p(flac), _p(raw), _p(offs), _p(cval),
+                                 ctypes.c_double(float(quanta[si]) if quanta is not None
+                                                 else 1.0), _p(row))
+        dest[di] = row
+    return dest

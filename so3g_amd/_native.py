@@ -93,6 +93,11 @@ _SIGS = {
     'so3g_boresight': (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, c_i64, c_vp,
                                c_vp, c_vp, c_vp]),
     'so3g_cuts_measure': (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_vp]),
+    'so3g_flac_encode': (c_int, [c_vp, c_i64, c_int, c_int, c_vp, c_i64, c_vp]),
+    'so3g_g3s_split': (c_int, [c_vp, c_int, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    'so3g_g3s_decode': (c_int, [c_vp, c_i64, c_vp, c_i64, c_int, c_i64, c_i64, c_i64, c_vp,
+                                c_i64, c_vp, c_i64, c_i64, c_vp]),
+    'so3g_g3s_stats': (c_int, [c_vp]),
     'so3g_process_cuts': (c_int, [c_vp, c_i64, c_int, c_int, c_int, c_int, c_vp, c_i64, c_i64,
                                   c_vp, c_i64, c_int, c_vp]),
     'so3g_translate_cuts': (c_int, [c_vp, c_vp, c_i64, c_int, c_int, c_int, c_i64, c_i64, c_vp,

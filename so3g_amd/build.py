@@ -1,7 +1,8 @@
 """Build libso3g_hip.so (the HIP kernels and their C ABI) for gfx950.
 
-Each ``csrc/*.hip`` translation unit is compiled by ``hipcc`` in parallel
-and linked into ``so3g_amd/lib/libso3g_hip.so``.  The float64 pointing code
+Each ``csrc/*.hip`` translation unit is compiled by ``hipcc`` (host-only
+``csrc/*.cpp`` by ``g++``) in parallel and linked into
+``so3g_amd/lib/libso3g_hip.so``.  The float64 pointing code
 must reproduce the CPU path bit for bit, so FMA contraction is off
 (``-ffp-contract=off``) and float32 denormals are kept.  Packed-FP32
 instructions are disabled (``-packed-fp32-ops``).
@@ -21,6 +22,8 @@ LIB = os.path.join(HERE, 'lib', 'libso3g_hip.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('SO3G_OFFLOAD_ARCH', 'gfx950')
 
+CXX = os.environ.get('CXX', 'g++')
+CXXFLAGS = ['-O3', '-std=c++17', '-fPIC', '-ffp-contract=off']
 HIPFLAGS = [
     f'--offload-arch={ARCH}', '-O3', '-std=c++17', '-fPIC',
     '-ffp-contract=off', '-fno-gpu-flush-denormals-to-zero',
@@ -36,11 +39,15 @@ def _headers():
 
 
 def _compile(src, force):
-    obj = os.path.join(OBJ, os.path.basename(src).replace('.hip', '.o'))
+    base, ext = os.path.splitext(os.path.basename(src))
+    obj = os.path.join(OBJ, base + ('.o' if ext == '.hip' else '_host.o'))
     newest_dep = max(os.path.getmtime(p) for p in [src] + _headers())
     if (not force) and os.path.exists(obj) and os.path.getmtime(obj) >= newest_dep:
         return obj, None
-    cmd = [HIPCC] + HIPFLAGS + ['-c', src, '-o', obj]
+    if ext == '.hip':
+        cmd = [HIPCC] + HIPFLAGS + ['-c', src, '-o', obj]
+    else:   # host-only C++ (no device code)
+        cmd = [CXX] + CXXFLAGS + ['-c', src, '-o', obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         return obj, f'{" ".join(cmd)}\n{r.stdout}\n{r.stderr}'
@@ -51,7 +58,7 @@ def build(jobs=None, force=True, verbose=False):
     """Compile every HIP source and link the shared library.  Returns the
     library path.  Raises RuntimeError with the compiler output on failure."""
     os.makedirs(OBJ, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, '*.hip')))
+    srcs = sorted(glob.glob(os.path.join(CSRC, '*.hip')) + glob.glob(os.path.join(CSRC, '*.cpp')))
     jobs = jobs or min(len(srcs), os.cpu_count() or 4)
     objs, errors = [], []
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:

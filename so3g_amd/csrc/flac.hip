@@ -1,0 +1,783 @@
+// flac.hip -- G3SuperTimestream decoding on the GPU.
+//
+// A G3SuperTimestream array blob holds, per channel, an algo byte and then
+// a FLAC stream of the channel's 24-bit part, a bz2 block of slow
+// full-width offsets or a CONST datum; float channels are finally scaled
+// by their quantum (G3SuperTimestream.cxx:136-181 for the layout, :664-812
+// Extract for the decode).  The host side (so3g_amd/g3super.py) walks the
+// algo bytes, inflates the rare bz2 blocks and reads each FLAC stream's
+// STREAMINFO; this file decodes the FLAC frames -- the bulk of the bytes and
+// of the work -- and writes the final values.
+//
+// FLAC frames of one stream are independent once their byte offsets are
+// known, and a fixed-blocksize stream (libFLAC's stream encoder writes
+// those) numbers its frames.  So:
+//   1. k_flac_sync: every byte of every FLAC region is tested for a frame
+//      header (sync code, valid fields, CRC-8); a header numbered k keeps
+//      the smallest such offset in slot[k] (atomicMin).  Compressed data
+//      occasionally contain such a header: it can only lower a slot.
+//   2. k_flac_frames: one lane per (channel, frame) decodes frame k from
+//      slot[k] if the header there carries number k and the expected block
+//      size, writes its samples, checks the CRC-16 footer, records where the
+//      frame ends (ends[k]) and whether that is where slot[k + 1] begins.
+//   3. k_flac_walk: channels with a broken chain (a false sync), a
+//      variable-blocksize stream or an LPC order above kMaxLpcPar are walked
+//      frame by frame from the first, one lane per channel: a frame that
+//      starts where its predecessor ended and decoded correctly in step 2 is
+//      skipped, any other is decoded here (overwriting what step 2 wrote for
+//      its samples).  A malformed frame or a CRC mismatch is an error.
+// Channels without FLAC (raw, bz2-only or CONST-only) go through
+// k_g3s_plain.  Per sample the value is combined exactly as Extract does:
+// FLAC (or raw) value, plus the inflated offsets, plus the CONST datum, in
+// the full-width integer type with wrap-around, then (float32)((double)
+// (float)v * quantum) or (double)v * quantum for float channels.
+//
+// The bit reader keeps the next 16-byte chunk in flight while the current
+// one is consumed, so a lane's loads are not serialised behind its sample
+// stores.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/so3g_hip.h"
+
+namespace so3g {
+int set_error(int code, const std::string &msg);
+}
+
+namespace {
+
+#define FL_TRY(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return so3g::set_error(SO3G_ERR_HIP, std::string("HIP error in ") + #expr + ": " + \
+                                                     hipGetErrorString(_e));               \
+    } while (0)
+
+constexpr int kFlBlock = 256;
+constexpr int kScanBytes = 64;     // bytes tested per thread by the sync scan
+constexpr int64_t kSlack = 64;     // bit-reader read-ahead allowed past a region
+constexpr int kMaxLpcPar = 12;     // LPC orders decoded frame-parallel
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+// Why a channel leaves the frame-parallel pass (bits of its flag).
+constexpr int kWhyVariable = 1, kWhyNoSync = 2, kWhyBadFrame = 4, kWhyChain = 8,
+              kWhyHeader = 16, kWhyLpc = 32;
+
+// Per extracted channel, on the device.
+struct Chan {
+    int64_t flac_pos, flac_end;   // first frame and end of the FLAC block (-1: none)
+    int64_t raw_pos;              // ALGO_NONE data (-1: none)
+    int64_t off_row;              // row of the inflated offsets (-1: none)
+    uint64_t cval;                // CONST datum (full-width bits)
+    double quantum;
+    int64_t dest_row;
+    int64_t slot0;                // first frame slot of this channel
+    int32_t n_frames, block, bps, has_const;
+};
+
+// Where decoded samples go.
+struct Out {
+    const void *offs;             // inflated offsets, rows of n_samp
+    void *dest;                   // rows of stride elements
+    int64_t n_samp, start, stop, stride;
+};
+
+// -- bit reader over the payload (MSB first) ----------------------------------
+
+__device__ __forceinline__ uint32_t bswap32(uint32_t v) {
+    return (v >> 24) | ((v >> 8) & 0xFF00u) | ((v << 8) & 0xFF0000u) | (v << 24);
+}
+
+struct Reader {
+    const uint8_t *p;
+    int64_t nbytes;     // readable payload bytes
+    int64_t limit;      // a word beyond this byte is an error
+    uint64_t buf;       // valid bits left-aligned, the rest zero
+    int bits;
+    uint32_t q0, q1, q2, q3;   // words of the current chunk (big-endian values)
+    int nq;                    // of which not yet in buf
+    uint4 pre;                 // the next chunk, in flight
+    int64_t cpos;              // byte address of `pre`
+    int64_t wpos;              // byte address of the next word to enter buf
+    bool err;
+
+    __device__ uint4 chunk(int64_t a) const {   // 16 bytes at a (a % 16 == 0)
+        if (a + 16 <= nbytes) return *reinterpret_cast<const uint4 *>(p + a);
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int i = 3; i >= 0; i--) {
+                const int64_t b = a + 4 * j + i;
+                v = (v << 8) | (b < nbytes ? p[b] : 0u);
+            }
+            w[j] = v;
+        }
+        return uint4{w[0], w[1], w[2], w[3]};
+    }
+    __device__ __forceinline__ uint32_t pop() {
+        const uint32_t w = q0;
+        q0 = q1; q1 = q2; q2 = q3;
+        if (--nq == 0) {
+            q0 = bswap32(pre.x); q1 = bswap32(pre.y); q2 = bswap32(pre.z); q3 = bswap32(pre.w);
+            nq = 4;
+            cpos += 16;
+            pre = chunk(cpos);
+        }
+        wpos += 4;
+        return w;
+    }
+    __device__ void start(int64_t pos) {
+        const int64_t a = pos & ~(int64_t)15;
+        const uint4 c0 = chunk(a);
+        q0 = bswap32(c0.x); q1 = bswap32(c0.y); q2 = bswap32(c0.z); q3 = bswap32(c0.w);
+        nq = 4;
+        cpos = a + 16;
+        pre = chunk(cpos);
+        wpos = a;
+        err = false;
+        for (int i = (int)((pos & 15) >> 2); i > 0; i--) pop();
+        const int skip = (int)(pos & 3) * 8;
+        buf = ((uint64_t)pop() << 32) << skip;
+        bits = 32 - skip;
+    }
+    __device__ __forceinline__ void refill() {
+        if (bits <= 32) {
+            if (wpos > limit) { err = true; buf |= 1ull << (63 - bits); }
+            buf |= (uint64_t)pop() << (32 - bits);
+            bits += 32;
+        }
+    }
+    __device__ __forceinline__ uint32_t get(int n) {   // 0 <= n <= 32
+        refill();
+        if (n == 0) return 0;
+        const uint32_t v = (uint32_t)(buf >> (64 - n));
+        buf <<= n;
+        bits -= n;
+        return v;
+    }
+    __device__ __forceinline__ int32_t sget(int n) {   // two's complement, n <= 32
+        if (n == 0) return 0;
+        const uint32_t v = get(n);
+        return n == 32 ? (int32_t)v : (int32_t)(v << (32 - n)) >> (32 - n);
+    }
+    __device__ __forceinline__ uint32_t unary() {      // zeros before the next one
+        uint32_t q = 0;
+        for (;;) {
+            refill();
+            if (buf != 0) {
+                const int z = __clzll(buf);
+                q += z;
+                buf = (buf << z) << 1;
+                bits -= z + 1;
+                return q;
+            }
+            q += bits;
+            bits = 0;
+            if (err) return q;
+        }
+    }
+    __device__ int64_t bitpos() const { return wpos * 8 - bits; }
+    __device__ uint32_t byte(int64_t a) const { return a < nbytes ? p[a] : 0u; }
+};
+
+__device__ __forceinline__ uint8_t crc8_bytes(const Reader &r, int64_t a, int64_t n) {
+    uint32_t c = 0;
+    for (int64_t i = 0; i < n; i++) {
+        c ^= r.byte(a + i);
+        for (int b = 0; b < 8; b++) c = (c & 0x80) ? ((c << 1) ^ 0x07) & 0xFF : (c << 1) & 0xFF;
+    }
+    return (uint8_t)c;
+}
+
+// CRC-16 of payload bytes [a, b), 16-byte chunks at a time.
+__device__ __forceinline__ uint32_t crc16_range(const Reader &r, int64_t a, int64_t b, const uint16_t *tab) {
+    uint32_t crc = 0;
+    for (int64_t c = a & ~(int64_t)15; c < b; c += 16) {
+        const uint4 v = r.chunk(c);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int64_t at = c + i;
+            if (at >= a && at < b) {
+                const uint32_t byte = (w[i >> 2] >> (8 * (i & 3))) & 0xFF;
+                crc = ((crc << 8) ^ tab[((crc >> 8) ^ byte) & 0xFF]) & 0xFFFF;
+            }
+        }
+    }
+    return crc;
+}
+
+struct Frame {
+    int block, bps;
+    uint64_t number;
+    int blocking;
+};
+
+__device__ __forceinline__ int sample_size_bits(int code, int stream_bps) {
+    switch (code) {
+    case 0: return stream_bps;
+    case 1: return 8;
+    case 2: return 12;
+    case 4: return 16;
+    case 5: return 20;
+    case 6: return 24;
+    case 7: return 32;
+    default: return -1;
+    }
+}
+
+// Frame header at byte `at`: returns its length in bytes (CRC-8 included)
+// or -1.  stream_bps: bits per sample of STREAMINFO.
+__device__ __forceinline__ int frame_header(Reader &r, int64_t at, int stream_bps, Frame &f) {
+    r.start(at);
+    if (r.get(14) != 0x3FFE || r.get(1) != 0) return -1;
+    f.blocking = (int)r.get(1);
+    const int bs_code = (int)r.get(4), rate = (int)r.get(4), chan = (int)r.get(4);
+    const int ss = (int)r.get(3);
+    if (r.get(1) != 0 || chan != 0 || bs_code == 0 || rate == 15 || ss == 3) return -1;
+    const uint32_t b0 = r.get(8);
+    int extra;
+    uint64_t num;
+    if (b0 < 0x80) { num = b0; extra = 0; }
+    else if ((b0 & 0xE0) == 0xC0) { num = b0 & 0x1F; extra = 1; }
+    else if ((b0 & 0xF0) == 0xE0) { num = b0 & 0x0F; extra = 2; }
+    else if ((b0 & 0xF8) == 0xF0) { num = b0 & 0x07; extra = 3; }
+    else if ((b0 & 0xFC) == 0xF8) { num = b0 & 0x03; extra = 4; }
+    else if ((b0 & 0xFE) == 0xFC) { num = b0 & 0x01; extra = 5; }
+    else if (b0 == 0xFE) { num = 0; extra = 6; }
+    else return -1;
+    if (f.blocking == 0 && extra > 5) return -1;
+    for (int i = 0; i < extra; i++) {
+        const uint32_t c = r.get(8);
+        if ((c & 0xC0) != 0x80) return -1;
+        num = (num << 6) | (c & 0x3F);
+    }
+    f.number = num;
+    if (bs_code == 1) f.block = 192;
+    else if (bs_code <= 5) f.block = 576 << (bs_code - 2);
+    else if (bs_code == 6) f.block = (int)r.get(8) + 1;
+    else if (bs_code == 7) f.block = (int)r.get(16) + 1;
+    else f.block = 256 << (bs_code - 8);
+    if (rate == 12) r.get(8);
+    else if (rate == 13 || rate == 14) r.get(16);
+    f.bps = sample_size_bits(ss, stream_bps);
+    const int64_t hl = r.bitpos() / 8 - at;   // header bytes before the CRC
+    const uint32_t c = r.get(8);
+    if (r.err || f.bps <= 0 || c != crc8_bytes(r, at, hl)) return -1;
+    return (int)(hl + 1);
+}
+
+// Writes sample t of a channel (the FLAC value v) into dest, combined as
+// Extract does.
+template <int TYPE>
+__device__ __forceinline__ void emit(const Chan &c, const Out &o, int64_t t, uint64_t v_bits) {
+    if (t < o.start || t >= o.stop) return;
+    const int64_t at = c.dest_row * o.stride + (t - o.start);
+    if constexpr (TYPE == 1 || TYPE == 3) {
+        uint64_t v = v_bits;
+        if (c.off_row >= 0) v += static_cast<const uint64_t *>(o.offs)[c.off_row * o.n_samp + t];
+        if (c.has_const) v += c.cval;
+        if constexpr (TYPE == 3) static_cast<double *>(o.dest)[at] = (double)(int64_t)v * c.quantum;
+        else static_cast<uint64_t *>(o.dest)[at] = v;
+    } else {
+        uint32_t v = (uint32_t)v_bits;
+        if (c.off_row >= 0) v += static_cast<const uint32_t *>(o.offs)[c.off_row * o.n_samp + t];
+        if (c.has_const) v += (uint32_t)c.cval;
+        if constexpr (TYPE == 2)
+            static_cast<float *>(o.dest)[at] = (float)((double)(float)(int32_t)v * c.quantum);
+        else static_cast<uint32_t *>(o.dest)[at] = v;
+    }
+}
+
+__device__ __forceinline__ uint64_t shifted(int32_t x, int wasted) {
+    return (uint64_t)((int64_t)x << wasted);
+}
+
+// Residual partitions of a subframe; pred(residual) returns the sample and
+// updates the predictor's state.
+template <int TYPE, typename P>
+__device__ __forceinline__ bool residuals(Reader &r, const Chan &c, const Out &o, int block, int order,
+                          int wasted, int64_t t0, P &&pred) {
+    const int method = (int)r.get(2);
+    if (method > 1) return false;
+    const int porder = (int)r.get(4);
+    const int parts = 1 << porder;
+    if ((block & (parts - 1)) != 0 || (block >> porder) < order) return false;
+    const uint32_t escape = method ? 31u : 15u;
+    int i = order;
+    for (int p = 0; p < parts; p++) {
+        const int cnt = (block >> porder) - (p == 0 ? order : 0);
+        const uint32_t k = r.get(method ? 5 : 4);
+        const int nb = k == escape ? (int)r.get(5) : 0;
+        for (int j = 0; j < cnt; j++, i++) {
+            int32_t res;
+            if (k == escape) {
+                res = r.sget(nb);
+            } else {
+                const uint32_t q = r.unary();
+                const uint32_t u = (q << k) | r.get((int)k);
+                res = (int32_t)(u >> 1) ^ -(int32_t)(u & 1);
+            }
+            emit<TYPE>(c, o, t0 + i, shifted(pred(res), wasted));
+        }
+        if (r.err) return false;
+    }
+    return true;
+}
+
+template <int TYPE>
+__device__ __forceinline__ bool fixed_body(Reader &r, const Chan &c, const Out &o, int block, int sbps,
+                           int wasted, int order, int64_t t0) {
+    int32_t x1 = 0, x2 = 0, x3 = 0, x4 = 0;
+    for (int i = 0; i < order; i++) {
+        const int32_t x = r.sget(sbps);
+        x4 = x3; x3 = x2; x2 = x1; x1 = x;
+        emit<TYPE>(c, o, t0 + i, shifted(x, wasted));
+    }
+    return residuals<TYPE>(r, c, o, block, order, wasted, t0, [&](int32_t res) {
+        int64_t p;
+        switch (order) {
+        case 0: p = 0; break;
+        case 1: p = x1; break;
+        case 2: p = 2 * (int64_t)x1 - x2; break;
+        case 3: p = 3 * (int64_t)x1 - 3 * (int64_t)x2 + x3; break;
+        default: p = 4 * (int64_t)x1 - 6 * (int64_t)x2 + 4 * (int64_t)x3 - x4; break;
+        }
+        const int32_t x = (int32_t)(res + p);
+        x4 = x3; x3 = x2; x2 = x1; x1 = x;
+        return x;
+    });
+}
+
+template <int TYPE, int MAXO>
+__device__ __forceinline__ bool lpc_body(Reader &r, const Chan &c, const Out &o, int block, int sbps,
+                         int wasted, int order, int64_t t0) {
+    int32_t h[MAXO];     // h[j] = x[i - 1 - j]
+    int32_t coef[MAXO];
+#pragma unroll
+    for (int j = 0; j < MAXO; j++) { h[j] = 0; coef[j] = 0; }
+    for (int i = 0; i < order; i++) {
+        const int32_t x = r.sget(sbps);
+#pragma unroll
+        for (int j = MAXO - 1; j > 0; j--) h[j] = h[j - 1];
+        h[0] = x;
+        emit<TYPE>(c, o, t0 + i, shifted(x, wasted));
+    }
+    const int prec = (int)r.get(4) + 1;
+    const int shift = r.sget(5);
+    if (prec == 16 || shift < 0) return false;
+    for (int j = 0; j < order; j++) {
+        const int32_t q = r.sget(prec);
+#pragma unroll
+        for (int m = 0; m < MAXO; m++)
+            if (m == j) coef[m] = q;
+    }
+    return residuals<TYPE>(r, c, o, block, order, wasted, t0, [&](int32_t res) {
+        int64_t sum = 0;
+#pragma unroll
+        for (int m = 0; m < MAXO; m++) sum += (int64_t)coef[m] * h[m];
+        const int32_t x = res + (int32_t)(sum >> shift);
+#pragma unroll
+        for (int m = MAXO - 1; m > 0; m--) h[m] = h[m - 1];
+        h[0] = x;
+        return x;
+    });
+}
+
+// Decodes one frame at byte `at` of channel c, emitting samples from t0 on.
+// want_num >= 0: the header must carry that number and want_block samples
+// (checked before anything is written).  Returns the byte offset just past
+// the frame (CRC-16 included), or -1 with a reason in why.
+template <int TYPE, int MAXO>
+__device__ __forceinline__ int64_t decode_frame(Reader &r, const Chan &c, const Out &o, int64_t at,
+                                int64_t want_num, int want_block, int64_t t0,
+                                const uint16_t *crc_tab, Frame &f, int &why) {
+    why = kWhyBadFrame;
+    const int hl = frame_header(r, at, c.bps, f);
+    if (hl < 0) return -1;
+    if (want_num >= 0 && (f.number != (uint64_t)want_num || f.block != want_block)) {
+        why = kWhyHeader;
+        return -1;
+    }
+    if (r.get(1) != 0) return -1;
+    const int type = (int)r.get(6);
+    int wasted = 0;
+    if (r.get(1)) wasted = (int)r.unary() + 1;
+    const int sbps = f.bps - wasted;
+    if (sbps <= 0 || sbps > 32) return -1;
+    bool ok;
+    if (type == 0) {
+        const int32_t v = r.sget(sbps);
+        for (int i = 0; i < f.block; i++) emit<TYPE>(c, o, t0 + i, shifted(v, wasted));
+        ok = true;
+    } else if (type == 1) {
+        for (int i = 0; i < f.block; i++) emit<TYPE>(c, o, t0 + i, shifted(r.sget(sbps), wasted));
+        ok = true;
+    } else if (type >= 8 && type <= 12) {
+        const int order = type - 8;
+        ok = order <= f.block && fixed_body<TYPE>(r, c, o, f.block, sbps, wasted, order, t0);
+    } else if (type >= 32) {
+        const int order = (type & 31) + 1;
+        if (order > MAXO) { why = kWhyLpc; return -1; }
+        ok = order <= f.block && lpc_body<TYPE, MAXO>(r, c, o, f.block, sbps, wasted, order, t0);
+    } else {
+        ok = false;
+    }
+    if (!ok || r.err) return -1;
+    const int64_t body_end = (r.bitpos() + 7) / 8;
+    if (body_end + 2 > c.flac_end) return -1;
+    const uint32_t want = (r.byte(body_end) << 8) | r.byte(body_end + 1);
+    if (crc16_range(r, at, body_end, crc_tab) != want) return -1;
+    why = 0;
+    return body_end + 2;
+}
+
+__device__ void load_crc_table(uint16_t *tab) {
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+        uint32_t c = (uint32_t)i << 8;
+        for (int b = 0; b < 8; b++) c = (c & 0x8000) ? ((c << 1) ^ 0x8005) : (c << 1);
+        tab[i] = (uint16_t)c;
+    }
+    __syncthreads();
+}
+
+// Sync scan: thread i tests kScanBytes byte offsets of one scan tile.
+// tiles[k] = (channel, first byte, end byte) of a slice of a FLAC region.
+__global__ __launch_bounds__(kFlBlock) void k_flac_sync(const uint8_t *__restrict__ payload,
+                                                        int64_t nbytes, const Chan *chans,
+                                                        const int64_t *tiles, int64_t n_tiles,
+                                                        uint32_t *slots, int32_t *flags) {
+    const int64_t tile = blockIdx.x;
+    if (tile >= n_tiles) return;
+    const int64_t ci = tiles[3 * tile], a0 = tiles[3 * tile + 1], a1 = tiles[3 * tile + 2];
+    const Chan c = chans[ci];
+    Reader r{payload, nbytes, c.flac_end + kSlack};
+    const int64_t lo = a0 + (int64_t)threadIdx.x * kScanBytes;
+    const int64_t hi = min(a1, lo + kScanBytes);
+    for (int64_t a16 = lo & ~(int64_t)15; a16 < hi; a16 += 16) {
+        const uint4 v = r.chunk(a16);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int64_t a = a16 + i;
+            if (((w[i >> 2] >> (8 * (i & 3))) & 0xFF) != 0xFF || a < lo || a >= hi) continue;
+            if (a + 1 >= c.flac_end || (r.byte(a + 1) & 0xFE) != 0xF8) continue;
+            Frame f;
+            if (frame_header(r, a, c.bps, f) < 0) continue;
+            // The first frame tells the blocking strategy; elsewhere a
+            // variable-blocksize header can only be a false match.
+            if (a == c.flac_pos && f.blocking) atomicOr(flags + ci, kWhyVariable);
+            if (!f.blocking && f.number < (uint64_t)c.n_frames)
+                atomicMin(slots + c.slot0 + f.number, (uint32_t)(a - c.flac_pos));
+        }
+    }
+}
+
+// One lane per frame of the parallel pass.  frame_chan[g] = channel of
+// global frame g (frames of a channel are consecutive).
+template <int TYPE>
+__global__ __launch_bounds__(kFlBlock) void k_flac_frames(
+    const uint8_t *__restrict__ payload, int64_t nbytes, const Chan *chans,
+    const int32_t *frame_chan, int64_t n_frames, const uint32_t *slots, Out o,
+    uint32_t *ends, int32_t *flags) {
+    __shared__ uint16_t crc_tab[256];
+    load_crc_table(crc_tab);
+    const int64_t g = (int64_t)blockIdx.x * kFlBlock + threadIdx.x;
+    if (g >= n_frames) return;
+    const int32_t ci = frame_chan[g];
+    const Chan c = chans[ci];
+    if (flags[ci] & kWhyVariable) return;
+    const int64_t k = g - c.slot0;
+    const int64_t t0 = k * c.block;
+    // Frames after the requested samples are not needed; the frame before
+    // them is decoded (emitting nothing) so that the first needed frame's
+    // start is checked against its predecessor's end.
+    if (t0 >= o.stop || (k + 2) * c.block <= o.start) return;
+    const uint32_t s = slots[g];
+    if (s == kNone) { atomicOr(flags + ci, kWhyNoSync); return; }
+    Reader r{payload, nbytes, c.flac_end + kSlack};
+    Frame f;
+    int why;
+    const int want_block = (int)min((int64_t)c.block, o.n_samp - t0);
+    const int64_t end = decode_frame<TYPE, kMaxLpcPar>(r, c, o, c.flac_pos + s, k, want_block,
+                                                       t0, crc_tab, f, why);
+    if (end >= 0) {
+        ends[g] = (uint32_t)(end - c.flac_pos);
+        int64_t want_end = c.flac_end;
+        if (k + 1 < c.n_frames) want_end = slots[g + 1] == kNone ? -1 : c.flac_pos + slots[g + 1];
+        if (end != want_end) why = kWhyChain;
+    }
+    if (why) atomicOr(flags + ci, why);
+}
+
+// One lane per channel: the frames in sequence from the first.  A frame
+// that starts where its predecessor ended and decoded correctly in the
+// parallel pass (ends[] set) is skipped; any other is decoded here.
+// err[i] = 1 on a malformed stream.
+template <int TYPE>
+__global__ __launch_bounds__(64) void k_flac_walk(
+    const uint8_t *__restrict__ payload, int64_t nbytes, const Chan *chans,
+    const int32_t *which, int64_t n, const uint32_t *slots, const uint32_t *ends,
+    const int32_t *flags, Out o, int32_t *err) {
+    __shared__ uint16_t crc_tab[256];
+    load_crc_table(crc_tab);
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    const int32_t ci = which[i];
+    const Chan c = chans[ci];
+    const bool fixed = !(flags[ci] & kWhyVariable);
+    Reader r{payload, nbytes, c.flac_end + kSlack};
+    int64_t at = c.flac_pos, t0 = 0;
+    for (int64_t k = 0; at < c.flac_end && t0 < o.stop; k++) {
+        if (fixed && k < c.n_frames) {
+            const int64_t g = c.slot0 + k;
+            if (slots[g] == (uint32_t)(at - c.flac_pos) && ends[g] != kNone) {
+                at = c.flac_pos + ends[g];
+                t0 += min((int64_t)c.block, o.n_samp - t0);
+                continue;
+            }
+        }
+        Frame f;
+        int why;
+        const int64_t next = decode_frame<TYPE, 32>(r, c, o, at, -1, 0, t0, crc_tab, f, why);
+        if (next < 0) { err[i] = 1; return; }
+        at = next;
+        t0 += f.block;
+    }
+}
+
+// Channels without FLAC: raw data or the caller's contents, plus offsets
+// and the CONST datum.
+template <int TYPE>
+__global__ __launch_bounds__(kFlBlock) void k_g3s_plain(const uint8_t *__restrict__ payload,
+                                                        const Chan *chans, const int32_t *which,
+                                                        int64_t n, Out o) {
+    constexpr bool W = TYPE == 1 || TYPE == 3;
+    const int64_t m = o.stop - o.start;
+    for (int64_t e = (int64_t)blockIdx.x * kFlBlock + threadIdx.x; e < n * m;
+         e += (int64_t)gridDim.x * kFlBlock) {
+        const Chan c = chans[which[e / m]];
+        const int64_t t = o.start + e % m;
+        const int64_t at = c.dest_row * o.stride + (t - o.start);
+        uint64_t v;
+        if (c.raw_pos >= 0) {
+            const uint8_t *s = payload + c.raw_pos + t * (W ? 8 : 4);
+            uint64_t b = 0;
+            for (int q = (W ? 7 : 3); q >= 0; q--) b = (b << 8) | s[q];
+            v = b;
+        } else {
+            v = W ? static_cast<const uint64_t *>(o.dest)[at]
+                  : static_cast<const uint32_t *>(o.dest)[at];
+        }
+        emit<TYPE>(c, o, t, v);
+    }
+}
+
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Counts of the last decode call: frames decoded in parallel, channels
+// walked frame by frame, OR of their reasons.
+int64_t g_fl_stats[3] = {0, 0, 0};
+
+// Grow-only device workspace of the decoder (one per process).
+void *g_fl_ws = nullptr;
+size_t g_fl_ws_cap = 0;
+
+int workspace(size_t bytes, void **p) {
+    if (bytes > g_fl_ws_cap) {
+        if (g_fl_ws) (void)hipFree(g_fl_ws);
+        g_fl_ws = nullptr;
+        g_fl_ws_cap = 0;
+        FL_TRY(hipMalloc(&g_fl_ws, bytes));
+        g_fl_ws_cap = bytes;
+    }
+    *p = g_fl_ws;
+    return SO3G_OK;
+}
+
+unsigned blocks_for(int64_t n, int per) {
+    return (unsigned)std::max<int64_t>(1, (n + per - 1) / per);
+}
+
+template <int TYPE>
+int decode_typed(const uint8_t *payload, int64_t nbytes, const std::vector<Chan> &ch, Out o,
+                 hipStream_t s) {
+    const int64_t n = (int64_t)ch.size();
+    g_fl_stats[0] = g_fl_stats[1] = g_fl_stats[2] = 0;
+    std::vector<Chan> chans = ch;
+    std::vector<int32_t> frame_chan, plain;
+    std::vector<int64_t> tiles;
+    int64_t n_frames = 0;
+    for (int64_t i = 0; i < n; i++) {
+        Chan &c = chans[i];
+        if (c.flac_pos < 0) { plain.push_back((int32_t)i); c.n_frames = 0; continue; }
+        c.slot0 = n_frames;
+        c.n_frames = (int32_t)((o.n_samp + c.block - 1) / c.block);
+        frame_chan.insert(frame_chan.end(), (size_t)c.n_frames, (int32_t)i);
+        n_frames += c.n_frames;
+        const int64_t tile = (int64_t)kFlBlock * kScanBytes;
+        for (int64_t a = c.flac_pos; a < c.flac_end; a += tile) {
+            tiles.push_back(i);
+            tiles.push_back(a);
+            tiles.push_back(std::min(c.flac_end, a + tile));
+        }
+    }
+    const int64_t n_tiles = (int64_t)tiles.size() / 3;
+    // Workspace: chans | tiles | frame_chan | slots | ends | flags | which | err
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_ch = up(n * sizeof(Chan)), b_ti = up(tiles.size() * 8),
+                 b_fc = up(frame_chan.size() * 4), b_sl = up(n_frames * 4), b_en = b_sl,
+                 b_fl = up(n * 4), b_wh = up(n * 4), b_er = up(n * 4);
+    void *ws = nullptr;
+    int rc = workspace(b_ch + b_ti + b_fc + b_sl + b_en + b_fl + b_wh + b_er, &ws);
+    if (rc != SO3G_OK) return rc;
+    char *w = static_cast<char *>(ws);
+    Chan *d_ch = reinterpret_cast<Chan *>(w);
+    w += b_ch;
+    int64_t *d_ti = reinterpret_cast<int64_t *>(w);
+    w += b_ti;
+    int32_t *d_fc = reinterpret_cast<int32_t *>(w);
+    w += b_fc;
+    uint32_t *d_sl = reinterpret_cast<uint32_t *>(w);
+    w += b_sl;
+    uint32_t *d_en = reinterpret_cast<uint32_t *>(w);
+    w += b_en;
+    int32_t *d_fl = reinterpret_cast<int32_t *>(w);
+    w += b_fl;
+    int32_t *d_wh = reinterpret_cast<int32_t *>(w);
+    w += b_wh;
+    int32_t *d_er = reinterpret_cast<int32_t *>(w);
+    FL_TRY(hipMemcpyAsync(d_ch, chans.data(), n * sizeof(Chan), hipMemcpyHostToDevice, s));
+    if (n_tiles)
+        FL_TRY(hipMemcpyAsync(d_ti, tiles.data(), tiles.size() * 8, hipMemcpyHostToDevice, s));
+    if (n_frames)
+        FL_TRY(hipMemcpyAsync(d_fc, frame_chan.data(), frame_chan.size() * 4,
+                              hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, d_sl, n_frames, kNone);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, d_en, n_frames, kNone);
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, s, reinterpret_cast<uint32_t *>(d_fl),
+                       n, 0u);
+    if (!plain.empty()) {
+        FL_TRY(hipMemcpyAsync(d_wh, plain.data(), plain.size() * 4, hipMemcpyHostToDevice, s));
+        const int64_t work = (int64_t)plain.size() * (o.stop - o.start);
+        hipLaunchKernelGGL(k_g3s_plain<TYPE>,
+                           dim3(std::min<unsigned>(blocks_for(work, kFlBlock), 1u << 16)),
+                           dim3(kFlBlock), 0, s, payload, d_ch, d_wh, (int64_t)plain.size(), o);
+        // The walk below reuses d_wh.
+        FL_TRY(hipStreamSynchronize(s));
+    }
+    if (n_frames > 0) {
+        hipLaunchKernelGGL(k_flac_sync, dim3((unsigned)n_tiles), dim3(kFlBlock), 0, s, payload,
+                           nbytes, d_ch, d_ti, n_tiles, d_sl, d_fl);
+        hipLaunchKernelGGL(k_flac_frames<TYPE>, dim3(blocks_for(n_frames, kFlBlock)),
+                           dim3(kFlBlock), 0, s, payload, nbytes, d_ch, d_fc, n_frames, d_sl, o,
+                           d_en, d_fl);
+        FL_TRY(hipGetLastError());
+        std::vector<int32_t> flags(n);
+        FL_TRY(hipMemcpyAsync(flags.data(), d_fl, n * 4, hipMemcpyDeviceToHost, s));
+        FL_TRY(hipStreamSynchronize(s));
+        std::vector<int32_t> redo;
+        for (int64_t i = 0; i < n; i++)
+            if (flags[i] && chans[i].flac_pos >= 0) {
+                redo.push_back((int32_t)i);
+                g_fl_stats[2] |= flags[i];
+            }
+        g_fl_stats[0] = n_frames;
+        g_fl_stats[1] = (int64_t)redo.size();
+        if (!redo.empty()) {
+            const int64_t nr = (int64_t)redo.size();
+            FL_TRY(hipMemcpyAsync(d_wh, redo.data(), nr * 4, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, s,
+                               reinterpret_cast<uint32_t *>(d_er), nr, 0u);
+            hipLaunchKernelGGL(k_flac_walk<TYPE>, dim3(blocks_for(nr, 64)), dim3(64), 0, s,
+                               payload, nbytes, d_ch, d_wh, nr, d_sl, d_en, d_fl, o, d_er);
+            FL_TRY(hipGetLastError());
+            std::vector<int32_t> err(nr);
+            FL_TRY(hipMemcpyAsync(err.data(), d_er, nr * 4, hipMemcpyDeviceToHost, s));
+            FL_TRY(hipStreamSynchronize(s));
+            for (int64_t i = 0; i < nr; i++)
+                if (err[i])
+                    return so3g::set_error(SO3G_ERR_VALUE,
+                                           "corrupt FLAC stream in extracted channel " +
+                                               std::to_string(redo[i]));
+        }
+    }
+    FL_TRY(hipGetLastError());
+    FL_TRY(hipStreamSynchronize(s));   // the workspace is reused by the next call
+    return SO3G_OK;
+}
+
+}  // namespace
+
+extern "C" int so3g_g3s_decode(const uint8_t *payload, int64_t payload_bytes,
+                               const so3g_g3s_chan *chans, int64_t n_chan, int32_t type,
+                               int64_t n_samp, int64_t start, int64_t stop, const void *offsets,
+                               int64_t n_offset_rows, void *dest, int64_t dest_rows,
+                               int64_t dest_stride, void *stream) {
+    if (type < 0 || type > 3) return so3g::set_error(SO3G_ERR_VALUE, "bad type code");
+    if (n_chan < 0 || n_samp < 0) return so3g::set_error(SO3G_ERR_VALUE, "negative shape");
+    if (start < 0 || stop < start || stop > n_samp)
+        return so3g::set_error(SO3G_ERR_VALUE, "sample range out of bounds");
+    if (n_chan == 0 || stop == start) return SO3G_OK;
+    if (payload == nullptr || (reinterpret_cast<uintptr_t>(payload) & 15) != 0)
+        return so3g::set_error(SO3G_ERR_VALUE, "payload must be a 16-byte aligned device buffer");
+    if (dest == nullptr || dest_stride < stop - start)
+        return so3g::set_error(SO3G_ERR_VALUE, "bad destination");
+    const int W = (type == 1 || type == 3) ? 8 : 4;
+    std::vector<Chan> ch((size_t)n_chan);
+    for (int64_t i = 0; i < n_chan; i++) {
+        const so3g_g3s_chan &a = chans[i];
+        Chan &c = ch[i];
+        if (a.dest_row < 0 || a.dest_row >= dest_rows)
+            return so3g::set_error(SO3G_ERR_VALUE, "destination row out of range");
+        if (a.off_row >= n_offset_rows || (a.off_row >= 0 && offsets == nullptr))
+            return so3g::set_error(SO3G_ERR_VALUE, "offset row out of range");
+        if (a.flac_pos >= 0) {
+            if (a.flac_end > payload_bytes || a.flac_pos >= a.flac_end)
+                return so3g::set_error(SO3G_ERR_VALUE, "FLAC block outside the payload");
+            if (a.block < 16 || a.block > 65535 || a.bps < 4 || a.bps > 32)
+                return so3g::set_error(SO3G_ERR_VALUE, "bad FLAC stream parameters");
+            if (a.flac_end - a.flac_pos >= (int64_t)UINT32_MAX)
+                return so3g::set_error(SO3G_ERR_VALUE, "FLAC block too large");
+        }
+        if (a.raw_pos >= 0 && a.raw_pos + n_samp * W > payload_bytes)
+            return so3g::set_error(SO3G_ERR_VALUE, "raw block outside the payload");
+        c.flac_pos = a.flac_pos;
+        c.flac_end = a.flac_pos >= 0 ? a.flac_end : -1;
+        c.raw_pos = a.flac_pos >= 0 ? -1 : a.raw_pos;
+        c.off_row = a.off_row;
+        c.cval = (uint64_t)a.const_bits;
+        c.has_const = a.has_const;
+        c.quantum = a.quantum;
+        c.dest_row = a.dest_row;
+        c.slot0 = 0;
+        c.n_frames = 0;
+        c.block = a.block;
+        c.bps = a.bps;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const Out o{offsets, dest, n_samp, start, stop, dest_stride};
+    switch (type) {
+    case 0: return decode_typed<0>(payload, payload_bytes, ch, o, s);
+    case 1: return decode_typed<1>(payload, payload_bytes, ch, o, s);
+    case 2: return decode_typed<2>(payload, payload_bytes, ch, o, s);
+    default: return decode_typed<3>(payload, payload_bytes, ch, o, s);
+    }
+}
+
+extern "C" int so3g_g3s_stats(int64_t *out) {
+    if (out == nullptr) return so3g::set_error(SO3G_ERR_VALUE, "out is NULL");
+    out[0] = g_fl_stats[0];
+    out[1] = g_fl_stats[1];
+    out[2] = g_fl_stats[2];
+    return SO3G_OK;
+}

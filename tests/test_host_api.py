@@ -23,6 +23,8 @@ def test_library_exports_header_symbols():
     assert len(names) >= 18
     for n in sorted(names):
         assert hasattr(lib, n), n
+        # and a ctypes signature, so 64-bit parameters are not passed as ints
+        assert n in _native.exported_symbols(), n
     assert lib.so3g_abi_version() == 1
 
 
