@@ -71,6 +71,8 @@ def parse():
     ap.add_argument('--pmc-otf', action='store_true',
                     help='with --pmc-pass: also one launch of each on-the-fly kernel')
     ap.add_argument('--no-small', action='store_true', help='skip the small-observation leg')
+    ap.add_argument('--no-g3', action='store_true',
+                    help='skip the G3SuperTimestream decoding leg')
     ap.add_argument('--cpu-full', action='store_true',
                     help='print the full CPU baseline table (oracle, median of 5) and exit')
     return ap.parse_args()
@@ -271,6 +273,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(obs, pix, spin, sig, sky, pixargs)
     small = None if args.no_small else small_config(torch, so3g_amd, N, workloads, dev, stream)
     nmat = None if args.no_small else nmat_timing(torch, so3g_amd, dev, n_det, n_t)
+    g3 = None if args.no_g3 else g3super_timing(torch, n_det, n_t, not args.no_cpu)
 
     out = {
         'metric': 'det*samples/s for to_map+from_map (% HBM roofline)',
@@ -312,6 +315,7 @@ def main():
         'on_the_fly': otf,
         'small_config': small,
         'nmat_detvecs_apply': nmat,
+        'g3super_decode': g3,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)
@@ -441,6 +445,75 @@ def nmat_timing(torch, so3g_amd, dev, n_det, n_t, nvec=10, nbin=20, reps=5):
     log(f'nmat_detvecs_apply: {ms:.3f} ms')
     return {'shape': [n_det, nmode], 'nvec': nvec, 'nbin': len(bins), 'ms': ms,
             'hbm_frac': b / (ms * 1e-3) / HBM_PEAK}
+
+
+def g3super_timing(torch, n_det, n_t, with_cpu, reps=3, quantum=1e-3):
+    """G3SuperTimestream.extract of the benchmark's observation size from an
+    encoded blob resident on the GPU into a GPU array: float32 channels, one
+    quantum each, white noise of 300 quanta on a slow sinusoid (FLAC level
+    1, the reference's default; ~1.37 bytes per sample).  The time is the
+    whole extract call (host walk of the blob, sync scan, frame decoding).
+    CPU baseline: the oracle's decoder (sequential, RFC 9639 restatement)
+    over 32 channels on a 16-thread pool."""
+    import concurrent.futures
+    from so3g_amd import g3super as g3
+    rng = np.random.default_rng(11)
+    t = np.arange(n_t)
+    d = np.empty((n_det, n_t), np.float32)
+    for i in range(n_det):
+        d[i] = np.round(rng.normal(0, 300, n_t) + 1e4 * np.sin(t * 1e-3 + i)) * quantum
+    q = np.full(n_det, quantum)
+    ts = g3.G3SuperTimestream([f'c{i}' for i in range(n_det)], t, None, q)
+    ts.data = d
+    t0 = time.time()
+    ts.encode()
+    t_enc = time.time() - t0
+    nbytes = len(ts.payload)
+    dest = torch.empty((n_det, n_t), dtype=torch.float32, device='cuda')
+    ts.extract(dest)
+    torch.cuda.synchronize()
+    stats = g3.last_decode_stats()
+    # spot check against the quantised input
+    y, r, _ = g3.split(d[:4], q[:4])
+    want = ((r + y).astype(np.int32).astype(np.float32).astype(np.float64) * quantum
+            ).astype(np.float32)
+    exact = bool(np.array_equal(dest[:4].cpu().numpy(), want))
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.time()
+        ts.extract(dest)
+        torch.cuda.synchronize()
+        times.append(time.time() - t0)
+    ms = float(np.median(times)) * 1e3
+    n = n_det * n_t
+    out = {'workload': f'{n_det} float32 channels x {n_t} samples, FLAC level 1 '
+                       f'(1152-sample frames), {nbytes / n:.3f} bytes per sample',
+           'ms': ms, 'samples_per_s': n / (ms * 1e-3),
+           'compressed_GBps': nbytes / (ms * 1e-3) / 1e9,
+           'decoded_GBps': 4 * n / (ms * 1e-3) / 1e9,
+           'frames': stats['frames'], 'walked_channels': stats['serial'],
+           'spot_check_exact': exact, 'encode_s_host': t_enc, 'median_of': reps}
+    log(f'g3super decode: {ms:.3f} ms, {n / (ms * 1e-3):.3g} samples/s')
+    if with_cpu:
+        import oracle
+        k = min(32, n_det)
+        offs = ts.offsets
+
+        def one(i):
+            o = np.zeros((1, n_t), np.float32)
+            oracle.g3s_extract(ts.payload, [offs[i], offs[i + 1]], 'float32', n_t, q[i:i + 1], o)
+            return o
+        with concurrent.futures.ThreadPoolExecutor(16) as ex:
+            t0 = time.time()
+            list(ex.map(one, range(k)))
+            dt = time.time() - t0
+        out['cpu_baseline'] = {'value': k * n_t / dt, 'unit': 'samples/s', 'cores': 16,
+                               'kind': 'port',
+                               'sample': f'{k} of {n_det} channels, oracle decoder, '
+                                         f'16 threads: {dt:.2f} s'}
+    del dest
+    return out
 
 
 def _stripe_segments(dom, n_thread):

@@ -12,20 +12,26 @@
 // FLAC frames of one stream are independent once their byte offsets are
 // known, and a fixed-blocksize stream (libFLAC's stream encoder writes
 // those) numbers its frames.  So:
-//   1. k_flac_sync: every byte of every FLAC region is tested for a frame
-//      header (sync code, valid fields, CRC-8); a header numbered k keeps
-//      the smallest such offset in slot[k] (atomicMin).  Compressed data
-//      occasionally contain such a header: it can only lower a slot.
+//   1. k_flac_scan lists every byte offset of every FLAC region where a
+//      header of that stream could start (sync code, the first frame's
+//      fixed header bytes); k_flac_headers parses the listed headers (valid
+//      fields, CRC-8) and a header numbered k keeps the smallest such offset
+//      in slot[k] (atomicMin).  Compressed data occasionally contain such a
+//      header: it can only lower a slot.
 //   2. k_flac_frames: one lane per (channel, frame) decodes frame k from
 //      slot[k] if the header there carries number k and the expected block
 //      size, writes its samples, checks the CRC-16 footer, records where the
 //      frame ends (ends[k]) and whether that is where slot[k + 1] begins.
+//      FIXED frames are decoded by k_flac_frames; LPC frames are listed
+//      there and decoded by k_flac_frames_lpc, whose larger register
+//      footprint then does not lower the occupancy of the common case.
 //   3. k_flac_walk: channels with a broken chain (a false sync), a
 //      variable-blocksize stream or an LPC order above kMaxLpcPar are walked
-//      frame by frame from the first, one lane per channel: a frame that
-//      starts where its predecessor ended and decoded correctly in step 2 is
-//      skipped, any other is decoded here (overwriting what step 2 wrote for
-//      its samples).  A malformed frame or a CRC mismatch is an error.
+//      frame by frame from the first, one wave per channel: runs of frames
+//      that start where their predecessor ended and decoded correctly in
+//      step 2 are skipped 64 at a time, any other frame is decoded here
+//      (overwriting what step 2 wrote for its samples).  A malformed frame
+//      or a CRC mismatch is an error.
 // Channels without FLAC (raw, bz2-only or CONST-only) go through
 // k_g3s_plain.  Per sample the value is combined exactly as Extract does:
 // FLAC (or raw) value, plus the inflated offsets, plus the CONST datum, in
@@ -39,6 +45,7 @@
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/so3g_hip.h"
@@ -75,8 +82,25 @@ struct Chan {
     double quantum;
     int64_t dest_row;
     int64_t slot0;                // first frame slot of this channel
+    int64_t tile0;                // first sync-scan tile of this channel
     int32_t n_frames, block, bps, has_const;
 };
+
+constexpr int64_t kTile = (int64_t)kFlBlock * kScanBytes;   // bytes per scan tile
+
+// The channel owning global index x of a per-channel prefix field (slot0
+// or tile0, ascending over FLAC channels): binary search.
+template <typename F>
+__device__ __forceinline__ int32_t owner(const Chan *chans, const int32_t *flac_ch, int32_t n,
+                                         int64_t x, F &&first) {
+    int32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (first(chans[flac_ch[mid]]) <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return flac_ch[lo];
+}
 
 // Where decoded samples go.
 struct Out {
@@ -272,8 +296,86 @@ __device__ __forceinline__ int frame_header(Reader &r, int64_t at, int stream_bp
     return (int)(hl + 1);
 }
 
-// Writes sample t of a channel (the FLAC value v) into dest, combined as
-// Extract does.
+// The samples of one frame, frame-relative index i (sample t0 + i): only
+// i in [lo, hi) -- the requested range -- is written, combined as Extract
+// does (value, plus the channel's offsets, plus its CONST datum, in the
+// full-width integer type; float types scaled by the quantum).  When the
+// frame's row is 16-byte aligned, samples are gathered in registers and
+// written 16 bytes at a time: on this GPU a load's wait also waits for the
+// lane's earlier stores, so fewer, larger stores keep the bit reader's
+// prefetches from queuing behind them.
+template <int TYPE>
+struct Sink {
+    using E = typename std::conditional<TYPE == 1 || TYPE == 3, uint64_t, uint32_t>::type;
+    static constexpr int G = 32 / sizeof(E);   // samples per 32-byte group
+    uintptr_t base;     // address of sample t0 (i = 0) in its row
+    const E *off;       // offsets of sample t0 + lo, or NULL
+    E cval;
+    double quantum;
+    int lo, hi;
+    bool vec;
+    E g[G];             // the group's samples (g[G - 1] the newest)
+
+    __device__ __forceinline__ Sink(const Chan &c, const Out &o, int64_t t0, int block) {
+        const int64_t a = max(o.start, t0), b = min(o.stop, t0 + block);
+        lo = (int)(a - t0);
+        hi = (int)max(b - t0, a - t0);
+        base = reinterpret_cast<uintptr_t>(o.dest) +
+               (uintptr_t)((c.dest_row * o.stride + (t0 - o.start)) * (int64_t)sizeof(E));
+        off = c.off_row >= 0 ? static_cast<const E *>(o.offs) + c.off_row * o.n_samp + a : nullptr;
+        cval = c.has_const ? (E)c.cval : (E)0;
+        quantum = c.quantum;
+        vec = (base & 15) == 0;
+#pragma unroll
+        for (int j = 0; j < G; j++) g[j] = 0;
+    }
+    __device__ __forceinline__ E value(int i, int64_t x) const {
+        E v = (E)x;
+        if (off && i >= lo && i < hi) v += off[i - lo];
+        v += cval;
+        if constexpr (TYPE == 3) return (E)__double_as_longlong((double)(int64_t)v * quantum);
+        else if constexpr (TYPE == 2)
+            return (E)__float_as_uint((float)((double)(float)(int32_t)v * quantum));
+        else return v;
+    }
+    __device__ __forceinline__ void store1(int i, E v) const {
+        if (i >= lo && i < hi) reinterpret_cast<E *>(base)[i] = v;
+    }
+    // The group ending at sample `last` (g[G - 1]), of which samples
+    // i0 = last - G + 1 + skip .. last are valid.
+    __device__ __forceinline__ void flush(int last) const {
+        const int i0 = last - (G - 1);
+        if (i0 >= lo && last < hi && (i0 & (G - 1)) == 0) {
+            uint4 *p = reinterpret_cast<uint4 *>(base + (uintptr_t)i0 * sizeof(E));
+            if constexpr (G == 8) {
+                p[0] = uint4{g[0], g[1], g[2], g[3]};
+                p[1] = uint4{g[4], g[5], g[6], g[7]};
+            } else {
+                p[0] = uint4{(uint32_t)g[0], (uint32_t)(g[0] >> 32), (uint32_t)g[1],
+                             (uint32_t)(g[1] >> 32)};
+                p[1] = uint4{(uint32_t)g[2], (uint32_t)(g[2] >> 32), (uint32_t)g[3],
+                             (uint32_t)(g[3] >> 32)};
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < G; j++) store1(i0 + j, g[j]);
+        }
+    }
+    __device__ __forceinline__ void put(int i, int64_t x) {
+        const E v = value(i, x);
+        if (!vec) { store1(i, v); return; }
+#pragma unroll
+        for (int j = 0; j < G - 1; j++) g[j] = g[j + 1];
+        g[G - 1] = v;
+        // A group closes at its last sample or at the last written sample;
+        // a partial group's stale entries fall outside [lo, hi) or are
+        // rewritten with the same values.
+        if ((i & (G - 1)) == G - 1 || i == hi - 1) flush(i);
+    }
+};
+
+// Writes sample t of a channel (the value v) into dest; the per-sample form
+// of Sink::put for k_g3s_plain.
 template <int TYPE>
 __device__ __forceinline__ void emit(const Chan &c, const Out &o, int64_t t, uint64_t v_bits) {
     if (t < o.start || t >= o.stop) return;
@@ -294,69 +396,89 @@ __device__ __forceinline__ void emit(const Chan &c, const Out &o, int64_t t, uin
     }
 }
 
-__device__ __forceinline__ uint64_t shifted(int32_t x, int wasted) {
-    return (uint64_t)((int64_t)x << wasted);
+// One Rice code with parameter k (<= 30): zigzag-decoded residual.  Codes
+// of at most 32 bits (nearly all) are taken from the bit buffer after a
+// single refill.
+__device__ __forceinline__ int32_t rice(Reader &r, uint32_t k) {
+    r.refill();
+    const int z = __clzll(r.buf);
+    uint32_t u;
+    if (r.buf != 0 && z + 1 + (int)k <= 32) {
+        const uint64_t t = (r.buf << z) << 1;
+        const uint32_t rem = k ? (uint32_t)(t >> (64 - k)) : 0u;
+        r.buf = t << k;
+        r.bits -= z + 1 + (int)k;
+        u = ((uint32_t)z << k) | rem;
+    } else {
+        const uint32_t q = r.unary();
+        u = (q << k) | r.get((int)k);
+    }
+    return (int32_t)(u >> 1) ^ -(int32_t)(u & 1);
 }
 
 // Residual partitions of a subframe; pred(residual) returns the sample and
-// updates the predictor's state.
-template <int TYPE, typename P>
-__device__ __forceinline__ bool residuals(Reader &r, const Chan &c, const Out &o, int block, int order,
-                          int wasted, int64_t t0, P &&pred) {
+// updates the predictor's state.  One loop over the samples (a partition's
+// parameter is read when its first sample comes up): lanes of a wave decode
+// different frames, and per-partition loops would run out of step whenever
+// their partition orders differ.
+template <typename S, typename P>
+__device__ __forceinline__ bool residuals(Reader &r, S &sink, int block, int order, int wasted,
+                                          P &&pred) {
     const int method = (int)r.get(2);
     if (method > 1) return false;
     const int porder = (int)r.get(4);
-    const int parts = 1 << porder;
-    if ((block & (parts - 1)) != 0 || (block >> porder) < order) return false;
+    const int psize = block >> porder;
+    if ((block & ((1 << porder) - 1)) != 0 || psize < order) return false;
     const uint32_t escape = method ? 31u : 15u;
-    int i = order;
-    for (int p = 0; p < parts; p++) {
-        const int cnt = (block >> porder) - (p == 0 ? order : 0);
-        const uint32_t k = r.get(method ? 5 : 4);
-        const int nb = k == escape ? (int)r.get(5) : 0;
-        for (int j = 0; j < cnt; j++, i++) {
-            int32_t res;
-            if (k == escape) {
-                res = r.sget(nb);
-            } else {
-                const uint32_t q = r.unary();
-                const uint32_t u = (q << k) | r.get((int)k);
-                res = (int32_t)(u >> 1) ^ -(int32_t)(u & 1);
-            }
-            emit<TYPE>(c, o, t0 + i, shifted(pred(res), wasted));
+    int boundary = order;   // first sample of the next partition (the first: `order`)
+    uint32_t k = 0;
+    int nb = -1;            // raw width of an escaped partition, -1 for Rice
+    for (int i = order; i < block; i++) {
+        if (i == boundary) {
+            k = r.get(method ? 5 : 4);
+            nb = k == escape ? (int)r.get(5) : -1;
+            boundary = (i < psize ? psize : boundary + psize);
         }
-        if (r.err) return false;
+        const int32_t res = nb >= 0 ? r.sget(nb) : rice(r, k);
+        sink.put(i, (int64_t)pred(res) << wasted);
     }
-    return true;
+    return !r.err;
 }
 
-template <int TYPE>
-__device__ __forceinline__ bool fixed_body(Reader &r, const Chan &c, const Out &o, int block, int sbps,
-                           int wasted, int order, int64_t t0) {
+// FIXED predictor of order 0..4 as the coefficients of its differences
+// (x = r + c1 x1 + c2 x2 + c3 x3 + c4 x4), in 32-bit arithmetic when the
+// samples are narrow enough that no term overflows, else in 64-bit.
+template <typename S>
+__device__ __forceinline__ bool fixed_body(Reader &r, S &sink, int block, int sbps, int wasted,
+                                           int order) {
     int32_t x1 = 0, x2 = 0, x3 = 0, x4 = 0;
     for (int i = 0; i < order; i++) {
         const int32_t x = r.sget(sbps);
         x4 = x3; x3 = x2; x2 = x1; x1 = x;
-        emit<TYPE>(c, o, t0 + i, shifted(x, wasted));
+        sink.put(i, (int64_t)x << wasted);
     }
-    return residuals<TYPE>(r, c, o, block, order, wasted, t0, [&](int32_t res) {
-        int64_t p;
-        switch (order) {
-        case 0: p = 0; break;
-        case 1: p = x1; break;
-        case 2: p = 2 * (int64_t)x1 - x2; break;
-        case 3: p = 3 * (int64_t)x1 - 3 * (int64_t)x2 + x3; break;
-        default: p = 4 * (int64_t)x1 - 6 * (int64_t)x2 + 4 * (int64_t)x3 - x4; break;
-        }
-        const int32_t x = (int32_t)(res + p);
+    const int c1 = order == 0 ? 0 : order == 1 ? 1 : order == 2 ? 2 : order == 3 ? 3 : 4;
+    const int c2 = order <= 1 ? 0 : order == 2 ? -1 : order == 3 ? -3 : -6;
+    const int c3 = order <= 2 ? 0 : order == 3 ? 1 : 4;
+    const int c4 = order <= 3 ? 0 : -1;
+    if (sbps <= 26) {
+        return residuals(r, sink, block, order, wasted, [&](int32_t res) {
+            const int32_t x = res + c1 * x1 + c2 * x2 + c3 * x3 + c4 * x4;
+            x4 = x3; x3 = x2; x2 = x1; x1 = x;
+            return x;
+        });
+    }
+    return residuals(r, sink, block, order, wasted, [&](int32_t res) {
+        const int32_t x = (int32_t)((int64_t)res + (int64_t)c1 * x1 + (int64_t)c2 * x2 +
+                                    (int64_t)c3 * x3 + (int64_t)c4 * x4);
         x4 = x3; x3 = x2; x2 = x1; x1 = x;
         return x;
     });
 }
 
-template <int TYPE, int MAXO>
-__device__ __forceinline__ bool lpc_body(Reader &r, const Chan &c, const Out &o, int block, int sbps,
-                         int wasted, int order, int64_t t0) {
+template <int MAXO, typename S>
+__device__ __forceinline__ bool lpc_body(Reader &r, S &sink, int block, int sbps, int wasted,
+                                         int order) {
     int32_t h[MAXO];     // h[j] = x[i - 1 - j]
     int32_t coef[MAXO];
 #pragma unroll
@@ -366,7 +488,7 @@ __device__ __forceinline__ bool lpc_body(Reader &r, const Chan &c, const Out &o,
 #pragma unroll
         for (int j = MAXO - 1; j > 0; j--) h[j] = h[j - 1];
         h[0] = x;
-        emit<TYPE>(c, o, t0 + i, shifted(x, wasted));
+        sink.put(i, (int64_t)x << wasted);
     }
     const int prec = (int)r.get(4) + 1;
     const int shift = r.sget(5);
@@ -377,7 +499,7 @@ __device__ __forceinline__ bool lpc_body(Reader &r, const Chan &c, const Out &o,
         for (int m = 0; m < MAXO; m++)
             if (m == j) coef[m] = q;
     }
-    return residuals<TYPE>(r, c, o, block, order, wasted, t0, [&](int32_t res) {
+    return residuals(r, sink, block, order, wasted, [&](int32_t res) {
         int64_t sum = 0;
 #pragma unroll
         for (int m = 0; m < MAXO; m++) sum += (int64_t)coef[m] * h[m];
@@ -394,9 +516,10 @@ __device__ __forceinline__ bool lpc_body(Reader &r, const Chan &c, const Out &o,
 // (checked before anything is written).  Returns the byte offset just past
 // the frame (CRC-16 included), or -1 with a reason in why.
 template <int TYPE, int MAXO>
-__device__ __forceinline__ int64_t decode_frame(Reader &r, const Chan &c, const Out &o, int64_t at,
-                                int64_t want_num, int want_block, int64_t t0,
-                                const uint16_t *crc_tab, Frame &f, int &why) {
+__device__ __forceinline__ int64_t decode_frame(Reader &r, const Chan &c, const Out &o,
+                                                int64_t at, int64_t want_num, int want_block,
+                                                int64_t t0, const uint16_t *crc_tab, Frame &f,
+                                                int &why) {
     why = kWhyBadFrame;
     const int hl = frame_header(r, at, c.bps, f);
     if (hl < 0) return -1;
@@ -410,21 +533,25 @@ __device__ __forceinline__ int64_t decode_frame(Reader &r, const Chan &c, const 
     if (r.get(1)) wasted = (int)r.unary() + 1;
     const int sbps = f.bps - wasted;
     if (sbps <= 0 || sbps > 32) return -1;
+    Sink<TYPE> sink(c, o, t0, f.block);
     bool ok;
     if (type == 0) {
-        const int32_t v = r.sget(sbps);
-        for (int i = 0; i < f.block; i++) emit<TYPE>(c, o, t0 + i, shifted(v, wasted));
+        const int64_t v = (int64_t)r.sget(sbps) << wasted;
+        for (int i = 0; i < f.block; i++) sink.put(i, v);
         ok = true;
     } else if (type == 1) {
-        for (int i = 0; i < f.block; i++) emit<TYPE>(c, o, t0 + i, shifted(r.sget(sbps), wasted));
+        for (int i = 0; i < f.block; i++) sink.put(i, (int64_t)r.sget(sbps) << wasted);
         ok = true;
     } else if (type >= 8 && type <= 12) {
         const int order = type - 8;
-        ok = order <= f.block && fixed_body<TYPE>(r, c, o, f.block, sbps, wasted, order, t0);
+        ok = order <= f.block && fixed_body(r, sink, f.block, sbps, wasted, order);
     } else if (type >= 32) {
         const int order = (type & 31) + 1;
         if (order > MAXO) { why = kWhyLpc; return -1; }
-        ok = order <= f.block && lpc_body<TYPE, MAXO>(r, c, o, f.block, sbps, wasted, order, t0);
+        if constexpr (MAXO > 0)
+            ok = order <= f.block && lpc_body<MAXO>(r, sink, f.block, sbps, wasted, order);
+        else
+            ok = false;
     } else {
         ok = false;
     }
@@ -446,50 +573,125 @@ __device__ void load_crc_table(uint16_t *tab) {
     __syncthreads();
 }
 
-// Sync scan: thread i tests kScanBytes byte offsets of one scan tile.
-// tiles[k] = (channel, first byte, end byte) of a slice of a FLAC region.
-__global__ __launch_bounds__(kFlBlock) void k_flac_sync(const uint8_t *__restrict__ payload,
+// Sync scan, first stage: thread i tests kScanBytes byte offsets of one
+// kTile-byte scan tile (a channel's tiles are consecutive from its tile0)
+// for the bytes a frame header of this stream starts with -- the
+// sync code, then header bytes 2 and 3 as in the stream's first frame
+// (block size / rate codes and channels / sample size codes; every frame
+// of a fixed-blocksize stream repeats them, the last may change its block
+// size code) -- and lists the offsets found (channel << 40 | offset).
+__global__ __launch_bounds__(kFlBlock) void k_flac_scan(const uint8_t *__restrict__ payload,
                                                         int64_t nbytes, const Chan *chans,
-                                                        const int64_t *tiles, int64_t n_tiles,
-                                                        uint32_t *slots, int32_t *flags) {
+                                                        const int32_t *flac_ch, int32_t n_fch,
+                                                        int64_t n_tiles, uint64_t *cand,
+                                                        uint32_t *n_cand, uint32_t cap) {
     const int64_t tile = blockIdx.x;
     if (tile >= n_tiles) return;
-    const int64_t ci = tiles[3 * tile], a0 = tiles[3 * tile + 1], a1 = tiles[3 * tile + 2];
-    const Chan c = chans[ci];
-    Reader r{payload, nbytes, c.flac_end + kSlack};
+    const int32_t ci = owner(chans, flac_ch, n_fch, tile, [](const Chan &c) { return c.tile0; });
+    const int64_t a0 = chans[ci].flac_pos + (tile - chans[ci].tile0) * kTile;
+    const int64_t a1 = min(chans[ci].flac_end, a0 + kTile);
+    const int64_t fpos = chans[ci].flac_pos, fend = chans[ci].flac_end;
+    const uint32_t b2 = fpos + 2 < nbytes ? payload[fpos + 2] : 0u;
+    const uint32_t b3 = fpos + 3 < nbytes ? payload[fpos + 3] : 0u;
     const int64_t lo = a0 + (int64_t)threadIdx.x * kScanBytes;
     const int64_t hi = min(a1, lo + kScanBytes);
-    for (int64_t a16 = lo & ~(int64_t)15; a16 < hi; a16 += 16) {
-        const uint4 v = r.chunk(a16);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const int64_t a = a16 + i;
-            if (((w[i >> 2] >> (8 * (i & 3))) & 0xFF) != 0xFF || a < lo || a >= hi) continue;
-            if (a + 1 >= c.flac_end || (r.byte(a + 1) & 0xFE) != 0xF8) continue;
-            Frame f;
-            if (frame_header(r, a, c.bps, f) < 0) continue;
-            // The first frame tells the blocking strategy; elsewhere a
-            // variable-blocksize header can only be a false match.
-            if (a == c.flac_pos && f.blocking) atomicOr(flags + ci, kWhyVariable);
-            if (!f.blocking && f.number < (uint64_t)c.n_frames)
-                atomicMin(slots + c.slot0 + f.number, (uint32_t)(a - c.flac_pos));
+    // Offsets base .. base + kScanBytes + 15 in 16-byte chunks (the range
+    // [lo, hi) starts up to 15 bytes past base), each tested with the first
+    // word of the next chunk at hand for headers that cross into it; hits
+    // are kept as a bit mask (bit j: offset base + j).
+    constexpr int kChunks = kScanBytes / 16 + 1;
+    const int64_t base = lo & ~(int64_t)15;
+    auto ld = [&](int64_t c) {
+        if (lo >= hi || c >= nbytes) return uint4{0u, 0u, 0u, 0u};
+        if (c + 16 <= nbytes) return *reinterpret_cast<const uint4 *>(payload + c);
+        uint32_t t[4];
+        for (int j = 0; j < 4; j++) {
+            uint32_t x = 0;
+            for (int i = 3; i >= 0; i--)
+                x = (x << 8) | (c + 4 * j + i < nbytes ? payload[c + 4 * j + i] : 0u);
+            t[j] = x;
         }
+        return uint4{t[0], t[1], t[2], t[3]};
+    };
+    uint64_t mlo = 0;
+    uint32_t mhi = 0;
+    // Bounds relative to base, in 32 bits.
+    const int rl = (int)(lo - base), rh = (int)(hi - base);
+    const int rf = (int)min(fend - 3 - base, (int64_t)(1 << 20));
+    const int rp = (fpos >= base && fpos < base + 16 * kChunks) ? (int)(fpos - base) : -1;
+    uint4 cur = ld(base);
+#pragma unroll 1
+    for (int q = 0; q < kChunks; q++) {
+        const uint4 nxt = ld(base + 16 * (q + 1));
+        const uint32_t c[5] = {cur.x, cur.y, cur.z, cur.w, nxt.x};
+        uint32_t m = 0;
+#pragma unroll
+        for (int b = 0; b < 16; b++) {
+            auto B = [&](int k) { return (c[k >> 2] >> (8 * (k & 3))) & 0xFFu; };
+            const int j = 16 * q + b;
+            const bool h = B(b) == 0xFF && (B(b + 1) & 0xFE) == 0xF8 && j >= rl && j < rh &&
+                           j < rf &&
+                           (j == rp || (B(b + 3) == b3 && (B(b + 2) & 0x0F) == (b2 & 0x0F)));
+            m |= (uint32_t)h << b;
+        }
+        if (q < 4) mlo |= (uint64_t)m << (16 * q);
+        else mhi |= m;
+        cur = nxt;
     }
+    const uint32_t mine = (uint32_t)(__popcll(mlo) + __popc(mhi));
+    // One atomic per wave: lanes take consecutive slots after a wave prefix sum.
+    const int lane = threadIdx.x & 63;
+    uint32_t incl = mine;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d);
+        if (lane >= d) incl += y;
+    }
+    const uint32_t total = __shfl(incl, 63);
+    uint32_t first = 0;
+    if (lane == 63 && total) first = atomicAdd(n_cand, total);
+    first = __shfl(first, 63) + incl - mine;
+    for (; mlo; mlo &= mlo - 1, first++)
+        if (first < cap) cand[first] = ((uint64_t)ci << 40) | (uint64_t)(base + __ffsll((long long)mlo) - 1);
+    for (; mhi; mhi &= mhi - 1, first++)
+        if (first < cap) cand[first] = ((uint64_t)ci << 40) | (uint64_t)(base + 64 + __ffs(mhi) - 1);
 }
 
-// One lane per frame of the parallel pass.  frame_chan[g] = channel of
-// global frame g (frames of a channel are consecutive).
-template <int TYPE>
-__global__ __launch_bounds__(kFlBlock) void k_flac_frames(
+// Sync scan, second stage: one lane per listed offset parses the header
+// (valid fields, CRC-8); a header numbered k keeps the smallest offset in
+// slot[k].  The first frame decides whether the stream uses variable block
+// sizes; elsewhere a variable-blocksize header can only be a false match.
+__global__ __launch_bounds__(kFlBlock) void k_flac_headers(
     const uint8_t *__restrict__ payload, int64_t nbytes, const Chan *chans,
-    const int32_t *frame_chan, int64_t n_frames, const uint32_t *slots, Out o,
-    uint32_t *ends, int32_t *flags) {
-    __shared__ uint16_t crc_tab[256];
-    load_crc_table(crc_tab);
-    const int64_t g = (int64_t)blockIdx.x * kFlBlock + threadIdx.x;
-    if (g >= n_frames) return;
-    const int32_t ci = frame_chan[g];
+    const uint64_t *cand, const uint32_t *n_cand, uint32_t cap, uint32_t *slots,
+    int32_t *flags) {
+    const uint32_t n = min(*n_cand, cap);
+    const int64_t idx = (int64_t)blockIdx.x * kFlBlock + threadIdx.x;
+    if (idx >= n) return;
+    const uint64_t e = cand[idx];
+    const int64_t ci = (int64_t)(e >> 40), at = (int64_t)(e & ((1ull << 40) - 1));
+    const Chan c = chans[ci];
+    Reader r{payload, nbytes, c.flac_end + kSlack};
+    Frame f;
+    if (frame_header(r, at, c.bps, f) < 0) return;
+    if (at == c.flac_pos && f.blocking) atomicOr(flags + ci, kWhyVariable);
+    const bool last = f.number + 1 == (uint64_t)c.n_frames;
+    if (!f.blocking && f.number < (uint64_t)c.n_frames &&
+        (last || at == c.flac_pos || r.byte(at + 2) == r.byte(c.flac_pos + 2)))
+        atomicMin(slots + c.slot0 + f.number, (uint32_t)(at - c.flac_pos));
+}
+
+// The frame-parallel task of global frame g (frames of a channel are
+// consecutive from its slot0; flac_ch lists the FLAC channels in order).  MAXO = 0: FIXED frames only,
+// an LPC frame is appended to lpc_list for the LPC-capable pass.
+template <int TYPE, int MAXO>
+__device__ __forceinline__ void frame_task(int64_t g, const uint8_t *__restrict__ payload,
+                                           int64_t nbytes, const Chan *chans,
+                                           const int32_t *flac_ch, int32_t n_fch,
+                                           const uint32_t *slots,
+                                           const Out &o, uint32_t *ends, int32_t *flags,
+                                           uint32_t *lpc_list, uint32_t *lpc_count,
+                                           const uint16_t *crc_tab) {
+    const int32_t ci = owner(chans, flac_ch, n_fch, g, [](const Chan &c) { return c.slot0; });
     const Chan c = chans[ci];
     if (flags[ci] & kWhyVariable) return;
     const int64_t k = g - c.slot0;
@@ -504,21 +706,55 @@ __global__ __launch_bounds__(kFlBlock) void k_flac_frames(
     Frame f;
     int why;
     const int want_block = (int)min((int64_t)c.block, o.n_samp - t0);
-    const int64_t end = decode_frame<TYPE, kMaxLpcPar>(r, c, o, c.flac_pos + s, k, want_block,
-                                                       t0, crc_tab, f, why);
+    const int64_t end = decode_frame<TYPE, MAXO>(r, c, o, c.flac_pos + s, k, want_block, t0,
+                                                 crc_tab, f, why);
     if (end >= 0) {
         ends[g] = (uint32_t)(end - c.flac_pos);
         int64_t want_end = c.flac_end;
         if (k + 1 < c.n_frames) want_end = slots[g + 1] == kNone ? -1 : c.flac_pos + slots[g + 1];
         if (end != want_end) why = kWhyChain;
+    } else if (MAXO == 0 && why == kWhyLpc) {
+        lpc_list[atomicAdd(lpc_count, 1u)] = (uint32_t)g;
+        return;
     }
     if (why) atomicOr(flags + ci, why);
 }
 
-// One lane per channel: the frames in sequence from the first.  A frame
-// that starts where its predecessor ended and decoded correctly in the
-// parallel pass (ends[] set) is skipped; any other is decoded here.
-// err[i] = 1 on a malformed stream.
+// Frame-parallel pass over every frame, FIXED predictors only (LPC frames
+// are listed for k_flac_frames_lpc): one lane per frame.
+template <int TYPE>
+__global__ __launch_bounds__(kFlBlock) void k_flac_frames(
+    const uint8_t *__restrict__ payload, int64_t nbytes, const Chan *chans,
+    const int32_t *flac_ch, int32_t n_fch, int64_t n_frames, const uint32_t *slots, Out o,
+    uint32_t *ends, int32_t *flags, uint32_t *lpc_list, uint32_t *lpc_count) {
+    __shared__ uint16_t crc_tab[256];
+    load_crc_table(crc_tab);
+    const int64_t g = (int64_t)blockIdx.x * kFlBlock + threadIdx.x;
+    if (g >= n_frames) return;
+    frame_task<TYPE, 0>(g, payload, nbytes, chans, flac_ch, n_fch, slots, o, ends, flags,
+                        lpc_list, lpc_count, crc_tab);
+}
+
+// The listed LPC frames (orders up to kMaxLpcPar), one lane each.
+template <int TYPE>
+__global__ __launch_bounds__(kFlBlock) void k_flac_frames_lpc(
+    const uint8_t *__restrict__ payload, int64_t nbytes, const Chan *chans,
+    const int32_t *flac_ch, int32_t n_fch, const uint32_t *slots, Out o, uint32_t *ends,
+    int32_t *flags, const uint32_t *lpc_list, const uint32_t *lpc_count) {
+    __shared__ uint16_t crc_tab[256];
+    const uint32_t n = *lpc_count;
+    if ((int64_t)blockIdx.x * kFlBlock >= n) return;   // whole block idle
+    load_crc_table(crc_tab);
+    const int64_t idx = (int64_t)blockIdx.x * kFlBlock + threadIdx.x;
+    if (idx >= n) return;
+    frame_task<TYPE, kMaxLpcPar>(lpc_list[idx], payload, nbytes, chans, flac_ch, n_fch, slots,
+                                 o, ends, flags, nullptr, nullptr, crc_tab);
+}
+
+// One wave per channel: the frames in sequence from the first.  Runs of
+// frames that start where their predecessor ended and decoded correctly in
+// the parallel passes (ends[] set) are skipped 64 at a time (ballot); any
+// other frame is decoded here by lane 0.  err[i] = 1 on a malformed stream.
 template <int TYPE>
 __global__ __launch_bounds__(64) void k_flac_walk(
     const uint8_t *__restrict__ payload, int64_t nbytes, const Chan *chans,
@@ -526,28 +762,49 @@ __global__ __launch_bounds__(64) void k_flac_walk(
     const int32_t *flags, Out o, int32_t *err) {
     __shared__ uint16_t crc_tab[256];
     load_crc_table(crc_tab);
-    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int64_t i = blockIdx.x;
     if (i >= n) return;
+    const int lane = threadIdx.x;
     const int32_t ci = which[i];
     const Chan c = chans[ci];
     const bool fixed = !(flags[ci] & kWhyVariable);
-    Reader r{payload, nbytes, c.flac_end + kSlack};
-    int64_t at = c.flac_pos, t0 = 0;
-    for (int64_t k = 0; at < c.flac_end && t0 < o.stop; k++) {
+    int64_t at = c.flac_pos, t0 = 0, k = 0;
+    while (at < c.flac_end && t0 < o.stop) {
         if (fixed && k < c.n_frames) {
-            const int64_t g = c.slot0 + k;
-            if (slots[g] == (uint32_t)(at - c.flac_pos) && ends[g] != kNone) {
-                at = c.flac_pos + ends[g];
-                t0 += min((int64_t)c.block, o.n_samp - t0);
+            const int64_t kk = k + lane;
+            bool ok = false;
+            if (kk < c.n_frames) {
+                const int64_t g = c.slot0 + kk;
+                const uint32_t want = kk == k ? (uint32_t)(at - c.flac_pos) : ends[g - 1];
+                ok = ends[g] != kNone && slots[g] == want;
+            }
+            const uint64_t m = __ballot(ok);
+            const int run = m == ~0ull ? 64 : __ffsll((long long)~m) - 1;
+            if (run > 0) {
+                k += run;
+                at = c.flac_pos + ends[c.slot0 + k - 1];
+                t0 = min(k * (int64_t)c.block, o.n_samp);
                 continue;
             }
         }
-        Frame f;
-        int why;
-        const int64_t next = decode_frame<TYPE, 32>(r, c, o, at, -1, 0, t0, crc_tab, f, why);
-        if (next < 0) { err[i] = 1; return; }
+        int64_t next = -1;
+        int blk = 0;
+        if (lane == 0) {
+            Reader r{payload, nbytes, c.flac_end + kSlack};
+            Frame f;
+            int why;
+            next = decode_frame<TYPE, 32>(r, c, o, at, -1, 0, t0, crc_tab, f, why);
+            blk = f.block;
+        }
+        next = __shfl(next, 0);
+        blk = __shfl(blk, 0);
+        if (next < 0) {
+            if (lane == 0) err[i] = 1;
+            return;
+        }
         at = next;
-        t0 += f.block;
+        t0 += blk;
+        k++;
     }
 }
 
@@ -614,58 +871,61 @@ int decode_typed(const uint8_t *payload, int64_t nbytes, const std::vector<Chan>
     const int64_t n = (int64_t)ch.size();
     g_fl_stats[0] = g_fl_stats[1] = g_fl_stats[2] = 0;
     std::vector<Chan> chans = ch;
-    std::vector<int32_t> frame_chan, plain;
-    std::vector<int64_t> tiles;
-    int64_t n_frames = 0;
+    std::vector<int32_t> flac_ch, plain;
+    int64_t n_frames = 0, n_tiles = 0;
     for (int64_t i = 0; i < n; i++) {
         Chan &c = chans[i];
         if (c.flac_pos < 0) { plain.push_back((int32_t)i); c.n_frames = 0; continue; }
         c.slot0 = n_frames;
         c.n_frames = (int32_t)((o.n_samp + c.block - 1) / c.block);
-        frame_chan.insert(frame_chan.end(), (size_t)c.n_frames, (int32_t)i);
         n_frames += c.n_frames;
-        const int64_t tile = (int64_t)kFlBlock * kScanBytes;
-        for (int64_t a = c.flac_pos; a < c.flac_end; a += tile) {
-            tiles.push_back(i);
-            tiles.push_back(a);
-            tiles.push_back(std::min(c.flac_end, a + tile));
-        }
+        c.tile0 = n_tiles;
+        n_tiles += (c.flac_end - c.flac_pos + kTile - 1) / kTile;
+        flac_ch.push_back((int32_t)i);
     }
-    const int64_t n_tiles = (int64_t)tiles.size() / 3;
-    // Workspace: chans | tiles | frame_chan | slots | ends | flags | which | err
+    const int32_t n_fch = (int32_t)flac_ch.size();
+    // Workspace: chans | flac channels | slots | ends | lpc list | flags |
+    // which | err | counters | header candidates
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t b_ch = up(n * sizeof(Chan)), b_ti = up(tiles.size() * 8),
-                 b_fc = up(frame_chan.size() * 4), b_sl = up(n_frames * 4), b_en = b_sl,
-                 b_fl = up(n * 4), b_wh = up(n * 4), b_er = up(n * 4);
+    const size_t b_ch = up(n * sizeof(Chan)), b_fc = up(flac_ch.size() * 4),
+                 b_sl = up(n_frames * 4), b_en = b_sl, b_ll = b_sl, b_fl = up(n * 4),
+                 b_wh = up(n * 4), b_er = up(n * 4), b_lc = 256;
+    // Header candidates: frames plus a margin for false matches; a list that
+    // overflows sends every channel through the walk.
+    const uint32_t cap = (uint32_t)std::min<int64_t>(2 * n_frames + 65536, 1ll << 31);
+    const size_t b_ca = up((size_t)cap * 8);
     void *ws = nullptr;
-    int rc = workspace(b_ch + b_ti + b_fc + b_sl + b_en + b_fl + b_wh + b_er, &ws);
+    int rc = workspace(b_ch + b_fc + b_sl + b_en + b_ll + b_fl + b_wh + b_er + b_lc + b_ca, &ws);
     if (rc != SO3G_OK) return rc;
     char *w = static_cast<char *>(ws);
     Chan *d_ch = reinterpret_cast<Chan *>(w);
     w += b_ch;
-    int64_t *d_ti = reinterpret_cast<int64_t *>(w);
-    w += b_ti;
     int32_t *d_fc = reinterpret_cast<int32_t *>(w);
     w += b_fc;
     uint32_t *d_sl = reinterpret_cast<uint32_t *>(w);
     w += b_sl;
     uint32_t *d_en = reinterpret_cast<uint32_t *>(w);
     w += b_en;
+    uint32_t *d_ll = reinterpret_cast<uint32_t *>(w);
+    w += b_ll;
     int32_t *d_fl = reinterpret_cast<int32_t *>(w);
     w += b_fl;
     int32_t *d_wh = reinterpret_cast<int32_t *>(w);
     w += b_wh;
     int32_t *d_er = reinterpret_cast<int32_t *>(w);
+    w += b_er;
+    uint32_t *d_lc = reinterpret_cast<uint32_t *>(w);   // [0] lpc count, [1] candidates
+    uint32_t *d_nc = d_lc + 1;
+    w += b_lc;
+    uint64_t *d_ca = reinterpret_cast<uint64_t *>(w);
     FL_TRY(hipMemcpyAsync(d_ch, chans.data(), n * sizeof(Chan), hipMemcpyHostToDevice, s));
-    if (n_tiles)
-        FL_TRY(hipMemcpyAsync(d_ti, tiles.data(), tiles.size() * 8, hipMemcpyHostToDevice, s));
-    if (n_frames)
-        FL_TRY(hipMemcpyAsync(d_fc, frame_chan.data(), frame_chan.size() * 4,
-                              hipMemcpyHostToDevice, s));
+    if (n_fch)
+        FL_TRY(hipMemcpyAsync(d_fc, flac_ch.data(), flac_ch.size() * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, d_sl, n_frames, kNone);
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, d_en, n_frames, kNone);
     hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, s, reinterpret_cast<uint32_t *>(d_fl),
                        n, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, d_lc, (int64_t)2, 0u);
     if (!plain.empty()) {
         FL_TRY(hipMemcpyAsync(d_wh, plain.data(), plain.size() * 4, hipMemcpyHostToDevice, s));
         const int64_t work = (int64_t)plain.size() * (o.stop - o.start);
@@ -676,15 +936,24 @@ int decode_typed(const uint8_t *payload, int64_t nbytes, const std::vector<Chan>
         FL_TRY(hipStreamSynchronize(s));
     }
     if (n_frames > 0) {
-        hipLaunchKernelGGL(k_flac_sync, dim3((unsigned)n_tiles), dim3(kFlBlock), 0, s, payload,
-                           nbytes, d_ch, d_ti, n_tiles, d_sl, d_fl);
+        hipLaunchKernelGGL(k_flac_scan, dim3((unsigned)n_tiles), dim3(kFlBlock), 0, s, payload,
+                           nbytes, d_ch, d_fc, n_fch, n_tiles, d_ca, d_nc, cap);
+        hipLaunchKernelGGL(k_flac_headers, dim3(blocks_for(cap, kFlBlock)), dim3(kFlBlock), 0, s,
+                           payload, nbytes, d_ch, d_ca, d_nc, cap, d_sl, d_fl);
         hipLaunchKernelGGL(k_flac_frames<TYPE>, dim3(blocks_for(n_frames, kFlBlock)),
-                           dim3(kFlBlock), 0, s, payload, nbytes, d_ch, d_fc, n_frames, d_sl, o,
-                           d_en, d_fl);
+                           dim3(kFlBlock), 0, s, payload, nbytes, d_ch, d_fc, n_fch, n_frames,
+                           d_sl, o, d_en, d_fl, d_ll, d_lc);
+        hipLaunchKernelGGL(k_flac_frames_lpc<TYPE>, dim3(blocks_for(n_frames, kFlBlock)),
+                           dim3(kFlBlock), 0, s, payload, nbytes, d_ch, d_fc, n_fch, d_sl, o, d_en,
+                           d_fl, d_ll, d_lc);
         FL_TRY(hipGetLastError());
         std::vector<int32_t> flags(n);
+        uint32_t n_cand = 0;
         FL_TRY(hipMemcpyAsync(flags.data(), d_fl, n * 4, hipMemcpyDeviceToHost, s));
+        FL_TRY(hipMemcpyAsync(&n_cand, d_nc, 4, hipMemcpyDeviceToHost, s));
         FL_TRY(hipStreamSynchronize(s));
+        if (n_cand > cap)   // candidates dropped: no frame can be trusted
+            for (int64_t i = 0; i < n; i++) flags[i] |= kWhyNoSync;
         std::vector<int32_t> redo;
         for (int64_t i = 0; i < n; i++)
             if (flags[i] && chans[i].flac_pos >= 0) {
@@ -698,8 +967,8 @@ int decode_typed(const uint8_t *payload, int64_t nbytes, const std::vector<Chan>
             FL_TRY(hipMemcpyAsync(d_wh, redo.data(), nr * 4, hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, s,
                                reinterpret_cast<uint32_t *>(d_er), nr, 0u);
-            hipLaunchKernelGGL(k_flac_walk<TYPE>, dim3(blocks_for(nr, 64)), dim3(64), 0, s,
-                               payload, nbytes, d_ch, d_wh, nr, d_sl, d_en, d_fl, o, d_er);
+            hipLaunchKernelGGL(k_flac_walk<TYPE>, dim3((unsigned)nr), dim3(64), 0, s, payload,
+                               nbytes, d_ch, d_wh, nr, d_sl, d_en, d_fl, o, d_er);
             FL_TRY(hipGetLastError());
             std::vector<int32_t> err(nr);
             FL_TRY(hipMemcpyAsync(err.data(), d_er, nr * 4, hipMemcpyDeviceToHost, s));
@@ -760,6 +1029,7 @@ extern "C" int so3g_g3s_decode(const uint8_t *payload, int64_t payload_bytes,
         c.quantum = a.quantum;
         c.dest_row = a.dest_row;
         c.slot0 = 0;
+        c.tile0 = 0;
         c.n_frames = 0;
         c.block = a.block;
         c.bps = a.bps;

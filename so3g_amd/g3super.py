@@ -45,6 +45,11 @@ class Chan(ctypes.Structure):
                 ('pad_', ctypes.c_int32)]
 
 
+CHAN_DTYPE = np.dtype([(name, np.int64 if ct is ctypes.c_int64 else np.float64
+                         if ct is ctypes.c_double else np.int32) for name, ct in Chan._fields_])
+assert CHAN_DTYPE.itemsize == ctypes.sizeof(Chan)
+
+
 def _workers():
     return max(1, min(16, os.cpu_count() or 1))
 
@@ -141,6 +146,9 @@ class G3SuperTimestream:
         self._array = None
         self._blob = None            # (payload bytes, offsets list)
         self._dev_payload = None     # device copy of the payload (torch uint8)
+        self._table = None           # parsed channel table of the blob
+        self._offset_rows = []       # inflated bz2 offsets, rows of the table
+        self._dev_offsets = None
         self._dtype = None
         self._shape = None
         self._opts = {}
@@ -208,6 +216,9 @@ class G3SuperTimestream:
         self._array = a
         self._blob = None
         self._dev_payload = None
+        self._table = None
+        self._offset_rows = []
+        self._dev_offsets = None
         self._dataful = True
         self._float_mode = floaty
         self._dtype = a.dtype
@@ -289,6 +300,9 @@ class G3SuperTimestream:
         self._blob = self._encode_array(self._array)
         self._array = None
         self._dev_payload = None
+        self._table = None
+        self._offset_rows = []
+        self._dev_offsets = None
         return True
 
     def _encode_array(self, a):
@@ -398,6 +412,9 @@ class G3SuperTimestream:
         self._array = None
         self._blob = (bytes(payload), [int(o) for o in offsets])
         self._dev_payload = None
+        self._table = None
+        self._offset_rows = []
+        self._dev_offsets = None
 
     # -- decode / extract (:636-812) -----------------------------------------
     def decode(self):
@@ -410,6 +427,9 @@ class G3SuperTimestream:
         self._array = out
         self._blob = None
         self._dev_payload = None
+        self._table = None
+        self._offset_rows = []
+        self._dev_offsets = None
         return True
 
     def extract(self, dest, dest_indices=None, src_indices=None, start=0, stop=-1):
@@ -454,14 +474,16 @@ class G3SuperTimestream:
         if self._blob is None:
             raise ValueError('These data are already fully decoded; use .data.')
         torch = _torch()
-        chans, offs = self._channels(src_indices, dest_indices, n_ex, shape[0])
-        if not chans:
+        table = self._channel_table()
+        si = np.arange(n_ex) if src_indices is None else src_indices
+        di = np.arange(n_ex) if dest_indices is None else dest_indices
+        keep = (si >= 0) & (si < n_det) & (di >= 0) & (di < shape[0])
+        chans = table[si[keep]]
+        chans['dest_row'] = di[keep]
+        if len(chans) == 0:
             return True
         payload = self._device_payload(torch)
-        if offs:
-            d_offs = torch.from_numpy(np.stack(offs)).to(payload.device)
-        else:
-            d_offs = None
+        d_offs = self._device_offsets(torch)
         if is_torch:
             d_dest = dest
             stride = dest.stride(0)
@@ -469,12 +491,12 @@ class G3SuperTimestream:
             tdt = getattr(torch, self._dtype.name)
             d_dest = torch.from_numpy(np.ascontiguousarray(dest)).to(payload.device, tdt)
             stride = m
-        arr = (Chan * len(chans))(*chans)
+        chans = np.ascontiguousarray(chans)
         N.check(N.load().so3g_g3s_decode(
-            payload.data_ptr(), self._payload_len, ctypes.addressof(arr), len(chans),
+            payload.data_ptr(), self._payload_len, chans.ctypes.data, len(chans),
             TYPE_CODE[self._dtype.name], n_samp, start, stop,
-            None if d_offs is None else d_offs.data_ptr(), len(offs), d_dest.data_ptr(),
-            shape[0], stride, _stream(torch)))
+            None if d_offs is None else d_offs.data_ptr(), len(self._offset_rows),
+            d_dest.data_ptr(), shape[0], stride, _stream(torch)))
         if not is_torch:
             dest[...] = d_dest.cpu().numpy()
         return True
@@ -490,48 +512,53 @@ class G3SuperTimestream:
             self._payload_len = len(p)
         return self._dev_payload
 
-    def _channels(self, src_indices, dest_indices, n_ex, dest_rows):
-        """so3g_g3s_chan entries of the extracted channels and the inflated
-        bz2 offset rows they use."""
+    def _device_offsets(self, torch):
+        if self._dev_offsets is None and self._offset_rows:
+            self._dev_offsets = torch.from_numpy(np.stack(self._offset_rows)).to('cuda')
+        return self._dev_offsets
+
+    def _channel_table(self):
+        """so3g_g3s_chan entries of every channel of the blob (dest_row
+        unset), parsed once per blob: the algo byte and block lengths, each
+        FLAC stream's STREAMINFO, the inflated bz2 offset rows
+        (self._offset_rows) and CONST data."""
+        if self._table is not None:
+            return self._table
         payload, offsets = self._blob
         n_det, n_samp = self._shape
         item = self._dtype.itemsize
         itype = np.int64 if item == 8 else np.int32
-        chans, offs = [], []
-        for i in range(n_ex):
-            si = int(src_indices[i]) if src_indices is not None else i
-            if si < 0 or si >= n_det:
-                continue
-            di = int(dest_indices[i]) if dest_indices is not None else i
-            if di < 0 or di >= dest_rows:
-                continue
+        t = np.zeros(n_det, CHAN_DTYPE)
+        t['flac_pos'] = t['flac_end'] = t['raw_pos'] = t['off_row'] = -1
+        t['quantum'] = np.asarray(self._quanta, np.float64) if self._float_mode else 1.0
+        rows = []
+        mv = memoryview(payload)
+        for si in range(n_det):
             at = offsets[si]
-            algo = int(np.frombuffer(payload, np.int8, 1, at)[0])
+            algo = payload[at] if payload[at] < 128 else payload[at] - 256
             at += 1
-            c = Chan(flac_pos=-1, flac_end=-1, raw_pos=-1, off_row=-1, const_bits=0,
-                     quantum=float(self._quanta[si]) if self._float_mode else 1.0,
-                     dest_row=di, block=0, bps=0, has_const=0, pad_=0)
             if algo == ALGO_NONE:
-                c.raw_pos = at
+                t['raw_pos'][si] = at
             if algo & ALGO_DO_FLAC:
-                n = int(np.frombuffer(payload, np.int32, 1, at)[0])
-                info = streaminfo(memoryview(payload)[at + 4:at + 4 + min(n, 1 << 16)])
+                n = int.from_bytes(payload[at:at + 4], 'little', signed=True)
+                info = streaminfo(mv[at + 4:at + 4 + min(n, 1 << 16)])
                 if info is None:
                     raise ValueError(f'channel {si}: FLAC block without a stream header')
-                c.flac_pos = at + 4 + info[0]
-                c.flac_end = at + 4 + n
-                c.block, c.bps = info[1], info[2]
+                t['flac_pos'][si] = at + 4 + info[0]
+                t['flac_end'][si] = at + 4 + n
+                t['block'][si], t['bps'][si] = info[1], info[2]
                 at += 4 + n
             if algo & ALGO_DO_BZ:
-                n = int(np.frombuffer(payload, np.int32, 1, at)[0])
+                n = int.from_bytes(payload[at:at + 4], 'little', signed=True)
                 row = np.frombuffer(bz2.decompress(payload[at + 4:at + 4 + n]), itype)
                 if len(row) < n_samp:
                     raise ValueError(f'channel {si}: bz2 block shorter than the channel')
-                c.off_row = len(offs)
-                offs.append(row[:n_samp].copy())
+                t['off_row'][si] = len(rows)
+                rows.append(row[:n_samp].copy())
                 at += 4 + n
             if algo & ALGO_DO_CONST:
-                c.const_bits = int(np.frombuffer(payload, itype, 1, at)[0])
-                c.has_const = 1
-            chans.append(c)
-        return chans, offs
+                t['const_bits'][si] = int(np.frombuffer(payload, itype, 1, at)[0])
+                t['has_const'][si] = 1
+        self._table = t
+        self._offset_rows = rows
+        return t
