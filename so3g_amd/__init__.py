@@ -13,6 +13,7 @@ from .engine import make_engine_classes as _make_engine_classes
 from .engine import PrecompPlan  # noqa: F401
 from .batch import ObservationBatch  # noqa: F401
 from .array_ops import nmat_detvecs_apply, process_cuts, translate_cuts  # noqa: F401
+from .g3super import G3SuperTimestream  # noqa: F401
 from .proj.ranges import Ranges, RangesInt32, RangesMatrix  # noqa: F401
 
 globals().update(_make_engine_classes())

@@ -86,20 +86,19 @@ struct Chan {
     int32_t n_frames, block, bps, has_const;
 };
 
-constexpr int64_t kTile = (int64_t)kFlBlock * kScanBytes;   // bytes per scan tile
+constexpr int kScanIters = 16;   // kFlBlock * kScanBytes slices per scan block
+constexpr int64_t kTile = (int64_t)kFlBlock * kScanBytes * kScanIters;   // bytes per block
 
-// The channel owning global index x of a per-channel prefix field (slot0
-// or tile0, ascending over FLAC channels): binary search.
-template <typename F>
-__device__ __forceinline__ int32_t owner(const Chan *chans, const int32_t *flac_ch, int32_t n,
-                                         int64_t x, F &&first) {
+// Index j of the FLAC channel owning global index x of a per-channel prefix
+// (first[j] = first index of the j-th FLAC channel, ascending): binary search.
+__device__ __forceinline__ int32_t owner(const int64_t *first, int32_t n, int64_t x) {
     int32_t lo = 0, hi = n - 1;
     while (lo < hi) {
         const int32_t mid = (lo + hi + 1) >> 1;
-        if (first(chans[flac_ch[mid]]) <= x) lo = mid;
+        if (first[mid] <= x) lo = mid;
         else hi = mid - 1;
     }
-    return flac_ch[lo];
+    return lo;
 }
 
 // Where decoded samples go.
@@ -573,102 +572,118 @@ __device__ void load_crc_table(uint16_t *tab) {
     __syncthreads();
 }
 
-// Sync scan, first stage: thread i tests kScanBytes byte offsets of one
-// kTile-byte scan tile (a channel's tiles are consecutive from its tile0)
-// for the bytes a frame header of this stream starts with -- the
+// Sync scan, first stage: a block scans one kTile-byte tile (a channel's
+// tiles are consecutive from its tile0), a thread kScanBytes byte offsets
+// per pass, for the bytes a frame header of this stream starts with -- the
 // sync code, then header bytes 2 and 3 as in the stream's first frame
 // (block size / rate codes and channels / sample size codes; every frame
 // of a fixed-blocksize stream repeats them, the last may change its block
-// size code) -- and lists the offsets found (channel << 40 | offset).
+// size code) -- and lists the offsets found in the channel's region of the
+// candidate list (one counter per channel: appends rarely contend).
 __global__ __launch_bounds__(kFlBlock) void k_flac_scan(const uint8_t *__restrict__ payload,
                                                         int64_t nbytes, const Chan *chans,
-                                                        const int32_t *flac_ch, int32_t n_fch,
+                                                        const int32_t *flac_ch,
+                                                        const int64_t *tile0,
+                                                        const int64_t *cand0, int32_t n_fch,
                                                         int64_t n_tiles, uint64_t *cand,
-                                                        uint32_t *n_cand, uint32_t cap) {
+                                                        uint32_t *n_cand) {
     const int64_t tile = blockIdx.x;
     if (tile >= n_tiles) return;
-    const int32_t ci = owner(chans, flac_ch, n_fch, tile, [](const Chan &c) { return c.tile0; });
+    __shared__ int32_t s_j;
+    if (threadIdx.x == 0) s_j = owner(tile0, n_fch, tile);
+    __syncthreads();
+    const int32_t j = s_j, ci = flac_ch[j];
+    // The channel's candidate region: cand[cand0[j] .. cand0[j + 1]).
+    uint64_t *mine_list = cand + cand0[j];
+    const uint32_t cap = (uint32_t)(cand0[j + 1] - cand0[j]);
     const int64_t a0 = chans[ci].flac_pos + (tile - chans[ci].tile0) * kTile;
     const int64_t a1 = min(chans[ci].flac_end, a0 + kTile);
     const int64_t fpos = chans[ci].flac_pos, fend = chans[ci].flac_end;
     const uint32_t b2 = fpos + 2 < nbytes ? payload[fpos + 2] : 0u;
     const uint32_t b3 = fpos + 3 < nbytes ? payload[fpos + 3] : 0u;
-    const int64_t lo = a0 + (int64_t)threadIdx.x * kScanBytes;
-    const int64_t hi = min(a1, lo + kScanBytes);
-    // Offsets base .. base + kScanBytes + 15 in 16-byte chunks (the range
-    // [lo, hi) starts up to 15 bytes past base), each tested with the first
-    // word of the next chunk at hand for headers that cross into it; hits
-    // are kept as a bit mask (bit j: offset base + j).
-    constexpr int kChunks = kScanBytes / 16 + 1;
-    const int64_t base = lo & ~(int64_t)15;
-    auto ld = [&](int64_t c) {
-        if (lo >= hi || c >= nbytes) return uint4{0u, 0u, 0u, 0u};
-        if (c + 16 <= nbytes) return *reinterpret_cast<const uint4 *>(payload + c);
-        uint32_t t[4];
-        for (int j = 0; j < 4; j++) {
-            uint32_t x = 0;
-            for (int i = 3; i >= 0; i--)
-                x = (x << 8) | (c + 4 * j + i < nbytes ? payload[c + 4 * j + i] : 0u);
-            t[j] = x;
+    for (int it = 0; it < kScanIters; it++) {
+        const int64_t lo = a0 + ((int64_t)it * kFlBlock + threadIdx.x) * kScanBytes;
+        const int64_t hi = min(a1, lo + kScanBytes);
+        // Offsets base .. base + kScanBytes + 15 in 16-byte chunks (the range
+        // [lo, hi) starts up to 15 bytes past base), each tested with the first
+        // word of the next chunk at hand for headers that cross into it; hits
+        // are kept as a bit mask (bit j: offset base + j).
+        constexpr int kChunks = kScanBytes / 16 + 1;
+        const int64_t base = lo & ~(int64_t)15;
+        auto ld = [&](int64_t c) {
+            if (lo >= hi || c >= nbytes) return uint4{0u, 0u, 0u, 0u};
+            if (c + 16 <= nbytes) return *reinterpret_cast<const uint4 *>(payload + c);
+            uint32_t t[4];
+            for (int j = 0; j < 4; j++) {
+                uint32_t x = 0;
+                for (int i = 3; i >= 0; i--)
+                    x = (x << 8) | (c + 4 * j + i < nbytes ? payload[c + 4 * j + i] : 0u);
+                t[j] = x;
+            }
+            return uint4{t[0], t[1], t[2], t[3]};
+        };
+        uint64_t mlo = 0;
+        uint32_t mhi = 0;
+        // Bounds relative to base, in 32 bits.
+        const int rl = (int)(lo - base), rh = (int)(hi - base);
+        const int rf = (int)min(fend - 3 - base, (int64_t)(1 << 20));
+        const int rp = (fpos >= base && fpos < base + 16 * kChunks) ? (int)(fpos - base) : -1;
+        uint4 cur = ld(base);
+    #pragma unroll 1
+        for (int q = 0; q < kChunks; q++) {
+            const uint4 nxt = ld(base + 16 * (q + 1));
+            const uint32_t c[5] = {cur.x, cur.y, cur.z, cur.w, nxt.x};
+            uint32_t m = 0;
+    #pragma unroll
+            for (int b = 0; b < 16; b++) {
+                auto B = [&](int k) { return (c[k >> 2] >> (8 * (k & 3))) & 0xFFu; };
+                const int j = 16 * q + b;
+                const bool h = B(b) == 0xFF && (B(b + 1) & 0xFE) == 0xF8 && j >= rl && j < rh &&
+                               j < rf &&
+                               (j == rp || (B(b + 3) == b3 && (B(b + 2) & 0x0F) == (b2 & 0x0F)));
+                m |= (uint32_t)h << b;
+            }
+            if (q < 4) mlo |= (uint64_t)m << (16 * q);
+            else mhi |= m;
+            cur = nxt;
         }
-        return uint4{t[0], t[1], t[2], t[3]};
-    };
-    uint64_t mlo = 0;
-    uint32_t mhi = 0;
-    // Bounds relative to base, in 32 bits.
-    const int rl = (int)(lo - base), rh = (int)(hi - base);
-    const int rf = (int)min(fend - 3 - base, (int64_t)(1 << 20));
-    const int rp = (fpos >= base && fpos < base + 16 * kChunks) ? (int)(fpos - base) : -1;
-    uint4 cur = ld(base);
-#pragma unroll 1
-    for (int q = 0; q < kChunks; q++) {
-        const uint4 nxt = ld(base + 16 * (q + 1));
-        const uint32_t c[5] = {cur.x, cur.y, cur.z, cur.w, nxt.x};
-        uint32_t m = 0;
-#pragma unroll
-        for (int b = 0; b < 16; b++) {
-            auto B = [&](int k) { return (c[k >> 2] >> (8 * (k & 3))) & 0xFFu; };
-            const int j = 16 * q + b;
-            const bool h = B(b) == 0xFF && (B(b + 1) & 0xFE) == 0xF8 && j >= rl && j < rh &&
-                           j < rf &&
-                           (j == rp || (B(b + 3) == b3 && (B(b + 2) & 0x0F) == (b2 & 0x0F)));
-            m |= (uint32_t)h << b;
+        const uint32_t mine = (uint32_t)(__popcll(mlo) + __popc(mhi));
+        // One atomic per wave: lanes take consecutive slots after a wave prefix sum.
+        const int lane = threadIdx.x & 63;
+        uint32_t incl = mine;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(incl, d);
+            if (lane >= d) incl += y;
         }
-        if (q < 4) mlo |= (uint64_t)m << (16 * q);
-        else mhi |= m;
-        cur = nxt;
+        const uint32_t total = __shfl(incl, 63);
+        uint32_t first = 0;
+        if (lane == 63 && total) first = atomicAdd(n_cand + j, total);
+        first = __shfl(first, 63) + incl - mine;
+        for (; mlo; mlo &= mlo - 1, first++)
+            if (first < cap) mine_list[first] = (uint64_t)(base + __ffsll((long long)mlo) - 1);
+        for (; mhi; mhi &= mhi - 1, first++)
+            if (first < cap) mine_list[first] = (uint64_t)(base + 64 + __ffs(mhi) - 1);
     }
-    const uint32_t mine = (uint32_t)(__popcll(mlo) + __popc(mhi));
-    // One atomic per wave: lanes take consecutive slots after a wave prefix sum.
-    const int lane = threadIdx.x & 63;
-    uint32_t incl = mine;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(incl, d);
-        if (lane >= d) incl += y;
-    }
-    const uint32_t total = __shfl(incl, 63);
-    uint32_t first = 0;
-    if (lane == 63 && total) first = atomicAdd(n_cand, total);
-    first = __shfl(first, 63) + incl - mine;
-    for (; mlo; mlo &= mlo - 1, first++)
-        if (first < cap) cand[first] = ((uint64_t)ci << 40) | (uint64_t)(base + __ffsll((long long)mlo) - 1);
-    for (; mhi; mhi &= mhi - 1, first++)
-        if (first < cap) cand[first] = ((uint64_t)ci << 40) | (uint64_t)(base + 64 + __ffs(mhi) - 1);
 }
 
-// Sync scan, second stage: one lane per listed offset parses the header
-// (valid fields, CRC-8); a header numbered k keeps the smallest offset in
-// slot[k].  The first frame decides whether the stream uses variable block
-// sizes; elsewhere a variable-blocksize header can only be a false match.
+// Sync scan, second stage: one lane per candidate slot parses the listed
+// header (valid fields, CRC-8); a header numbered k keeps the smallest
+// offset in slot[k].  The first frame decides whether the stream uses
+// variable block sizes; elsewhere a variable-blocksize header can only be a
+// false match.  A channel whose region overflowed is sent to the walk.
 __global__ __launch_bounds__(kFlBlock) void k_flac_headers(
     const uint8_t *__restrict__ payload, int64_t nbytes, const Chan *chans,
-    const uint64_t *cand, const uint32_t *n_cand, uint32_t cap, uint32_t *slots,
-    int32_t *flags) {
-    const uint32_t n = min(*n_cand, cap);
+    const int32_t *flac_ch, const int64_t *cand0, int32_t n_fch, const uint64_t *cand,
+    const uint32_t *n_cand, uint32_t *slots, int32_t *flags) {
     const int64_t idx = (int64_t)blockIdx.x * kFlBlock + threadIdx.x;
-    if (idx >= n) return;
-    const uint64_t e = cand[idx];
-    const int64_t ci = (int64_t)(e >> 40), at = (int64_t)(e & ((1ull << 40) - 1));
+    if (idx >= cand0[n_fch]) return;
+    const int32_t j = owner(cand0, n_fch, idx);
+    const int32_t ci = flac_ch[j];
+    const uint32_t cap = (uint32_t)(cand0[j + 1] - cand0[j]);
+    const uint32_t local = (uint32_t)(idx - cand0[j]);
+    if (local == 0 && n_cand[j] > cap) atomicOr(flags + ci, kWhyNoSync);
+    if (local >= min(n_cand[j], cap)) return;
+    const int64_t at = (int64_t)cand[idx];
     const Chan c = chans[ci];
     Reader r{payload, nbytes, c.flac_end + kSlack};
     Frame f;
@@ -686,12 +701,12 @@ __global__ __launch_bounds__(kFlBlock) void k_flac_headers(
 template <int TYPE, int MAXO>
 __device__ __forceinline__ void frame_task(int64_t g, const uint8_t *__restrict__ payload,
                                            int64_t nbytes, const Chan *chans,
-                                           const int32_t *flac_ch, int32_t n_fch,
-                                           const uint32_t *slots,
+                                           const int32_t *flac_ch, const int64_t *slot0,
+                                           int32_t n_fch, const uint32_t *slots,
                                            const Out &o, uint32_t *ends, int32_t *flags,
                                            uint32_t *lpc_list, uint32_t *lpc_count,
                                            const uint16_t *crc_tab) {
-    const int32_t ci = owner(chans, flac_ch, n_fch, g, [](const Chan &c) { return c.slot0; });
+    const int32_t ci = flac_ch[owner(slot0, n_fch, g)];
     const Chan c = chans[ci];
     if (flags[ci] & kWhyVariable) return;
     const int64_t k = g - c.slot0;
@@ -725,13 +740,14 @@ __device__ __forceinline__ void frame_task(int64_t g, const uint8_t *__restrict_
 template <int TYPE>
 __global__ __launch_bounds__(kFlBlock) void k_flac_frames(
     const uint8_t *__restrict__ payload, int64_t nbytes, const Chan *chans,
-    const int32_t *flac_ch, int32_t n_fch, int64_t n_frames, const uint32_t *slots, Out o,
-    uint32_t *ends, int32_t *flags, uint32_t *lpc_list, uint32_t *lpc_count) {
+    const int32_t *flac_ch, const int64_t *slot0, int32_t n_fch, int64_t n_frames,
+    const uint32_t *slots, Out o, uint32_t *ends, int32_t *flags, uint32_t *lpc_list,
+    uint32_t *lpc_count) {
     __shared__ uint16_t crc_tab[256];
     load_crc_table(crc_tab);
     const int64_t g = (int64_t)blockIdx.x * kFlBlock + threadIdx.x;
     if (g >= n_frames) return;
-    frame_task<TYPE, 0>(g, payload, nbytes, chans, flac_ch, n_fch, slots, o, ends, flags,
+    frame_task<TYPE, 0>(g, payload, nbytes, chans, flac_ch, slot0, n_fch, slots, o, ends, flags,
                         lpc_list, lpc_count, crc_tab);
 }
 
@@ -739,16 +755,16 @@ __global__ __launch_bounds__(kFlBlock) void k_flac_frames(
 template <int TYPE>
 __global__ __launch_bounds__(kFlBlock) void k_flac_frames_lpc(
     const uint8_t *__restrict__ payload, int64_t nbytes, const Chan *chans,
-    const int32_t *flac_ch, int32_t n_fch, const uint32_t *slots, Out o, uint32_t *ends,
-    int32_t *flags, const uint32_t *lpc_list, const uint32_t *lpc_count) {
+    const int32_t *flac_ch, const int64_t *slot0, int32_t n_fch, const uint32_t *slots, Out o,
+    uint32_t *ends, int32_t *flags, const uint32_t *lpc_list, const uint32_t *lpc_count) {
     __shared__ uint16_t crc_tab[256];
     const uint32_t n = *lpc_count;
     if ((int64_t)blockIdx.x * kFlBlock >= n) return;   // whole block idle
     load_crc_table(crc_tab);
     const int64_t idx = (int64_t)blockIdx.x * kFlBlock + threadIdx.x;
     if (idx >= n) return;
-    frame_task<TYPE, kMaxLpcPar>(lpc_list[idx], payload, nbytes, chans, flac_ch, n_fch, slots,
-                                 o, ends, flags, nullptr, nullptr, crc_tab);
+    frame_task<TYPE, kMaxLpcPar>(lpc_list[idx], payload, nbytes, chans, flac_ch, slot0, n_fch,
+                                 slots, o, ends, flags, nullptr, nullptr, crc_tab);
 }
 
 // One wave per channel: the frames in sequence from the first.  Runs of
@@ -872,6 +888,7 @@ int decode_typed(const uint8_t *payload, int64_t nbytes, const std::vector<Chan>
     g_fl_stats[0] = g_fl_stats[1] = g_fl_stats[2] = 0;
     std::vector<Chan> chans = ch;
     std::vector<int32_t> flac_ch, plain;
+    std::vector<int64_t> firsts;   // slot0 of each FLAC channel, then tile0 of each
     int64_t n_frames = 0, n_tiles = 0;
     for (int64_t i = 0; i < n; i++) {
         Chan &c = chans[i];
@@ -884,24 +901,35 @@ int decode_typed(const uint8_t *payload, int64_t nbytes, const std::vector<Chan>
         flac_ch.push_back((int32_t)i);
     }
     const int32_t n_fch = (int32_t)flac_ch.size();
+    for (int32_t j = 0; j < n_fch; j++) firsts.push_back(chans[flac_ch[j]].slot0);
+    for (int32_t j = 0; j < n_fch; j++) firsts.push_back(chans[flac_ch[j]].tile0);
+    // Candidate regions: a channel's frames twice over plus a margin for
+    // false matches.
+    int64_t n_cand_total = 0;
+    for (int32_t j = 0; j < n_fch; j++) {
+        firsts.push_back(n_cand_total);
+        n_cand_total += 2 * (int64_t)chans[flac_ch[j]].n_frames + 64;
+    }
+    firsts.push_back(n_cand_total);
     // Workspace: chans | flac channels | slots | ends | lpc list | flags |
     // which | err | counters | header candidates
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t b_ch = up(n * sizeof(Chan)), b_fc = up(flac_ch.size() * 4),
+                 b_fi = up(firsts.size() * 8),
                  b_sl = up(n_frames * 4), b_en = b_sl, b_ll = b_sl, b_fl = up(n * 4),
-                 b_wh = up(n * 4), b_er = up(n * 4), b_lc = 256;
-    // Header candidates: frames plus a margin for false matches; a list that
-    // overflows sends every channel through the walk.
-    const uint32_t cap = (uint32_t)std::min<int64_t>(2 * n_frames + 65536, 1ll << 31);
-    const size_t b_ca = up((size_t)cap * 8);
+                 b_wh = up(n * 4), b_er = up(n * 4), b_lc = up(4 + 4 * (size_t)n_fch);
+    const size_t b_ca = up((size_t)n_cand_total * 8);
     void *ws = nullptr;
-    int rc = workspace(b_ch + b_fc + b_sl + b_en + b_ll + b_fl + b_wh + b_er + b_lc + b_ca, &ws);
+    int rc = workspace(b_ch + b_fc + b_fi + b_sl + b_en + b_ll + b_fl + b_wh + b_er + b_lc + b_ca,
+                       &ws);
     if (rc != SO3G_OK) return rc;
     char *w = static_cast<char *>(ws);
     Chan *d_ch = reinterpret_cast<Chan *>(w);
     w += b_ch;
     int32_t *d_fc = reinterpret_cast<int32_t *>(w);
     w += b_fc;
+    int64_t *d_s0 = reinterpret_cast<int64_t *>(w), *d_t0 = d_s0 + n_fch, *d_c0 = d_t0 + n_fch;
+    w += b_fi;
     uint32_t *d_sl = reinterpret_cast<uint32_t *>(w);
     w += b_sl;
     uint32_t *d_en = reinterpret_cast<uint32_t *>(w);
@@ -914,18 +942,20 @@ int decode_typed(const uint8_t *payload, int64_t nbytes, const std::vector<Chan>
     w += b_wh;
     int32_t *d_er = reinterpret_cast<int32_t *>(w);
     w += b_er;
-    uint32_t *d_lc = reinterpret_cast<uint32_t *>(w);   // [0] lpc count, [1] candidates
-    uint32_t *d_nc = d_lc + 1;
+    uint32_t *d_lc = reinterpret_cast<uint32_t *>(w);   // [0] lpc count, [1..] candidates
+    uint32_t *d_nc = d_lc + 1;                          // per FLAC channel
     w += b_lc;
     uint64_t *d_ca = reinterpret_cast<uint64_t *>(w);
     FL_TRY(hipMemcpyAsync(d_ch, chans.data(), n * sizeof(Chan), hipMemcpyHostToDevice, s));
-    if (n_fch)
+    if (n_fch) {
         FL_TRY(hipMemcpyAsync(d_fc, flac_ch.data(), flac_ch.size() * 4, hipMemcpyHostToDevice, s));
+        FL_TRY(hipMemcpyAsync(d_s0, firsts.data(), firsts.size() * 8, hipMemcpyHostToDevice, s));
+    }
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, d_sl, n_frames, kNone);
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, d_en, n_frames, kNone);
     hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, s, reinterpret_cast<uint32_t *>(d_fl),
                        n, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, d_lc, (int64_t)2, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, s, d_lc, (int64_t)1 + n_fch, 0u);
     if (!plain.empty()) {
         FL_TRY(hipMemcpyAsync(d_wh, plain.data(), plain.size() * 4, hipMemcpyHostToDevice, s));
         const int64_t work = (int64_t)plain.size() * (o.stop - o.start);
@@ -937,23 +967,20 @@ int decode_typed(const uint8_t *payload, int64_t nbytes, const std::vector<Chan>
     }
     if (n_frames > 0) {
         hipLaunchKernelGGL(k_flac_scan, dim3((unsigned)n_tiles), dim3(kFlBlock), 0, s, payload,
-                           nbytes, d_ch, d_fc, n_fch, n_tiles, d_ca, d_nc, cap);
-        hipLaunchKernelGGL(k_flac_headers, dim3(blocks_for(cap, kFlBlock)), dim3(kFlBlock), 0, s,
-                           payload, nbytes, d_ch, d_ca, d_nc, cap, d_sl, d_fl);
+                           nbytes, d_ch, d_fc, d_t0, d_c0, n_fch, n_tiles, d_ca, d_nc);
+        hipLaunchKernelGGL(k_flac_headers, dim3(blocks_for(n_cand_total, kFlBlock)),
+                           dim3(kFlBlock), 0, s, payload, nbytes, d_ch, d_fc, d_c0, n_fch, d_ca,
+                           d_nc, d_sl, d_fl);
         hipLaunchKernelGGL(k_flac_frames<TYPE>, dim3(blocks_for(n_frames, kFlBlock)),
-                           dim3(kFlBlock), 0, s, payload, nbytes, d_ch, d_fc, n_fch, n_frames,
-                           d_sl, o, d_en, d_fl, d_ll, d_lc);
+                           dim3(kFlBlock), 0, s, payload, nbytes, d_ch, d_fc, d_s0, n_fch,
+                           n_frames, d_sl, o, d_en, d_fl, d_ll, d_lc);
         hipLaunchKernelGGL(k_flac_frames_lpc<TYPE>, dim3(blocks_for(n_frames, kFlBlock)),
-                           dim3(kFlBlock), 0, s, payload, nbytes, d_ch, d_fc, n_fch, d_sl, o, d_en,
-                           d_fl, d_ll, d_lc);
+                           dim3(kFlBlock), 0, s, payload, nbytes, d_ch, d_fc, d_s0, n_fch, d_sl, o,
+                           d_en, d_fl, d_ll, d_lc);
         FL_TRY(hipGetLastError());
         std::vector<int32_t> flags(n);
-        uint32_t n_cand = 0;
         FL_TRY(hipMemcpyAsync(flags.data(), d_fl, n * 4, hipMemcpyDeviceToHost, s));
-        FL_TRY(hipMemcpyAsync(&n_cand, d_nc, 4, hipMemcpyDeviceToHost, s));
         FL_TRY(hipStreamSynchronize(s));
-        if (n_cand > cap)   // candidates dropped: no frame can be trusted
-            for (int64_t i = 0; i < n; i++) flags[i] |= kWhyNoSync;
         std::vector<int32_t> redo;
         for (int64_t i = 0; i < n; i++)
             if (flags[i] && chans[i].flac_pos >= 0) {
