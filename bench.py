@@ -223,21 +223,22 @@ def main():
 
     # Algorithmic bytes per launch of the planned kernels: what each must
     # move at least from its own storage (see the plan's layout):
-    #   from_map: per sample 4 B flat pixel + 8 B (Q, U) factors + 4 B signal
-    #             written (the T factor is one float per detector), plus the
-    #             float64 TQU map read once;
+    #   from_map: per sample 4 B compact pixel + 8 B (Q, U) factors + 4 B
+    #             signal written (the T factor is one float per detector), plus
+    #             the float64 TQU values of the visited pixels read once;
     #   to_map:   per on-map sample 4 B sample index + 8 B (Q, U) factors +
     #             4 B signal gathered + 1 bit run start, plus 4 B per pixel run
-    #             and per 512-sample group, plus the map read and written once.
+    #             and per 512-sample group, plus the visited pixels' float64
+    #             TQU values read and written once.
     # The ProjEng_Precomp interface itself streams 24 B per sample (int32
     # pixel pair, 3 float32 factors, float32 signal): `precomp_equivalent`
     # reports the rates in those bytes.
     npix = ny * nx
     pinfo = plan.info()
-    n_on, n_runs, ns = pinfo['n_on'], pinfo['runs'], pinfo['stored_spin']
-    b_from = N_s * (4 + 4 * ns + 4) + 8 * 3 * npix
+    n_on, n_runs, ns, n_vis = pinfo['n_on'], pinfo['runs'], pinfo['stored_spin'], pinfo['n_vis']
+    b_from = N_s * (4 + 4 * ns + 4) + 8 * 3 * n_vis
     b_to = n_on * (4 + 4 * ns + 4) + n_on // 8 + 4 * n_runs + 4 * (n_on // 512) + \
-        2 * 8 * 3 * npix
+        2 * 8 * 3 * n_vis
     b_pc = N_s * (4 * 2 + 4 * 3 + 4)
     gbs_from = b_from / (ms_from * 1e-3)
     gbs_to = b_to / (ms_to * 1e-3)
@@ -274,6 +275,8 @@ def main():
     small = None if args.no_small else small_config(torch, so3g_amd, N, workloads, dev, stream)
     nmat = None if args.no_small else nmat_timing(torch, so3g_amd, dev, n_det, n_t)
     g3 = None if args.no_g3 else g3super_timing(torch, n_det, n_t, not args.no_cpu)
+    variants = None if args.no_small else c2_variants(torch, so3g_amd, N, lib, obs, pixargs, bore,
+                                                      dets, resp, sig, dev, stream, run)
 
     out = {
         'metric': 'det*samples/s for to_map+from_map (% HBM roofline)',
@@ -316,6 +319,7 @@ def main():
         'small_config': small,
         'nmat_detvecs_apply': nmat,
         'g3super_decode': g3,
+        'c2_variants': variants,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)
@@ -445,6 +449,137 @@ def nmat_timing(torch, so3g_amd, dev, n_det, n_t, nvec=10, nbin=20, reps=5):
     log(f'nmat_detvecs_apply: {ms:.3f} ms')
     return {'shape': [n_det, nmode], 'nvec': nvec, 'nbin': len(bins), 'ms': ms,
             'hbm_frac': b / (ms * 1e-3) / HBM_PEAK}
+
+
+def c2_variants(torch, so3g_amd, N, lib, obs, pixargs, bore, dets, resp, sig, dev, stream, run,
+                steps=3):
+    """The benchmark observation projected into the other maps of the C2
+    case: a tiled CAR 0.5 arcmin TQU map (100 x 100 tiles, the tiles the
+    observation hits) and a HEALPix NEST nside 4096 TQU map (the boresight
+    taken as celestial).  Each is timed as the on-the-fly pair (pointing
+    recomputed in both kernels) and the cached-pointing (ProjEng_Precomp)
+    pair, and for HEALPix also through a PrecompPlan (whose pixel tables
+    cover the visited pixels only); ms per op and det*samples/s per pair."""
+    n_det, n_t = obs.n_det, obs.n_samp
+    N_s = n_det * n_t
+    out = {}
+    gen = torch.Generator(device=dev).manual_seed(5)
+    sig_out = torch.empty_like(sig)
+
+    def chk(rc):
+        N.check(rc)
+
+    # -- tiled CAR -------------------------------------------------------
+    probe = so3g_amd.ProjEng_CAR_TQU_Tiled(tuple(pixargs) + (100, 100))
+    hits = probe.tile_hits(bore, dets)
+    active = [i for i, h in enumerate(hits) if h > 0]
+    eng = so3g_amd.ProjEng_CAR_TQU_Tiled(tuple(pixargs) + (100, 100, active))
+    g = eng._g
+    dims = eng._tile_dims()
+    sizes = [3 * int(dims[t, 0]) * int(dims[t, 1]) for t in active]
+    buf_in = torch.randn(sum(sizes), dtype=torch.float64, device=dev, generator=gen)
+    buf_out = torch.zeros(sum(sizes), dtype=torch.float64, device=dev)
+    ptr_in = np.zeros(g.n_tile, np.int64)
+    ptr_out = np.zeros(g.n_tile, np.int64)
+    at = 0
+    for t, sz in zip(active, sizes):
+        ptr_in[t] = buf_in.data_ptr() + 8 * at
+        ptr_out[t] = buf_out.data_ptr() + 8 * at
+        at += sz
+    d_in = torch.from_numpy(ptr_in).to(dev)
+    d_out = torch.from_numpy(ptr_out).to(dev)
+    d_dims = torch.from_numpy(np.ascontiguousarray(dims)).to(dev)
+    pix = torch.empty((n_det, n_t, 3), dtype=torch.int32, device=dev)
+    spin = torch.empty((n_det, n_t, 3), dtype=torch.float32, device=dev)
+    chk(lib.so3g_pointing_matrix(g, bore.data_ptr(), n_t, dets.data_ptr(), n_det,
+                                 resp.data_ptr(), pix.data_ptr(), spin.data_ptr(), stream))
+
+    def otf_from():
+        chk(lib.so3g_from_map(g, None, d_in.data_ptr(), d_dims.data_ptr(), bore.data_ptr(), n_t,
+                              dets.data_ptr(), n_det, resp.data_ptr(), sig_out.data_ptr(), 0,
+                              stream))
+
+    def otf_to():
+        buf_out.zero_()
+        chk(lib.so3g_to_map(g, None, d_out.data_ptr(), d_dims.data_ptr(), bore.data_ptr(), n_t,
+                            dets.data_ptr(), n_det, resp.data_ptr(), sig.data_ptr(), None, None,
+                            0, stream))
+
+    def pc_from():
+        chk(lib.so3g_precomp_from_map(None, None, d_in.data_ptr(), d_dims.data_ptr(),
+                                      pix.data_ptr(), 3, spin.data_ptr(), 3, n_det, n_t,
+                                      sig_out.data_ptr(), 0, stream))
+
+    def pc_to():
+        buf_out.zero_()
+        chk(lib.so3g_precomp_to_map(None, None, d_out.data_ptr(), d_dims.data_ptr(),
+                                    pix.data_ptr(), 3, spin.data_ptr(), 3, n_det, n_t,
+                                    sig.data_ptr(), None, None, 0, stream))
+    res = {'map': f'CAR 0.5 arcmin TQU {g.naxis[0]}x{g.naxis[1]}, 100x100 tiles, '
+                  f'{len(active)} of {g.n_tile} active'}
+    for name, ops in (('on_the_fly', [otf_from, otf_to]), ('precomp', [pc_from, pc_to])):
+        _, (mf, mt) = run(ops, steps, 1)
+        res[name] = {'ms_from_map': mf, 'ms_to_map': mt, 'det_samples_per_s': N_s / ((mf + mt) * 1e-3)}
+    out['tiled_car'] = res
+    del pix, spin, buf_in, buf_out
+    log(f'tiled CAR: {res}')
+
+    # -- HEALPix NEST nside 4096 ------------------------------------------
+    nside = 4096
+    eng = so3g_amd.ProjEng_HP_TQU_NonTiled((nside, True))
+    g = eng._g
+    npix = 12 * nside * nside
+    hmap = torch.randn((3, npix), dtype=torch.float64, device=dev, generator=gen)
+    hout = torch.zeros((3, npix), dtype=torch.float64, device=dev)
+    shape = np.array([npix, 1], np.int64)
+    pix = torch.empty((n_det, n_t, 1), dtype=torch.int32, device=dev)
+    spin = torch.empty((n_det, n_t, 3), dtype=torch.float32, device=dev)
+    chk(lib.so3g_pointing_matrix(g, bore.data_ptr(), n_t, dets.data_ptr(), n_det,
+                                 resp.data_ptr(), pix.data_ptr(), spin.data_ptr(), stream))
+
+    def hotf_from():
+        chk(lib.so3g_from_map(g, hmap.data_ptr(), None, None, bore.data_ptr(), n_t,
+                              dets.data_ptr(), n_det, resp.data_ptr(), sig_out.data_ptr(), 0,
+                              stream))
+
+    def hotf_to():
+        hout.zero_()
+        chk(lib.so3g_to_map(g, hout.data_ptr(), None, None, bore.data_ptr(), n_t,
+                            dets.data_ptr(), n_det, resp.data_ptr(), sig.data_ptr(), None, None,
+                            0, stream))
+
+    def hpc_from():
+        chk(lib.so3g_precomp_from_map(hmap.data_ptr(), shape.ctypes.data, None, None,
+                                      pix.data_ptr(), 1, spin.data_ptr(), 3, n_det, n_t,
+                                      sig_out.data_ptr(), 0, stream))
+
+    def hpc_to():
+        hout.zero_()
+        chk(lib.so3g_precomp_to_map(hout.data_ptr(), shape.ctypes.data, None, None,
+                                    pix.data_ptr(), 1, spin.data_ptr(), 3, n_det, n_t,
+                                    sig.data_ptr(), None, None, 0, stream))
+    torch.cuda.synchronize()
+    t0 = time.time()
+    hplan = so3g_amd.PrecompPlan(pix, spin, (npix,))
+    torch.cuda.synchronize()
+    t_hplan = time.time() - t0
+
+    def hplan_from():
+        chk(lib.so3g_pc_plan_from_map(hplan._h, hmap.data_ptr(), sig_out.data_ptr(), 0, stream))
+
+    def hplan_to():
+        hout.zero_()
+        chk(lib.so3g_pc_plan_to_map(hplan._h, hout.data_ptr(), sig.data_ptr(), None, stream))
+    res = {'map': f'HEALPix NEST nside {nside} TQU ({npix} pixels, 4.8 GB per map)',
+           'plan_visited_pixels': hplan.info()['n_vis'], 'plan_build_s': t_hplan}
+    for name, ops in (('on_the_fly', [hotf_from, hotf_to]), ('precomp', [hpc_from, hpc_to]),
+                      ('plan', [hplan_from, hplan_to])):
+        _, (mf, mt) = run(ops, steps, 1)
+        res[name] = {'ms_from_map': mf, 'ms_to_map': mt, 'det_samples_per_s': N_s / ((mf + mt) * 1e-3)}
+    out['healpix_nest_4096'] = res
+    del pix, spin, hmap, hout, hplan
+    log(f'HEALPix: {res}')
+    return out
 
 
 def g3super_timing(torch, n_det, n_t, with_cpu, reps=3, quantum=1e-3):

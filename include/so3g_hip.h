@@ -207,14 +207,17 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape,
 
 /* Projection plan for repeated cached-pointing projections with one
  * pointing matrix (no counterpart in the reference; for CG-style loops).
- * Built once from a Precomp pixel array (k = 2, non-tiled flat map of
- * map_shape = (ny, nx), host array) and spin array; the arrays are copied,
- * so they may be freed afterwards.  Samples are grouped in time blocks whose
+ * Built once from a Precomp pixel array and spin array: k = 2 with a
+ * non-tiled flat map of map_shape = (ny, nx), or k = 1 (HEALPix pixel
+ * numbers) with map_shape = (npix, 1); host array.  The arrays are copied,
+ * so they may be freed afterwards.  The pixels the samples visit are numbered
+ * in ascending order (compact pixels), so the plan's pixel tables scale with
+ * the visited pixels, not the map.  Samples are grouped in time blocks whose
  * signal slice is about t_block_mb MiB (sized to stay in the Infinity Cache;
  * <= 0: one block), then sorted by pixel within each block.  Stored: per
  * on-map sample the sample index and the spin factors (the T factor once per
  * detector when it is constant over the detector, as in so3g's pointing
- * matrices), run-start bits and run pixels; per sample a time-ordered flat
+ * matrices), run-start bits and run pixels; per sample a time-ordered compact
  * pixel and the same spin factors.  so3g_pc_plan_to_map adds the same
  * contributions as so3g_precomp_to_map over every sample (float64 sums in
  * another order); so3g_pc_plan_from_map writes what so3g_precomp_from_map
@@ -239,8 +242,9 @@ int so3g_precomp_from_map_f64(const double *map, const int64_t *map_shape,
                               int64_t n_det, int64_t n_t, double *signal, int accumulate,
                               void *stream);
 int so3g_pc_plan_info(const so3g_pc_plan *plan, int64_t *n_on, int64_t *bytes);
-/* stats: n_on, pixel runs, T factor constant per detector (0/1), stored spin
- * factors per sample, samples per time block, device bytes */
+/* stats[7]: n_on, pixel runs, T factor constant per detector (0/1), stored
+ * spin factors per sample, samples per time block, device bytes, visited
+ * pixels */
 int so3g_pc_plan_stats(const so3g_pc_plan *plan, int64_t *stats);
 int so3g_pc_plan_destroy(so3g_pc_plan *plan);
 
@@ -435,10 +439,7 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *   "plan_from_layout"  1 (default): a lane's gather samples are 64 apart, so
  *                   one gather instruction covers 64 consecutive samples;
  *                   0: consecutive samples per lane (measured slower, 3.51 vs
- *                   3.05 ms for pack + gather)
- *   "plan_packed"   planned scatter into a pixel-major scratch map, added to
- *                   the map afterwards (default 0: measured slower, 3.79 vs
- *                   3.51 ms) */
+ *                   3.05 ms for pack + gather) */
 int so3g_set_option(const char *name, int64_t value);
 int so3g_get_option(const char *name, int64_t *value);
 

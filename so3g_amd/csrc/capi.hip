@@ -18,7 +18,6 @@
 namespace so3g {
 extern int g_plan_div;
 extern int g_plan_from_v;
-extern int g_plan_packed;
 extern int g_plan_from_layout;
 extern int g_plan_to_lds;
 extern int g_plan_to_cpb;
@@ -740,7 +739,6 @@ int so3g_set_option(const char *name, int64_t value) {
     else if (n == "pc_from_dpb") g_opt.pc_from_dpb = (int)std::max<int64_t>(1, value);
     else if (n == "xcd_budget_mb") g_opt.xcd_budget_mb = value;
     else if (n == "plan_div") so3g::g_plan_div = (int)(value != 0);
-    else if (n == "plan_packed") so3g::g_plan_packed = (int)(value != 0);
     else if (n == "plan_from_layout") so3g::g_plan_from_layout = (int)(value != 0);
     else if (n == "plan_to_lds") so3g::g_plan_to_lds = (int)(value != 0);
     else if (n == "plan_to_cpb") {
@@ -763,7 +761,6 @@ int so3g_get_option(const char *name, int64_t *value) {
     else if (n == "pc_from_dpb") *value = g_opt.pc_from_dpb;
     else if (n == "xcd_budget_mb") *value = g_opt.xcd_budget_mb;
     else if (n == "plan_div") *value = so3g::g_plan_div;
-    else if (n == "plan_packed") *value = so3g::g_plan_packed;
     else if (n == "plan_from_layout") *value = so3g::g_plan_from_layout;
     else if (n == "plan_to_lds") *value = so3g::g_plan_to_lds;
     else if (n == "plan_to_cpb") *value = so3g::g_plan_to_cpb;

@@ -22,8 +22,15 @@
 //   runs         pixel of each run (one per distinct (time block, pixel))
 //                and the run count before each 512-sample group
 // so that a sample's pixel is run_pix[run index], the run index being a
-// popcount over the head bits.  from_map reads a time-ordered copy: flat
-// pixel (4 B, -1 off the map) and the ns stored spin factors.
+// popcount over the head bits.  from_map reads a time-ordered copy: compact
+// pixel index (4 B, -1 off the map) and the ns stored spin factors.
+//
+// Pixels are numbered compactly over the pixels the observation visits (a
+// bitmap of the map and a prefix count of its words): sort keys, the
+// time-ordered copy and from_map's packed map then scale with the visited
+// pixels, not the map -- a 1.5 degree scan visits 0.3 M of a HEALPix nside
+// 4096 map's 201 M pixels.  Maps are (ny, nx) with (iy, ix) pixel pairs
+// (k = 2) or (npix, 1) with pixel numbers (k = 1).
 //
 // Arithmetic per sample is the Precomp one (Projection.cxx:2100-2121 for
 // to_map, (det_wt * sig) * pf in float32 summed in float64; :2060-2085 for
@@ -60,7 +67,9 @@ struct so3g_pc_plan {
     uint32_t *run_pix;     // [n_runs] flat pixel iy * nx + ix of each run
     int32_t *tpix;         // [n] time-ordered flat pixel, -1 off the map
     float *tspin;          // [n * ns] time-ordered stored spin factors
-    double *packed;        // [ny * nx * 4] pixel-major map copy for from_map
+    double *packed;        // [n_vis * 4] pixel-major copy of the visited pixels
+    int32_t *comp2pix;     // [n_vis] flat pixel of each compact index (ascending)
+    int64_t n_vis;         // visited pixels
     int64_t bytes;
 };
 
@@ -83,7 +92,6 @@ constexpr int kPlanBlock = 256;
 // or consecutive.
 int g_plan_div = 1;
 int g_plan_from_v = 4;
-int g_plan_packed = 0;   // "plan_packed": scatter into a pixel-major scratch map
 int g_plan_from_layout = 1;
 // Planned to_map: per-detector tables staged in LDS ("plan_to_lds") and
 // 2048-sample chunks per block ("plan_to_cpb").
@@ -98,17 +106,74 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 constexpr int kPlanR = 8;                 // sorted samples per lane
 constexpr int kGroup = 64 * kPlanR;       // samples per wave = per head group
 
-// Sort keys: (time block, row-major pixel) as tb * npix + pixel, or the
+// Flat pixel of sample i, -1 off the map: (iy, ix) pairs on an (ny, nx)
+// map (k = 2) or pixel numbers on an (ny = npix, 1) map (k = 1).
+__device__ __forceinline__ int64_t flat_pixel(const int32_t *pix, int k, int64_t i, int64_t ny,
+                                              int64_t nx) {
+    if (k == 1) {
+        const int32_t p = pix[i];
+        return (p >= 0 && p < ny) ? (int64_t)p : -1;
+    }
+    const int2 p = *reinterpret_cast<const int2 *>(pix + 2 * i);
+    return (p.x >= 0 && p.x < ny && p.y >= 0 && p.y < nx) ? (int64_t)p.x * nx + p.y : -1;
+}
+
+// Compact index of a visited pixel: visited pixels before it.
+__device__ __forceinline__ uint32_t compact_of(int64_t p, const uint32_t *bitmap,
+                                               const uint32_t *wscan) {
+    const uint32_t w = bitmap[p >> 5];
+    return wscan[p >> 5] + __popc(w & ((1u << (p & 31)) - 1u));
+}
+
+// Visited-pixel bitmap.
+__global__ __launch_bounds__(kPlanBlock) void k_plan_mark(const int32_t *pix, int k, int64_t n,
+                                                          int64_t ny, int64_t nx,
+                                                          uint32_t *bitmap) {
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kPlanBlock) {
+        const int64_t p = flat_pixel(pix, k, i, ny, nx);
+        if (p >= 0 && !(bitmap[p >> 5] & (1u << (p & 31))))
+            atomicOr(bitmap + (p >> 5), 1u << (p & 31));
+    }
+}
+
+__global__ __launch_bounds__(kPlanBlock) void k_plan_wcount(const uint32_t *bitmap,
+                                                            int64_t n_words, uint32_t *cnt) {
+    for (int64_t w = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; w < n_words;
+         w += (int64_t)gridDim.x * kPlanBlock)
+        cnt[w] = __popc(bitmap[w]);
+}
+
+// comp2pix: the visited pixels in ascending order.
+__global__ __launch_bounds__(kPlanBlock) void k_plan_compact(const uint32_t *bitmap,
+                                                             const uint32_t *wscan,
+                                                             int64_t n_words, int32_t *comp2pix) {
+    for (int64_t w = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; w < n_words;
+         w += (int64_t)gridDim.x * kPlanBlock) {
+        uint32_t bits = bitmap[w];
+        uint32_t at = wscan[w];
+        while (bits) {
+            const int b = __ffs(bits) - 1;
+            comp2pix[at++] = (int32_t)(w * 32 + b);
+            bits &= bits - 1;
+        }
+    }
+}
+
+// Sort keys: (time block, compact pixel) as tb * n_vis + compact, or the
 // largest key for an off-map sample (sorted last).
-__global__ __launch_bounds__(kPlanBlock) void k_plan_keys(const int32_t *pix, int64_t n,
-                                                          int64_t nx, int32_t npix, int64_t n_t,
-                                                          int64_t t_block, int32_t off_key,
-                                                          int32_t *keys, uint32_t *vals) {
+__global__ __launch_bounds__(kPlanBlock) void k_plan_keys(const int32_t *pix, int k, int64_t n,
+                                                          int64_t ny, int64_t nx, int64_t n_t,
+                                                          int64_t t_block, int32_t n_vis,
+                                                          int32_t off_key,
+                                                          const uint32_t *bitmap,
+                                                          const uint32_t *wscan, int32_t *keys,
+                                                          uint32_t *vals) {
     const int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
     if (i >= n) return;
-    const int2 p = *reinterpret_cast<const int2 *>(pix + 2 * i);
+    const int64_t p = flat_pixel(pix, k, i, ny, nx);
     const int64_t tb = (i % n_t) / t_block;
-    keys[i] = p.x >= 0 ? (int32_t)(tb * npix + p.x * nx + p.y) : off_key;
+    keys[i] = p >= 0 ? (int32_t)(tb * n_vis + compact_of(p, bitmap, wscan)) : off_key;
     vals[i] = (uint32_t)i;
 }
 
@@ -154,17 +219,18 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_sspin(const float *spin, in
     }
 }
 
-// Time-ordered copy: flat pixel (or -1) and the stored spin factors.
-__global__ __launch_bounds__(kPlanBlock) void k_plan_tcopy(const int32_t *pix,
+// Time-ordered copy: compact pixel (or -1) and the stored spin factors.
+__global__ __launch_bounds__(kPlanBlock) void k_plan_tcopy(const int32_t *pix, int k,
                                                            const float *spin, int nc, int c0,
                                                            int64_t n, int64_t ny, int64_t nx,
-                                                           int32_t *tpix, float *tspin) {
+                                                           const uint32_t *bitmap,
+                                                           const uint32_t *wscan, int32_t *tpix,
+                                                           float *tspin) {
     const int ns = nc - c0;
     for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kPlanBlock) {
-        const int2 p = *reinterpret_cast<const int2 *>(pix + 2 * i);
-        const bool on = p.x >= 0 && p.x < ny && p.y >= 0 && p.y < nx;
-        tpix[i] = on ? (int32_t)(p.x * nx + p.y) : -1;
+        const int64_t p = flat_pixel(pix, k, i, ny, nx);
+        tpix[i] = p >= 0 ? (int32_t)compact_of(p, bitmap, wscan) : -1;
         for (int c = 0; c < ns; c++) tspin[i * ns + c] = spin[i * nc + c0 + c];
     }
 }
@@ -231,7 +297,8 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
 
 // run_pix[run] = pixel of the run's first sample.
 __global__ __launch_bounds__(kPlanBlock) void k_plan_runs(const int32_t *key, int64_t n_bytes,
-                                                          int32_t npix, const uint8_t *heads,
+                                                          int32_t n_vis, const int32_t *comp2pix,
+                                                          const uint8_t *heads,
                                                           const uint32_t *wbase,
                                                           uint32_t *run_pix) {
     const int64_t j = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
@@ -242,7 +309,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_runs(const int32_t *key, in
     uint32_t k = 0;
     for (int r = 0; r < 8; r++) {
         if (bits & (1u << r)) {
-            run_pix[base + k] = (uint32_t)(key[j * 8 + r] % npix);
+            run_pix[base + k] = (uint32_t)comp2pix[key[j * 8 + r] % n_vis];
             k++;
         }
     }
@@ -255,7 +322,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_runs(const int32_t *key, in
 // continue from earlier lanes and goes through the wave-level run
 // aggregation; runs that start inside the lane and end inside it, and its
 // last run when it holds more than one, are added to the map directly.
-template <int NC, bool PKD, typename XF>
+template <int NC, typename XF>
 __device__ __forceinline__ void plan_lane_scatter(uint32_t hb, uint32_t lane_base, int nr,
                                                   const uint32_t *__restrict__ run_pix,
                                                   uint32_t plane, double *map, XF &&xf) {
@@ -281,7 +348,7 @@ __device__ __forceinline__ void plan_lane_scatter(uint32_t hb, uint32_t lane_bas
                 } else {
 #pragma unroll
                     for (int c = 0; c < NC; c++)
-                        unsafeAtomicAdd(map + (PKD ? 4 * (int64_t)cpix + c : (int64_t)c * plane + cpix), cv[c]);
+                        unsafeAtomicAdd(map + (int64_t)c * plane + cpix, cv[c]);
                 }
             }
             ck = rid;
@@ -301,13 +368,13 @@ __device__ __forceinline__ void plan_lane_scatter(uint32_t hb, uint32_t lane_bas
     } else if (ck >= 0) {
 #pragma unroll
         for (int c = 0; c < NC; c++)
-            unsafeAtomicAdd(map + (PKD ? 4 * (int64_t)cpix + c : (int64_t)c * plane + cpix), cv[c]);
+            unsafeAtomicAdd(map + (int64_t)c * plane + cpix, cv[c]);
         hpix = hk >= 0 ? run_pix[hk] : 0u;
     }
     if (wave_runs<NC>(hk, hv) && hk >= 0) {
 #pragma unroll
         for (int c = 0; c < NC; c++)
-            unsafeAtomicAdd(map + (PKD ? 4 * (int64_t)hpix + c : (int64_t)c * plane + hpix), hv[c]);
+            unsafeAtomicAdd(map + (int64_t)c * plane + hpix, hv[c]);
     }
 }
 
@@ -316,7 +383,7 @@ __device__ __forceinline__ void plan_lane_scatter(uint32_t hb, uint32_t lane_bas
 // blocks stay on adjacent pixels and signal rows); with LDT the per-detector
 // tables (T factor, weights) are staged in LDS once per block so their
 // lookups, which differ lane to lane, stay off the vector memory path.
-template <int NC, bool TC, typename ST, int DIV, bool PKD, bool LDT>
+template <int NC, bool TC, typename ST, int DIV, bool LDT>
 __global__ __launch_bounds__(kPlanBlock, 6) void k_plan_to_map_c(
     const uint32_t *__restrict__ idx, const float *__restrict__ sspin,
     const float *__restrict__ tval, const uint8_t *__restrict__ heads,
@@ -367,7 +434,7 @@ __global__ __launch_bounds__(kPlanBlock, 6) void k_plan_to_map_c(
         const uint32_t hb = nr > 0 ? heads[base / 8] : 0u;
         const uint32_t wb = nr > 0 ? wbase[base / kGroup] : 0u;
         const uint32_t lane_base = wb + wave_excl_scan(__popc(hb));
-        plan_lane_scatter<NC, PKD>(hb, lane_base, nr, run_pix, plane, map, [&](int r, double *x) {
+        plan_lane_scatter<NC>(hb, lane_base, nr, run_pix, plane, map, [&](int r, double *x) {
             const uint32_t si = ii[r];
             uint32_t det = 0;
             if (TC || det_weights != nullptr) det = fast_div<DIV>(si, n_t);
@@ -403,35 +470,20 @@ __global__ __launch_bounds__(kPlanBlock, 6) void k_plan_to_map_c(
     }
 }
 
-// Pixel-major scatter target: zeroed before, added into the (NC, plane) map
-// after (one pass over the map each).
-__global__ __launch_bounds__(kPlanBlock) void k_plan_zero(double *p, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kPlanBlock)
-        p[i] = 0.;
-}
-
-template <int NC>
-__global__ __launch_bounds__(kPlanBlock) void k_plan_unpack_add(const double *packed,
-                                                                int64_t plane, double *map) {
-    for (int64_t p = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; p < plane;
-         p += (int64_t)gridDim.x * kPlanBlock) {
-#pragma unroll
-        for (int c = 0; c < NC; c++) map[c * plane + p] += packed[4 * p + c];
-    }
-}
-
-// Pixel-major copy of an (NC, plane) map, 4 doubles per pixel.
+// Pixel-major copy of the visited pixels of an (NC, plane) map, 4 doubles
+// per pixel, in compact order (comp2pix[i] = flat pixel of compact index i).
 template <int NC>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_pack(const double *map, int64_t plane,
+                                                          const int32_t *comp2pix, int64_t n_vis,
                                                           double *packed) {
-    for (int64_t p = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; p < plane;
-         p += (int64_t)gridDim.x * kPlanBlock) {
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n_vis;
+         i += (int64_t)gridDim.x * kPlanBlock) {
+        const int64_t p = comp2pix[i];
         double v[4] = {0., 0., 0., 0.};
 #pragma unroll
         for (int c = 0; c < NC; c++) v[c] = map[c * plane + p];
-        reinterpret_cast<double2 *>(packed + 4 * p)[0] = double2{v[0], v[1]};
-        reinterpret_cast<double2 *>(packed + 4 * p)[1] = double2{v[2], v[3]};
+        reinterpret_cast<double2 *>(packed + 4 * i)[0] = double2{v[0], v[1]};
+        reinterpret_cast<double2 *>(packed + 4 * i)[1] = double2{v[2], v[3]};
     }
 }
 
@@ -604,6 +656,7 @@ void free_plan_arrays(so3g_pc_plan *p) {
     (void)hipFree(p->idx); (void)hipFree(p->sspin); (void)hipFree(p->tval);
     (void)hipFree(p->heads); (void)hipFree(p->wbase); (void)hipFree(p->run_pix);
     (void)hipFree(p->tpix); (void)hipFree(p->tspin); (void)hipFree(p->packed);
+    (void)hipFree(p->comp2pix);
 }
 
 template <typename ST>
@@ -614,12 +667,8 @@ int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
     hipStream_t s = (hipStream_t)stream;
     const uint32_t plane = (uint32_t)(p->ny * p->nx);
     const FastDiv nt{(uint32_t)p->n_t, 1.f / (float)p->n_t};
-    const bool pkd = g_plan_packed != 0;
-    double *target = pkd ? p->packed : map;
-    if (pkd) hipLaunchKernelGGL(k_plan_zero, dim3(grid_of(4 * (int64_t)plane)), dim3(kPlanBlock),
-                                0, s, p->packed, 4 * (int64_t)plane);
     const int tabs = (p->tconst ? 1 : 0) + (det_weights != nullptr ? 1 : 0);
-    const bool ldt = g_plan_to_lds && !pkd && tabs > 0 &&
+    const bool ldt = g_plan_to_lds && tabs > 0 &&
                      p->n_det * tabs * (int64_t)sizeof(float) <= kPlanLdsBytes;
     const size_t lds = ldt ? (size_t)p->n_det * tabs * sizeof(float) : 0;
     const int32_t cpb = g_plan_to_cpb;
@@ -627,26 +676,21 @@ int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
     const int32_t nd = (int32_t)p->n_det;
 #define PLAN_TO(NC, TC)                                                                     \
     do {                                                                                    \
-        if (pkd)                                                                            \
-            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 1, true, false>), g,            \
-                               dim3(kPlanBlock), 0, s, p->idx, p->sspin, p->tval, p->heads, \
-                               p->wbase, p->run_pix, p->n_on, nt, plane, signal,            \
-                               det_weights, nd, cpb, target);                                    \
-        else if (ldt)                                                                       \
-            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 1, false, true>), g,            \
-                               dim3(kPlanBlock), lds, s, p->idx, p->sspin, p->tval,         \
-                               p->heads, p->wbase, p->run_pix, p->n_on, nt, plane, signal,  \
-                               det_weights, nd, cpb, map);                                       \
+        if (ldt)                                                                            \
+            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 1, true>), g, dim3(kPlanBlock),  \
+                               lds, s, p->idx, p->sspin, p->tval, p->heads, p->wbase,       \
+                               p->run_pix, p->n_on, nt, plane, signal, det_weights, nd, cpb, \
+                               map);                                                        \
         else if (g_plan_div)                                                                \
-            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 1, false, false>), g,           \
-                               dim3(kPlanBlock), 0, s, p->idx, p->sspin, p->tval, p->heads, \
-                               p->wbase, p->run_pix, p->n_on, nt, plane, signal,            \
-                               det_weights, nd, cpb, map);                                       \
+            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 1, false>), g, dim3(kPlanBlock), \
+                               0, s, p->idx, p->sspin, p->tval, p->heads, p->wbase,         \
+                               p->run_pix, p->n_on, nt, plane, signal, det_weights, nd, cpb, \
+                               map);                                                        \
         else                                                                                \
-            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 0, false, false>), g,           \
-                               dim3(kPlanBlock), 0, s, p->idx, p->sspin, p->tval, p->heads, \
-                               p->wbase, p->run_pix, p->n_on, nt, plane, signal,            \
-                               det_weights, nd, cpb, map);                                       \
+            hipLaunchKernelGGL((k_plan_to_map_c<NC, TC, ST, 0, false>), g, dim3(kPlanBlock), \
+                               0, s, p->idx, p->sspin, p->tval, p->heads, p->wbase,         \
+                               p->run_pix, p->n_on, nt, plane, signal, det_weights, nd, cpb, \
+                               map);                                                        \
     } while (0)
     if (p->tconst) {
         if (p->nc == 1) PLAN_TO(1, true);
@@ -657,12 +701,6 @@ int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
         else PLAN_TO(3, false);
     }
 #undef PLAN_TO
-    if (pkd) {
-        const dim3 gu(grid_of(plane));
-        if (p->nc == 1) hipLaunchKernelGGL(k_plan_unpack_add<1>, gu, dim3(kPlanBlock), 0, s, p->packed, (int64_t)plane, map);
-        else if (p->nc == 2) hipLaunchKernelGGL(k_plan_unpack_add<2>, gu, dim3(kPlanBlock), 0, s, p->packed, (int64_t)plane, map);
-        else hipLaunchKernelGGL(k_plan_unpack_add<3>, gu, dim3(kPlanBlock), 0, s, p->packed, (int64_t)plane, map);
-    }
     PLAN_TRY(hipGetLastError());
     return SO3G_OK;
 }
@@ -700,7 +738,8 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
                         int64_t t_block_mb, so3g_pc_plan **out, void *stream) {
     if (out == nullptr) return set_error(SO3G_ERR_VALUE, "out is NULL");
     *out = nullptr;
-    if (k != 2) return set_error(SO3G_ERR_RUNTIME, "plans support non-tiled flat maps (k = 2)");
+    if (k != 1 && k != 2)
+        return set_error(SO3G_ERR_RUNTIME, "plans support non-tiled maps (k = 1 or 2)");
     if (nc < 1 || nc > 3) return set_error(SO3G_ERR_RUNTIME, "spin component count must be 1..3");
     if (n_det <= 0 || n_t <= 0)
         return set_error(SO3G_ERR_RUNTIME, "plan needs 0 < n_det * n_t < 2^32");
@@ -709,28 +748,16 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
     if (n > (int64_t)UINT32_MAX)
         return set_error(SO3G_ERR_RUNTIME, "plan needs 0 < n_det * n_t < 2^32");
     const int64_t ny = map_shape[0], nx = map_shape[1];
-    if (ny <= 0 || nx <= 0 || ny * nx >= INT32_MAX)
-        return set_error(SO3G_ERR_RUNTIME, "map too large for a plan");
+    if (ny <= 0 || nx <= 0 || ny * nx >= INT32_MAX || (k == 1 && nx != 1))
+        return set_error(SO3G_ERR_RUNTIME, "map too large for a plan, or not (npix, 1) for k = 1");
     hipStream_t s = (hipStream_t)stream;
-    const int32_t npix = (int32_t)(ny * nx);
-    // Time blocks whose signal slice (n_det x t_block float32) stays resident
-    // in the Infinity Cache while the block's samples are gathered.
-    int64_t t_block = std::max<int64_t>(1, (t_block_mb << 20) / (4 * n_det));
-    if (t_block_mb <= 0 || t_block > n_t) t_block = n_t;
-    int64_t n_tb = (n_t + t_block - 1) / t_block;
-    while (n_tb * npix >= INT32_MAX) {  // keys must fit int32
-        t_block *= 2;
-        n_tb = (n_t + t_block - 1) / t_block;
-    }
-    const int32_t off_key = (int32_t)(n_tb * npix);
-    int end_bit = 1;
-    while ((1LL << end_bit) <= off_key) end_bit++;
+    const int64_t npix = ny * nx;
 
     so3g_pc_plan *p = new so3g_pc_plan();
     p->n_det = n_det; p->n_t = n_t; p->ny = ny; p->nx = nx; p->nc = nc;
-    p->t_block = t_block;
     int32_t *keys_in = nullptr, *keys_out = nullptr;
-    uint32_t *vals_in = nullptr, *gcount = nullptr;
+    uint32_t *vals_in = nullptr, *gcount = nullptr, *bitmap = nullptr, *wcnt = nullptr,
+             *wscan = nullptr;
     unsigned long long *d_on = nullptr;
     int *bad = nullptr;
     void *tmp = nullptr;
@@ -738,6 +765,8 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
     auto cleanup = [&]() {
         (void)hipFree(keys_in); (void)hipFree(keys_out); (void)hipFree(vals_in);
         (void)hipFree(gcount); (void)hipFree(d_on); (void)hipFree(bad); (void)hipFree(tmp);
+        (void)hipFree(bitmap); (void)hipFree(wcnt); (void)hipFree(wscan);
+        tmp = nullptr;
     };
     auto oom = [&]() {
         cleanup();
@@ -760,14 +789,58 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         }                                                                            \
     } while (0)
 
-    // 1. sort the samples by (time block, pixel)
+    // 0. visited pixels: bitmap, per-word counts, their prefix, comp2pix
+    const int64_t n_words = (npix + 31) / 32;
+    PLAN_ALLOC(bitmap, n_words * 4);
+    PLAN_ALLOC(wcnt, n_words * 4);
+    PLAN_ALLOC(wscan, n_words * 4);
+    PLAN_STEP(hipMemsetAsync(bitmap, 0, n_words * 4, s));
+    hipLaunchKernelGGL(k_plan_mark, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, k, n, ny,
+                       nx, bitmap);
+    hipLaunchKernelGGL(k_plan_wcount, dim3(grid_of(n_words)), dim3(kPlanBlock), 0, s, bitmap,
+                       n_words, wcnt);
+    PLAN_STEP(hipGetLastError());
+    PLAN_STEP(rocprim::exclusive_scan(nullptr, tmp_bytes, wcnt, wscan, 0u, (size_t)n_words,
+                                      rocprim::plus<uint32_t>(), s));
+    PLAN_ALLOC(tmp, tmp_bytes);
+    PLAN_STEP(rocprim::exclusive_scan(tmp, tmp_bytes, wcnt, wscan, 0u, (size_t)n_words,
+                                      rocprim::plus<uint32_t>(), s));
+    {
+        uint32_t lb = 0, lc = 0;
+        PLAN_STEP(hipMemcpyAsync(&lb, wscan + n_words - 1, 4, hipMemcpyDeviceToHost, s));
+        PLAN_STEP(hipMemcpyAsync(&lc, wcnt + n_words - 1, 4, hipMemcpyDeviceToHost, s));
+        PLAN_STEP(hipStreamSynchronize(s));
+        p->n_vis = (int64_t)lb + lc;
+    }
+    (void)hipFree(tmp); tmp = nullptr;
+    PLAN_ALLOC(p->comp2pix, p->n_vis * 4);
+    hipLaunchKernelGGL(k_plan_compact, dim3(grid_of(n_words)), dim3(kPlanBlock), 0, s, bitmap,
+                       wscan, n_words, p->comp2pix);
+    PLAN_STEP(hipGetLastError());
+    const int32_t n_vis = (int32_t)std::max<int64_t>(1, p->n_vis);
+    // Time blocks whose signal slice (n_det x t_block float32) stays resident
+    // in the Infinity Cache while the block's samples are gathered.
+    int64_t t_block = std::max<int64_t>(1, (t_block_mb << 20) / (4 * n_det));
+    if (t_block_mb <= 0 || t_block > n_t) t_block = n_t;
+    int64_t n_tb = (n_t + t_block - 1) / t_block;
+    while (n_tb * n_vis >= INT32_MAX) {  // keys must fit int32
+        t_block *= 2;
+        n_tb = (n_t + t_block - 1) / t_block;
+    }
+    p->t_block = t_block;
+    const int32_t off_key = (int32_t)(n_tb * n_vis);
+    int end_bit = 1;
+    while ((1LL << end_bit) <= off_key) end_bit++;
+
+    // 1. sort the samples by (time block, compact pixel)
     PLAN_ALLOC(keys_in, n * 4);
     PLAN_ALLOC(vals_in, n * 4);
     PLAN_ALLOC(keys_out, n * 4);
     PLAN_ALLOC(p->idx, n * 4);
     PLAN_ALLOC(d_on, sizeof(unsigned long long));
     hipLaunchKernelGGL(k_plan_keys, dim3(grid_of(n, kPlanBlock, INT32_MAX)), dim3(kPlanBlock), 0,
-                       s, pixel, n, nx, npix, n_t, t_block, off_key, keys_in, vals_in);
+                       s, pixel, k, n, ny, nx, n_t, t_block, n_vis, off_key, bitmap, wscan,
+                       keys_in, vals_in);
     PLAN_STEP(hipGetLastError());
     PLAN_STEP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, vals_in, p->idx,
                                         (size_t)n, 0, end_bit, s));
@@ -835,23 +908,23 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
     PLAN_STEP(hipStreamSynchronize(s));
     p->n_runs = (int64_t)last_base + last_count;
     PLAN_ALLOC(p->run_pix, p->n_runs * sizeof(uint32_t));
-    hipLaunchKernelGGL(k_plan_runs, gb, dim3(kPlanBlock), 0, s, keys_out, n_bytes, npix,
-                       p->heads, p->wbase, p->run_pix);
+    hipLaunchKernelGGL(k_plan_runs, gb, dim3(kPlanBlock), 0, s, keys_out, n_bytes, n_vis,
+                       p->comp2pix, p->heads, p->wbase, p->run_pix);
     PLAN_STEP(hipGetLastError());
 
     // 5. time-ordered copy for from_map, and the packed map buffer
     PLAN_ALLOC(p->tpix, n * sizeof(int32_t));
     PLAN_ALLOC(p->tspin, n * p->ns * sizeof(float));
-    hipLaunchKernelGGL(k_plan_tcopy, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, spin, nc, c0,
-                       n, ny, nx, p->tpix, p->tspin);
+    hipLaunchKernelGGL(k_plan_tcopy, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, k, spin, nc,
+                       c0, n, ny, nx, bitmap, wscan, p->tpix, p->tspin);
     PLAN_STEP(hipGetLastError());
-    PLAN_ALLOC(p->packed, (int64_t)npix * 4 * sizeof(double));
+    PLAN_ALLOC(p->packed, std::max<int64_t>(1, p->n_vis) * 4 * sizeof(double));
     PLAN_STEP(hipStreamSynchronize(s));
     cleanup();
 #undef PLAN_ALLOC
 #undef PLAN_STEP
     p->bytes = p->n_on * (4 + 4 * p->ns) + n_groups * (kGroup / 8 + 4) + p->n_runs * 4 +
-               n * (4 + 4 * p->ns) + (int64_t)npix * 32 + (p->tconst ? n_det * 4 : 0);
+               n * (4 + 4 * p->ns) + p->n_vis * (32 + 4) + (p->tconst ? n_det * 4 : 0);
     *out = p;
     return SO3G_OK;
 }
@@ -873,9 +946,9 @@ int so3g_pc_plan_from_map(const so3g_pc_plan *p, const double *map, float *signa
     const int64_t plane = p->ny * p->nx;
     const int64_t n = p->n_det * p->n_t;
     switch (p->nc) {
-    case 1: hipLaunchKernelGGL(k_plan_pack<1>, dim3(grid_of(plane)), dim3(kPlanBlock), 0, s, map, plane, p->packed); break;
-    case 2: hipLaunchKernelGGL(k_plan_pack<2>, dim3(grid_of(plane)), dim3(kPlanBlock), 0, s, map, plane, p->packed); break;
-    default: hipLaunchKernelGGL(k_plan_pack<3>, dim3(grid_of(plane)), dim3(kPlanBlock), 0, s, map, plane, p->packed); break;
+    case 1: hipLaunchKernelGGL(k_plan_pack<1>, dim3(grid_of(p->n_vis)), dim3(kPlanBlock), 0, s, map, plane, p->comp2pix, p->n_vis, p->packed); break;
+    case 2: hipLaunchKernelGGL(k_plan_pack<2>, dim3(grid_of(p->n_vis)), dim3(kPlanBlock), 0, s, map, plane, p->comp2pix, p->n_vis, p->packed); break;
+    default: hipLaunchKernelGGL(k_plan_pack<3>, dim3(grid_of(p->n_vis)), dim3(kPlanBlock), 0, s, map, plane, p->comp2pix, p->n_vis, p->packed); break;
     }
     PLAN_TRY(hipGetLastError());
     const bool strided = g_plan_from_layout != 0;
@@ -955,6 +1028,7 @@ int so3g_pc_plan_stats(const so3g_pc_plan *p, int64_t *stats) {
     stats[3] = p->ns;
     stats[4] = p->t_block;
     stats[5] = p->bytes;
+    stats[6] = p->n_vis;
     return SO3G_OK;
 }
 

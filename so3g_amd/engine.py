@@ -760,8 +760,10 @@ class PrecompPlan:
     """Pixel-sorted plan for repeated cached-pointing projection with one
     pointing matrix (CG-style loops).  No counterpart in the reference: built
     once from a ProjEng_Precomp pixel_index (k = 2, non-tiled flat map of
-    `map_shape` = (ny, nx)) and spin_proj (both copied into the plan), it
-    sorts the on-map samples by pixel so that to_map sums each pixel's
+    `map_shape` = (ny, nx); or k = 1, a non-tiled HEALPix map of `map_shape`
+    = (npix,)) and spin_proj (both copied into the plan), it numbers the
+    visited pixels compactly, sorts the on-map samples by pixel so that
+    to_map sums each pixel's
     contributions with a wave-level segmented reduction instead of
     per-sample atomics, and keeps a compact time-ordered copy for from_map.
     to_map adds what ProjEng_Precomp_NonTiled.to_map adds over every sample
@@ -772,15 +774,18 @@ class PrecompPlan:
     def __init__(self, pixel_index, spin_proj, map_shape, t_block_mb=64):
         import ctypes
         call = _Call()
-        pix = call.array(pixel_index, 'int32', 'pixel_index', spec=[-1, -1, 2])
-        n_det, n_t, _ = pix.shape
+        pix = call.array(pixel_index, 'int32', 'pixel_index', spec=[-1, -1, -1])
+        n_det, n_t, k = pix.shape
+        if k not in (1, 2) or len(map_shape) != k:
+            raise ValueError('plans take k = 2 pixels with map_shape (ny, nx) or '
+                             'k = 1 pixels with map_shape (npix,)')
         spin = call.array(spin_proj, 'float32', 'spin_proj', spec=[n_det, n_t, -1])
-        self.n_det, self.n_t = int(n_det), int(n_t)
+        self.n_det, self.n_t, self.k = int(n_det), int(n_t), int(k)
         self.nc = int(spin.shape[2])
-        self.map_shape = (int(map_shape[0]), int(map_shape[1]))
-        self._shape = np.array(self.map_shape, np.int64)
+        self.map_shape = tuple(int(x) for x in map_shape)
+        self._shape = np.array(self.map_shape + (1,) * (2 - k), np.int64)
         h = ctypes.c_void_p()
-        N.check(N.load().so3g_pc_plan_create(pix.data_ptr(), 2, spin.data_ptr(), self.nc,
+        N.check(N.load().so3g_pc_plan_create(pix.data_ptr(), k, spin.data_ptr(), self.nc,
                                              self.n_det, self.n_t, self._shape.ctypes.data,
                                              int(t_block_mb), ctypes.byref(h), _stream()))
         self._h = h
@@ -788,11 +793,12 @@ class PrecompPlan:
     def info(self):
         """n_on (on-map samples), runs (pixel runs in sorted order),
         t_const (T factor stored once per detector), stored_spin (factors
-        stored per sample), t_block (samples per time block), bytes."""
-        a = (ctypes.c_int64 * 6)()
+        stored per sample), t_block (samples per time block), bytes, n_vis
+        (visited pixels)."""
+        a = (ctypes.c_int64 * 7)()
         N.check(N.load().so3g_pc_plan_stats(self._h, a))
         return {'n_on': a[0], 'runs': a[1], 't_const': bool(a[2]), 'stored_spin': a[3],
-                't_block': a[4], 'bytes': a[5]}
+                't_block': a[4], 'bytes': a[5], 'n_vis': a[6]}
 
     def to_map(self, map, signal, det_weights=None):
         call = _Call()

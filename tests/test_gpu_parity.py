@@ -381,6 +381,7 @@ def test_precomp_plan(gpu, n_det, n_samp, comps, tconst, t_block_mb):
     tb = (np.arange(n_samp) // info['t_block'])[None, :].repeat(n_det, 0)
     keys = (tb * 30000 + pix[..., 0] * 300 + pix[..., 1])[on]
     assert info['runs'] == len(np.unique(keys))
+    assert info['n_vis'] == len(np.unique((pix[..., 0] * 300 + pix[..., 1])[on]))
     dw = c.det_weights.copy()
     dw[3] = 0.
     from so3g_amd import _native
@@ -426,6 +427,43 @@ def test_precomp_plan(gpu, n_det, n_samp, comps, tconst, t_block_mb):
                                              None))
     r = oracle.precomp_to_map(c.og, np.zeros((nc, 100, 300)), pix, spin, c.signal)
     maps_close(mf.cpu().numpy(), r, 'plan to_map f64', atol=1e-5)
+
+
+@pytest.mark.parametrize('nest,nside,t_block_mb', [(True, 512, 64), (False, 4096, 1)])
+def test_precomp_plan_healpix(gpu, nest, nside, t_block_mb):
+    """PrecompPlan on a non-tiled HEALPix map (k = 1 pixel numbers, map
+    (nc, npix)): visited pixels numbered compactly, to_map == Precomp to_map
+    (oracle), from_map == Precomp from_map bit for bit, including samples
+    flagged off the map (pixel -1)."""
+    import so3g_amd
+    c = Case('HP', comps='TQU', nest=nest, nside=nside, n_det=24, n_samp=3000)
+    pix, spin = [np.array(x) for x in c.engine.pointing_matrix(c.bore, c.dets, c.resps)]
+    assert pix.shape[2] == 1
+    pix[2, 100:400, 0] = -1
+    pix[5, ::7, 0] = -1
+    npix = 12 * nside * nside
+    plan = so3g_amd.PrecompPlan(pix, spin, (npix,), t_block_mb=t_block_mb)
+    info = plan.info()
+    on = pix[..., 0] >= 0
+    assert info['n_on'] == int(on.sum())
+    assert info['n_vis'] == len(np.unique(pix[..., 0][on]))
+    tb = (np.arange(c.signal.shape[1]) // info['t_block'])[None, :].repeat(24, 0)
+    assert info['runs'] == len(np.unique((tb.astype(np.int64) * npix + pix[..., 0])[on]))
+    dw = c.det_weights.copy()
+    dw[1] = 0.
+    for w in (None, dw):
+        g = plan.to_map(np.zeros((3, npix)), c.signal, w)
+        r = oracle.precomp_to_map(c.og, np.zeros((3, npix)), pix, spin, c.signal, w)
+        maps_close(g, r, f'HP plan to_map nside={nside}', atol=1e-9)
+    m = c.map_like([3])
+    s0 = np.array(c.signal)
+    assert_bit_equal(np.array(plan.from_map(m)), oracle.precomp_from_map(c.og, m, pix, spin),
+                     'HP plan from_map')
+    assert_bit_equal(np.array(plan.from_map(m, s0.copy())),
+                     oracle.precomp_from_map(c.og, m, pix, spin, s0.copy()),
+                     'HP plan from_map accumulate')
+    with pytest.raises(ValueError):
+        so3g_amd.PrecompPlan(pix, spin, (npix, 1))
 
 
 def test_precision_modes(gpu):
