@@ -213,12 +213,16 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape,
  * so they may be freed afterwards.  The pixels the samples visit are numbered
  * in ascending order (compact pixels), so the plan's pixel tables scale with
  * the visited pixels, not the map.  Samples are grouped in time blocks whose
- * signal slice is about t_block_mb MiB (sized to stay in the Infinity Cache;
- * <= 0: one block), then sorted by pixel within each block.  Stored: per
- * on-map sample the sample index and the spin factors (the T factor once per
- * detector when it is constant over the detector, as in so3g's pointing
- * matrices), run-start bits and run pixels; per sample a time-ordered compact
- * pixel and the same spin factors.  so3g_pc_plan_to_map adds the same
+ * signal slice is about t_block_mb MiB (0: one block; < 0: the layout's
+ * default), then sorted within each block: by map tile (64 x 64 pixels, or
+ * 4096 visited HEALPix pixels), keeping sample order, in the tiled layout
+ * (option "plan_layout" 1, default; one time block); by pixel in layout 0
+ * (64 MiB blocks, sized to stay in the Infinity Cache).  Stored: per on-map
+ * sample the spin factors (the T factor once per detector when it is
+ * constant over the detector, as in so3g's pointing matrices) and the tile
+ * slot (tiled) or the sample index and run-start bits (layout 0); per
+ * sample a time-ordered compact pixel and the same spin factors (unless
+ * "plan_time_copy" is 0 in the tiled layout).  so3g_pc_plan_to_map adds the same
  * contributions as so3g_precomp_to_map over every sample (float64 sums in
  * another order); so3g_pc_plan_from_map writes what so3g_precomp_from_map
  * writes, bit for bit. */
@@ -242,9 +246,9 @@ int so3g_precomp_from_map_f64(const double *map, const int64_t *map_shape,
                               int64_t n_det, int64_t n_t, double *signal, int accumulate,
                               void *stream);
 int so3g_pc_plan_info(const so3g_pc_plan *plan, int64_t *n_on, int64_t *bytes);
-/* stats[7]: n_on, pixel runs, T factor constant per detector (0/1), stored
- * spin factors per sample, samples per time block, device bytes, visited
- * pixels */
+/* stats[10]: n_on, pixel runs (layout 0), T factor constant per detector
+ * (0/1), stored spin factors per sample, samples per time block, device
+ * bytes, visited pixels, layout, stretches and work items (tiled layout) */
 int so3g_pc_plan_stats(const so3g_pc_plan *plan, int64_t *stats);
 int so3g_pc_plan_destroy(so3g_pc_plan *plan);
 
@@ -436,6 +440,13 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   tables (T factor, weights) in LDS when they fit in 32 KB
  *   "plan_to_cpb"   2048-sample chunks per block of the planned scatter
  *                   (default 2, 1..64)
+ *   "plan_layout"   layout of plans created afterwards: 1 (default) map
+ *                   tiles summed in LDS, 0 pixel runs
+ *   "plan_tile_item"  sorted positions per work item of the tiled layout
+ *                   (default 262144, 64..2^30)
+ *   "plan_time_copy"  1 (default): tiled plans also keep the time-ordered
+ *                   copy for from_map; 0: from_map gathers from the tiles
+ *                   (smaller plan, slower gather)
  *   "plan_from_layout"  1 (default): a lane's gather samples are 64 apart, so
  *                   one gather instruction covers 64 consecutive samples;
  *                   0: consecutive samples per lane (measured slower, 3.51 vs

@@ -21,6 +21,9 @@ extern int g_plan_from_v;
 extern int g_plan_from_layout;
 extern int g_plan_to_lds;
 extern int g_plan_to_cpb;
+extern int g_plan_layout;
+extern int64_t g_plan_tile_item;
+extern int g_plan_time_copy;
 extern template int dispatch<PROJ_FLAT>(const OpArgs &);
 extern template int dispatch<PROJ_QUAT>(const OpArgs &);
 extern template int dispatch<PROJ_CAR>(const OpArgs &);
@@ -741,6 +744,16 @@ int so3g_set_option(const char *name, int64_t value) {
     else if (n == "plan_div") so3g::g_plan_div = (int)(value != 0);
     else if (n == "plan_from_layout") so3g::g_plan_from_layout = (int)(value != 0);
     else if (n == "plan_to_lds") so3g::g_plan_to_lds = (int)(value != 0);
+    else if (n == "plan_layout") {
+        if (value < 0 || value > 1) return fail(SO3G_ERR_VALUE, "plan_layout must be 0 or 1");
+        so3g::g_plan_layout = (int)value;
+    }
+    else if (n == "plan_time_copy") so3g::g_plan_time_copy = (int)(value != 0);
+    else if (n == "plan_tile_item") {
+        if (value < 64 || value > (1 << 30))
+            return fail(SO3G_ERR_VALUE, "plan_tile_item must be 64..2^30");
+        so3g::g_plan_tile_item = value;
+    }
     else if (n == "plan_to_cpb") {
         if (value < 1 || value > 64) return fail(SO3G_ERR_VALUE, "plan_to_cpb must be 1..64");
         so3g::g_plan_to_cpb = (int)value;
@@ -764,6 +777,9 @@ int so3g_get_option(const char *name, int64_t *value) {
     else if (n == "plan_from_layout") *value = so3g::g_plan_from_layout;
     else if (n == "plan_to_lds") *value = so3g::g_plan_to_lds;
     else if (n == "plan_to_cpb") *value = so3g::g_plan_to_cpb;
+    else if (n == "plan_layout") *value = so3g::g_plan_layout;
+    else if (n == "plan_tile_item") *value = so3g::g_plan_tile_item;
+    else if (n == "plan_time_copy") *value = so3g::g_plan_time_copy;
     else if (n == "plan_from_v") *value = so3g::g_plan_from_v;
     else return fail(SO3G_ERR_VALUE, "unknown option " + n);
     return SO3G_OK;

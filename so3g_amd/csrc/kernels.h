@@ -410,15 +410,32 @@ __global__ __launch_bounds__(kBlock) void k_to_weights(PtgIn in, Geo g,
 // --------------------------------------------------------------------------
 // tile_hits and per-sample domains (thread-assignment helpers).
 // --------------------------------------------------------------------------
+// A wave's samples mostly fall in one or two tiles: the active lanes that
+// share a tile add their count with one atomic (a per-sample atomic on a few
+// counters serialises, ~3.5 s for 7.2e8 samples).
 template <int PROJ, int PK, int INTERP>
 __global__ __launch_bounds__(kBlock) void k_tile_hits(PtgIn in, Geo g, int32_t *hits) {
+    const int lane = (int)__lane_id();
     for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
         const Q4 qd = load_q(in.dets + 4 * d);
         double co[4];
         get_coords<PROJ>(qb, qd, in.tabs, co);
         Pt p[4];
         const int np = get_pixels<PK, INTERP>(g, co, p);
-        for (int ip = 0; ip < np; ip++) atomicAdd(hits + p[ip].i0, 1);
+#pragma unroll
+        for (int ip = 0; ip < (INTERP ? 4 : 1); ip++) {
+            const int key = ip < np ? (int)p[ip].i0 : -1;
+            bool mine = key >= 0;
+            uint64_t todo = __ballot(mine);
+            while (todo) {
+                const int leader = __ffsll((unsigned long long)todo) - 1;
+                const int lk = __shfl(key, leader);
+                const uint64_t same = __ballot(mine && key == lk);
+                if (lane == leader) atomicAdd(hits + lk, (int)__popcll(same));
+                if (key == lk) mine = false;
+                todo &= ~same;
+            }
+        }
     });
 }
 

@@ -25,6 +25,18 @@
 // popcount over the head bits.  from_map reads a time-ordered copy: compact
 // pixel index (4 B, -1 off the map) and the ns stored spin factors.
 //
+// Tiled layout (the default, "plan_layout" 1): the samples are sorted by
+// (time block, map tile) instead -- 64 x 64 pixels of a flat map, or 4096
+// consecutive visited pixels of a HEALPix map -- and keep their sample order
+// inside a tile, so a detector's pass over a tile is a stretch of
+// consecutive samples (~100 for the 1.5 degree scan) and to_map's signal
+// loads are nearly contiguous.  Work items of up to 256 K positions sum into
+// the tile held in LDS and add it to the map once.  Per position: the slot
+// in the tile (2 B) and the stored factors; per 64 positions: a stretch-start
+// mask, the first sample and a start count; per stretch: its first sample.
+// from_map keeps the time-ordered gather (or, with "plan_time_copy" 0 and a
+// smaller plan, gathers from the tile in LDS).
+//
 // Pixels are numbered compactly over the pixels the observation visits (a
 // bitmap of the map and a prefix count of its words): sort keys, the
 // time-ordered copy and from_map's packed map then scale with the visited
@@ -71,6 +83,18 @@ struct so3g_pc_plan {
     int32_t *comp2pix;     // [n_vis] flat pixel of each compact index (ascending)
     int64_t n_vis;         // visited pixels
     int64_t bytes;
+    // Tiled layout (layout 1): idx, heads, wbase, run_pix, tpix, tspin and
+    // packed are unused; sspin holds the factors in tiled sorted order.
+    int32_t layout;        // 0: pixel-sorted runs, 1: tiles
+    uint16_t *tslot;       // [n_on] slot within the tile
+    uint64_t *tmask;       // [n_chunks] stretch starts per 64 positions
+    uint32_t *tfirst;      // [n_chunks] sample of each chunk's first position
+    uint32_t *thead;       // [n_chunks] stretch starts before each chunk
+    uint32_t *head_i;      // [n_heads] sample of each stretch start
+    int32_t *tile_pix;     // [n_tiles * 4096] flat pixel of each slot, -1 off the map
+    void *items;           // [n_items] work items (first position, count, tile)
+    uint32_t *off_idx;     // [n_off] samples off the map
+    int64_t n_items, n_tiles, n_heads, n_off;
 };
 
 namespace so3g {
@@ -97,6 +121,13 @@ int g_plan_from_layout = 1;
 // 2048-sample chunks per block ("plan_to_cpb").
 int g_plan_to_lds = 1;
 int g_plan_to_cpb = 2;
+// Plan layout chosen at creation ("plan_layout"): 0 pixel-sorted runs, 1
+// map tiles in LDS; and the tiled layout's positions per work item.
+int g_plan_layout = 1;
+int64_t g_plan_tile_item = 262144;
+// Tiled layout: also keep the time-ordered copy for from_map ("plan_time_copy",
+// 1: 2.15 ms gather vs 3.51 ms over the tiles; 0: a smaller plan).
+int g_plan_time_copy = 1;
 constexpr int64_t kPlanLdsBytes = 32768;
 // The planned gather's streams (time-ordered pixels and spin factors, read
 // once per pass) and its signal stores are non-temporal, so that they do not
@@ -125,15 +156,31 @@ __device__ __forceinline__ uint32_t compact_of(int64_t p, const uint32_t *bitmap
     return wscan[p >> 5] + __popc(w & ((1u << (p & 31)) - 1u));
 }
 
-// Visited-pixel bitmap.
+// Visited-pixel bitmap.  A wave's 64 consecutive samples fall in a few
+// bitmap words: the lanes that share a word OR their bits across the wave
+// and one of them adds them with a single atomic (one atomic per sample on
+// the few thousand words of a map took 12.8 ms for 7.2e8 samples).  The
+// loop bound is uniform per wave, so every lane takes part in the shuffles.
 __global__ __launch_bounds__(kPlanBlock) void k_plan_mark(const int32_t *pix, int k, int64_t n,
                                                           int64_t ny, int64_t nx,
                                                           uint32_t *bitmap) {
-    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kPlanBlock) {
-        const int64_t p = flat_pixel(pix, k, i, ny, nx);
-        if (p >= 0 && !(bitmap[p >> 5] & (1u << (p & 31))))
-            atomicOr(bitmap + (p >> 5), 1u << (p & 31));
+    const int lane = (int)__lane_id();
+    for (int64_t i0 = (int64_t)blockIdx.x * kPlanBlock + (threadIdx.x & ~63); i0 < n;
+         i0 += (int64_t)gridDim.x * kPlanBlock) {
+        const int64_t i = i0 + lane;
+        const int64_t p = i < n ? flat_pixel(pix, k, i, ny, nx) : -1;
+        const int32_t w = p >= 0 ? (int32_t)(p >> 5) : -1;
+        const uint32_t bit = p >= 0 ? 1u << (p & 31) : 0u;
+        uint64_t todo = __ballot(w >= 0);
+        while (todo) {
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const int32_t lw = __shfl(w, leader);
+            uint32_t v = w == lw ? bit : 0u;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v |= (uint32_t)__shfl_xor((int)v, off);
+            if (lane == leader && (bitmap[lw] & v) != v) atomicOr(bitmap + lw, v);
+            todo &= ~__ballot(w == lw);
+        }
     }
 }
 
@@ -644,6 +691,374 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tiled layout.  Samples are sorted by (time block, map tile) -- 64 x 64
+// pixels of a flat map, or 4096 consecutive visited pixels of a HEALPix map
+// -- and stay in sample order inside a tile, so a wave's 64 positions fall in
+// one or two stretches of consecutive samples of one detector and its signal
+// loads are nearly contiguous.  Work items (tile, positions [b, e)) sum into
+// or gather from the tile's values held in LDS.
+//
+// Per position: the slot within the tile (u16) and the stored factors.  The
+// sample index of position j is encoded by stretches: bit j % 64 of
+// tmask[j / 64] marks a position that does not continue the previous one's
+// stretch (different tile or time block, or not the next sample), tfirst[j /
+// 64] is the sample index of the chunk's first position, thead[j / 64] counts
+// the stretch starts before the chunk and head_i lists each start's sample.
+// ---------------------------------------------------------------------------
+constexpr int kTileBits = 6;                   // 64 x 64 tiles of a flat map
+constexpr int kTS = 1 << (2 * kTileBits);      // slots per tile
+constexpr int kTBlock = 1024;                  // threads per work item
+
+struct TItem {
+    int64_t b;      // first sorted position
+    int32_t n;      // positions
+    int32_t tile;   // compact tile id
+};
+
+// Sample index of this lane's position in 64-position chunk c.
+__device__ __forceinline__ uint32_t tile_sample(uint64_t mask, uint32_t first, uint32_t hbase,
+                                                const uint32_t *__restrict__ head_i, int lane) {
+    const uint64_t h = mask & (~0ull >> (63 - lane));
+    if (h == 0) return first + (uint32_t)lane;
+    const int p = 63 - __clzll((long long)h);
+    return head_i[hbase + __popcll(h) - 1] + (uint32_t)(lane - p);
+}
+
+// Flat-map tiles touched by the samples (byte flags; every writer stores 1).
+__global__ __launch_bounds__(kPlanBlock) void k_tp_tmark(const int32_t *pix, int64_t n,
+                                                         int64_t ny, int64_t nx, int64_t gtx,
+                                                         uint8_t *used) {
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kPlanBlock) {
+        const int2 q = *reinterpret_cast<const int2 *>(pix + 2 * i);
+        if (q.x >= 0 && q.x < ny && q.y >= 0 && q.y < nx)
+            used[(q.x >> kTileBits) * gtx + (q.y >> kTileBits)] = 1;
+    }
+}
+
+// Tile and slot of sample i (tile < 0 off the map).  k = 2: 64 x 64 tiles,
+// tile_of maps the flat tile number to the compact one; k = 1: compact pixel
+// blocks of kTS.
+__device__ __forceinline__ void tile_slot(const int32_t *pix, int k, int64_t i, int64_t ny,
+                                          int64_t nx, int64_t gtx, const int32_t *tile_of,
+                                          const uint32_t *bitmap, const uint32_t *wscan,
+                                          int32_t &tile, int32_t &slot) {
+    if (k == 1) {
+        const int32_t q = pix[i];
+        if (q < 0 || q >= ny) { tile = -1; slot = 0; return; }
+        const uint32_t c = compact_of(q, bitmap, wscan);
+        tile = (int32_t)(c >> (2 * kTileBits));
+        slot = (int32_t)(c & (kTS - 1));
+        return;
+    }
+    const int2 q = *reinterpret_cast<const int2 *>(pix + 2 * i);
+    if (q.x < 0 || q.x >= ny || q.y < 0 || q.y >= nx) { tile = -1; slot = 0; return; }
+    tile = tile_of[(q.x >> kTileBits) * gtx + (q.y >> kTileBits)];
+    slot = ((q.x & ((1 << kTileBits) - 1)) << kTileBits) | (q.y & ((1 << kTileBits) - 1));
+}
+
+__global__ __launch_bounds__(kPlanBlock) void k_tp_keys(const int32_t *pix, int k, int64_t n,
+                                                        int64_t ny, int64_t nx, int64_t gtx,
+                                                        const int32_t *tile_of,
+                                                        const uint32_t *bitmap,
+                                                        const uint32_t *wscan, int64_t n_t,
+                                                        int64_t t_block, int32_t n_tiles,
+                                                        int32_t off_key, int32_t *keys,
+                                                        uint32_t *vals) {
+    const int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
+    if (i >= n) return;
+    int32_t tile, slot;
+    tile_slot(pix, k, i, ny, nx, gtx, tile_of, bitmap, wscan, tile, slot);
+    const int64_t tb = (i % n_t) / t_block;
+    keys[i] = tile >= 0 ? (int32_t)(tb * n_tiles + tile) : off_key;
+    vals[i] = (uint32_t)i;
+}
+
+// Slot and stored factors of each sorted position.
+__global__ __launch_bounds__(kPlanBlock) void k_tp_fill(const int32_t *pix, int k,
+                                                        const float *spin, int nc, int c0,
+                                                        const uint32_t *idx, int64_t n_on,
+                                                        int64_t ny, int64_t nx, int64_t gtx,
+                                                        const int32_t *tile_of,
+                                                        const uint32_t *bitmap,
+                                                        const uint32_t *wscan, uint16_t *slot,
+                                                        float *fac) {
+    const int ns = nc - c0;
+    for (int64_t j = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; j < n_on;
+         j += (int64_t)gridDim.x * kPlanBlock) {
+        const int64_t i = idx[j];
+        int32_t tile, sl;
+        tile_slot(pix, k, i, ny, nx, gtx, tile_of, bitmap, wscan, tile, sl);
+        slot[j] = (uint16_t)sl;
+        for (int c = 0; c < ns; c++) fac[j * ns + c] = spin[i * nc + c0 + c];
+    }
+}
+
+// Per 64-position chunk: stretch-start mask, their count, first sample.  One
+// wave per chunk; the grid covers n_chunks * 64 threads exactly.
+__global__ __launch_bounds__(kPlanBlock) void k_tp_chunks(const int32_t *key,
+                                                          const uint32_t *idx, int64_t n_on,
+                                                          int64_t n_chunks, uint64_t *mask,
+                                                          uint32_t *cnt, uint32_t *first) {
+    const int64_t j = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
+    const int64_t c = j >> 6;
+    if (c >= n_chunks) return;   // whole waves
+    const int lane = (int)(j & 63);
+    bool head = false;
+    if (j < n_on)
+        head = j == 0 || key[j] != key[j - 1] || idx[j] != idx[j - 1] + 1u;
+    const uint64_t m = __ballot(head);
+    if (lane == 0) {
+        mask[c] = m;
+        cnt[c] = (uint32_t)__popcll(m);
+        first[c] = idx[j];
+    }
+}
+
+__global__ __launch_bounds__(kPlanBlock) void k_tp_heads(const uint64_t *mask,
+                                                         const uint32_t *hbase,
+                                                         const uint32_t *idx, int64_t n_on,
+                                                         uint32_t *head_i) {
+    for (int64_t j = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; j < n_on;
+         j += (int64_t)gridDim.x * kPlanBlock) {
+        const uint64_t m = mask[j >> 6];
+        const int l = (int)(j & 63);
+        if ((m >> l) & 1ull)
+            head_i[hbase[j >> 6] + __popcll(m & ((1ull << l) - 1ull))] = idx[j];
+    }
+}
+
+// seg[v] = first sorted position with key >= v, v in [0, n_keys].
+__global__ __launch_bounds__(kPlanBlock) void k_tp_segs(const int32_t *key, int64_t n_on,
+                                                        int32_t n_keys, int64_t *seg) {
+    const int64_t v = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
+    if (v > n_keys) return;
+    int64_t lo = 0, hi = n_on;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (key[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    seg[v] = lo;
+}
+
+// Flat pixel of each (compact tile, slot), -1 for slots off the map.
+__global__ __launch_bounds__(kPlanBlock) void k_tp_tilepix(int k, int64_t ny, int64_t nx,
+                                                           int64_t gtx,
+                                                           const int32_t *tile_flat,
+                                                           const int32_t *comp2pix,
+                                                           int64_t n_vis, int64_t n_slots,
+                                                           int32_t *tile_pix) {
+    for (int64_t q = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; q < n_slots;
+         q += (int64_t)gridDim.x * kPlanBlock) {
+        if (k == 1) {
+            tile_pix[q] = q < n_vis ? comp2pix[q] : -1;
+        } else {
+            const int64_t tf = tile_flat[q >> (2 * kTileBits)];
+            if (tf < 0) { tile_pix[q] = -1; continue; }
+            const int sl = (int)(q & (kTS - 1));
+            const int64_t iy = (tf / gtx) * (1 << kTileBits) + (sl >> kTileBits);
+            const int64_t ix = (tf % gtx) * (1 << kTileBits) + (sl & ((1 << kTileBits) - 1));
+            tile_pix[q] = (iy < ny && ix < nx) ? (int32_t)(iy * nx + ix) : -1;
+        }
+    }
+}
+
+// Tiled to_map: one work item per block; contributions summed in the
+// item's tile in LDS (float64 LDS atomics), then added to the map, one
+// atomic per touched slot and component.  Each wave takes kTU consecutive
+// 64-position chunks at a time and issues each stage's loads for all of them
+// before using any (chunk tables, then stretch starts, slots and factors,
+// then the signal), so the dependent loads overlap.  LDT: per-detector
+// tables in LDS.
+constexpr int kTU = 8;
+
+template <int NC, bool TC, typename ST, bool LDT>
+__global__ __launch_bounds__(kTBlock) void k_tp_to_map(
+    const TItem *__restrict__ items, const uint16_t *__restrict__ slot,
+    const float *__restrict__ fac, const uint64_t *__restrict__ tmask,
+    const uint32_t *__restrict__ tfirst, const uint32_t *__restrict__ thead,
+    const uint32_t *__restrict__ head_i, const int32_t *__restrict__ tile_pix,
+    const float *__restrict__ tval, FastDiv n_t, uint32_t plane, const ST *__restrict__ sig,
+    const float *__restrict__ det_weights, int32_t n_det, double *map) {
+    constexpr int NS = TC ? NC - 1 : NC;
+    __shared__ double acc[NC * kTS];
+    extern __shared__ float s_tab[];   // LDT: [n_det] T factors (TC), [n_det] weights
+    const int w_off = TC ? n_det : 0;
+    const TItem it = items[blockIdx.x];
+    for (int q = threadIdx.x; q < NC * kTS; q += kTBlock) acc[q] = 0.;
+    if constexpr (LDT) {
+        for (int d = threadIdx.x; d < n_det; d += kTBlock) {
+            if (TC) s_tab[d] = tval[d];
+            if (det_weights != nullptr) s_tab[w_off + d] = det_weights[d];
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t e = it.b + it.n;
+    const int64_t c_end = (e + 63) >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int64_t cb = (it.b >> 6) + (int64_t)wave * kTU; cb < c_end;
+         cb += (int64_t)(kTBlock / 64) * kTU) {
+        uint64_t m[kTU];
+        uint32_t f0[kTU], hb[kTU];
+#pragma unroll
+        for (int u = 0; u < kTU; u++) {
+            const int64_t c = min(cb + u, c_end - 1);
+            m[u] = tmask[c]; f0[u] = tfirst[c]; hb[u] = thead[c];
+        }
+        uint32_t si[kTU];
+        int sl[kTU];
+        float sp[kTU][NS > 0 ? NS : 1];
+        bool act[kTU];
+#pragma unroll
+        for (int u = 0; u < kTU; u++) {
+            const int64_t j = (cb + u) * 64 + lane;
+            act[u] = j >= it.b && j < e;
+            si[u] = act[u] ? tile_sample(m[u], f0[u], hb[u], head_i, lane) : 0u;
+            sl[u] = act[u] ? slot[j] : 0;
+            if constexpr (NS == 2) {
+                const f32x2 v = act[u] ? reinterpret_cast<const f32x2 *>(fac)[j]
+                                       : f32x2{0.f, 0.f};
+                sp[u][0] = v.x; sp[u][1] = v.y;
+            } else {
+#pragma unroll
+                for (int q = 0; q < NS; q++) sp[u][q] = act[u] ? fac[j * NS + q] : 0.f;
+            }
+        }
+        ST sg[kTU];
+#pragma unroll
+        for (int u = 0; u < kTU; u++) sg[u] = act[u] ? sig[si[u]] : (ST)0;
+#pragma unroll
+        for (int u = 0; u < kTU; u++) {
+            if (!act[u]) continue;
+            float pf[NC];
+            uint32_t det = 0;
+            if (TC || det_weights != nullptr) det = fast_div<1>(si[u], n_t);
+            if constexpr (TC) {
+                if constexpr (LDT) pf[0] = s_tab[det];
+                else pf[0] = tval[det];
+#pragma unroll
+                for (int q = 1; q < NC; q++) pf[q] = sp[u][q - 1];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NC; q++) pf[q] = sp[u][q];
+            }
+            float w = 1.f;
+            if (det_weights != nullptr) {
+                if constexpr (LDT) w = s_tab[w_off + det];
+                else w = det_weights[det];
+            }
+            if (w == 0.f) continue;   // a zero weight skips the detector (Projection.cxx:2104)
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                double x;
+                if constexpr (std::is_same<ST, double>::value)
+                    x = (double)w * sg[u] * (double)pf[q];
+                else
+                    x = (double)(w * sg[u] * pf[q]);
+                atomicAdd(acc + q * kTS + sl[u], x);
+            }
+        }
+    }
+    __syncthreads();
+    const int32_t *tp = tile_pix + (int64_t)it.tile * kTS;
+    for (int q = threadIdx.x; q < kTS; q += kTBlock) {
+        const int32_t px = tp[q];
+        if (px < 0) continue;
+#pragma unroll
+        for (int cc = 0; cc < NC; cc++) {
+            const double v = acc[cc * kTS + q];
+            if (v != 0.) unsafeAtomicAdd(map + (int64_t)cc * plane + px, v);
+        }
+    }
+}
+
+// Tiled from_map: the item's tile values are loaded into LDS, then each
+// position's signal is written (or accumulated into), kTU chunks per wave
+// at a time as in k_tp_to_map; arithmetic as k_plan_from_map.
+template <int NC, bool TC>
+__global__ __launch_bounds__(kTBlock) void k_tp_from_map(
+    const TItem *__restrict__ items, const uint16_t *__restrict__ slot,
+    const float *__restrict__ fac, const uint64_t *__restrict__ tmask,
+    const uint32_t *__restrict__ tfirst, const uint32_t *__restrict__ thead,
+    const uint32_t *__restrict__ head_i, const int32_t *__restrict__ tile_pix,
+    const float *__restrict__ tval, FastDiv n_t, uint32_t plane, const double *__restrict__ map,
+    float *__restrict__ sig, int accumulate) {
+    constexpr int NS = TC ? NC - 1 : NC;
+    __shared__ double mv[NC * kTS];
+    const TItem it = items[blockIdx.x];
+    const int32_t *tp = tile_pix + (int64_t)it.tile * kTS;
+    for (int q = threadIdx.x; q < kTS; q += kTBlock) {
+        const int32_t px = tp[q];
+#pragma unroll
+        for (int cc = 0; cc < NC; cc++)
+            mv[cc * kTS + q] = px >= 0 ? map[(int64_t)cc * plane + px] : 0.;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t e = it.b + it.n;
+    const int64_t c_end = (e + 63) >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int64_t cb = (it.b >> 6) + (int64_t)wave * kTU; cb < c_end;
+         cb += (int64_t)(kTBlock / 64) * kTU) {
+        uint64_t m[kTU];
+        uint32_t f0[kTU], hb[kTU];
+#pragma unroll
+        for (int u = 0; u < kTU; u++) {
+            const int64_t c = min(cb + u, c_end - 1);
+            m[u] = tmask[c]; f0[u] = tfirst[c]; hb[u] = thead[c];
+        }
+        uint32_t si[kTU];
+        int sl[kTU];
+        float sp[kTU][NS > 0 ? NS : 1];
+        bool act[kTU];
+        float o[kTU];
+#pragma unroll
+        for (int u = 0; u < kTU; u++) {
+            const int64_t j = (cb + u) * 64 + lane;
+            act[u] = j >= it.b && j < e;
+            si[u] = act[u] ? tile_sample(m[u], f0[u], hb[u], head_i, lane) : 0u;
+            sl[u] = act[u] ? slot[j] : 0;
+            if constexpr (NS == 2) {
+                const f32x2 v = act[u] ? reinterpret_cast<const f32x2 *>(fac)[j]
+                                       : f32x2{0.f, 0.f};
+                sp[u][0] = v.x; sp[u][1] = v.y;
+            } else {
+#pragma unroll
+                for (int q = 0; q < NS; q++) sp[u][q] = act[u] ? fac[j * NS + q] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kTU; u++) o[u] = (accumulate && act[u]) ? sig[si[u]] : 0.f;
+#pragma unroll
+        for (int u = 0; u < kTU; u++) {
+            if (!act[u]) continue;
+            float pf[NC];
+            if constexpr (TC) {
+                pf[0] = tval[fast_div<1>(si[u], n_t)];
+#pragma unroll
+                for (int q = 1; q < NC; q++) pf[q] = sp[u][q - 1];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NC; q++) pf[q] = sp[u][q];
+            }
+            float sr = 0;
+#pragma unroll
+            for (int q = 0; q < NC; q++) sr = sr + pf[q] * mv[q * kTS + sl[u]];
+            sig[si[u]] = accumulate ? o[u] + sr : sr;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kPlanBlock) void k_tp_zero(const uint32_t *off, int64_t n_off,
+                                                        float *sig) {
+    for (int64_t q = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; q < n_off;
+         q += (int64_t)gridDim.x * kPlanBlock)
+        sig[off[q]] = 0.f;
+}
+
 }  // namespace so3g
 
 namespace {
@@ -657,6 +1072,9 @@ void free_plan_arrays(so3g_pc_plan *p) {
     (void)hipFree(p->heads); (void)hipFree(p->wbase); (void)hipFree(p->run_pix);
     (void)hipFree(p->tpix); (void)hipFree(p->tspin); (void)hipFree(p->packed);
     (void)hipFree(p->comp2pix);
+    (void)hipFree(p->tslot); (void)hipFree(p->tmask); (void)hipFree(p->tfirst);
+    (void)hipFree(p->thead); (void)hipFree(p->head_i); (void)hipFree(p->tile_pix);
+    (void)hipFree(p->items); (void)hipFree(p->off_idx);
 }
 
 template <typename ST>
@@ -674,6 +1092,34 @@ int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
     const int32_t cpb = g_plan_to_cpb;
     const dim3 g(grid_of(p->n_on, (int64_t)kPlanBlock * kPlanR * cpb, INT32_MAX));
     const int32_t nd = (int32_t)p->n_det;
+    if (p->layout == 1) {
+        const dim3 gi((unsigned)p->n_items);
+        const TItem *items = (const TItem *)p->items;
+#define TP_TO(NC, TC)                                                                       \
+    do {                                                                                    \
+        if (ldt)                                                                            \
+            hipLaunchKernelGGL((k_tp_to_map<NC, TC, ST, true>), gi, dim3(kTBlock), lds, s,  \
+                               items, p->tslot, p->sspin, p->tmask, p->tfirst, p->thead,   \
+                               p->head_i, p->tile_pix, p->tval, nt, plane, signal,          \
+                               det_weights, nd, map);                                       \
+        else                                                                                \
+            hipLaunchKernelGGL((k_tp_to_map<NC, TC, ST, false>), gi, dim3(kTBlock), 0, s,   \
+                               items, p->tslot, p->sspin, p->tmask, p->tfirst, p->thead,   \
+                               p->head_i, p->tile_pix, p->tval, nt, plane, signal,          \
+                               det_weights, nd, map);                                       \
+    } while (0)
+        if (p->tconst) {
+            if (p->nc == 1) TP_TO(1, true);
+            else TP_TO(3, true);
+        } else {
+            if (p->nc == 1) TP_TO(1, false);
+            else if (p->nc == 2) TP_TO(2, false);
+            else TP_TO(3, false);
+        }
+#undef TP_TO
+        PLAN_TRY(hipGetLastError());
+        return SO3G_OK;
+    }
 #define PLAN_TO(NC, TC)                                                                     \
     do {                                                                                    \
         if (ldt)                                                                            \
@@ -758,6 +1204,9 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
     int32_t *keys_in = nullptr, *keys_out = nullptr;
     uint32_t *vals_in = nullptr, *gcount = nullptr, *bitmap = nullptr, *wcnt = nullptr,
              *wscan = nullptr;
+    uint8_t *used = nullptr;
+    int32_t *tile_of = nullptr, *tile_flat = nullptr;
+    int64_t *seg = nullptr;
     unsigned long long *d_on = nullptr;
     int *bad = nullptr;
     void *tmp = nullptr;
@@ -766,7 +1215,10 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         (void)hipFree(keys_in); (void)hipFree(keys_out); (void)hipFree(vals_in);
         (void)hipFree(gcount); (void)hipFree(d_on); (void)hipFree(bad); (void)hipFree(tmp);
         (void)hipFree(bitmap); (void)hipFree(wcnt); (void)hipFree(wscan);
-        tmp = nullptr;
+        (void)hipFree(used); (void)hipFree(tile_of); (void)hipFree(tile_flat); (void)hipFree(seg);
+        keys_in = keys_out = nullptr; vals_in = gcount = bitmap = wcnt = wscan = nullptr;
+        d_on = nullptr; bad = nullptr; used = nullptr; tile_of = tile_flat = nullptr;
+        seg = nullptr; tmp = nullptr;
     };
     auto oom = [&]() {
         cleanup();
@@ -818,8 +1270,190 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
                        wscan, n_words, p->comp2pix);
     PLAN_STEP(hipGetLastError());
     const int32_t n_vis = (int32_t)std::max<int64_t>(1, p->n_vis);
+    if (g_plan_layout == 1) {
+        // ---- tiled layout ----
+        p->layout = 1;
+        int64_t gtx = 1;
+        std::vector<int32_t> h_tile_flat;
+        if (k == 2) {
+            const int64_t gty = (ny + (1 << kTileBits) - 1) >> kTileBits;
+            gtx = (nx + (1 << kTileBits) - 1) >> kTileBits;
+            PLAN_ALLOC(used, gty * gtx);
+            PLAN_STEP(hipMemsetAsync(used, 0, gty * gtx, s));
+            hipLaunchKernelGGL(k_tp_tmark, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, n, ny,
+                               nx, gtx, used);
+            PLAN_STEP(hipGetLastError());
+            std::vector<uint8_t> h_used(gty * gtx);
+            PLAN_STEP(hipMemcpyAsync(h_used.data(), used, gty * gtx, hipMemcpyDeviceToHost, s));
+            PLAN_STEP(hipStreamSynchronize(s));
+            std::vector<int32_t> h_tile_of(gty * gtx, -1);
+            for (int64_t t = 0; t < gty * gtx; t++)
+                if (h_used[t]) {
+                    h_tile_of[t] = (int32_t)h_tile_flat.size();
+                    h_tile_flat.push_back((int32_t)t);
+                }
+            PLAN_ALLOC(tile_of, gty * gtx * 4);
+            PLAN_STEP(hipMemcpyAsync(tile_of, h_tile_of.data(), gty * gtx * 4,
+                                     hipMemcpyHostToDevice, s));
+            p->n_tiles = (int64_t)h_tile_flat.size();
+            PLAN_ALLOC(tile_flat, std::max<int64_t>(1, p->n_tiles) * 4);
+            PLAN_STEP(hipMemsetAsync(tile_flat, 0xff, 4, s));
+            if (p->n_tiles > 0)
+                PLAN_STEP(hipMemcpyAsync(tile_flat, h_tile_flat.data(), p->n_tiles * 4,
+                                         hipMemcpyHostToDevice, s));
+        } else {
+            p->n_tiles = (p->n_vis + kTS - 1) / kTS;
+        }
+        const int64_t n_tiles = std::max<int64_t>(1, p->n_tiles);
+        if (t_block_mb < 0) t_block_mb = 0;   // one time block: stretches are contiguous
+        int64_t t_block = std::max<int64_t>(1, (t_block_mb << 20) / (4 * n_det));
+        if (t_block_mb <= 0 || t_block > n_t) t_block = n_t;
+        int64_t n_tb = (n_t + t_block - 1) / t_block;
+        while (n_tb * n_tiles >= (int64_t)1 << 24) {  // keys and the segment table stay small
+            t_block *= 2;
+            n_tb = (n_t + t_block - 1) / t_block;
+        }
+        p->t_block = t_block;
+        const int32_t off_key = (int32_t)(n_tb * n_tiles);
+        int end_bit = 1;
+        while ((1LL << end_bit) <= off_key) end_bit++;
+        // sort by (time block, tile); stable, so sample order within a key
+        PLAN_ALLOC(keys_in, n * 4);
+        PLAN_ALLOC(vals_in, n * 4);
+        PLAN_ALLOC(keys_out, n * 4);
+        PLAN_ALLOC(p->idx, n * 4);
+        PLAN_ALLOC(d_on, sizeof(unsigned long long));
+        hipLaunchKernelGGL(k_tp_keys, dim3(grid_of(n, kPlanBlock, INT32_MAX)), dim3(kPlanBlock),
+                           0, s, pixel, k, n, ny, nx, gtx, tile_of, bitmap, wscan, n_t, t_block,
+                           (int32_t)n_tiles, off_key, keys_in, vals_in);
+        PLAN_STEP(hipGetLastError());
+        PLAN_STEP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, vals_in,
+                                            p->idx, (size_t)n, 0, end_bit, s));
+        PLAN_ALLOC(tmp, tmp_bytes);
+        PLAN_STEP(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, p->idx,
+                                            (size_t)n, 0, end_bit, s));
+        hipLaunchKernelGGL(k_plan_count, dim3(1), dim3(1), 0, s, keys_out, n, off_key, d_on);
+        PLAN_STEP(hipGetLastError());
+        unsigned long long n_on = 0;
+        PLAN_STEP(hipMemcpyAsync(&n_on, d_on, sizeof(n_on), hipMemcpyDeviceToHost, s));
+        PLAN_STEP(hipStreamSynchronize(s));
+        p->n_on = (int64_t)n_on;
+        (void)hipFree(keys_in); keys_in = nullptr;
+        (void)hipFree(vals_in); vals_in = nullptr;
+        (void)hipFree(tmp); tmp = nullptr;
+        // T factor constant per detector?
+        p->tconst = 0;
+        if (nc != 2) {
+            PLAN_ALLOC(bad, n_det * sizeof(int));
+            PLAN_STEP(hipMemsetAsync(bad, 0, n_det * sizeof(int), s));
+            hipLaunchKernelGGL(k_plan_tcheck, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, spin, nc,
+                               n_t, n, bad);
+            PLAN_STEP(hipGetLastError());
+            std::vector<int> hbad(n_det);
+            PLAN_STEP(hipMemcpyAsync(hbad.data(), bad, n_det * sizeof(int),
+                                     hipMemcpyDeviceToHost, s));
+            PLAN_STEP(hipStreamSynchronize(s));
+            p->tconst = std::all_of(hbad.begin(), hbad.end(), [](int b) { return b == 0; });
+        }
+        const int c0 = p->tconst ? 1 : 0;
+        p->ns = nc - c0;
+        if (p->tconst) {
+            PLAN_ALLOC(p->tval, n_det * sizeof(float));
+            hipLaunchKernelGGL(k_plan_tval, dim3(grid_of(n_det)), dim3(kPlanBlock), 0, s, spin,
+                               nc, n_t, n_det, p->tval);
+            PLAN_STEP(hipGetLastError());
+        }
+        // slots and stored factors per sorted position
+        PLAN_ALLOC(p->tslot, p->n_on * 2);
+        PLAN_ALLOC(p->sspin, p->n_on * p->ns * sizeof(float));
+        if (p->n_on > 0) {
+            hipLaunchKernelGGL(k_tp_fill, dim3(grid_of(p->n_on)), dim3(kPlanBlock), 0, s, pixel,
+                               k, spin, nc, c0, p->idx, p->n_on, ny, nx, gtx, tile_of, bitmap,
+                               wscan, p->tslot, p->sspin);
+            PLAN_STEP(hipGetLastError());
+        }
+        // stretches
+        const int64_t n_chunks = (p->n_on + 63) / 64;
+        PLAN_ALLOC(p->tmask, n_chunks * 8);
+        PLAN_ALLOC(p->tfirst, n_chunks * 4);
+        PLAN_ALLOC(p->thead, n_chunks * 4);
+        PLAN_ALLOC(gcount, n_chunks * 4);
+        p->n_heads = 0;
+        if (n_chunks > 0) {
+            hipLaunchKernelGGL(k_tp_chunks, dim3(grid_of(n_chunks * 64, kPlanBlock, INT32_MAX)),
+                               dim3(kPlanBlock), 0, s, keys_out, p->idx, p->n_on, n_chunks,
+                               p->tmask, gcount, p->tfirst);
+            PLAN_STEP(hipGetLastError());
+            PLAN_STEP(rocprim::exclusive_scan(nullptr, tmp_bytes, gcount, p->thead, 0u,
+                                              (size_t)n_chunks, rocprim::plus<uint32_t>(), s));
+            PLAN_ALLOC(tmp, tmp_bytes);
+            PLAN_STEP(rocprim::exclusive_scan(tmp, tmp_bytes, gcount, p->thead, 0u,
+                                              (size_t)n_chunks, rocprim::plus<uint32_t>(), s));
+            uint32_t lb = 0, lc = 0;
+            PLAN_STEP(hipMemcpyAsync(&lb, p->thead + n_chunks - 1, 4, hipMemcpyDeviceToHost, s));
+            PLAN_STEP(hipMemcpyAsync(&lc, gcount + n_chunks - 1, 4, hipMemcpyDeviceToHost, s));
+            PLAN_STEP(hipStreamSynchronize(s));
+            p->n_heads = (int64_t)lb + lc;
+            (void)hipFree(tmp); tmp = nullptr;
+        }
+        PLAN_ALLOC(p->head_i, p->n_heads * 4);
+        if (p->n_on > 0) {
+            hipLaunchKernelGGL(k_tp_heads, dim3(grid_of(p->n_on)), dim3(kPlanBlock), 0, s,
+                               p->tmask, p->thead, p->idx, p->n_on, p->head_i);
+            PLAN_STEP(hipGetLastError());
+        }
+        // work items: each (time block, tile) segment cut into pieces
+        PLAN_ALLOC(seg, ((int64_t)off_key + 1) * 8);
+        hipLaunchKernelGGL(k_tp_segs, dim3(grid_of((int64_t)off_key + 1, kPlanBlock, INT32_MAX)),
+                           dim3(kPlanBlock), 0, s, keys_out, p->n_on, off_key, seg);
+        PLAN_STEP(hipGetLastError());
+        std::vector<int64_t> h_seg((size_t)off_key + 1);
+        PLAN_STEP(hipMemcpyAsync(h_seg.data(), seg, h_seg.size() * 8, hipMemcpyDeviceToHost, s));
+        PLAN_STEP(hipStreamSynchronize(s));
+        std::vector<TItem> h_items;
+        const int64_t per = std::max<int64_t>(64, g_plan_tile_item);
+        for (int64_t v = 0; v < off_key; v++)
+            for (int64_t b0 = h_seg[v]; b0 < h_seg[v + 1]; b0 += per)
+                h_items.push_back(TItem{b0, (int32_t)std::min<int64_t>(per, h_seg[v + 1] - b0),
+                                        (int32_t)(v % n_tiles)});
+        p->n_items = (int64_t)h_items.size();
+        PLAN_ALLOC(p->items, std::max<int64_t>(1, p->n_items) * (int64_t)sizeof(TItem));
+        if (p->n_items > 0)
+            PLAN_STEP(hipMemcpyAsync(p->items, h_items.data(), p->n_items * sizeof(TItem),
+                                     hipMemcpyHostToDevice, s));
+        // flat pixel of every slot
+        PLAN_ALLOC(p->tile_pix, n_tiles * kTS * 4);
+        hipLaunchKernelGGL(k_tp_tilepix, dim3(grid_of(n_tiles * kTS)), dim3(kPlanBlock), 0, s, k,
+                           ny, nx, gtx, tile_flat, p->comp2pix, p->n_vis, n_tiles * kTS,
+                           p->tile_pix);
+        PLAN_STEP(hipGetLastError());
+        // samples off the map (zeroed by a fresh from_map)
+        p->n_off = n - p->n_on;
+        PLAN_ALLOC(p->off_idx, p->n_off * 4);
+        if (p->n_off > 0)
+            PLAN_STEP(hipMemcpyAsync(p->off_idx, p->idx + p->n_on, p->n_off * 4,
+                                     hipMemcpyDeviceToDevice, s));
+        if (g_plan_time_copy) {
+            PLAN_ALLOC(p->tpix, n * sizeof(int32_t));
+            PLAN_ALLOC(p->tspin, n * p->ns * sizeof(float));
+            hipLaunchKernelGGL(k_plan_tcopy, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, k,
+                               spin, nc, c0, n, ny, nx, bitmap, wscan, p->tpix, p->tspin);
+            PLAN_STEP(hipGetLastError());
+            PLAN_ALLOC(p->packed, std::max<int64_t>(1, p->n_vis) * 4 * sizeof(double));
+        }
+        PLAN_STEP(hipStreamSynchronize(s));
+        (void)hipFree(p->idx); p->idx = nullptr;
+        cleanup();
+        p->bytes = p->n_on * (2 + 4 * p->ns) + n_chunks * 16 + p->n_heads * 4 +
+                   p->n_items * (int64_t)sizeof(TItem) + n_tiles * kTS * 4 + p->n_off * 4 +
+                   p->n_vis * 4 + (p->tconst ? n_det * 4 : 0) +
+                   (p->tpix != nullptr ? n * (4 + 4 * p->ns) + p->n_vis * 32 : 0);
+        *out = p;
+        return SO3G_OK;
+    }
     // Time blocks whose signal slice (n_det x t_block float32) stays resident
     // in the Infinity Cache while the block's samples are gathered.
+    if (t_block_mb < 0) t_block_mb = 64;
     int64_t t_block = std::max<int64_t>(1, (t_block_mb << 20) / (4 * n_det));
     if (t_block_mb <= 0 || t_block > n_t) t_block = n_t;
     int64_t n_tb = (n_t + t_block - 1) / t_block;
@@ -945,6 +1579,33 @@ int so3g_pc_plan_from_map(const so3g_pc_plan *p, const double *map, float *signa
     hipStream_t s = (hipStream_t)stream;
     const int64_t plane = p->ny * p->nx;
     const int64_t n = p->n_det * p->n_t;
+    if (p->layout == 1 && p->tpix == nullptr) {
+        if (!accumulate && p->n_off > 0) {
+            hipLaunchKernelGGL(k_tp_zero, dim3(grid_of(p->n_off)), dim3(kPlanBlock), 0, s,
+                               p->off_idx, p->n_off, signal);
+            PLAN_TRY(hipGetLastError());
+        }
+        if (p->n_items == 0) return SO3G_OK;
+        const dim3 gi((unsigned)p->n_items);
+        const TItem *items = (const TItem *)p->items;
+        const FastDiv nt{(uint32_t)p->n_t, 1.f / (float)p->n_t};
+        const uint32_t pl = (uint32_t)plane;
+#define TP_FROM(NC, TC)                                                                     \
+    hipLaunchKernelGGL((k_tp_from_map<NC, TC>), gi, dim3(kTBlock), 0, s, items, p->tslot,    \
+                       p->sspin, p->tmask, p->tfirst, p->thead, p->head_i, p->tile_pix,     \
+                       p->tval, nt, pl, map, signal, accumulate)
+        if (p->tconst) {
+            if (p->nc == 1) TP_FROM(1, true);
+            else TP_FROM(3, true);
+        } else {
+            if (p->nc == 1) TP_FROM(1, false);
+            else if (p->nc == 2) TP_FROM(2, false);
+            else TP_FROM(3, false);
+        }
+#undef TP_FROM
+        PLAN_TRY(hipGetLastError());
+        return SO3G_OK;
+    }
     switch (p->nc) {
     case 1: hipLaunchKernelGGL(k_plan_pack<1>, dim3(grid_of(p->n_vis)), dim3(kPlanBlock), 0, s, map, plane, p->comp2pix, p->n_vis, p->packed); break;
     case 2: hipLaunchKernelGGL(k_plan_pack<2>, dim3(grid_of(p->n_vis)), dim3(kPlanBlock), 0, s, map, plane, p->comp2pix, p->n_vis, p->packed); break;
@@ -1029,6 +1690,9 @@ int so3g_pc_plan_stats(const so3g_pc_plan *p, int64_t *stats) {
     stats[4] = p->t_block;
     stats[5] = p->bytes;
     stats[6] = p->n_vis;
+    stats[7] = p->layout;
+    stats[8] = p->n_heads;
+    stats[9] = p->n_items;
     return SO3G_OK;
 }
 

@@ -355,6 +355,33 @@ def test_scatter_variants(gpu, agg, xcd, lds):
             _native.set_option(n, v)
 
 
+def tiled_plan_counts(pix, t_block, n_tiles_of, item):
+    """Stretches and work items of the tiled plan layout, from the pixel
+    array: samples ordered by (time block, tile) keeping sample order;
+    a stretch is a run of consecutive samples with one key."""
+    n_det, n_t = pix.shape[:2]
+    on = (pix[..., 0] >= 0).ravel()
+    i = np.arange(n_det * n_t)[on]
+    tile, n_tiles = n_tiles_of(pix.reshape(n_det * n_t, -1)[on])
+    key = (i % n_t) // t_block * n_tiles + tile
+    o = np.argsort(key, kind='stable')
+    key, i = key[o], i[o]
+    heads = 1 + int(np.count_nonzero((key[1:] != key[:-1]) | (i[1:] != i[:-1] + 1)))
+    _, counts = np.unique(key, return_counts=True)
+    items = int(np.sum((counts + item - 1) // item))
+    return heads, items
+
+
+def flat_tiles(p, nx=300):
+    """Compact 64 x 64 tile of (iy, ix) pixels, tiles numbered in flat
+    order among the tiles visited."""
+    tf = (p[:, 0] // 64) * ((nx + 63) // 64) + p[:, 1] // 64
+    u = np.unique(tf)
+    return np.searchsorted(u, tf), len(u)
+
+
+@pytest.mark.parametrize('layout,tcopy', [(0, 1), (1, 1), (1, 0)],
+                         ids=['runs', 'tiles', 'tiles-notcopy'])
 @pytest.mark.parametrize('n_det,n_samp,comps,tconst,t_block_mb', [
     (30, 7000, 'TQU', True, 64),      # one time block, T factor per detector
     (201, 3001, 'TQU', True, 1),      # several time blocks, odd sample count
@@ -362,29 +389,44 @@ def test_scatter_variants(gpu, agg, xcd, lds):
     (40, 2000, 'TQU', False, 64),     # T factor varies: all factors stored
     (40, 2000, 'T', True, 64),        # T only: nothing stored per sample
     (40, 2000, 'QU', False, 1)])
-def test_precomp_plan(gpu, n_det, n_samp, comps, tconst, t_block_mb):
+def test_precomp_plan(gpu, n_det, n_samp, comps, tconst, t_block_mb, layout, tcopy):
     """PrecompPlan.to_map == Precomp to_map over all samples (oracle),
-    PrecompPlan.from_map == Precomp from_map bit for bit."""
+    PrecompPlan.from_map == Precomp from_map bit for bit, for the
+    pixel-sorted (0) and tiled (1) layouts, the latter with and without the
+    time-ordered copy for from_map."""
     import so3g_amd
+    from so3g_amd import _native
     c = Case('CAR', comps=comps, n_det=n_det, n_samp=n_samp, ny=100, nx=300, clip=True)
     pix, spin = [np.array(x) for x in c.engine.pointing_matrix(c.bore, c.dets, c.resps)]
     nc = spin.shape[2]
     if not tconst and comps != 'QU':
         spin[..., 0] *= np.linspace(0.5, 1.5, n_samp, dtype=np.float32)
-    plan = so3g_amd.PrecompPlan(pix, spin, (100, 300), t_block_mb=t_block_mb)
+    # small work items on one case: segments cut into many items
+    item = 1000 if (layout == 1 and n_det == 201) else _native.get_option('plan_tile_item')
+    old_item = _native.get_option('plan_tile_item')
+    _native.set_option('plan_tile_item', item)
+    try:
+        plan = so3g_amd.PrecompPlan(pix, spin, (100, 300), t_block_mb=t_block_mb, layout=layout,
+                                    time_copy=tcopy)
+    finally:
+        _native.set_option('plan_tile_item', old_item)
     info = plan.info()
+    assert info['layout'] == layout
     on = pix[..., 0] >= 0
     assert info['n_on'] == int(on.sum())
     assert info['t_const'] == (tconst and comps != 'QU')
     assert info['stored_spin'] == nc - int(info['t_const'])
-    # runs = distinct (time block, pixel) among on-map samples
-    tb = (np.arange(n_samp) // info['t_block'])[None, :].repeat(n_det, 0)
-    keys = (tb * 30000 + pix[..., 0] * 300 + pix[..., 1])[on]
-    assert info['runs'] == len(np.unique(keys))
+    if layout == 0:
+        # runs = distinct (time block, pixel) among on-map samples
+        tb = (np.arange(n_samp) // info['t_block'])[None, :].repeat(n_det, 0)
+        keys = (tb * 30000 + pix[..., 0] * 300 + pix[..., 1])[on]
+        assert info['runs'] == len(np.unique(keys))
+    else:
+        assert (info['stretches'], info['items']) == tiled_plan_counts(
+            pix, info['t_block'], flat_tiles, item)
     assert info['n_vis'] == len(np.unique((pix[..., 0] * 300 + pix[..., 1])[on]))
     dw = c.det_weights.copy()
     dw[3] = 0.
-    from so3g_amd import _native
     names = ('plan_to_lds', 'plan_to_cpb')
     old = [_native.get_option(n) for n in names]
     try:
@@ -429,8 +471,11 @@ def test_precomp_plan(gpu, n_det, n_samp, comps, tconst, t_block_mb):
     maps_close(mf.cpu().numpy(), r, 'plan to_map f64', atol=1e-5)
 
 
-@pytest.mark.parametrize('nest,nside,t_block_mb', [(True, 512, 64), (False, 4096, 1)])
-def test_precomp_plan_healpix(gpu, nest, nside, t_block_mb):
+@pytest.mark.parametrize('layout,tcopy', [(0, 1), (1, 1), (1, 0)],
+                         ids=['runs', 'tiles', 'tiles-notcopy'])
+@pytest.mark.parametrize('nest,nside,t_block_mb', [(True, 512, 64), (False, 4096, 1),
+                                                   (True, 4096, None)])
+def test_precomp_plan_healpix(gpu, nest, nside, t_block_mb, layout, tcopy):
     """PrecompPlan on a non-tiled HEALPix map (k = 1 pixel numbers, map
     (nc, npix)): visited pixels numbered compactly, to_map == Precomp to_map
     (oracle), from_map == Precomp from_map bit for bit, including samples
@@ -442,13 +487,23 @@ def test_precomp_plan_healpix(gpu, nest, nside, t_block_mb):
     pix[2, 100:400, 0] = -1
     pix[5, ::7, 0] = -1
     npix = 12 * nside * nside
-    plan = so3g_amd.PrecompPlan(pix, spin, (npix,), t_block_mb=t_block_mb)
+    plan = so3g_amd.PrecompPlan(pix, spin, (npix,), t_block_mb=t_block_mb, layout=layout,
+                                time_copy=tcopy)
     info = plan.info()
     on = pix[..., 0] >= 0
     assert info['n_on'] == int(on.sum())
-    assert info['n_vis'] == len(np.unique(pix[..., 0][on]))
-    tb = (np.arange(c.signal.shape[1]) // info['t_block'])[None, :].repeat(24, 0)
-    assert info['runs'] == len(np.unique((tb.astype(np.int64) * npix + pix[..., 0])[on]))
+    vis = np.unique(pix[..., 0][on])
+    assert info['n_vis'] == len(vis)
+    if layout == 0:
+        tb = (np.arange(c.signal.shape[1]) // info['t_block'])[None, :].repeat(24, 0)
+        assert info['runs'] == len(np.unique((tb.astype(np.int64) * npix + pix[..., 0])[on]))
+    else:
+        # tiles: blocks of 4096 visited pixels in ascending pixel order
+        from so3g_amd import _native
+        assert (info['stretches'], info['items']) == tiled_plan_counts(
+            pix, info['t_block'],
+            lambda p: (np.searchsorted(vis, p[:, 0]) // 4096, (len(vis) + 4095) // 4096),
+            _native.get_option('plan_tile_item'))
     dw = c.det_weights.copy()
     dw[1] = 0.
     for w in (None, dw):
