@@ -57,6 +57,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def plan_bytes(pinfo, n_s):
+    """Algorithmic bytes of the planned from_map and to_map kernels (as
+    described in main) and the to_map kernel's name."""
+    n_on, ns, n_vis = pinfo['n_on'], pinfo['stored_spin'], pinfo['n_vis']
+    b_from = n_s * (4 + 4 * ns + 4) + 8 * 3 * n_vis
+    if pinfo['layout'] == 1:
+        b_to = n_on * (2 + 4 * ns + 4) + 16 * ((n_on + 63) // 64) + 4 * pinfo['stretches'] + \
+            pinfo['items'] * (16 + 4 * 4096) + 2 * 8 * 3 * n_vis
+        return b_from, b_to, 'k_tp_to_map<3,true,float,true>'
+    b_to = n_on * (4 + 4 * ns + 4) + n_on // 8 + 4 * pinfo['runs'] + 4 * (n_on // 512) + \
+        2 * 8 * 3 * n_vis
+    return b_from, b_to, 'k_plan_to_map_c<3,true,float,1,true>'
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -226,25 +240,29 @@ def main():
     #   from_map: per sample 4 B compact pixel + 8 B (Q, U) factors + 4 B
     #             signal written (the T factor is one float per detector), plus
     #             the float64 TQU values of the visited pixels read once;
-    #   to_map:   per on-map sample 4 B sample index + 8 B (Q, U) factors +
-    #             4 B signal gathered + 1 bit run start, plus 4 B per pixel run
-    #             and per 512-sample group, plus the visited pixels' float64
-    #             TQU values read and written once.
+    #   to_map (tiled layout): per on-map sample 2 B tile slot + 8 B (Q, U)
+    #             factors + 4 B signal, plus 16 B per 64 positions (stretch
+    #             mask, first sample, start count), 4 B per stretch and 16 B
+    #             per work item, the 16 KB slot-to-pixel table of each item's
+    #             tile, and the visited pixels' float64 TQU values read and
+    #             written once;
+    #   to_map (pixel-run layout): per on-map sample 4 B sample index + 8 B
+    #             factors + 4 B signal + 1 bit run start, plus 4 B per pixel run
+    #             and per 512-sample group, plus the visited pixels' values
+    #             read and written once.
     # The ProjEng_Precomp interface itself streams 24 B per sample (int32
     # pixel pair, 3 float32 factors, float32 signal): `precomp_equivalent`
     # reports the rates in those bytes.
     npix = ny * nx
     pinfo = plan.info()
     n_on, n_runs, ns, n_vis = pinfo['n_on'], pinfo['runs'], pinfo['stored_spin'], pinfo['n_vis']
-    b_from = N_s * (4 + 4 * ns + 4) + 8 * 3 * n_vis
-    b_to = n_on * (4 + 4 * ns + 4) + n_on // 8 + 4 * n_runs + 4 * (n_on // 512) + \
-        2 * 8 * 3 * n_vis
+    b_from, b_to, to_kernel = plan_bytes(pinfo, N_s)
     b_pc = N_s * (4 * 2 + 4 * 3 + 4)
     gbs_from = b_from / (ms_from * 1e-3)
     gbs_to = b_to / (ms_to * 1e-3)
     tr = measured_traffic() or {}
     if ms_to >= ms_from:
-        dom = ('k_plan_to_map_c<3,true,float,1,false,true>', gbs_to, b_to, tr.get('to_map_bytes'))
+        dom = (to_kernel, gbs_to, b_to, tr.get('to_map_bytes'))
     else:
         dom = ('k_plan_from_map_s<3,true,4,1>', gbs_from, b_from, tr.get('from_map_bytes'))
     # The plain ProjEng_Precomp kernels (no plan), for reference.
@@ -304,7 +322,9 @@ def main():
         'ms_from_map_without_plan': ms_from_direct,
         'ms_to_map_without_plan': ms_to_direct,
         'plan': {'n_on': n_on, 'runs': n_runs, 'stored_spin_per_sample': ns,
-                 't_block': pinfo['t_block'], 'bytes': pinfo['bytes']},
+                 't_block': pinfo['t_block'], 'bytes': pinfo['bytes'], 'visited_pixels': n_vis,
+                 'layout': pinfo['layout'], 'stretches': pinfo['stretches'],
+                 'items': pinfo['items']},
         'roofline': {
             'bound': 'hbm', 'kernel': dom[0], 'achieved': dom[1] / 1e9,
             'peak': HBM_PEAK / 1e9, 'unit': 'GB/s', 'frac': dom[1] / HBM_PEAK,
@@ -406,7 +426,8 @@ def small_config(torch, so3g_amd, N, workloads, dev, stream, reps=20):
     n_s = n_det * n_t
     npix = ny * nx
     info = plan.info()
-    b = n_s * 16 + info['n_on'] * 16 + info['n_on'] // 8 + 4 * info['runs'] + 3 * 8 * npix * 3
+    bf, bt, _ = plan_bytes(info, n_s)
+    b = bf + bt
     b_pc = n_s * 24 * 2 + 3 * 8 * npix * 3
     out = {'workload': f'{n_det} det x {n_t} samples, CAR 1 arcmin TQU {ny}x{nx}, planned '
                        'from_map + to_map',

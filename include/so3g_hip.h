@@ -216,7 +216,8 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape,
  * signal slice is about t_block_mb MiB (0: one block; < 0: the layout's
  * default), then sorted within each block: by map tile (64 x 64 pixels, or
  * 4096 visited HEALPix pixels), keeping sample order, in the tiled layout
- * (option "plan_layout" 1, default; one time block); by pixel in layout 0
+ * (option "plan_layout" 1, the default from 2^25 samples up; one time
+ * block); by pixel in layout 0
  * (64 MiB blocks, sized to stay in the Infinity Cache).  Stored: per on-map
  * sample the spin factors (the T factor once per detector when it is
  * constant over the detector, as in so3g's pointing matrices) and the tile
@@ -440,8 +441,9 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   tables (T factor, weights) in LDS when they fit in 32 KB
  *   "plan_to_cpb"   2048-sample chunks per block of the planned scatter
  *                   (default 2, 1..64)
- *   "plan_layout"   layout of plans created afterwards: 1 (default) map
- *                   tiles summed in LDS, 0 pixel runs
+ *   "plan_layout"   layout of plans created afterwards: 1 map tiles summed
+ *                   in LDS, 0 pixel runs, -1 (default) tiles from 2^25
+ *                   samples up
  *   "plan_tile_item"  sorted positions per work item of the tiled layout
  *                   (default 262144, 64..2^30)
  *   "plan_time_copy"  1 (default): tiled plans also keep the time-ordered

@@ -745,7 +745,8 @@ int so3g_set_option(const char *name, int64_t value) {
     else if (n == "plan_from_layout") so3g::g_plan_from_layout = (int)(value != 0);
     else if (n == "plan_to_lds") so3g::g_plan_to_lds = (int)(value != 0);
     else if (n == "plan_layout") {
-        if (value < 0 || value > 1) return fail(SO3G_ERR_VALUE, "plan_layout must be 0 or 1");
+        if (value < -1 || value > 1)
+            return fail(SO3G_ERR_VALUE, "plan_layout must be -1 (auto), 0 or 1");
         so3g::g_plan_layout = (int)value;
     }
     else if (n == "plan_time_copy") so3g::g_plan_time_copy = (int)(value != 0);

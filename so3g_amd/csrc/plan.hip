@@ -25,7 +25,8 @@
 // popcount over the head bits.  from_map reads a time-ordered copy: compact
 // pixel index (4 B, -1 off the map) and the ns stored spin factors.
 //
-// Tiled layout (the default, "plan_layout" 1): the samples are sorted by
+// Tiled layout ("plan_layout" 1, the default from 2^25 samples up): the
+// samples are sorted by
 // (time block, map tile) instead -- 64 x 64 pixels of a flat map, or 4096
 // consecutive visited pixels of a HEALPix map -- and keep their sample order
 // inside a tile, so a detector's pass over a tile is a stretch of
@@ -122,8 +123,12 @@ int g_plan_from_layout = 1;
 int g_plan_to_lds = 1;
 int g_plan_to_cpb = 2;
 // Plan layout chosen at creation ("plan_layout"): 0 pixel-sorted runs, 1
-// map tiles in LDS; and the tiled layout's positions per work item.
-int g_plan_layout = 1;
+// map tiles in LDS, -1 (default) tiles from 2^25 samples up (below that a
+// plan has too few work items to fill the GPU: 56.8 vs 48.6 us for the
+// to_map + from_map pair of 4e6 samples); and the tiled layout's positions
+// per work item.
+int g_plan_layout = -1;
+constexpr int64_t kPlanTiledMin = (int64_t)1 << 25;
 int64_t g_plan_tile_item = 262144;
 // Tiled layout: also keep the time-ordered copy for from_map ("plan_time_copy",
 // 1: 2.15 ms gather vs 3.51 ms over the tiles; 0: a smaller plan).
@@ -706,8 +711,15 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
 // 64] is the sample index of the chunk's first position, thead[j / 64] counts
 // the stretch starts before the chunk and head_i lists each start's sample.
 // ---------------------------------------------------------------------------
-constexpr int kTileBits = 6;                   // 64 x 64 tiles of a flat map
-constexpr int kTS = 1 << (2 * kTileBits);      // slots per tile
+#ifndef SO3G_TILE_BITS_Y
+#define SO3G_TILE_BITS_Y 6
+#endif
+#ifndef SO3G_TILE_BITS_X
+#define SO3G_TILE_BITS_X 6
+#endif
+constexpr int kTileBY = SO3G_TILE_BITS_Y;       // 64 x 64 tiles of a flat map
+constexpr int kTileBX = SO3G_TILE_BITS_X;
+constexpr int kTS = 1 << (kTileBY + kTileBX);  // slots per tile
 constexpr int kTBlock = 1024;                  // threads per work item
 
 struct TItem {
@@ -733,7 +745,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_tmark(const int32_t *pix, int
          i += (int64_t)gridDim.x * kPlanBlock) {
         const int2 q = *reinterpret_cast<const int2 *>(pix + 2 * i);
         if (q.x >= 0 && q.x < ny && q.y >= 0 && q.y < nx)
-            used[(q.x >> kTileBits) * gtx + (q.y >> kTileBits)] = 1;
+            used[(q.x >> kTileBY) * gtx + (q.y >> kTileBX)] = 1;
     }
 }
 
@@ -748,14 +760,14 @@ __device__ __forceinline__ void tile_slot(const int32_t *pix, int k, int64_t i, 
         const int32_t q = pix[i];
         if (q < 0 || q >= ny) { tile = -1; slot = 0; return; }
         const uint32_t c = compact_of(q, bitmap, wscan);
-        tile = (int32_t)(c >> (2 * kTileBits));
+        tile = (int32_t)(c >> (kTileBY + kTileBX));
         slot = (int32_t)(c & (kTS - 1));
         return;
     }
     const int2 q = *reinterpret_cast<const int2 *>(pix + 2 * i);
     if (q.x < 0 || q.x >= ny || q.y < 0 || q.y >= nx) { tile = -1; slot = 0; return; }
-    tile = tile_of[(q.x >> kTileBits) * gtx + (q.y >> kTileBits)];
-    slot = ((q.x & ((1 << kTileBits) - 1)) << kTileBits) | (q.y & ((1 << kTileBits) - 1));
+    tile = tile_of[(q.x >> kTileBY) * gtx + (q.y >> kTileBX)];
+    slot = ((q.x & ((1 << kTileBY) - 1)) << kTileBX) | (q.y & ((1 << kTileBX) - 1));
 }
 
 __global__ __launch_bounds__(kPlanBlock) void k_tp_keys(const int32_t *pix, int k, int64_t n,
@@ -855,11 +867,11 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_tilepix(int k, int64_t ny, in
         if (k == 1) {
             tile_pix[q] = q < n_vis ? comp2pix[q] : -1;
         } else {
-            const int64_t tf = tile_flat[q >> (2 * kTileBits)];
+            const int64_t tf = tile_flat[q >> (kTileBY + kTileBX)];
             if (tf < 0) { tile_pix[q] = -1; continue; }
             const int sl = (int)(q & (kTS - 1));
-            const int64_t iy = (tf / gtx) * (1 << kTileBits) + (sl >> kTileBits);
-            const int64_t ix = (tf % gtx) * (1 << kTileBits) + (sl & ((1 << kTileBits) - 1));
+            const int64_t iy = (tf / gtx) * (1 << kTileBY) + (sl >> kTileBX);
+            const int64_t ix = (tf % gtx) * (1 << kTileBX) + (sl & ((1 << kTileBX) - 1));
             tile_pix[q] = (iy < ny && ix < nx) ? (int32_t)(iy * nx + ix) : -1;
         }
     }
@@ -872,7 +884,10 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_tilepix(int k, int64_t ny, in
 // before using any (chunk tables, then stretch starts, slots and factors,
 // then the signal), so the dependent loads overlap.  LDT: per-detector
 // tables in LDS.
-constexpr int kTU = 8;
+#ifndef SO3G_TILE_U
+#define SO3G_TILE_U 8
+#endif
+constexpr int kTU = SO3G_TILE_U;
 
 template <int NC, bool TC, typename ST, bool LDT>
 __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
@@ -1270,14 +1285,14 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
                        wscan, n_words, p->comp2pix);
     PLAN_STEP(hipGetLastError());
     const int32_t n_vis = (int32_t)std::max<int64_t>(1, p->n_vis);
-    if (g_plan_layout == 1) {
+    if (g_plan_layout == 1 || (g_plan_layout < 0 && n >= kPlanTiledMin)) {
         // ---- tiled layout ----
         p->layout = 1;
         int64_t gtx = 1;
         std::vector<int32_t> h_tile_flat;
         if (k == 2) {
-            const int64_t gty = (ny + (1 << kTileBits) - 1) >> kTileBits;
-            gtx = (nx + (1 << kTileBits) - 1) >> kTileBits;
+            const int64_t gty = (ny + (1 << kTileBY) - 1) >> kTileBY;
+            gtx = (nx + (1 << kTileBX) - 1) >> kTileBX;
             PLAN_ALLOC(used, gty * gtx);
             PLAN_STEP(hipMemsetAsync(used, 0, gty * gtx, s));
             hipLaunchKernelGGL(k_tp_tmark, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, n, ny,
@@ -1411,7 +1426,10 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         PLAN_STEP(hipMemcpyAsync(h_seg.data(), seg, h_seg.size() * 8, hipMemcpyDeviceToHost, s));
         PLAN_STEP(hipStreamSynchronize(s));
         std::vector<TItem> h_items;
-        const int64_t per = std::max<int64_t>(64, g_plan_tile_item);
+        // at most plan_tile_item positions, and about 1024 items or more so
+        // that a small observation still fills the GPU
+        const int64_t per = std::max<int64_t>(
+            64, std::min<int64_t>(g_plan_tile_item, std::max<int64_t>(8192, p->n_on / 1024)));
         for (int64_t v = 0; v < off_key; v++)
             for (int64_t b0 = h_seg[v]; b0 < h_seg[v + 1]; b0 += per)
                 h_items.push_back(TItem{b0, (int32_t)std::min<int64_t>(per, h_seg[v + 1] - b0),

@@ -763,7 +763,8 @@ class PrecompPlan:
     `map_shape` = (ny, nx); or k = 1, a non-tiled HEALPix map of `map_shape`
     = (npix,)) and spin_proj (both copied into the plan).  It numbers the
     visited pixels compactly and sorts the on-map samples so that to_map
-    needs no per-sample atomics: layout 1 (the default) sorts them by map
+    needs no per-sample atomics: layout 1 (the default from 2^25 samples up;
+    layout=None chooses by size) sorts them by map
     tile, keeping sample order inside a tile, and sums each work item in the
     tile held in LDS; layout 0 sorts them by pixel and sums each pixel's run
     with a wave-level segmented reduction.  from_map reads a compact

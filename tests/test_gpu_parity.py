@@ -368,7 +368,9 @@ def tiled_plan_counts(pix, t_block, n_tiles_of, item):
     key, i = key[o], i[o]
     heads = 1 + int(np.count_nonzero((key[1:] != key[:-1]) | (i[1:] != i[:-1] + 1)))
     _, counts = np.unique(key, return_counts=True)
-    items = int(np.sum((counts + item - 1) // item))
+    # positions per item: at most `item`, about 1024 items or more
+    per = max(64, min(item, max(8192, len(i) // 1024)))
+    items = int(np.sum((counts + per - 1) // per))
     return heads, items
 
 

@@ -180,3 +180,26 @@ def test_compute_path_refuses_without_gpu():
     e = so3g_amd.ProjEng_CAR_T_NonTiled((5, 5, 1., 1., 1., 1.))
     with pytest.raises(RuntimeError, match='no CPU fallback'):
         e.pixels(np.zeros((3, 4)), np.zeros((2, 4)))
+
+
+def test_plan_options():
+    """Plan tunables documented in include/so3g_hip.h: defaults, round trip,
+    and rejection of values outside their ranges."""
+    defaults = {'plan_layout': -1, 'plan_tile_item': 262144, 'plan_time_copy': 1,
+                'plan_to_lds': 1, 'plan_to_cpb': 2, 'plan_from_layout': 1, 'plan_from_v': 4}
+    for name, v in defaults.items():
+        assert _native.get_option(name) == v, name
+    for name, good, bad in (('plan_layout', 0, 2), ('plan_layout', 1, -2),
+                            ('plan_tile_item', 4096, 63), ('plan_to_cpb', 5, 0),
+                            ('plan_from_v', 8, 5)):
+        old = _native.get_option(name)
+        try:
+            _native.set_option(name, good)
+            assert _native.get_option(name) == good
+            with pytest.raises(ValueError):
+                _native.set_option(name, bad)
+            assert _native.get_option(name) == good
+        finally:
+            _native.set_option(name, old)
+    with pytest.raises(ValueError):
+        _native.set_option('plan_no_such_option', 1)
