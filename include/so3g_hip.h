@@ -249,7 +249,8 @@ int so3g_precomp_from_map_f64(const double *map, const int64_t *map_shape,
 int so3g_pc_plan_info(const so3g_pc_plan *plan, int64_t *n_on, int64_t *bytes);
 /* stats[10]: n_on, pixel runs (layout 0), T factor constant per detector
  * (0/1), stored spin factors per sample, samples per time block, device
- * bytes, visited pixels, layout, stretches and work items (tiled layout) */
+ * bytes, visited pixels, layout, stretch starts (counting each 64-position
+ * chunk's first position) and work items (tiled layout) */
 int so3g_pc_plan_stats(const so3g_pc_plan *plan, int64_t *stats);
 int so3g_pc_plan_destroy(so3g_pc_plan *plan);
 

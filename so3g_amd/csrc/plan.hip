@@ -34,7 +34,7 @@
 // loads are nearly contiguous.  Work items of up to 256 K positions sum into
 // the tile held in LDS and add it to the map once.  Per position: the slot
 // in the tile (2 B) and the stored factors; per 64 positions: a stretch-start
-// mask, the first sample and a start count; per stretch: its first sample.
+// mask and a start count; per start (stretches and chunk starts): its sample.
 // from_map keeps the time-ordered gather (or, with "plan_time_copy" 0 and a
 // smaller plan, gathers from the tile in LDS).
 //
@@ -89,7 +89,6 @@ struct so3g_pc_plan {
     int32_t layout;        // 0: pixel-sorted runs, 1: tiles
     uint16_t *tslot;       // [n_on] slot within the tile
     uint64_t *tmask;       // [n_chunks] stretch starts per 64 positions
-    uint32_t *tfirst;      // [n_chunks] sample of each chunk's first position
     uint32_t *thead;       // [n_chunks] stretch starts before each chunk
     uint32_t *head_i;      // [n_heads] sample of each stretch start
     int32_t *tile_pix;     // [n_tiles * 4096] flat pixel of each slot, -1 off the map
@@ -706,10 +705,13 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
 //
 // Per position: the slot within the tile (u16) and the stored factors.  The
 // sample index of position j is encoded by stretches: bit j % 64 of
-// tmask[j / 64] marks a position that does not continue the previous one's
-// stretch (different tile or time block, or not the next sample), tfirst[j /
-// 64] is the sample index of the chunk's first position, thead[j / 64] counts
-// the stretch starts before the chunk and head_i lists each start's sample.
+// tmask[j / 64] marks a start -- a position that does not continue the
+// previous one's stretch (different tile or time block, or not the next
+// sample), and every chunk's first position --, thead[j / 64] counts the
+// starts before the chunk and head_i lists each start's sample.  With a
+// start at every chunk's first position a lane always finds one at or before
+// it, so the lookup has no branch and a wave's loads for several chunks
+// overlap.
 // ---------------------------------------------------------------------------
 #ifndef SO3G_TILE_BITS_Y
 #define SO3G_TILE_BITS_Y 6
@@ -728,11 +730,11 @@ struct TItem {
     int32_t tile;   // compact tile id
 };
 
-// Sample index of this lane's position in 64-position chunk c.
-__device__ __forceinline__ uint32_t tile_sample(uint64_t mask, uint32_t first, uint32_t hbase,
+// Sample index of this lane's position in a 64-position chunk (bit 0 of
+// mask is always set).
+__device__ __forceinline__ uint32_t tile_sample(uint64_t mask, uint32_t hbase,
                                                 const uint32_t *__restrict__ head_i, int lane) {
     const uint64_t h = mask & (~0ull >> (63 - lane));
-    if (h == 0) return first + (uint32_t)lane;
     const int p = 63 - __clzll((long long)h);
     return head_i[hbase + __popcll(h) - 1] + (uint32_t)(lane - p);
 }
@@ -807,24 +809,23 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_fill(const int32_t *pix, int 
     }
 }
 
-// Per 64-position chunk: stretch-start mask, their count, first sample.  One
-// wave per chunk; the grid covers n_chunks * 64 threads exactly.
+// Per 64-position chunk: stretch-start mask and start count.  One wave per
+// chunk; the grid covers n_chunks * 64 threads exactly.
 __global__ __launch_bounds__(kPlanBlock) void k_tp_chunks(const int32_t *key,
                                                           const uint32_t *idx, int64_t n_on,
                                                           int64_t n_chunks, uint64_t *mask,
-                                                          uint32_t *cnt, uint32_t *first) {
+                                                          uint32_t *cnt) {
     const int64_t j = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
     const int64_t c = j >> 6;
     if (c >= n_chunks) return;   // whole waves
     const int lane = (int)(j & 63);
     bool head = false;
     if (j < n_on)
-        head = j == 0 || key[j] != key[j - 1] || idx[j] != idx[j - 1] + 1u;
+        head = lane == 0 || key[j] != key[j - 1] || idx[j] != idx[j - 1] + 1u;
     const uint64_t m = __ballot(head);
     if (lane == 0) {
         mask[c] = m;
         cnt[c] = (uint32_t)__popcll(m);
-        first[c] = idx[j];
     }
 }
 
@@ -893,7 +894,7 @@ template <int NC, bool TC, typename ST, bool LDT>
 __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
     const TItem *__restrict__ items, const uint16_t *__restrict__ slot,
     const float *__restrict__ fac, const uint64_t *__restrict__ tmask,
-    const uint32_t *__restrict__ tfirst, const uint32_t *__restrict__ thead,
+    const uint32_t *__restrict__ thead,
     const uint32_t *__restrict__ head_i, const int32_t *__restrict__ tile_pix,
     const float *__restrict__ tval, FastDiv n_t, uint32_t plane, const ST *__restrict__ sig,
     const float *__restrict__ det_weights, int32_t n_det, double *map) {
@@ -917,11 +918,11 @@ __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
     for (int64_t cb = (it.b >> 6) + (int64_t)wave * kTU; cb < c_end;
          cb += (int64_t)(kTBlock / 64) * kTU) {
         uint64_t m[kTU];
-        uint32_t f0[kTU], hb[kTU];
+        uint32_t hb[kTU];
 #pragma unroll
         for (int u = 0; u < kTU; u++) {
             const int64_t c = min(cb + u, c_end - 1);
-            m[u] = tmask[c]; f0[u] = tfirst[c]; hb[u] = thead[c];
+            m[u] = tmask[c]; hb[u] = thead[c];
         }
         uint32_t si[kTU];
         int sl[kTU];
@@ -929,22 +930,24 @@ __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
         bool act[kTU];
 #pragma unroll
         for (int u = 0; u < kTU; u++) {
-            const int64_t j = (cb + u) * 64 + lane;
-            act[u] = j >= it.b && j < e;
-            si[u] = act[u] ? tile_sample(m[u], f0[u], hb[u], head_i, lane) : 0u;
-            sl[u] = act[u] ? slot[j] : 0;
+            // every lane loads (inactive lanes at the item's first position)
+            const int64_t j0 = (cb + u) * 64 + lane;
+            act[u] = j0 >= it.b && j0 < e;
+            const int64_t j = act[u] ? j0 : it.b;
+            const uint32_t ts = tile_sample(m[u], hb[u], head_i, lane);
+            si[u] = act[u] ? ts : 0u;
+            sl[u] = slot[j];
             if constexpr (NS == 2) {
-                const f32x2 v = act[u] ? reinterpret_cast<const f32x2 *>(fac)[j]
-                                       : f32x2{0.f, 0.f};
+                const f32x2 v = reinterpret_cast<const f32x2 *>(fac)[j];
                 sp[u][0] = v.x; sp[u][1] = v.y;
             } else {
 #pragma unroll
-                for (int q = 0; q < NS; q++) sp[u][q] = act[u] ? fac[j * NS + q] : 0.f;
+                for (int q = 0; q < NS; q++) sp[u][q] = fac[j * NS + q];
             }
         }
         ST sg[kTU];
 #pragma unroll
-        for (int u = 0; u < kTU; u++) sg[u] = act[u] ? sig[si[u]] : (ST)0;
+        for (int u = 0; u < kTU; u++) sg[u] = sig[si[u]];
 #pragma unroll
         for (int u = 0; u < kTU; u++) {
             if (!act[u]) continue;
@@ -997,7 +1000,7 @@ template <int NC, bool TC>
 __global__ __launch_bounds__(kTBlock) void k_tp_from_map(
     const TItem *__restrict__ items, const uint16_t *__restrict__ slot,
     const float *__restrict__ fac, const uint64_t *__restrict__ tmask,
-    const uint32_t *__restrict__ tfirst, const uint32_t *__restrict__ thead,
+    const uint32_t *__restrict__ thead,
     const uint32_t *__restrict__ head_i, const int32_t *__restrict__ tile_pix,
     const float *__restrict__ tval, FastDiv n_t, uint32_t plane, const double *__restrict__ map,
     float *__restrict__ sig, int accumulate) {
@@ -1019,11 +1022,11 @@ __global__ __launch_bounds__(kTBlock) void k_tp_from_map(
     for (int64_t cb = (it.b >> 6) + (int64_t)wave * kTU; cb < c_end;
          cb += (int64_t)(kTBlock / 64) * kTU) {
         uint64_t m[kTU];
-        uint32_t f0[kTU], hb[kTU];
+        uint32_t hb[kTU];
 #pragma unroll
         for (int u = 0; u < kTU; u++) {
             const int64_t c = min(cb + u, c_end - 1);
-            m[u] = tmask[c]; f0[u] = tfirst[c]; hb[u] = thead[c];
+            m[u] = tmask[c]; hb[u] = thead[c];
         }
         uint32_t si[kTU];
         int sl[kTU];
@@ -1032,21 +1035,23 @@ __global__ __launch_bounds__(kTBlock) void k_tp_from_map(
         float o[kTU];
 #pragma unroll
         for (int u = 0; u < kTU; u++) {
-            const int64_t j = (cb + u) * 64 + lane;
-            act[u] = j >= it.b && j < e;
-            si[u] = act[u] ? tile_sample(m[u], f0[u], hb[u], head_i, lane) : 0u;
-            sl[u] = act[u] ? slot[j] : 0;
+            // every lane loads (inactive lanes at the item's first position)
+            const int64_t j0 = (cb + u) * 64 + lane;
+            act[u] = j0 >= it.b && j0 < e;
+            const int64_t j = act[u] ? j0 : it.b;
+            const uint32_t ts = tile_sample(m[u], hb[u], head_i, lane);
+            si[u] = act[u] ? ts : 0u;
+            sl[u] = slot[j];
             if constexpr (NS == 2) {
-                const f32x2 v = act[u] ? reinterpret_cast<const f32x2 *>(fac)[j]
-                                       : f32x2{0.f, 0.f};
+                const f32x2 v = reinterpret_cast<const f32x2 *>(fac)[j];
                 sp[u][0] = v.x; sp[u][1] = v.y;
             } else {
 #pragma unroll
-                for (int q = 0; q < NS; q++) sp[u][q] = act[u] ? fac[j * NS + q] : 0.f;
+                for (int q = 0; q < NS; q++) sp[u][q] = fac[j * NS + q];
             }
         }
 #pragma unroll
-        for (int u = 0; u < kTU; u++) o[u] = (accumulate && act[u]) ? sig[si[u]] : 0.f;
+        for (int u = 0; u < kTU; u++) o[u] = accumulate ? sig[si[u]] : 0.f;
 #pragma unroll
         for (int u = 0; u < kTU; u++) {
             if (!act[u]) continue;
@@ -1087,7 +1092,7 @@ void free_plan_arrays(so3g_pc_plan *p) {
     (void)hipFree(p->heads); (void)hipFree(p->wbase); (void)hipFree(p->run_pix);
     (void)hipFree(p->tpix); (void)hipFree(p->tspin); (void)hipFree(p->packed);
     (void)hipFree(p->comp2pix);
-    (void)hipFree(p->tslot); (void)hipFree(p->tmask); (void)hipFree(p->tfirst);
+    (void)hipFree(p->tslot); (void)hipFree(p->tmask);
     (void)hipFree(p->thead); (void)hipFree(p->head_i); (void)hipFree(p->tile_pix);
     (void)hipFree(p->items); (void)hipFree(p->off_idx);
 }
@@ -1114,12 +1119,12 @@ int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
     do {                                                                                    \
         if (ldt)                                                                            \
             hipLaunchKernelGGL((k_tp_to_map<NC, TC, ST, true>), gi, dim3(kTBlock), lds, s,  \
-                               items, p->tslot, p->sspin, p->tmask, p->tfirst, p->thead,   \
+                               items, p->tslot, p->sspin, p->tmask, p->thead,   \
                                p->head_i, p->tile_pix, p->tval, nt, plane, signal,          \
                                det_weights, nd, map);                                       \
         else                                                                                \
             hipLaunchKernelGGL((k_tp_to_map<NC, TC, ST, false>), gi, dim3(kTBlock), 0, s,   \
-                               items, p->tslot, p->sspin, p->tmask, p->tfirst, p->thead,   \
+                               items, p->tslot, p->sspin, p->tmask, p->thead,   \
                                p->head_i, p->tile_pix, p->tval, nt, plane, signal,          \
                                det_weights, nd, map);                                       \
     } while (0)
@@ -1390,14 +1395,13 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         // stretches
         const int64_t n_chunks = (p->n_on + 63) / 64;
         PLAN_ALLOC(p->tmask, n_chunks * 8);
-        PLAN_ALLOC(p->tfirst, n_chunks * 4);
         PLAN_ALLOC(p->thead, n_chunks * 4);
         PLAN_ALLOC(gcount, n_chunks * 4);
         p->n_heads = 0;
         if (n_chunks > 0) {
             hipLaunchKernelGGL(k_tp_chunks, dim3(grid_of(n_chunks * 64, kPlanBlock, INT32_MAX)),
                                dim3(kPlanBlock), 0, s, keys_out, p->idx, p->n_on, n_chunks,
-                               p->tmask, gcount, p->tfirst);
+                               p->tmask, gcount);
             PLAN_STEP(hipGetLastError());
             PLAN_STEP(rocprim::exclusive_scan(nullptr, tmp_bytes, gcount, p->thead, 0u,
                                               (size_t)n_chunks, rocprim::plus<uint32_t>(), s));
@@ -1462,7 +1466,7 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         PLAN_STEP(hipStreamSynchronize(s));
         (void)hipFree(p->idx); p->idx = nullptr;
         cleanup();
-        p->bytes = p->n_on * (2 + 4 * p->ns) + n_chunks * 16 + p->n_heads * 4 +
+        p->bytes = p->n_on * (2 + 4 * p->ns) + n_chunks * 12 + p->n_heads * 4 +
                    p->n_items * (int64_t)sizeof(TItem) + n_tiles * kTS * 4 + p->n_off * 4 +
                    p->n_vis * 4 + (p->tconst ? n_det * 4 : 0) +
                    (p->tpix != nullptr ? n * (4 + 4 * p->ns) + p->n_vis * 32 : 0);
@@ -1610,7 +1614,7 @@ int so3g_pc_plan_from_map(const so3g_pc_plan *p, const double *map, float *signa
         const uint32_t pl = (uint32_t)plane;
 #define TP_FROM(NC, TC)                                                                     \
     hipLaunchKernelGGL((k_tp_from_map<NC, TC>), gi, dim3(kTBlock), 0, s, items, p->tslot,    \
-                       p->sspin, p->tmask, p->tfirst, p->thead, p->head_i, p->tile_pix,     \
+                       p->sspin, p->tmask, p->thead, p->head_i, p->tile_pix,     \
                        p->tval, nt, pl, map, signal, accumulate)
         if (p->tconst) {
             if (p->nc == 1) TP_FROM(1, true);

@@ -811,8 +811,9 @@ class PrecompPlan:
         t_const (T factor stored once per detector), stored_spin (factors
         stored per sample), t_block (samples per time block), bytes, n_vis
         (visited pixels), layout (0 pixel-sorted runs, 1 map tiles), and for
-        the tiled layout stretches (runs of consecutive samples within a
-        tile) and items (work items)."""
+        the tiled layout stretches (starts of runs of consecutive samples
+        within a tile, counting each 64-position chunk's first position) and
+        items (work items)."""
         a = (ctypes.c_int64 * 10)()
         N.check(N.load().so3g_pc_plan_stats(self._h, a))
         return {'n_on': a[0], 'runs': a[1], 't_const': bool(a[2]), 'stored_spin': a[3],

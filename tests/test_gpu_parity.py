@@ -356,9 +356,10 @@ def test_scatter_variants(gpu, agg, xcd, lds):
 
 
 def tiled_plan_counts(pix, t_block, n_tiles_of, item):
-    """Stretches and work items of the tiled plan layout, from the pixel
-    array: samples ordered by (time block, tile) keeping sample order;
-    a stretch is a run of consecutive samples with one key."""
+    """Stretch starts and work items of the tiled plan layout, from the pixel
+    array: samples ordered by (time block, tile) keeping sample order; a
+    stretch is a run of consecutive samples with one key, and each
+    64-position chunk's first position also counts as a start."""
     n_det, n_t = pix.shape[:2]
     on = (pix[..., 0] >= 0).ravel()
     i = np.arange(n_det * n_t)[on]
@@ -366,7 +367,10 @@ def tiled_plan_counts(pix, t_block, n_tiles_of, item):
     key = (i % n_t) // t_block * n_tiles + tile
     o = np.argsort(key, kind='stable')
     key, i = key[o], i[o]
-    heads = 1 + int(np.count_nonzero((key[1:] != key[:-1]) | (i[1:] != i[:-1] + 1)))
+    start = np.ones(len(i), bool)
+    start[1:] = (key[1:] != key[:-1]) | (i[1:] != i[:-1] + 1)
+    start[::64] = True   # every 64-position chunk's first position
+    heads = int(np.count_nonzero(start))
     _, counts = np.unique(key, return_counts=True)
     # positions per item: at most `item`, about 1024 items or more
     per = max(64, min(item, max(8192, len(i) // 1024)))
@@ -471,6 +475,54 @@ def test_precomp_plan(gpu, n_det, n_samp, comps, tconst, t_block_mb, layout, tco
                                              None))
     r = oracle.precomp_to_map(c.og, np.zeros((nc, 100, 300)), pix, spin, c.signal)
     maps_close(mf.cpu().numpy(), r, 'plan to_map f64', atol=1e-5)
+
+
+@pytest.mark.parametrize('layout,tcopy', [(0, 1), (1, 1), (1, 0)],
+                         ids=['runs', 'tiles', 'tiles-notcopy'])
+@pytest.mark.parametrize('case', ['all_off', 'tiny', 'one_pixel', 'edges'])
+def test_precomp_plan_edges(gpu, case, layout, tcopy):
+    """Plans over degenerate pointing: every sample off the map (to_map adds
+    nothing, a fresh from_map is zero), 5 samples (less than one 64-position
+    chunk), a detector parked on one pixel (one stretch / run of all its
+    samples) and samples on the map's last row and column (partial edge
+    tiles); against the oracle's Precomp."""
+    import so3g_amd
+    rng = np.random.default_rng(7)
+    ny, nx = 70, 130
+    n_det, n_t = {'tiny': (1, 5)}.get(case, (6, 700))
+    pix = np.stack([rng.integers(0, ny, (n_det, n_t)), rng.integers(0, nx, (n_det, n_t))],
+                   axis=-1).astype(np.int32)
+    if case == 'all_off':
+        pix[..., 0] = -1
+    elif case == 'one_pixel':
+        pix[2, :, 0], pix[2, :, 1] = 33, 77
+    elif case == 'edges':
+        pix[:, ::2, 0] = ny - 1
+        pix[:, 1::2, 1] = nx - 1
+        pix[1, 5:50] = -1
+    spin = np.empty((n_det, n_t, 3), np.float32)
+    spin[..., 0] = rng.uniform(0.5, 1.5, (n_det, 1))
+    spin[..., 1:] = rng.uniform(-1, 1, (n_det, n_t, 2))
+    sig = rng.standard_normal((n_det, n_t)).astype(np.float32)
+    og = oracle.geom('CAR', 3, ny=ny, nx=nx, cdelt=(1., 1.), crpix=(1., 1.), tiled=False,
+                     tile_shape=(1, 1))
+    plan = so3g_amd.PrecompPlan(pix, spin, (ny, nx), layout=layout, time_copy=tcopy)
+    info = plan.info()
+    assert info['layout'] == layout
+    assert info['n_on'] == int((pix[..., 0] >= 0).sum())
+    m0 = rng.standard_normal((3, ny, nx))
+    g = plan.to_map(m0.copy(), sig)
+    r = oracle.precomp_to_map(og, m0.copy(), pix, spin, sig)
+    maps_close(g, r, f'plan to_map {case}', atol=1e-9)
+    m = rng.standard_normal((3, ny, nx))
+    assert_bit_equal(np.array(plan.from_map(m)), oracle.precomp_from_map(og, m, pix, spin),
+                     f'plan from_map {case}')
+    s0 = rng.standard_normal((n_det, n_t)).astype(np.float32)
+    assert_bit_equal(np.array(plan.from_map(m, s0.copy())),
+                     oracle.precomp_from_map(og, m, pix, spin, s0.copy()),
+                     f'plan from_map accumulate {case}')
+    if case == 'all_off':
+        assert np.array_equal(g, m0)
 
 
 @pytest.mark.parametrize('layout,tcopy', [(0, 1), (1, 1), (1, 0)],
