@@ -1052,12 +1052,15 @@ __global__ __launch_bounds__(kTBlock) void k_tp_from_map(
         }
 #pragma unroll
         for (int u = 0; u < kTU; u++) o[u] = accumulate ? sig[si[u]] : 0.f;
+        float tv[kTU];
+#pragma unroll
+        for (int u = 0; u < kTU; u++) tv[u] = TC ? tval[fast_div<1>(si[u], n_t)] : 0.f;
 #pragma unroll
         for (int u = 0; u < kTU; u++) {
             if (!act[u]) continue;
             float pf[NC];
             if constexpr (TC) {
-                pf[0] = tval[fast_div<1>(si[u], n_t)];
+                pf[0] = tv[u];
 #pragma unroll
                 for (int q = 1; q < NC; q++) pf[q] = sp[u][q - 1];
             } else {
