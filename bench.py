@@ -295,6 +295,8 @@ def main():
     g3 = None if args.no_g3 else g3super_timing(torch, n_det, n_t, not args.no_cpu)
     variants = None if args.no_small else c2_variants(torch, so3g_amd, N, lib, obs, pixargs, bore,
                                                       dets, resp, sig, dev, stream, run)
+    other = None if args.no_small else c2_other_ops(torch, N, lib, g, bore, dets, resp, pix,
+                                                    spin, ny, nx, dev, stream, run)
 
     out = {
         'metric': 'det*samples/s for to_map+from_map (% HBM roofline)',
@@ -340,6 +342,7 @@ def main():
         'nmat_detvecs_apply': nmat,
         'g3super_decode': g3,
         'c2_variants': variants,
+        'c2_other_ops': other,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)
@@ -470,6 +473,53 @@ def nmat_timing(torch, so3g_amd, dev, n_det, n_t, nvec=10, nbin=20, reps=5):
     log(f'nmat_detvecs_apply: {ms:.3f} ms')
     return {'shape': [n_det, nmode], 'nvec': nvec, 'nbin': len(bins), 'ms': ms,
             'hbm_frac': b / (ms * 1e-3) / HBM_PEAK}
+
+
+def c2_other_ops(torch, N, lib, g, bore, dets, resp, pix, spin, ny, nx, dev, stream, run,
+                 steps=3):
+    """The other ops of the C1/C2 list on the benchmark observation (CAR 0.5
+    arcmin TQU non-tiled): get_pixels and get_pointing_matrix (pointing
+    recomputed; their outputs are 8 and 20 B per det*sample written), and
+    to_weights on the fly and through the cached pointing matrix (ms; the
+    weights map is zeroed first).  to_weights is atomic-bound: time only."""
+    n_det, n_t = pix.shape[0], pix.shape[1]
+    n_s = n_det * n_t
+    pbuf = torch.empty_like(pix)
+    wmap = torch.zeros((3, 3, ny, nx), dtype=torch.float64, device=dev)
+    shape = np.array([ny, nx], np.int64)
+
+    def chk(rc):
+        N.check(rc)
+
+    def get_pixels():
+        chk(lib.so3g_pixels(g, bore.data_ptr(), n_t, dets.data_ptr(), n_det, pbuf.data_ptr(),
+                            stream))
+
+    def get_pmat():
+        chk(lib.so3g_pointing_matrix(g, bore.data_ptr(), n_t, dets.data_ptr(), n_det,
+                                     resp.data_ptr(), pix.data_ptr(), spin.data_ptr(), stream))
+
+    def otf_weights():
+        wmap.zero_()
+        chk(lib.so3g_to_weights(g, wmap.data_ptr(), None, None, bore.data_ptr(), n_t,
+                                dets.data_ptr(), n_det, resp.data_ptr(), None, None, 0, stream))
+
+    def pc_weights():
+        wmap.zero_()
+        chk(lib.so3g_precomp_to_weights(wmap.data_ptr(), shape.ctypes.data, None, None,
+                                        pix.data_ptr(), 2, spin.data_ptr(), 3, n_det, n_t, None,
+                                        None, 0, stream))
+    _, ms = run([get_pixels, get_pmat, otf_weights, pc_weights], steps, 1)
+    del pbuf, wmap
+    out = {}
+    for name, t, b in (('get_pixels', ms[0], 8), ('get_pointing_matrix', ms[1], 20),
+                       ('to_weights_on_the_fly', ms[2], None),
+                       ('to_weights_precomp', ms[3], None)):
+        out[name] = {'ms': t, 'det_samples_per_s': n_s / (t * 1e-3)}
+        if b:
+            out[name]['hbm_frac_output'] = b * n_s / (t * 1e-3) / HBM_PEAK
+    log(f'other ops: {out}')
+    return out
 
 
 def c2_variants(torch, so3g_amd, N, lib, obs, pixargs, bore, dets, resp, sig, dev, stream, run,

@@ -215,7 +215,7 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape,
  * the visited pixels, not the map.  Samples are grouped in time blocks whose
  * signal slice is about t_block_mb MiB (0: one block; < 0: the layout's
  * default), then sorted within each block: by map tile (64 x 64 pixels, or
- * 4096 visited HEALPix pixels), keeping sample order, in the tiled layout
+ * 4096 consecutive HEALPix pixel numbers), keeping sample order, in the tiled layout
  * (option "plan_layout" 1, the default from 2^25 samples up; one time
  * block); by pixel in layout 0
  * (64 MiB blocks, sized to stay in the Infinity Cache).  Stored: per on-map

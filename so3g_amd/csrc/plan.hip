@@ -28,7 +28,7 @@
 // Tiled layout ("plan_layout" 1, the default from 2^25 samples up): the
 // samples are sorted by
 // (time block, map tile) instead -- 64 x 64 pixels of a flat map, or 4096
-// consecutive visited pixels of a HEALPix map -- and keep their sample order
+// consecutive pixel numbers of a HEALPix map -- and keep their sample order
 // inside a tile, so a detector's pass over a tile is a stretch of
 // consecutive samples (~100 for the 1.5 degree scan) and to_map's signal
 // loads are nearly contiguous.  Work items of up to 256 K positions sum into
@@ -697,8 +697,8 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
 
 // ---------------------------------------------------------------------------
 // Tiled layout.  Samples are sorted by (time block, map tile) -- 64 x 64
-// pixels of a flat map, or 4096 consecutive visited pixels of a HEALPix map
-// -- and stay in sample order inside a tile, so a wave's 64 positions fall in
+// pixels of a flat map, or 4096 consecutive pixel numbers of a HEALPix map
+// (an aligned 64 x 64 sub-square in NEST order) -- and stay in sample order inside a tile, so a wave's 64 positions fall in
 // one or two stretches of consecutive samples of one detector and its signal
 // loads are nearly contiguous.  Work items (tile, positions [b, e)) sum into
 // or gather from the tile's values held in LDS.
@@ -739,51 +739,54 @@ __device__ __forceinline__ uint32_t tile_sample(uint64_t mask, uint32_t hbase,
     return head_i[hbase + __popcll(h) - 1] + (uint32_t)(lane - p);
 }
 
-// Flat-map tiles touched by the samples (byte flags; every writer stores 1).
-__global__ __launch_bounds__(kPlanBlock) void k_tp_tmark(const int32_t *pix, int64_t n,
+// Map tile of a sample (-1 off the map) before compaction and its slot.  k = 2:
+// 64 x 64 pixel tiles, numbered row-major over a gtx-wide tile grid; k = 1:
+// blocks of kTS consecutive pixel numbers (in HEALPix NEST order an aligned
+// 64 x 64 sub-square of a base face).
+__device__ __forceinline__ int64_t flat_tile(const int32_t *pix, int k, int64_t i, int64_t ny,
+                                             int64_t nx, int64_t gtx, int32_t &slot) {
+    if (k == 1) {
+        const int32_t q = pix[i];
+        slot = q & (kTS - 1);
+        return (q >= 0 && q < ny) ? (int64_t)(q >> (kTileBY + kTileBX)) : -1;
+    }
+    const int2 q = *reinterpret_cast<const int2 *>(pix + 2 * i);
+    slot = ((q.x & ((1 << kTileBY) - 1)) << kTileBX) | (q.y & ((1 << kTileBX) - 1));
+    if (q.x < 0 || q.x >= ny || q.y < 0 || q.y >= nx) return -1;
+    return (int64_t)(q.x >> kTileBY) * gtx + (q.y >> kTileBX);
+}
+
+// Tiles touched by the samples (byte flags; every writer stores 1).
+__global__ __launch_bounds__(kPlanBlock) void k_tp_tmark(const int32_t *pix, int k, int64_t n,
                                                          int64_t ny, int64_t nx, int64_t gtx,
                                                          uint8_t *used) {
     for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kPlanBlock) {
-        const int2 q = *reinterpret_cast<const int2 *>(pix + 2 * i);
-        if (q.x >= 0 && q.x < ny && q.y >= 0 && q.y < nx)
-            used[(q.x >> kTileBY) * gtx + (q.y >> kTileBX)] = 1;
+        int32_t sl;
+        const int64_t tf = flat_tile(pix, k, i, ny, nx, gtx, sl);
+        if (tf >= 0) used[tf] = 1;
     }
 }
 
-// Tile and slot of sample i (tile < 0 off the map).  k = 2: 64 x 64 tiles,
-// tile_of maps the flat tile number to the compact one; k = 1: compact pixel
-// blocks of kTS.
+// Compact tile (tile_of maps the flat tile number to the visited tiles'
+// numbering, -1 off the map) and slot of sample i.
 __device__ __forceinline__ void tile_slot(const int32_t *pix, int k, int64_t i, int64_t ny,
                                           int64_t nx, int64_t gtx, const int32_t *tile_of,
-                                          const uint32_t *bitmap, const uint32_t *wscan,
                                           int32_t &tile, int32_t &slot) {
-    if (k == 1) {
-        const int32_t q = pix[i];
-        if (q < 0 || q >= ny) { tile = -1; slot = 0; return; }
-        const uint32_t c = compact_of(q, bitmap, wscan);
-        tile = (int32_t)(c >> (kTileBY + kTileBX));
-        slot = (int32_t)(c & (kTS - 1));
-        return;
-    }
-    const int2 q = *reinterpret_cast<const int2 *>(pix + 2 * i);
-    if (q.x < 0 || q.x >= ny || q.y < 0 || q.y >= nx) { tile = -1; slot = 0; return; }
-    tile = tile_of[(q.x >> kTileBY) * gtx + (q.y >> kTileBX)];
-    slot = ((q.x & ((1 << kTileBY) - 1)) << kTileBX) | (q.y & ((1 << kTileBX) - 1));
+    const int64_t tf = flat_tile(pix, k, i, ny, nx, gtx, slot);
+    tile = tf >= 0 ? tile_of[tf] : -1;
 }
 
 __global__ __launch_bounds__(kPlanBlock) void k_tp_keys(const int32_t *pix, int k, int64_t n,
                                                         int64_t ny, int64_t nx, int64_t gtx,
-                                                        const int32_t *tile_of,
-                                                        const uint32_t *bitmap,
-                                                        const uint32_t *wscan, int64_t n_t,
+                                                        const int32_t *tile_of, int64_t n_t,
                                                         int64_t t_block, int32_t n_tiles,
                                                         int32_t off_key, int32_t *keys,
                                                         uint32_t *vals) {
     const int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
     if (i >= n) return;
     int32_t tile, slot;
-    tile_slot(pix, k, i, ny, nx, gtx, tile_of, bitmap, wscan, tile, slot);
+    tile_slot(pix, k, i, ny, nx, gtx, tile_of, tile, slot);
     const int64_t tb = (i % n_t) / t_block;
     keys[i] = tile >= 0 ? (int32_t)(tb * n_tiles + tile) : off_key;
     vals[i] = (uint32_t)i;
@@ -794,16 +797,14 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_fill(const int32_t *pix, int 
                                                         const float *spin, int nc, int c0,
                                                         const uint32_t *idx, int64_t n_on,
                                                         int64_t ny, int64_t nx, int64_t gtx,
-                                                        const int32_t *tile_of,
-                                                        const uint32_t *bitmap,
-                                                        const uint32_t *wscan, uint16_t *slot,
+                                                        const int32_t *tile_of, uint16_t *slot,
                                                         float *fac) {
     const int ns = nc - c0;
     for (int64_t j = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; j < n_on;
          j += (int64_t)gridDim.x * kPlanBlock) {
         const int64_t i = idx[j];
         int32_t tile, sl;
-        tile_slot(pix, k, i, ny, nx, gtx, tile_of, bitmap, wscan, tile, sl);
+        tile_slot(pix, k, i, ny, nx, gtx, tile_of, tile, sl);
         slot[j] = (uint16_t)sl;
         for (int c = 0; c < ns; c++) fac[j * ns + c] = spin[i * nc + c0 + c];
     }
@@ -860,17 +861,18 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_segs(const int32_t *key, int6
 __global__ __launch_bounds__(kPlanBlock) void k_tp_tilepix(int k, int64_t ny, int64_t nx,
                                                            int64_t gtx,
                                                            const int32_t *tile_flat,
-                                                           const int32_t *comp2pix,
-                                                           int64_t n_vis, int64_t n_slots,
+                                                           int64_t n_slots,
                                                            int32_t *tile_pix) {
     for (int64_t q = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; q < n_slots;
          q += (int64_t)gridDim.x * kPlanBlock) {
-        if (k == 1) {
-            tile_pix[q] = q < n_vis ? comp2pix[q] : -1;
+        const int64_t tf = tile_flat[q >> (kTileBY + kTileBX)];
+        const int sl = (int)(q & (kTS - 1));
+        if (tf < 0) {
+            tile_pix[q] = -1;
+        } else if (k == 1) {
+            const int64_t px = tf * kTS + sl;
+            tile_pix[q] = px < ny ? (int32_t)px : -1;
         } else {
-            const int64_t tf = tile_flat[q >> (kTileBY + kTileBX)];
-            if (tf < 0) { tile_pix[q] = -1; continue; }
-            const int sl = (int)(q & (kTS - 1));
             const int64_t iy = (tf / gtx) * (1 << kTileBY) + (sl >> kTileBX);
             const int64_t ix = (tf % gtx) * (1 << kTileBX) + (sl & ((1 << kTileBX) - 1));
             tile_pix[q] = (iy < ny && ix < nx) ? (int32_t)(iy * nx + ix) : -1;
@@ -1296,27 +1298,31 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
     if (g_plan_layout == 1 || (g_plan_layout < 0 && n >= kPlanTiledMin)) {
         // ---- tiled layout ----
         p->layout = 1;
-        int64_t gtx = 1;
-        std::vector<int32_t> h_tile_flat;
+        int64_t gtx = 1, n_tflat;
         if (k == 2) {
-            const int64_t gty = (ny + (1 << kTileBY) - 1) >> kTileBY;
             gtx = (nx + (1 << kTileBX) - 1) >> kTileBX;
-            PLAN_ALLOC(used, gty * gtx);
-            PLAN_STEP(hipMemsetAsync(used, 0, gty * gtx, s));
-            hipLaunchKernelGGL(k_tp_tmark, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, n, ny,
-                               nx, gtx, used);
+            n_tflat = ((ny + (1 << kTileBY) - 1) >> kTileBY) * gtx;
+        } else {
+            n_tflat = (ny + kTS - 1) / kTS;
+        }
+        std::vector<int32_t> h_tile_flat;
+        {
+            PLAN_ALLOC(used, n_tflat);
+            PLAN_STEP(hipMemsetAsync(used, 0, n_tflat, s));
+            hipLaunchKernelGGL(k_tp_tmark, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, k, n,
+                               ny, nx, gtx, used);
             PLAN_STEP(hipGetLastError());
-            std::vector<uint8_t> h_used(gty * gtx);
-            PLAN_STEP(hipMemcpyAsync(h_used.data(), used, gty * gtx, hipMemcpyDeviceToHost, s));
+            std::vector<uint8_t> h_used(n_tflat);
+            PLAN_STEP(hipMemcpyAsync(h_used.data(), used, n_tflat, hipMemcpyDeviceToHost, s));
             PLAN_STEP(hipStreamSynchronize(s));
-            std::vector<int32_t> h_tile_of(gty * gtx, -1);
-            for (int64_t t = 0; t < gty * gtx; t++)
+            std::vector<int32_t> h_tile_of(n_tflat, -1);
+            for (int64_t t = 0; t < n_tflat; t++)
                 if (h_used[t]) {
                     h_tile_of[t] = (int32_t)h_tile_flat.size();
                     h_tile_flat.push_back((int32_t)t);
                 }
-            PLAN_ALLOC(tile_of, gty * gtx * 4);
-            PLAN_STEP(hipMemcpyAsync(tile_of, h_tile_of.data(), gty * gtx * 4,
+            PLAN_ALLOC(tile_of, n_tflat * 4);
+            PLAN_STEP(hipMemcpyAsync(tile_of, h_tile_of.data(), n_tflat * 4,
                                      hipMemcpyHostToDevice, s));
             p->n_tiles = (int64_t)h_tile_flat.size();
             PLAN_ALLOC(tile_flat, std::max<int64_t>(1, p->n_tiles) * 4);
@@ -1324,8 +1330,6 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
             if (p->n_tiles > 0)
                 PLAN_STEP(hipMemcpyAsync(tile_flat, h_tile_flat.data(), p->n_tiles * 4,
                                          hipMemcpyHostToDevice, s));
-        } else {
-            p->n_tiles = (p->n_vis + kTS - 1) / kTS;
         }
         const int64_t n_tiles = std::max<int64_t>(1, p->n_tiles);
         if (t_block_mb < 0) t_block_mb = 0;   // one time block: stretches are contiguous
@@ -1347,7 +1351,7 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         PLAN_ALLOC(p->idx, n * 4);
         PLAN_ALLOC(d_on, sizeof(unsigned long long));
         hipLaunchKernelGGL(k_tp_keys, dim3(grid_of(n, kPlanBlock, INT32_MAX)), dim3(kPlanBlock),
-                           0, s, pixel, k, n, ny, nx, gtx, tile_of, bitmap, wscan, n_t, t_block,
+                           0, s, pixel, k, n, ny, nx, gtx, tile_of, n_t, t_block,
                            (int32_t)n_tiles, off_key, keys_in, vals_in);
         PLAN_STEP(hipGetLastError());
         PLAN_STEP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, vals_in,
@@ -1391,8 +1395,8 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         PLAN_ALLOC(p->sspin, p->n_on * p->ns * sizeof(float));
         if (p->n_on > 0) {
             hipLaunchKernelGGL(k_tp_fill, dim3(grid_of(p->n_on)), dim3(kPlanBlock), 0, s, pixel,
-                               k, spin, nc, c0, p->idx, p->n_on, ny, nx, gtx, tile_of, bitmap,
-                               wscan, p->tslot, p->sspin);
+                               k, spin, nc, c0, p->idx, p->n_on, ny, nx, gtx, tile_of,
+                               p->tslot, p->sspin);
             PLAN_STEP(hipGetLastError());
         }
         // stretches
@@ -1449,8 +1453,7 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         // flat pixel of every slot
         PLAN_ALLOC(p->tile_pix, n_tiles * kTS * 4);
         hipLaunchKernelGGL(k_tp_tilepix, dim3(grid_of(n_tiles * kTS)), dim3(kPlanBlock), 0, s, k,
-                           ny, nx, gtx, tile_flat, p->comp2pix, p->n_vis, n_tiles * kTS,
-                           p->tile_pix);
+                           ny, nx, gtx, tile_flat, n_tiles * kTS, p->tile_pix);
         PLAN_STEP(hipGetLastError());
         // samples off the map (zeroed by a fresh from_map)
         p->n_off = n - p->n_on;
