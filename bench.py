@@ -529,8 +529,9 @@ def c2_variants(torch, so3g_amd, N, lib, obs, pixargs, bore, dets, resp, sig, de
     observation hits) and a HEALPix NEST nside 4096 TQU map (the boresight
     taken as celestial).  Each is timed as the on-the-fly pair (pointing
     recomputed in both kernels) and the cached-pointing (ProjEng_Precomp)
-    pair, and for HEALPix also through a PrecompPlan (whose pixel tables
-    cover the visited pixels only); ms per op and det*samples/s per pair."""
+    pair, and through a PrecompPlan (tiled map: tile pointer lists; HEALPix:
+    pixel tables over the visited pixels only); ms per op and det*samples/s
+    per pair."""
     n_det, n_t = obs.n_det, obs.n_samp
     N_s = n_det * n_t
     out = {}
@@ -586,13 +587,28 @@ def c2_variants(torch, so3g_amd, N, lib, obs, pixargs, bore, dets, resp, sig, de
         chk(lib.so3g_precomp_to_map(None, None, d_out.data_ptr(), d_dims.data_ptr(),
                                     pix.data_ptr(), 3, spin.data_ptr(), 3, n_det, n_t,
                                     sig.data_ptr(), None, None, 0, stream))
+    torch.cuda.synchronize()
+    t0 = time.time()
+    tplan = so3g_amd.PrecompPlan(pix, spin, tile_dims=dims)
+    torch.cuda.synchronize()
+    t_tplan = time.time() - t0
+
+    def plan_from():
+        chk(lib.so3g_pc_plan_from_map_tiles(tplan._h, ptr_in.ctypes.data, sig_out.data_ptr(), 0,
+                                            stream))
+
+    def plan_to():
+        buf_out.zero_()
+        chk(lib.so3g_pc_plan_to_map_tiles(tplan._h, ptr_out.ctypes.data, sig.data_ptr(), None,
+                                          stream))
     res = {'map': f'CAR 0.5 arcmin TQU {g.naxis[0]}x{g.naxis[1]}, 100x100 tiles, '
-                  f'{len(active)} of {g.n_tile} active'}
-    for name, ops in (('on_the_fly', [otf_from, otf_to]), ('precomp', [pc_from, pc_to])):
+                  f'{len(active)} of {g.n_tile} active', 'plan_build_s': t_tplan}
+    for name, ops in (('on_the_fly', [otf_from, otf_to]), ('precomp', [pc_from, pc_to]),
+                      ('plan', [plan_from, plan_to])):
         _, (mf, mt) = run(ops, steps, 1)
         res[name] = {'ms_from_map': mf, 'ms_to_map': mt, 'det_samples_per_s': N_s / ((mf + mt) * 1e-3)}
     out['tiled_car'] = res
-    del pix, spin, buf_in, buf_out
+    del pix, spin, buf_in, buf_out, tplan
     log(f'tiled CAR: {res}')
 
     # -- HEALPix NEST nside 4096 ------------------------------------------

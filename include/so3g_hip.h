@@ -246,6 +246,23 @@ int so3g_precomp_from_map_f64(const double *map, const int64_t *map_shape,
                               const int32_t *pixel, const float *spin, int32_t nc,
                               int64_t n_det, int64_t n_t, double *signal, int accumulate,
                               void *stream);
+/* Plans for tiled maps (ProjEng_Precomp<Tiled>, Projection.cxx:2012-2253:
+ * pixel (n_det, n_t, 3) = (tile, row, column within the tile)).  tile_dims:
+ * host (n_tile, 2) int32 tile shapes (rows, cols) of the whole tiling,
+ * inactive tiles included; always the tiled layout.  The _tiles calls take a
+ * HOST array of n_tile device tile pointers ((nc, rows, cols) float64 each;
+ * NULL for an inactive tile) and fail with a RuntimeError when a tile that
+ * the plan's samples hit is NULL, as the reference does for an inactive tile;
+ * the pointer array may be reused once the call returns.  The non-tiled
+ * calls refuse a tiled plan and the other way round. */
+int so3g_pc_plan_create_tiled(const int32_t *pixel, const float *spin, int32_t nc,
+                              int64_t n_det, int64_t n_t, const int32_t *tile_dims,
+                              int32_t n_tile, int64_t t_block_mb, so3g_pc_plan **out,
+                              void *stream);
+int so3g_pc_plan_to_map_tiles(const so3g_pc_plan *plan, double *const *tiles,
+                              const float *signal, const float *det_weights, void *stream);
+int so3g_pc_plan_from_map_tiles(const so3g_pc_plan *plan, const double *const *tiles,
+                                float *signal, int accumulate, void *stream);
 int so3g_pc_plan_info(const so3g_pc_plan *plan, int64_t *n_on, int64_t *bytes);
 /* stats[10]: n_on, pixel runs (layout 0), T factor constant per detector
  * (0/1), stored spin factors per sample, samples per time block, device

@@ -95,6 +95,17 @@ struct so3g_pc_plan {
     void *items;           // [n_items] work items (first position, count, tile)
     uint32_t *off_idx;     // [n_off] samples off the map
     int64_t n_items, n_tiles, n_heads, n_off;
+    // Tiled maps (k = 3 pixel arrays; tiled layout only): pixels are
+    // numbered by concatenating the map tiles.
+    int32_t n_tile_map;    // map tiles (0: a non-tiled map)
+    int64_t n_sub;         // 64 x 64 sub-blocks over all map tiles
+    int32_t *tdims_d;      // [n_tile_map * 2] tile shapes
+    int64_t *tbase_d;      // [n_tile_map + 1] first pixel of each tile
+    int32_t *sbase_d;      // [n_tile_map + 1] first sub-block of each tile
+    int32_t *tile_map;     // [n_tiles] map tile of each plan tile
+    int32_t *comp_tile;    // [n_vis] map tile of each visited pixel (comp2pix: offset in it)
+    double **tiles_d;      // [n_tile_map] tile pointers of the current call
+    std::vector<uint8_t> tile_used;   // map tiles the plan's samples touch
 };
 
 namespace so3g {
@@ -141,13 +152,39 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 constexpr int kPlanR = 8;                 // sorted samples per lane
 constexpr int kGroup = 64 * kPlanR;       // samples per wave = per head group
 
-// Flat pixel of sample i, -1 off the map: (iy, ix) pairs on an (ny, nx)
-// map (k = 2) or pixel numbers on an (ny = npix, 1) map (k = 1).
-__device__ __forceinline__ int64_t flat_pixel(const int32_t *pix, int k, int64_t i, int64_t ny,
-                                              int64_t nx) {
-    if (k == 1) {
+// Pixel array geometry: k = 1 pixel numbers on an (ny = npix, 1) map, k = 2
+// (iy, ix) pairs on an (ny, nx) map, k = 3 (tile, iy, ix) triples on a tiled
+// map whose tiles are numbered by concatenation (ny = total pixels, nx = 1;
+// tdims: per tile (rows, cols); tbase: first pixel of each tile; sbase: first
+// 64 x 64 sub-block of each tile, n_tile + 1 entries each).
+struct PixGeom {
+    int32_t k;
+    int32_t n_tile;
+    const int32_t *tdims;
+    const int64_t *tbase;
+    const int32_t *sbase;
+};
+
+// Tile, row and column of a k = 3 sample, false off the map (tile or
+// position outside the tiles' shapes).
+__device__ __forceinline__ bool tile_pos(const int32_t *pix, const PixGeom &g, int64_t i,
+                                         int32_t &t, int32_t &iy, int32_t &ix, int32_t &tx) {
+    t = pix[3 * i]; iy = pix[3 * i + 1]; ix = pix[3 * i + 2];
+    if (t < 0 || t >= g.n_tile) return false;
+    tx = g.tdims[2 * t + 1];
+    return iy >= 0 && iy < g.tdims[2 * t] && ix >= 0 && ix < tx;
+}
+
+// Flat pixel of sample i, -1 off the map.
+__device__ __forceinline__ int64_t flat_pixel(const int32_t *pix, const PixGeom &g, int64_t i,
+                                              int64_t ny, int64_t nx) {
+    if (g.k == 1) {
         const int32_t p = pix[i];
         return (p >= 0 && p < ny) ? (int64_t)p : -1;
+    }
+    if (g.k == 3) {
+        int32_t t, iy, ix, tx;
+        return tile_pos(pix, g, i, t, iy, ix, tx) ? g.tbase[t] + (int64_t)iy * tx + ix : -1;
     }
     const int2 p = *reinterpret_cast<const int2 *>(pix + 2 * i);
     return (p.x >= 0 && p.x < ny && p.y >= 0 && p.y < nx) ? (int64_t)p.x * nx + p.y : -1;
@@ -165,14 +202,14 @@ __device__ __forceinline__ uint32_t compact_of(int64_t p, const uint32_t *bitmap
 // and one of them adds them with a single atomic (one atomic per sample on
 // the few thousand words of a map took 12.8 ms for 7.2e8 samples).  The
 // loop bound is uniform per wave, so every lane takes part in the shuffles.
-__global__ __launch_bounds__(kPlanBlock) void k_plan_mark(const int32_t *pix, int k, int64_t n,
+__global__ __launch_bounds__(kPlanBlock) void k_plan_mark(const int32_t *pix, PixGeom pg, int64_t n,
                                                           int64_t ny, int64_t nx,
                                                           uint32_t *bitmap) {
     const int lane = (int)__lane_id();
     for (int64_t i0 = (int64_t)blockIdx.x * kPlanBlock + (threadIdx.x & ~63); i0 < n;
          i0 += (int64_t)gridDim.x * kPlanBlock) {
         const int64_t i = i0 + lane;
-        const int64_t p = i < n ? flat_pixel(pix, k, i, ny, nx) : -1;
+        const int64_t p = i < n ? flat_pixel(pix, pg, i, ny, nx) : -1;
         const int32_t w = p >= 0 ? (int32_t)(p >> 5) : -1;
         const uint32_t bit = p >= 0 ? 1u << (p & 31) : 0u;
         uint64_t todo = __ballot(w >= 0);
@@ -213,7 +250,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_compact(const uint32_t *bit
 
 // Sort keys: (time block, compact pixel) as tb * n_vis + compact, or the
 // largest key for an off-map sample (sorted last).
-__global__ __launch_bounds__(kPlanBlock) void k_plan_keys(const int32_t *pix, int k, int64_t n,
+__global__ __launch_bounds__(kPlanBlock) void k_plan_keys(const int32_t *pix, PixGeom pg, int64_t n,
                                                           int64_t ny, int64_t nx, int64_t n_t,
                                                           int64_t t_block, int32_t n_vis,
                                                           int32_t off_key,
@@ -222,7 +259,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_keys(const int32_t *pix, in
                                                           uint32_t *vals) {
     const int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
     if (i >= n) return;
-    const int64_t p = flat_pixel(pix, k, i, ny, nx);
+    const int64_t p = flat_pixel(pix, pg, i, ny, nx);
     const int64_t tb = (i % n_t) / t_block;
     keys[i] = p >= 0 ? (int32_t)(tb * n_vis + compact_of(p, bitmap, wscan)) : off_key;
     vals[i] = (uint32_t)i;
@@ -271,7 +308,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_sspin(const float *spin, in
 }
 
 // Time-ordered copy: compact pixel (or -1) and the stored spin factors.
-__global__ __launch_bounds__(kPlanBlock) void k_plan_tcopy(const int32_t *pix, int k,
+__global__ __launch_bounds__(kPlanBlock) void k_plan_tcopy(const int32_t *pix, PixGeom pg,
                                                            const float *spin, int nc, int c0,
                                                            int64_t n, int64_t ny, int64_t nx,
                                                            const uint32_t *bitmap,
@@ -280,7 +317,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_tcopy(const int32_t *pix, i
     const int ns = nc - c0;
     for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kPlanBlock) {
-        const int64_t p = flat_pixel(pix, k, i, ny, nx);
+        const int64_t p = flat_pixel(pix, pg, i, ny, nx);
         tpix[i] = p >= 0 ? (int32_t)compact_of(p, bitmap, wscan) : -1;
         for (int c = 0; c < ns; c++) tspin[i * ns + c] = spin[i * nc + c0 + c];
     }
@@ -538,6 +575,45 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_pack(const double *map, int
     }
 }
 
+// Tiled maps: split each visited pixel's concatenated number into its map
+// tile (comp_tile) and its offset in the tile (written over comp2pix).
+__global__ __launch_bounds__(kPlanBlock) void k_plan_comp_tiles(int32_t *comp2pix, int64_t n_vis,
+                                                                PixGeom pg, int32_t *comp_tile) {
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n_vis;
+         i += (int64_t)gridDim.x * kPlanBlock) {
+        const int64_t p = comp2pix[i];
+        int lo = 0, hi = pg.n_tile;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (pg.tbase[mid] <= p) lo = mid;
+            else hi = mid;
+        }
+        comp_tile[i] = lo;
+        comp2pix[i] = (int32_t)(p - pg.tbase[lo]);
+    }
+}
+
+// Pixel-major copy of the visited pixels of a tiled map (tiles[t]: the
+// (NC, rows, cols) tile t).
+template <int NC>
+__global__ __launch_bounds__(kPlanBlock) void k_plan_pack_tiles(const double *const *tiles,
+                                                                const int32_t *tdims,
+                                                                const int32_t *comp_tile,
+                                                                const int32_t *comp2pix,
+                                                                int64_t n_vis, double *packed) {
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n_vis;
+         i += (int64_t)gridDim.x * kPlanBlock) {
+        const int32_t t = comp_tile[i];
+        const int64_t pl = (int64_t)tdims[2 * t] * tdims[2 * t + 1];
+        const double *base = tiles[t] + comp2pix[i];
+        double v[4] = {0., 0., 0., 0.};
+#pragma unroll
+        for (int c = 0; c < NC; c++) v[c] = base[c * pl];
+        reinterpret_cast<double2 *>(packed + 4 * i)[0] = double2{v[0], v[1]};
+        reinterpret_cast<double2 *>(packed + 4 * i)[1] = double2{v[2], v[3]};
+    }
+}
+
 // Planned from_map over the time-ordered copy: V consecutive samples per
 // lane (V = 8 with vector loads when n % 8 == 0, else V = 1).  sr = sum_c
 // pf_c * map_c accumulated into a float (each addition rounded to float),
@@ -743,9 +819,19 @@ __device__ __forceinline__ uint32_t tile_sample(uint64_t mask, uint32_t hbase,
 // 64 x 64 pixel tiles, numbered row-major over a gtx-wide tile grid; k = 1:
 // blocks of kTS consecutive pixel numbers (in HEALPix NEST order an aligned
 // 64 x 64 sub-square of a base face).
-__device__ __forceinline__ int64_t flat_tile(const int32_t *pix, int k, int64_t i, int64_t ny,
-                                             int64_t nx, int64_t gtx, int32_t &slot) {
-    if (k == 1) {
+__device__ __forceinline__ int64_t flat_tile(const int32_t *pix, const PixGeom &pg, int64_t i,
+                                             int64_t ny, int64_t nx, int64_t gtx,
+                                             int32_t &slot) {
+    if (pg.k == 3) {
+        // 64 x 64 sub-blocks of each map tile, numbered from sbase[t]
+        int32_t t, iy, ix, tx;
+        slot = 0;
+        if (!tile_pos(pix, pg, i, t, iy, ix, tx)) return -1;
+        slot = ((iy & ((1 << kTileBY) - 1)) << kTileBX) | (ix & ((1 << kTileBX) - 1));
+        const int32_t nbx = (tx + (1 << kTileBX) - 1) >> kTileBX;
+        return (int64_t)pg.sbase[t] + (iy >> kTileBY) * nbx + (ix >> kTileBX);
+    }
+    if (pg.k == 1) {
         const int32_t q = pix[i];
         slot = q & (kTS - 1);
         return (q >= 0 && q < ny) ? (int64_t)(q >> (kTileBY + kTileBX)) : -1;
@@ -757,27 +843,28 @@ __device__ __forceinline__ int64_t flat_tile(const int32_t *pix, int k, int64_t 
 }
 
 // Tiles touched by the samples (byte flags; every writer stores 1).
-__global__ __launch_bounds__(kPlanBlock) void k_tp_tmark(const int32_t *pix, int k, int64_t n,
+__global__ __launch_bounds__(kPlanBlock) void k_tp_tmark(const int32_t *pix, PixGeom pg, int64_t n,
                                                          int64_t ny, int64_t nx, int64_t gtx,
                                                          uint8_t *used) {
     for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kPlanBlock) {
         int32_t sl;
-        const int64_t tf = flat_tile(pix, k, i, ny, nx, gtx, sl);
+        const int64_t tf = flat_tile(pix, pg, i, ny, nx, gtx, sl);
         if (tf >= 0) used[tf] = 1;
     }
 }
 
 // Compact tile (tile_of maps the flat tile number to the visited tiles'
 // numbering, -1 off the map) and slot of sample i.
-__device__ __forceinline__ void tile_slot(const int32_t *pix, int k, int64_t i, int64_t ny,
-                                          int64_t nx, int64_t gtx, const int32_t *tile_of,
-                                          int32_t &tile, int32_t &slot) {
-    const int64_t tf = flat_tile(pix, k, i, ny, nx, gtx, slot);
+__device__ __forceinline__ void tile_slot(const int32_t *pix, const PixGeom &pg, int64_t i,
+                                          int64_t ny, int64_t nx, int64_t gtx,
+                                          const int32_t *tile_of, int32_t &tile,
+                                          int32_t &slot) {
+    const int64_t tf = flat_tile(pix, pg, i, ny, nx, gtx, slot);
     tile = tf >= 0 ? tile_of[tf] : -1;
 }
 
-__global__ __launch_bounds__(kPlanBlock) void k_tp_keys(const int32_t *pix, int k, int64_t n,
+__global__ __launch_bounds__(kPlanBlock) void k_tp_keys(const int32_t *pix, PixGeom pg, int64_t n,
                                                         int64_t ny, int64_t nx, int64_t gtx,
                                                         const int32_t *tile_of, int64_t n_t,
                                                         int64_t t_block, int32_t n_tiles,
@@ -786,14 +873,14 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_keys(const int32_t *pix, int 
     const int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
     if (i >= n) return;
     int32_t tile, slot;
-    tile_slot(pix, k, i, ny, nx, gtx, tile_of, tile, slot);
+    tile_slot(pix, pg, i, ny, nx, gtx, tile_of, tile, slot);
     const int64_t tb = (i % n_t) / t_block;
     keys[i] = tile >= 0 ? (int32_t)(tb * n_tiles + tile) : off_key;
     vals[i] = (uint32_t)i;
 }
 
 // Slot and stored factors of each sorted position.
-__global__ __launch_bounds__(kPlanBlock) void k_tp_fill(const int32_t *pix, int k,
+__global__ __launch_bounds__(kPlanBlock) void k_tp_fill(const int32_t *pix, PixGeom pg,
                                                         const float *spin, int nc, int c0,
                                                         const uint32_t *idx, int64_t n_on,
                                                         int64_t ny, int64_t nx, int64_t gtx,
@@ -804,7 +891,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_fill(const int32_t *pix, int 
          j += (int64_t)gridDim.x * kPlanBlock) {
         const int64_t i = idx[j];
         int32_t tile, sl;
-        tile_slot(pix, k, i, ny, nx, gtx, tile_of, tile, sl);
+        tile_slot(pix, pg, i, ny, nx, gtx, tile_of, tile, sl);
         slot[j] = (uint16_t)sl;
         for (int c = 0; c < ns; c++) fac[j * ns + c] = spin[i * nc + c0 + c];
     }
@@ -858,18 +945,36 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_segs(const int32_t *key, int6
 }
 
 // Flat pixel of each (compact tile, slot), -1 for slots off the map.
-__global__ __launch_bounds__(kPlanBlock) void k_tp_tilepix(int k, int64_t ny, int64_t nx,
+// k = 3: the slot's offset in its map tile (row * cols + col), and per plan
+// tile the map tile (tile_map) -- the last t with sbase[t] <= sub-block, so
+// that tiles without sub-blocks are skipped.
+__global__ __launch_bounds__(kPlanBlock) void k_tp_tilepix(PixGeom pg, int64_t ny, int64_t nx,
                                                            int64_t gtx,
                                                            const int32_t *tile_flat,
                                                            int64_t n_slots,
-                                                           int32_t *tile_pix) {
+                                                           int32_t *tile_pix,
+                                                           int32_t *tile_map) {
     for (int64_t q = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; q < n_slots;
          q += (int64_t)gridDim.x * kPlanBlock) {
         const int64_t tf = tile_flat[q >> (kTileBY + kTileBX)];
         const int sl = (int)(q & (kTS - 1));
         if (tf < 0) {
             tile_pix[q] = -1;
-        } else if (k == 1) {
+        } else if (pg.k == 3) {
+            int lo = 0, hi = pg.n_tile;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) / 2;
+                if (pg.sbase[mid] <= tf) lo = mid;
+                else hi = mid;
+            }
+            const int32_t ty = pg.tdims[2 * lo], tx = pg.tdims[2 * lo + 1];
+            const int32_t nbx = (tx + (1 << kTileBX) - 1) >> kTileBX;
+            const int32_t local = (int32_t)(tf - pg.sbase[lo]);
+            const int32_t iy = (local / nbx) * (1 << kTileBY) + (sl >> kTileBX);
+            const int32_t ix = (local % nbx) * (1 << kTileBX) + (sl & ((1 << kTileBX) - 1));
+            tile_pix[q] = (iy < ty && ix < tx) ? iy * tx + ix : -1;
+            if (sl == 0) tile_map[q >> (kTileBY + kTileBX)] = lo;
+        } else if (pg.k == 1) {
             const int64_t px = tf * kTS + sl;
             tile_pix[q] = px < ny ? (int32_t)px : -1;
         } else {
@@ -899,7 +1004,9 @@ __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
     const uint32_t *__restrict__ thead,
     const uint32_t *__restrict__ head_i, const int32_t *__restrict__ tile_pix,
     const float *__restrict__ tval, FastDiv n_t, uint32_t plane, const ST *__restrict__ sig,
-    const float *__restrict__ det_weights, int32_t n_det, double *map) {
+    const float *__restrict__ det_weights, int32_t n_det, double *map,
+    double *const *__restrict__ tiles, const int32_t *__restrict__ tile_map,
+    const int32_t *__restrict__ tdims) {
     constexpr int NS = TC ? NC - 1 : NC;
     __shared__ double acc[NC * kTS];
     extern __shared__ float s_tab[];   // LDT: [n_det] T factors (TC), [n_det] weights
@@ -984,13 +1091,20 @@ __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
     }
     __syncthreads();
     const int32_t *tp = tile_pix + (int64_t)it.tile * kTS;
+    double *mbase = map;     // non-tiled: the map; tiled: the item's map tile
+    int64_t pl = plane;
+    if (tiles != nullptr) {
+        const int32_t mt = tile_map[it.tile];
+        mbase = tiles[mt];
+        pl = (int64_t)tdims[2 * mt] * tdims[2 * mt + 1];
+    }
     for (int q = threadIdx.x; q < kTS; q += kTBlock) {
         const int32_t px = tp[q];
         if (px < 0) continue;
 #pragma unroll
         for (int cc = 0; cc < NC; cc++) {
             const double v = acc[cc * kTS + q];
-            if (v != 0.) unsafeAtomicAdd(map + (int64_t)cc * plane + px, v);
+            if (v != 0.) unsafeAtomicAdd(mbase + cc * pl + px, v);
         }
     }
 }
@@ -1005,16 +1119,24 @@ __global__ __launch_bounds__(kTBlock) void k_tp_from_map(
     const uint32_t *__restrict__ thead,
     const uint32_t *__restrict__ head_i, const int32_t *__restrict__ tile_pix,
     const float *__restrict__ tval, FastDiv n_t, uint32_t plane, const double *__restrict__ map,
-    float *__restrict__ sig, int accumulate) {
+    float *__restrict__ sig, int accumulate, const double *const *__restrict__ tiles,
+    const int32_t *__restrict__ tile_map, const int32_t *__restrict__ tdims) {
     constexpr int NS = TC ? NC - 1 : NC;
     __shared__ double mv[NC * kTS];
     const TItem it = items[blockIdx.x];
     const int32_t *tp = tile_pix + (int64_t)it.tile * kTS;
+    const double *mbase = map;
+    int64_t pl = plane;
+    if (tiles != nullptr) {
+        const int32_t mt = tile_map[it.tile];
+        mbase = tiles[mt];
+        pl = (int64_t)tdims[2 * mt] * tdims[2 * mt + 1];
+    }
     for (int q = threadIdx.x; q < kTS; q += kTBlock) {
         const int32_t px = tp[q];
 #pragma unroll
         for (int cc = 0; cc < NC; cc++)
-            mv[cc * kTS + q] = px >= 0 ? map[(int64_t)cc * plane + px] : 0.;
+            mv[cc * kTS + q] = px >= 0 ? mbase[cc * pl + px] : 0.;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
@@ -1100,11 +1222,13 @@ void free_plan_arrays(so3g_pc_plan *p) {
     (void)hipFree(p->tslot); (void)hipFree(p->tmask);
     (void)hipFree(p->thead); (void)hipFree(p->head_i); (void)hipFree(p->tile_pix);
     (void)hipFree(p->items); (void)hipFree(p->off_idx);
+    (void)hipFree(p->tdims_d); (void)hipFree(p->tbase_d); (void)hipFree(p->sbase_d);
+    (void)hipFree(p->tile_map); (void)hipFree(p->comp_tile); (void)hipFree(p->tiles_d);
 }
 
 template <typename ST>
 int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
-                const float *det_weights, void *stream) {
+                const float *det_weights, void *stream, double *const *tiles = nullptr) {
     if (p == nullptr) return set_error(SO3G_ERR_VALUE, "plan is NULL");
     if (p->n_on == 0) return SO3G_OK;
     hipStream_t s = (hipStream_t)stream;
@@ -1126,12 +1250,12 @@ int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
             hipLaunchKernelGGL((k_tp_to_map<NC, TC, ST, true>), gi, dim3(kTBlock), lds, s,  \
                                items, p->tslot, p->sspin, p->tmask, p->thead,   \
                                p->head_i, p->tile_pix, p->tval, nt, plane, signal,          \
-                               det_weights, nd, map);                                       \
+                               det_weights, nd, map, tiles, p->tile_map, p->tdims_d);       \
         else                                                                                \
             hipLaunchKernelGGL((k_tp_to_map<NC, TC, ST, false>), gi, dim3(kTBlock), 0, s,   \
                                items, p->tslot, p->sspin, p->tmask, p->thead,   \
                                p->head_i, p->tile_pix, p->tval, nt, plane, signal,          \
-                               det_weights, nd, map);                                       \
+                               det_weights, nd, map, tiles, p->tile_map, p->tdims_d);       \
     } while (0)
         if (p->tconst) {
             if (p->nc == 1) TP_TO(1, true);
@@ -1202,29 +1326,17 @@ __global__ __launch_bounds__(kPlanBlock) void k_pc_from_map_f64(int64_t n, int32
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int32_t nc,
-                        int64_t n_det, int64_t n_t, const int64_t *map_shape,
-                        int64_t t_block_mb, so3g_pc_plan **out, void *stream) {
-    if (out == nullptr) return set_error(SO3G_ERR_VALUE, "out is NULL");
-    *out = nullptr;
-    if (k != 1 && k != 2)
-        return set_error(SO3G_ERR_RUNTIME, "plans support non-tiled maps (k = 1 or 2)");
-    if (nc < 1 || nc > 3) return set_error(SO3G_ERR_RUNTIME, "spin component count must be 1..3");
-    if (n_det <= 0 || n_t <= 0)
-        return set_error(SO3G_ERR_RUNTIME, "plan needs 0 < n_det * n_t < 2^32");
-    if (n_det >= (1 << 22)) return set_error(SO3G_ERR_RUNTIME, "plan needs n_det < 2^22");
+// Shared plan construction.  p holds the map description (and, for tiled
+// maps, its device tables); it is freed here on failure.  ny, nx: the map's
+// flat pixel grid ((npix, 1) for k = 1 and 3).
+int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_t nc,
+                     int64_t n_det, int64_t n_t, int64_t ny, int64_t nx, int64_t t_block_mb,
+                     so3g_pc_plan *p, so3g_pc_plan **out, hipStream_t s) {
+    const int k = pg.k;
     const int64_t n = n_det * n_t;
-    if (n > (int64_t)UINT32_MAX)
-        return set_error(SO3G_ERR_RUNTIME, "plan needs 0 < n_det * n_t < 2^32");
-    const int64_t ny = map_shape[0], nx = map_shape[1];
-    if (ny <= 0 || nx <= 0 || ny * nx >= INT32_MAX || (k == 1 && nx != 1))
-        return set_error(SO3G_ERR_RUNTIME, "map too large for a plan, or not (npix, 1) for k = 1");
-    hipStream_t s = (hipStream_t)stream;
     const int64_t npix = ny * nx;
-
-    so3g_pc_plan *p = new so3g_pc_plan();
     p->n_det = n_det; p->n_t = n_t; p->ny = ny; p->nx = nx; p->nc = nc;
     int32_t *keys_in = nullptr, *keys_out = nullptr;
     uint32_t *vals_in = nullptr, *gcount = nullptr, *bitmap = nullptr, *wcnt = nullptr,
@@ -1272,7 +1384,7 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
     PLAN_ALLOC(wcnt, n_words * 4);
     PLAN_ALLOC(wscan, n_words * 4);
     PLAN_STEP(hipMemsetAsync(bitmap, 0, n_words * 4, s));
-    hipLaunchKernelGGL(k_plan_mark, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, k, n, ny,
+    hipLaunchKernelGGL(k_plan_mark, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, pg, n, ny,
                        nx, bitmap);
     hipLaunchKernelGGL(k_plan_wcount, dim3(grid_of(n_words)), dim3(kPlanBlock), 0, s, bitmap,
                        n_words, wcnt);
@@ -1294,14 +1406,22 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
     hipLaunchKernelGGL(k_plan_compact, dim3(grid_of(n_words)), dim3(kPlanBlock), 0, s, bitmap,
                        wscan, n_words, p->comp2pix);
     PLAN_STEP(hipGetLastError());
+    if (k == 3) {
+        PLAN_ALLOC(p->comp_tile, p->n_vis * 4);
+        hipLaunchKernelGGL(k_plan_comp_tiles, dim3(grid_of(p->n_vis)), dim3(kPlanBlock), 0, s,
+                           p->comp2pix, p->n_vis, pg, p->comp_tile);
+        PLAN_STEP(hipGetLastError());
+    }
     const int32_t n_vis = (int32_t)std::max<int64_t>(1, p->n_vis);
-    if (g_plan_layout == 1 || (g_plan_layout < 0 && n >= kPlanTiledMin)) {
+    if (k == 3 || g_plan_layout == 1 || (g_plan_layout < 0 && n >= kPlanTiledMin)) {
         // ---- tiled layout ----
         p->layout = 1;
         int64_t gtx = 1, n_tflat;
         if (k == 2) {
             gtx = (nx + (1 << kTileBX) - 1) >> kTileBX;
             n_tflat = ((ny + (1 << kTileBY) - 1) >> kTileBY) * gtx;
+        } else if (k == 3) {
+            n_tflat = p->n_sub;
         } else {
             n_tflat = (ny + kTS - 1) / kTS;
         }
@@ -1309,7 +1429,7 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         {
             PLAN_ALLOC(used, n_tflat);
             PLAN_STEP(hipMemsetAsync(used, 0, n_tflat, s));
-            hipLaunchKernelGGL(k_tp_tmark, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, k, n,
+            hipLaunchKernelGGL(k_tp_tmark, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, pg, n,
                                ny, nx, gtx, used);
             PLAN_STEP(hipGetLastError());
             std::vector<uint8_t> h_used(n_tflat);
@@ -1351,7 +1471,7 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         PLAN_ALLOC(p->idx, n * 4);
         PLAN_ALLOC(d_on, sizeof(unsigned long long));
         hipLaunchKernelGGL(k_tp_keys, dim3(grid_of(n, kPlanBlock, INT32_MAX)), dim3(kPlanBlock),
-                           0, s, pixel, k, n, ny, nx, gtx, tile_of, n_t, t_block,
+                           0, s, pixel, pg, n, ny, nx, gtx, tile_of, n_t, t_block,
                            (int32_t)n_tiles, off_key, keys_in, vals_in);
         PLAN_STEP(hipGetLastError());
         PLAN_STEP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, vals_in,
@@ -1395,7 +1515,7 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         PLAN_ALLOC(p->sspin, p->n_on * p->ns * sizeof(float));
         if (p->n_on > 0) {
             hipLaunchKernelGGL(k_tp_fill, dim3(grid_of(p->n_on)), dim3(kPlanBlock), 0, s, pixel,
-                               k, spin, nc, c0, p->idx, p->n_on, ny, nx, gtx, tile_of,
+                               pg, spin, nc, c0, p->idx, p->n_on, ny, nx, gtx, tile_of,
                                p->tslot, p->sspin);
             PLAN_STEP(hipGetLastError());
         }
@@ -1452,9 +1572,20 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
                                      hipMemcpyHostToDevice, s));
         // flat pixel of every slot
         PLAN_ALLOC(p->tile_pix, n_tiles * kTS * 4);
-        hipLaunchKernelGGL(k_tp_tilepix, dim3(grid_of(n_tiles * kTS)), dim3(kPlanBlock), 0, s, k,
-                           ny, nx, gtx, tile_flat, n_tiles * kTS, p->tile_pix);
+        if (k == 3) PLAN_ALLOC(p->tile_map, n_tiles * 4);
+        hipLaunchKernelGGL(k_tp_tilepix, dim3(grid_of(n_tiles * kTS)), dim3(kPlanBlock), 0, s, pg,
+                           ny, nx, gtx, tile_flat, n_tiles * kTS, p->tile_pix, p->tile_map);
         PLAN_STEP(hipGetLastError());
+        if (k == 3) {
+            // the map tiles the samples touch, checked against each call's tiles
+            std::vector<int32_t> h_map(n_tiles, 0);
+            if (p->n_tiles > 0)
+                PLAN_STEP(hipMemcpyAsync(h_map.data(), p->tile_map, p->n_tiles * 4,
+                                         hipMemcpyDeviceToHost, s));
+            PLAN_STEP(hipStreamSynchronize(s));
+            p->tile_used.assign(p->n_tile_map, 0);
+            for (int64_t t = 0; t < p->n_tiles; t++) p->tile_used[h_map[t]] = 1;
+        }
         // samples off the map (zeroed by a fresh from_map)
         p->n_off = n - p->n_on;
         PLAN_ALLOC(p->off_idx, p->n_off * 4);
@@ -1464,7 +1595,7 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
         if (g_plan_time_copy) {
             PLAN_ALLOC(p->tpix, n * sizeof(int32_t));
             PLAN_ALLOC(p->tspin, n * p->ns * sizeof(float));
-            hipLaunchKernelGGL(k_plan_tcopy, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, k,
+            hipLaunchKernelGGL(k_plan_tcopy, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, pg,
                                spin, nc, c0, n, ny, nx, bitmap, wscan, p->tpix, p->tspin);
             PLAN_STEP(hipGetLastError());
             PLAN_ALLOC(p->packed, std::max<int64_t>(1, p->n_vis) * 4 * sizeof(double));
@@ -1501,7 +1632,7 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
     PLAN_ALLOC(p->idx, n * 4);
     PLAN_ALLOC(d_on, sizeof(unsigned long long));
     hipLaunchKernelGGL(k_plan_keys, dim3(grid_of(n, kPlanBlock, INT32_MAX)), dim3(kPlanBlock), 0,
-                       s, pixel, k, n, ny, nx, n_t, t_block, n_vis, off_key, bitmap, wscan,
+                       s, pixel, pg, n, ny, nx, n_t, t_block, n_vis, off_key, bitmap, wscan,
                        keys_in, vals_in);
     PLAN_STEP(hipGetLastError());
     PLAN_STEP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, vals_in, p->idx,
@@ -1577,7 +1708,7 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
     // 5. time-ordered copy for from_map, and the packed map buffer
     PLAN_ALLOC(p->tpix, n * sizeof(int32_t));
     PLAN_ALLOC(p->tspin, n * p->ns * sizeof(float));
-    hipLaunchKernelGGL(k_plan_tcopy, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, k, spin, nc,
+    hipLaunchKernelGGL(k_plan_tcopy, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, pg, spin, nc,
                        c0, n, ny, nx, bitmap, wscan, p->tpix, p->tspin);
     PLAN_STEP(hipGetLastError());
     PLAN_ALLOC(p->packed, std::max<int64_t>(1, p->n_vis) * 4 * sizeof(double));
@@ -1591,18 +1722,126 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
     return SO3G_OK;
 }
 
+// Arguments shared by both entry points.
+int plan_check_args(int32_t nc, int64_t n_det, int64_t n_t, so3g_pc_plan **out) {
+    if (out == nullptr) return set_error(SO3G_ERR_VALUE, "out is NULL");
+    *out = nullptr;
+    if (nc < 1 || nc > 3) return set_error(SO3G_ERR_RUNTIME, "spin component count must be 1..3");
+    if (n_det <= 0 || n_t <= 0 || n_det * n_t > (int64_t)UINT32_MAX)
+        return set_error(SO3G_ERR_RUNTIME, "plan needs 0 < n_det * n_t < 2^32");
+    if (n_det >= (1 << 22)) return set_error(SO3G_ERR_RUNTIME, "plan needs n_det < 2^22");
+    return SO3G_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int32_t nc,
+                        int64_t n_det, int64_t n_t, const int64_t *map_shape,
+                        int64_t t_block_mb, so3g_pc_plan **out, void *stream) {
+    if (int rc = plan_check_args(nc, n_det, n_t, out)) return rc;
+    if (k != 1 && k != 2)
+        return set_error(SO3G_ERR_RUNTIME, "plans take k = 1 or 2 pixel indices here "
+                                           "(tiled maps: so3g_pc_plan_create_tiled)");
+    const int64_t ny = map_shape[0], nx = map_shape[1];
+    if (ny <= 0 || nx <= 0 || ny * nx >= INT32_MAX || (k == 1 && nx != 1))
+        return set_error(SO3G_ERR_RUNTIME, "map too large for a plan, or not (npix, 1) for k = 1");
+    so3g_pc_plan *p = new so3g_pc_plan();
+    return plan_create_impl(pixel, PixGeom{k, 0, nullptr, nullptr, nullptr}, spin, nc, n_det, n_t,
+                            ny, nx, t_block_mb, p, out, (hipStream_t)stream);
+}
+
+int so3g_pc_plan_create_tiled(const int32_t *pixel, const float *spin, int32_t nc,
+                              int64_t n_det, int64_t n_t, const int32_t *tile_dims,
+                              int32_t n_tile, int64_t t_block_mb, so3g_pc_plan **out,
+                              void *stream) {
+    if (int rc = plan_check_args(nc, n_det, n_t, out)) return rc;
+    if (tile_dims == nullptr || n_tile <= 0)
+        return set_error(SO3G_ERR_VALUE, "tiled plan needs tile_dims for n_tile > 0 tiles");
+    std::vector<int64_t> tbase(n_tile + 1, 0);
+    std::vector<int32_t> sbase(n_tile + 1, 0);
+    for (int32_t t = 0; t < n_tile; t++) {
+        const int64_t ty = tile_dims[2 * t], tx = tile_dims[2 * t + 1];
+        if (ty < 0 || tx < 0) return set_error(SO3G_ERR_VALUE, "negative tile shape");
+        tbase[t + 1] = tbase[t] + ty * tx;
+        sbase[t + 1] = (int32_t)(sbase[t] + ((ty + (1 << kTileBY) - 1) >> kTileBY) *
+                                                ((tx + (1 << kTileBX) - 1) >> kTileBX));
+        if (tbase[t + 1] >= INT32_MAX) return set_error(SO3G_ERR_RUNTIME, "map too large for a plan");
+    }
+    if (tbase[n_tile] == 0) return set_error(SO3G_ERR_VALUE, "tiles hold no pixels");
+    hipStream_t s = (hipStream_t)stream;
+    so3g_pc_plan *p = new so3g_pc_plan();
+    p->n_tile_map = n_tile;
+    p->n_sub = sbase[n_tile];
+    auto fail = [&](hipError_t e) {
+        free_plan_arrays(p);
+        delete p;
+        (void)hipGetLastError();
+        return set_error(SO3G_ERR_HIP, std::string("HIP error setting up a tiled plan: ") +
+                                           hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&p->tdims_d, (size_t)n_tile * 8)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&p->tbase_d, (size_t)(n_tile + 1) * 8)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&p->sbase_d, (size_t)(n_tile + 1) * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&p->tiles_d, (size_t)n_tile * 8)) != hipSuccess) return fail(e);
+    if ((e = hipMemcpyAsync(p->tdims_d, tile_dims, (size_t)n_tile * 8, hipMemcpyHostToDevice,
+                            s)) != hipSuccess ||
+        (e = hipMemcpyAsync(p->tbase_d, tbase.data(), (size_t)(n_tile + 1) * 8,
+                            hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(p->sbase_d, sbase.data(), (size_t)(n_tile + 1) * 4,
+                            hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+        return fail(e);
+    return plan_create_impl(pixel, PixGeom{3, n_tile, p->tdims_d, p->tbase_d, p->sbase_d}, spin,
+                            nc, n_det, n_t, tbase[n_tile], 1, t_block_mb, p, out, s);
+}
+
+static int plan_untiled(const so3g_pc_plan *p) {
+    if (p != nullptr && p->n_tile_map > 0)
+        return set_error(SO3G_ERR_RUNTIME, "plan was built for a tiled map: use the *_tiles calls");
+    return SO3G_OK;
+}
+
+// Host array of the map's tile pointers into the plan's device table; every
+// tile the plan's samples touch must be present.
+static int plan_upload_tiles(const so3g_pc_plan *p, const void *const *tiles, hipStream_t s) {
+    if (p == nullptr) return set_error(SO3G_ERR_VALUE, "plan is NULL");
+    if (p->n_tile_map == 0)
+        return set_error(SO3G_ERR_RUNTIME, "plan was built for a non-tiled map");
+    if (tiles == nullptr) return set_error(SO3G_ERR_VALUE, "tiles is NULL");
+    for (int32_t t = 0; t < p->n_tile_map; t++)
+        if (p->tile_used[t] && tiles[t] == nullptr)
+            return set_error(SO3G_ERR_RUNTIME, "Tiling problem (index " + std::to_string(t) +
+                                                   "): the plan's samples hit this tile but "
+                                                   "it is missing.");
+    PLAN_TRY(hipMemcpyAsync(p->tiles_d, tiles, (size_t)p->n_tile_map * sizeof(void *),
+                            hipMemcpyHostToDevice, s));
+    return SO3G_OK;
+}
+
 int so3g_pc_plan_to_map(const so3g_pc_plan *p, double *map, const float *signal,
                         const float *det_weights, void *stream) {
+    if (int rc = plan_untiled(p)) return rc;
     return plan_to_map<float>(p, map, signal, det_weights, stream);
 }
 
 int so3g_pc_plan_to_map_f64(const so3g_pc_plan *p, double *map, const double *signal,
                             const float *det_weights, void *stream) {
+    if (int rc = plan_untiled(p)) return rc;
     return plan_to_map<double>(p, map, signal, det_weights, stream);
 }
 
-int so3g_pc_plan_from_map(const so3g_pc_plan *p, const double *map, float *signal,
-                          int accumulate, void *stream) {
+int so3g_pc_plan_to_map_tiles(const so3g_pc_plan *p, double *const *tiles, const float *signal,
+                              const float *det_weights, void *stream) {
+    if (int rc = plan_upload_tiles(p, (const void *const *)tiles, (hipStream_t)stream)) return rc;
+    return plan_to_map<float>(p, nullptr, signal, det_weights, stream, p->tiles_d);
+}
+
+static int plan_from_map_impl(const so3g_pc_plan *p, const double *map,
+                              const double *const *tiles, float *signal, int accumulate,
+                              void *stream) {
     if (p == nullptr) return set_error(SO3G_ERR_VALUE, "plan is NULL");
     hipStream_t s = (hipStream_t)stream;
     const int64_t plane = p->ny * p->nx;
@@ -1621,7 +1860,7 @@ int so3g_pc_plan_from_map(const so3g_pc_plan *p, const double *map, float *signa
 #define TP_FROM(NC, TC)                                                                     \
     hipLaunchKernelGGL((k_tp_from_map<NC, TC>), gi, dim3(kTBlock), 0, s, items, p->tslot,    \
                        p->sspin, p->tmask, p->thead, p->head_i, p->tile_pix,     \
-                       p->tval, nt, pl, map, signal, accumulate)
+                       p->tval, nt, pl, map, signal, accumulate, tiles, p->tile_map, p->tdims_d)
         if (p->tconst) {
             if (p->nc == 1) TP_FROM(1, true);
             else TP_FROM(3, true);
@@ -1634,7 +1873,14 @@ int so3g_pc_plan_from_map(const so3g_pc_plan *p, const double *map, float *signa
         PLAN_TRY(hipGetLastError());
         return SO3G_OK;
     }
-    switch (p->nc) {
+    if (tiles != nullptr) {
+        const dim3 gp(grid_of(p->n_vis));
+        switch (p->nc) {
+        case 1: hipLaunchKernelGGL(k_plan_pack_tiles<1>, gp, dim3(kPlanBlock), 0, s, tiles, p->tdims_d, p->comp_tile, p->comp2pix, p->n_vis, p->packed); break;
+        case 2: hipLaunchKernelGGL(k_plan_pack_tiles<2>, gp, dim3(kPlanBlock), 0, s, tiles, p->tdims_d, p->comp_tile, p->comp2pix, p->n_vis, p->packed); break;
+        default: hipLaunchKernelGGL(k_plan_pack_tiles<3>, gp, dim3(kPlanBlock), 0, s, tiles, p->tdims_d, p->comp_tile, p->comp2pix, p->n_vis, p->packed); break;
+        }
+    } else switch (p->nc) {
     case 1: hipLaunchKernelGGL(k_plan_pack<1>, dim3(grid_of(p->n_vis)), dim3(kPlanBlock), 0, s, map, plane, p->comp2pix, p->n_vis, p->packed); break;
     case 2: hipLaunchKernelGGL(k_plan_pack<2>, dim3(grid_of(p->n_vis)), dim3(kPlanBlock), 0, s, map, plane, p->comp2pix, p->n_vis, p->packed); break;
     default: hipLaunchKernelGGL(k_plan_pack<3>, dim3(grid_of(p->n_vis)), dim3(kPlanBlock), 0, s, map, plane, p->comp2pix, p->n_vis, p->packed); break;
@@ -1700,6 +1946,18 @@ int so3g_precomp_from_map_f64(const double *map, const int64_t *map_shape,
     }
     PLAN_TRY(hipGetLastError());
     return SO3G_OK;
+}
+
+int so3g_pc_plan_from_map(const so3g_pc_plan *p, const double *map, float *signal,
+                          int accumulate, void *stream) {
+    if (int rc = plan_untiled(p)) return rc;
+    return plan_from_map_impl(p, map, nullptr, signal, accumulate, stream);
+}
+
+int so3g_pc_plan_from_map_tiles(const so3g_pc_plan *p, const double *const *tiles,
+                                float *signal, int accumulate, void *stream) {
+    if (int rc = plan_upload_tiles(p, (const void *const *)tiles, (hipStream_t)stream)) return rc;
+    return plan_from_map_impl(p, nullptr, p->tiles_d, signal, accumulate, stream);
 }
 
 int so3g_pc_plan_info(const so3g_pc_plan *p, int64_t *n_on, int64_t *bytes) {

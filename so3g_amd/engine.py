@@ -761,7 +761,9 @@ class PrecompPlan:
     matrix (CG-style loops).  No counterpart in the reference: built once
     from a ProjEng_Precomp pixel_index (k = 2, non-tiled flat map of
     `map_shape` = (ny, nx); or k = 1, a non-tiled HEALPix map of `map_shape`
-    = (npix,)) and spin_proj (both copied into the plan).  It numbers the
+    = (npix,); or k = 3, a tiled flat map, with `tile_dims` the (n_tile, 2)
+    tile shapes, maps then given as tile lists) and spin_proj (both copied
+    into the plan).  It numbers the
     visited pixels compactly and sorts the on-map samples so that to_map
     needs no per-sample atomics: layout 1 (the default from 2^25 samples up;
     layout=None chooses by size) sorts them by map
@@ -776,20 +778,29 @@ class PrecompPlan:
     time_copy=False: a tiled plan without the time-ordered copy (from_map
     then gathers from the tiles; 7.4 vs 16 GB for 7.2e8 samples)."""
 
-    def __init__(self, pixel_index, spin_proj, map_shape, t_block_mb=None, layout=None,
-                 time_copy=None):
+    def __init__(self, pixel_index, spin_proj, map_shape=None, t_block_mb=None, layout=None,
+                 time_copy=None, tile_dims=None):
         import ctypes
         call = _Call()
         pix = call.array(pixel_index, 'int32', 'pixel_index', spec=[-1, -1, -1])
         n_det, n_t, k = pix.shape
-        if k not in (1, 2) or len(map_shape) != k:
-            raise ValueError('plans take k = 2 pixels with map_shape (ny, nx) or '
-                             'k = 1 pixels with map_shape (npix,)')
         spin = call.array(spin_proj, 'float32', 'spin_proj', spec=[n_det, n_t, -1])
         self.n_det, self.n_t, self.k = int(n_det), int(n_t), int(k)
         self.nc = int(spin.shape[2])
-        self.map_shape = tuple(int(x) for x in map_shape)
-        self._shape = np.array(self.map_shape + (1,) * (2 - k), np.int64)
+        if k == 3:
+            # tiled map: pixel = (tile, row, column); tile_dims (n_tile, 2)
+            if tile_dims is None:
+                raise ValueError('plans for k = 3 (tiled) pixels need tile_dims')
+            self.tile_dims = np.ascontiguousarray(tile_dims, np.int32).reshape(-1, 2)
+            self.map_shape = None
+        else:
+            if k not in (1, 2) or map_shape is None or len(map_shape) != k:
+                raise ValueError('plans take k = 2 pixels with map_shape (ny, nx), k = 1 '
+                                 'pixels with map_shape (npix,) or k = 3 pixels with '
+                                 'tile_dims')
+            self.tile_dims = None
+            self.map_shape = tuple(int(x) for x in map_shape)
+            self._shape = np.array(self.map_shape + (1,) * (2 - k), np.int64)
         h = ctypes.c_void_p()
         opts = {'plan_layout': layout, 'plan_time_copy': time_copy}
         old = {n: N.get_option(n) for n in opts}
@@ -797,10 +808,17 @@ class PrecompPlan:
             for n, v in opts.items():
                 if v is not None:
                     N.set_option(n, int(v))
-            N.check(N.load().so3g_pc_plan_create(pix.data_ptr(), k, spin.data_ptr(), self.nc,
-                                                 self.n_det, self.n_t, self._shape.ctypes.data,
-                                                 -1 if t_block_mb is None else int(t_block_mb),
-                                                 ctypes.byref(h), _stream()))
+            tb = -1 if t_block_mb is None else int(t_block_mb)
+            if k == 3:
+                N.check(N.load().so3g_pc_plan_create_tiled(
+                    pix.data_ptr(), spin.data_ptr(), self.nc, self.n_det, self.n_t,
+                    self.tile_dims.ctypes.data, len(self.tile_dims), tb, ctypes.byref(h),
+                    _stream()))
+            else:
+                N.check(N.load().so3g_pc_plan_create(pix.data_ptr(), k, spin.data_ptr(),
+                                                     self.nc, self.n_det, self.n_t,
+                                                     self._shape.ctypes.data, tb,
+                                                     ctypes.byref(h), _stream()))
         finally:
             for n, v in old.items():
                 N.set_option(n, v)
@@ -820,14 +838,34 @@ class PrecompPlan:
                 't_block': a[4], 'bytes': a[5], 'n_vis': a[6], 'layout': a[7],
                 'stretches': a[8], 'items': a[9]}
 
+    def _tiles(self, call, map, writeback):
+        """Host array of the tile list's device pointers (0 for None)."""
+        if not isinstance(map, (list, tuple)) or len(map) != len(self.tile_dims):
+            raise RuntimeError(f"Buffer 'map' has incompatible shape: expected a list of "
+                               f"{len(self.tile_dims)} tiles.")
+        ptrs = np.zeros(len(map), np.int64)
+        for i, tile in enumerate(map):
+            if tile is not None:
+                spec = [self.nc] + [int(x) for x in self.tile_dims[i]]
+                ptrs[i] = call.array(tile, 'float64', 'map', spec=spec, out=writeback).data_ptr()
+        call.keep.append(ptrs)
+        return ptrs.ctypes.data
+
     def to_map(self, map, signal, det_weights=None):
+        """map += P^T signal; map is an (nc, ...) array, or a list of tiles
+        (None for inactive ones) for a k = 3 plan."""
         call = _Call()
-        m = call.array(map, 'float64', 'map', spec=[self.nc] + list(self.map_shape), out=True)
         sig = call.array(signal, 'float32', 'signal', spec=[self.n_det, self.n_t])
         dw = call.array(det_weights, 'float32', 'det_weights', spec=[self.n_det],
                         optional=True)
-        N.check(N.load().so3g_pc_plan_to_map(self._h, m.data_ptr(), sig.data_ptr(), _ptr(dw),
-                                             _stream()))
+        if self.k == 3:
+            N.check(N.load().so3g_pc_plan_to_map_tiles(self._h, self._tiles(call, map, True),
+                                                       sig.data_ptr(), _ptr(dw), _stream()))
+        else:
+            m = call.array(map, 'float64', 'map', spec=[self.nc] + list(self.map_shape),
+                           out=True)
+            N.check(N.load().so3g_pc_plan_to_map(self._h, m.data_ptr(), sig.data_ptr(),
+                                                 _ptr(dw), _stream()))
         call.finish()
         return map
 
@@ -835,11 +873,16 @@ class PrecompPlan:
         """signal (+)= P map; a given signal is accumulated into, as
         ProjEng_Precomp.from_map does; None returns fresh rows."""
         call = _Call()
-        m = call.array(map, 'float64', 'map', spec=[self.nc] + list(self.map_shape))
         sig, obj, fresh = _signal_space(call, signal, 'signal', 'float32', self.n_det,
                                         self.n_t)
-        N.check(N.load().so3g_pc_plan_from_map(self._h, m.data_ptr(), sig.data_ptr(),
-                                               0 if fresh else 1, _stream()))
+        if self.k == 3:
+            N.check(N.load().so3g_pc_plan_from_map_tiles(self._h, self._tiles(call, map, False),
+                                                         sig.data_ptr(), 0 if fresh else 1,
+                                                         _stream()))
+        else:
+            m = call.array(map, 'float64', 'map', spec=[self.nc] + list(self.map_shape))
+            N.check(N.load().so3g_pc_plan_from_map(self._h, m.data_ptr(), sig.data_ptr(),
+                                                   0 if fresh else 1, _stream()))
         call.finish()
         return _signal_result(call, sig, obj)
 

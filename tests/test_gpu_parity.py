@@ -525,6 +525,53 @@ def test_precomp_plan_edges(gpu, case, layout, tcopy):
         assert np.array_equal(g, m0)
 
 
+@pytest.mark.parametrize('tile_shape', [(100, 100), (30, 70)])
+@pytest.mark.parametrize('tcopy', [1, 0], ids=['tcopy', 'notcopy'])
+def test_precomp_plan_tiled_map(gpu, tile_shape, tcopy):
+    """PrecompPlan on a tiled flat map (k = 3 pixel indices, maps as tile
+    lists with inactive tiles None): to_map == ProjEng_Precomp_Tiled.to_map
+    (oracle), from_map bit for bit, tiles larger than one 64 x 64 plan
+    sub-block and ragged edge tiles; a hit tile passed as None raises
+    RuntimeError, as the reference does."""
+    import so3g_amd
+    c = Case('CAR', comps='TQU', n_det=30, n_samp=4000, ny=150, nx=260)
+    ty, tx = tile_shape
+    n_tile = ((150 + ty - 1) // ty) * ((260 + tx - 1) // tx)
+    probe = so3g_amd.ProjEng_CAR_TQU_Tiled(c.pixargs[:6] + (ty, tx))
+    hits = np.asarray(probe.tile_hits(c.bore, c.dets))
+    active = [i for i in range(n_tile) if hits[i] > 0]
+    assert 0 < len(active) < n_tile or len(active) == n_tile
+    args = c.pixargs[:6] + (ty, tx, active)
+    eng = so3g_amd.ProjEng_CAR_TQU_Tiled(args)
+    og = oracle.geom_from_args('CAR', 'TQU', args)
+    pix, spin = [np.array(x) for x in eng.pointing_matrix(c.bore, c.dets, c.resps)]
+    assert pix.shape[2] == 3
+    pix[4, 200:900] = -1
+    dims = np.asarray(eng._tile_dims())
+    plan = so3g_amd.PrecompPlan(pix, spin, tile_dims=dims, time_copy=tcopy)
+    info = plan.info()
+    assert info['layout'] == 1 and info['n_on'] == int((pix[..., 0] >= 0).sum())
+    dw = c.det_weights.copy()
+    dw[2] = 0.
+    for w in (None, dw):
+        g = plan.to_map(eng.zeros((3,)), c.signal, w)
+        r = oracle.precomp_to_map(og, eng.zeros((3,)), pix, spin, c.signal, w)
+        maps_close(g, r, f'tiled-map plan to_map {tile_shape}', atol=1e-9)
+    rng = np.random.default_rng(3)
+    m = [None if t is None else rng.standard_normal(t.shape) for t in eng.zeros((3,))]
+    assert_bit_equal(np.array(plan.from_map(m)), oracle.precomp_from_map(og, m, pix, spin),
+                     'tiled-map plan from_map')
+    s0 = np.array(c.signal)
+    assert_bit_equal(np.array(plan.from_map(m, s0.copy())),
+                     oracle.precomp_from_map(og, m, pix, spin, s0.copy()),
+                     'tiled-map plan from_map accumulate')
+    m[active[0]] = None
+    with pytest.raises(RuntimeError):
+        plan.from_map(m)
+    with pytest.raises(RuntimeError):
+        plan.to_map(m, c.signal)
+
+
 @pytest.mark.parametrize('layout,tcopy', [(0, 1), (1, 1), (1, 0)],
                          ids=['runs', 'tiles', 'tiles-notcopy'])
 @pytest.mark.parametrize('nest,nside,t_block_mb', [(True, 512, 64), (False, 4096, 1),
