@@ -459,6 +459,8 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   tables (T factor, weights) in LDS when they fit in 32 KB
  *   "plan_to_cpb"   2048-sample chunks per block of the planned scatter
  *                   (default 2, 1..64)
+ *   "nmat_dchunk"   detectors per block of nmat_detvecs_apply's passes
+ *                   (default 64, 1..65536)
  *   "plan_layout"   layout of plans created afterwards: 1 map tiles summed
  *                   in LDS, 0 pixel runs, -1 (default) tiles from 2^25
  *                   samples up

@@ -24,6 +24,7 @@ extern int g_plan_to_cpb;
 extern int g_plan_layout;
 extern int64_t g_plan_tile_item;
 extern int g_plan_time_copy;
+extern int g_nmat_dchunk;
 extern template int dispatch<PROJ_FLAT>(const OpArgs &);
 extern template int dispatch<PROJ_QUAT>(const OpArgs &);
 extern template int dispatch<PROJ_CAR>(const OpArgs &);
@@ -750,6 +751,10 @@ int so3g_set_option(const char *name, int64_t value) {
         so3g::g_plan_layout = (int)value;
     }
     else if (n == "plan_time_copy") so3g::g_plan_time_copy = (int)(value != 0);
+    else if (n == "nmat_dchunk") {
+        if (value < 1 || value > 65536) return fail(SO3G_ERR_VALUE, "nmat_dchunk must be 1..65536");
+        so3g::g_nmat_dchunk = (int)value;
+    }
     else if (n == "plan_tile_item") {
         if (value < 64 || value > (1 << 30))
             return fail(SO3G_ERR_VALUE, "plan_tile_item must be 64..2^30");
@@ -781,6 +786,7 @@ int so3g_get_option(const char *name, int64_t *value) {
     else if (n == "plan_layout") *value = so3g::g_plan_layout;
     else if (n == "plan_tile_item") *value = so3g::g_plan_tile_item;
     else if (n == "plan_time_copy") *value = so3g::g_plan_time_copy;
+    else if (n == "nmat_dchunk") *value = so3g::g_nmat_dchunk;
     else if (n == "plan_from_v") *value = so3g::g_plan_from_v;
     else return fail(SO3G_ERR_VALUE, "unknown option " + n);
     return SO3G_OK;

@@ -53,14 +53,22 @@ __device__ __forceinline__ void load_cols(const float *row, int64_t col0, int nc
     for (int j = 0; j < C; j++) x[j] = j < nc ? row[col0 + j] : 0.f;
 }
 
-constexpr int kDChunk = 64;     // detectors per block (blockIdx.y)
+}  // namespace
+
+namespace so3g {
+// Detectors per block of both passes ("nmat_dchunk"): pass 1 writes one
+// partial Q per chunk of detectors, which pass 1's reduction then sums.
+int g_nmat_dchunk = 64;
+}  // namespace so3g
+
+namespace {
 
 template <int NV, int C>
 __global__ __launch_bounds__(kNT) void k_nmat_q(const float *__restrict__ ft, int ndet,
                                                 int64_t nmode, const int32_t *__restrict__ cbin,
                                                 const float *__restrict__ iV, int nvec,
-                                                float *__restrict__ Qp) {
-    const int d0 = blockIdx.y * kDChunk, d1 = min(ndet, d0 + kDChunk);
+                                                float *__restrict__ Qp, int dchunk) {
+    const int d0 = blockIdx.y * dchunk, d1 = min(ndet, d0 + dchunk);
     float *Q = Qp + (int64_t)blockIdx.y * nvec * nmode;   // this chunk's partial
     const int64_t col0 = ((int64_t)blockIdx.x * kNT + threadIdx.x) * C;
     if (col0 >= nmode) return;
@@ -132,13 +140,55 @@ __global__ __launch_bounds__(kNT) void k_nmat_reduce(const float *__restrict__ Q
 }
 
 template <int NV, int C>
+__device__ __forceinline__ void update_row(const float (&x)[C], float (&o)[C], const int (&b)[C],
+                                           bool uni, int bw, int d, int ndet,
+                                           const float *__restrict__ iD,
+                                           const float *__restrict__ iV, int nvec, float norm,
+                                           float a, const float (&q)[NV][C]) {
+    if (uni) {
+        const float *vr = iV + ((int64_t)bw * ndet + d) * nvec;
+        const float dn = iD[(int64_t)bw * ndet + d] / norm;
+        float acc[C];
+#pragma unroll
+        for (int j = 0; j < C; j++) acc[j] = 0.f;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            if (v < nvec) {
+                const float r = vr[v];
+#pragma unroll
+                for (int j = 0; j < C; j++) acc[j] += r * q[v][j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < C; j++) o[j] = x[j] * dn + a * acc[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < C; j++) {
+            if (b[j] < 0) { o[j] = x[j]; continue; }
+            const float *vr = iV + ((int64_t)b[j] * ndet + d) * nvec;
+            float acc = 0.f;
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                if (v < nvec) acc += vr[v] * q[v][j];
+            o[j] = x[j] * (iD[(int64_t)b[j] * ndet + d] / norm) + a * acc;
+        }
+    }
+}
+
+// Pass 2 walks its detector rows kRU at a time and loads the next group's
+// rows before storing the current group's: a load's wait (vmcnt) also
+// covers the wave's older stores, so loads issued after the stores would
+// wait for them.
+constexpr int kRU = 4;
+
+template <int NV, int C>
 __global__ __launch_bounds__(kNT) void k_nmat_update(float *__restrict__ ft, int ndet,
                                                      int64_t nmode,
                                                      const int32_t *__restrict__ cbin,
                                                      const float *__restrict__ iD,
                                                      const float *__restrict__ iV, int nvec,
                                                      float s, float norm,
-                                                     const float *__restrict__ Q) {
+                                                     const float *__restrict__ Q, int dchunk) {
     const int64_t col0 = ((int64_t)blockIdx.x * kNT + threadIdx.x) * C;
     if (col0 >= nmode) return;
     const int nc = (int)min((int64_t)C, nmode - col0);
@@ -161,50 +211,37 @@ __global__ __launch_bounds__(kNT) void k_nmat_update(float *__restrict__ ft, int
         for (int j = 0; j < C; j++)
             q[v][j] = (v < nvec && j < nc) ? Q[(int64_t)v * nmode + col0 + j] : 0.f;
     const float a = s / norm;
-    const int d0 = blockIdx.y * kDChunk, d1 = min(ndet, d0 + kDChunk);
-#pragma unroll 4
-    for (int d = d0; d < d1; d++) {
-        float *row = ft + (int64_t)d * nmode;
-        float x[C];
-        load_cols<NV, C>(row, col0, nc, vec, x);
-        float o[C];
-        if (uni) {
-            const float *vr = iV + ((int64_t)bw * ndet + d) * nvec;
-            const float dn = iD[(int64_t)bw * ndet + d] / norm;
-            float acc[C];
+    const int d0 = blockIdx.y * dchunk, d1 = min(ndet, d0 + dchunk);
+    float x[kRU][C], xn[kRU][C];
 #pragma unroll
-            for (int j = 0; j < C; j++) acc[j] = 0.f;
+    for (int r = 0; r < kRU; r++)
+        if (d0 + r < d1) load_cols<NV, C>(ft + (int64_t)(d0 + r) * nmode, col0, nc, vec, x[r]);
+    for (int d = d0; d < d1; d += kRU) {
+        // next group's loads first
 #pragma unroll
-            for (int v = 0; v < NV; v++) {
-                if (v < nvec) {
-                    const float r = vr[v];
+        for (int r = 0; r < kRU; r++)
+            if (d + kRU + r < d1)
+                load_cols<NV, C>(ft + (int64_t)(d + kRU + r) * nmode, col0, nc, vec, xn[r]);
 #pragma unroll
-                    for (int j = 0; j < C; j++) acc[j] += r * q[v][j];
+        for (int r = 0; r < kRU; r++) {
+            if (d + r >= d1) break;
+            float *row = ft + (int64_t)(d + r) * nmode;
+            float o[C];
+            update_row<NV, C>(x[r], o, b, uni, bw, d + r, ndet, iD, iV, nvec, norm, a, q);
+            if constexpr (C == 4) {
+                if (vec && nc == 4) {
+                    *reinterpret_cast<float4 *>(row + col0) = float4{o[0], o[1], o[2], o[3]};
+                    continue;
                 }
             }
 #pragma unroll
-            for (int j = 0; j < C; j++) o[j] = x[j] * dn + a * acc[j];
-        } else {
-#pragma unroll
-        for (int j = 0; j < C; j++) {
-            if (b[j] < 0) { o[j] = x[j]; continue; }
-            const float *vr = iV + ((int64_t)b[j] * ndet + d) * nvec;
-            float acc = 0.f;
-#pragma unroll
-            for (int v = 0; v < NV; v++)
-                if (v < nvec) acc += vr[v] * q[v][j];
-            o[j] = x[j] * (iD[(int64_t)b[j] * ndet + d] / norm) + a * acc;
-        }
-        }
-        if constexpr (C == 4) {
-            if (vec && nc == 4) {
-                *reinterpret_cast<float4 *>(row + col0) = float4{o[0], o[1], o[2], o[3]};
-                continue;
-            }
+            for (int j = 0; j < C; j++)
+                if (j < nc && b[j] >= 0) row[col0 + j] = o[j];
         }
 #pragma unroll
-        for (int j = 0; j < C; j++)
-            if (j < nc && b[j] >= 0) row[col0 + j] = o[j];
+        for (int r = 0; r < kRU; r++)
+#pragma unroll
+            for (int j = 0; j < C; j++) x[r][j] = xn[r][j];
     }
 }
 
@@ -231,18 +268,18 @@ std::mutex g_nm_mutex;
 template <int NV, int C>
 void launch_pair(float *ft, int ndet, int64_t nmode, const int32_t *cbin, const float *iD,
                  const float *iV, int nvec, float s, float norm, float *Qp, float *Q,
-                 hipStream_t st) {
-    const int nchunk = (ndet + kDChunk - 1) / kDChunk;
+                 int dchunk, hipStream_t st) {
+    const int nchunk = (ndet + dchunk - 1) / dchunk;
     const dim3 g((unsigned)((nmode + (int64_t)kNT * C - 1) / ((int64_t)kNT * C)),
                  (unsigned)nchunk);
     hipLaunchKernelGGL((k_nmat_q<NV, C>), g, dim3(kNT), 0, st, ft, ndet, nmode, cbin, iV, nvec,
-                       Qp);
+                       Qp, dchunk);
     const int64_t n = (int64_t)nvec * nmode;
     if (n > 0)
         hipLaunchKernelGGL(k_nmat_reduce, dim3((unsigned)std::min<int64_t>((n + kNT - 1) / kNT, 65536)),
                            dim3(kNT), 0, st, Qp, nchunk, n, cbin, nmode, Q);
     hipLaunchKernelGGL((k_nmat_update<NV, C>), g, dim3(kNT), 0, st, ft, ndet, nmode, cbin, iD,
-                       iV, nvec, s, norm, Q);
+                       iV, nvec, s, norm, Q, dchunk);
 }
 
 }  // namespace
@@ -298,7 +335,8 @@ int so3g_nmat_detvecs_apply(float *ft, int64_t ndet, int64_t nmode, const int32_
         g_nm_cache.key = key;
         g_nm_cache.nmode = nmode;
     }
-    const int64_t nchunk = (ndet + kDChunk - 1) / kDChunk;
+    const int dchunk = std::max(1, so3g::g_nmat_dchunk);
+    const int64_t nchunk = (ndet + dchunk - 1) / dchunk;
     const size_t need = (size_t)std::max<int64_t>(1, (nchunk + 1) * nvec * nmode);
     if (need > g_nm_q_cap) {
         // earlier calls on other streams may still read the old buffer
@@ -311,16 +349,18 @@ int so3g_nmat_detvecs_apply(float *ft, int64_t ndet, int64_t nmode, const int32_
     }
     float *Q = g_nm_q, *Qp = g_nm_q + nvec * nmode;
     int rc = SO3G_OK;
-    const int nv = nvec <= 4 ? 4 : nvec <= 8 ? 8 : nvec <= 16 ? 16 : nvec <= 32 ? 32 : 64;
+    const int nv = nvec <= 4 ? 4 : nvec <= 8 ? 8 : nvec <= 12 ? 12 : nvec <= 16 ? 16
+                                                                  : nvec <= 32 ? 32 : 64;
     for (size_t gi = 0; gi < groups.size(); gi++) {
         const int32_t *cbin = g_nm_cache.dev[gi];
         const int nd = (int)ndet, nvi = (int)nvec;
         switch (nv) {
-        case 4: launch_pair<4, 4>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, st); break;
-        case 8: launch_pair<8, 4>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, st); break;
-        case 16: launch_pair<16, 4>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, st); break;
-        case 32: launch_pair<32, 2>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, st); break;
-        default: launch_pair<64, 1>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, st); break;
+        case 4: launch_pair<4, 4>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, dchunk, st); break;
+        case 8: launch_pair<8, 4>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, dchunk, st); break;
+        case 12: launch_pair<12, 4>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, dchunk, st); break;
+        case 16: launch_pair<16, 4>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, dchunk, st); break;
+        case 32: launch_pair<32, 2>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, dchunk, st); break;
+        default: launch_pair<64, 1>(ft, nd, nmode, cbin, iD, iV, nvi, s, norm, Qp, Q, dchunk, st); break;
         }
         if (hipGetLastError() != hipSuccess) {
             rc = set_error(SO3G_ERR_HIP, "nmat kernel launch failed");

@@ -186,12 +186,13 @@ def test_plan_options():
     """Plan tunables documented in include/so3g_hip.h: defaults, round trip,
     and rejection of values outside their ranges."""
     defaults = {'plan_layout': -1, 'plan_tile_item': 262144, 'plan_time_copy': 1,
-                'plan_to_lds': 1, 'plan_to_cpb': 2, 'plan_from_layout': 1, 'plan_from_v': 4}
+                'plan_to_lds': 1, 'plan_to_cpb': 2, 'plan_from_layout': 1, 'plan_from_v': 4,
+                'nmat_dchunk': 64}
     for name, v in defaults.items():
         assert _native.get_option(name) == v, name
     for name, good, bad in (('plan_layout', 0, 2), ('plan_layout', 1, -2),
                             ('plan_tile_item', 4096, 63), ('plan_to_cpb', 5, 0),
-                            ('plan_from_v', 8, 5)):
+                            ('plan_from_v', 8, 5), ('nmat_dchunk', 128, 0)):
         old = _native.get_option(name)
         try:
             _native.set_option(name, good)
