@@ -63,25 +63,68 @@ __device__ __forceinline__ int xcc_id() {
 // true for that lane.  A detector crossing a map pixel stays in it for a
 // few consecutive samples, so this divides the number of memory atomics by
 // the mean run length.  All 64 lanes must call this together.
+// One-lane shifts across the wave by DPP (GFX9 whole-wave shifts): the
+// previous lane's value (lane 0 gets `old`) and the next lane's (lane 63
+// gets `old`).
+__device__ __forceinline__ int dpp_prev(int v, int old) {
+    return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int dpp_next(int v, int old) {
+    return __builtin_amdgcn_update_dpp(old, v, 0x130, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int64_t dpp_prev64(int64_t v) {
+    const int lo = dpp_prev((int)(v & 0xffffffffll), 0), hi = dpp_prev((int)(v >> 32), 0);
+    return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+__device__ __forceinline__ int64_t dpp_next64(int64_t v) {
+    const int lo = dpp_next((int)(v & 0xffffffffll), 0), hi = dpp_next((int)(v >> 32), 0);
+    return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+template <typename VT>
+__device__ __forceinline__ VT dpp_prev_val(VT v) {
+    if constexpr (sizeof(VT) == 8) {
+        return __longlong_as_double(dpp_prev64(__double_as_longlong(v)));
+    } else {
+        return __int_as_float(dpp_prev(__float_as_int(v), 0));
+    }
+}
+
+// Wave-level run aggregation for scatter-adds.  The 64 lanes of a wave hold
+// (key, v[NC]) for consecutive samples; a key < 0 marks a lane with nothing
+// to add.  Lanes in a run of equal keys sum their values (segmented
+// inclusive scan, Hillis-Steele: at step `off` a lane adds the value `off`
+// lanes back when that lane is in its run) so that the run's last lane
+// holds the run total; returns true for that lane.  A detector crossing a
+// map pixel stays in it for a few consecutive samples, so this divides the
+// number of memory atomics by the mean run length.  Run starts come from one
+// ballot, one-lane shifts from DPP, and the scan stops after the last step
+// in which some lane still adds (runs are short: the steps skipped would add
+// nothing, so the sums are those of the full scan).  All 64 lanes must call
+// this together.
 template <int NC, typename VT>
 __device__ __forceinline__ bool wave_runs(int64_t key, VT (&v)[NC]) {
     const int lane = (int)__lane_id();
-    const int64_t kprev = __shfl_up(key, 1);
-    int head = (lane == 0 || kprev != key) ? lane : 0;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int h = __shfl_up(head, off);
-        if (lane >= off) head = max(head, h);
-    }
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
+    const int64_t kprev = dpp_prev64(key);
+    const uint64_t starts = __ballot(lane == 0 || kprev != key);
+    const int head = 63 - __clzll((long long)(starts & (~0ull >> (63 - lane))));
+    const int rank = lane - head;
+    if (__any(rank >= 1)) {
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            const VT u = __shfl_up(v[c], off);
-            if (lane - off >= head) v[c] += u;
+            const VT u = dpp_prev_val(v[c]);
+            if (rank >= 1) v[c] += u;
+        }
+#pragma unroll
+        for (int off = 2; off < 64; off <<= 1) {
+            if (!__any(rank >= off)) break;
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const VT u = __shfl_up(v[c], off);
+                if (rank >= off) v[c] += u;
+            }
         }
     }
-    const int64_t knext = __shfl_down(key, 1);
+    const int64_t knext = dpp_next64(key);
     return lane == 63 || knext != key;
 }
 
