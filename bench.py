@@ -60,11 +60,13 @@ def log(*a):
 def plan_bytes(pinfo, n_s):
     """Algorithmic bytes of the planned from_map and to_map kernels (as
     described in main) and the to_map kernel's name."""
+    from so3g_amd import _native as N
     n_on, ns, n_vis = pinfo['n_on'], pinfo['stored_spin'], pinfo['n_vis']
     b_from = n_s * (4 + 4 * ns + 4) + 8 * 3 * n_vis
     if pinfo['layout'] == 1:
+        slots = N.get_option('plan_tile_rows') * N.get_option('plan_tile_cols')
         b_to = n_on * (2 + 4 * ns + 4) + 12 * ((n_on + 63) // 64) + 4 * pinfo['stretches'] + \
-            pinfo['items'] * (16 + 4 * 4096) + 2 * 8 * 3 * n_vis
+            pinfo['items'] * (16 + 4 * slots) + 2 * 8 * 3 * n_vis
         return b_from, b_to, 'k_tp_to_map<3,true,float,true>'
     b_to = n_on * (4 + 4 * ns + 4) + n_on // 8 + 4 * pinfo['runs'] + 4 * (n_on // 512) + \
         2 * 8 * 3 * n_vis
@@ -243,7 +245,7 @@ def main():
     #   to_map (tiled layout): per on-map sample 2 B tile slot + 8 B (Q, U)
     #             factors + 4 B signal, plus 12 B per 64 positions (start mask,
     #             start count), 4 B per start (stretches and chunk starts) and
-    #             16 B per work item, the 16 KB slot-to-pixel table of each item's
+    #             16 B per work item, the 8 KB slot-to-pixel table of each item's
     #             tile, and the visited pixels' float64 TQU values read and
     #             written once;
     #   to_map (pixel-run layout): per on-map sample 4 B sample index + 8 B

@@ -214,8 +214,8 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape,
  * in ascending order (compact pixels), so the plan's pixel tables scale with
  * the visited pixels, not the map.  Samples are grouped in time blocks whose
  * signal slice is about t_block_mb MiB (0: one block; < 0: the layout's
- * default), then sorted within each block: by map tile (64 x 64 pixels, or
- * 4096 consecutive HEALPix pixel numbers), keeping sample order, in the tiled layout
+ * default), then sorted within each block: by map tile (32 x 64 pixels, or
+ * 2048 consecutive HEALPix pixel numbers), keeping sample order, in the tiled layout
  * (option "plan_layout" 1, the default from 2^25 samples up; one time
  * block); by pixel in layout 0
  * (64 MiB blocks, sized to stay in the Infinity Cache).  Stored: per on-map
@@ -466,6 +466,8 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   samples up
  *   "plan_tile_item"  sorted positions per work item of the tiled layout
  *                   (default 262144, 64..2^30)
+ *   "plan_tile_rows", "plan_tile_cols"  (read only) the tiled plan's tile
+ *                   shape, a build constant (32 x 64)
  *   "plan_time_copy"  1 (default): tiled plans also keep the time-ordered
  *                   copy for from_map; 0: from_map gathers from the tiles
  *                   (smaller plan, slower gather)

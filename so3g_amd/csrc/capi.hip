@@ -25,6 +25,7 @@ extern int g_plan_layout;
 extern int64_t g_plan_tile_item;
 extern int g_plan_time_copy;
 extern int g_nmat_dchunk;
+extern const int g_plan_tile_rows, g_plan_tile_cols;
 extern template int dispatch<PROJ_FLAT>(const OpArgs &);
 extern template int dispatch<PROJ_QUAT>(const OpArgs &);
 extern template int dispatch<PROJ_CAR>(const OpArgs &);
@@ -787,6 +788,8 @@ int so3g_get_option(const char *name, int64_t *value) {
     else if (n == "plan_tile_item") *value = so3g::g_plan_tile_item;
     else if (n == "plan_time_copy") *value = so3g::g_plan_time_copy;
     else if (n == "nmat_dchunk") *value = so3g::g_nmat_dchunk;
+    else if (n == "plan_tile_rows") *value = so3g::g_plan_tile_rows;
+    else if (n == "plan_tile_cols") *value = so3g::g_plan_tile_cols;
     else if (n == "plan_from_v") *value = so3g::g_plan_from_v;
     else return fail(SO3G_ERR_VALUE, "unknown option " + n);
     return SO3G_OK;

@@ -27,12 +27,13 @@
 //
 // Tiled layout ("plan_layout" 1, the default from 2^25 samples up): the
 // samples are sorted by
-// (time block, map tile) instead -- 64 x 64 pixels of a flat map, or 4096
+// (time block, map tile) instead -- 32 x 64 pixels of a flat map, or 2048
 // consecutive pixel numbers of a HEALPix map -- and keep their sample order
 // inside a tile, so a detector's pass over a tile is a stretch of
 // consecutive samples (~100 for the 1.5 degree scan) and to_map's signal
 // loads are nearly contiguous.  Work items of up to 256 K positions sum into
-// the tile held in LDS and add it to the map once.  Per position: the slot
+// the tile held in LDS (two copies, by lane parity) and add it to the map
+// once.  Per position: the slot
 // in the tile (2 B) and the stored factors; per 64 positions: a stretch-start
 // mask and a start count; per start (stretches and chunk starts): its sample.
 // from_map keeps the time-ordered gather (or, with "plan_time_copy" 0 and a
@@ -91,14 +92,14 @@ struct so3g_pc_plan {
     uint64_t *tmask;       // [n_chunks] stretch starts per 64 positions
     uint32_t *thead;       // [n_chunks] stretch starts before each chunk
     uint32_t *head_i;      // [n_heads] sample of each stretch start
-    int32_t *tile_pix;     // [n_tiles * 4096] flat pixel of each slot, -1 off the map
+    int32_t *tile_pix;     // [n_tiles * kTS] flat pixel of each slot, -1 off the map
     void *items;           // [n_items] work items (first position, count, tile)
     uint32_t *off_idx;     // [n_off] samples off the map
     int64_t n_items, n_tiles, n_heads, n_off;
     // Tiled maps (k = 3 pixel arrays; tiled layout only): pixels are
     // numbered by concatenating the map tiles.
     int32_t n_tile_map;    // map tiles (0: a non-tiled map)
-    int64_t n_sub;         // 64 x 64 sub-blocks over all map tiles
+    int64_t n_sub;         // plan-tile-sized sub-blocks over all map tiles
     int32_t *tdims_d;      // [n_tile_map * 2] tile shapes
     int64_t *tbase_d;      // [n_tile_map + 1] first pixel of each tile
     int32_t *sbase_d;      // [n_tile_map + 1] first sub-block of each tile
@@ -156,7 +157,7 @@ constexpr int kGroup = 64 * kPlanR;       // samples per wave = per head group
 // (iy, ix) pairs on an (ny, nx) map, k = 3 (tile, iy, ix) triples on a tiled
 // map whose tiles are numbered by concatenation (ny = total pixels, nx = 1;
 // tdims: per tile (rows, cols); tbase: first pixel of each tile; sbase: first
-// 64 x 64 sub-block of each tile, n_tile + 1 entries each).
+// plan-tile-sized (32 x 64) sub-block of each tile, n_tile + 1 entries each).
 struct PixGeom {
     int32_t k;
     int32_t n_tile;
@@ -772,9 +773,9 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
 }
 
 // ---------------------------------------------------------------------------
-// Tiled layout.  Samples are sorted by (time block, map tile) -- 64 x 64
-// pixels of a flat map, or 4096 consecutive pixel numbers of a HEALPix map
-// (an aligned 64 x 64 sub-square in NEST order) -- and stay in sample order inside a tile, so a wave's 64 positions fall in
+// Tiled layout.  Samples are sorted by (time block, map tile) -- 32 x 64
+// pixels of a flat map, or 2048 consecutive pixel numbers of a HEALPix map
+// (two aligned 32 x 32 sub-squares in NEST order) -- and stay in sample order inside a tile, so a wave's 64 positions fall in
 // one or two stretches of consecutive samples of one detector and its signal
 // loads are nearly contiguous.  Work items (tile, positions [b, e)) sum into
 // or gather from the tile's values held in LDS.
@@ -790,14 +791,17 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
 // overlap.
 // ---------------------------------------------------------------------------
 #ifndef SO3G_TILE_BITS_Y
-#define SO3G_TILE_BITS_Y 6
+#define SO3G_TILE_BITS_Y 5
 #endif
 #ifndef SO3G_TILE_BITS_X
 #define SO3G_TILE_BITS_X 6
 #endif
-constexpr int kTileBY = SO3G_TILE_BITS_Y;       // 64 x 64 tiles of a flat map
+constexpr int kTileBY = SO3G_TILE_BITS_Y;       // 32 x 64 tiles of a flat map
 constexpr int kTileBX = SO3G_TILE_BITS_X;
 constexpr int kTS = 1 << (kTileBY + kTileBX);  // slots per tile
+// Plan tile shape, read through so3g_get_option ("plan_tile_rows", "_cols").
+extern const int g_plan_tile_rows = 1 << kTileBY;
+extern const int g_plan_tile_cols = 1 << kTileBX;
 constexpr int kTBlock = 1024;                  // threads per work item
 
 struct TItem {
@@ -816,14 +820,14 @@ __device__ __forceinline__ uint32_t tile_sample(uint64_t mask, uint32_t hbase,
 }
 
 // Map tile of a sample (-1 off the map) before compaction and its slot.  k = 2:
-// 64 x 64 pixel tiles, numbered row-major over a gtx-wide tile grid; k = 1:
+// 32 x 64 pixel tiles, numbered row-major over a gtx-wide tile grid; k = 1:
 // blocks of kTS consecutive pixel numbers (in HEALPix NEST order an aligned
-// 64 x 64 sub-square of a base face).
+// pair of adjacent 32 x 32 sub-squares of a base face).
 __device__ __forceinline__ int64_t flat_tile(const int32_t *pix, const PixGeom &pg, int64_t i,
                                              int64_t ny, int64_t nx, int64_t gtx,
                                              int32_t &slot) {
     if (pg.k == 3) {
-        // 64 x 64 sub-blocks of each map tile, numbered from sbase[t]
+        // plan-tile-sized sub-blocks of each map tile, numbered from sbase[t]
         int32_t t, iy, ix, tx;
         slot = 0;
         if (!tile_pos(pix, pg, i, t, iy, ix, tx)) return -1;
@@ -996,6 +1000,13 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_tilepix(PixGeom pg, int64_t n
 #define SO3G_TILE_U 8
 #endif
 constexpr int kTU = SO3G_TILE_U;
+// Copies of the tile sums in LDS: lane parity picks the copy, so that two
+// neighbouring lanes (often the same pixel) never add to one address in one
+// instruction; the flush adds the copies in order.
+#ifndef SO3G_TILE_COPIES
+#define SO3G_TILE_COPIES 2
+#endif
+constexpr int kTCopies = SO3G_TILE_COPIES;
 
 template <int NC, bool TC, typename ST, bool LDT>
 __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
@@ -1008,11 +1019,11 @@ __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
     double *const *__restrict__ tiles, const int32_t *__restrict__ tile_map,
     const int32_t *__restrict__ tdims) {
     constexpr int NS = TC ? NC - 1 : NC;
-    __shared__ double acc[NC * kTS];
+    __shared__ double acc[kTCopies * NC * kTS];
     extern __shared__ float s_tab[];   // LDT: [n_det] T factors (TC), [n_det] weights
     const int w_off = TC ? n_det : 0;
     const TItem it = items[blockIdx.x];
-    for (int q = threadIdx.x; q < NC * kTS; q += kTBlock) acc[q] = 0.;
+    for (int q = threadIdx.x; q < kTCopies * NC * kTS; q += kTBlock) acc[q] = 0.;
     if constexpr (LDT) {
         for (int d = threadIdx.x; d < n_det; d += kTBlock) {
             if (TC) s_tab[d] = tval[d];
@@ -1085,7 +1096,7 @@ __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
                     x = (double)w * sg[u] * (double)pf[q];
                 else
                     x = (double)(w * sg[u] * pf[q]);
-                atomicAdd(acc + q * kTS + sl[u], x);
+                atomicAdd(acc + ((lane & (kTCopies - 1)) * NC + q) * kTS + sl[u], x);
             }
         }
     }
@@ -1103,7 +1114,9 @@ __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
         if (px < 0) continue;
 #pragma unroll
         for (int cc = 0; cc < NC; cc++) {
-            const double v = acc[cc * kTS + q];
+            double v = acc[cc * kTS + q];
+#pragma unroll
+            for (int cp = 1; cp < kTCopies; cp++) v += acc[(cp * NC + cc) * kTS + q];
             if (v != 0.) unsafeAtomicAdd(mbase + cc * pl + px, v);
         }
     }

@@ -378,10 +378,17 @@ def tiled_plan_counts(pix, t_block, n_tiles_of, item):
     return heads, items
 
 
+def plan_tile_shape():
+    """Rows and columns of a tiled plan's map tiles (a build constant)."""
+    from so3g_amd import _native
+    return _native.get_option('plan_tile_rows'), _native.get_option('plan_tile_cols')
+
+
 def flat_tiles(p, nx=300):
-    """Compact 64 x 64 tile of (iy, ix) pixels, tiles numbered in flat
-    order among the tiles visited."""
-    tf = (p[:, 0] // 64) * ((nx + 63) // 64) + p[:, 1] // 64
+    """Compact plan tile of (iy, ix) pixels, tiles numbered in flat order
+    among the tiles visited."""
+    ty, tx = plan_tile_shape()
+    tf = (p[:, 0] // ty) * ((nx + tx - 1) // tx) + p[:, 1] // tx
     u = np.unique(tf)
     return np.searchsorted(u, tf), len(u)
 
@@ -530,8 +537,8 @@ def test_precomp_plan_edges(gpu, case, layout, tcopy):
 def test_precomp_plan_tiled_map(gpu, tile_shape, tcopy):
     """PrecompPlan on a tiled flat map (k = 3 pixel indices, maps as tile
     lists with inactive tiles None): to_map == ProjEng_Precomp_Tiled.to_map
-    (oracle), from_map bit for bit, tiles larger than one 64 x 64 plan
-    sub-block and ragged edge tiles; a hit tile passed as None raises
+    (oracle), from_map bit for bit, map tiles larger than one plan tile
+    and ragged edge tiles; a hit tile passed as None raises
     RuntimeError, as the reference does."""
     import so3g_amd
     c = Case('CAR', comps='TQU', n_det=30, n_samp=4000, ny=150, nx=260)
@@ -599,13 +606,14 @@ def test_precomp_plan_healpix(gpu, nest, nside, t_block_mb, layout, tcopy):
         tb = (np.arange(c.signal.shape[1]) // info['t_block'])[None, :].repeat(24, 0)
         assert info['runs'] == len(np.unique((tb.astype(np.int64) * npix + pix[..., 0])[on]))
     else:
-        # tiles: blocks of 4096 consecutive pixel numbers, numbered over
-        # the blocks visited
+        # tiles: blocks of rows x cols consecutive pixel numbers, numbered
+        # over the blocks visited
         from so3g_amd import _native
-        blocks = np.unique(vis // 4096)
+        ty, tx = plan_tile_shape()
+        blocks = np.unique(vis // (ty * tx))
         assert (info['stretches'], info['items']) == tiled_plan_counts(
             pix, info['t_block'],
-            lambda p: (np.searchsorted(blocks, p[:, 0] // 4096), len(blocks)),
+            lambda p: (np.searchsorted(blocks, p[:, 0] // (ty * tx)), len(blocks)),
             _native.get_option('plan_tile_item'))
     dw = c.det_weights.copy()
     dw[1] = 0.

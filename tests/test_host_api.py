@@ -204,3 +204,7 @@ def test_plan_options():
             _native.set_option(name, old)
     with pytest.raises(ValueError):
         _native.set_option('plan_no_such_option', 1)
+    # the tiled plan's tile shape is a build constant, readable only
+    assert (_native.get_option('plan_tile_rows'), _native.get_option('plan_tile_cols')) == (32, 64)
+    with pytest.raises(ValueError):
+        _native.set_option('plan_tile_rows', 64)
