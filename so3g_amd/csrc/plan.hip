@@ -1619,7 +1619,9 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
         p->bytes = p->n_on * (2 + 4 * p->ns) + n_chunks * 12 + p->n_heads * 4 +
                    p->n_items * (int64_t)sizeof(TItem) + n_tiles * kTS * 4 + p->n_off * 4 +
                    p->n_vis * 4 + (p->tconst ? n_det * 4 : 0) +
-                   (p->tpix != nullptr ? n * (4 + 4 * p->ns) + p->n_vis * 32 : 0);
+                   (p->tpix != nullptr ? n * (4 + 4 * p->ns) + p->n_vis * 32 : 0) +
+                   (k == 3 ? (int64_t)p->n_tile_map * 28 + 12 + p->n_tiles * 4 + p->n_vis * 4
+                           : 0);
         *out = p;
         return SO3G_OK;
     }
