@@ -442,7 +442,11 @@ int so3g_nn_adam(float *w, const float *g, float *m, float *v, int64_t n, float 
                  float b2, float eps, int step, void *w_bf16, void *stream);
 int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
 
-/* Implementation tunables (no counterpart in the reference):
+/* Implementation tunables (no counterpart in the reference).  They are
+ * process-wide values read when a call starts (the plan options when a plan
+ * is created): set them before, not during, calls from other threads.
+ * so3g_get_option also reads the build constants "plan_tile_rows" and
+ * "plan_tile_cols".
  *   "scatter_agg"   1: sum runs of equal pixels within a wave before the
  *                   map atomics (default 1)
  *   "scatter_xcd"   1: XCD-private partial maps for non-tiled scatters,
