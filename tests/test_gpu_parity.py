@@ -286,6 +286,63 @@ def test_small_config_full_size(gpu):
                'to_weights', atol=1e-9)
 
 
+def test_full_size_plan_properties(gpu):
+    """The benchmark workload at full size (1000 det x 720000 samples, CAR
+    0.5' TQU 191 x 2701, 7.2e8 samples): for both plan layouts, from_map is
+    bit-identical to ProjEng_Precomp's from_map and to_map matches its
+    to_map to float64 summation-order rounding; the two layouts agree; and P
+    and P^T are adjoint, <P m, s> = <m, P^T s> to float32 rounding."""
+    import torch
+    import so3g_amd
+    from so3g_amd import _native as N, workloads
+    dev = torch.device('cuda')
+    lib = N.load()
+    obs = workloads.Observation(1000, 720000, 1.5, 0.5, seed=0)
+    eng = so3g_amd.ProjEng_CAR_TQU_NonTiled(obs.projectionist()._get_pixelizor_args())
+    ny, nx = obs.shape
+    n_det, n_t = obs.n_det, obs.n_samp
+    bore, dets, resp = [torch.from_numpy(a).to(dev) for a in (obs.bore, obs.dets, obs.resps)]
+    pix = torch.empty((n_det, n_t, 2), dtype=torch.int32, device=dev)
+    spin = torch.empty((n_det, n_t, 3), dtype=torch.float32, device=dev)
+    N.check(lib.so3g_pointing_matrix(eng._g, bore.data_ptr(), n_t, dets.data_ptr(), n_det,
+                                     resp.data_ptr(), pix.data_ptr(), spin.data_ptr(), None))
+    gen = torch.Generator(device=dev).manual_seed(2)
+    sky = torch.randn((3, ny, nx), dtype=torch.float64, device=dev, generator=gen)
+    sig = torch.randn((n_det, n_t), dtype=torch.float32, device=dev, generator=gen)
+    shape = np.array([ny, nx], np.int64)
+    s_pc = torch.empty_like(sig)
+    N.check(lib.so3g_precomp_from_map(sky.data_ptr(), shape.ctypes.data, None, None,
+                                      pix.data_ptr(), 2, spin.data_ptr(), 3, n_det, n_t,
+                                      s_pc.data_ptr(), 0, None))
+    m_pc = torch.zeros((3, ny, nx), dtype=torch.float64, device=dev)
+    N.check(lib.so3g_precomp_to_map(m_pc.data_ptr(), shape.ctypes.data, None, None,
+                                    pix.data_ptr(), 2, spin.data_ptr(), 3, n_det, n_t,
+                                    sig.data_ptr(), None, None, 0, None))
+    scale = float(m_pc.abs().max())
+    maps = []
+    for layout in (1, 0):
+        plan = so3g_amd.PrecompPlan(pix, spin, (ny, nx), layout=layout)
+        assert plan.info()['layout'] == layout
+        assert plan.info()['n_on'] == int((pix[..., 0] >= 0).sum())
+        s = torch.empty_like(sig)
+        N.check(lib.so3g_pc_plan_from_map(plan._h, sky.data_ptr(), s.data_ptr(), 0, None))
+        assert torch.equal(s.view(torch.int32), s_pc.view(torch.int32)), f'from_map {layout}'
+        m = torch.zeros_like(m_pc)
+        N.check(lib.so3g_pc_plan_to_map(plan._h, m.data_ptr(), sig.data_ptr(), None, None))
+        torch.cuda.synchronize()
+        err = float((m - m_pc).abs().max()) / scale
+        assert err < 1e-12, (layout, err)
+        maps.append(m)
+        del plan, s
+    assert float((maps[0] - maps[1]).abs().max()) / scale < 1e-12
+    # adjointness in float64 over all 7.2e8 samples
+    prods = s_pc.double() * sig.double()
+    lhs = float(prods.sum())
+    rhs = float((sky * maps[0]).sum())
+    norm = float(prods.square().sum().sqrt())
+    assert abs(lhs - rhs) < 1e-5 * norm, (lhs, rhs, norm)
+
+
 def test_projectionist_torch_path(gpu):
     """Projectionist with torch device tensors (zero-copy) matches numpy."""
     import torch
