@@ -208,3 +208,36 @@ def test_plan_options():
     assert (_native.get_option('plan_tile_rows'), _native.get_option('plan_tile_cols')) == (32, 64)
     with pytest.raises(ValueError):
         _native.set_option('plan_tile_rows', 64)
+
+
+def test_plan_argument_errors():
+    """Plan entry points reject bad arguments on the host, before any device
+    work, with the reference's exception classes."""
+    import ctypes
+    lib = _native.load()
+    h = ctypes.c_void_p()
+    shape = np.array([10, 10], np.int64)
+    dims = np.array([[4, 4]], np.int32)
+    cases = [
+        # k must be 1 or 2 here (tiled maps have their own entry point)
+        (RuntimeError, lambda: lib.so3g_pc_plan_create(None, 3, None, 3, 1, 1, shape.ctypes.data,
+                                                       -1, ctypes.byref(h), None)),
+        (RuntimeError, lambda: lib.so3g_pc_plan_create(None, 2, None, 4, 1, 1, shape.ctypes.data,
+                                                       -1, ctypes.byref(h), None)),
+        (RuntimeError, lambda: lib.so3g_pc_plan_create(None, 2, None, 3, 1 << 22, 1,
+                                                       shape.ctypes.data, -1, ctypes.byref(h),
+                                                       None)),
+        (ValueError, lambda: lib.so3g_pc_plan_create_tiled(None, None, 3, 1, 1, None, 0, -1,
+                                                           ctypes.byref(h), None)),
+        (ValueError, lambda: lib.so3g_pc_plan_create_tiled(None, None, 3, 1, 1,
+                                                           np.array([[-1, 4]], np.int32).ctypes.data,
+                                                           1, -1, ctypes.byref(h), None)),
+        (RuntimeError, lambda: lib.so3g_pc_plan_create_tiled(None, None, 0, 1, 1, dims.ctypes.data,
+                                                             1, -1, ctypes.byref(h), None)),
+        (ValueError, lambda: lib.so3g_pc_plan_to_map_tiles(None, None, None, None, None)),
+        (ValueError, lambda: lib.so3g_pc_plan_from_map_tiles(None, None, None, 0, None)),
+        (ValueError, lambda: lib.so3g_pc_plan_from_map(None, None, None, 0, None)),
+    ]
+    for exc, call in cases:
+        with pytest.raises(exc):
+            _native.check(call())
