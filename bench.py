@@ -298,7 +298,7 @@ def main():
     variants = None if args.no_small else c2_variants(torch, so3g_amd, N, lib, obs, pixargs, bore,
                                                       dets, resp, sig, dev, stream, run)
     other = None if args.no_small else c2_other_ops(torch, N, lib, g, bore, dets, resp, pix,
-                                                    spin, ny, nx, dev, stream, run)
+                                                    spin, plan, ny, nx, dev, stream, run)
 
     out = {
         'metric': 'det*samples/s for to_map+from_map (% HBM roofline)',
@@ -477,13 +477,13 @@ def nmat_timing(torch, so3g_amd, dev, n_det, n_t, nvec=10, nbin=20, reps=5):
             'hbm_frac': b / (ms * 1e-3) / HBM_PEAK}
 
 
-def c2_other_ops(torch, N, lib, g, bore, dets, resp, pix, spin, ny, nx, dev, stream, run,
-                 steps=3):
+def c2_other_ops(torch, N, lib, g, bore, dets, resp, pix, spin, plan, ny, nx, dev, stream,
+                 run, steps=3):
     """The other ops of the C1/C2 list on the benchmark observation (CAR 0.5
     arcmin TQU non-tiled): get_pixels and get_pointing_matrix (pointing
     recomputed; their outputs are 8 and 20 B per det*sample written), and
-    to_weights on the fly and through the cached pointing matrix (ms; the
-    weights map is zeroed first).  to_weights is atomic-bound: time only."""
+    to_weights on the fly, through the cached pointing matrix and through
+    the plan (ms; the weights map is zeroed first).  Time only."""
     n_det, n_t = pix.shape[0], pix.shape[1]
     n_s = n_det * n_t
     pbuf = torch.empty_like(pix)
@@ -511,12 +511,17 @@ def c2_other_ops(torch, N, lib, g, bore, dets, resp, pix, spin, ny, nx, dev, str
         chk(lib.so3g_precomp_to_weights(wmap.data_ptr(), shape.ctypes.data, None, None,
                                         pix.data_ptr(), 2, spin.data_ptr(), 3, n_det, n_t, None,
                                         None, 0, stream))
-    _, ms = run([get_pixels, get_pmat, otf_weights, pc_weights], steps, 1)
+
+    def plan_weights():
+        wmap.zero_()
+        chk(lib.so3g_pc_plan_to_weights(plan._h, wmap.data_ptr(), None, stream))
+    _, ms = run([get_pixels, get_pmat, otf_weights, pc_weights, plan_weights], steps, 1)
     del pbuf, wmap
     out = {}
     for name, t, b in (('get_pixels', ms[0], 8), ('get_pointing_matrix', ms[1], 20),
                        ('to_weights_on_the_fly', ms[2], None),
-                       ('to_weights_precomp', ms[3], None)):
+                       ('to_weights_precomp', ms[3], None),
+                       ('to_weights_plan', ms[4], None)):
         out[name] = {'ms': t, 'det_samples_per_s': n_s / (t * 1e-3)}
         if b:
             out[name]['hbm_frac_output'] = b * n_s / (t * 1e-3) / HBM_PEAK

@@ -263,6 +263,15 @@ int so3g_pc_plan_to_map_tiles(const so3g_pc_plan *plan, double *const *tiles,
                               const float *signal, const float *det_weights, void *stream);
 int so3g_pc_plan_from_map_tiles(const so3g_pc_plan *plan, const double *const *tiles,
                                 float *signal, int accumulate, void *stream);
+/* Precomp to_weight_map (Projection.cxx:2124-2154) through a tiled-layout
+ * plan: map (nc, nc, ...) += sum of (pf_a * pf_b) * det_wt (float32) for
+ * b >= a, the upper triangle as the reference fills it; summed per work item
+ * in LDS (float64, another order than the reference's sequential sums).
+ * Layout 0 plans are refused. */
+int so3g_pc_plan_to_weights(const so3g_pc_plan *plan, double *map, const float *det_weights,
+                            void *stream);
+int so3g_pc_plan_to_weights_tiles(const so3g_pc_plan *plan, double *const *tiles,
+                                  const float *det_weights, void *stream);
 int so3g_pc_plan_info(const so3g_pc_plan *plan, int64_t *n_on, int64_t *bytes);
 /* stats[10]: n_on, pixel runs (layout 0), T factor constant per detector
  * (0/1), stored spin factors per sample, samples per time block, device

@@ -78,6 +78,8 @@ _SIGS = {
     'so3g_pc_plan_create_tiled': (c_int, [c_vp, c_vp, c_i32, c_i64, c_i64, c_vp, c_i32, c_i64,
                                           c_vp, c_vp]),
     'so3g_pc_plan_to_map_tiles': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    'so3g_pc_plan_to_weights': (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    'so3g_pc_plan_to_weights_tiles': (c_int, [c_vp, c_vp, c_vp, c_vp]),
     'so3g_pc_plan_from_map_tiles': (c_int, [c_vp, c_vp, c_vp, c_int, c_vp]),
     'so3g_precomp_from_map_f64': (c_int, [c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_i64, c_vp,
                                           c_int, c_vp]),

@@ -1122,6 +1122,119 @@ __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
     }
 }
 
+// Tiled to_weights (Precomp to_weight_map, Projection.cxx:2135-2154): per
+// position (pf_a * pf_b) * det_wt in float32 for b >= a, summed per work item
+// in the tile held in LDS (one copy: NC (NC + 1) / 2 planes), then added to
+// the (NC, NC, ...) weights map's upper triangle, one atomic per touched slot
+// and entry.  Loads as k_tp_to_map (no signal).
+template <int NC, bool TC, bool LDT>
+__global__ __launch_bounds__(kTBlock) void k_tp_to_weights(
+    const TItem *__restrict__ items, const uint16_t *__restrict__ slot,
+    const float *__restrict__ fac, const uint64_t *__restrict__ tmask,
+    const uint32_t *__restrict__ thead, const uint32_t *__restrict__ head_i,
+    const int32_t *__restrict__ tile_pix, const float *__restrict__ tval, FastDiv n_t,
+    uint32_t plane, const float *__restrict__ det_weights, int32_t n_det, double *map,
+    double *const *__restrict__ tiles, const int32_t *__restrict__ tile_map,
+    const int32_t *__restrict__ tdims) {
+    constexpr int NS = TC ? NC - 1 : NC;
+    constexpr int NW = NC * (NC + 1) / 2;
+    __shared__ double acc[NW * kTS];
+    extern __shared__ float s_tab[];   // LDT: [n_det] T factors (TC), [n_det] weights
+    const int w_off = TC ? n_det : 0;
+    const TItem it = items[blockIdx.x];
+    for (int q = threadIdx.x; q < NW * kTS; q += kTBlock) acc[q] = 0.;
+    if constexpr (LDT) {
+        for (int d = threadIdx.x; d < n_det; d += kTBlock) {
+            if (TC) s_tab[d] = tval[d];
+            if (det_weights != nullptr) s_tab[w_off + d] = det_weights[d];
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t e = it.b + it.n;
+    const int64_t c_end = (e + 63) >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int64_t cb = (it.b >> 6) + (int64_t)wave * kTU; cb < c_end;
+         cb += (int64_t)(kTBlock / 64) * kTU) {
+        uint64_t m[kTU];
+        uint32_t hb[kTU];
+#pragma unroll
+        for (int u = 0; u < kTU; u++) {
+            const int64_t c = min(cb + u, c_end - 1);
+            m[u] = tmask[c]; hb[u] = thead[c];
+        }
+        uint32_t si[kTU];
+        int sl[kTU];
+        float sp[kTU][NS > 0 ? NS : 1];
+        bool act[kTU];
+#pragma unroll
+        for (int u = 0; u < kTU; u++) {
+            const int64_t j0 = (cb + u) * 64 + lane;
+            act[u] = j0 >= it.b && j0 < e;
+            const int64_t j = act[u] ? j0 : it.b;
+            const uint32_t ts = tile_sample(m[u], hb[u], head_i, lane);
+            si[u] = act[u] ? ts : 0u;
+            sl[u] = slot[j];
+            if constexpr (NS == 2) {
+                const f32x2 v = reinterpret_cast<const f32x2 *>(fac)[j];
+                sp[u][0] = v.x; sp[u][1] = v.y;
+            } else {
+#pragma unroll
+                for (int q = 0; q < NS; q++) sp[u][q] = fac[j * NS + q];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kTU; u++) {
+            if (!act[u]) continue;
+            float pf[NC];
+            uint32_t det = 0;
+            if (TC || det_weights != nullptr) det = fast_div<1>(si[u], n_t);
+            if constexpr (TC) {
+                if constexpr (LDT) pf[0] = s_tab[det];
+                else pf[0] = tval[det];
+#pragma unroll
+                for (int q = 1; q < NC; q++) pf[q] = sp[u][q - 1];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NC; q++) pf[q] = sp[u][q];
+            }
+            float w = 1.f;
+            if (det_weights != nullptr) {
+                if constexpr (LDT) w = s_tab[w_off + det];
+                else w = det_weights[det];
+            }
+            if (w == 0.f) continue;   // a zero weight skips the detector (Projection.cxx:2139)
+            int k = 0;
+#pragma unroll
+            for (int a = 0; a < NC; a++)
+#pragma unroll
+                for (int b = a; b < NC; b++, k++)
+                    atomicAdd(acc + k * kTS + sl[u], (double)(pf[a] * pf[b] * w));
+        }
+    }
+    __syncthreads();
+    const int32_t *tp = tile_pix + (int64_t)it.tile * kTS;
+    double *mbase = map;
+    int64_t pl = plane;
+    if (tiles != nullptr) {
+        const int32_t mt = tile_map[it.tile];
+        mbase = tiles[mt];
+        pl = (int64_t)tdims[2 * mt] * tdims[2 * mt + 1];
+    }
+    for (int q = threadIdx.x; q < kTS; q += kTBlock) {
+        const int32_t px = tp[q];
+        if (px < 0) continue;
+        int k = 0;
+#pragma unroll
+        for (int a = 0; a < NC; a++)
+#pragma unroll
+            for (int b = a; b < NC; b++, k++) {
+                const double v = acc[k * kTS + q];
+                if (v != 0.) unsafeAtomicAdd(mbase + (a * NC + b) * pl + px, v);
+            }
+    }
+}
+
 // Tiled from_map: the item's tile values are loaded into LDS, then each
 // position's signal is written (or accumulated into), kTU chunks per wave
 // at a time as in k_tp_to_map; arithmetic as k_plan_from_map.
@@ -1335,6 +1448,48 @@ __global__ __launch_bounds__(kPlanBlock) void k_pc_from_map_f64(int64_t n, int32
         }
         sig[i] = v;
     }
+}
+
+int plan_to_weights(const so3g_pc_plan *p, double *map, const float *det_weights,
+                    void *stream, double *const *tiles) {
+    if (p == nullptr) return set_error(SO3G_ERR_VALUE, "plan is NULL");
+    if (p->layout != 1)
+        return set_error(SO3G_ERR_RUNTIME, "to_weights needs a tiled-layout plan (plan_layout 1)");
+    if (p->n_on == 0 || p->n_items == 0) return SO3G_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t plane = (uint32_t)(p->ny * p->nx);
+    const FastDiv nt{(uint32_t)p->n_t, 1.f / (float)p->n_t};
+    const int tabs = (p->tconst ? 1 : 0) + (det_weights != nullptr ? 1 : 0);
+    const bool ldt = g_plan_to_lds && tabs > 0 &&
+                     p->n_det * tabs * (int64_t)sizeof(float) <= kPlanLdsBytes;
+    const size_t lds = ldt ? (size_t)p->n_det * tabs * sizeof(float) : 0;
+    const int32_t nd = (int32_t)p->n_det;
+    const dim3 gi((unsigned)p->n_items);
+    const TItem *items = (const TItem *)p->items;
+#define TP_W(NC, TC)                                                                        \
+    do {                                                                                    \
+        if (ldt)                                                                            \
+            hipLaunchKernelGGL((k_tp_to_weights<NC, TC, true>), gi, dim3(kTBlock), lds, s,  \
+                               items, p->tslot, p->sspin, p->tmask, p->thead, p->head_i,    \
+                               p->tile_pix, p->tval, nt, plane, det_weights, nd, map,       \
+                               tiles, p->tile_map, p->tdims_d);                             \
+        else                                                                                \
+            hipLaunchKernelGGL((k_tp_to_weights<NC, TC, false>), gi, dim3(kTBlock), 0, s,   \
+                               items, p->tslot, p->sspin, p->tmask, p->thead, p->head_i,    \
+                               p->tile_pix, p->tval, nt, plane, det_weights, nd, map,       \
+                               tiles, p->tile_map, p->tdims_d);                             \
+    } while (0)
+    if (p->tconst) {
+        if (p->nc == 1) TP_W(1, true);
+        else TP_W(3, true);
+    } else {
+        if (p->nc == 1) TP_W(1, false);
+        else if (p->nc == 2) TP_W(2, false);
+        else TP_W(3, false);
+    }
+#undef TP_W
+    PLAN_TRY(hipGetLastError());
+    return SO3G_OK;
 }
 
 }  // namespace
@@ -1961,6 +2116,18 @@ int so3g_precomp_from_map_f64(const double *map, const int64_t *map_shape,
     }
     PLAN_TRY(hipGetLastError());
     return SO3G_OK;
+}
+
+int so3g_pc_plan_to_weights(const so3g_pc_plan *p, double *map, const float *det_weights,
+                            void *stream) {
+    if (int rc = plan_untiled(p)) return rc;
+    return plan_to_weights(p, map, det_weights, stream, nullptr);
+}
+
+int so3g_pc_plan_to_weights_tiles(const so3g_pc_plan *p, double *const *tiles,
+                                  const float *det_weights, void *stream) {
+    if (int rc = plan_upload_tiles(p, (const void *const *)tiles, (hipStream_t)stream)) return rc;
+    return plan_to_weights(p, nullptr, det_weights, stream, p->tiles_d);
 }
 
 int so3g_pc_plan_from_map(const so3g_pc_plan *p, const double *map, float *signal,

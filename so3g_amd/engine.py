@@ -869,6 +869,34 @@ class PrecompPlan:
         call.finish()
         return map
 
+    def to_weights(self, map, det_weights=None):
+        """map += P^T diag(det_weights) P as ProjEng_Precomp.to_weight_map
+        fills it: the upper triangle of an (nc, nc, ...) array, or of each
+        tile of a tile list (k = 3).  Tiled-layout plans only."""
+        call = _Call()
+        dw = call.array(det_weights, 'float32', 'det_weights', spec=[self.n_det],
+                        optional=True)
+        if self.k == 3:
+            if not isinstance(map, (list, tuple)) or len(map) != len(self.tile_dims):
+                raise RuntimeError(f"Buffer 'map' has incompatible shape: expected a list of "
+                                   f"{len(self.tile_dims)} tiles.")
+            ptrs = np.zeros(len(map), np.int64)
+            for i, tile in enumerate(map):
+                if tile is not None:
+                    spec = [self.nc, self.nc] + [int(x) for x in self.tile_dims[i]]
+                    ptrs[i] = call.array(tile, 'float64', 'map', spec=spec,
+                                         out=True).data_ptr()
+            call.keep.append(ptrs)
+            N.check(N.load().so3g_pc_plan_to_weights_tiles(self._h, ptrs.ctypes.data, _ptr(dw),
+                                                           _stream()))
+        else:
+            m = call.array(map, 'float64', 'map',
+                           spec=[self.nc, self.nc] + list(self.map_shape), out=True)
+            N.check(N.load().so3g_pc_plan_to_weights(self._h, m.data_ptr(), _ptr(dw),
+                                                     _stream()))
+        call.finish()
+        return map
+
     def from_map(self, map, signal=None):
         """signal (+)= P map; a given signal is accumulated into, as
         ProjEng_Precomp.from_map does; None returns fresh rows."""

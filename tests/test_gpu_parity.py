@@ -512,6 +512,15 @@ def test_precomp_plan(gpu, n_det, n_samp, comps, tconst, t_block_mb, layout, tco
     finally:
         for n, v in zip(names, old):
             _native.set_option(n, v)
+    if layout == 1:
+        # to_weights through the plan == Precomp to_weight_map (upper triangle)
+        for w in (None, dw):
+            g = plan.to_weights(np.zeros((nc, nc, 100, 300)), w)
+            r = oracle.precomp_to_weights(c.og, np.zeros((nc, nc, 100, 300)), pix, spin, w)
+            maps_close(g, r, f'plan to_weights {comps}', atol=1e-9)
+    else:
+        with pytest.raises(RuntimeError):
+            plan.to_weights(np.zeros((nc, nc, 100, 300)))
     m = c.map_like([nc])
     ref = oracle.precomp_from_map(c.og, m, pix, spin)
     s0 = np.array(c.signal)
@@ -622,6 +631,9 @@ def test_precomp_plan_tiled_map(gpu, tile_shape, tcopy):
         r = oracle.precomp_to_map(og, eng.zeros((3,)), pix, spin, c.signal, w)
         maps_close(g, r, f'tiled-map plan to_map {tile_shape}', atol=1e-9)
     rng = np.random.default_rng(3)
+    g = plan.to_weights(eng.zeros((3, 3)), dw)
+    r = oracle.precomp_to_weights(og, eng.zeros((3, 3)), pix, spin, dw)
+    maps_close(g, r, f'tiled-map plan to_weights {tile_shape}', atol=1e-9)
     m = [None if t is None else rng.standard_normal(t.shape) for t in eng.zeros((3,))]
     assert_bit_equal(np.array(plan.from_map(m)), oracle.precomp_from_map(og, m, pix, spin),
                      'tiled-map plan from_map')
@@ -678,6 +690,10 @@ def test_precomp_plan_healpix(gpu, nest, nside, t_block_mb, layout, tcopy):
         g = plan.to_map(np.zeros((3, npix)), c.signal, w)
         r = oracle.precomp_to_map(c.og, np.zeros((3, npix)), pix, spin, c.signal, w)
         maps_close(g, r, f'HP plan to_map nside={nside}', atol=1e-9)
+    if layout == 1:
+        g = plan.to_weights(np.zeros((3, 3, npix)), dw)
+        r = oracle.precomp_to_weights(c.og, np.zeros((3, 3, npix)), pix, spin, dw)
+        maps_close(g, r, f'HP plan to_weights nside={nside}', atol=1e-9)
     m = c.map_like([3])
     s0 = np.array(c.signal)
     assert_bit_equal(np.array(plan.from_map(m)), oracle.precomp_from_map(c.og, m, pix, spin),
