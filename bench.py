@@ -62,7 +62,10 @@ def plan_bytes(pinfo, n_s):
     described in main) and the to_map kernel's name."""
     from so3g_amd import _native as N
     n_on, ns, n_vis = pinfo['n_on'], pinfo['stored_spin'], pinfo['n_vis']
-    b_from = n_s * (4 + 4 * ns + 4) + 8 * 3 * n_vis
+    # u16 pixel offsets plus one int32 base per 512 samples; the wide groups
+    # read int32 pixels (and their group number) instead
+    b_from = n_s * (2 + 4 * ns + 4) + 4 * ((n_s + 511) // 512) + \
+        pinfo['wide_groups'] * (512 * 2 + 4) + 8 * 3 * n_vis
     if pinfo['layout'] == 1:
         slots = N.get_option('plan_tile_rows') * N.get_option('plan_tile_cols')
         b_to = n_on * (2 + 4 * ns + 4) + 12 * ((n_on + 63) // 64) + 4 * pinfo['stretches'] + \
@@ -239,9 +242,11 @@ def main():
 
     # Algorithmic bytes per launch of the planned kernels: what each must
     # move at least from its own storage (see the plan's layout):
-    #   from_map: per sample 4 B compact pixel + 8 B (Q, U) factors + 4 B
-    #             signal written (the T factor is one float per detector), plus
-    #             the float64 TQU values of the visited pixels read once;
+    #   from_map: per sample 2 B compact-pixel offset + 8 B (Q, U) factors +
+    #             4 B signal written (the T factor is one float per detector),
+    #             4 B base per 512 samples (int32 pixels, 4 B per sample, for
+    #             the groups whose pixels span more than 65534), plus the
+    #             float64 TQU values of the visited pixels read once;
     #   to_map (tiled layout): per on-map sample 2 B tile slot + 8 B (Q, U)
     #             factors + 4 B signal, plus 12 B per 64 positions (start mask,
     #             start count), 4 B per start (stretches and chunk starts) and
@@ -266,7 +271,8 @@ def main():
     if ms_to >= ms_from:
         dom = (to_kernel, gbs_to, b_to, tr.get('to_map_bytes'))
     else:
-        dom = ('k_plan_from_map_s<3,true,4,1>', gbs_from, b_from, tr.get('from_map_bytes'))
+        dom = ('k_plan_from_map_s<3,true,4,1,false>', gbs_from, b_from,
+               tr.get('from_map_bytes'))
     # The plain ProjEng_Precomp kernels (no plan), for reference.
     _, (ms_from_direct, ms_to_direct) = run([pc_from_direct, pc_to_direct],
                                             max(1, min(args.steps, 3)), 1)
@@ -328,7 +334,7 @@ def main():
         'plan': {'n_on': n_on, 'runs': n_runs, 'stored_spin_per_sample': ns,
                  't_block': pinfo['t_block'], 'bytes': pinfo['bytes'], 'visited_pixels': n_vis,
                  'layout': pinfo['layout'], 'stretches': pinfo['stretches'],
-                 'items': pinfo['items']},
+                 'items': pinfo['items'], 'wide_groups': pinfo['wide_groups']},
         'roofline': {
             'bound': 'hbm', 'kernel': dom[0], 'achieved': dom[1] / 1e9,
             'peak': HBM_PEAK / 1e9, 'unit': 'GB/s', 'frac': dom[1] / HBM_PEAK,

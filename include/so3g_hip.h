@@ -273,10 +273,12 @@ int so3g_pc_plan_to_weights(const so3g_pc_plan *plan, double *map, const float *
 int so3g_pc_plan_to_weights_tiles(const so3g_pc_plan *plan, double *const *tiles,
                                   const float *det_weights, void *stream);
 int so3g_pc_plan_info(const so3g_pc_plan *plan, int64_t *n_on, int64_t *bytes);
-/* stats[10]: n_on, pixel runs (layout 0), T factor constant per detector
+/* stats[11]: n_on, pixel runs (layout 0), T factor constant per detector
  * (0/1), stored spin factors per sample, samples per time block, device
  * bytes, visited pixels, layout, stretch starts (counting each 64-position
- * chunk's first position) and work items (tiled layout) */
+ * chunk's first position) and work items (tiled layout), and the 512-sample
+ * groups of the time-ordered copy whose compact pixels span more than 65535
+ * values (stored as int32 instead of u16 offsets) */
 int so3g_pc_plan_stats(const so3g_pc_plan *plan, int64_t *stats);
 int so3g_pc_plan_destroy(so3g_pc_plan *plan);
 

@@ -79,14 +79,23 @@ struct so3g_pc_plan {
     uint8_t *heads;        // [n_groups * 64] run-start bits
     uint32_t *wbase;       // [n_groups] runs before each 512-sample group
     uint32_t *run_pix;     // [n_runs] flat pixel iy * nx + ix of each run
-    int32_t *tpix;         // [n] time-ordered flat pixel, -1 off the map
+    // Time-ordered compact pixels in groups of kTGroup samples: tpb[g] is
+    // the group's base (its smallest on-map pixel) and
+    // tpd[i] = pixel - base (kTOff off the map), or, for a group whose
+    // pixels span more than kTOff - 1 values, -1 - k with the group's pixels
+    // (-1 off the map) in tpw[k * kTGroup ...] and its number in wgrp[k].
+    uint16_t *tpd;         // [n]
+    int32_t *tpb;          // [n_groups]
+    int32_t *tpw;          // [n_wide * kTGroup]
+    int32_t *wgrp;         // [n_wide]
+    int64_t n_wide;
     float *tspin;          // [n * ns] time-ordered stored spin factors
     double *packed;        // [n_vis * 4] pixel-major copy of the visited pixels
     int32_t *comp2pix;     // [n_vis] flat pixel of each compact index (ascending)
     int64_t n_vis;         // visited pixels
     int64_t bytes;
-    // Tiled layout (layout 1): idx, heads, wbase, run_pix, tpix, tspin and
-    // packed are unused; sspin holds the factors in tiled sorted order.
+    // Tiled layout (layout 1): idx, heads, wbase and run_pix are unused,
+    // and the time-ordered copy (tpd ... tspin, packed) is optional; sspin holds the factors in tiled sorted order.
     int32_t layout;        // 0: pixel-sorted runs, 1: tiles
     uint16_t *tslot;       // [n_on] slot within the tile
     uint64_t *tmask;       // [n_chunks] stretch starts per 64 positions
@@ -152,6 +161,12 @@ constexpr int64_t kPlanLdsBytes = 32768;
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 constexpr int kPlanR = 8;                 // sorted samples per lane
 constexpr int kGroup = 64 * kPlanR;       // samples per wave = per head group
+// Time-ordered pixel copy: samples per base, and the offset of an off-map
+// sample.  The planned gathers take 64 x {1, 4, 8} samples per wave from an
+// aligned start, so a wave never spans two groups.
+constexpr int kTGroup = 512;
+constexpr uint16_t kTOff = 0xFFFF;
+constexpr int32_t kTWideMark = INT32_MIN;
 
 // Pixel array geometry: k = 1 pixel numbers on an (ny = npix, 1) map, k = 2
 // (iy, ix) pairs on an (ny, nx) map, k = 3 (tile, iy, ix) triples on a tiled
@@ -321,6 +336,71 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_tcopy(const int32_t *pix, P
         const int64_t p = flat_pixel(pix, pg, i, ny, nx);
         tpix[i] = p >= 0 ? (int32_t)compact_of(p, bitmap, wscan) : -1;
         for (int c = 0; c < ns; c++) tspin[i * ns + c] = spin[i * nc + c0 + c];
+    }
+}
+
+// Base of each kTGroup-sample group of the time-ordered compact pixels
+// (smallest on-map pixel, 0 for a group off the map), or kTWideMark when
+// they span more than kTOff - 1 values; counts the wide groups.  One wave per
+// group.
+__global__ __launch_bounds__(kPlanBlock) void k_plan_tbase(const int32_t *tpix, int64_t n,
+                                                           int64_t n_groups, int32_t *tpb,
+                                                           uint32_t *n_wide) {
+    const int lane = threadIdx.x & 63;
+    constexpr int WPB = kPlanBlock / 64;
+    for (int64_t g = (int64_t)blockIdx.x * WPB + threadIdx.x / 64; g < n_groups;
+         g += (int64_t)gridDim.x * WPB) {
+        int32_t lo = INT32_MAX, hi = -1;
+        for (int r = 0; r < kTGroup / 64; r++) {
+            const int64_t i = g * kTGroup + r * 64 + lane;
+            const int32_t v = i < n ? tpix[i] : -1;
+            if (v >= 0) { lo = min(lo, v); hi = max(hi, v); }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, __shfl_xor(lo, o));
+            hi = max(hi, __shfl_xor(hi, o));
+        }
+        if (lane == 0) {
+            const bool wide = hi >= 0 && hi - lo > (int32_t)kTOff - 1;
+            tpb[g] = hi < 0 ? 0 : (wide ? kTWideMark : lo);
+            if (wide) atomicAdd(n_wide, 1u);
+        }
+    }
+}
+
+// Offsets of the narrow groups; each wide group takes the next index k
+// (in no set order), tpb[g] = -1 - k, wgrp[k] = g and its pixels go to
+// tpw[k * kTGroup ...] (-1 past n).  One wave per group.
+__global__ __launch_bounds__(kPlanBlock) void k_plan_tpack(const int32_t *tpix, int64_t n,
+                                                           int64_t n_groups, int32_t *tpb,
+                                                           uint16_t *tpd, int32_t *tpw,
+                                                           int32_t *wgrp, uint32_t *next) {
+    const int lane = threadIdx.x & 63;
+    constexpr int WPB = kPlanBlock / 64;
+    for (int64_t g = (int64_t)blockIdx.x * WPB + threadIdx.x / 64; g < n_groups;
+         g += (int64_t)gridDim.x * WPB) {
+        const int32_t b = tpb[g];
+        if (b == kTWideMark) {
+            uint32_t k = 0;
+            if (lane == 0) {
+                k = atomicAdd(next, 1u);
+                tpb[g] = -1 - (int32_t)k;
+                wgrp[k] = (int32_t)g;
+            }
+            k = __shfl(k, 0);
+            for (int r = 0; r < kTGroup / 64; r++) {
+                const int64_t i = g * kTGroup + r * 64 + lane;
+                tpw[(int64_t)k * kTGroup + r * 64 + lane] = i < n ? tpix[i] : -1;
+            }
+        } else {
+            for (int r = 0; r < kTGroup / 64; r++) {
+                const int64_t i = g * kTGroup + r * 64 + lane;
+                if (i < n) {
+                    const int32_t v = tpix[i];
+                    tpd[i] = v < 0 ? kTOff : (uint16_t)(v - b);
+                }
+            }
+        }
     }
 }
 
@@ -615,28 +695,46 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_pack_tiles(const double *co
     }
 }
 
+// Compact pixel of a time-ordered offset (-1 off the map).
+__device__ __forceinline__ int32_t tpix_of(uint32_t d, int32_t base) {
+    return d == kTOff ? -1 : base + (int32_t)d;
+}
+
 // Planned from_map over the time-ordered copy: V consecutive samples per
-// lane (V = 8 with vector loads when n % 8 == 0, else V = 1).  sr = sum_c
+// lane (V = 8 with vector loads when n % 8 == 0, else V = 1), narrow groups
+// only (the wide ones are left to k_plan_from_map_s<..., true>).  sr = sum_c
 // pf_c * map_c accumulated into a float (each addition rounded to float),
 // then sig = accumulate ? sig + sr : sr, as ProjEng_Precomp's from_map does;
 // off-map samples keep sig (or 0).
 template <int NC, bool TC, int V, int DIV>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map(
-    const int32_t *__restrict__ tpix, const float *__restrict__ tspin,
+    const uint16_t *__restrict__ tpd, const int32_t *__restrict__ tpb,
+    const float *__restrict__ tspin,
     const float *__restrict__ tval, const double *__restrict__ packed, int64_t n,
     FastDiv n_t, float *__restrict__ sig, int accumulate) {
     constexpr int NS = TC ? NC - 1 : NC;
     for (int64_t i0 = ((int64_t)blockIdx.x * kPlanBlock + threadIdx.x) * V; i0 < n;
          i0 += (int64_t)gridDim.x * kPlanBlock * V) {
+        // a wave's 64 V samples lie in one group
+        const int32_t b = tpb[__builtin_amdgcn_readfirstlane((int32_t)(i0 / kTGroup))];
+        if (b < 0) continue;
         int32_t px[V];
         float sp[V * (NS > 0 ? NS : 1)];
         float o[V];
-        if constexpr (V > 1) {
+        if constexpr (V == 8) {
+            const uint4 a = *reinterpret_cast<const uint4 *>(tpd + i0);
+            const uint32_t w[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
-            for (int q = 0; q < V / 4; q++) {
-                const int4 a = reinterpret_cast<const int4 *>(tpix + i0)[q];
-                px[4 * q] = a.x; px[4 * q + 1] = a.y; px[4 * q + 2] = a.z; px[4 * q + 3] = a.w;
+            for (int q = 0; q < 4; q++) {
+                px[2 * q] = tpix_of(w[q] & 0xFFFFu, b);
+                px[2 * q + 1] = tpix_of(w[q] >> 16, b);
             }
+        } else if constexpr (V == 4) {
+            const uint2 a = *reinterpret_cast<const uint2 *>(tpd + i0);
+            px[0] = tpix_of(a.x & 0xFFFFu, b); px[1] = tpix_of(a.x >> 16, b);
+            px[2] = tpix_of(a.y & 0xFFFFu, b); px[3] = tpix_of(a.y >> 16, b);
+        }
+        if constexpr (V > 1) {
             if constexpr (NS > 0) {
 #pragma unroll
                 for (int q = 0; q < V * NS / 4; q++) {
@@ -651,7 +749,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map(
                 o[4 * q] = s.x; o[4 * q + 1] = s.y; o[4 * q + 2] = s.z; o[4 * q + 3] = s.w;
             }
         } else {
-            px[0] = tpix[i0];
+            px[0] = tpix_of(tpd[i0], b);
 #pragma unroll
             for (int c = 0; c < NS; c++) sp[c] = tspin[i0 * NS + c];
             o[0] = accumulate ? sig[i0] : 0.f;
@@ -707,17 +805,35 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map(
 // Lane-strided planned from_map: a wave takes 64·V consecutive samples and
 // lane l the samples l, l + 64, ..., so one gather instruction covers 64
 // consecutive samples of a detector (a few map cache lines) instead of 64·V.
-// Arithmetic and off-map handling are those of k_plan_from_map.
-template <int NC, bool TC, int V, int DIV>
+// WIDE = false takes the narrow groups (u16 offsets from the group's base;
+// a wide group's waves leave at once), WIDE = true the n_wide groups listed
+// in wgrp (int32 pixels).  Arithmetic and off-map handling are those of
+// k_plan_from_map.
+template <int NC, bool TC, int V, int DIV, bool WIDE>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
-    const int32_t *__restrict__ tpix, const float *__restrict__ tspin,
+    const uint16_t *__restrict__ tpd, const int32_t *__restrict__ tpb,
+    const int32_t *__restrict__ tpw, const int32_t *__restrict__ wgrp, int64_t n_wide,
+    const float *__restrict__ tspin,
     const float *__restrict__ tval, const double *__restrict__ packed, int64_t n,
     FastDiv n_t, float *__restrict__ sig, int accumulate) {
     constexpr int NS = TC ? NC - 1 : NC;
     constexpr int WPB = kPlanBlock / 64;
+    constexpr int WPG = kTGroup / (64 * V);   // waves per group
     const int lane = threadIdx.x & 63;
-    for (int64_t w = (int64_t)blockIdx.x * WPB + threadIdx.x / 64; w * 64 * V < n;
-         w += (int64_t)gridDim.x * WPB) {
+    const int64_t n_waves = WIDE ? n_wide * WPG : (n + 64 * V - 1) / (64 * V);
+    for (int64_t q = (int64_t)blockIdx.x * WPB + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+         q < n_waves; q += (int64_t)gridDim.x * WPB) {
+        int64_t w = q, woff = 0;   // woff: tpw index minus sample index
+        int32_t b = 0;
+        if constexpr (WIDE) {
+            const int64_t k = q / WPG;
+            const int64_t g = wgrp[k];
+            w = g * WPG + q % WPG;
+            woff = (k - g) * kTGroup;
+        } else {
+            b = tpb[w / WPG];
+            if (b < 0) continue;
+        }
         const int64_t i0 = w * 64 * V + lane;
         int32_t px[V];
         float sp[V * (NS > 0 ? NS : 1)];
@@ -726,7 +842,10 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
         for (int r = 0; r < V; r++) {
             const int64_t i = i0 + 64 * r;
             const bool ok = i < n;
-            px[r] = ok ? __builtin_nontemporal_load(tpix + i) : -1;
+            if constexpr (WIDE)
+                px[r] = ok ? tpw[i + woff] : -1;
+            else
+                px[r] = tpix_of(ok ? (uint32_t)__builtin_nontemporal_load(tpd + i) : kTOff, b);
             if constexpr (NS == 2) {
                 const f32x2 f = ok ? __builtin_nontemporal_load(reinterpret_cast<const f32x2 *>(tspin) + i)
                                    : f32x2{0.f, 0.f};
@@ -1343,13 +1462,61 @@ unsigned grid_of(int64_t n, int64_t per = kPlanBlock, int64_t cap = 1 << 20) {
 void free_plan_arrays(so3g_pc_plan *p) {
     (void)hipFree(p->idx); (void)hipFree(p->sspin); (void)hipFree(p->tval);
     (void)hipFree(p->heads); (void)hipFree(p->wbase); (void)hipFree(p->run_pix);
-    (void)hipFree(p->tpix); (void)hipFree(p->tspin); (void)hipFree(p->packed);
+    (void)hipFree(p->tpd); (void)hipFree(p->tpb); (void)hipFree(p->tpw);
+    (void)hipFree(p->wgrp); (void)hipFree(p->tspin); (void)hipFree(p->packed);
     (void)hipFree(p->comp2pix);
     (void)hipFree(p->tslot); (void)hipFree(p->tmask);
     (void)hipFree(p->thead); (void)hipFree(p->head_i); (void)hipFree(p->tile_pix);
     (void)hipFree(p->items); (void)hipFree(p->off_idx);
     (void)hipFree(p->tdims_d); (void)hipFree(p->tbase_d); (void)hipFree(p->sbase_d);
     (void)hipFree(p->tile_map); (void)hipFree(p->comp_tile); (void)hipFree(p->tiles_d);
+}
+
+// Time-ordered copy for from_map: spin factors, and the compact pixels as
+// u16 offsets from a base per kTGroup samples (int32 for the wide groups).
+// Arrays land in p (freed with it on failure); temporaries are freed here.
+hipError_t plan_build_tcopy(so3g_pc_plan *p, const int32_t *pixel, PixGeom pg, const float *spin,
+                            int nc, int c0, int64_t n, int64_t ny, int64_t nx,
+                            const uint32_t *bitmap, const uint32_t *wscan, hipStream_t s) {
+    const int64_t n_groups = (n + kTGroup - 1) / kTGroup;
+    int32_t *tpix = nullptr;
+    uint32_t *cnt = nullptr;
+    auto done = [&](hipError_t e) {
+        (void)hipFree(tpix); (void)hipFree(cnt);
+        return e;
+    };
+    hipError_t e;
+#define TC_STEP(expr) if ((e = (expr)) != hipSuccess) return done(e)
+    TC_STEP(hipMalloc(&tpix, std::max<int64_t>(1, n) * 4));
+    TC_STEP(hipMalloc(&cnt, 8));
+    TC_STEP(hipMalloc(&p->tspin, std::max<int64_t>(1, n * p->ns) * 4));
+    TC_STEP(hipMalloc(&p->tpd, std::max<int64_t>(1, n) * 2));
+    TC_STEP(hipMalloc(&p->tpb, std::max<int64_t>(1, n_groups) * 4));
+    TC_STEP(hipMemsetAsync(cnt, 0, 8, s));
+    hipLaunchKernelGGL(k_plan_tcopy, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, pg, spin,
+                       nc, c0, n, ny, nx, bitmap, wscan, tpix, p->tspin);
+    const dim3 gg(grid_of(n_groups * 64));
+    hipLaunchKernelGGL(k_plan_tbase, gg, dim3(kPlanBlock), 0, s, tpix, n, n_groups, p->tpb, cnt);
+    TC_STEP(hipGetLastError());
+    uint32_t n_wide = 0;
+    TC_STEP(hipMemcpyAsync(&n_wide, cnt, 4, hipMemcpyDeviceToHost, s));
+    TC_STEP(hipStreamSynchronize(s));
+    p->n_wide = n_wide;
+    TC_STEP(hipMalloc(&p->tpw, std::max<int64_t>(1, p->n_wide) * kTGroup * 4));
+    TC_STEP(hipMalloc(&p->wgrp, std::max<int64_t>(1, p->n_wide) * 4));
+    hipLaunchKernelGGL(k_plan_tpack, gg, dim3(kPlanBlock), 0, s, tpix, n, n_groups, p->tpb,
+                       p->tpd, p->tpw, p->wgrp, cnt + 1);
+    TC_STEP(hipGetLastError());
+    TC_STEP(hipMalloc(&p->packed, std::max<int64_t>(1, p->n_vis) * 4 * sizeof(double)));
+    TC_STEP(hipStreamSynchronize(s));
+#undef TC_STEP
+    return done(hipSuccess);
+}
+
+// Device bytes of the time-ordered copy.
+int64_t tcopy_bytes(const so3g_pc_plan *p, int64_t n) {
+    return n * (2 + 4 * p->ns) + (n + kTGroup - 1) / kTGroup * 4 +
+           p->n_wide * (kTGroup * 4 + 4) + p->n_vis * 32;
 }
 
 template <typename ST>
@@ -1761,12 +1928,7 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
             PLAN_STEP(hipMemcpyAsync(p->off_idx, p->idx + p->n_on, p->n_off * 4,
                                      hipMemcpyDeviceToDevice, s));
         if (g_plan_time_copy) {
-            PLAN_ALLOC(p->tpix, n * sizeof(int32_t));
-            PLAN_ALLOC(p->tspin, n * p->ns * sizeof(float));
-            hipLaunchKernelGGL(k_plan_tcopy, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, pg,
-                               spin, nc, c0, n, ny, nx, bitmap, wscan, p->tpix, p->tspin);
-            PLAN_STEP(hipGetLastError());
-            PLAN_ALLOC(p->packed, std::max<int64_t>(1, p->n_vis) * 4 * sizeof(double));
+            PLAN_STEP(plan_build_tcopy(p, pixel, pg, spin, nc, c0, n, ny, nx, bitmap, wscan, s));
         }
         PLAN_STEP(hipStreamSynchronize(s));
         (void)hipFree(p->idx); p->idx = nullptr;
@@ -1774,7 +1936,7 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
         p->bytes = p->n_on * (2 + 4 * p->ns) + n_chunks * 12 + p->n_heads * 4 +
                    p->n_items * (int64_t)sizeof(TItem) + n_tiles * kTS * 4 + p->n_off * 4 +
                    p->n_vis * 4 + (p->tconst ? n_det * 4 : 0) +
-                   (p->tpix != nullptr ? n * (4 + 4 * p->ns) + p->n_vis * 32 : 0) +
+                   (p->tpd != nullptr ? tcopy_bytes(p, n) : 0) +
                    (k == 3 ? (int64_t)p->n_tile_map * 28 + 12 + p->n_tiles * 4 + p->n_vis * 4
                            : 0);
         *out = p;
@@ -1876,18 +2038,12 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
     PLAN_STEP(hipGetLastError());
 
     // 5. time-ordered copy for from_map, and the packed map buffer
-    PLAN_ALLOC(p->tpix, n * sizeof(int32_t));
-    PLAN_ALLOC(p->tspin, n * p->ns * sizeof(float));
-    hipLaunchKernelGGL(k_plan_tcopy, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, pg, spin, nc,
-                       c0, n, ny, nx, bitmap, wscan, p->tpix, p->tspin);
-    PLAN_STEP(hipGetLastError());
-    PLAN_ALLOC(p->packed, std::max<int64_t>(1, p->n_vis) * 4 * sizeof(double));
-    PLAN_STEP(hipStreamSynchronize(s));
+    PLAN_STEP(plan_build_tcopy(p, pixel, pg, spin, nc, c0, n, ny, nx, bitmap, wscan, s));
     cleanup();
 #undef PLAN_ALLOC
 #undef PLAN_STEP
     p->bytes = p->n_on * (4 + 4 * p->ns) + n_groups * (kGroup / 8 + 4) + p->n_runs * 4 +
-               n * (4 + 4 * p->ns) + p->n_vis * (32 + 4) + (p->tconst ? n_det * 4 : 0);
+               tcopy_bytes(p, n) + p->n_vis * 4 + (p->tconst ? n_det * 4 : 0);
     *out = p;
     return SO3G_OK;
 }
@@ -2016,7 +2172,7 @@ static int plan_from_map_impl(const so3g_pc_plan *p, const double *map,
     hipStream_t s = (hipStream_t)stream;
     const int64_t plane = p->ny * p->nx;
     const int64_t n = p->n_det * p->n_t;
-    if (p->layout == 1 && p->tpix == nullptr) {
+    if (p->layout == 1 && p->tpd == nullptr) {
         if (!accumulate && p->n_off > 0) {
             hipLaunchKernelGGL(k_tp_zero, dim3(grid_of(p->n_off)), dim3(kPlanBlock), 0, s,
                                p->off_idx, p->n_off, signal);
@@ -2059,29 +2215,34 @@ static int plan_from_map_impl(const so3g_pc_plan *p, const double *map,
     const bool strided = g_plan_from_layout != 0;
     const int vv = strided ? g_plan_from_v : (n % 8 == 0 ? g_plan_from_v : (n % 4 == 0 ? 4 : 1));
     const dim3 g(grid_of(n, (int64_t)kPlanBlock * vv, INT32_MAX));
+    const dim3 gw(grid_of(p->n_wide * (kTGroup / 256), kPlanBlock / 64));
     const FastDiv nt{(uint32_t)p->n_t, 1.f / (float)p->n_t};
 #define PLAN_FROM(NC, TC)                                                                   \
     do {                                                                                    \
         if (strided && vv == 8)                                                             \
-            hipLaunchKernelGGL((k_plan_from_map_s<NC, TC, 8, 1>), g, dim3(kPlanBlock), 0, s, \
-                               p->tpix, p->tspin, p->tval, p->packed, n, nt, signal,        \
-                               accumulate);                                                 \
+            hipLaunchKernelGGL((k_plan_from_map_s<NC, TC, 8, 1, false>), g, dim3(kPlanBlock), \
+                               0, s, p->tpd, p->tpb, p->tpw, p->wgrp, p->n_wide, p->tspin,  \
+                               p->tval, p->packed, n, nt, signal, accumulate);              \
         else if (strided)                                                                   \
-            hipLaunchKernelGGL((k_plan_from_map_s<NC, TC, 4, 1>), g, dim3(kPlanBlock), 0, s, \
-                               p->tpix, p->tspin, p->tval, p->packed, n, nt, signal,        \
-                               accumulate);                                                 \
+            hipLaunchKernelGGL((k_plan_from_map_s<NC, TC, 4, 1, false>), g, dim3(kPlanBlock), \
+                               0, s, p->tpd, p->tpb, p->tpw, p->wgrp, p->n_wide, p->tspin,  \
+                               p->tval, p->packed, n, nt, signal, accumulate);              \
         else if (vv == 8)                                                                   \
             hipLaunchKernelGGL((k_plan_from_map<NC, TC, 8, 1>), g, dim3(kPlanBlock), 0, s,   \
-                               p->tpix, p->tspin, p->tval, p->packed, n, nt, signal,        \
+                               p->tpd, p->tpb, p->tspin, p->tval, p->packed, n, nt, signal, \
                                accumulate);                                                 \
         else if (vv == 4)                                                                   \
             hipLaunchKernelGGL((k_plan_from_map<NC, TC, 4, 1>), g, dim3(kPlanBlock), 0, s,   \
-                               p->tpix, p->tspin, p->tval, p->packed, n, nt, signal,        \
+                               p->tpd, p->tpb, p->tspin, p->tval, p->packed, n, nt, signal, \
                                accumulate);                                                 \
         else                                                                                \
             hipLaunchKernelGGL((k_plan_from_map<NC, TC, 1, 1>), g, dim3(kPlanBlock), 0, s,   \
-                               p->tpix, p->tspin, p->tval, p->packed, n, nt, signal,        \
+                               p->tpd, p->tpb, p->tspin, p->tval, p->packed, n, nt, signal, \
                                accumulate);                                                 \
+        if (p->n_wide > 0)                                                                  \
+            hipLaunchKernelGGL((k_plan_from_map_s<NC, TC, 4, 1, true>), gw, dim3(kPlanBlock), \
+                               0, s, p->tpd, p->tpb, p->tpw, p->wgrp, p->n_wide, p->tspin,  \
+                               p->tval, p->packed, n, nt, signal, accumulate);              \
     } while (0)
     if (p->tconst) {
         if (p->nc == 1) PLAN_FROM(1, true);
@@ -2161,6 +2322,7 @@ int so3g_pc_plan_stats(const so3g_pc_plan *p, int64_t *stats) {
     stats[7] = p->layout;
     stats[8] = p->n_heads;
     stats[9] = p->n_items;
+    stats[10] = p->n_wide;
     return SO3G_OK;
 }
 

@@ -831,12 +831,13 @@ class PrecompPlan:
         (visited pixels), layout (0 pixel-sorted runs, 1 map tiles), and for
         the tiled layout stretches (starts of runs of consecutive samples
         within a tile, counting each 64-position chunk's first position) and
-        items (work items)."""
-        a = (ctypes.c_int64 * 10)()
+        items (work items); wide_groups: 512-sample groups of the
+        time-ordered copy whose compact pixels span more than 65535 values."""
+        a = (ctypes.c_int64 * 11)()
         N.check(N.load().so3g_pc_plan_stats(self._h, a))
         return {'n_on': a[0], 'runs': a[1], 't_const': bool(a[2]), 'stored_spin': a[3],
                 't_block': a[4], 'bytes': a[5], 'n_vis': a[6], 'layout': a[7],
-                'stretches': a[8], 'items': a[9]}
+                'stretches': a[8], 'items': a[9], 'wide_groups': a[10]}
 
     def _tiles(self, call, map, writeback):
         """Host array of the tile list's device pointers (0 for None)."""

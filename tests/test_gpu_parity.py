@@ -598,6 +598,80 @@ def test_precomp_plan_edges(gpu, case, layout, tcopy):
         assert np.array_equal(g, m0)
 
 
+def time_copy_wide_groups(pix, nx):
+    """512-sample groups of the time-ordered compact pixels (index among the
+    sorted visited flat pixels) whose on-map values span more than 65534."""
+    flat = pix[..., 0].astype(np.int64) * nx + pix[..., 1]
+    on = pix[..., 0] >= 0
+    comp = np.full(flat.shape, -1, np.int64)
+    comp[on] = np.searchsorted(np.unique(flat[on]), flat[on])
+    c = comp.ravel()
+    c = np.concatenate([c, np.full(-c.size % 512, -1)]).reshape(-1, 512)
+    hi = c.max(1)
+    lo = np.where(c >= 0, c, 1 << 40).min(1)
+    return int(((hi >= 0) & (hi - lo > 65534)).sum())
+
+
+@pytest.mark.parametrize('n_det,n_t', [(4, 140000), (5, 140001)])
+@pytest.mark.parametrize('layout', [0, 1], ids=['runs', 'tiles'])
+def test_precomp_plan_wide_groups(gpu, n_det, n_t, layout):
+    """The time-ordered copy stores compact pixels as u16 offsets from a base
+    per 512 samples, and int32 pixels for the groups whose pixels span more
+    than 65534 values: from_map over a map with 131072 visited pixels
+    (detector 0 sweeps them all, so compact index = flat pixel), a detector
+    hopping at random (wide groups), a smooth scan with one group of span
+    exactly 65534 (narrow) and one of 65535 (wide), a detector off the map
+    for half its samples, groups across detector ends and a partial last
+    group; bit for bit against the oracle's Precomp, every gather variant,
+    fresh and accumulating."""
+    import so3g_amd
+    from so3g_amd import _native
+    rng = np.random.default_rng(11)
+    ny, nx = 256, 512
+    t = np.arange(n_t)
+    pix = np.zeros((n_det, n_t, 2), np.int32)
+    pix[0, :, 0], pix[0, :, 1] = (t % (ny * nx)) // nx, t % nx
+    pix[1, :, 0], pix[1, :, 1] = rng.integers(0, ny, n_t), rng.integers(0, nx, n_t)
+    pix[2, :, 0], pix[2, :, 1] = 10 + (t // 5000) % 90, (t // 3) % nx
+    g0 = -(-(2 * n_t + 500) // 512)          # first whole group of detector 2 past t = 500
+    t0 = g0 * 512 - 2 * n_t
+    pix[2, t0 + 20], pix[2, t0 + 21] = (0, 0), (65534 // nx, 65534 % nx)
+    pix[2, t0 + 532], pix[2, t0 + 533] = (0, 0), (65535 // nx, 65535 % nx)
+    pix[3, : n_t // 2] = -1
+    pix[3, n_t // 2:, 0], pix[3, n_t // 2:, 1] = 200 + (t[n_t // 2:] // 7000) % 50, \
+        (t[n_t // 2:] // 2) % nx
+    if n_det > 4:
+        pix[4] = (ny - 1, nx - 1)
+    spin = np.empty((n_det, n_t, 3), np.float32)
+    spin[..., 0] = rng.uniform(0.5, 1.5, (n_det, 1))
+    if n_det > 4:
+        spin[..., 0] *= np.linspace(0.5, 1.5, n_t, dtype=np.float32)
+    spin[..., 1:] = rng.uniform(-1, 1, (n_det, n_t, 2))
+    og = oracle.geom('CAR', 3, ny=ny, nx=nx, cdelt=(1., 1.), crpix=(1., 1.), tiled=False,
+                     tile_shape=(1, 1))
+    plan = so3g_amd.PrecompPlan(pix, spin, (ny, nx), layout=layout, time_copy=1)
+    info = plan.info()
+    assert info['n_vis'] == ny * nx
+    wide = time_copy_wide_groups(pix, nx)
+    assert wide > 0 and info['wide_groups'] == wide
+    m = rng.standard_normal((3, ny, nx))
+    ref = oracle.precomp_from_map(og, m, pix, spin)
+    s0 = rng.standard_normal((n_det, n_t)).astype(np.float32)
+    ref_acc = oracle.precomp_from_map(og, m, pix, spin, s0.copy())
+    names = ('plan_from_layout', 'plan_from_v')
+    old = [_native.get_option(n) for n in names]
+    try:
+        for lay, v in ((1, 4), (1, 8), (0, 4), (0, 8)):
+            _native.set_option('plan_from_layout', lay)
+            _native.set_option('plan_from_v', v)
+            tag = f'wide-group plan from_map layout={lay} v={v}'
+            assert_bit_equal(np.array(plan.from_map(m)), ref, tag)
+            assert_bit_equal(np.array(plan.from_map(m, s0.copy())), ref_acc, tag + ' accumulate')
+    finally:
+        for n, v in zip(names, old):
+            _native.set_option(n, v)
+
+
 @pytest.mark.parametrize('tile_shape', [(100, 100), (30, 70)])
 @pytest.mark.parametrize('tcopy', [1, 0], ids=['tcopy', 'notcopy'])
 def test_precomp_plan_tiled_map(gpu, tile_shape, tcopy):
