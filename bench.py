@@ -672,6 +672,8 @@ def c2_variants(torch, so3g_amd, N, lib, obs, pixargs, bore, dets, resp, sig, de
         chk(lib.so3g_pc_plan_to_map(hplan._h, hout.data_ptr(), sig.data_ptr(), None, stream))
     res = {'map': f'HEALPix NEST nside {nside} TQU ({npix} pixels, 4.8 GB per map)',
            'plan_visited_pixels': hplan.info()['n_vis'], 'plan_build_s': t_hplan}
+    # every to_map above zeroes the 4.8 GB output map first; that alone:
+    _, (res['ms_map_zero'],) = run([lambda: hout.zero_()], steps, 1)
     for name, ops in (('on_the_fly', [hotf_from, hotf_to]), ('precomp', [hpc_from, hpc_to]),
                       ('plan', [hplan_from, hplan_to])):
         _, (mf, mt) = run(ops, steps, 1)
