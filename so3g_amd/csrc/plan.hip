@@ -806,8 +806,8 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map(
 // lane l the samples l, l + 64, ..., so one gather instruction covers 64
 // consecutive samples of a detector (a few map cache lines) instead of 64·V.
 // WIDE = false takes the narrow groups (u16 offsets from the group's base;
-// a wide group's waves leave at once), WIDE = true the n_wide groups listed
-// in wgrp (int32 pixels).  Arithmetic and off-map handling are those of
+// a wide group's waves leave after issuing their loads), WIDE = true the
+// n_wide groups listed in wgrp (int32 pixels).  Arithmetic and off-map handling are those of
 // k_plan_from_map.
 template <int NC, bool TC, int V, int DIV, bool WIDE>
 __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
@@ -832,10 +832,10 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
             woff = (k - g) * kTGroup;
         } else {
             b = tpb[w / WPG];
-            if (b < 0) continue;
         }
         const int64_t i0 = w * 64 * V + lane;
         int32_t px[V];
+        uint32_t dv[V];
         float sp[V * (NS > 0 ? NS : 1)];
         float o[V];
 #pragma unroll
@@ -845,7 +845,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
             if constexpr (WIDE)
                 px[r] = ok ? tpw[i + woff] : -1;
             else
-                px[r] = tpix_of(ok ? (uint32_t)__builtin_nontemporal_load(tpd + i) : kTOff, b);
+                dv[r] = ok ? (uint32_t)__builtin_nontemporal_load(tpd + i) : kTOff;
             if constexpr (NS == 2) {
                 const f32x2 f = ok ? __builtin_nontemporal_load(reinterpret_cast<const f32x2 *>(tspin) + i)
                                    : f32x2{0.f, 0.f};
@@ -855,6 +855,13 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_from_map_s(
                 for (int c = 0; c < NS; c++) sp[r * NS + c] = ok ? tspin[i * NS + c] : 0.f;
             }
             o[r] = (accumulate && ok) ? sig[i] : 0.f;
+        }
+        // the group's base is tested once the streams' loads are in flight
+        // (a wide group's wave leaves here, its loads unused)
+        if constexpr (!WIDE) {
+            if (b < 0) continue;
+#pragma unroll
+            for (int r = 0; r < V; r++) px[r] = tpix_of(dv[r], b);
         }
         double mv[V][4];
 #pragma unroll
