@@ -108,7 +108,9 @@ typedef struct so3g_geom {
 int so3g_init(int device);
 const char *so3g_last_error(void);
 int so3g_abi_version(void);
-/* Copies the host-built tables (2 x 16384 float64) for verification. */
+/* Copies the trig tables (2 x 16384 float64) for verification: the device
+ * copies the kernels read once the library is initialised, else tables
+ * freshly built on the host. */
 int so3g_get_tables(double *asin_vals, double *atan2_vals);
 
 /* Validates the geometry (check_nside, Projection.cxx:607-614, 785-792;

@@ -275,6 +275,14 @@ int so3g_init(int device) {
 }
 
 int so3g_get_tables(double *asin_vals, double *atan2_vals) {
+    if (g_state.device >= 0) {
+        // The tables the kernels read: copied back from the device.
+        HIP_TRY(hipMemcpy(asin_vals, g_state.asin_d, kTrigN * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(atan2_vals, g_state.atan2_d, kTrigN * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        return SO3G_OK;
+    }
     std::vector<double> as, at;
     build_tables(as, at);
     memcpy(asin_vals, as.data(), kTrigN * sizeof(double));

@@ -1,0 +1,241 @@
+"""The reference's engine-contract and thread-assignment tests, run through
+the Projectionist / ProjectionistHealpix on the GPU engines.
+
+Ported from test/test_proj_eng.py:58-151 and test/test_proj_eng_hp.py:36-124
+with the same scan (az triangle 59-61 deg at 50 deg elevation, 600 s at
+10 Hz) and focal plane (two detectors 0.1 deg apart).  Two inputs are built
+differently because their libraries are absent here: the boresight comes
+from CelestialSightLine.naive_az_el (the reference uses the qpoint-based
+az_el, coords.py:127-169) and the TAN geometry from wcslite (the reference
+uses pixell's enmap.geometry, 0.01 deg pixels centred on the scan, a
+(200, 350) map that holds every sample, a (250, 300) map that clips some,
+and one offset by 10 deg that misses them all).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from so3g_amd import proj
+from so3g_amd.proj.wcslite import make_wcs
+
+pytestmark = pytest.mark.gpu
+
+DEG = np.pi / 180
+
+
+def get_scan():
+    dt = np.arange(0., 600, .1)
+    T = 10.
+    phase = (dt % (2 * T)) / T
+    phase[phase > 1] = 2 - phase[phase > 1]
+    v = 2.
+    az = v * (phase - .5) + 60.
+    el = 50. + dt * 0
+    times = 1654041600 + dt
+    return times, az * DEG, el * DEG
+
+
+def get_assembly():
+    t, az, el = get_scan()
+    csl = proj.CelestialSightLine.naive_az_el(t, az, el, site='so')
+    fp = proj.FocalPlane.from_xieta([0., .1 * DEG], [0, .1 * DEG])
+    return (t, az, el), proj.Assembly.attach(csl, fp), csl
+
+
+def get_basics(clippedness=0):
+    scan, asm, csl = get_assembly()
+    co = np.asarray(csl.coords())
+    ra0, dec0 = co[:, 0].mean(), co[:, 1].mean()
+    shape = (200, 350)      # holds every sample
+    if clippedness == 1:
+        shape = (250, 300)  # clips some
+    elif clippedness == 2:
+        ra0 = ra0 - 10 * DEG  # misses the observation
+    ny, nx = shape
+    wcs = make_wcs('TAN', (ra0 / DEG, dec0 / DEG), (-0.01, 0.01), (nx / 2 + .5, ny / 2 + .5))
+    return scan, asm, (shape, wcs)
+
+
+def test_proj_eng_basic(gpu):
+    scan, asm, (shape, wcs) = get_basics()
+    p = proj.Projectionist.for_geom(shape, wcs)
+    sig = np.ones((2, len(scan[0])), 'float32')
+    for comps in ['T', 'QU', 'TQU']:
+        m = p.to_map(sig, asm, comps=comps)[0]
+        assert np.any(m != 0)
+        w = p.to_weights(asm, comps=comps)[0, 0]
+        assert np.any(w != 0)
+
+    target = np.zeros((1, ) + m.shape[:-1])
+    with pytest.raises(RuntimeError):
+        p.to_map(sig, asm, comps='T', output=target)
+
+    # det_weights = 0 gives an empty map
+    m = p.to_map(sig, asm, comps='T', det_weights=np.array([0., 0.], dtype='float32'))[0]
+    assert np.all(m == 0)
+
+    # a NaN detector quaternion raises ValueError
+    asm.fplane.quats[1] = asm.fplane.quats[1] * np.nan
+    with pytest.raises(ValueError):
+        p.to_map(sig, asm, comps='T')
+    with pytest.raises(ValueError):
+        p.to_weights(asm, comps='T')
+
+
+def test_proj_eng_no_output_no_comps(gpu):
+    scan, asm, (shape, wcs) = get_basics()
+    p = proj.Projectionist.for_geom(shape, wcs)
+    sig = np.ones((2, len(scan[0])), 'float32')
+    with pytest.raises(ValueError):
+        p.to_map(sig, asm)
+    with pytest.raises(ValueError):
+        p.to_weights(asm)
+    with pytest.raises(ValueError):
+        p.assign_threads(asm, method='no-such-method')
+
+
+def test_proj_eng_tiled(gpu):
+    scan, asm, (shape, wcs) = get_basics()
+    p = proj.Projectionist.for_tiled(shape, wcs, (150, 150))
+    sig = np.ones((2, len(scan[0])), 'float32')
+    for comps in ['T', 'QU', 'TQU']:
+        m = p.to_map(sig, asm, comps=comps)[0][0]
+        assert np.any(m != 0)
+        w = p.to_weights(asm, comps=comps)[0][0, 0]
+        assert np.any(w != 0)
+    # active sub-tiles and a two-group hit-balanced assignment
+    p = proj.Projectionist.for_tiled(shape, wcs, (20, 20))
+    info = p.get_active_tiles(asm, assign=2)
+    hits = np.array(p.get_ProjEng('T').tile_hits(
+        np.asarray(p._cache_q_fp_to_native(asm.Q)), asm.fplane.quats))
+    assert info['active_tiles'] == list(np.nonzero(hits)[0])
+    assert sum(info['hit_counts']) == 2 * len(scan[0])
+    assert sorted(sum(info['group_tiles'], [])) == sorted(info['active_tiles'])
+    # the groups' ranges cover every sample exactly once
+    counts = sum(r.mask().sum(axis=0) for r in info['group_ranges'])
+    assert set(np.unique(counts)) == {1}
+
+
+@pytest.mark.parametrize('clippedness,tiled,interpol,method', list(itertools.product(
+    [0, 1, 2], [False, True], ['nearest', 'bilinear'], proj.wcs.THREAD_ASSIGNMENT_METHODS)))
+def test_proj_eng_threads(gpu, clippedness, tiled, interpol, method):
+    detail = f'(method={method}, tiled={tiled}, clippedness={clippedness}, interpol={interpol})'
+    scan, asm, (shape, wcs) = get_basics(clippedness=clippedness)
+    if tiled:
+        p = proj.Projectionist.for_tiled(shape, wcs, (150, 150), active_tiles=False,
+                                         interpol=interpol)
+    else:
+        p = proj.Projectionist.for_geom(shape, wcs, interpol=interpol)
+    sig = np.ones((2, len(scan[0])), 'float32')
+    n_threads = 3
+
+    if method in ['tiles'] and not tiled:
+        with pytest.raises(RuntimeError):
+            p.assign_threads(asm, method=method, n_threads=n_threads)
+        return
+    threads = p.assign_threads(asm, method=method, n_threads=n_threads)
+    assert isinstance(threads, list), detail
+
+    counts0 = threads[0].mask().sum(axis=0)
+    counts1 = np.zeros(counts0.shape, int)
+    assert threads[0].shape == (n_threads,) + sig.shape, detail
+    for t in threads[1:]:
+        counts1 += t.mask().sum(axis=0)
+        assert t.shape[1:] == sig.shape, detail
+
+    target = {1: set([0, 1]), 0: set([1]), 2: set([0])}[clippedness]
+    assert set((counts0 + counts1).ravel()) == target, detail
+    # Only the first bunch is non-empty, unless bilinear.
+    if interpol == 'nearest':
+        assert counts1.sum() == 0, detail
+
+    # The assignment, used as a work list, reproduces the unthreaded map.
+    if clippedness != 2:
+        if tiled and p.active_tiles is None:
+            p.active_tiles = p.get_active_tiles(asm)['active_tiles']
+        ref = p.to_map(sig, asm, comps='TQU')
+        got = p.to_map(sig, asm, comps='TQU', threads=threads)
+        if tiled:
+            ref = np.concatenate([t.ravel() for t in ref if t is not None])
+            got = np.concatenate([t.ravel() for t in got if t is not None])
+        assert np.allclose(got, ref, rtol=0, atol=1e-9), detail
+
+
+def test_proj_eng_hp_basic(gpu):
+    scan, asm, _ = get_assembly()
+    nside = 128
+    p = proj.ProjectionistHealpix.for_healpix(nside)
+    sig = np.ones((2, len(scan[0])), 'float32')
+    for comps in ['T', 'QU', 'TQU']:
+        m = p.to_map(sig, asm, comps=comps)
+        assert np.any(m != 0)
+        w = p.to_weights(asm, comps=comps)
+        assert np.any(w != 0)
+        m = np.asarray(m, dtype=np.float64)
+        s = np.asarray(p.from_map(m, asm, comps=comps))
+        assert s.shape == sig.shape and np.all(np.isfinite(s)) and np.any(s != 0)
+
+    m = p.to_map(sig, asm, comps='T', det_weights=np.array([0., 0.], dtype='float32'))[0]
+    assert np.all(m == 0)
+
+    asm.fplane.quats[1] = asm.fplane.quats[1] * np.nan
+    with pytest.raises(ValueError):
+        p.to_map(sig, asm, comps='T')
+    with pytest.raises(ValueError):
+        p.to_weights(asm, comps='T')
+
+
+@pytest.mark.parametrize('nside_tile', [8, 'auto'])
+def test_proj_eng_hp_tiled(gpu, nside_tile):
+    scan, asm, _ = get_assembly()
+    nside = 128
+    p = proj.ProjectionistHealpix.for_healpix(nside, nside_tile)
+    p.active_tiles = p.get_active_tiles(asm)['active_tiles']
+    assert isinstance(p.nside_tile, int) and 1 <= p.nside_tile <= nside
+    sig = np.ones((2, len(scan[0])), 'float32')
+    ref = proj.ProjectionistHealpix.for_healpix(nside, ordering='NEST')
+    for comps in ['T', 'QU', 'TQU']:
+        m = p.to_map(sig, asm, comps=comps)
+        m2 = [tile for tile in m if tile is not None]
+        assert np.any(m2)
+        w = p.to_weights(asm, comps=comps)
+        w2 = [tile for tile in w if tile is not None]
+        assert np.any(w2)
+        # tiled == untiled, tile by tile (NEST pixel ipix is tile ipix // npt)
+        full = np.asarray(ref.to_map(sig, asm, comps=comps))
+        npt = full.shape[-1] // len(m)
+        for i, tile in enumerate(m):
+            want = full[:, i * npt:(i + 1) * npt]
+            if tile is None:
+                assert np.all(want == 0)
+            else:
+                assert np.allclose(tile, want, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize('tiled,method', list(itertools.product([False, True],
+                                                                ['simple', 'tiles'])))
+def test_proj_eng_hp_threads(gpu, tiled, method):
+    interpol = 'nearest'
+    scan, asm, _ = get_assembly()
+    nside = 128
+    nside_tile = 8 if tiled else None
+    p = proj.ProjectionistHealpix.for_healpix(nside, nside_tile, interpol=interpol)
+    sig = np.ones((2, len(scan[0])), 'float32')
+    n_threads = 3
+    if method in ['tiles'] and not tiled:
+        with pytest.raises(RuntimeError):
+            p.assign_threads(asm, method=method, n_threads=n_threads)
+        return
+    threads = p.assign_threads(asm, method=method, n_threads=n_threads)
+    assert isinstance(threads, list)
+    counts0 = threads[0].mask().sum(axis=0)
+    counts1 = np.zeros(counts0.shape, int)
+    assert threads[0].shape == (n_threads,) + sig.shape
+    for t in threads[1:]:
+        counts1 += t.mask().sum(axis=0)
+        assert t.shape[1:] == sig.shape
+    assert set((counts0 + counts1).ravel()) == {1}
+    assert counts1.sum() == 0
+    with pytest.raises(ValueError):
+        p.assign_threads(asm, method='domdir', n_threads=n_threads)
