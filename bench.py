@@ -574,6 +574,7 @@ def c2_variants(torch, so3g_amd, N, lib, obs, pixargs, bore, dets, resp, sig, de
     d_in = torch.from_numpy(ptr_in).to(dev)
     d_out = torch.from_numpy(ptr_out).to(dev)
     d_dims = torch.from_numpy(np.ascontiguousarray(dims)).to(dev)
+    tshape = np.array([g.n_tile, 1], np.int64)
     pix = torch.empty((n_det, n_t, 3), dtype=torch.int32, device=dev)
     spin = torch.empty((n_det, n_t, 3), dtype=torch.float32, device=dev)
     chk(lib.so3g_pointing_matrix(g, bore.data_ptr(), n_t, dets.data_ptr(), n_det,
@@ -591,13 +592,13 @@ def c2_variants(torch, so3g_amd, N, lib, obs, pixargs, bore, dets, resp, sig, de
                             0, stream))
 
     def pc_from():
-        chk(lib.so3g_precomp_from_map(None, None, d_in.data_ptr(), d_dims.data_ptr(),
+        chk(lib.so3g_precomp_from_map(None, tshape.ctypes.data, d_in.data_ptr(), d_dims.data_ptr(),
                                       pix.data_ptr(), 3, spin.data_ptr(), 3, n_det, n_t,
                                       sig_out.data_ptr(), 0, stream))
 
     def pc_to():
         buf_out.zero_()
-        chk(lib.so3g_precomp_to_map(None, None, d_out.data_ptr(), d_dims.data_ptr(),
+        chk(lib.so3g_precomp_to_map(None, tshape.ctypes.data, d_out.data_ptr(), d_dims.data_ptr(),
                                     pix.data_ptr(), 3, spin.data_ptr(), 3, n_det, n_t,
                                     sig.data_ptr(), None, None, 0, stream))
     torch.cuda.synchronize()

@@ -189,7 +189,11 @@ int so3g_sample_domains(const so3g_geom *g, const double *bore, int64_t n_t,
 /* ProjEng_Precomp (Projection.cxx:2012-2253): projection with a cached
  * pointing matrix.  pixel: (n_det, n_t, k) int32 (k = 2 non-tiled flat,
  * 3 tiled flat, 1 for a 1-D (ncomp, npix) map); spin: (n_det, n_t, nc).
- * Non-tiled maps: map_shape = (ny, nx) (k=2) or (npix, 1) (k=1). */
+ * map_shape (host int64 pair): (ny, nx) for k = 2, (npix, 1) for k = 1,
+ * (n_tile, 1) for k = 3 (tile_dims then holds n_tile (rows, cols) pairs).
+ * Samples with pixel slot 0 < 0 are off the map.  An index outside the map
+ * (the reference does not check) skips its sample and makes the call
+ * return SO3G_ERR_RUNTIME; every call synchronises its stream to report it. */
 int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
                           const double *const *tiles, const int32_t *tile_dims,
                           const int32_t *pixel, int32_t k, const float *spin,
@@ -464,11 +468,8 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   map atomics (default 1)
  *   "scatter_xcd"   1: XCD-private partial maps for non-tiled scatters,
  *                   reduced after the kernel (default 0)
- *   "scatter_lds"   1: LDS-privatised partial maps for the cached (Precomp)
- *                   non-tiled scatter (default 0: measured slower, 12.8 vs 10.9 ms)
  *   "xcd_budget_mb" device memory allowed for those copies (default 4096)
- *   "pc_from_variant", "pc_from_dpb"  Precomp gather variant (default 2)
- *                   and detectors per block (default 2)
+ *   "pc_from_dpb"   detectors per block of the Precomp gather (default 2)
  *   "plan_div"      plan kernels' detector index by float reciprocal (1,
  *                   default) or integer division (0)
  *   "plan_from_v"   samples per lane of the planned gather (4 default, 8)

@@ -65,12 +65,6 @@ struct State {
     size_t chunk_cap = 0;
     double *copies_d = nullptr;   // XCD-private partial maps
     size_t copies_cap = 0;        // in doubles
-    // Detector order for the LDS-privatised scatter, cached per pixel array.
-    int64_t *keys_d = nullptr;
-    int32_t *order_d = nullptr;
-    int64_t order_cap = 0;
-    const void *order_pix = nullptr;
-    int64_t order_ndet = -1, order_nt = -1;
     double *packed_d = nullptr;   // pixel-major map copy for gathers
     size_t packed_cap = 0;
 };
@@ -80,10 +74,8 @@ struct State {
 struct Options {
     int scatter_agg = 1;
     int scatter_xcd = 0;
-    int scatter_lds = 0;
     int64_t xcd_budget_mb = 4096;
-    int pc_from_variant = 2;   // 0: 4 samples/lane, 1: + 2 detectors per iteration, 2: + pixel-major map copy
-    int pc_from_dpb = 2;       // detectors per block of the vectorised from_map
+    int pc_from_dpb = 2;       // detectors per block of the packed-map from_map
 };
 Options g_opt;
 constexpr int kNumXcd = 8;
@@ -196,7 +188,8 @@ int zero_flags(hipStream_t stream) {
 }
 
 // Synchronises and reports an inactive-tile hit the way the reference's
-// tiling_exception does (a RuntimeError).
+// tiling_exception does (a RuntimeError), and a cached pixel index outside
+// the map (which the reference does not check) as a RuntimeError too.
 int check_tile_flag(hipStream_t stream) {
     HIP_TRY(hipMemcpyAsync(g_state.flags_h, g_state.flags_d, FLAG_COUNT * sizeof(int),
                            hipMemcpyDeviceToHost, stream));
@@ -204,6 +197,8 @@ int check_tile_flag(hipStream_t stream) {
     if (g_state.flags_h[FLAG_TILE])
         return fail(SO3G_ERR_RUNTIME,
                     "Tiling problem: Attempted pointing operation on non-instantiated tile.");
+    if (g_state.flags_h[FLAG_RANGE])
+        return fail(SO3G_ERR_RUNTIME, "pixel_index holds an index outside the map");
     return SO3G_OK;
 }
 
@@ -538,7 +533,10 @@ static int pc_setup(const int64_t *map_shape, const int32_t *tile_dims, int32_t 
         pm.nx = map_shape[1];
     } else {
         if (tile_dims == nullptr) return fail(SO3G_ERR_RUNTIME, "tile dims required");
+        if (map_shape == nullptr || map_shape[0] < 0 || map_shape[0] > INT32_MAX)
+            return fail(SO3G_ERR_RUNTIME, "tiled maps need map_shape = (n_tile, 1)");
         pm.tile_dims = tile_dims;
+        pm.n_tile = (int32_t)map_shape[0];
     }
     (void)stream;
     return SO3G_OK;
@@ -579,11 +577,9 @@ int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
     if (r) return r;
     if (n_t == 0 || n_det == 0) return SO3G_OK;
     hipStream_t s = (hipStream_t)stream;
-    if (k == 3) {
-        r = zero_flags(s);
-        if (r) return r;
-    }
-    if (k == 2 && n_t % 4 == 0 && g_opt.pc_from_variant == 2) {
+    r = zero_flags(s);
+    if (r) return r;
+    if (k == 2 && n_t % 4 == 0) {
         in.dpb = std::max<int32_t>(in.dpb, (int32_t)g_opt.pc_from_dpb);
         const int64_t plane = pm.ny * pm.nx;
         if ((size_t)(plane * 4) > g_state.packed_cap) {
@@ -594,92 +590,30 @@ int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
             g_state.packed_cap = plane * 4;
         }
         double *pk = g_state.packed_d;
+        int *fl = g_state.flags_d;
         const dim3 gp(1024), g4((unsigned)((n_t / 4 + kBlock - 1) / kBlock),
                                 (unsigned)((n_det + in.dpb - 1) / in.dpb));
         switch (nc) {
         case 1:
             hipLaunchKernelGGL((k_pack_map<1>), gp, dim3(kBlock), 0, s, map, plane, pk);
-            hipLaunchKernelGGL((k_pc_from_map_packed<1>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate);
+            hipLaunchKernelGGL((k_pc_from_map_packed<1>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate, fl);
             break;
         case 2:
             hipLaunchKernelGGL((k_pack_map<2>), gp, dim3(kBlock), 0, s, map, plane, pk);
-            hipLaunchKernelGGL((k_pc_from_map_packed<2>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate);
+            hipLaunchKernelGGL((k_pc_from_map_packed<2>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate, fl);
             break;
         default:
             hipLaunchKernelGGL((k_pack_map<3>), gp, dim3(kBlock), 0, s, map, plane, pk);
-            hipLaunchKernelGGL((k_pc_from_map_packed<3>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate);
+            hipLaunchKernelGGL((k_pc_from_map_packed<3>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate, fl);
             break;
         }
         HIP_TRY(hipGetLastError());
-        return SO3G_OK;
-    }
-    if (k == 2 && n_t % 4 == 0 && g_opt.pc_from_variant == 1) {
-        in.dpb = std::max<int32_t>(in.dpb, (int32_t)g_opt.pc_from_dpb);
-        const dim3 g4((unsigned)((n_t / 4 + kBlock - 1) / kBlock),
-                      (unsigned)((n_det + in.dpb - 1) / in.dpb));
-        switch (nc) {
-        case 1: hipLaunchKernelGGL((k_pc_from_map_v4x2<1>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
-        case 2: hipLaunchKernelGGL((k_pc_from_map_v4x2<2>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
-        default: hipLaunchKernelGGL((k_pc_from_map_v4x2<3>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
-        }
-        HIP_TRY(hipGetLastError());
-        return SO3G_OK;
-    }
-    if (k == 2 && n_t % 4 == 0) {
-        in.dpb = std::max<int32_t>(in.dpb, (int32_t)g_opt.pc_from_dpb);
-        const dim3 g4((unsigned)((n_t / 4 + kBlock - 1) / kBlock),
-                      (unsigned)((n_det + in.dpb - 1) / in.dpb));
-        switch (nc) {
-        case 1: hipLaunchKernelGGL((k_pc_from_map_v4<1>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
-        case 2: hipLaunchKernelGGL((k_pc_from_map_v4<2>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
-        default: hipLaunchKernelGGL((k_pc_from_map_v4<3>), g4, dim3(kBlock), 0, s, in, pm, map, pixel, spin, signal, accumulate); break;
-        }
-        HIP_TRY(hipGetLastError());
-        return SO3G_OK;
+        return check_tile_flag(s);
     }
     PC_SWITCH(k, nc, k_pc_from_map, pc_grid(in, 0), dim3(kBlock), 0, s, in, pm, map, tiles,
               pixel, spin, signal, accumulate, g_state.flags_d);
     HIP_TRY(hipGetLastError());
-    return k == 3 ? check_tile_flag(s) : SO3G_OK;
-}
-
-// Detector order for k_pc_to_map_lds: detectors sorted by their pixel at
-// the middle sample (row-major), so each block's detectors sweep the same
-// map rows.  Cached while the same pixel array is passed again.
-static int det_order(const int32_t *pixel, int64_t n_det, int64_t n_t, int64_t nx,
-                     hipStream_t s, const int32_t **order) {
-    if (g_state.order_pix == pixel && g_state.order_ndet == n_det && g_state.order_nt == n_t) {
-        *order = g_state.order_d;
-        return SO3G_OK;
-    }
-    if (n_det > g_state.order_cap) {
-        if (g_state.keys_d) HIP_TRY(hipFree(g_state.keys_d));
-        if (g_state.order_d) HIP_TRY(hipFree(g_state.order_d));
-        g_state.keys_d = nullptr;
-        g_state.order_d = nullptr;
-        HIP_TRY(hipMalloc(&g_state.keys_d, n_det * sizeof(int64_t)));
-        HIP_TRY(hipMalloc(&g_state.order_d, n_det * sizeof(int32_t)));
-        g_state.order_cap = n_det;
-    }
-    hipLaunchKernelGGL(k_det_keys<int32_t>, dim3((unsigned)((n_det + 255) / 256)), dim3(256),
-                       0, s, pixel, n_det, n_t, nx, g_state.keys_d);
-    HIP_TRY(hipGetLastError());
-    std::vector<int64_t> keys(n_det);
-    HIP_TRY(hipMemcpyAsync(keys.data(), g_state.keys_d, n_det * sizeof(int64_t),
-                           hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    std::vector<int32_t> idx(n_det);
-    for (int64_t i = 0; i < n_det; i++) idx[i] = (int32_t)i;
-    std::stable_sort(idx.begin(), idx.end(),
-                     [&](int32_t a, int32_t b) { return keys[a] < keys[b]; });
-    HIP_TRY(hipMemcpyAsync(g_state.order_d, idx.data(), n_det * sizeof(int32_t),
-                           hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    g_state.order_pix = pixel;
-    g_state.order_ndet = n_det;
-    g_state.order_nt = n_t;
-    *order = g_state.order_d;
-    return SO3G_OK;
+    return check_tile_flag(s);
 }
 
 // Launches the precomp to_map variant selected by the options.
@@ -689,19 +623,6 @@ static int pc_to_map_launch(dim3 grid, hipStream_t s, const PcIn &in, const PcMa
                             const float *spin, const float *signal,
                             const float *det_weights) {
     const int64_t n = (int64_t)NC * pm.ny * pm.nx;
-    if constexpr (K == 2) {
-        if (g_opt.scatter_lds && in.chunks == nullptr && !g_opt.scatter_xcd) {
-            const int32_t *order = nullptr;
-            int r = det_order(pixel, in.n_det, in.n_t, pm.nx, s, &order);
-            if (r) return r;
-            const dim3 g2((unsigned)((in.n_t + kLdsT - 1) / kLdsT),
-                          (unsigned)((in.n_det + kLdsDets - 1) / kLdsDets));
-            hipLaunchKernelGGL((k_pc_to_map_lds<NC>), g2, dim3(kBlock), 0, s, in, pm, map,
-                               pixel, spin, signal, det_weights, order);
-            HIP_TRY(hipGetLastError());
-            return SO3G_OK;
-        }
-    }
     bool xcd = g_opt.scatter_xcd && K < 3 &&
                n * kNumXcd * 8 <= g_opt.xcd_budget_mb * (int64_t)(1 << 20);
     if (xcd && (size_t)(n * kNumXcd) > g_state.copies_cap) {
@@ -747,8 +668,6 @@ int so3g_set_option(const char *name, int64_t value) {
     std::string n(name ? name : "");
     if (n == "scatter_agg") g_opt.scatter_agg = (int)(value != 0);
     else if (n == "scatter_xcd") g_opt.scatter_xcd = (int)(value != 0);
-    else if (n == "scatter_lds") g_opt.scatter_lds = (int)(value != 0);
-    else if (n == "pc_from_variant") g_opt.pc_from_variant = (int)value;
     else if (n == "pc_from_dpb") g_opt.pc_from_dpb = (int)std::max<int64_t>(1, value);
     else if (n == "xcd_budget_mb") g_opt.xcd_budget_mb = value;
     else if (n == "plan_div") so3g::g_plan_div = (int)(value != 0);
@@ -784,8 +703,6 @@ int so3g_get_option(const char *name, int64_t *value) {
     std::string n(name ? name : "");
     if (n == "scatter_agg") *value = g_opt.scatter_agg;
     else if (n == "scatter_xcd") *value = g_opt.scatter_xcd;
-    else if (n == "scatter_lds") *value = g_opt.scatter_lds;
-    else if (n == "pc_from_variant") *value = g_opt.pc_from_variant;
     else if (n == "pc_from_dpb") *value = g_opt.pc_from_dpb;
     else if (n == "xcd_budget_mb") *value = g_opt.xcd_budget_mb;
     else if (n == "plan_div") *value = so3g::g_plan_div;
@@ -821,10 +738,8 @@ int so3g_precomp_to_map(double *map, const int64_t *map_shape, double *const *ti
         if (n_chunks == 0) return SO3G_OK;
         in.chunks = g_state.chunks_d;
     }
-    if (k == 3) {
-        r = zero_flags(s);
-        if (r) return r;
-    }
+    r = zero_flags(s);
+    if (r) return r;
     {
         const dim3 grid = pc_grid(in, n_chunks);
         switch (k * 4 + nc) {
@@ -841,7 +756,7 @@ int so3g_precomp_to_map(double *map, const int64_t *map_shape, double *const *ti
         }
         if (r) return r;
     }
-    return k == 3 ? check_tile_flag(s) : SO3G_OK;
+    return check_tile_flag(s);
 }
 
 int so3g_precomp_to_weights(double *map, const int64_t *map_shape, double *const *tiles,
@@ -862,14 +777,12 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape, double *const
         if (n_chunks == 0) return SO3G_OK;
         in.chunks = g_state.chunks_d;
     }
-    if (k == 3) {
-        r = zero_flags(s);
-        if (r) return r;
-    }
+    r = zero_flags(s);
+    if (r) return r;
     PC_SWITCH(k, nc, k_pc_to_weights, pc_grid(in, n_chunks), dim3(kBlock), 0, s, in, pm,
               map, tiles, pixel, spin, det_weights, g_opt.scatter_agg, g_state.flags_d);
     HIP_TRY(hipGetLastError());
-    return k == 3 ? check_tile_flag(s) : SO3G_OK;
+    return check_tile_flag(s);
 }
 
 }  // extern "C"

@@ -129,7 +129,7 @@ __device__ __forceinline__ bool wave_runs(int64_t key, VT (&v)[NC]) {
 }
 
 // Library-owned device flags.
-enum { FLAG_TILE = 0, FLAG_OFFMAP = 1, FLAG_COUNT = 4 };
+enum { FLAG_TILE = 0, FLAG_OFFMAP = 1, FLAG_RANGE = 2, FLAG_COUNT = 4 };
 
 __device__ __forceinline__ void raise_flag(int *flags, int which) {
     __hip_atomic_fetch_or(flags + which, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

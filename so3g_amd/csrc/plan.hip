@@ -218,14 +218,17 @@ __device__ __forceinline__ uint32_t compact_of(int64_t p, const uint32_t *bitmap
 // and one of them adds them with a single atomic (one atomic per sample on
 // the few thousand words of a map took 12.8 ms for 7.2e8 samples).  The
 // loop bound is uniform per wave, so every lane takes part in the shuffles.
+// A sample whose slot 0 is >= 0 but whose index lies outside the map sets
+// *range_bad (plan creation then fails, as the Precomp calls do).
 __global__ __launch_bounds__(kPlanBlock) void k_plan_mark(const int32_t *pix, PixGeom pg, int64_t n,
                                                           int64_t ny, int64_t nx,
-                                                          uint32_t *bitmap) {
+                                                          uint32_t *bitmap, int *range_bad) {
     const int lane = (int)__lane_id();
     for (int64_t i0 = (int64_t)blockIdx.x * kPlanBlock + (threadIdx.x & ~63); i0 < n;
          i0 += (int64_t)gridDim.x * kPlanBlock) {
         const int64_t i = i0 + lane;
         const int64_t p = i < n ? flat_pixel(pix, pg, i, ny, nx) : -1;
+        if (__any(i < n && p < 0 && pix[(int64_t)pg.k * i] >= 0) && lane == 0) *range_bad = 1;
         const int32_t w = p >= 0 ? (int32_t)(p >> 5) : -1;
         const uint32_t bit = p >= 0 ? 1u << (p & 31) : 0u;
         uint64_t todo = __ballot(w >= 0);
@@ -445,7 +448,7 @@ __device__ __forceinline__ uint32_t fast_div(uint32_t x, const FastDiv &f) {
         return x / f.d;
     } else {
         uint32_t q = (uint32_t)((float)x * f.inv);
-        const int32_t r = (int32_t)(x - q * f.d);   // |r| < 2 d < 2^31
+        const int32_t r = (int32_t)(x - q * f.d);   // -d < r < 2 d < 2^31 (d < 2^30)
         if (r < 0) q--;
         else if (r >= (int32_t)f.d) q++;
         return q;
@@ -1687,12 +1690,13 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
     int32_t *tile_of = nullptr, *tile_flat = nullptr;
     int64_t *seg = nullptr;
     unsigned long long *d_on = nullptr;
-    int *bad = nullptr;
+    int *bad = nullptr, *rbad = nullptr;
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
     auto cleanup = [&]() {
         (void)hipFree(keys_in); (void)hipFree(keys_out); (void)hipFree(vals_in);
         (void)hipFree(gcount); (void)hipFree(d_on); (void)hipFree(bad); (void)hipFree(tmp);
+        (void)hipFree(rbad); rbad = nullptr;
         (void)hipFree(bitmap); (void)hipFree(wcnt); (void)hipFree(wscan);
         (void)hipFree(used); (void)hipFree(tile_of); (void)hipFree(tile_flat); (void)hipFree(seg);
         keys_in = keys_out = nullptr; vals_in = gcount = bitmap = wcnt = wscan = nullptr;
@@ -1725,9 +1729,11 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
     PLAN_ALLOC(bitmap, n_words * 4);
     PLAN_ALLOC(wcnt, n_words * 4);
     PLAN_ALLOC(wscan, n_words * 4);
+    PLAN_ALLOC(rbad, sizeof(int));
     PLAN_STEP(hipMemsetAsync(bitmap, 0, n_words * 4, s));
+    PLAN_STEP(hipMemsetAsync(rbad, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_plan_mark, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, pg, n, ny,
-                       nx, bitmap);
+                       nx, bitmap, rbad);
     hipLaunchKernelGGL(k_plan_wcount, dim3(grid_of(n_words)), dim3(kPlanBlock), 0, s, bitmap,
                        n_words, wcnt);
     PLAN_STEP(hipGetLastError());
@@ -1738,10 +1744,18 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
                                       rocprim::plus<uint32_t>(), s));
     {
         uint32_t lb = 0, lc = 0;
+        int hbad = 0;
         PLAN_STEP(hipMemcpyAsync(&lb, wscan + n_words - 1, 4, hipMemcpyDeviceToHost, s));
         PLAN_STEP(hipMemcpyAsync(&lc, wcnt + n_words - 1, 4, hipMemcpyDeviceToHost, s));
+        PLAN_STEP(hipMemcpyAsync(&hbad, rbad, sizeof(int), hipMemcpyDeviceToHost, s));
         PLAN_STEP(hipStreamSynchronize(s));
         p->n_vis = (int64_t)lb + lc;
+        if (hbad) {
+            cleanup();
+            free_plan_arrays(p);
+            delete p;
+            return set_error(SO3G_ERR_RUNTIME, "pixel_index holds an index outside the map");
+        }
     }
     (void)hipFree(tmp); tmp = nullptr;
     PLAN_ALLOC(p->comp2pix, p->n_vis * 4);
@@ -2063,6 +2077,9 @@ int plan_check_args(int32_t nc, int64_t n_det, int64_t n_t, so3g_pc_plan **out) 
     if (n_det <= 0 || n_t <= 0 || n_det * n_t > (int64_t)UINT32_MAX)
         return set_error(SO3G_ERR_RUNTIME, "plan needs 0 < n_det * n_t < 2^32");
     if (n_det >= (1 << 22)) return set_error(SO3G_ERR_RUNTIME, "plan needs n_det < 2^22");
+    // fast_div's remainder x - q * n_t lies in (-n_t, 2 n_t) and is held in
+    // an int32.
+    if (n_t >= (1 << 30)) return set_error(SO3G_ERR_RUNTIME, "plan needs n_t < 2^30");
     return SO3G_OK;
 }
 

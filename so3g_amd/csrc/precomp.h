@@ -12,25 +12,41 @@ namespace so3g {
 // 3 (tile, y, x) indices.
 struct PcMap {
     int64_t ny, nx;            // non-tiled: plane = ny * nx (k=1: nx = 1)
-    const int32_t *tile_dims;  // tiled
+    const int32_t *tile_dims;  // tiled: (rows, cols) per tile
+    int32_t n_tile;            // tiled: tiles in tile_dims
 };
 
+// Status of a cached index tuple whose slot 0 is >= 0.  The reference
+// indexes the map without checking (Projection.cxx:2066-2154); here an
+// index outside the map is reported (PC_RANGE, raised as FLAG_RANGE and
+// turned into a RuntimeError after the launch) and the sample is skipped.
+enum { PC_OK = 0, PC_NOTILE = 1, PC_RANGE = 2 };
+
 template <int K, typename T>
-__device__ __forceinline__ T *pc_elem(T *map, T *const *tiles, const PcMap &pm,
-                                      const int32_t *pi, int64_t *plane) {
+__device__ __forceinline__ int pc_elem(T *map, T *const *tiles, const PcMap &pm,
+                                       const int32_t *pi, T **e, int64_t *plane) {
     if constexpr (K == 1) {
         *plane = pm.ny * pm.nx;
-        return map + pi[0];
+        if (pi[0] >= *plane) return PC_RANGE;
+        *e = map + pi[0];
     } else if constexpr (K == 2) {
         *plane = pm.ny * pm.nx;
-        return map + ((int64_t)pi[0] * pm.nx + pi[1]);
+        if (pi[0] >= pm.ny || pi[1] < 0 || pi[1] >= pm.nx) return PC_RANGE;
+        *e = map + ((int64_t)pi[0] * pm.nx + pi[1]);
     } else {
+        if (pi[0] >= pm.n_tile) return PC_RANGE;
         T *b = tiles[pi[0]];
-        if (b == nullptr) return nullptr;
+        if (b == nullptr) return PC_NOTILE;
         const int th = pm.tile_dims[2 * pi[0]], tw = pm.tile_dims[2 * pi[0] + 1];
+        if (pi[1] < 0 || pi[1] >= th || pi[2] < 0 || pi[2] >= tw) return PC_RANGE;
         *plane = (int64_t)th * tw;
-        return b + ((int64_t)pi[1] * tw + pi[2]);
+        *e = b + ((int64_t)pi[1] * tw + pi[2]);
     }
+    return PC_OK;
+}
+
+__device__ __forceinline__ void pc_flag(int *flags, int st) {
+    raise_flag(flags, st == PC_NOTILE ? FLAG_TILE : FLAG_RANGE);
 }
 
 struct PcIn {
@@ -73,9 +89,10 @@ __global__ __launch_bounds__(kBlock) void k_pc_from_map(PcIn in, PcMap pm, const
             return;
         }
         int64_t plane;
-        const double *e = pc_elem<K>(map, tiles, pm, pi, &plane);
-        if (e == nullptr) {
-            raise_flag(flags, FLAG_TILE);
+        const double *e = nullptr;
+        const int st = pc_elem<K>(map, tiles, pm, pi, &e, &plane);
+        if (st != PC_OK) {
+            pc_flag(flags, st);
             if (!accumulate) *sp = 0.f;
             return;
         }
@@ -84,110 +101,6 @@ __global__ __launch_bounds__(kBlock) void k_pc_from_map(PcIn in, PcMap pm, const
         for (int c = 0; c < NC; c++) s = s + spin[i * NC + c] * e[c * plane];
         *sp = accumulate ? *sp + s : s;
     });
-}
-
-// Vectorised precomp from_map for k = 2 maps and n_t % 4 == 0: four
-// consecutive samples per lane with 16-byte loads of the pixel indices, spin
-// factors and signal.  Same arithmetic as k_pc_from_map.
-template <int NC>
-__global__ __launch_bounds__(kBlock) void k_pc_from_map_v4(PcIn in, PcMap pm, const double *map,
-                                                           const int32_t *pix,
-                                                           const float *spin, float *sig,
-                                                           int accumulate) {
-    const int64_t t = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 4;
-    if (t >= in.n_t) return;
-    const int64_t plane = pm.ny * pm.nx;
-    const int64_t d0 = (int64_t)blockIdx.y * in.dpb;
-    const int64_t d1 = min(d0 + in.dpb, in.n_det);
-    for (int64_t d = d0; d < d1; d++) {
-        const int64_t i = d * in.n_t + t;
-        const int4 pa = *reinterpret_cast<const int4 *>(pix + 2 * i);
-        const int4 pb = *reinterpret_cast<const int4 *>(pix + 2 * i + 4);
-        float sp[4 * NC];
-#pragma unroll
-        for (int q = 0; q < NC; q++) {
-            const float4 f = reinterpret_cast<const float4 *>(spin + i * NC)[q];
-            sp[4 * q] = f.x; sp[4 * q + 1] = f.y; sp[4 * q + 2] = f.z; sp[4 * q + 3] = f.w;
-        }
-        const int py[4] = {pa.x, pa.z, pb.x, pb.z};
-        const int px[4] = {pa.y, pa.w, pb.y, pb.w};
-        float4 out = accumulate ? *reinterpret_cast<const float4 *>(sig + i)
-                                : float4{0.f, 0.f, 0.f, 0.f};
-        float o[4] = {out.x, out.y, out.z, out.w};
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            if (py[r] < 0) continue;
-            const double *e = map + ((int64_t)py[r] * pm.nx + px[r]);
-            float sr = 0;
-#pragma unroll
-            for (int c = 0; c < NC; c++) sr = sr + sp[r * NC + c] * e[c * plane];
-            o[r] = accumulate ? o[r] + sr : sr;
-        }
-        *reinterpret_cast<float4 *>(sig + i) = float4{o[0], o[1], o[2], o[3]};
-    }
-}
-
-// As k_pc_from_map_v4, two detectors per iteration with every independent
-// load issued before the map gathers (more bytes in flight per lane).
-template <int NC>
-__global__ __launch_bounds__(kBlock) void k_pc_from_map_v4x2(PcIn in, PcMap pm,
-                                                             const double *map,
-                                                             const int32_t *pix,
-                                                             const float *spin, float *sig,
-                                                             int accumulate) {
-    const int64_t t = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 4;
-    if (t >= in.n_t) return;
-    const int64_t plane = pm.ny * pm.nx;
-    const int64_t d0 = (int64_t)blockIdx.y * in.dpb;
-    const int64_t d1 = min(d0 + in.dpb, in.n_det);
-    for (int64_t d = d0; d < d1; d += 2) {
-        const bool two = d + 1 < d1;
-        int4 pa[2], pb[2];
-        float sp[2][4 * NC];
-        float o[2][4];
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-            const int64_t i = (d + (two ? u : 0)) * in.n_t + t;
-            pa[u] = *reinterpret_cast<const int4 *>(pix + 2 * i);
-            pb[u] = *reinterpret_cast<const int4 *>(pix + 2 * i + 4);
-#pragma unroll
-            for (int q = 0; q < NC; q++) {
-                const float4 f = reinterpret_cast<const float4 *>(spin + i * NC)[q];
-                sp[u][4 * q] = f.x; sp[u][4 * q + 1] = f.y;
-                sp[u][4 * q + 2] = f.z; sp[u][4 * q + 3] = f.w;
-            }
-            const float4 a = accumulate ? *reinterpret_cast<const float4 *>(sig + i)
-                                        : float4{0.f, 0.f, 0.f, 0.f};
-            o[u][0] = a.x; o[u][1] = a.y; o[u][2] = a.z; o[u][3] = a.w;
-        }
-        double mv[2][4][NC];
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-            const int py[4] = {pa[u].x, pa[u].z, pb[u].x, pb[u].z};
-            const int px[4] = {pa[u].y, pa[u].w, pb[u].y, pb[u].w};
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const double *e = map + (py[r] >= 0 ? ((int64_t)py[r] * pm.nx + px[r]) : 0);
-#pragma unroll
-                for (int c = 0; c < NC; c++) mv[u][r][c] = py[r] >= 0 ? e[c * plane] : 0.;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-            if (u == 1 && !two) break;
-            const int py[4] = {pa[u].x, pa[u].z, pb[u].x, pb[u].z};
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                if (py[r] < 0) continue;
-                float sr = 0;
-#pragma unroll
-                for (int c = 0; c < NC; c++) sr = sr + sp[u][r * NC + c] * mv[u][r][c];
-                o[u][r] = accumulate ? o[u][r] + sr : sr;
-            }
-            const int64_t i = (d + u) * in.n_t + t;
-            *reinterpret_cast<float4 *>(sig + i) = float4{o[u][0], o[u][1], o[u][2], o[u][3]};
-        }
-    }
 }
 
 // Pixel-major copy of a (NC, npix) map, padded to 4 components, so that the
@@ -205,13 +118,16 @@ __global__ __launch_bounds__(kBlock) void k_pack_map(const double *map, int64_t 
     }
 }
 
-// k_pc_from_map_v4x2 reading the packed map.
+// Precomp from_map for k = 2 maps and n_t % 4 == 0, reading the packed map:
+// four consecutive samples per lane (16-byte loads of the pixel pairs, spin
+// factors and signal), two detectors per iteration with every independent
+// load issued before the map gathers.  Same arithmetic as k_pc_from_map.
 template <int NC>
 __global__ __launch_bounds__(kBlock) void k_pc_from_map_packed(PcIn in, PcMap pm,
                                                                const double *packed,
                                                                const int32_t *pix,
                                                                const float *spin, float *sig,
-                                                               int accumulate) {
+                                                               int accumulate, int *flags) {
     const int64_t t = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 4;
     if (t >= in.n_t) return;
     const int64_t d0 = (int64_t)blockIdx.y * in.dpb;
@@ -243,7 +159,9 @@ __global__ __launch_bounds__(kBlock) void k_pc_from_map_packed(PcIn in, PcMap pm
             const int px[4] = {pa[u].y, pa[u].w, pb[u].y, pb[u].w};
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t p = py[r] >= 0 ? ((int64_t)py[r] * pm.nx + px[r]) : 0;
+                const bool ok = py[r] >= 0 && py[r] < pm.ny && px[r] >= 0 && px[r] < pm.nx;
+                if (py[r] >= 0 && !ok) raise_flag(flags, FLAG_RANGE);
+                const int64_t p = ok ? ((int64_t)py[r] * pm.nx + px[r]) : 0;
                 const double2 lo = reinterpret_cast<const double2 *>(packed + 4 * p)[0];
                 mv[u][r][0] = lo.x; mv[u][r][1] = lo.y;
                 if (NC > 2) {
@@ -256,9 +174,10 @@ __global__ __launch_bounds__(kBlock) void k_pc_from_map_packed(PcIn in, PcMap pm
         for (int u = 0; u < 2; u++) {
             if (u == 1 && !two) break;
             const int py[4] = {pa[u].x, pa[u].z, pb[u].x, pb[u].z};
+            const int px[4] = {pa[u].y, pa[u].w, pb[u].y, pb[u].w};
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                if (py[r] < 0) continue;
+                if (py[r] < 0 || py[r] >= pm.ny || px[r] < 0 || px[r] >= pm.nx) continue;
                 float sr = 0;
 #pragma unroll
                 for (int c = 0; c < NC; c++) sr = sr + sp[u][r * NC + c] * mv[u][r][c];
@@ -317,9 +236,10 @@ __global__ __launch_bounds__(kBlock) void k_pc_to_map(PcIn in, PcMap pm, double 
 #pragma unroll
             for (int j = 0; j < K; j++) pi[j] = pix[i * K + j];
             if (pi[0] >= 0) {
-                e = pc_elem<K>(mbase, tiles, pm, pi, &plane);
-                if (e == nullptr) {
-                    raise_flag(flags, FLAG_TILE);
+                const int st = pc_elem<K>(mbase, tiles, pm, pi, &e, &plane);
+                if (st != PC_OK) {
+                    pc_flag(flags, st);
+                    e = nullptr;
                 } else {
                     key = (int64_t)(uintptr_t)e;
                     const float s = sig[i];
@@ -336,143 +256,6 @@ __global__ __launch_bounds__(kBlock) void k_pc_to_map(PcIn in, PcMap pm, double 
             for (int c = 0; c < NC; c++) unsafeAtomicAdd(e + c * plane, v[c]);
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// LDS-privatised cached to_map for non-tiled flat maps (k = 2).
-//
-// A block owns kLdsDets detectors (taken in det_order, which groups
-// detectors that sweep the same map rows) x kLdsT consecutive samples.  It
-// estimates the block's pixel bounding box from every kLdsSub-th sample,
-// keeps a float64 partial map of that box in LDS, adds the run-aggregated
-// sums of its samples there with ds_add_f64, and finally adds the non-zero
-// box entries to the global map with coalesced FP64 atomics.  Samples that
-// fall outside the estimated box, and whole blocks whose box exceeds the LDS
-// capacity, use the direct run-aggregated global atomics instead, so the
-// result never depends on the estimate.
-// ---------------------------------------------------------------------------
-constexpr int kLdsDets = 16;
-constexpr int kLdsT = 1024;
-constexpr int kLdsSub = 8;
-constexpr int kLdsBytes = 64 * 1024;
-
-template <int NC>
-__global__ __launch_bounds__(kBlock) void k_pc_to_map_lds(PcIn in, PcMap pm, double *map,
-                                                          const int32_t *pix,
-                                                          const float *spin,
-                                                          const float *sig,
-                                                          const float *det_weights,
-                                                          const int32_t *det_order) {
-    constexpr int CAP = kLdsBytes / (8 * NC);     // pixels per component
-    __shared__ double part[NC * CAP];
-    __shared__ int red[4][kBlock / 64];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wid = tid >> 6;
-    const int64_t t0 = (int64_t)blockIdx.x * kLdsT;
-    const int64_t g0 = (int64_t)blockIdx.y * kLdsDets;
-    const int nd = (int)min((int64_t)kLdsDets, in.n_det - g0);
-    const int nt = (int)min((int64_t)kLdsT, in.n_t - t0);
-
-    // 1. Bounding box of a subsample of the block's on-map samples.
-    int ymin = INT32_MAX, xmin = INT32_MAX, ymax = -1, xmax = -1;
-    const int nsub = nd * ((nt + kLdsSub - 1) / kLdsSub);
-    const int per_det = (nt + kLdsSub - 1) / kLdsSub;
-    for (int k = tid; k < nsub; k += kBlock) {
-        const int64_t d = det_order[g0 + k / per_det];
-        const int64_t t = t0 + (int64_t)(k % per_det) * kLdsSub;
-        const int2 pi = *reinterpret_cast<const int2 *>(pix + 2 * (d * in.n_t + t));
-        if (pi.x >= 0) {
-            ymin = min(ymin, pi.x); ymax = max(ymax, pi.x);
-            xmin = min(xmin, pi.y); xmax = max(xmax, pi.y);
-        }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        ymin = min(ymin, __shfl_xor(ymin, off)); ymax = max(ymax, __shfl_xor(ymax, off));
-        xmin = min(xmin, __shfl_xor(xmin, off)); xmax = max(xmax, __shfl_xor(xmax, off));
-    }
-    if (lane == 0) { red[0][wid] = ymin; red[1][wid] = ymax; red[2][wid] = xmin; red[3][wid] = xmax; }
-    __syncthreads();
-    ymin = red[0][0]; ymax = red[1][0]; xmin = red[2][0]; xmax = red[3][0];
-#pragma unroll
-    for (int w = 1; w < kBlock / 64; w++) {
-        ymin = min(ymin, red[0][w]); ymax = max(ymax, red[1][w]);
-        xmin = min(xmin, red[2][w]); xmax = max(xmax, red[3][w]);
-    }
-    // One pixel of margin for the samples between the subsampled ones.
-    ymin = max(ymin - 1, 0); xmin = max(xmin - 1, 0);
-    ymax = min(ymax + 1, (int)pm.ny - 1); xmax = min(xmax + 1, (int)pm.nx - 1);
-    const int bh = ymax - ymin + 1, bw = xmax - xmin + 1;
-    const bool use_lds = ymax >= ymin && xmax >= xmin && (int64_t)bh * bw <= CAP;
-    const int area = use_lds ? bh * bw : 0;
-    for (int k = tid; k < NC * area; k += kBlock) part[k] = 0.;
-    __syncthreads();
-
-    // 2. Accumulate.
-    const int64_t plane = pm.ny * pm.nx;
-    for (int j = 0; j < nd; j++) {
-        const int64_t d = det_order[g0 + j];
-        float det_wt = 1.f;
-        if (det_weights != nullptr) {
-            det_wt = det_weights[d];
-            if (det_wt == 0.f) continue;
-        }
-        for (int s0 = 0; s0 < nt; s0 += kBlock) {
-            const int64_t t = t0 + s0 + tid;
-            int64_t key = -1;
-            double v[NC];
-#pragma unroll
-            for (int c = 0; c < NC; c++) v[c] = 0.;
-            if (s0 + tid < nt) {
-                const int64_t i = d * in.n_t + t;
-                const int2 pi = *reinterpret_cast<const int2 *>(pix + 2 * i);
-                if (pi.x >= 0) {
-                    key = (int64_t)pi.x * pm.nx + pi.y;
-                    const float sg = sig[i];
-#pragma unroll
-                    for (int c = 0; c < NC; c++)
-                        v[c] = (double)(det_wt * sg * spin[i * NC + c]);
-                }
-            }
-            if (wave_runs<NC>(key, v) && key >= 0) {
-                const int y = (int)(key / pm.nx), x = (int)(key - (int64_t)y * pm.nx);
-                if (use_lds && y >= ymin && y <= ymax && x >= xmin && x <= xmax) {
-                    const int o = (y - ymin) * bw + (x - xmin);
-#pragma unroll
-                    for (int c = 0; c < NC; c++)
-                        __hip_atomic_fetch_add(part + c * area + o, v[c], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                } else {
-#pragma unroll
-                    for (int c = 0; c < NC; c++) unsafeAtomicAdd(map + c * plane + key, v[c]);
-                }
-            }
-        }
-    }
-    __syncthreads();
-
-    // 3. Flush the box.
-    for (int k = tid; k < NC * area; k += kBlock) {
-        const double v = part[k];
-        if (v != 0.) {
-            const int c = k / area, o = k - c * area;
-            const int y = ymin + o / bw, x = xmin + o % bw;
-            unsafeAtomicAdd(map + c * plane + (int64_t)y * pm.nx + x, v);
-        }
-    }
-}
-
-// det_key[d] = row-major pixel of detector d at the middle sample (or a
-// large value when off the map): sorting detectors by it groups those that
-// sweep the same rows.
-template <typename T>
-__global__ void k_det_keys(const T *pix, int64_t n_det, int64_t n_t, int64_t nx,
-                           int64_t *keys) {
-    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= n_det) return;
-    const int64_t t = n_t / 2;
-    const int2 pi = *reinterpret_cast<const int2 *>(pix + 2 * (d * n_t + t));
-    keys[d] = pi.x >= 0 ? (int64_t)pi.x * nx + pi.y : INT64_MAX;
 }
 
 // map[i] += sum over the n_copy XCD-private partial maps, fixed order.
@@ -537,9 +320,10 @@ __global__ __launch_bounds__(kBlock) void k_pc_to_weights(PcIn in, PcMap pm, dou
             for (int j = 0; j < K; j++) pi[j] = pix[i * K + j];
             if (pi[0] >= 0) {
                 int64_t plane;
-                double *ej = pc_elem<K>(map, tiles, pm, pi, &plane);
-                if (ej == nullptr) {
-                    raise_flag(flags, FLAG_TILE);
+                double *ej = nullptr;
+                const int st = pc_elem<K>(map, tiles, pm, pi, &ej, &plane);
+                if (st != PC_OK) {
+                    pc_flag(flags, st);
                 } else {
                     e[0] = ej;
                     key[0] = (int64_t)(uintptr_t)ej;

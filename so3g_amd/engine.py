@@ -143,12 +143,15 @@ def _ptr(t):
 
 # -- signal-like objects (SignalSpace, Projection.cxx:1212-1298) -------------
 
-def _signal_space(call, obj, name, dtype, n_det, n_t, third=None, zero_if_none=True):
+def _signal_space(call, obj, name, dtype, n_det, n_t, third=None, zero_if_none=True,
+                  out=True):
     """Return (device tensor (n_det, n_t[, third]), result object, fresh).
 
     obj may be None (zeros allocated; result is a list of rows), a list of
     1-d arrays, a 2-d (or 3-d) numpy array (rows written back in place) or
-    a torch tensor (used in place)."""
+    a torch tensor (used in place).  out=False marks an input that the call
+    only reads: a staged copy is then not written back to the caller's
+    arrays."""
     tail = (n_t,) if third is None else (n_t, third)
     if obj is None:
         t = call.alloc((n_det,) + tail, dtype)
@@ -156,7 +159,7 @@ def _signal_space(call, obj, name, dtype, n_det, n_t, third=None, zero_if_none=T
     if _is_torch(obj):
         spec = [n_det] + list(tail)
         spec = [s if (s is not None and s >= 0) else -1 for s in spec]
-        t = call.array(obj, dtype, name, spec=spec, out=True)
+        t = call.array(obj, dtype, name, spec=spec, out=out)
         return t, obj, False
     if isinstance(obj, (list, tuple)):
         rows = list(obj)
@@ -193,7 +196,8 @@ def _signal_space(call, obj, name, dtype, n_det, n_t, third=None, zero_if_none=T
                 for r, v in zip(self.rows, val):
                     r[...] = v
 
-        call.writeback.append((t, _RowsWriter(arrs)))
+        if out:
+            call.writeback.append((t, _RowsWriter(arrs)))
         return t, rows, False
     a = np.asarray(obj)
     spec = [n_det] + [s if (s is not None and s >= 0) else -1 for s in tail]
@@ -201,7 +205,7 @@ def _signal_space(call, obj, name, dtype, n_det, n_t, third=None, zero_if_none=T
         raise ValueError(f"Expected buffer '{name}' to contain items of type "
                          f"{_type_name(dtype)}.")
     _check_shape(name, a.shape, spec)
-    t = call.array(a, dtype, name, out=True)
+    t = call.array(a, dtype, name, out=out)
     return t, a, False
 
 
@@ -499,7 +503,7 @@ class ProjEngBase:
         if map is None:
             map = self.zeros((self.comp_count,))
         mp, tp, dp = self._map(call, map, [self.comp_count])
-        sig, _, _ = _signal_space(call, signal, 'signal', 'float32', n_det, n_t)
+        sig, _, _ = _signal_space(call, signal, 'signal', 'float32', n_det, n_t, out=False)
         dw = call.array(det_weights, 'float32', 'det_weights', spec=[n_det], optional=True)
         resp = call.array(response, 'float32', 'response', spec=[n_det, 2])
         segs = _flatten_intervals(thread_intervals, n_det, n_t)
@@ -544,7 +548,7 @@ class ProjEngBase:
         if map is None:
             map = self._zeros_dtype((self.comp_count,), dt)
         mp, tp, dp = self._map(call, map, [self.comp_count], dtype=dt)
-        sig, _, _ = _signal_space(call, signal, 'signal', 'float32', n_det, n_t)
+        sig, _, _ = _signal_space(call, signal, 'signal', 'float32', n_det, n_t, out=False)
         dw = call.array(det_weights, 'float32', 'det_weights', spec=[n_det], optional=True)
         resp = call.array(response, 'float32', 'response', spec=[n_det, 2])
         N.check(N.load().so3g_to_map_prec(self._g, mp, tp, dp, bore.data_ptr(), n_t,
@@ -638,7 +642,8 @@ class ProjEngPrecompBase:
         pass
 
     def _pmat(self, call, pixel_index, spin_proj):
-        pix, _, _ = _signal_space(call, pixel_index, 'pixel_index', 'int32', -1, -1, -1)
+        pix, _, _ = _signal_space(call, pixel_index, 'pixel_index', 'int32', -1, -1, -1,
+                                  out=False)
         if pix.dim() != 3:
             raise RuntimeError("Buffer 'pixel_index' has incompatible shape: Expected "
                                "(*, *, *).")
@@ -648,7 +653,8 @@ class ProjEngPrecompBase:
             if a.ndim == 3 and a.strides[2] != a.itemsize and a.shape[2] > 1:
                 raise RuntimeError("Buffer 'pixel_index' has incompatible shape: Fast "
                                    "dimension of pixel indices must be close-packed..")
-        spin, _, _ = _signal_space(call, spin_proj, 'spin_proj', 'float32', n_det, n_t, -1)
+        spin, _, _ = _signal_space(call, spin_proj, 'spin_proj', 'float32', n_det, n_t, -1,
+                                   out=False)
         return pix, spin, n_det, n_t, k, spin.shape[2]
 
     def _map(self, call, m, nlead, k, writeback):
@@ -677,8 +683,9 @@ class ProjEngPrecompBase:
             dims[i] = (t.shape[-2], t.shape[-1])
         ptr_t = torch.from_numpy(ptrs).to(call.dev)
         dims_t = torch.from_numpy(dims).to(call.dev)
-        call.keep += [ptr_t, dims_t]
-        return None, None, ptr_t.data_ptr(), dims_t.data_ptr()
+        shape = np.array([len(m), 1], np.int64)
+        call.keep += [ptr_t, dims_t, shape]
+        return None, shape.ctypes.data, ptr_t.data_ptr(), dims_t.data_ptr()
 
     def from_map(self, map, pixel_index, spin_proj, signal=None):
         call = _Call()
@@ -696,7 +703,7 @@ class ProjEngPrecompBase:
         call = _Call()
         pix, spin, n_det, n_t, k, nc = self._pmat(call, pixel_index, spin_proj)
         mp, sp, tp, dp = self._map(call, map, 1, k, writeback=True)
-        sig, _, _ = _signal_space(call, signal, 'signal', 'float32', n_det, n_t)
+        sig, _, _ = _signal_space(call, signal, 'signal', 'float32', n_det, n_t, out=False)
         dw = call.array(weights, 'float32', 'det_weights', spec=[n_det], optional=True)
         segs = _flatten_intervals(thread_intervals, n_det, n_t)
         seg_ptr, n_seg = (None, 0) if segs is None else (segs.ctypes.data, len(segs))

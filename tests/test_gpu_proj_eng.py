@@ -151,7 +151,12 @@ def test_proj_eng_threads(gpu, clippedness, tiled, interpol, method):
         assert counts1.sum() == 0, detail
 
     # The assignment, used as a work list, reproduces the unthreaded map.
-    if clippedness != 2:
+    # Not for 'domdir' on a clipped bilinear map: the reference labels samples
+    # through a nearest-neighbour Projectionist (mapthreads.py:84-88), so
+    # edge samples that only bilinear interpolation puts on the map are in no
+    # range and are skipped, there as here.
+    if clippedness == 0 or (clippedness == 1 and not (method == 'domdir' and
+                                                     interpol == 'bilinear')):
         if tiled and p.active_tiles is None:
             p.active_tiles = p.get_active_tiles(asm)['active_tiles']
         ref = p.to_map(sig, asm, comps='TQU')

@@ -227,6 +227,14 @@ def test_plan_argument_errors():
         (RuntimeError, lambda: lib.so3g_pc_plan_create(None, 2, None, 3, 1 << 22, 1,
                                                        shape.ctypes.data, -1, ctypes.byref(h),
                                                        None)),
+        # the detector index's float-reciprocal division needs n_t < 2^30,
+        # and sample indices must fit 32 bits
+        (RuntimeError, lambda: lib.so3g_pc_plan_create(None, 2, None, 3, 1, 1 << 30,
+                                                       shape.ctypes.data, -1, ctypes.byref(h),
+                                                       None)),
+        (RuntimeError, lambda: lib.so3g_pc_plan_create(None, 2, None, 3, 3, (1 << 30) - 1,
+                                                       shape.ctypes.data, -1, ctypes.byref(h),
+                                                       None)),
         (ValueError, lambda: lib.so3g_pc_plan_create_tiled(None, None, 3, 1, 1, None, 0, -1,
                                                            ctypes.byref(h), None)),
         (ValueError, lambda: lib.so3g_pc_plan_create_tiled(None, None, 3, 1, 1,
