@@ -67,6 +67,16 @@ struct State {
     size_t copies_cap = 0;        // in doubles
     double *packed_d = nullptr;   // pixel-major map copy for gathers
     size_t packed_cap = 0;
+    // Detector order of the LDS-privatised scatter, cached for the last
+    // (boresight, detectors, geometry) it was computed for.  It only steers
+    // which detectors share a block, so a stale entry costs speed, never
+    // correctness.
+    int32_t *order_d = nullptr;
+    double *order_co_d = nullptr;
+    int64_t order_cap = 0;
+    const void *order_bore = nullptr, *order_dets = nullptr;
+    int64_t order_nt = -1, order_ndet = -1;
+    so3g_geom order_geom{};
 };
 
 // Tunables (so3g_set_option): wave-level run aggregation of scatter-adds,
@@ -76,6 +86,12 @@ struct Options {
     int scatter_xcd = 0;
     int64_t xcd_budget_mb = 4096;
     int pc_from_dpb = 2;       // detectors per block of the packed-map from_map
+    // On-the-fly to_map / to_weights through LDS-privatised partial maps:
+    // 0 never, 1 where measured faster than global atomics (HEALPix maps,
+    // and to_weights on non-tiled flat maps), 2 always.
+    int scatter_hash = 1;
+    int hash_dets = 2;         // detectors per block of that scatter
+    int hash_span = 32768;     // samples per block of that scatter
 };
 Options g_opt;
 constexpr int kNumXcd = 8;
@@ -397,6 +413,74 @@ int so3g_from_map(const so3g_geom *g, const double *map, const double *const *ti
     return g->tiled ? check_tile_flag(a.stream) : SO3G_OK;
 }
 
+// Detector order for k_scatter_lds: each detector's coordinates at the
+// middle sample (one k_coords launch over n_det samples), sorted by map row
+// (the coordinate across the scan, in units of the pixel height) and then
+// along the row, so that a block's detectors sweep the same pixels.
+static int scatter_order(const so3g_geom *g, const OpArgs &a0, const int32_t **order) {
+    const int64_t n_det = a0.in.n_det, n_t = a0.in.n_t;
+    if (g_state.order_bore == a0.in.bore && g_state.order_dets == a0.in.dets &&
+        g_state.order_nt == n_t && g_state.order_ndet == n_det &&
+        memcmp(&g_state.order_geom, g, sizeof(*g)) == 0) {
+        *order = g_state.order_d;
+        return SO3G_OK;
+    }
+    if (n_det > g_state.order_cap) {
+        if (g_state.order_d) HIP_TRY(hipFree(g_state.order_d));
+        if (g_state.order_co_d) HIP_TRY(hipFree(g_state.order_co_d));
+        g_state.order_d = nullptr;
+        g_state.order_co_d = nullptr;
+        g_state.order_cap = 0;
+        HIP_TRY(hipMalloc(&g_state.order_d, n_det * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&g_state.order_co_d, n_det * 4 * sizeof(double)));
+        g_state.order_cap = n_det;
+    }
+    OpArgs a = a0;
+    a.op = OP_COORDS;
+    a.in.bore = a0.in.bore + 4 * (n_t / 2);
+    a.in.n_t = 1;
+    a.in.chunks = nullptr;
+    a.coords_out = g_state.order_co_d;
+    int r = run_dispatch(g->proj, a);
+    if (r) return r;
+    std::vector<double> co(n_det * 4);
+    HIP_TRY(hipMemcpyAsync(co.data(), g_state.order_co_d, n_det * 4 * sizeof(double),
+                           hipMemcpyDeviceToHost, a.stream));
+    HIP_TRY(hipStreamSynchronize(a.stream));
+    double row = 1.;
+    if (g->pixelizor == SO3G_PIX_FLAT) row = std::fabs(g->cdelt[0]);
+    else row = std::sqrt(4. * M_PI / (12. * (double)g->nside * g->nside));
+    if (!(row > 0.)) row = 1.;
+    std::vector<int64_t> rows(n_det);
+    for (int64_t d = 0; d < n_det; d++) {
+        const double y = std::floor(co[4 * d + 1] / row);
+        rows[d] = std::isfinite(y) ? (int64_t)std::max(-1e15, std::min(1e15, y)) : 0;
+    }
+    std::vector<int32_t> idx(n_det);
+    for (int64_t i = 0; i < n_det; i++) idx[i] = (int32_t)i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
+        if (rows[x] != rows[y]) return rows[x] < rows[y];
+        return co[4 * x] < co[4 * y];
+    });
+    HIP_TRY(hipMemcpyAsync(g_state.order_d, idx.data(), n_det * sizeof(int32_t),
+                           hipMemcpyHostToDevice, a.stream));
+    HIP_TRY(hipStreamSynchronize(a.stream));
+    g_state.order_bore = a0.in.bore;
+    g_state.order_dets = a0.in.dets;
+    g_state.order_nt = n_t;
+    g_state.order_ndet = n_det;
+    g_state.order_geom = *g;
+    *order = g_state.order_d;
+    return SO3G_OK;
+}
+
+// True when every map element of the geometry has an int32 key.
+static bool hash_keys_fit(const so3g_geom *g) {
+    if (g->pixelizor != SO3G_PIX_FLAT) return true;   // 12 nside^2 <= 8.1e8
+    if (!g->tiled) return g->npix < INT32_MAX;
+    return (int64_t)g->n_tile * g->tile_shape[0] * g->tile_shape[1] < INT32_MAX;
+}
+
 static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
                          const int32_t *tile_dims, const double *bore, int64_t n_t,
                          const double *dets, int64_t n_det, const float *resp,
@@ -426,6 +510,24 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
     a.resp = resp;
     a.sig_in = signal;
     a.det_w = det_weights;
+    a.hash_cls = 0;
+    // Where the LDS tables pay (1000 x 720000 samples, ms, global atomics
+    // -> LDS): HEALPix nside 4096 to_map 18.5 -> 15.0, to_weights 33.6 ->
+    // 20.0; CAR 0.5' to_weights 18.9 -> 15.7.  Not for CAR to_map (10.2 ->
+    // 11.3) nor tiled CAR to_map (11.0 -> 13.8): a 112 KB table leaves one
+    // block (4 waves per SIMD) per compute unit, and the latency-bound
+    // float64 pointing loses more to that than the atomics it saves there.
+    const bool hash_pays = g->pixelizor == SO3G_PIX_HEALPIX ||
+                           (op == OP_TO_WEIGHTS && !g->tiled);
+    if ((g_opt.scatter_hash == 2 || (g_opt.scatter_hash == 1 && hash_pays)) &&
+        prec == SO3G_PREC_REF && segs == nullptr && hash_keys_fit(g)) {
+        r = scatter_order(g, a, &a.hw.order);
+        if (r) return r;
+        a.hash_cls = 1;
+        // gridDim.y (detector groups) stays within 65535
+        a.hw.dets = (int32_t)std::max<int64_t>(g_opt.hash_dets, (n_det + 65534) / 65535);
+        a.hw.span = g_opt.hash_span;
+    }
     r = run_dispatch(g->proj, a);
     if (r) return r;
     return g->tiled ? check_tile_flag(a.stream) : SO3G_OK;
@@ -669,6 +771,17 @@ int so3g_set_option(const char *name, int64_t value) {
     if (n == "scatter_agg") g_opt.scatter_agg = (int)(value != 0);
     else if (n == "scatter_xcd") g_opt.scatter_xcd = (int)(value != 0);
     else if (n == "pc_from_dpb") g_opt.pc_from_dpb = (int)std::max<int64_t>(1, value);
+    else if (n == "scatter_hash") {
+        if (value < 0 || value > 2) return fail(SO3G_ERR_VALUE, "scatter_hash must be 0..2");
+        g_opt.scatter_hash = (int)value;
+    } else if (n == "hash_dets") {
+        if (value < 1 || value > 4096) return fail(SO3G_ERR_VALUE, "hash_dets must be 1..4096");
+        g_opt.hash_dets = (int)value;
+    } else if (n == "hash_span") {
+        if (value < 64 || value > (1 << 24))
+            return fail(SO3G_ERR_VALUE, "hash_span must be 64..2^24");
+        g_opt.hash_span = (int)value;
+    }
     else if (n == "xcd_budget_mb") g_opt.xcd_budget_mb = value;
     else if (n == "plan_div") so3g::g_plan_div = (int)(value != 0);
     else if (n == "plan_from_layout") so3g::g_plan_from_layout = (int)(value != 0);
@@ -704,6 +817,9 @@ int so3g_get_option(const char *name, int64_t *value) {
     if (n == "scatter_agg") *value = g_opt.scatter_agg;
     else if (n == "scatter_xcd") *value = g_opt.scatter_xcd;
     else if (n == "pc_from_dpb") *value = g_opt.pc_from_dpb;
+    else if (n == "scatter_hash") *value = g_opt.scatter_hash;
+    else if (n == "hash_dets") *value = g_opt.hash_dets;
+    else if (n == "hash_span") *value = g_opt.hash_span;
     else if (n == "xcd_budget_mb") *value = g_opt.xcd_budget_mb;
     else if (n == "plan_div") *value = so3g::g_plan_div;
     else if (n == "plan_from_layout") *value = so3g::g_plan_from_layout;

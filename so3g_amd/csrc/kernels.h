@@ -451,6 +451,195 @@ __global__ __launch_bounds__(kBlock) void k_to_weights(PtgIn in, Geo g,
 }
 
 // --------------------------------------------------------------------------
+// to_map / to_weights through LDS-privatised partial maps.
+//
+// A block owns G detectors, taken in `order` (detectors sorted by the map
+// row they see at mid-observation, then by column: row-mates, which in a
+// constant-elevation scan sweep the same pixels), over T consecutive
+// samples.  Each thread keeps its sample's boresight in registers and walks
+// the block's detectors; runs of equal pixels within a wave are summed
+// first (wave_runs) and each run's total is added to an open-addressing
+// hash table of the block's pixels in LDS (int32 pixel keys, NV float64
+// sums per slot, linear probing).  A long time span lets the scan's
+// repeated sweeps add into the same slots, so each touched pixel costs one
+// global FP64 atomic per value per block when the table is flushed instead
+// of one per run.  A run that finds no slot within kProbes probes (table
+// full) adds straight to the map, so the result does not depend on the
+// table size; the sums are float64 sums of the same float32 products as
+// k_to_map / k_to_weights in another order.
+// --------------------------------------------------------------------------
+struct HashWork {
+    const int32_t *order;   // detector order, n_det entries
+    int32_t dets;           // G: detectors per block
+    int32_t span;           // T: samples per block
+};
+
+constexpr int32_t kHashEmpty = -1;
+constexpr int kProbes = 32;
+
+// int32 key of a map element (map pixel numbering of the pixelizor; tiled
+// flat maps number tile t's pixels from t * ty * tx).
+template <int PK>
+__device__ __forceinline__ int32_t hash_key(const Geo &g, const int32_t *tile_dims,
+                                            const Pt &p) {
+    if constexpr (PK == PK_FLAT) return p.i0 * g.nx + p.i1;
+    else if constexpr (PK == PK_HP) return p.i0;
+    else if constexpr (PK == PK_HP_TILED) return p.i0 * g.npt + p.i1;
+    else return p.i0 * (g.ty * g.tx) + p.i1 * tile_dims[2 * p.i0 + 1] + p.i2;
+}
+
+// Map element of slot `lead` for a key (inverse of hash_key).
+template <int PK>
+__device__ __forceinline__ double *hash_elem(const Geo &g, const MapView<double> &m,
+                                             int32_t key, int lead) {
+    if constexpr (PK == PK_FLAT || PK == PK_HP) {
+        return m.map + lead * g.npix + key;
+    } else if constexpr (PK == PK_HP_TILED) {
+        const int32_t t = key / g.npt;
+        return m.tiles[t] + (int64_t)lead * g.npt + (key - t * g.npt);
+    } else {
+        const int32_t ts = g.ty * g.tx;
+        const int32_t t = key / ts;
+        const int64_t plane = (int64_t)m.tile_dims[2 * t] * m.tile_dims[2 * t + 1];
+        return m.tiles[t] + lead * plane + (key - t * ts);
+    }
+}
+
+// Adds v to the slot of key; false when no slot was found in kProbes probes.
+template <int NV, int H>
+__device__ __forceinline__ bool hash_add(int32_t *hk, double *hv, int32_t key,
+                                         const double (&v)[NV]) {
+    constexpr int kBits = __builtin_ctz(H);
+    uint32_t h = ((uint32_t)key * 2654435761u) >> (32 - kBits);
+    for (int probe = 0; probe < kProbes; probe++) {
+        // One LDS round trip per probe: the compare-and-swap claims an empty
+        // slot and returns the key already there otherwise.
+        int32_t cur = kHashEmpty;
+        __hip_atomic_compare_exchange_strong(hk + h, &cur, key, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == kHashEmpty || cur == key) {
+#pragma unroll
+            for (int c = 0; c < NV; c++)
+                __hip_atomic_fetch_add(hv + c * H + h, v[c], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            return true;
+        }
+        h = (h + 1) & (H - 1);
+    }
+    return false;
+}
+
+// Map slot (a * NC + b) of the k-th upper-triangle product (a <= b) of a
+// weights map, in the order the products are formed.
+template <int NC>
+__device__ __forceinline__ int hash_lead(int k) {
+    for (int a = 0; a < NC; a++)
+        for (int b = a; b < NC; b++, k--)
+            if (k == 0) return a * NC + b;
+    return 0;
+}
+
+// W = false: to_map (NV = NC values per sample, Projection.cxx:1807);
+// W = true: to_weights (the NC (NC + 1) / 2 upper-triangle products,
+// Projection.cxx:1850).
+template <int PROJ, int PK, int INTERP, int NC, bool W, int NT, int H>
+__global__ __launch_bounds__(NT) void k_scatter_lds(PtgIn in, Geo g, MapView<double> m,
+                                                    const float *resp, const float *sig,
+                                                    const float *det_weights, HashWork hw,
+                                                    int *flags) {
+    constexpr int NP = INTERP ? 4 : 1;
+    constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
+    __shared__ int32_t hk[H];
+    __shared__ double hv[NV * H];
+    for (int i = threadIdx.x; i < H; i += NT) {
+        hk[i] = kHashEmpty;
+#pragma unroll
+        for (int c = 0; c < NV; c++) hv[c * H + i] = 0.;
+    }
+    __syncthreads();
+    const int64_t g0 = (int64_t)blockIdx.y * hw.dets;
+    const int nd = (int)min((int64_t)hw.dets, in.n_det - g0);
+    const int64_t t_beg = (int64_t)blockIdx.x * hw.span;
+    const int64_t t_end = min(t_beg + (int64_t)hw.span, in.n_t);
+    for (int64_t tb = t_beg; tb < t_end; tb += NT) {
+        const int64_t t = tb + threadIdx.x;
+        const bool valid = t < t_end;
+        const Q4 qb = valid ? load_q(in.bore + 4 * t) : Q4{1., 0., 0., 0.};
+        for (int j = 0; j < nd; j++) {
+            const int64_t d = hw.order[g0 + j];
+            float det_wt = 1.f;
+            if (det_weights != nullptr) {
+                det_wt = det_weights[d];
+                if (det_wt == 0.f) continue;
+            }
+            int32_t key[NP];
+            double v[NP][NV];
+#pragma unroll
+            for (int jp = 0; jp < NP; jp++) {
+                key[jp] = -1;
+#pragma unroll
+                for (int c = 0; c < NV; c++) v[jp][c] = 0.;
+            }
+            if (valid) {
+                const Q4 qd = load_q(in.dets + 4 * d);
+                double co[4];
+                get_coords<PROJ>(qb, qd, in.tabs, co);
+                float pf[3];
+                spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
+                float s = 0.f;
+                if constexpr (!W) s = sig[d * in.n_t + t];
+                Pt p[4];
+                const int np = get_pixels<PK, INTERP>(g, co, p);
+#pragma unroll
+                for (int jp = 0; jp < NP; jp++) {
+                    if (jp >= np) break;
+                    if constexpr (PK == PK_FLAT_TILED || PK == PK_HP_TILED) {
+                        if (m.tiles[p[jp].i0] == nullptr) {
+                            raise_flag(flags, FLAG_TILE);
+                            continue;
+                        }
+                    }
+                    key[jp] = hash_key<PK>(g, m.tile_dims, p[jp]);
+                    const float w = p[jp].w;
+                    if constexpr (W) {
+                        int k = 0;
+#pragma unroll
+                        for (int a = 0; a < NC; a++)
+#pragma unroll
+                            for (int b = a; b < NC; b++, k++)
+                                v[jp][k] = (double)(pf[a] * pf[b] * w * w * det_wt);
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < NC; c++) v[jp][c] = (double)(s * pf[c] * w * det_wt);
+                    }
+                }
+            }
+#pragma unroll
+            for (int jp = 0; jp < NP; jp++) {
+                const bool emit = wave_runs<NV>((int64_t)key[jp], v[jp]) && key[jp] >= 0;
+                if (emit && !hash_add<NV, H>(hk, hv, key[jp], v[jp])) {
+#pragma unroll
+                    for (int c = 0; c < NV; c++) {
+                        const int lead = W ? hash_lead<NC>(c) : c;
+                        unsafeAtomicAdd(hash_elem<PK>(g, m, key[jp], lead), v[jp][c]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < H; i += NT) {
+        const int32_t key = hk[i];
+        if (key == kHashEmpty) continue;
+#pragma unroll
+        for (int c = 0; c < NV; c++) {
+            const int lead = W ? hash_lead<NC>(c) : c;
+            unsafeAtomicAdd(hash_elem<PK>(g, m, key, lead), hv[c * H + i]);
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
 // tile_hits and per-sample domains (thread-assignment helpers).
 // --------------------------------------------------------------------------
 // A wave's samples mostly fall in one or two tiles: the active lanes that

@@ -36,12 +36,40 @@ struct OpArgs {
     const int32_t *tile_dom;
     int32_t *dom_out;
     int32_t *hits;
+    // LDS-privatised scatter (to_map / to_weights; 1024 threads and a
+    // 112 KB table per block): hash_cls 0 = global atomics, 1 = LDS.
+    int hash_cls;
+    HashWork hw;
     // batch ops
     const ObsDev *obs;
     int n_obs;
     int64_t n_items;
     int grid_blocks;
 };
+
+// Slots of an LDS hash table of `budget` bytes with NV float64 sums and an
+// int32 key per slot (a power of two).
+constexpr int hash_slots(int nv, int budget) {
+    int h = 1;
+    while (2 * h * (8 * nv + 4) <= budget) h *= 2;
+    return h;
+}
+
+template <int PROJ, int PK, int INTERP, int NC, bool W, int NT, int BUDGET>
+void launch_scatter_lds(const OpArgs &a) {
+    constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
+    constexpr int H = hash_slots(NV, BUDGET);
+    const dim3 grid((unsigned)((a.in.n_t + a.hw.span - 1) / a.hw.span),
+                    (unsigned)((a.in.n_det + a.hw.dets - 1) / a.hw.dets));
+    MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
+    hipLaunchKernelGGL((k_scatter_lds<PROJ, PK, INTERP, NC, W, NT, H>), grid, dim3(NT), 0,
+                       a.stream, a.in, a.g, m, a.resp, a.sig_in, a.det_w, a.hw, a.flags);
+}
+
+template <int PROJ, int PK, int INTERP, int NC, bool W>
+void launch_scatter_cls(const OpArgs &a) {
+    launch_scatter_lds<PROJ, PK, INTERP, NC, W, 1024, 112 * 1024>(a);
+}
 
 inline dim3 grid_for(const OpArgs &a) {
     if (a.in.chunks != nullptr) return dim3((unsigned)a.n_chunks);
@@ -67,6 +95,12 @@ int launch_map_ops(const OpArgs &a) {
                                a.stream, a.obs, a.n_obs, a.n_items, a.in.dpb, a.in.tabs, a.g,
                                m, a.accumulate, a.flags);
         }
+        return 0;
+    }
+    if ((a.op == OP_TO_MAP || a.op == OP_TO_WEIGHTS) && a.prec == 0 && a.hash_cls > 0 &&
+        a.in.chunks == nullptr) {
+        if (a.op == OP_TO_MAP) launch_scatter_cls<PROJ, PK, INTERP, NC, false>(a);
+        else launch_scatter_cls<PROJ, PK, INTERP, NC, true>(a);
         return 0;
     }
     const dim3 grid = grid_for(a);

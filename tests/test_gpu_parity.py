@@ -370,15 +370,17 @@ def test_projectionist_torch_path(gpu):
     assert np.abs(s_t.cpu().numpy() - s_np).max() < 1e-3
 
 
-@pytest.mark.parametrize('agg,xcd', [(0, 0), (1, 1), (0, 1), (1, 0)])
-def test_scatter_variants(gpu, agg, xcd):
+@pytest.mark.parametrize('agg,xcd,lds', [(0, 0, 0), (1, 1, 0), (0, 1, 1), (1, 0, 1), (1, 0, 2),
+                                         (1, 1, 2)])
+def test_scatter_variants(gpu, agg, xcd, lds):
     """Every scatter variant (wave-level run aggregation on/off, XCD-private
-    partial maps on/off) gives the oracle's maps."""
+    partial maps on/off, LDS-privatised tables never / where they pay /
+    always) gives the oracle's maps."""
     import so3g_amd
     from so3g_amd import _native
-    names = ('scatter_agg', 'scatter_xcd')
+    names = ('scatter_agg', 'scatter_xcd', 'scatter_hash')
     old = [_native.get_option(n) for n in names]
-    for n, v in zip(names, (agg, xcd)):
+    for n, v in zip(names, (agg, xcd, lds)):
         _native.set_option(n, v)
     try:
         for proj, tiled, interp in [('CAR', False, 0), ('CAR', True, 1), ('ZEA', False, 1)]:
@@ -410,6 +412,41 @@ def test_scatter_variants(gpu, agg, xcd):
                                       np.ascontiguousarray(spin[..., sl]), c.signal,
                                       c.det_weights)
             maps_close(g, r, f'precomp to_map large {comps}', atol=1e-9)
+    finally:
+        for n, v in zip(names, old):
+            _native.set_option(n, v)
+
+
+@pytest.mark.parametrize('proj,tiled,interp', [('CAR', False, 0), ('CAR', False, 1),
+                                               ('CAR', True, 0), ('HP', True, 0)])
+def test_lds_scatter_overflow(gpu, proj, tiled, interp):
+    """LDS tables that overflow: a fine map on which a block's samples visit
+    tens of thousands of pixels (CAR 300 x 4000 over the footprint; HEALPix
+    nside 4096 in 4096-pixel tiles, active tiles only) against 4096 or 2048
+    slots, so many runs take the global-atomic fallback; maps still equal
+    the oracle's.  Detector groups of 1 and 3 and spans of 64 and 1000
+    samples cover partial groups and partial spans."""
+    from so3g_amd import _native
+    names = ('scatter_hash', 'hash_dets', 'hash_span')
+    old = [_native.get_option(n) for n in names]
+    try:
+        for dets, span in ((1, 64), (3, 1000), (2, 32768)):
+            for n, v in zip(names, (2, dets, span)):
+                _native.set_option(n, v)
+            if proj == 'HP':
+                c = Case('HP', comps='TQU', tiled=True, nside=4096, nside_tile=64, n_det=7,
+                         n_samp=12000)
+            else:
+                c = Case(proj, comps='TQU', tiled=tiled, interp=interp, n_det=7, n_samp=12000,
+                         ny=300, nx=4000)
+            g = c.engine.to_map(None, c.bore, c.dets, c.resps, c.signal, c.det_weights)
+            r = oracle.to_map(c.og, c.zeros([3]), c.bore, c.dets, c.resps, c.signal,
+                              c.det_weights)
+            maps_close(g, r, f'to_map {proj} {dets} {span}')
+            g = c.engine.to_weight_map(None, c.bore, c.dets, c.resps, c.det_weights)
+            r = oracle.to_weights(c.og, c.zeros([3, 3]), c.bore, c.dets, c.resps,
+                                  c.det_weights)
+            maps_close(g, r, f'to_weights {proj} {dets} {span}')
     finally:
         for n, v in zip(names, old):
             _native.set_option(n, v)
