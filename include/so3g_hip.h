@@ -199,6 +199,20 @@ int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
                           const int32_t *pixel, int32_t k, const float *spin,
                           int32_t nc, int64_t n_det, int64_t n_t, float *signal,
                           int accumulate, void *stream);
+/* so3g_precomp_from_map for a non-tiled (ny, nx) map (k = 2) that allocates
+ * nothing: no counterpart in the reference; for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  The pixel-major packed copy of the map lives in
+ * a fixed 32 MiB device array per GPU (kept until the process ends), so the
+ * map may hold at most 1048576 pixels (SO3G_ERR_VALUE beyond that).  The
+ * library must be initialised (so3g_init) first.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (several streams or host
+ * threads) race and must be serialised by the caller.  Same results,
+ * checks and errors as so3g_precomp_from_map, bit for bit. */
+int so3g_precomp_from_map_noalloc(const double *map, const int64_t *map_shape,
+                                  const int32_t *pixel, const float *spin, int32_t nc,
+                                  int64_t n_det, int64_t n_t, float *signal, int accumulate,
+                                  void *stream);
 int so3g_precomp_to_map(double *map, const int64_t *map_shape, double *const *tiles,
                         const int32_t *tile_dims, const int32_t *pixel, int32_t k,
                         const float *spin, int32_t nc, int64_t n_det, int64_t n_t,

@@ -130,6 +130,8 @@ _SIGS = {
     'so3g_get_option': (c_int, [ctypes.c_char_p, ctypes.POINTER(c_i64)]),
     'so3g_precomp_from_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_i64,
                                       c_i64, c_vp, c_int, c_vp]),
+    'so3g_precomp_from_map_noalloc': (c_int, [c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_i64,
+                                              c_vp, c_int, c_vp]),
     'so3g_precomp_to_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_i64,
                                     c_i64, c_vp, c_vp, c_vp, c_i64, c_vp]),
     'so3g_precomp_to_weights': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32,

@@ -67,6 +67,7 @@ struct State {
     size_t copies_cap = 0;        // in doubles
     double *packed_d = nullptr;   // pixel-major map copy for gathers
     size_t packed_cap = 0;
+    double *scratch_d = nullptr;  // device address of `scratch` (so3g_init)
     // Detector order of the LDS-privatised scatter, cached for the last
     // (boresight, detectors, geometry) it was computed for.  It only steers
     // which detectors share a block, so a stale entry costs speed, never
@@ -95,6 +96,14 @@ struct Options {
 };
 Options g_opt;
 constexpr int kNumXcd = 8;
+
+// Scratch of so3g_precomp_from_map_noalloc: the pixel-major packed copy of
+// the map (4 float64 per pixel), 32 MiB, held by every device the library
+// runs on until the process ends.  Host code uses g_state.scratch_d (its
+// device address, resolved at so3g_init), never the array's name.
+constexpr int64_t kScratchPixels = 1 << 20;
+static __device__ __align__(256) double scratch[4 * kScratchPixels];
+
 State g_state;
 std::mutex g_mutex;
 
@@ -281,6 +290,7 @@ int so3g_init(int device) {
     HIP_TRY(hipMalloc(&g_state.flags_d, FLAG_COUNT * sizeof(int)));
     HIP_TRY(hipMemset(g_state.flags_d, 0, FLAG_COUNT * sizeof(int)));
     HIP_TRY(hipHostMalloc(&g_state.flags_h, FLAG_COUNT * sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipGetSymbolAddress((void **)&g_state.scratch_d, HIP_SYMBOL(scratch)));
     g_state.device = device;
     return SO3G_OK;
 }
@@ -668,6 +678,35 @@ static dim3 pc_grid(const PcIn &in, int64_t n_chunks) {
         if (!_ok) return fail(SO3G_ERR_RUNTIME, "unsupported pixel/spin layout");       \
     } while (0)
 
+// The k = 2, n_t % 4 == 0 gather: map packed pixel-major into pk (4 * plane
+// doubles), then the packed gather.
+static int pc_from_map_packed(const double *map, double *pk, const PcIn &in0, const PcMap &pm,
+                              const int32_t *pixel, const float *spin, int32_t nc,
+                              float *signal, int accumulate, hipStream_t s) {
+    PcIn in = in0;
+    in.dpb = std::max<int32_t>(in.dpb, (int32_t)g_opt.pc_from_dpb);
+    const int64_t plane = pm.ny * pm.nx;
+    int *fl = g_state.flags_d;
+    const dim3 gp(1024), g4((unsigned)((in.n_t / 4 + kBlock - 1) / kBlock),
+                            (unsigned)((in.n_det + in.dpb - 1) / in.dpb));
+    switch (nc) {
+    case 1:
+        hipLaunchKernelGGL((k_pack_map<1>), gp, dim3(kBlock), 0, s, map, plane, pk);
+        hipLaunchKernelGGL((k_pc_from_map_packed<1>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate, fl);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_pack_map<2>), gp, dim3(kBlock), 0, s, map, plane, pk);
+        hipLaunchKernelGGL((k_pc_from_map_packed<2>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate, fl);
+        break;
+    default:
+        hipLaunchKernelGGL((k_pack_map<3>), gp, dim3(kBlock), 0, s, map, plane, pk);
+        hipLaunchKernelGGL((k_pc_from_map_packed<3>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate, fl);
+        break;
+    }
+    HIP_TRY(hipGetLastError());
+    return check_tile_flag(s);
+}
+
 int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
                           const double *const *tiles, const int32_t *tile_dims,
                           const int32_t *pixel, int32_t k, const float *spin, int32_t nc,
@@ -682,7 +721,6 @@ int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
     r = zero_flags(s);
     if (r) return r;
     if (k == 2 && n_t % 4 == 0) {
-        in.dpb = std::max<int32_t>(in.dpb, (int32_t)g_opt.pc_from_dpb);
         const int64_t plane = pm.ny * pm.nx;
         if ((size_t)(plane * 4) > g_state.packed_cap) {
             if (g_state.packed_d) HIP_TRY(hipFree(g_state.packed_d));
@@ -691,28 +729,39 @@ int so3g_precomp_from_map(const double *map, const int64_t *map_shape,
             HIP_TRY(hipMalloc(&g_state.packed_d, plane * 4 * sizeof(double)));
             g_state.packed_cap = plane * 4;
         }
-        double *pk = g_state.packed_d;
-        int *fl = g_state.flags_d;
-        const dim3 gp(1024), g4((unsigned)((n_t / 4 + kBlock - 1) / kBlock),
-                                (unsigned)((n_det + in.dpb - 1) / in.dpb));
-        switch (nc) {
-        case 1:
-            hipLaunchKernelGGL((k_pack_map<1>), gp, dim3(kBlock), 0, s, map, plane, pk);
-            hipLaunchKernelGGL((k_pc_from_map_packed<1>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate, fl);
-            break;
-        case 2:
-            hipLaunchKernelGGL((k_pack_map<2>), gp, dim3(kBlock), 0, s, map, plane, pk);
-            hipLaunchKernelGGL((k_pc_from_map_packed<2>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate, fl);
-            break;
-        default:
-            hipLaunchKernelGGL((k_pack_map<3>), gp, dim3(kBlock), 0, s, map, plane, pk);
-            hipLaunchKernelGGL((k_pc_from_map_packed<3>), g4, dim3(kBlock), 0, s, in, pm, pk, pixel, spin, signal, accumulate, fl);
-            break;
-        }
-        HIP_TRY(hipGetLastError());
-        return check_tile_flag(s);
+        return pc_from_map_packed(map, g_state.packed_d, in, pm, pixel, spin, nc, signal,
+                                  accumulate, s);
     }
     PC_SWITCH(k, nc, k_pc_from_map, pc_grid(in, 0), dim3(kBlock), 0, s, in, pm, map, tiles,
+              pixel, spin, signal, accumulate, g_state.flags_d);
+    HIP_TRY(hipGetLastError());
+    return check_tile_flag(s);
+}
+
+int so3g_precomp_from_map_noalloc(const double *map, const int64_t *map_shape,
+                                  const int32_t *pixel, const float *spin, int32_t nc,
+                                  int64_t n_det, int64_t n_t, float *signal, int accumulate,
+                                  void *stream) {
+    // Initialisation allocates (tables, flags): it must have happened.
+    if (g_state.device < 0 || g_state.scratch_d == nullptr)
+        return fail(SO3G_ERR_RUNTIME, "so3g_precomp_from_map_noalloc: call so3g_init first");
+    PcIn in;
+    PcMap pm;
+    int r = pc_setup(map_shape, nullptr, 2, nc, n_det, n_t, stream, in, pm);
+    if (r) return r;
+    if (pm.ny < 0 || pm.nx < 0 || pm.ny * pm.nx > kScratchPixels)
+        return fail(SO3G_ERR_VALUE, "so3g_precomp_from_map_noalloc: the map has " +
+                                        std::to_string(pm.ny * pm.nx) + " pixels, more than "
+                                        "the " + std::to_string(kScratchPixels) +
+                                        " its scratch holds");
+    if (n_t == 0 || n_det == 0) return SO3G_OK;
+    hipStream_t s = (hipStream_t)stream;
+    r = zero_flags(s);
+    if (r) return r;
+    if (n_t % 4 == 0)
+        return pc_from_map_packed(map, g_state.scratch_d, in, pm, pixel, spin, nc, signal,
+                                  accumulate, s);
+    PC_SWITCH(2, nc, k_pc_from_map, pc_grid(in, 0), dim3(kBlock), 0, s, in, pm, map, nullptr,
               pixel, spin, signal, accumulate, g_state.flags_d);
     HIP_TRY(hipGetLastError());
     return check_tile_flag(s);
