@@ -635,11 +635,67 @@ class ProjEngBase:
 
 class ProjEngPrecompBase:
     """ProjEng_Precomp_{NonTiled,Tiled} (Projection.cxx:2012-2253): projection
-    through a cached pixel_index / spin_proj pointing matrix."""
+    through a cached pixel_index / spin_proj pointing matrix.
+
+    When pixel_index and spin_proj are torch tensors on the GPU, the engine
+    builds a PrecompPlan from them on first use and keeps it (plan_cache,
+    default True), so that repeated calls with the same pointing matrix (a
+    CG loop) run the planned kernels: same from_map results bit for bit,
+    to_map / to_weight_map the same float64 sums of the same float32 products
+    in another order.  The plan is keyed on both tensors' storage, shape,
+    strides and version counters, so an in-place edit of either rebuilds it;
+    it holds device memory (about 20 B per sample) until clear_plan() or
+    another pointing matrix replaces it.  thread_intervals, a tile list with
+    inactive (None) tiles, numpy inputs (restaged every call anyway) and
+    to_weight_map on a pixel-run plan use the direct kernels."""
     _tiled = False
+    plan_cache = True
 
     def __init__(self):
-        pass
+        self._plan = None
+        self._plan_key = None
+
+    def clear_plan(self):
+        """Free the cached plan."""
+        self._plan = None
+        self._plan_key = None
+
+    def _cached_plan(self, pixel_index, spin_proj, pix, spin, map):
+        """The PrecompPlan for (pix, spin) and this map's geometry, or None
+        when the direct kernels are to be used."""
+        if not (self.plan_cache and _is_torch(pixel_index) and _is_torch(spin_proj)):
+            return None
+        if pix.data_ptr() != pixel_index.data_ptr() or spin.data_ptr() != spin_proj.data_ptr():
+            return None      # staged copies: not the caller's tensors
+        n_det, n_t, k = pix.shape
+        if n_det * n_t == 0 or n_det * n_t >= 2 ** 32 or n_det >= 2 ** 22 or n_t >= 2 ** 30:
+            return None
+        if k == 3:
+            if not isinstance(map, (list, tuple)) or any(t is None for t in map):
+                return None
+            geo = ('tiles', tuple(tuple(int(x) for x in t.shape[-2:]) for t in map))
+        else:
+            if isinstance(map, (list, tuple)) or not hasattr(map, 'shape'):
+                return None
+            geo = ('map', tuple(int(x) for x in map.shape[-k:]))
+        key = (pixel_index.data_ptr(), tuple(pixel_index.shape), tuple(pixel_index.stride()),
+               pixel_index._version, spin_proj.data_ptr(), tuple(spin_proj.shape),
+               tuple(spin_proj.stride()), spin_proj._version, geo, pixel_index.device)
+        if key == self._plan_key:
+            return self._plan
+        self._plan = None
+        self._plan_key = None
+        try:
+            if k == 3:
+                plan = PrecompPlan(pix, spin, tile_dims=np.array(geo[1], np.int32))
+            else:
+                plan = PrecompPlan(pix, spin, geo[1])
+        except RuntimeError as e:
+            if 'out of device memory' in str(e):
+                return None
+            raise
+        self._plan, self._plan_key = plan, key
+        return plan
 
     def _pmat(self, call, pixel_index, spin_proj):
         pix, _, _ = _signal_space(call, pixel_index, 'pixel_index', 'int32', -1, -1, -1,
@@ -656,6 +712,18 @@ class ProjEngPrecompBase:
         spin, _, _ = _signal_space(call, spin_proj, 'spin_proj', 'float32', n_det, n_t, -1,
                                    out=False)
         return pix, spin, n_det, n_t, k, spin.shape[2]
+
+    @staticmethod
+    def _check_lead(m, nlead, nc, k):
+        """The map's leading (component) axes for a planned call: (nc,) for
+        maps, (nc, nc) for weights, as the direct kernels index them."""
+        tiles = m if isinstance(m, (list, tuple)) else [m]
+        for t in tiles:
+            shape = tuple(t.shape)
+            want = (nc,) * nlead
+            if len(shape) != nlead + (1 if k == 1 else 2) or shape[:nlead] != want:
+                raise RuntimeError(f"Buffer 'map' has incompatible shape: Expected leading "
+                                   f"axes {want} but got {shape}.")
 
     def _map(self, call, m, nlead, k, writeback):
         """Returns (map_ptr, shape_ptr, tiles_ptr, dims_ptr)."""
@@ -690,6 +758,16 @@ class ProjEngPrecompBase:
     def from_map(self, map, pixel_index, spin_proj, signal=None):
         call = _Call()
         pix, spin, n_det, n_t, k, nc = self._pmat(call, pixel_index, spin_proj)
+        plan = self._cached_plan(pixel_index, spin_proj, pix, spin, map)
+        if plan is not None:
+            self._check_lead(map, 1, nc, k)
+            if signal is None:
+                # a fresh signal comes back as a device tensor, as on the
+                # direct path for torch pointing matrices
+                out = torch.empty((n_det, n_t), dtype=torch.float32, device=pix.device)
+                plan.from_map(map, out, fresh=True)
+                return out
+            return plan.from_map(map, signal)
         mp, sp, tp, dp = self._map(call, map, 1, k, writeback=False)
         sig, obj, fresh = _signal_space(call, signal, 'signal', 'float32', n_det, n_t)
         N.check(N.load().so3g_precomp_from_map(mp, sp, tp, dp, pix.data_ptr(), k,
@@ -702,6 +780,11 @@ class ProjEngPrecompBase:
                thread_intervals=None):
         call = _Call()
         pix, spin, n_det, n_t, k, nc = self._pmat(call, pixel_index, spin_proj)
+        if thread_intervals is None:
+            plan = self._cached_plan(pixel_index, spin_proj, pix, spin, map)
+            if plan is not None:
+                self._check_lead(map, 1, nc, k)
+                return plan.to_map(map, signal, weights)
         mp, sp, tp, dp = self._map(call, map, 1, k, writeback=True)
         sig, _, _ = _signal_space(call, signal, 'signal', 'float32', n_det, n_t, out=False)
         dw = call.array(weights, 'float32', 'det_weights', spec=[n_det], optional=True)
@@ -719,6 +802,11 @@ class ProjEngPrecompBase:
                       thread_intervals=None):
         call = _Call()
         pix, spin, n_det, n_t, k, nc = self._pmat(call, pixel_index, spin_proj)
+        if thread_intervals is None:
+            plan = self._cached_plan(pixel_index, spin_proj, pix, spin, map)
+            if plan is not None and plan.info()['layout'] == 1:
+                self._check_lead(map, 2, nc, k)
+                return plan.to_weights(map, weights)
         mp, sp, tp, dp = self._map(call, map, 2, k, writeback=True)
         dw = call.array(weights, 'float32', 'det_weights', spec=[n_det], optional=True)
         segs = _flatten_intervals(thread_intervals, n_det, n_t)
@@ -905,12 +993,14 @@ class PrecompPlan:
         call.finish()
         return map
 
-    def from_map(self, map, signal=None):
+    def from_map(self, map, signal=None, fresh=None):
         """signal (+)= P map; a given signal is accumulated into, as
-        ProjEng_Precomp.from_map does; None returns fresh rows."""
+        ProjEng_Precomp.from_map does (fresh=True overwrites it instead);
+        None returns fresh rows."""
         call = _Call()
-        sig, obj, fresh = _signal_space(call, signal, 'signal', 'float32', self.n_det,
-                                        self.n_t)
+        sig, obj, new = _signal_space(call, signal, 'signal', 'float32', self.n_det,
+                                      self.n_t)
+        fresh = new if fresh is None else bool(fresh)
         if self.k == 3:
             N.check(N.load().so3g_pc_plan_from_map_tiles(self._h, self._tiles(call, map, False),
                                                          sig.data_ptr(), 0 if fresh else 1,
