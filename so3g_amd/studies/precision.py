@@ -13,8 +13,13 @@ entries) the max and RMS errors of ref and f32 relative to f64 are reported:
 over the hit pixels.
 
 Geometries: ZEA and TAN at 0.25 arcmin centred on the scan, and HEALPix RING
-nside 8192 (full-sky maps, admitted with allow_nside_max=True; to_map only,
-its 3x3 weight maps would need 58 GB per copy).
+nside 8192 (full-sky maps, admitted with allow_nside_max=True).  The
+HEALPix 3x3 weight maps (58 GB per float64 copy) are made one arithmetic at
+a time and reduced to the hit pixels before the next is made.
+
+With keep=True the float64 comparator maps (hit pixels only for HEALPix)
+are returned too, under '_f64', for checks against an independent CPU
+accumulation.
 
 Run: python -m so3g_amd.studies.precision [--n-det N] [--n-samp N] [--out FILE]
 """
@@ -47,7 +52,7 @@ def zenithal_projectionist(proj, res_arcmin, el_deg=50., throw_deg=10., disk_deg
     return Projectionist.for_geom(shape, wcs)
 
 
-def run(n_det=100, n_samp=40000, res_arcmin=0.25, nside=8192, seed=1):
+def run(n_det=100, n_samp=40000, res_arcmin=0.25, nside=8192, seed=1, keep=False):
     import so3g_amd
     from .. import workloads
     from ..proj.wcs import ProjectionistHealpix
@@ -76,7 +81,10 @@ def run(n_det=100, n_samp=40000, res_arcmin=0.25, nside=8192, seed=1):
                             for a, b in W_ENTRIES},
             }
         out['cases'][f'{proj} {res_arcmin} arcmin'] = case
-    # HEALPix RING nside 8192 (to_map only).
+        if keep:
+            out.setdefault('_f64', {})[proj] = {'map': maps[2], 'weights': wts[2],
+                                                'projectionist': p, 'resp': resp}
+    # HEALPix RING nside 8192.
     hp = ProjectionistHealpix.for_healpix(nside, ordering='RING', allow_nside_max=True)
     eng = hp.get_ProjEng('TQU')
     q = np.asarray(hp._cache_q_fp_to_native(obs.asm.Q))
@@ -99,8 +107,26 @@ def run(n_det=100, n_samp=40000, res_arcmin=0.25, nside=8192, seed=1):
         mm = res[pr][:, idx].cpu().numpy()
         allhit = np.ones(idx.numel(), bool)
         case[name] = {'map': {c: _errors(mm[i], m64[i], allhit) for i, c in enumerate(COMPS)}}
-    out['cases'][f'HEALPix RING nside {nside}'] = case
     del res
+    wh = {}
+    for pr in (2, 0, 1):
+        dt = torch.float32 if pr == 1 else torch.float64
+        w = torch.zeros((3, 3, npix), dtype=dt, device=dev)
+        eng.to_weight_map_prec(w, torch.from_numpy(q).to(dev),
+                               torch.from_numpy(obs.dets).to(dev),
+                               torch.from_numpy(resp).to(dev), None, pr)
+        wh[pr] = w[:, :, idx].cpu().numpy()
+        del w
+        torch.cuda.empty_cache()
+    allhit = np.ones(idx.numel(), bool)
+    for name, pr in (('ref', 0), ('f32', 1)):
+        case[name]['weights'] = {f'W{a}{b}': _errors(wh[pr][a, b], wh[2][a, b], allhit)
+                                 for a, b in W_ENTRIES}
+    out['cases'][f'HEALPix RING nside {nside}'] = case
+    if keep:
+        out.setdefault('_f64', {})['HP'] = {'map': m64, 'weights': wh[2],
+                                            'pixels': idx.cpu().numpy(), 'projectionist': hp,
+                                            'resp': resp}
     return out
 
 

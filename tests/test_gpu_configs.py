@@ -1,0 +1,105 @@
+"""The precision study and the map-making solve at their own configurations.
+
+Precision: 100 detectors x 40000 samples (el 50 deg, az +-10 deg at 1 deg/s,
+200 Hz), projected with ZEA and TAN at 0.25 arcmin and into a HEALPix RING
+nside 8192 map (allow_nside_max); to_map and to_weights in three
+arithmetics (float32 products into a float64 map -- so3g's -- all float32,
+and float64 products as the comparator).  Every output of so3g's arithmetic
+must be within 1e-6 relative of the float64 comparator, and the comparator
+itself is pinned by an independent float64 accumulation on the CPU (numpy
+bincount over the oracle's pointing matrix).
+
+Map-making: the 100 x 40000 observation with 1/f noise and a Gaussian
+random sky, solved by binned-preconditioned CG, two-level CG and flexible
+CG with the learned correction (a small training budget), each to 1e-6
+relative residual where it converges.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _cpu_f64(pix_flat, spin, sig, n_pix_keys):
+    """float64 to_map and to_weights sums per key from float32 spin factors
+    and signal (products formed in float64, w = det_wt = 1)."""
+    s = sig.astype(np.float64).ravel()
+    pf = spin.reshape(-1, 3).astype(np.float64)
+    on = pix_flat >= 0
+    keys = pix_flat[on]
+    m = np.stack([np.bincount(keys, weights=(s * pf[:, c])[on], minlength=n_pix_keys)
+                  for c in range(3)])
+    w = np.zeros((3, 3, n_pix_keys))
+    for a in range(3):
+        for b in range(a, 3):
+            w[a, b] = np.bincount(keys, weights=(pf[:, a] * pf[:, b])[on], minlength=n_pix_keys)
+    return m, w
+
+
+def test_c3_precision_study(gpu):
+    from so3g_amd import workloads
+    from so3g_amd.studies import precision
+    n_det, n_samp = 100, 40000
+    r = precision.run(n_det=n_det, n_samp=n_samp, res_arcmin=0.25, nside=8192, keep=True)
+    for cname, c in r['cases'].items():
+        assert c['hit_pixels'] > 10000, cname
+        assert set(c['ref']) == {'map', 'weights'}, cname
+        for kind in ('map', 'weights'):
+            for k, e in c['ref'][kind].items():
+                assert e['max_rel'] < 1e-6, (cname, k, e)
+                # the all-float32 arithmetic is reported; it is looser
+                assert c['f32'][kind][k]['max_rel'] >= e['max_rel'] * 0.5, (cname, k)
+    obs = workloads.make('small', n_det=n_det, n_samp=n_samp)
+    sig = workloads.signal(n_det, n_samp, seed=1)
+    f64 = r['_f64']
+    # ZEA and TAN: the full float64 GPU maps against the CPU sums
+    for proj in ('ZEA', 'TAN'):
+        d = f64[proj]
+        p = d['projectionist']
+        args = p._get_pixelizor_args()
+        og = oracle.geom_from_args(proj, 'TQU', args)
+        q = np.ascontiguousarray(np.asarray(p._cache_q_fp_to_native(obs.asm.Q)), np.float64)
+        pix, spin = oracle.pointing_matrix(og, q, obs.dets, d['resp'])
+        ny, nx = args[0], args[1]
+        flat = np.where(pix[..., 0] >= 0, pix[..., 0].astype(np.int64) * nx + pix[..., 1], -1)
+        m, w = _cpu_f64(flat.ravel(), spin, sig, ny * nx)
+        gm = d['map'].reshape(3, -1)
+        gw = d['weights'].reshape(3, 3, -1)
+        assert np.abs(gm - m).max() <= 1e-12 * np.abs(m).max(), proj
+        for a in range(3):
+            for b in range(a, 3):
+                assert np.abs(gw[a, b] - w[a, b]).max() <= 1e-12 * np.abs(w[a, b]).max(), \
+                    (proj, a, b)
+    # HEALPix RING 8192: the hit pixels
+    d = f64['HP']
+    og = oracle.geom_from_args('HP', 'TQU', (8192, 0))
+    pix, spin = oracle.pointing_matrix(og, np.ascontiguousarray(obs.bore), obs.dets, d['resp'])
+    keys, inv = np.unique(pix[..., 0].ravel(), return_inverse=True)
+    assert np.array_equal(keys, d['pixels'])
+    m, w = _cpu_f64(inv.astype(np.int64), spin, sig, len(keys))
+    assert np.abs(d['map'] - m).max() <= 1e-12 * np.abs(m).max()
+    for a in range(3):
+        for b in range(a, 3):
+            assert np.abs(d['weights'][a, b] - w[a, b]).max() <= \
+                1e-12 * np.abs(w[a, b]).max(), (a, b)
+
+
+def test_c5_cg_solvers(gpu):
+    """Binned PCG stalls far above 1e-6 within the iteration cap; two-level
+    PCG reaches it; flexible CG with the learned correction (300 training
+    steps on 400 pairs here) reaches it too, falling back to two-level PCG
+    when the network stops making progress."""
+    from so3g_amd.studies import cg
+    r = cg.run(tol=1e-6, maxiter=2500, learned=True, n_rhs=2, per_rhs=200, steps=300)
+    assert r['binned']['final_residual'] > 1e-6
+    assert r['binned']['iters'] == 2500
+    assert r['two_level']['final_residual'] <= 1e-6
+    assert r['two_level']['iters'] < 2500
+    assert r['learned']['final_residual'] <= 1e-6
+    assert r['learned']['iters'] < 2500
+    # both converged solvers leave the same error against the true sky (it is
+    # noise in the poorly measured modes, not a convergence effect)
+    for a, b in zip(r['two_level']['map_rms_error_TQU'], r['learned']['map_rms_error_TQU']):
+        assert abs(a - b) < 0.05 * max(a, b)
