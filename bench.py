@@ -12,6 +12,16 @@ the results of ProjEng_Precomp's from_map / to_map.  The plain
 ProjEng_Precomp kernels and the on-the-fly path (pointing recomputed in both
 kernels, the Projectionist's default) are timed beside it.
 
+Beside the headline, the line carries: the drop-in ProjEng_Precomp class on
+the same tensors (its cached plan), the unplanned Precomp kernels, the
+on-the-fly path, the small (100 x 40000) observation, the tiled CAR and
+HEALPix nside 4096 maps of the same observation, get_pixels /
+get_pointing_matrix / to_weights, N^-1 (nmat_detvecs_apply), G3SuperTimestream
+decoding, the precision study (100 x 40000 samples into ZEA / TAN 0.25 arcmin
+and HEALPix RING nside 8192) and the map-making solve (binned, two-level and
+learned-correction CG to 1e-6), and the CPU baseline (the oracle on the host
+cores).
+
 Prints ONE JSON line.  `python bench.py [--steps K] [--warmup W] [--gpus 1]`.
 
 `--pmc-pass` runs, instead of the timing, one plain copy of 1 GiB (the
@@ -92,6 +102,8 @@ def parse():
     ap.add_argument('--no-small', action='store_true', help='skip the small-observation leg')
     ap.add_argument('--no-g3', action='store_true',
                     help='skip the G3SuperTimestream decoding leg')
+    ap.add_argument('--no-studies', action='store_true',
+                    help='skip the precision (C3) and map-making (C5) legs')
     ap.add_argument('--cpu-full', action='store_true',
                     help='print the full CPU baseline table (oracle, median of 5) and exit')
     return ap.parse_args()
@@ -276,6 +288,9 @@ def main():
     # The plain ProjEng_Precomp kernels (no plan), for reference.
     _, (ms_from_direct, ms_to_direct) = run([pc_from_direct, pc_to_direct],
                                             max(1, min(args.steps, 3)), 1)
+    # The drop-in ProjEng_Precomp class with these torch tensors: its first
+    # call builds and caches the plan, later calls reuse it.
+    precomp_api = precomp_api_timing(torch, so3g_amd, pix, spin, sky, sig, mapo)
 
     otf = None
     if not args.no_otf:
@@ -305,6 +320,12 @@ def main():
                                                       dets, resp, sig, dev, stream, run)
     other = None if args.no_small else c2_other_ops(torch, N, lib, g, bore, dets, resp, pix,
                                                     spin, plan, ny, nx, dev, stream, run)
+    c3 = c5 = None
+    if not args.no_studies:
+        del plan
+        torch.cuda.empty_cache()
+        c3 = c3_precision()
+        c5 = c5_mapmaking()
 
     out = {
         'metric': 'det*samples/s for to_map+from_map (% HBM roofline)',
@@ -339,21 +360,109 @@ def main():
             'bound': 'hbm', 'kernel': dom[0], 'achieved': dom[1] / 1e9,
             'peak': HBM_PEAK / 1e9, 'unit': 'GB/s', 'frac': dom[1] / HBM_PEAK,
             'traffic': dom[3], 'algorithmic_bytes': dom[2],
+            'bytes_basis': 'the plan kernel\'s own storage (14 B per sample for the '
+                           'gather, 10-16 B for the scatter)',
             'from_map_frac': gbs_from / HBM_PEAK, 'to_map_frac': gbs_to / HBM_PEAK,
+            # The same times on the bytes of the cached pointing matrix the
+            # ProjEng_Precomp interface streams (int32 pixel pair + 3 float32
+            # factors + float32 signal = 24 B per sample): above 1 because
+            # the plan re-encodes that matrix in fewer bytes.
             'precomp_equivalent': {
+                'bytes_per_sample': 24,
                 'bytes_per_launch': b_pc + 8 * 3 * npix,
                 'from_map_frac': (b_pc + 8 * 3 * npix) / (ms_from * 1e-3) / HBM_PEAK,
-                'to_map_frac': (b_pc + 16 * 3 * npix) / (ms_to * 1e-3) / HBM_PEAK},
+                'to_map_frac': (b_pc + 16 * 3 * npix) / (ms_to * 1e-3) / HBM_PEAK,
+                'pair_frac': (2 * b_pc + 24 * 3 * npix) / (ms_step * 1e-3) / HBM_PEAK},
+            # The unplanned ProjEng_Precomp kernels on those 24 B per sample.
+            'precomp_kernels': {
+                'from_map_frac': (b_pc + 8 * 3 * npix) / (ms_from_direct * 1e-3) / HBM_PEAK,
+                'to_map_frac': (b_pc + 16 * 3 * npix) / (ms_to_direct * 1e-3) / HBM_PEAK},
         },
+        'plan_cost': {
+            'pointing_matrix_ms': t_pmat * 1e3, 'plan_build_ms': t_plan * 1e3,
+            'saved_ms_per_step': (ms_from_direct + ms_to_direct) - (ms_from + ms_to),
+            'break_even_steps': t_plan * 1e3 / max(1e-9, (ms_from_direct + ms_to_direct) -
+                                                  (ms_from + ms_to))},
+        'precomp_api': precomp_api,
         'on_the_fly': otf,
         'small_config': small,
         'nmat_detvecs_apply': nmat,
         'g3super_decode': g3,
         'c2_variants': variants,
         'c2_other_ops': other,
+        'c3_precision': c3,
+        'c5_mapmaking': c5,
         'cpu_baseline': cpu,
     }
-    print(json.dumps(out), flush=True)
+    # numpy scalars (study outputs) are written as plain numbers
+    print(json.dumps(out, default=lambda o: o.item() if hasattr(o, 'item') else str(o)),
+          flush=True)
+
+
+def precomp_api_timing(torch, so3g_amd, pix, spin, sky, sig, mapo, calls=5):
+    """ProjEng_Precomp_NonTiled.from_map + to_map with the benchmark's torch
+    tensors, wall time per pair (synchronised): the first pair includes
+    building the cached plan; later pairs reuse it."""
+    pe = so3g_amd.ProjEng_Precomp_NonTiled()
+    ts = []
+    for _ in range(calls):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pe.from_map(sky, pix, spin)
+        mapo.zero_()
+        pe.to_map(mapo, pix, spin, sig)
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    pe.clear_plan()
+    log(f'ProjEng_Precomp pair: {ts}')
+    return {'ms_first_pair': ts[0], 'ms_pair': float(np.median(ts[1:])),
+            'det_samples_per_s': pix.shape[0] * pix.shape[1] / (np.median(ts[1:]) * 1e-3)}
+
+
+def c3_precision():
+    """Precision study at its configuration: 100 det x 40000 samples into ZEA
+    and TAN 0.25 arcmin and HEALPix RING nside 8192 maps, to_map and
+    to_weights with float32 products into float64 (so3g's arithmetic) and
+    all-float32, errors relative to float64 products: the largest max and
+    rms relative error over the outputs of each case."""
+    from so3g_amd.studies import precision
+    t0 = time.time()
+    r = precision.run(n_det=100, n_samp=40000, res_arcmin=0.25, nside=8192)
+    out = {'workload': r['workload'], 'seconds': time.time() - t0, 'cases': {}}
+    for cname, c in r['cases'].items():
+        row = {'hit_pixels': c['hit_pixels']}
+        for ar in ('ref', 'f32'):
+            for kind in ('map', 'weights'):
+                es = list(c[ar][kind].values())
+                row[f'{ar}_{kind}_max_rel'] = max(e['max_rel'] for e in es)
+                row[f'{ar}_{kind}_rms_rel'] = max(e['rms_rel'] for e in es)
+        out['cases'][cname] = row
+    log(f'C3 precision: {out}')
+    return out
+
+
+def c5_mapmaking():
+    """Map-making at its configuration (100 det x 40000 samples, CAR 1
+    arcmin TQU, 1/f noise, Gaussian random sky): wall time of the solve to
+    1e-6 relative residual with binned-preconditioned CG (capped at 40000
+    iterations), two-level CG, and flexible CG with the learned U-Net
+    correction on top of two-level (falling back to two-level PCG when it
+    stalls); setup and training times listed apart."""
+    from so3g_amd.studies import cg
+    r = cg.run()
+    keep = ('iters', 'seconds', 'final_residual', 'fallback_at', 'map_rms_error_TQU',
+            'ms_per_iter', 'setup_s', 'train_s', 'pairs_s', 'train_steps', 'pairs',
+            'loss_first_last')
+    out = {'workload': r['workload']}
+    for k in ('binned', 'two_level', 'learned'):
+        if k in r:
+            out[k] = {kk: v for kk, v in r[k].items() if kk in keep}
+            out[k]['converged'] = bool(r[k]['final_residual'] <= 1e-6)
+    for k in ('speedup_two_level', 'speedup_learned_vs_binned', 'speedup_learned_vs_two_level'):
+        if k in r:
+            out[k] = r[k]
+    log(f'C5 map-making: {out}')
+    return out
 
 
 def cpu_baseline(obs, pix, spin, sig, sky, pixargs):
