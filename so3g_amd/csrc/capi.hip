@@ -242,8 +242,11 @@ int base_args(const so3g_geom *g, const double *bore, int64_t n_t, const double 
     a.in.n_det = n_det;
     a.in.tabs = Tabs{g_state.asin_d, g_state.atan2_d};
     // Detectors per block: at least 4 (boresight reuse), more when the
-    // detector count would overflow gridDim.y.
-    a.in.dpb = (int32_t)std::max<int64_t>(4, (n_det + 65534) / 65535);
+    // 1-D grid of (detector group, 256-sample chunk) blocks would exceed
+    // 2^31 - 1 blocks.
+    a.in.dpb = 4;
+    const int64_t n_chunk = (n_t + kBlock - 1) / kBlock;
+    while (n_chunk * ((n_det + a.in.dpb - 1) / a.in.dpb) > INT32_MAX) a.in.dpb *= 2;
     a.in.agg = g_opt.scatter_agg;
     a.interp = g->interp;
     a.nc = g->ncomp;
@@ -522,9 +525,9 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
     a.det_w = det_weights;
     a.hash_cls = 0;
     // Where the LDS tables pay (1000 x 720000 samples, ms, global atomics
-    // -> LDS): HEALPix nside 4096 to_map 18.5 -> 15.0, to_weights 33.6 ->
-    // 20.0; CAR 0.5' to_weights 18.9 -> 15.7.  Not for CAR to_map (10.2 ->
-    // 11.3) nor tiled CAR to_map (11.0 -> 13.8): a 112 KB table leaves one
+    // -> LDS): HEALPix nside 4096 to_map 17.2 -> 14.2, to_weights 30.8 ->
+    // 17.8; CAR 0.5' to_weights 17.7 -> 15.3.  Not for CAR to_map (9.1 ->
+    // 11.5) nor tiled CAR to_map (11.0 -> 13.8): a 112 KB table leaves one
     // block (4 waves per SIMD) per compute unit, and the latency-bound
     // float64 pointing loses more to that than the atomics it saves there.
     const bool hash_pays = g->pixelizor == SO3G_PIX_HEALPIX ||
@@ -534,9 +537,12 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
         r = scatter_order(g, a, &a.hw.order);
         if (r) return r;
         a.hash_cls = 1;
-        // gridDim.y (detector groups) stays within 65535
-        a.hw.dets = (int32_t)std::max<int64_t>(g_opt.hash_dets, (n_det + 65534) / 65535);
+        a.hw.dets = g_opt.hash_dets;
         a.hw.span = g_opt.hash_span;
+        // the 1-D grid of (group, span) blocks stays below 2^31
+        while (((n_t + a.hw.span - 1) / a.hw.span) * ((n_det + a.hw.dets - 1) / a.hw.dets) >
+               INT32_MAX)
+            a.hw.dets *= 2;
     }
     r = run_dispatch(g->proj, a);
     if (r) return r;

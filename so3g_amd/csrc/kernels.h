@@ -31,6 +31,20 @@ struct PtgIn {
     const Chunk *chunks;       // segment mode when non-null
 };
 
+// Dense-mode block -> (detector group, 256-sample chunk).  The grid is 1-D
+// and the detector group varies fastest, so the blocks that share a chunk of
+// the boresight run together, one after another across the XCDs: each XCD's
+// L2 fetches the chunk once instead of once per detector group (with the
+// chunk varying fastest, 250 groups re-read the whole 23 MB boresight of a
+// 720000-sample observation from beyond the L2, 5.8 GB).
+__device__ __forceinline__ void dense_block(const PtgIn &in, int64_t &t0, int64_t &d0) {
+    const int64_t ng = (in.n_det + in.dpb - 1) / in.dpb;
+    const int64_t b = blockIdx.x;
+    const int64_t tc = b / ng;
+    d0 = (b - tc * ng) * in.dpb;
+    t0 = tc * kBlock;
+}
+
 // Calls f(det, t, qbore) for the samples this thread owns.
 template <class F>
 __device__ __forceinline__ void for_samples(const PtgIn &in, F &&f) {
@@ -40,10 +54,11 @@ __device__ __forceinline__ void for_samples(const PtgIn &in, F &&f) {
         if (t < c.t1) f(c.det, (int64_t)t, load_q(in.bore + 4 * (int64_t)t));
         return;
     }
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    int64_t t0, d0;
+    dense_block(in, t0, d0);
+    const int64_t t = t0 + threadIdx.x;
     if (t >= in.n_t) return;
     const Q4 qb = load_q(in.bore + 4 * t);
-    const int64_t d0 = (int64_t)blockIdx.y * in.dpb;
     const int64_t d1 = min(d0 + in.dpb, in.n_det);
     for (int64_t d = d0; d < d1; d++) f(d, t, qb);
 }
@@ -291,9 +306,10 @@ __device__ __forceinline__ void for_samples_all(const PtgIn &in, F &&f) {
         t = c.t0 + (int64_t)threadIdx.x;
         t_end = c.t1;
     } else {
-        t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        int64_t t0;
+        dense_block(in, t0, d0);
+        t = t0 + threadIdx.x;
         t_end = in.n_t;
-        d0 = (int64_t)blockIdx.y * in.dpb;
         d1 = min(d0 + in.dpb, in.n_det);
     }
     const bool valid = t < t_end;
@@ -557,9 +573,13 @@ __global__ __launch_bounds__(NT) void k_scatter_lds(PtgIn in, Geo g, MapView<dou
         for (int c = 0; c < NV; c++) hv[c * H + i] = 0.;
     }
     __syncthreads();
-    const int64_t g0 = (int64_t)blockIdx.y * hw.dets;
+    // 1-D grid, detector group fastest (as dense_block): the blocks over one
+    // span of the boresight run together
+    const int64_t ng = (in.n_det + hw.dets - 1) / hw.dets;
+    const int64_t ts = (int64_t)blockIdx.x / ng;
+    const int64_t g0 = ((int64_t)blockIdx.x - ts * ng) * hw.dets;
     const int nd = (int)min((int64_t)hw.dets, in.n_det - g0);
-    const int64_t t_beg = (int64_t)blockIdx.x * hw.span;
+    const int64_t t_beg = ts * hw.span;
     const int64_t t_end = min(t_beg + (int64_t)hw.span, in.n_t);
     for (int64_t tb = t_beg; tb < t_end; tb += NT) {
         const int64_t t = tb + threadIdx.x;

@@ -59,8 +59,8 @@ template <int PROJ, int PK, int INTERP, int NC, bool W, int NT, int BUDGET>
 void launch_scatter_lds(const OpArgs &a) {
     constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
     constexpr int H = hash_slots(NV, BUDGET);
-    const dim3 grid((unsigned)((a.in.n_t + a.hw.span - 1) / a.hw.span),
-                    (unsigned)((a.in.n_det + a.hw.dets - 1) / a.hw.dets));
+    const dim3 grid((unsigned)(((a.in.n_t + a.hw.span - 1) / a.hw.span) *
+                               ((a.in.n_det + a.hw.dets - 1) / a.hw.dets)));
     MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
     hipLaunchKernelGGL((k_scatter_lds<PROJ, PK, INTERP, NC, W, NT, H>), grid, dim3(NT), 0,
                        a.stream, a.in, a.g, m, a.resp, a.sig_in, a.det_w, a.hw, a.flags);
@@ -71,11 +71,13 @@ void launch_scatter_cls(const OpArgs &a) {
     launch_scatter_lds<PROJ, PK, INTERP, NC, W, 1024, 112 * 1024>(a);
 }
 
+// Grid of the on-the-fly kernels: one block per chunk (segment mode), or
+// one per (detector group, 256-sample chunk), group fastest (dense_block).
 inline dim3 grid_for(const OpArgs &a) {
     if (a.in.chunks != nullptr) return dim3((unsigned)a.n_chunks);
-    const int64_t bx = (a.in.n_t + kBlock - 1) / kBlock;
-    const int64_t by = (a.in.n_det + a.in.dpb - 1) / a.in.dpb;
-    return dim3((unsigned)bx, (unsigned)by);
+    const int64_t bt = (a.in.n_t + kBlock - 1) / kBlock;
+    const int64_t bg = (a.in.n_det + a.in.dpb - 1) / a.in.dpb;
+    return dim3((unsigned)(bt * bg));
 }
 
 template <int PROJ, int PK, int INTERP, int NC>
