@@ -4,6 +4,7 @@ example (docs/proj.rst:75-324), FITS-II projection closed forms
 (Calabretta & Greisen 2002) via numpy quaternion decomposition, exact
 pixel-index arithmetic on chosen coordinates, a dense brute-force pointing
 matrix, and the reference tests' invariants."""
+import os
 import numpy as np
 import pytest
 
@@ -434,3 +435,19 @@ def test_zenithal_radius_decs():
             t = {'ARC': r, 'SIN': np.arcsin(np.clip(r, 0, 1)), 'TAN': np.arctan(r),
                  'ZEA': 2 * np.arcsin(np.clip(r / 2, 0, 1))}[proj]
             assert abs((90 - t.mean() / DEG) - dec) < 1
+
+
+def test_oracle_under_sanitizers():
+    """The oracle's C++ (every projection, flat non-tiled / tiled with both
+    interpolations, HEALPix NEST / RING / tiled, clipped footprints, Precomp
+    and thread segments) built with AddressSanitizer and
+    UndefinedBehaviorSanitizer on the host runs without a report."""
+    import shutil
+    import subprocess
+    if shutil.which('g++') is None:
+        pytest.skip('no host C++ compiler')
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'oracle')
+    r = subprocess.run(['make', '-s', '-C', here, 'sanitize'], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert 'sanitized oracle run: 0 failures' in r.stdout
