@@ -445,20 +445,22 @@ def c5_mapmaking():
     """Map-making at its configuration (100 det x 40000 samples, CAR 1
     arcmin TQU, 1/f noise, Gaussian random sky): wall time of the solve to
     1e-6 relative residual with binned-preconditioned CG (capped at 40000
-    iterations), two-level CG, and flexible CG with the learned U-Net
-    correction on top of two-level (falling back to two-level PCG when it
-    stalls); setup and training times listed apart."""
+    iterations), two-level CG, two-level CG with the learned deflation space,
+    and flexible CG with the U-Net correction on top of that (falling back
+    to it when it stalls); setup, learning and training times listed apart."""
     from so3g_amd.studies import cg
     r = cg.run()
     keep = ('iters', 'seconds', 'final_residual', 'fallback_at', 'map_rms_error_TQU',
             'ms_per_iter', 'setup_s', 'train_s', 'pairs_s', 'train_steps', 'pairs',
-            'loss_first_last')
+            'loss_first_last', 'learn_s', 'harvest_iters', 'ritz_vectors', 'ritz_values_max')
     out = {'workload': r['workload']}
-    for k in ('binned', 'two_level', 'learned'):
+    for k in ('binned', 'two_level', 'deflated', 'learned'):
         if k in r:
             out[k] = {kk: v for kk, v in r[k].items() if kk in keep}
             out[k]['converged'] = bool(r[k]['final_residual'] <= 1e-6)
-    for k in ('speedup_two_level', 'speedup_learned_vs_binned', 'speedup_learned_vs_two_level'):
+    for k in ('speedup_two_level', 'speedup_deflated_vs_binned', 'speedup_deflated_vs_two_level',
+              'speedup_learned_vs_binned', 'speedup_learned_vs_two_level',
+              'speedup_learned_vs_deflated'):
         if k in r:
             out[k] = r[k]
     log(f'C5 map-making: {out}')

@@ -228,6 +228,97 @@ def _dot(a, b):
     return float(torch.sum(a * b))
 
 
+def harvest_ritz(apply_A, M, b, iters, k, tol=1e-14):
+    """Ritz vectors for the k smallest eigenvalues of the M-preconditioned
+    operator M A, learned from the Lanczos process of an M-preconditioned CG
+    run on the right-hand side b (Paige 1971; the CG <-> Lanczos relation of
+    Golub & Van Loan, Matrix Computations, sec. 10.2).  With z_j = M r_j, the
+    Lanczos vectors are q_j = (-1)^j z_j / sqrt(r_j . z_j) and the Lanczos
+    tridiagonal has T_jj = 1/alpha_j + beta_(j-1)/alpha_(j-1), T_j,j+1 =
+    sqrt(beta_j)/alpha_j.  Returns (theta (k,), U (k, *b.shape)) with U = Q Y,
+    Y the eigenvectors of T for its k smallest eigenvalues theta.  The
+    eigenvectors M A has for isolated small eigenvalues -- near-null modes of
+    A that M does not resolve -- are properties of the operator, the same for
+    every right-hand side, so a run on one realisation learns them for all."""
+    x = torch.zeros_like(b)
+    r = b.clone()
+    z = M(r)
+    rz = _dot(r, z)
+    bn = np.sqrt(_dot(b, b))
+    Q, alphas, betas = [], [], []
+    p = z.clone()
+    for j in range(iters):
+        Q.append(((-1) ** j / np.sqrt(rz)) * z)
+        Ap = apply_A(p)
+        alpha = rz / _dot(p, Ap)
+        alphas.append(alpha)
+        x.add_(p, alpha=alpha)
+        r.sub_(Ap, alpha=alpha)
+        if np.sqrt(_dot(r, r)) / bn < tol:
+            break
+        z = M(r)
+        rz_new = _dot(r, z)
+        betas.append(rz_new / rz)
+        rz = rz_new
+        p.mul_(betas[-1]).add_(z)
+    m = len(alphas)
+    T = np.zeros((m, m))
+    for j in range(m):
+        T[j, j] = 1 / alphas[j] + (betas[j - 1] / alphas[j - 1] if j > 0 else 0.)
+        if j + 1 < m:
+            T[j, j + 1] = T[j + 1, j] = np.sqrt(betas[j]) / alphas[j]
+    theta, Y = np.linalg.eigh(T)
+    k = min(k, m)
+    Yk = torch.from_numpy(np.ascontiguousarray(Y[:, :k])).to(b.device)
+    Qm = torch.stack(Q).reshape(m, -1)
+    U = (Yk.T @ Qm).reshape((k,) + tuple(b.shape))
+    return theta[:k], U
+
+
+class DeflatedPreconditioner:
+    """A base SPD preconditioner M plus a deflation space W (vectors learned
+    from earlier solves, harvest_ritz):
+
+      balanced (default, "A-DEF2", Tang, Nabben, Vuik & Erlangga 2009):
+        z = P^T M P r + W E^-1 W^T r,  P = I - A W E^-1 W^T,  E = W^T A W;
+      additive:  z = M r + W E^-1 W^T r.
+
+    Both are symmetric positive definite maps of r (plain PCG applies): the
+    deflated directions get an exact coarse solve, the rest M.  W is first
+    orthonormalised (directions whose Gram eigenvalue is below rtol of the
+    largest are dropped: Lanczos repeats converged Ritz vectors) and A W is
+    kept, so an application costs one M, a few dot products of k map
+    vectors, and no extra A."""
+
+    def __init__(self, apply_A, base, W, mode='balanced', rtol=1e-10):
+        self.M = base
+        self.mode = mode
+        k = W.shape[0]
+        self.shape = tuple(W.shape[1:])
+        Wf = W.reshape(k, -1)
+        G = Wf @ Wf.T
+        ev, V = torch.linalg.eigh(0.5 * (G + G.T))
+        keep = ev > rtol * ev.max()
+        B = V[:, keep] / ev[keep].sqrt()
+        self.W = (B.T @ Wf).contiguous()                        # (k', n), orthonormal
+        self.AW = torch.stack([apply_A(w.reshape(self.shape)).reshape(-1) for w in self.W])
+        E = self.W @ self.AW.T
+        E = 0.5 * (E + E.T)
+        ee, EV = torch.linalg.eigh(E)
+        floor = ee.max() * 1e-14
+        self.Einv = (EV / ee.clamp_min(floor)) @ EV.T
+        self.k = int(self.W.shape[0])
+
+    def __call__(self, r):
+        rf = r.reshape(-1)
+        c = self.Einv @ (self.W @ rf)                            # E^-1 W^T r
+        if self.mode == 'additive':
+            return self.M(r) + (self.W.T @ c).reshape(self.shape)
+        z1 = self.M((rf - self.AW.T @ c).reshape(self.shape)).reshape(-1)
+        z = z1 - self.W.T @ (self.Einv @ (self.AW @ z1)) + self.W.T @ c
+        return z.reshape(self.shape)
+
+
 def pcg(apply_A, b, precond, tol=1e-6, maxiter=2000, x0=None, flexible=False,
         callback=None, fallback=None, stall=(200, 0.9)):
     """(Flexible) preconditioned CG.  Stops when ||r|| / ||b|| < tol.
@@ -377,6 +468,27 @@ def collect_pairs(mm, M, rhs_list, per_rhs, tol=1e-12):
         E.append(p.to(torch.float32))
     for b in rhs_list:
         pcg(mm.apply_A, b, M, tol=tol, maxiter=per_rhs, callback=cb)
+    return torch.stack(R), torch.stack(E)
+
+
+def collect_residual_error_pairs(mm, M, rhs_list, per_rhs, tol_exact=1e-8, rmin=1e-5):
+    """(r_j, e_j) pairs along M-preconditioned CG runs on each right-hand
+    side: r_j = b - A x_j and e_j = x* - x_j, with x* the solve to tol_exact
+    (a tighter solve amplifies rounding along A's near-null space, where
+    the errors would reach 1e20 without meaning).  Pairs are taken while
+    ||r_j|| > rmin ||b|| and scaled by 1/||r_j|| (e = A^-1 r is linear), so
+    float32 holds them.  Returns two float32 tensors (n, nc, ny, nx)."""
+    R, E = [], []
+    for b in rhs_list:
+        xs, _ = pcg(mm.apply_A, b, M, tol=tol_exact, maxiter=20000)
+        bn = float(b.norm())
+
+        def cb(it, x, r, p, Ap):
+            nr = float(r.norm())
+            if nr > rmin * bn:
+                R.append((r / nr).to(torch.float32))
+                E.append(((xs - x) / nr).to(torch.float32))
+        pcg(mm.apply_A, b, M, tol=rmin, maxiter=per_rhs, callback=cb)
     return torch.stack(R), torch.stack(E)
 
 

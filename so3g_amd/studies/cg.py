@@ -1,7 +1,11 @@
 """Map-making study: CG to ||r||/||b|| = 1e-6 on the small observation
 (100 detectors x 40000 samples, CAR 1 arcmin TQU) with 1/f noise:
 binned-preconditioned CG, the two-level (binned + along-scan coarse space)
-preconditioner, and flexible CG with the learned correction on top of it.
+preconditioner, the two-level preconditioner with a learned deflation space
+(Ritz vectors of its smallest eigenvalues, learned from the Lanczos process
+of a solve on an independent realisation), and flexible CG with the U-Net
+correction (trained on residual/error pairs of the deflated solves) on top
+of that, falling back to it when it stalls.
 
 Noise: PSD_d(f) = sigma_d^2 (1 + (f / 1 Hz)^-3), sigma_d ~ lognormal(0, 0.3).
 Sky: flat-sky Gaussian random field on the map grid, C_l ~ l^-2.5 for T and
@@ -108,9 +112,10 @@ def summary(x, info, ctx, M):
 
 
 def run(tol=1e-6, maxiter=40000, learned=True, n_rhs=4, per_rhs=500, steps=3000,
-        batch=8, lr=3e-4, factor=(2, 2), h=16):
+        batch=8, lr=3e-4, factor=(1, 4), h=16, harvest_iters=1200, k_defl=32):
     import torch
-    from ..mapmaker import LearnedPreconditioner, collect_pairs, pcg, train_correction
+    from ..mapmaker import (DeflatedPreconditioner, LearnedPreconditioner,
+                            collect_residual_error_pairs, harvest_ritz, pcg, train_correction)
     from ..unet import UNet
     ctx = setup()
     mm = ctx['mm']
@@ -139,33 +144,53 @@ def run(tol=1e-6, maxiter=40000, learned=True, n_rhs=4, per_rhs=500, steps=3000,
     print('two_level', json.dumps(out['two_level']), file=sys.stderr, flush=True)
     if not learned:
         return out
-    # training pairs from the two-level PCG directions on independent
-    # realisations; the network corrects the two-level preconditioner
+    # learned deflation space: Ritz vectors of the two-level-preconditioned
+    # operator's smallest eigenvalues, from the Lanczos process of a solve on
+    # an independent realisation (the training solve)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rhs = [mm.rhs(make_data(ctx, 100 + j)[1]) for j in range(n_rhs)]
-    R, E = collect_pairs(mm, T, rhs, per_rhs)
+    b_train = mm.rhs(make_data(ctx, 100)[1])
+    theta, U = harvest_ritz(mm.apply_A, T, b_train, harvest_iters, k_defl)
+    D = DeflatedPreconditioner(mm.apply_A, T, U, mode='additive')
+    del U
+    torch.cuda.synchronize()
+    t_defl = time.perf_counter() - t0
+    x4, info4 = pcg(mm.apply_A, b, D, tol=tol, maxiter=maxiter)
+    out['deflated'] = summary(x4, info4, ctx, M)
+    out['deflated'].update({'learn_s': t_defl, 'harvest_iters': harvest_iters,
+                            'ritz_vectors': D.k, 'ritz_values_max': float(theta[-1])})
+    out['speedup_deflated_vs_binned'] = out['binned']['seconds'] / out['deflated']['seconds']
+    out['speedup_deflated_vs_two_level'] = out['two_level']['seconds'] / \
+        out['deflated']['seconds']
+    print('deflated', json.dumps(out['deflated']), file=sys.stderr, flush=True)
+    # U-Net correction on top of the deflated preconditioner, trained on
+    # residual/error pairs of its own CG solves on independent realisations
+    t0 = time.perf_counter()
+    rhs = [mm.rhs(make_data(ctx, 200 + j)[1]) for j in range(n_rhs)]
+    R, E = collect_residual_error_pairs(mm, D, rhs, per_rhs)
     torch.cuda.synchronize()
     t_pairs = time.perf_counter() - t0
     net = UNet(seed=0, factor=factor)
     t0 = time.perf_counter()
-    hist = train_correction(net, T, R, E, steps=steps, batch=batch, lr=lr,
+    hist = train_correction(net, D, R, E, steps=steps, batch=batch, lr=lr,
                             log=lambda s, l: print(f'step {s}: loss {l:.4f}', file=sys.stderr,
                                                    flush=True))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
+    n_pairs = int(R.shape[0])
     del R, E
-    L = LearnedPreconditioner(T, net, mm.P.map_shape)
+    L = LearnedPreconditioner(D, net, mm.P.map_shape)
     L(mm.zeros() + 1.)       # warm-up
-    b = mm.rhs(ctx['d'])
-    x3, info3 = pcg(mm.apply_A, b, L, tol=tol, maxiter=maxiter, flexible=True, fallback=T)
+    x3, info3 = pcg(mm.apply_A, b, L, tol=tol, maxiter=maxiter, flexible=True, fallback=D)
     out['learned'] = summary(x3, info3, ctx, M)
-    out['learned'].update({'pairs': int(n_rhs * per_rhs), 'pairs_s': t_pairs,
+    out['learned'].update({'pairs': n_pairs, 'pairs_s': t_pairs,
                            'pool_factor': list(factor),
                            'train_steps': steps, 'train_s': t_train,
                            'loss_first_last': [float(np.mean(hist[:20])),
                                                float(np.mean(hist[-20:]))]})
     out['speedup_learned_vs_binned'] = out['binned']['seconds'] / out['learned']['seconds']
     out['speedup_learned_vs_two_level'] = out['two_level']['seconds'] / out['learned']['seconds']
+    out['speedup_learned_vs_deflated'] = out['deflated']['seconds'] / out['learned']['seconds']
     print('learned', json.dumps(out['learned']), file=sys.stderr, flush=True)
     return out
 
@@ -179,7 +204,7 @@ def main(argv=None):
     ap.add_argument('--per-rhs', type=int, default=500)
     ap.add_argument('--n-rhs', type=int, default=4)
     ap.add_argument('--lr', type=float, default=3e-4)
-    ap.add_argument('--factor', type=int, nargs=2, default=(2, 2))
+    ap.add_argument('--factor', type=int, nargs=2, default=(1, 4))
     a = ap.parse_args(argv)
     print(json.dumps(run(a.tol, a.maxiter, not a.no_learned, a.n_rhs, a.per_rhs, a.steps,
                          lr=a.lr, factor=tuple(a.factor))), flush=True)
