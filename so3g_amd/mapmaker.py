@@ -305,15 +305,20 @@ class DeflatedPreconditioner:
         E = self.W @ self.AW.T
         E = 0.5 * (E + E.T)
         ee, EV = torch.linalg.eigh(E)
-        floor = ee.max() * 1e-14
-        self.Einv = (EV / ee.clamp_min(floor)) @ EV.T
+        ee = ee.clamp_min(ee.max() * 1e-14)
+        self.Einv = (EV / ee) @ EV.T
+        # W E^-1 W^T = V V^T with V^T = E^-1/2 W: the additive correction is
+        # two matrix-vector products
+        self.Vt = ((EV / ee.sqrt()) @ EV.T @ self.W).contiguous()
         self.k = int(self.W.shape[0])
 
     def __call__(self, r):
         rf = r.reshape(-1)
-        c = self.Einv @ (self.W @ rf)                            # E^-1 W^T r
         if self.mode == 'additive':
-            return self.M(r) + (self.W.T @ c).reshape(self.shape)
+            z = self.M(r)
+            z.view(-1).addmv_(self.Vt.T, self.Vt @ rf)
+            return z
+        c = self.Einv @ (self.W @ rf)                            # E^-1 W^T r
         z1 = self.M((rf - self.AW.T @ c).reshape(self.shape)).reshape(-1)
         z = z1 - self.W.T @ (self.Einv @ (self.AW @ z1)) + self.W.T @ c
         return z.reshape(self.shape)

@@ -112,7 +112,8 @@ def summary(x, info, ctx, M):
 
 
 def run(tol=1e-6, maxiter=40000, learned=True, n_rhs=4, per_rhs=500, steps=3000,
-        batch=8, lr=3e-4, factor=(1, 4), h=16, harvest_iters=1200, k_defl=32):
+        batch=8, lr=3e-4, factor=(1, 4), h=16, harvest_iters=1200, k_defl=32,
+        n_train_solves=3):
     import torch
     from ..mapmaker import (DeflatedPreconditioner, LearnedPreconditioner,
                             collect_residual_error_pairs, harvest_ritz, pcg, train_correction)
@@ -149,16 +150,22 @@ def run(tol=1e-6, maxiter=40000, learned=True, n_rhs=4, per_rhs=500, steps=3000,
     # an independent realisation (the training solve)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    b_train = mm.rhs(make_data(ctx, 100)[1])
-    theta, U = harvest_ritz(mm.apply_A, T, b_train, harvest_iters, k_defl)
-    D = DeflatedPreconditioner(mm.apply_A, T, U, mode='additive')
-    del U
+    Us, thetas = [], []
+    for j in range(n_train_solves):
+        b_train = mm.rhs(make_data(ctx, 100 + j)[1])
+        th, Uj = harvest_ritz(mm.apply_A, T, b_train, harvest_iters, k_defl)
+        Us.append(Uj)
+        thetas.append(th)
+    D = DeflatedPreconditioner(mm.apply_A, T, torch.cat(Us), mode='additive')
+    theta = np.concatenate(thetas)
+    del Us
     torch.cuda.synchronize()
     t_defl = time.perf_counter() - t0
     x4, info4 = pcg(mm.apply_A, b, D, tol=tol, maxiter=maxiter)
     out['deflated'] = summary(x4, info4, ctx, M)
     out['deflated'].update({'learn_s': t_defl, 'harvest_iters': harvest_iters,
-                            'ritz_vectors': D.k, 'ritz_values_max': float(theta[-1])})
+                            'training_solves': n_train_solves, 'ritz_vectors': D.k,
+                            'ritz_values_max': float(theta.max())})
     out['speedup_deflated_vs_binned'] = out['binned']['seconds'] / out['deflated']['seconds']
     out['speedup_deflated_vs_two_level'] = out['two_level']['seconds'] / \
         out['deflated']['seconds']
