@@ -2,8 +2,9 @@
 (100 detectors x 40000 samples, CAR 1 arcmin TQU) with 1/f noise:
 binned-preconditioned CG, the two-level (binned + along-scan coarse space)
 preconditioner, the two-level preconditioner with a learned deflation space
-(Ritz vectors of its smallest eigenvalues, learned from the Lanczos process
-of a solve on an independent realisation), and flexible CG with the U-Net
+(Ritz vectors of its smallest eigenvalues, learned from the Lanczos
+processes of solves on three independent realisations), and flexible CG
+with the U-Net
 correction (trained on residual/error pairs of the deflated solves) on top
 of that, falling back to it when it stalls.
 
@@ -11,7 +12,10 @@ Noise: PSD_d(f) = sigma_d^2 (1 + (f / 1 Hz)^-3), sigma_d ~ lognormal(0, 0.3).
 Sky: flat-sky Gaussian random field on the map grid, C_l ~ l^-2.5 for T and
 0.1 x that for E and B, rotated to Q/U.
 
-Run: python -m so3g_amd.studies.cg [--no-learned] [--steps N] [--n-rhs 4 --per-rhs 500]
+Each solve reports its true residual ||b - A x|| / ||b|| and its distance
+from a reference solution (the deflated PCG run to 1e-10) in the A-norm.
+
+Run: python -m so3g_amd.studies.cg [--no-learned] [--no-unet] [--steps N] [--n-rhs 4 --per-rhs 500]
 """
 import argparse
 import json
@@ -101,9 +105,27 @@ def map_error(x, sky, mask):
     return [float(v) for v in d.pow(2).mean(dim=1).sqrt()]
 
 
+def true_residual(mm, x, b):
+    """||b - A x|| / ||b||, recomputed from x (CG's own residual is updated
+    recursively and drifts from it)."""
+    r = b - mm.apply_A(x)
+    return float((r.norm() / b.norm()).item())
+
+
+def a_norm_error(mm, x, x_ref):
+    """||x - x_ref||_A / ||x_ref||_A: the error CG minimises, weighted by A,
+    so the near-null modes that the data barely constrain count little."""
+    e = x - x_ref
+    num = float((e * mm.apply_A(e)).sum().item())
+    den = float((x_ref * mm.apply_A(x_ref)).sum().item())
+    return float(np.sqrt(max(num, 0.) / den))
+
+
 def summary(x, info, ctx, M):
+    mm = ctx['mm']
     return {'iters': info['iters'], 'seconds': info['seconds'],
             'final_residual': info['residuals'][-1],
+            'true_residual': true_residual(mm, x, mm.rhs(ctx['d'])),
             'residual_every_100': [info['residuals'][i]
                                    for i in range(0, len(info['residuals']), 100)],
             'fallback_at': info.get('fallback_at'),
@@ -113,7 +135,7 @@ def summary(x, info, ctx, M):
 
 def run(tol=1e-6, maxiter=40000, learned=True, n_rhs=4, per_rhs=500, steps=3000,
         batch=8, lr=3e-4, factor=(1, 4), h=16, harvest_iters=1200, k_defl=32,
-        n_train_solves=3):
+        n_train_solves=3, ref_tol=1e-10, unet=True):
     import torch
     from ..mapmaker import (DeflatedPreconditioner, LearnedPreconditioner,
                             collect_residual_error_pairs, harvest_ritz, pcg, train_correction)
@@ -170,6 +192,22 @@ def run(tol=1e-6, maxiter=40000, learned=True, n_rhs=4, per_rhs=500, steps=3000,
     out['speedup_deflated_vs_two_level'] = out['two_level']['seconds'] / \
         out['deflated']['seconds']
     print('deflated', json.dumps(out['deflated']), file=sys.stderr, flush=True)
+    # reference solution: the deflated PCG run to ref_tol.  Solutions that
+    # meet the same tol differ in the near-null modes of A (Q/U on rows seen
+    # at one angle, per-row offsets), whose components of A^-1 b are mostly
+    # noise; the closer a solve gets to A^-1 b, the more of that noise its
+    # map carries, so the sky error is compared with the reference's own.
+    xr, infor = pcg(mm.apply_A, b, D, tol=ref_tol, maxiter=maxiter)
+    out['reference'] = {'tol': ref_tol, 'iters': infor['iters'],
+                        'final_residual': infor['residuals'][-1],
+                        'true_residual': true_residual(mm, xr, b),
+                        'map_rms_error_TQU': map_error(xr, ctx['sky'], M.mask)}
+    for k, xk in (('binned', x), ('two_level', x2), ('deflated', x4)):
+        out[k]['a_norm_error_vs_reference'] = a_norm_error(mm, xk, xr)
+        out[k]['map_rms_diff_vs_reference_TQU'] = map_error(xk, xr, M.mask)
+    print('reference', json.dumps(out['reference']), file=sys.stderr, flush=True)
+    if not unet:
+        return out
     # U-Net correction on top of the deflated preconditioner, trained on
     # residual/error pairs of its own CG solves on independent realisations
     t0 = time.perf_counter()
@@ -190,6 +228,8 @@ def run(tol=1e-6, maxiter=40000, learned=True, n_rhs=4, per_rhs=500, steps=3000,
     L(mm.zeros() + 1.)       # warm-up
     x3, info3 = pcg(mm.apply_A, b, L, tol=tol, maxiter=maxiter, flexible=True, fallback=D)
     out['learned'] = summary(x3, info3, ctx, M)
+    out['learned']['a_norm_error_vs_reference'] = a_norm_error(mm, x3, xr)
+    out['learned']['map_rms_diff_vs_reference_TQU'] = map_error(x3, xr, M.mask)
     out['learned'].update({'pairs': n_pairs, 'pairs_s': t_pairs,
                            'pool_factor': list(factor),
                            'train_steps': steps, 'train_s': t_train,
@@ -207,6 +247,7 @@ def main(argv=None):
     ap.add_argument('--tol', type=float, default=1e-6)
     ap.add_argument('--maxiter', type=int, default=40000)
     ap.add_argument('--no-learned', action='store_true')
+    ap.add_argument('--no-unet', action='store_true')
     ap.add_argument('--steps', type=int, default=3000)
     ap.add_argument('--per-rhs', type=int, default=500)
     ap.add_argument('--n-rhs', type=int, default=4)
@@ -214,7 +255,7 @@ def main(argv=None):
     ap.add_argument('--factor', type=int, nargs=2, default=(1, 4))
     a = ap.parse_args(argv)
     print(json.dumps(run(a.tol, a.maxiter, not a.no_learned, a.n_rhs, a.per_rhs, a.steps,
-                         lr=a.lr, factor=tuple(a.factor))), flush=True)
+                         lr=a.lr, factor=tuple(a.factor), unet=not a.no_unet)), flush=True)
 
 
 if __name__ == '__main__':

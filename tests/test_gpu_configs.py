@@ -92,7 +92,9 @@ def test_c5_cg_solvers(gpu):
     three other noise realisations) takes two-level to it in at least 30 %
     fewer iterations; flexible CG with the U-Net correction (300 training steps
     here) reaches it too, falling back to the deflated preconditioner when
-    the network stops making progress."""
+    the network stops making progress.  Every converged solve is checked
+    against A^-1 b itself: its residual recomputed from x, and its distance
+    from a reference solve run to 1e-10."""
     from so3g_amd.studies import cg
     r = cg.run(tol=1e-6, maxiter=2500, learned=True, n_rhs=2, per_rhs=200, steps=300)
     assert r['binned']['final_residual'] > 1e-6
@@ -104,11 +106,23 @@ def test_c5_cg_solvers(gpu):
     assert r['deflated']['ritz_vectors'] >= 8
     assert r['learned']['final_residual'] <= 1e-6
     assert r['learned']['iters'] < 2500
-    # The error against the true sky is noise in the poorly measured
-    # (near-null) modes of A, which solutions with the same 1e-6 residual
-    # carry to different extents: the deflated solve resolves more of them
-    # (its Ritz vectors are those modes), so its sky error may be larger,
-    # within a bound.
+    ref = r['reference']
+    assert ref['true_residual'] <= 1e-9
+    for k in ('two_level', 'deflated', 'learned'):
+        # CG's recursive residual has not drifted from b - A x
+        assert r[k]['true_residual'] <= 1.1e-6, k
+        # ||x - x_ref||_A / ||x_ref||_A is of the residual's order (both
+        # solves measured 1.6e-4 and 2.4e-4 at 1e-6)
+        assert r[k]['a_norm_error_vs_reference'] < 2e-3, k
+    # The near-null modes of A (Q/U on rows seen at one angle, per-row
+    # offsets) are poorly constrained: their components of A^-1 b are mostly
+    # noise.  Two-level CG stops at 1e-6 with them barely started (its map
+    # differs from the reference by about the sky's rms); the deflation space
+    # holds them, so the deflated solves land on the reference map, and their
+    # sky error is the reference solution's own.
     for k in ('deflated', 'learned'):
-        for a, b in zip(r['two_level']['map_rms_error_TQU'], r[k]['map_rms_error_TQU']):
-            assert b < 1.5 * a, k
+        for a, b in zip(r['two_level']['map_rms_diff_vs_reference_TQU'],
+                        r[k]['map_rms_diff_vs_reference_TQU']):
+            assert b < 0.05 * a, k
+        for a, b in zip(ref['map_rms_error_TQU'], r[k]['map_rms_error_TQU']):
+            assert abs(b - a) < 0.02 * a, k
