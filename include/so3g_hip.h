@@ -483,6 +483,13 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *   "scatter_xcd"   1: XCD-private partial maps for non-tiled scatters,
  *                   reduced after the kernel (default 0)
  *   "xcd_budget_mb" device memory allowed for those copies (default 4096)
+ *   "scatter_hash"  on-the-fly to_map / to_weights through LDS-privatised
+ *                   partial maps: 0 never, 1 (default) where measured
+ *                   faster than global atomics, 2 always with small tables
+ *                   (256 threads, 512 slots), 3 always with large ones
+ *                   (1024 threads, 112 KB)
+ *   "hash_dets", "hash_span"  detectors and samples per block of modes 2
+ *                   and 3 (0, the default: 4 x 256 and 2 x 32768)
  *   "pc_from_dpb"   detectors per block of the Precomp gather (default 2)
  *   "plan_div"      plan kernels' detector index by float reciprocal (1,
  *                   default) or integer division (0)

@@ -88,11 +88,13 @@ struct Options {
     int64_t xcd_budget_mb = 4096;
     int pc_from_dpb = 2;       // detectors per block of the packed-map from_map
     // On-the-fly to_map / to_weights through LDS-privatised partial maps:
-    // 0 never, 1 where measured faster than global atomics (HEALPix maps,
-    // and to_weights on non-tiled flat maps), 2 always.
+    // 0 never, 1 where measured faster than global atomics (all but tiled
+    // flat to_map, the table per case), 2 always with the small table
+    // (256 threads, 512 slots), 3 always with the large one (1024 threads,
+    // 112 KB).
     int scatter_hash = 1;
-    int hash_dets = 2;         // detectors per block of that scatter
-    int hash_span = 32768;     // samples per block of that scatter
+    int hash_dets = 0;         // detectors per block in modes 2, 3 (0: 4 / 2)
+    int hash_span = 0;         // samples per block in modes 2, 3 (0: 256 / 32768)
 };
 Options g_opt;
 constexpr int kNumXcd = 8;
@@ -524,21 +526,39 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
     a.sig_in = signal;
     a.det_w = det_weights;
     a.hash_cls = 0;
-    // Where the LDS tables pay (1000 x 720000 samples, ms, global atomics
-    // -> LDS): HEALPix nside 4096 to_map 17.2 -> 14.2, to_weights 30.8 ->
-    // 17.8; CAR 0.5' to_weights 17.7 -> 15.3.  Not for CAR to_map (9.1 ->
-    // 11.5) nor tiled CAR to_map (11.0 -> 13.8): a 112 KB table leaves one
-    // block (4 waves per SIMD) per compute unit, and the latency-bound
-    // float64 pointing loses more to that than the atomics it saves there.
-    const bool hash_pays = g->pixelizor == SO3G_PIX_HEALPIX ||
-                           (op == OP_TO_WEIGHTS && !g->tiled);
-    if ((g_opt.scatter_hash == 2 || (g_opt.scatter_hash == 1 && hash_pays)) &&
-        prec == SO3G_PREC_REF && segs == nullptr && hash_keys_fit(g)) {
+    // LDS-privatised partial maps (k_scatter_lds) where they measured faster
+    // than global atomics (1000 x 720000 samples, ms; global / large table /
+    // small table): CAR 0.5' to_map 9.0 / 11.6 / 8.6; CAR to_weights
+    // 17.7 / 15.3 / 11.8 (8 detectors per block); HEALPix nside 4096 to_map
+    // 17.4 / 14.9 / 15.4; HEALPix to_weights 33.2 / 20.0 / 17.3; tiled CAR
+    // to_map 9.4 / 13.7 / 11.5; tiled CAR to_weights 17.7 / 19.7 / 13.9.
+    // Every global atomic is forwarded to the
+    // memory side, about 6 lanes per 64 B request for a wave's run ends; a
+    // small table (256 threads, 512 slots, home slot = pixel mod 512) sums
+    // four row-mates over one 256-sample chunk and flushes consecutive
+    // pixels together, and keeps more blocks per compute unit than the
+    // large one (1024 threads, 112 KB, a span of 32768 samples).
+    int cls = 0, per_block = 0, span = 0;
+    if (g_opt.scatter_hash == 1) {
+        if (g->pixelizor == SO3G_PIX_HEALPIX) {
+            if (op == OP_TO_MAP) cls = 1, per_block = 2, span = 32768;
+            else cls = 2, per_block = 4, span = 256;
+        } else if (op == OP_TO_WEIGHTS) {
+            cls = 2, per_block = g->tiled ? 4 : 8, span = 256;
+        } else if (!g->tiled) {
+            cls = 2, per_block = 4, span = 256;
+        }
+    } else if (g_opt.scatter_hash >= 2) {
+        cls = g_opt.scatter_hash == 2 ? 2 : 1;
+        per_block = g_opt.hash_dets > 0 ? g_opt.hash_dets : (cls == 2 ? 4 : 2);
+        span = g_opt.hash_span > 0 ? g_opt.hash_span : (cls == 2 ? 256 : 32768);
+    }
+    if (cls > 0 && prec == SO3G_PREC_REF && segs == nullptr && hash_keys_fit(g)) {
         r = scatter_order(g, a, &a.hw.order);
         if (r) return r;
-        a.hash_cls = 1;
-        a.hw.dets = g_opt.hash_dets;
-        a.hw.span = g_opt.hash_span;
+        a.hash_cls = cls;
+        a.hw.dets = per_block;
+        a.hw.span = span;
         // the 1-D grid of (group, span) blocks stays below 2^31
         while (((n_t + a.hw.span - 1) / a.hw.span) * ((n_det + a.hw.dets - 1) / a.hw.dets) >
                INT32_MAX)
@@ -827,14 +847,15 @@ int so3g_set_option(const char *name, int64_t value) {
     else if (n == "scatter_xcd") g_opt.scatter_xcd = (int)(value != 0);
     else if (n == "pc_from_dpb") g_opt.pc_from_dpb = (int)std::max<int64_t>(1, value);
     else if (n == "scatter_hash") {
-        if (value < 0 || value > 2) return fail(SO3G_ERR_VALUE, "scatter_hash must be 0..2");
+        if (value < 0 || value > 3) return fail(SO3G_ERR_VALUE, "scatter_hash must be 0..3");
         g_opt.scatter_hash = (int)value;
     } else if (n == "hash_dets") {
-        if (value < 1 || value > 4096) return fail(SO3G_ERR_VALUE, "hash_dets must be 1..4096");
+        if (value < 0 || value > 4096)
+            return fail(SO3G_ERR_VALUE, "hash_dets must be 0 (default) or 1..4096");
         g_opt.hash_dets = (int)value;
     } else if (n == "hash_span") {
-        if (value < 64 || value > (1 << 24))
-            return fail(SO3G_ERR_VALUE, "hash_span must be 64..2^24");
+        if (value != 0 && (value < 64 || value > (1 << 24)))
+            return fail(SO3G_ERR_VALUE, "hash_span must be 0 (default) or 64..2^24");
         g_opt.hash_span = (int)value;
     }
     else if (n == "xcd_budget_mb") g_opt.xcd_budget_mb = value;

@@ -522,11 +522,14 @@ __device__ __forceinline__ double *hash_elem(const Geo &g, const MapView<double>
 }
 
 // Adds v to the slot of key; false when no slot was found in kProbes probes.
-template <int NV, int H>
+// MOD: the home slot is key mod H, so consecutive keys (a stretch of map
+// pixels) take consecutive slots and the flush adds them to the map in
+// coalesced runs; otherwise a multiplicative hash spreads them.
+template <int NV, int H, bool MOD>
 __device__ __forceinline__ bool hash_add(int32_t *hk, double *hv, int32_t key,
                                          const double (&v)[NV]) {
     constexpr int kBits = __builtin_ctz(H);
-    uint32_t h = ((uint32_t)key * 2654435761u) >> (32 - kBits);
+    uint32_t h = MOD ? ((uint32_t)key & (H - 1)) : ((uint32_t)key * 2654435761u) >> (32 - kBits);
     for (int probe = 0; probe < kProbes; probe++) {
         // One LDS round trip per probe: the compare-and-swap claims an empty
         // slot and returns the key already there otherwise.
@@ -558,7 +561,7 @@ __device__ __forceinline__ int hash_lead(int k) {
 // W = false: to_map (NV = NC values per sample, Projection.cxx:1807);
 // W = true: to_weights (the NC (NC + 1) / 2 upper-triangle products,
 // Projection.cxx:1850).
-template <int PROJ, int PK, int INTERP, int NC, bool W, int NT, int H>
+template <int PROJ, int PK, int INTERP, int NC, bool W, int NT, int H, bool MOD>
 __global__ __launch_bounds__(NT) void k_scatter_lds(PtgIn in, Geo g, MapView<double> m,
                                                     const float *resp, const float *sig,
                                                     const float *det_weights, HashWork hw,
@@ -637,7 +640,7 @@ __global__ __launch_bounds__(NT) void k_scatter_lds(PtgIn in, Geo g, MapView<dou
 #pragma unroll
             for (int jp = 0; jp < NP; jp++) {
                 const bool emit = wave_runs<NV>((int64_t)key[jp], v[jp]) && key[jp] >= 0;
-                if (emit && !hash_add<NV, H>(hk, hv, key[jp], v[jp])) {
+                if (emit && !hash_add<NV, H, MOD>(hk, hv, key[jp], v[jp])) {
 #pragma unroll
                     for (int c = 0; c < NV; c++) {
                         const int lead = W ? hash_lead<NC>(c) : c;

@@ -36,8 +36,8 @@ struct OpArgs {
     const int32_t *tile_dom;
     int32_t *dom_out;
     int32_t *hits;
-    // LDS-privatised scatter (to_map / to_weights; 1024 threads and a
-    // 112 KB table per block): hash_cls 0 = global atomics, 1 = LDS.
+    // LDS-privatised scatter (to_map / to_weights): hash_cls 0 = global
+    // atomics, 1 = large tables, 2 = small tables (launch_scatter_cls).
     int hash_cls;
     HashWork hw;
     // batch ops
@@ -55,20 +55,25 @@ constexpr int hash_slots(int nv, int budget) {
     return h;
 }
 
-template <int PROJ, int PK, int INTERP, int NC, bool W, int NT, int BUDGET>
+template <int PROJ, int PK, int INTERP, int NC, bool W, int NT, int H, bool MOD>
 void launch_scatter_lds(const OpArgs &a) {
-    constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
-    constexpr int H = hash_slots(NV, BUDGET);
     const dim3 grid((unsigned)(((a.in.n_t + a.hw.span - 1) / a.hw.span) *
                                ((a.in.n_det + a.hw.dets - 1) / a.hw.dets)));
     MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
-    hipLaunchKernelGGL((k_scatter_lds<PROJ, PK, INTERP, NC, W, NT, H>), grid, dim3(NT), 0,
+    hipLaunchKernelGGL((k_scatter_lds<PROJ, PK, INTERP, NC, W, NT, H, MOD>), grid, dim3(NT), 0,
                        a.stream, a.in, a.g, m, a.resp, a.sig_in, a.det_w, a.hw, a.flags);
 }
 
+// hash_cls 1: 1024 threads and a 112 KB table per block (one block per
+// compute unit), multiplicative hash; hash_cls 2: 256 threads and 512 slots
+// (14 KB for to_map TQU), home slot = key mod 512.
 template <int PROJ, int PK, int INTERP, int NC, bool W>
 void launch_scatter_cls(const OpArgs &a) {
-    launch_scatter_lds<PROJ, PK, INTERP, NC, W, 1024, 112 * 1024>(a);
+    constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
+    if (a.hash_cls == 2)
+        launch_scatter_lds<PROJ, PK, INTERP, NC, W, 256, 512, true>(a);
+    else
+        launch_scatter_lds<PROJ, PK, INTERP, NC, W, 1024, hash_slots(NV, 112 * 1024), false>(a);
 }
 
 // Grid of the on-the-fly kernels: one block per chunk (segment mode), or

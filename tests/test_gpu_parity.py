@@ -371,11 +371,11 @@ def test_projectionist_torch_path(gpu):
 
 
 @pytest.mark.parametrize('agg,xcd,lds', [(0, 0, 0), (1, 1, 0), (0, 1, 1), (1, 0, 1), (1, 0, 2),
-                                         (1, 1, 2)])
+                                         (1, 1, 2), (1, 0, 3), (0, 0, 3)])
 def test_scatter_variants(gpu, agg, xcd, lds):
     """Every scatter variant (wave-level run aggregation on/off, XCD-private
     partial maps on/off, LDS-privatised tables never / where they pay /
-    always) gives the oracle's maps."""
+    always small / always large) gives the oracle's maps."""
     import so3g_amd
     from so3g_amd import _native
     names = ('scatter_agg', 'scatter_xcd', 'scatter_hash')
@@ -417,21 +417,23 @@ def test_scatter_variants(gpu, agg, xcd, lds):
             _native.set_option(n, v)
 
 
+@pytest.mark.parametrize('mode', [2, 3])
 @pytest.mark.parametrize('proj,tiled,interp', [('CAR', False, 0), ('CAR', False, 1),
                                                ('CAR', True, 0), ('HP', True, 0)])
-def test_lds_scatter_overflow(gpu, proj, tiled, interp):
+def test_lds_scatter_overflow(gpu, proj, tiled, interp, mode):
     """LDS tables that overflow: a fine map on which a block's samples visit
     tens of thousands of pixels (CAR 300 x 4000 over the footprint; HEALPix
-    nside 4096 in 4096-pixel tiles, active tiles only) against 4096 or 2048
-    slots, so many runs take the global-atomic fallback; maps still equal
-    the oracle's.  Detector groups of 1 and 3 and spans of 64 and 1000
-    samples cover partial groups and partial spans."""
+    nside 4096 in 4096-pixel tiles, active tiles only) against 512 slots
+    (mode 2) or 4096 / 2048 (mode 3), so many runs take the global-atomic
+    fallback; maps still equal the oracle's.  Detector groups of 1 and 3
+    and spans of 64 and 1000 samples cover partial groups and partial
+    spans."""
     from so3g_amd import _native
     names = ('scatter_hash', 'hash_dets', 'hash_span')
     old = [_native.get_option(n) for n in names]
     try:
         for dets, span in ((1, 64), (3, 1000), (2, 32768)):
-            for n, v in zip(names, (2, dets, span)):
+            for n, v in zip(names, (mode, dets, span)):
                 _native.set_option(n, v)
             if proj == 'HP':
                 c = Case('HP', comps='TQU', tiled=True, nside=4096, nside_tile=64, n_det=7,
