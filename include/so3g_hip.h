@@ -313,11 +313,14 @@ int so3g_pc_plan_destroy(so3g_pc_plan *plan);
  * launches the same kernel once per observation (for comparison).  Results
  * equal so3g_to_map / so3g_from_map called per observation (to_map up to
  * the order of the float64 additions).  The observation arrays must stay
- * allocated while the batch is used.  so3g_batch_tune times every launch
- * shape (dpb in {1,2,4,8,16} x blocks per CU in {2,4,8,16, one per item})
- * and keeps the fastest for the op (0 from_map, 1 to_map); results holds
- * (dpb, blocks per CU, ms) triples.  Tuning to_map adds into `map`: pass a
- * scratch map. */
+ * allocated while the batch is used.  to_map can sum each work item in an
+ * LDS-privatised partial map (so3g_batch_set_table: 0 global atomics, 512
+ * or 1024 slots; the detectors of an item taken in map-row order, computed
+ * at creation).  so3g_batch_tune times every launch shape (dpb in
+ * {1,2,4,8,16} x blocks per CU in {2,4,8,16, one per item}, and for to_map
+ * x table slots in {0, 512, 1024}) and keeps the fastest for the op (0
+ * from_map, 1 to_map); results holds (dpb, blocks per CU, slots, ms)
+ * quadruples.  Tuning to_map adds into `map`: pass a scratch map. */
 typedef struct so3g_batch so3g_batch;
 typedef struct {
     const double *bore;        /* (n_t, 4) native-frame boresight */
@@ -337,10 +340,12 @@ int so3g_batch_to_map(so3g_batch *b, double *map, double *const *tiles,
 int so3g_batch_from_map(so3g_batch *b, const double *map, const double *const *tiles,
                         const int32_t *tile_dims, int accumulate, int mode, void *stream);
 int so3g_batch_set_shape(so3g_batch *b, int op, int dpb, int grid_mult);
+int so3g_batch_set_table(so3g_batch *b, int slots);
 int so3g_batch_tune(so3g_batch *b, int op, double *map, double *const *tiles,
                     const int32_t *tile_dims, int reps, double *results, int max_results,
                     int *n_results, void *stream);
-/* info: n_obs, det*samples, from (dpb, blocks/CU), to (dpb, blocks/CU), CUs */
+/* info: n_obs, det*samples, from (dpb, blocks/CU), to (dpb, blocks/CU), CUs,
+ * to_map table slots */
 int so3g_batch_info(const so3g_batch *b, int64_t *info);
 int so3g_batch_destroy(so3g_batch *b);
 

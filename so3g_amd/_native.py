@@ -90,6 +90,7 @@ _SIGS = {
     'so3g_batch_to_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_vp]),
     'so3g_batch_from_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp]),
     'so3g_batch_set_shape': (c_int, [c_vp, c_int, c_int, c_int]),
+    'so3g_batch_set_table': (c_int, [c_vp, c_int]),
     'so3g_batch_tune': (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_vp, c_int,
                                 ctypes.POINTER(c_int), c_vp]),
     'so3g_batch_info': (c_int, [c_vp, c_vp]),

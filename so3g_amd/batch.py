@@ -107,23 +107,29 @@ class ObservationBatch:
     def set_shape(self, op, dpb, blocks_per_cu):
         N.check(N.load().so3g_batch_set_shape(self._h, OPS[op], int(dpb), int(blocks_per_cu)))
 
+    def set_table(self, slots):
+        """to_map through an LDS table of `slots` slots per work item (512
+        or 1024), or global atomics (0)."""
+        N.check(N.load().so3g_batch_set_table(self._h, int(slots)))
+
     def tune(self, op, map, reps=1):
         """Time every launch shape on this batch and keep the fastest for op.
         Tuning to_map adds into `map` (pass a scratch map).  Returns a list
-        of (dets per block, blocks per CU (0: one per item), ms)."""
+        of (dets per block, blocks per CU (0: one per item), LDS table slots
+        (0: global atomics), ms)."""
         call, m, tp, dp = self._map_args(map, False)
-        res = (ctypes.c_double * 300)()
+        res = (ctypes.c_double * 400)()
         n = ctypes.c_int()
         N.check(N.load().so3g_batch_tune(self._h, OPS[op], m, tp, dp, int(reps), res, 100,
                                          ctypes.byref(n), _stream()))
-        return [(int(res[3 * i]), int(res[3 * i + 1]), float(res[3 * i + 2]))
+        return [(int(res[4 * i]), int(res[4 * i + 1]), int(res[4 * i + 2]), float(res[4 * i + 3]))
                 for i in range(min(n.value, 100))]
 
     def info(self):
         a = (ctypes.c_int64 * 8)()
         N.check(N.load().so3g_batch_info(self._h, a))
         return {'n_obs': a[0], 'det_samples': a[1], 'from_map_shape': (a[2], a[3]),
-                'to_map_shape': (a[4], a[5]), 'n_cu': a[6]}
+                'to_map_shape': (a[4], a[5]), 'n_cu': a[6], 'to_map_slots': a[7]}
 
     def close(self):
         h = getattr(self, '_h', None)

@@ -7,8 +7,9 @@
 // them with a block stride, so that a single launch -- one node of a
 // captured graph -- covers observations of any mix of sizes without a
 // host round trip between them.  Each item runs the same per-sample body
-// as the single-observation kernels (from_map_sample, to_map_sample), so
-// results match one launch per observation.
+// as the single-observation kernels (from_map_sample, to_map_sample, or
+// lds_scatter_det with a table per item), so results match one launch per
+// observation.
 #pragma once
 
 #include "kernels.h"
@@ -25,6 +26,7 @@ struct ObsDev {
     int64_t n_t, n_det;
     int64_t item0;             // first work item of this observation
     int64_t n_tt;              // time tiles, ceil(n_t / kBlock)
+    const int32_t *order;      // detector order of the LDS-table scatter
 };
 
 // Calls f(obs, in, d, t, qbore, valid) for every detector of the block's
@@ -65,6 +67,48 @@ __global__ __launch_bounds__(kBlock) void k_batch_to_map(const ObsDev *obs, int 
                                                                o.det_w, flags, d, t, qb,
                                                                valid);
                     });
+}
+
+// to_map of a batch through LDS-privatised partial maps: each work item
+// (256 samples x dpb detectors of one observation, the detectors taken in
+// the observation's row order, so that an item's detectors sweep the same
+// pixels) adds into a table of H slots in LDS (home slot = element number
+// mod H, so a stretch of pixels takes a stretch of slots), flushed to the
+// map with one FP64 atomic per occupied slot and value before the block's
+// next item; a run that finds no slot adds to the map directly.
+template <int PROJ, int PK, int INTERP, int NC, int H>
+__global__ __launch_bounds__(kBlock) void k_batch_to_map_lds(const ObsDev *obs, int n_obs,
+                                                             int64_t n_items, int dpb,
+                                                             Tabs tabs, Geo g,
+                                                             MapView<double> m, int *flags) {
+    __shared__ int32_t hk[H];
+    __shared__ double hv[NC * H];
+    for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        lds_table_clear<NC, H>(hk, hv, kBlock);
+        __syncthreads();
+        int lo = 0, hi = n_obs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (obs[mid].item0 <= it) lo = mid;
+            else hi = mid - 1;
+        }
+        const ObsDev &o = obs[lo];
+        const int64_t local = it - o.item0;
+        const int64_t dg = local / o.n_tt;
+        const int64_t t = (local - dg * o.n_tt) * kBlock + threadIdx.x;
+        const bool valid = t < o.n_t;
+        const Q4 qb = valid ? load_q(o.bore + 4 * t) : Q4{1., 0., 0., 0.};
+        const PtgIn in{o.bore, o.dets, o.n_t, o.n_det, tabs, dpb, 1, nullptr};
+        const int64_t d0 = dg * dpb;
+        const int64_t d1 = min(d0 + (int64_t)dpb, o.n_det);
+        for (int64_t j = d0; j < d1; j++)
+            lds_scatter_det<PROJ, PK, INTERP, NC, false, H, true>(
+                in, g, m, o.resp, o.sig_in, o.det_w, flags, o.order ? o.order[j] : j, t, qb,
+                valid, hk, hv);
+        __syncthreads();
+        lds_table_flush<PK, NC, false, H>(g, m, hk, hv, kBlock);
+        __syncthreads();
+    }
 }
 
 template <int PROJ, int PK, int INTERP, int NC>

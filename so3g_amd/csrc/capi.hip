@@ -269,6 +269,33 @@ int op_args(const so3g_geom *g, void *stream, OpArgs &a) {
 int launch(int proj, const OpArgs &a) { return run_dispatch(proj, a); }
 int flags_zero(hipStream_t stream) { return zero_flags(stream); }
 int flags_check_tile(hipStream_t stream) { return check_tile_flag(stream); }
+
+// Detectors sorted by the map row of their coordinates co (n_det x 4, lon
+// lat ...; rows of the pixel height, or of the HEALPix pixel size), then by
+// longitude: row-mates of a constant-elevation scan sweep the same pixels.
+void row_order(const so3g_geom *g, const double *co, int64_t n_det, int32_t *idx) {
+    double row = 1.;
+    if (g->pixelizor == SO3G_PIX_FLAT) row = std::fabs(g->cdelt[0]);
+    else row = std::sqrt(4. * M_PI / (12. * (double)g->nside * g->nside));
+    if (!(row > 0.)) row = 1.;
+    std::vector<int64_t> rows(n_det);
+    for (int64_t d = 0; d < n_det; d++) {
+        const double y = std::floor(co[4 * d + 1] / row);
+        rows[d] = std::isfinite(y) ? (int64_t)std::max(-1e15, std::min(1e15, y)) : 0;
+    }
+    for (int64_t i = 0; i < n_det; i++) idx[i] = (int32_t)i;
+    std::stable_sort(idx, idx + n_det, [&](int32_t x, int32_t y) {
+        if (rows[x] != rows[y]) return rows[x] < rows[y];
+        return co[4 * x] < co[4 * y];
+    });
+}
+
+// True when every map element of the geometry has an int32 key.
+bool hash_keys_fit(const so3g_geom *g) {
+    if (g->pixelizor != SO3G_PIX_FLAT) return true;   // 12 nside^2 <= 8.1e8
+    if (!g->tiled) return g->npix < INT32_MAX;
+    return (int64_t)g->n_tile * g->tile_shape[0] * g->tile_shape[1] < INT32_MAX;
+}
 }  // namespace so3g
 
 extern "C" {
@@ -432,6 +459,7 @@ int so3g_from_map(const so3g_geom *g, const double *map, const double *const *ti
 // middle sample (one k_coords launch over n_det samples), sorted by map row
 // (the coordinate across the scan, in units of the pixel height) and then
 // along the row, so that a block's detectors sweep the same pixels.
+
 static int scatter_order(const so3g_geom *g, const OpArgs &a0, const int32_t **order) {
     const int64_t n_det = a0.in.n_det, n_t = a0.in.n_t;
     if (g_state.order_bore == a0.in.bore && g_state.order_dets == a0.in.dets &&
@@ -462,21 +490,8 @@ static int scatter_order(const so3g_geom *g, const OpArgs &a0, const int32_t **o
     HIP_TRY(hipMemcpyAsync(co.data(), g_state.order_co_d, n_det * 4 * sizeof(double),
                            hipMemcpyDeviceToHost, a.stream));
     HIP_TRY(hipStreamSynchronize(a.stream));
-    double row = 1.;
-    if (g->pixelizor == SO3G_PIX_FLAT) row = std::fabs(g->cdelt[0]);
-    else row = std::sqrt(4. * M_PI / (12. * (double)g->nside * g->nside));
-    if (!(row > 0.)) row = 1.;
-    std::vector<int64_t> rows(n_det);
-    for (int64_t d = 0; d < n_det; d++) {
-        const double y = std::floor(co[4 * d + 1] / row);
-        rows[d] = std::isfinite(y) ? (int64_t)std::max(-1e15, std::min(1e15, y)) : 0;
-    }
     std::vector<int32_t> idx(n_det);
-    for (int64_t i = 0; i < n_det; i++) idx[i] = (int32_t)i;
-    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
-        if (rows[x] != rows[y]) return rows[x] < rows[y];
-        return co[4 * x] < co[4 * y];
-    });
+    so3g::row_order(g, co.data(), n_det, idx.data());
     HIP_TRY(hipMemcpyAsync(g_state.order_d, idx.data(), n_det * sizeof(int32_t),
                            hipMemcpyHostToDevice, a.stream));
     HIP_TRY(hipStreamSynchronize(a.stream));
@@ -487,13 +502,6 @@ static int scatter_order(const so3g_geom *g, const OpArgs &a0, const int32_t **o
     g_state.order_geom = *g;
     *order = g_state.order_d;
     return SO3G_OK;
-}
-
-// True when every map element of the geometry has an int32 key.
-static bool hash_keys_fit(const so3g_geom *g) {
-    if (g->pixelizor != SO3G_PIX_FLAT) return true;   // 12 nside^2 <= 8.1e8
-    if (!g->tiled) return g->npix < INT32_MAX;
-    return (int64_t)g->n_tile * g->tile_shape[0] * g->tile_shape[1] < INT32_MAX;
 }
 
 static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,

@@ -494,30 +494,36 @@ constexpr int32_t kHashEmpty = -1;
 constexpr int kProbes = 32;
 
 // int32 key of a map element (map pixel numbering of the pixelizor; tiled
-// flat maps number tile t's pixels from t * ty * tx).
+// flat maps number tile t's pixels from t * ty * tx, rows of the nominal
+// tile width tx, so that no tile table is read per sample).
 template <int PK>
-__device__ __forceinline__ int32_t hash_key(const Geo &g, const int32_t *tile_dims,
-                                            const Pt &p) {
+__device__ __forceinline__ int32_t hash_key(const Geo &g, const Pt &p) {
     if constexpr (PK == PK_FLAT) return p.i0 * g.nx + p.i1;
     else if constexpr (PK == PK_HP) return p.i0;
     else if constexpr (PK == PK_HP_TILED) return p.i0 * g.npt + p.i1;
-    else return p.i0 * (g.ty * g.tx) + p.i1 * tile_dims[2 * p.i0 + 1] + p.i2;
+    else return p.i0 * (g.ty * g.tx) + p.i1 * g.tx + p.i2;
 }
 
-// Map element of slot `lead` for a key (inverse of hash_key).
+// Map element of a key's component 0 and the distance between its
+// components (inverse of hash_key; one tile lookup for all components).
 template <int PK>
 __device__ __forceinline__ double *hash_elem(const Geo &g, const MapView<double> &m,
-                                             int32_t key, int lead) {
+                                             int32_t key, int64_t &plane) {
     if constexpr (PK == PK_FLAT || PK == PK_HP) {
-        return m.map + lead * g.npix + key;
+        plane = g.npix;
+        return m.map + key;
     } else if constexpr (PK == PK_HP_TILED) {
         const int32_t t = key / g.npt;
-        return m.tiles[t] + (int64_t)lead * g.npt + (key - t * g.npt);
+        plane = g.npt;
+        return m.tiles[t] + (key - t * g.npt);
     } else {
         const int32_t ts = g.ty * g.tx;
         const int32_t t = key / ts;
-        const int64_t plane = (int64_t)m.tile_dims[2 * t] * m.tile_dims[2 * t + 1];
-        return m.tiles[t] + lead * plane + (key - t * ts);
+        const int32_t r = key - t * ts;
+        const int32_t row = r / g.tx;
+        const int32_t tw = m.tile_dims[2 * t + 1];
+        plane = (int64_t)m.tile_dims[2 * t] * tw;
+        return m.tiles[t] + ((int64_t)row * tw + (r - row * g.tx));
     }
 }
 
@@ -561,20 +567,114 @@ __device__ __forceinline__ int hash_lead(int k) {
 // W = false: to_map (NV = NC values per sample, Projection.cxx:1807);
 // W = true: to_weights (the NC (NC + 1) / 2 upper-triangle products,
 // Projection.cxx:1850).
+template <int NV, int H>
+__device__ __forceinline__ void lds_table_clear(int32_t *hk, double *hv, int nt) {
+    for (int i = threadIdx.x; i < H; i += nt) {
+        hk[i] = kHashEmpty;
+#pragma unroll
+        for (int c = 0; c < NV; c++) hv[c * H + i] = 0.;
+    }
+}
+
+// One detector's sample t into the block's table (all lanes of the block
+// call it together for the same detector).
+template <int PROJ, int PK, int INTERP, int NC, bool W, int H, bool MOD>
+__device__ __forceinline__ void lds_scatter_det(const PtgIn &in, const Geo &g,
+                                                const MapView<double> &m, const float *resp,
+                                                const float *sig, const float *det_weights,
+                                                int *flags, int64_t d, int64_t t, const Q4 &qb,
+                                                bool valid, int32_t *hk, double *hv) {
+    constexpr int NP = INTERP ? 4 : 1;
+    constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
+    float det_wt = 1.f;
+    if (det_weights != nullptr) {
+        det_wt = det_weights[d];
+        if (det_wt == 0.f) return;
+    }
+    int32_t key[NP];
+    double v[NP][NV];
+#pragma unroll
+    for (int jp = 0; jp < NP; jp++) {
+        key[jp] = -1;
+#pragma unroll
+        for (int c = 0; c < NV; c++) v[jp][c] = 0.;
+    }
+    if (valid) {
+        const Q4 qd = load_q(in.dets + 4 * d);
+        double co[4];
+        get_coords<PROJ>(qb, qd, in.tabs, co);
+        float pf[3];
+        spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
+        float s = 0.f;
+        if constexpr (!W) s = sig[d * in.n_t + t];
+        Pt p[4];
+        const int np = get_pixels<PK, INTERP>(g, co, p);
+#pragma unroll
+        for (int jp = 0; jp < NP; jp++) {
+            if (jp >= np) break;
+            if constexpr (PK == PK_FLAT_TILED || PK == PK_HP_TILED) {
+                if (m.tiles[p[jp].i0] == nullptr) {
+                    raise_flag(flags, FLAG_TILE);
+                    continue;
+                }
+            }
+            key[jp] = hash_key<PK>(g, p[jp]);
+            const float w = p[jp].w;
+            if constexpr (W) {
+                int k = 0;
+#pragma unroll
+                for (int a = 0; a < NC; a++)
+#pragma unroll
+                    for (int b = a; b < NC; b++, k++)
+                        v[jp][k] = (double)(pf[a] * pf[b] * w * w * det_wt);
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; c++) v[jp][c] = (double)(s * pf[c] * w * det_wt);
+            }
+        }
+    }
+#pragma unroll
+    for (int jp = 0; jp < NP; jp++) {
+        const bool emit = wave_runs<NV>((int64_t)key[jp], v[jp]) && key[jp] >= 0;
+        if (emit && !hash_add<NV, H, MOD>(hk, hv, key[jp], v[jp])) {
+            int64_t plane;
+            double *e = hash_elem<PK>(g, m, key[jp], plane);
+#pragma unroll
+            for (int c = 0; c < NV; c++) {
+                const int lead = W ? hash_lead<NC>(c) : c;
+                unsafeAtomicAdd(e + lead * plane, v[jp][c]);
+            }
+        }
+    }
+}
+
+// Adds the table's occupied slots to the map, one FP64 atomic per value.
+template <int PK, int NC, bool W, int H>
+__device__ __forceinline__ void lds_table_flush(const Geo &g, const MapView<double> &m,
+                                                const int32_t *hk, const double *hv, int nt) {
+    constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
+    for (int i = threadIdx.x; i < H; i += nt) {
+        const int32_t key = hk[i];
+        if (key == kHashEmpty) continue;
+        int64_t plane;
+        double *e = hash_elem<PK>(g, m, key, plane);
+#pragma unroll
+        for (int c = 0; c < NV; c++) {
+            const int lead = W ? hash_lead<NC>(c) : c;
+            unsafeAtomicAdd(e + lead * plane, hv[c * H + i]);
+        }
+    }
+}
+
 template <int PROJ, int PK, int INTERP, int NC, bool W, int NT, int H, bool MOD>
 __global__ __launch_bounds__(NT) void k_scatter_lds(PtgIn in, Geo g, MapView<double> m,
                                                     const float *resp, const float *sig,
                                                     const float *det_weights, HashWork hw,
                                                     int *flags) {
-    constexpr int NP = INTERP ? 4 : 1;
     constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
     __shared__ int32_t hk[H];
     __shared__ double hv[NV * H];
-    for (int i = threadIdx.x; i < H; i += NT) {
-        hk[i] = kHashEmpty;
-#pragma unroll
-        for (int c = 0; c < NV; c++) hv[c * H + i] = 0.;
-    }
+    lds_table_clear<NV, H>(hk, hv, NT);
     __syncthreads();
     // 1-D grid, detector group fastest (as dense_block): the blocks over one
     // span of the boresight run together
@@ -588,78 +688,13 @@ __global__ __launch_bounds__(NT) void k_scatter_lds(PtgIn in, Geo g, MapView<dou
         const int64_t t = tb + threadIdx.x;
         const bool valid = t < t_end;
         const Q4 qb = valid ? load_q(in.bore + 4 * t) : Q4{1., 0., 0., 0.};
-        for (int j = 0; j < nd; j++) {
-            const int64_t d = hw.order[g0 + j];
-            float det_wt = 1.f;
-            if (det_weights != nullptr) {
-                det_wt = det_weights[d];
-                if (det_wt == 0.f) continue;
-            }
-            int32_t key[NP];
-            double v[NP][NV];
-#pragma unroll
-            for (int jp = 0; jp < NP; jp++) {
-                key[jp] = -1;
-#pragma unroll
-                for (int c = 0; c < NV; c++) v[jp][c] = 0.;
-            }
-            if (valid) {
-                const Q4 qd = load_q(in.dets + 4 * d);
-                double co[4];
-                get_coords<PROJ>(qb, qd, in.tabs, co);
-                float pf[3];
-                spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
-                float s = 0.f;
-                if constexpr (!W) s = sig[d * in.n_t + t];
-                Pt p[4];
-                const int np = get_pixels<PK, INTERP>(g, co, p);
-#pragma unroll
-                for (int jp = 0; jp < NP; jp++) {
-                    if (jp >= np) break;
-                    if constexpr (PK == PK_FLAT_TILED || PK == PK_HP_TILED) {
-                        if (m.tiles[p[jp].i0] == nullptr) {
-                            raise_flag(flags, FLAG_TILE);
-                            continue;
-                        }
-                    }
-                    key[jp] = hash_key<PK>(g, m.tile_dims, p[jp]);
-                    const float w = p[jp].w;
-                    if constexpr (W) {
-                        int k = 0;
-#pragma unroll
-                        for (int a = 0; a < NC; a++)
-#pragma unroll
-                            for (int b = a; b < NC; b++, k++)
-                                v[jp][k] = (double)(pf[a] * pf[b] * w * w * det_wt);
-                    } else {
-#pragma unroll
-                        for (int c = 0; c < NC; c++) v[jp][c] = (double)(s * pf[c] * w * det_wt);
-                    }
-                }
-            }
-#pragma unroll
-            for (int jp = 0; jp < NP; jp++) {
-                const bool emit = wave_runs<NV>((int64_t)key[jp], v[jp]) && key[jp] >= 0;
-                if (emit && !hash_add<NV, H, MOD>(hk, hv, key[jp], v[jp])) {
-#pragma unroll
-                    for (int c = 0; c < NV; c++) {
-                        const int lead = W ? hash_lead<NC>(c) : c;
-                        unsafeAtomicAdd(hash_elem<PK>(g, m, key[jp], lead), v[jp][c]);
-                    }
-                }
-            }
-        }
+        for (int j = 0; j < nd; j++)
+            lds_scatter_det<PROJ, PK, INTERP, NC, W, H, MOD>(in, g, m, resp, sig, det_weights,
+                                                             flags, hw.order[g0 + j], t, qb,
+                                                             valid, hk, hv);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < H; i += NT) {
-        const int32_t key = hk[i];
-        if (key == kHashEmpty) continue;
-#pragma unroll
-        for (int c = 0; c < NV; c++) {
-            const int lead = W ? hash_lead<NC>(c) : c;
-            unsafeAtomicAdd(hash_elem<PK>(g, m, key, lead), hv[c * H + i]);
-        }
-    }
+    lds_table_flush<PK, NC, W, H>(g, m, hk, hv, NT);
 }
 
 // --------------------------------------------------------------------------

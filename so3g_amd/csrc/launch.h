@@ -41,6 +41,7 @@ struct OpArgs {
     int hash_cls;
     HashWork hw;
     // batch ops
+    int batch_slots = 0;       // LDS table of batch to_map: 0 none, 512, 1024
     const ObsDev *obs;
     int n_obs;
     int64_t n_items;
@@ -92,9 +93,18 @@ int launch_map_ops(const OpArgs &a) {
         const dim3 bgrid((unsigned)a.grid_blocks);
         if (a.op == OP_BATCH_TO_MAP) {
             MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
-            hipLaunchKernelGGL((k_batch_to_map<PROJ, PK, INTERP, NC>), bgrid, block, 0,
-                               a.stream, a.obs, a.n_obs, a.n_items, a.in.dpb, a.in.tabs,
-                               a.in.agg, a.g, m, a.flags);
+            if (a.batch_slots == 512)
+                hipLaunchKernelGGL((k_batch_to_map_lds<PROJ, PK, INTERP, NC, 512>), bgrid,
+                                   block, 0, a.stream, a.obs, a.n_obs, a.n_items, a.in.dpb,
+                                   a.in.tabs, a.g, m, a.flags);
+            else if (a.batch_slots == 1024)
+                hipLaunchKernelGGL((k_batch_to_map_lds<PROJ, PK, INTERP, NC, 1024>), bgrid,
+                                   block, 0, a.stream, a.obs, a.n_obs, a.n_items, a.in.dpb,
+                                   a.in.tabs, a.g, m, a.flags);
+            else
+                hipLaunchKernelGGL((k_batch_to_map<PROJ, PK, INTERP, NC>), bgrid, block, 0,
+                                   a.stream, a.obs, a.n_obs, a.n_items, a.in.dpb, a.in.tabs,
+                                   a.in.agg, a.g, m, a.flags);
         } else {
             MapView<const double> m{(const double *)a.map, (const double *const *)a.tiles,
                                     a.tile_dims};

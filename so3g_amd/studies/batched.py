@@ -16,7 +16,9 @@ Timed (to_map and from_map each, the whole batch per pass):
   per_obs   the batch kernel launched once per observation, no host sync
   eager     one persistent launch over all observations
   graph     the same launch captured in a HIP graph and replayed
-after the autotuner has picked the launch shape of each op.
+after the autotuner has picked the launch shape of each op (detectors per
+work item, blocks per compute unit and, for to_map, global atomics or an
+LDS table of 512 or 1024 slots per work item).
 
 Run: python -m so3g_amd.studies.batched [--n-obs 64] [--seed 4] [--reps 3]
 """
@@ -114,6 +116,9 @@ def run(n_obs=64, seed=4, reps=3):
     tune_to = batch.tune('to_map', acc)
     out['tune_s'] = time.time() - t0
     out['tune'] = {'from_map': tune_from, 'to_map': tune_to}
+    # best to_map launch per LDS table size (0: global atomics)
+    out['to_map_best_by_table'] = {
+        str(k): min(r[3] for r in tune_to if r[2] == k) for k in sorted({r[2] for r in tune_to})}
     out['shape'] = batch.info()
     print(f'tuned in {out["tune_s"]:.1f} s: {out["shape"]}', file=sys.stderr, flush=True)
 

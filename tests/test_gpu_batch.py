@@ -127,12 +127,18 @@ def test_batch_matches_per_observation(gpu, tiled):
             om = oracle.to_map(og, om, it['bore'], it['dets'], it['resp'], it['signal'],
                                it['det_weights'])
     np.testing.assert_allclose(ref_h, flat(om), rtol=0, atol=1e-9)
-    for mode in ['eager', 'per_obs', 'graph', 'graph', 'graph']:
-        m = zero_like(sky)
-        batch.to_map(m, mode=mode)
-        torch.cuda.synchronize()
-        _ = torch.ones(1000, device=dev).sum().item()   # eager work between replays
-        np.testing.assert_allclose(flat(host(m)), ref_h, rtol=0, atol=1e-9, err_msg=mode)
+    # global atomics, then each work item summed in an LDS table of 512 or
+    # 1024 slots first
+    for slots in (0, 512, 1024):
+        batch.set_table(slots)
+        for mode in ['eager', 'per_obs', 'graph', 'graph', 'graph']:
+            m = zero_like(sky)
+            batch.to_map(m, mode=mode)
+            torch.cuda.synchronize()
+            _ = torch.ones(1000, device=dev).sum().item()   # eager work between replays
+            np.testing.assert_allclose(flat(host(m)), ref_h, rtol=0, atol=1e-9,
+                                       err_msg=f'{mode} {slots}')
+    batch.set_table(0)
     # Graph replays into the same map accumulate.
     m = zero_like(sky)
     batch.to_map(m, mode='graph')
@@ -148,17 +154,21 @@ def test_batch_tune(gpu):
     sky = dev_map(eng)
     batch = so3g_amd.ObservationBatch(eng, items)
     res = batch.tune('from_map', sky)
-    assert len(res) == 25 and all(r[2] > 0 for r in res)
-    best = min(res, key=lambda r: r[2])
+    assert len(res) == 25 and all(r[3] > 0 and r[2] == 0 for r in res)
+    best = min(res, key=lambda r: r[3])
     assert batch.info()['from_map_shape'] == best[:2]
     scratch = zero_like(sky)
     res = batch.tune('to_map', scratch)
-    best = min(res, key=lambda r: r[2])
+    # 25 launch shapes x (global atomics, 512- and 1024-slot LDS tables)
+    assert len(res) == 75 and sorted({r[2] for r in res}) == [0, 512, 1024]
+    best = min(res, key=lambda r: r[3])
     assert batch.info()['to_map_shape'] == best[:2]
+    assert batch.info()['to_map_slots'] == best[2]
     # every tuned shape gives the same result
     ref = None
-    for dpb, gm in [(1, 2), (16, 0), (4, 8)]:
+    for dpb, gm, slots in [(1, 2, 0), (16, 0, 512), (4, 8, 1024), (2, 0, 1024)]:
         batch.set_shape('to_map', dpb, gm)
+        batch.set_table(slots)
         m = zero_like(sky)
         batch.to_map(m, mode='eager')
         h = flat(host(m))
@@ -167,6 +177,8 @@ def test_batch_tune(gpu):
         np.testing.assert_allclose(h, ref, rtol=0, atol=1e-9)
     with pytest.raises(ValueError):
         batch.set_shape('to_map', 3, 2)
+    with pytest.raises(ValueError):
+        batch.set_table(256)
 
 
 @pytest.mark.gpu
