@@ -18,9 +18,10 @@ on-the-fly path, the small (100 x 40000) observation, the tiled CAR and
 HEALPix nside 4096 maps of the same observation, get_pixels /
 get_pointing_matrix / to_weights, N^-1 (nmat_detvecs_apply), G3SuperTimestream
 decoding, the precision study (100 x 40000 samples into ZEA / TAN 0.25 arcmin
-and HEALPix RING nside 8192) and the map-making solve (binned, two-level and
-learned-correction CG to 1e-6), and the CPU baseline (the oracle on the host
-cores).
+and HEALPix RING nside 8192), the batch of 64 uneven observations in one
+tuned launch (against one launch per observation), the map-making solve
+(binned, two-level, learned-deflation and U-Net CG to 1e-6), and the CPU
+baseline (the oracle on the host cores).
 
 Prints ONE JSON line.  `python bench.py [--steps K] [--warmup W] [--gpus 1]`.
 
@@ -103,7 +104,7 @@ def parse():
     ap.add_argument('--no-g3', action='store_true',
                     help='skip the G3SuperTimestream decoding leg')
     ap.add_argument('--no-studies', action='store_true',
-                    help='skip the precision (C3) and map-making (C5) legs')
+                    help='skip the precision (C3), batched (C4) and map-making (C5) legs')
     ap.add_argument('--cpu-full', action='store_true',
                     help='print the full CPU baseline table (oracle, median of 5) and exit')
     return ap.parse_args()
@@ -320,11 +321,12 @@ def main():
                                                       dets, resp, sig, dev, stream, run)
     other = None if args.no_small else c2_other_ops(torch, N, lib, g, bore, dets, resp, pix,
                                                     spin, plan, ny, nx, dev, stream, run)
-    c3 = c5 = None
+    c3 = c4 = c5 = None
     if not args.no_studies:
         del plan
         torch.cuda.empty_cache()
         c3 = c3_precision()
+        c4 = c4_batched()
         c5 = c5_mapmaking()
 
     out = {
@@ -391,6 +393,7 @@ def main():
         'c2_variants': variants,
         'c2_other_ops': other,
         'c3_precision': c3,
+        'c4_batched': c4,
         'c5_mapmaking': c5,
         'cpu_baseline': cpu,
     }
@@ -441,6 +444,32 @@ def c3_precision():
     return out
 
 
+def c4_batched():
+    """Batched observations at their configuration: 64 uneven observations
+    (seed 4, 8.9e9 det*samples) into one tiled CAR 0.5 arcmin TQU map; the
+    autotuner picks each op's launch shape (detectors per work item, blocks
+    per compute unit, and for to_map global atomics or an LDS table of 512 /
+    1024 slots per item), then each op is timed as engine calls per
+    observation, one batch-kernel launch per observation, one launch over
+    the batch, and a HIP-graph replay of that launch (median of 3)."""
+    from so3g_amd.studies import batched
+    r = batched.run(n_obs=64, seed=4, reps=3)
+    out = {k: r[k] for k in ('workload', 'det_samples', 'setup_s', 'tune_s',
+                             'to_map_best_by_table', 'from_map_bit_exact',
+                             'to_map_max_rel_diff')}
+    out['shape'] = r['shape']
+    out['timing_ms'] = {op: {m: v['s'] * 1e3 for m, v in modes.items()}
+                        for op, modes in r['timing'].items()}
+    out['det_samples_per_s'] = {op: {m: v['det_samples_per_s'] for m, v in modes.items()}
+                                for op, modes in r['timing'].items()}
+    for op, modes in r['timing'].items():
+        out[f'{op}_one_launch_vs_per_obs'] = modes['per_obs']['s'] / modes['eager']['s']
+        out[f'{op}_graph_vs_per_obs'] = modes['per_obs']['s'] / modes['graph']['s']
+        out[f'{op}_one_launch_vs_api'] = modes['api']['s'] / modes['eager']['s']
+    log(f'C4 batched: {out}')
+    return out
+
+
 def c5_mapmaking():
     """Map-making at its configuration (100 det x 40000 samples, CAR 1
     arcmin TQU, 1/f noise, Gaussian random sky): wall time of the solve to
@@ -450,10 +479,11 @@ def c5_mapmaking():
     to it when it stalls); setup, learning and training times listed apart."""
     from so3g_amd.studies import cg
     r = cg.run()
-    keep = ('iters', 'seconds', 'final_residual', 'fallback_at', 'map_rms_error_TQU',
+    keep = ('iters', 'seconds', 'final_residual', 'true_residual', 'fallback_at',
+            'map_rms_error_TQU', 'a_norm_error_vs_reference', 'map_rms_diff_vs_reference_TQU',
             'ms_per_iter', 'setup_s', 'train_s', 'pairs_s', 'train_steps', 'pairs',
             'loss_first_last', 'learn_s', 'harvest_iters', 'ritz_vectors', 'ritz_values_max')
-    out = {'workload': r['workload']}
+    out = {'workload': r['workload'], 'reference': r.get('reference')}
     for k in ('binned', 'two_level', 'deflated', 'learned'):
         if k in r:
             out[k] = {kk: v for kk, v in r[k].items() if kk in keep}
