@@ -258,6 +258,101 @@ __global__ __launch_bounds__(kBlock) void k_pc_to_map(PcIn in, PcMap pm, double 
     }
 }
 
+// Precomp to_map (W = false) / to_weight_map (W = true) of a non-tiled map
+// (K = 1 or 2) through an LDS table per block, as the on-the-fly scatter's
+// small tables: a block takes `dets` detectors (in `order`: detectors
+// grouped by the pixel they see at mid-observation, so that a block's
+// detectors sweep overlapping pixels) x 256 samples; runs of equal pixels
+// within a wave are summed first, each run's total is added to the table
+// (key = element offset, home slot = key mod H; a run that finds no slot
+// adds to the map directly) and the occupied slots are added to the map at
+// the end, one FP64 atomic per value.  The float32 products are those of
+// k_pc_to_map / k_pc_to_weights; the float64 sums are the same values added
+// in another order.
+template <int K, int NC, bool W, int H>
+__global__ __launch_bounds__(kBlock) void k_pc_scatter_lds(PcIn in, PcMap pm, double *map,
+                                                           const int32_t *pix,
+                                                           const float *spin, const float *sig,
+                                                           const float *det_weights,
+                                                           const int32_t *order, int32_t dets,
+                                                           int *flags) {
+    static_assert(K == 1 || K == 2, "non-tiled maps only");
+    constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
+    __shared__ int32_t hk[H];
+    __shared__ double hv[NV * H];
+    lds_table_clear<NV, H>(hk, hv, kBlock);
+    __syncthreads();
+    const int64_t ng = (in.n_det + dets - 1) / dets;
+    const int64_t tc = (int64_t)blockIdx.x / ng;
+    const int64_t g0 = ((int64_t)blockIdx.x - tc * ng) * dets;
+    const int nd = (int)min((int64_t)dets, in.n_det - g0);
+    const int64_t t = tc * kBlock + threadIdx.x;
+    const bool valid = t < in.n_t;
+    const int64_t plane = pm.ny * pm.nx;
+    for (int j = 0; j < nd; j++) {
+        const int64_t d = order[g0 + j];
+        float det_wt = 1.f;
+        if (det_weights != nullptr) {
+            det_wt = det_weights[d];
+            if (det_wt == 0.f) continue;
+        }
+        int32_t key = -1;
+        double v[NV];
+#pragma unroll
+        for (int c = 0; c < NV; c++) v[c] = 0.;
+        if (valid) {
+            const int64_t i = d * in.n_t + t;
+            int32_t pi[K];
+#pragma unroll
+            for (int q = 0; q < K; q++) pi[q] = pix[i * K + q];
+            if (pi[0] >= 0) {
+                double *e = nullptr;
+                int64_t pl;
+                const int st = pc_elem<K>(map, (double *const *)nullptr, pm, pi, &e, &pl);
+                if (st != PC_OK) {
+                    pc_flag(flags, st);
+                } else {
+                    key = (int32_t)(e - map);
+                    if constexpr (W) {
+                        float pf[NC];
+#pragma unroll
+                        for (int c = 0; c < NC; c++) pf[c] = spin[i * NC + c];
+                        int k = 0;
+#pragma unroll
+                        for (int a = 0; a < NC; a++)
+#pragma unroll
+                            for (int b = a; b < NC; b++, k++)
+                                v[k] = (double)(pf[a] * pf[b] * det_wt);
+                    } else {
+                        const float s = sig[i];
+#pragma unroll
+                        for (int c = 0; c < NC; c++)
+                            v[c] = (double)(det_wt * s * spin[i * NC + c]);
+                    }
+                }
+            }
+        }
+        const bool emit = wave_runs<NV>((int64_t)key, v) && key >= 0;
+        if (emit && !hash_add<NV, H, true>(hk, hv, key, v)) {
+#pragma unroll
+            for (int c = 0; c < NV; c++) {
+                const int lead = W ? hash_lead<NC>(c) : c;
+                unsafeAtomicAdd(map + lead * plane + key, v[c]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < H; i += kBlock) {
+        const int32_t key = hk[i];
+        if (key == kHashEmpty) continue;
+#pragma unroll
+        for (int c = 0; c < NV; c++) {
+            const int lead = W ? hash_lead<NC>(c) : c;
+            unsafeAtomicAdd(map + lead * plane + key, hv[c * H + i]);
+        }
+    }
+}
+
 // map[i] += sum over the n_copy XCD-private partial maps, fixed order.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_reduce_copies(T *map, const T *copies, int64_t n,

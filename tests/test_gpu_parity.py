@@ -449,6 +449,20 @@ def test_lds_scatter_overflow(gpu, proj, tiled, interp, mode):
             r = oracle.to_weights(c.og, c.zeros([3, 3]), c.bore, c.dets, c.resps,
                                   c.det_weights)
             maps_close(g, r, f'to_weights {proj} {dets} {span}')
+            if proj == 'CAR' and not tiled and not interp:
+                # Precomp on a non-tiled map: small tables in mode 2 (hash_dets
+                # detectors per block), global atomics in mode 3
+                import so3g_amd
+                pix, spin = [np.array(x) for x in
+                             c.engine.pointing_matrix(c.bore, c.dets, c.resps)]
+                pe = so3g_amd.ProjEng_Precomp_NonTiled()
+                g = pe.to_map(c.zeros([3]), pix, spin, c.signal, c.det_weights)
+                r = oracle.precomp_to_map(c.og, c.zeros([3]), pix, spin, c.signal,
+                                          c.det_weights)
+                maps_close(g, r, f'precomp to_map {dets}')
+                g = pe.to_weight_map(c.zeros([3, 3]), pix, spin, c.det_weights)
+                r = oracle.precomp_to_weights(c.og, c.zeros([3, 3]), pix, spin, c.det_weights)
+                maps_close(g, r, f'precomp to_weights {dets}')
     finally:
         for n, v in zip(names, old):
             _native.set_option(n, v)
