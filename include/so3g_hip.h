@@ -492,8 +492,9 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   partial maps: 0 never, 1 (default) where measured
  *                   faster than global atomics, 2 always with small tables
  *                   (256 threads, 512 slots), 3 always with large ones
- *                   (1024 threads, 112 KB).  Precomp to_map / to_weights of
- *                   non-tiled maps use the small tables in modes 1 and 2
+ *                   (1024 threads, 112 KB).  Precomp to_map / to_weights
+ *                   use the small tables in modes 1 and 2 (tiled maps:
+ *                   to_weights only in mode 1)
  *   "hash_dets", "hash_span"  detectors and samples per block of modes 2
  *                   and 3 (0, the default: 4 x 256 and 2 x 32768); also the
  *                   detectors per block of the Precomp tables (0: measured

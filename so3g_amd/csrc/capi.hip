@@ -79,7 +79,7 @@ struct State {
     int64_t order_nt = -1, order_ndet = -1;
     so3g_geom order_geom{};
     // The same for the Precomp LDS scatter, keyed on the pixel array: the
-    // order (n_det entries) then the middle samples' indices (2 n_det).
+    // order (n_det entries) then the middle samples' indices (k n_det, k <= 3).
     int32_t *pc_order_d = nullptr;
     int64_t pc_order_cap = 0;
     const int32_t *pc_order_pix = nullptr;
@@ -865,8 +865,9 @@ static int pc_to_map_launch(dim3 grid, hipStream_t s, const PcIn &in, const PcMa
 }
 
 // Detector order of the Precomp LDS scatter: detectors sorted by the pixel
-// they see at the middle sample (k = 2: row, then column; k = 1: pixel
-// number; off-map detectors last), cached for the pixel array.  The order
+// they see at the middle sample (k = 2: row, then column; k = 3: tile, row,
+// column; k = 1: pixel number; off-map detectors last), cached for the
+// pixel array.  The order
 // only decides which detectors share a block: any order gives the same sums,
 // so an edit of the array in place costs speed, never correctness.
 static int pc_order(const int32_t *pixel, int32_t k, int64_t n_det, int64_t n_t, hipStream_t s,
@@ -880,7 +881,7 @@ static int pc_order(const int32_t *pixel, int32_t k, int64_t n_det, int64_t n_t,
         if (g_state.pc_order_d) HIP_TRY(hipFree(g_state.pc_order_d));
         g_state.pc_order_d = nullptr;
         g_state.pc_order_cap = 0;
-        HIP_TRY(hipMalloc(&g_state.pc_order_d, 3 * n_det * sizeof(int32_t)));
+        HIP_TRY(hipMalloc(&g_state.pc_order_d, 4 * n_det * sizeof(int32_t)));
         g_state.pc_order_cap = n_det;
     }
     g_state.pc_order_pix = nullptr;
@@ -897,6 +898,9 @@ static int pc_order(const int32_t *pixel, int32_t k, int64_t n_det, int64_t n_t,
         const int32_t a = mid[d * k];
         if (a < 0) key[d] = INT64_MAX;
         else if (k == 2) key[d] = ((int64_t)a << 32) | (uint32_t)mid[d * k + 1];
+        else if (k == 3)
+            key[d] = ((int64_t)a << 40) | ((int64_t)(mid[d * k + 1] & 0xFFFFF) << 20) |
+                     (mid[d * k + 2] & 0xFFFFF);
         else key[d] = a;
     }
     std::vector<int32_t> idx(n_det);
@@ -914,44 +918,48 @@ static int pc_order(const int32_t *pixel, int32_t k, int64_t n_det, int64_t n_t,
 }
 
 // Whether a Precomp to_map / to_weights goes through the LDS tables: modes
-// 1 (auto) and 2 (small tables) for non-tiled maps whose elements have
-// int32 keys, without thread segments (tiled Precomp maps and segment lists
-// keep the run-aggregated global atomics).
-static bool pc_lds_applies(int32_t k, const PcMap &pm, const int32_t *segs) {
-    return (g_opt.scatter_hash == 1 || g_opt.scatter_hash == 2) && k <= 2 && segs == nullptr &&
-           pm.ny * pm.nx < INT32_MAX;
+// 1 (auto) and 2 (small tables), without thread segments (segment lists
+// keep the run-aggregated global atomics); non-tiled maps whose elements
+// have int32 keys, and tiled maps for to_weights (tiled to_map, 1000 x
+// 720000 samples into 100 x 100 tiles: 12.1 ms with global atomics, 12.6 ms
+// at best with the tables; to_weights 21.6 -> 16.8 ms).
+static bool pc_lds_applies(int32_t k, const PcMap &pm, const int32_t *segs, bool w) {
+    if (!(g_opt.scatter_hash == 1 || g_opt.scatter_hash == 2) || segs != nullptr) return false;
+    if (k == 3) return w || g_opt.scatter_hash == 2;
+    return pm.ny * pm.nx < INT32_MAX;
 }
 
 extern "C++" template <int K, int NC, bool W>
 static void pc_lds_launch(dim3 grid, hipStream_t s, const PcIn &in, const PcMap &pm,
-                          double *map, const int32_t *pixel, const float *spin,
-                          const float *signal, const float *det_weights,
+                          double *map, double *const *tiles, const int32_t *pixel,
+                          const float *spin, const float *signal, const float *det_weights,
                           const int32_t *order, int32_t dets) {
     hipLaunchKernelGGL((k_pc_scatter_lds<K, NC, W, 512>), grid, dim3(kBlock), 0, s, in, pm,
-                       map, pixel, spin, signal, det_weights, order, dets, g_state.flags_d);
+                       map, tiles, pixel, spin, signal, det_weights, order, dets,
+                       g_state.flags_d);
 }
 
 static int pc_lds_scatter(bool w, hipStream_t s, const PcIn &in, const PcMap &pm, int32_t k,
-                          int32_t nc, double *map, const int32_t *pixel, const float *spin,
-                          const float *signal, const float *det_weights) {
+                          int32_t nc, double *map, double *const *tiles, const int32_t *pixel,
+                          const float *spin, const float *signal, const float *det_weights) {
     const int32_t *order = nullptr;
     int r = pc_order(pixel, k, in.n_det, in.n_t, s, &order);
     if (r) return r;
     // detectors per block measured best on 1000 x 720000 samples (ms, global
     // atomics -> table): CAR 0.5' to_map 10.6 -> 6.3 (2 detectors; 6.7 with
     // 4), to_weights 20.1 -> 9.9 (4); HEALPix nside 4096 to_map 18.9 -> 10.0
-    // (4), to_weights 34.8 -> 17.5 (8)
+    // (4), to_weights 34.8 -> 17.5 (8); tiled CAR to_weights 21.6 -> 16.8 (8)
     int64_t dets = g_opt.hash_dets > 0 ? g_opt.hash_dets
-                                        : (w ? (k == 1 ? 8 : 4) : (k == 1 ? 4 : 2));
+                                        : (w ? (k == 2 ? 4 : 8) : (k == 1 ? 4 : 2));
     const int64_t n_tc = (in.n_t + kBlock - 1) / kBlock;
     while (n_tc * ((in.n_det + dets - 1) / dets) > INT32_MAX) dets *= 2;
     const dim3 grid((unsigned)(n_tc * ((in.n_det + dets - 1) / dets)));
     const int32_t dd = (int32_t)dets;
-#define PC_LDS(K, NC)                                                                        \
-    (w ? pc_lds_launch<K, NC, true>(grid, s, in, pm, map, pixel, spin, signal, det_weights,  \
-                                    order, dd)                                               \
-       : pc_lds_launch<K, NC, false>(grid, s, in, pm, map, pixel, spin, signal, det_weights, \
-                                     order, dd))
+#define PC_LDS(K, NC)                                                                   \
+    (w ? pc_lds_launch<K, NC, true>(grid, s, in, pm, map, tiles, pixel, spin, signal,   \
+                                    det_weights, order, dd)                             \
+       : pc_lds_launch<K, NC, false>(grid, s, in, pm, map, tiles, pixel, spin, signal,  \
+                                     det_weights, order, dd))
     switch (k * 4 + nc) {
     case 5: PC_LDS(1, 1); break;
     case 6: PC_LDS(1, 2); break;
@@ -959,6 +967,9 @@ static int pc_lds_scatter(bool w, hipStream_t s, const PcIn &in, const PcMap &pm
     case 9: PC_LDS(2, 1); break;
     case 10: PC_LDS(2, 2); break;
     case 11: PC_LDS(2, 3); break;
+    case 13: PC_LDS(3, 1); break;
+    case 14: PC_LDS(3, 2); break;
+    case 15: PC_LDS(3, 3); break;
     default: return fail(SO3G_ERR_RUNTIME, "unsupported pixel/spin layout");
     }
 #undef PC_LDS
@@ -1057,8 +1068,9 @@ int so3g_precomp_to_map(double *map, const int64_t *map_shape, double *const *ti
     }
     r = zero_flags(s);
     if (r) return r;
-    if (pc_lds_applies(k, pm, segs))
-        return pc_lds_scatter(false, s, in, pm, k, nc, map, pixel, spin, signal, det_weights);
+    if (pc_lds_applies(k, pm, segs, false))
+        return pc_lds_scatter(false, s, in, pm, k, nc, map, tiles, pixel, spin, signal,
+                              det_weights);
     {
         const dim3 grid = pc_grid(in, n_chunks);
         switch (k * 4 + nc) {
@@ -1098,8 +1110,9 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape, double *const
     }
     r = zero_flags(s);
     if (r) return r;
-    if (pc_lds_applies(k, pm, segs))
-        return pc_lds_scatter(true, s, in, pm, k, nc, map, pixel, spin, nullptr, det_weights);
+    if (pc_lds_applies(k, pm, segs, true))
+        return pc_lds_scatter(true, s, in, pm, k, nc, map, tiles, pixel, spin, nullptr,
+                              det_weights);
     PC_SWITCH(k, nc, k_pc_to_weights, pc_grid(in, n_chunks), dim3(kBlock), 0, s, in, pm,
               map, tiles, pixel, spin, det_weights, g_opt.scatter_agg, g_state.flags_d);
     HIP_TRY(hipGetLastError());

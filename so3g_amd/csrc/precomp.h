@@ -258,9 +258,26 @@ __global__ __launch_bounds__(kBlock) void k_pc_to_map(PcIn in, PcMap pm, double 
     }
 }
 
-// Precomp to_map (W = false) / to_weight_map (W = true) of a non-tiled map
-// (K = 1 or 2) through an LDS table per block, as the on-the-fly scatter's
-// small tables: a block takes `dets` detectors (in `order`: detectors
+// Element and plane of a Precomp table key (k_pc_scatter_lds).
+template <int K>
+__device__ __forceinline__ double *pc_key_elem(double *map, double *const *tiles,
+                                               const PcMap &pm, int32_t key, int64_t *plane) {
+    if constexpr (K == 3) {
+        const int32_t t = key >> 16;
+        *plane = (int64_t)pm.tile_dims[2 * t] * pm.tile_dims[2 * t + 1];
+        return tiles[t] + (key & 0xFFFF);
+    } else {
+        *plane = pm.ny * pm.nx;
+        return map + key;
+    }
+}
+
+// Precomp to_map (W = false) / to_weight_map (W = true) through an LDS
+// table per block, as the on-the-fly scatter's small tables.  Keys: the
+// element offset in the plane (K = 1, 2), or tile << 16 | offset in the
+// tile (K = 3; an element of a tile past 65536 pixels or of a tile numbered
+// from 32768 up is added to the map directly).  A block takes `dets`
+// detectors (in `order`: detectors
 // grouped by the pixel they see at mid-observation, so that a block's
 // detectors sweep overlapping pixels) x 256 samples; runs of equal pixels
 // within a wave are summed first, each run's total is added to the table
@@ -271,12 +288,12 @@ __global__ __launch_bounds__(kBlock) void k_pc_to_map(PcIn in, PcMap pm, double 
 // in another order.
 template <int K, int NC, bool W, int H>
 __global__ __launch_bounds__(kBlock) void k_pc_scatter_lds(PcIn in, PcMap pm, double *map,
+                                                           double *const *tiles,
                                                            const int32_t *pix,
                                                            const float *spin, const float *sig,
                                                            const float *det_weights,
                                                            const int32_t *order, int32_t dets,
                                                            int *flags) {
-    static_assert(K == 1 || K == 2, "non-tiled maps only");
     constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
     __shared__ int32_t hk[H];
     __shared__ double hv[NV * H];
@@ -288,7 +305,6 @@ __global__ __launch_bounds__(kBlock) void k_pc_scatter_lds(PcIn in, PcMap pm, do
     const int nd = (int)min((int64_t)dets, in.n_det - g0);
     const int64_t t = tc * kBlock + threadIdx.x;
     const bool valid = t < in.n_t;
-    const int64_t plane = pm.ny * pm.nx;
     for (int j = 0; j < nd; j++) {
         const int64_t d = order[g0 + j];
         float det_wt = 1.f;
@@ -307,12 +323,19 @@ __global__ __launch_bounds__(kBlock) void k_pc_scatter_lds(PcIn in, PcMap pm, do
             for (int q = 0; q < K; q++) pi[q] = pix[i * K + q];
             if (pi[0] >= 0) {
                 double *e = nullptr;
-                int64_t pl;
-                const int st = pc_elem<K>(map, (double *const *)nullptr, pm, pi, &e, &pl);
+                int64_t pl = 0;
+                const int st = pc_elem<K>(map, tiles, pm, pi, &e, &pl);
                 if (st != PC_OK) {
                     pc_flag(flags, st);
                 } else {
-                    key = (int32_t)(e - map);
+                    bool direct = false;
+                    if constexpr (K == 3) {
+                        const int64_t off = e - tiles[pi[0]];
+                        direct = off >= 65536 || pi[0] >= 32768;
+                        key = direct ? -1 : (int32_t)((pi[0] << 16) | (int32_t)off);
+                    } else {
+                        key = (int32_t)(e - map);
+                    }
                     if constexpr (W) {
                         float pf[NC];
 #pragma unroll
@@ -329,15 +352,25 @@ __global__ __launch_bounds__(kBlock) void k_pc_scatter_lds(PcIn in, PcMap pm, do
                         for (int c = 0; c < NC; c++)
                             v[c] = (double)(det_wt * s * spin[i * NC + c]);
                     }
+                    if (direct) {
+#pragma unroll
+                        for (int c = 0; c < NV; c++) {
+                            const int lead = W ? hash_lead<NC>(c) : c;
+                            unsafeAtomicAdd(e + lead * pl, v[c]);
+                            v[c] = 0.;
+                        }
+                    }
                 }
             }
         }
         const bool emit = wave_runs<NV>((int64_t)key, v) && key >= 0;
         if (emit && !hash_add<NV, H, true>(hk, hv, key, v)) {
+            int64_t pl;
+            double *e = pc_key_elem<K>(map, tiles, pm, key, &pl);
 #pragma unroll
             for (int c = 0; c < NV; c++) {
                 const int lead = W ? hash_lead<NC>(c) : c;
-                unsafeAtomicAdd(map + lead * plane + key, v[c]);
+                unsafeAtomicAdd(e + lead * pl, v[c]);
             }
         }
     }
@@ -345,10 +378,12 @@ __global__ __launch_bounds__(kBlock) void k_pc_scatter_lds(PcIn in, PcMap pm, do
     for (int i = threadIdx.x; i < H; i += kBlock) {
         const int32_t key = hk[i];
         if (key == kHashEmpty) continue;
+        int64_t pl;
+        double *e = pc_key_elem<K>(map, tiles, pm, key, &pl);
 #pragma unroll
         for (int c = 0; c < NV; c++) {
             const int lead = W ? hash_lead<NC>(c) : c;
-            unsafeAtomicAdd(map + lead * plane + key, hv[c * H + i]);
+            unsafeAtomicAdd(e + lead * pl, hv[c * H + i]);
         }
     }
 }

@@ -449,13 +449,14 @@ def test_lds_scatter_overflow(gpu, proj, tiled, interp, mode):
             r = oracle.to_weights(c.og, c.zeros([3, 3]), c.bore, c.dets, c.resps,
                                   c.det_weights)
             maps_close(g, r, f'to_weights {proj} {dets} {span}')
-            if proj == 'CAR' and not tiled and not interp:
-                # Precomp on a non-tiled map: small tables in mode 2 (hash_dets
-                # detectors per block), global atomics in mode 3
+            if proj == 'CAR' and not interp:
+                # Precomp: small tables in mode 2 (hash_dets detectors per
+                # block), global atomics in mode 3
                 import so3g_amd
                 pix, spin = [np.array(x) for x in
                              c.engine.pointing_matrix(c.bore, c.dets, c.resps)]
-                pe = so3g_amd.ProjEng_Precomp_NonTiled()
+                pe = (so3g_amd.ProjEng_Precomp_Tiled if tiled
+                      else so3g_amd.ProjEng_Precomp_NonTiled)()
                 g = pe.to_map(c.zeros([3]), pix, spin, c.signal, c.det_weights)
                 r = oracle.precomp_to_map(c.og, c.zeros([3]), pix, spin, c.signal,
                                           c.det_weights)
@@ -466,6 +467,41 @@ def test_lds_scatter_overflow(gpu, proj, tiled, interp, mode):
     finally:
         for n, v in zip(names, old):
             _native.set_option(n, v)
+
+
+@pytest.mark.parametrize('mode', [1, 2])
+def test_precomp_lds_large_tiles(gpu, mode):
+    """Precomp through the LDS tables on a tiled map whose 300 x 400 tiles
+    hold 120000 pixels: elements past the tables' 65536-pixel key range in
+    a tile are added to the map directly; the maps equal the oracle's (mode
+    1: to_weights through the tables, to_map with global atomics; mode 2:
+    both through the tables)."""
+    import so3g_amd
+    from so3g_amd import _native
+    old = _native.get_option('scatter_hash')
+    _native.set_option('scatter_hash', mode)
+    try:
+        _precomp_large_tiles(so3g_amd)
+    finally:
+        _native.set_option('scatter_hash', old)
+
+
+def _precomp_large_tiles(so3g_amd):
+    c = Case('CAR', comps='TQU', n_det=7, n_samp=12000, ny=300, nx=4000)
+    ny, nx = c.pixargs[0], c.pixargs[1]
+    n_tile = ((ny + 299) // 300) * ((nx + 399) // 400)
+    args = tuple(c.pixargs[:6]) + (300, 400, list(range(n_tile)))
+    eng = so3g_amd.ProjEng_CAR_TQU_Tiled(args)
+    og = oracle.geom_from_args('CAR', 'TQU', args, interp=0)
+    pix, spin = [np.array(x) for x in eng.pointing_matrix(c.bore, c.dets, c.resps)]
+    assert (pix[..., 1] * 400 + pix[..., 2]).max() >= 65536     # some keys overflow
+    pe = so3g_amd.ProjEng_Precomp_Tiled()
+    g = pe.to_map(eng.zeros((3,)), pix, spin, c.signal, c.det_weights)
+    r = oracle.precomp_to_map(og, eng.zeros((3,)), pix, spin, c.signal, c.det_weights)
+    maps_close(g, r, 'precomp to_map, large tiles')
+    g = pe.to_weight_map(eng.zeros((3, 3)), pix, spin, c.det_weights)
+    r = oracle.precomp_to_weights(og, eng.zeros((3, 3)), pix, spin, c.det_weights)
+    maps_close(g, r, 'precomp to_weights, large tiles')
 
 
 def tiled_plan_counts(pix, t_block, n_tiles_of, item):
