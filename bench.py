@@ -73,6 +73,16 @@ def plan_bytes(pinfo, n_s):
     described in main) and the to_map kernel's name."""
     from so3g_amd import _native as N
     n_on, ns, n_vis = pinfo['n_on'], pinfo['stored_spin'], pinfo['n_vis']
+    if pinfo['layout'] == 2:
+        # windowed layout: per sample a u16 window slot + the stored factors
+        # + the float32 signal (written by from_map, read by to_map); per
+        # window slot its compact -> flat pixel (4 B) and the float64 TQU
+        # values, read once by from_map, read and written once by to_map's
+        # flush
+        slots = pinfo['window_slots']
+        b_from = n_s * (2 + 4 * ns + 4) + slots * (4 + 8 * 3)
+        b_to = n_s * (2 + 4 * ns + 4) + slots * (4 + 2 * 8 * 3)
+        return b_from, b_to, 'k_tw_to_map<3,true,float,true,1024,true>'
     # u16 pixel offsets plus one int32 base per 512 samples; the wide groups
     # read int32 pixels (and their group number) instead
     b_from = n_s * (2 + 4 * ns + 4) + 4 * ((n_s + 511) // 512) + \
@@ -554,9 +564,10 @@ def small_config(torch, so3g_amd, N, workloads, dev, stream, reps=20):
     flush_b = torch.empty_like(flush_a)
 
     def pair():
+        # from_map into a fresh signal, to_map into a fresh map (the plan
+        # writes the whole map: to_map_set)
         N.check(lib.so3g_pc_plan_from_map(plan._h, sky.data_ptr(), sig.data_ptr(), 0, stream))
-        m.zero_()
-        N.check(lib.so3g_pc_plan_to_map(plan._h, m.data_ptr(), sig.data_ptr(), None, stream))
+        N.check(lib.so3g_pc_plan_to_map_set(plan._h, m.data_ptr(), sig.data_ptr(), None, stream))
 
     def timed(cold):
         ts = []
@@ -582,7 +593,8 @@ def small_config(torch, so3g_amd, N, workloads, dev, stream, reps=20):
     b = bf + bt
     b_pc = n_s * 24 * 2 + 3 * 8 * npix * 3
     out = {'workload': f'{n_det} det x {n_t} samples, CAR 1 arcmin TQU {ny}x{nx}, planned '
-                       'from_map + to_map',
+                       'from_map (fresh signal) + to_map (fresh map)',
+           'plan_layout': info['layout'], 'plan_items': info['items'],
            'ms_pair_warm': warm, 'ms_pair_cold': cold,
            'det_samples_per_s_warm': n_s / (warm * 1e-3),
            'det_samples_per_s_cold': n_s / (cold * 1e-3),

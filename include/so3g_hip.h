@@ -238,7 +238,15 @@ int so3g_precomp_to_weights(double *map, const int64_t *map_shape,
  * 2048 consecutive HEALPix pixel numbers), keeping sample order, in the tiled layout
  * (option "plan_layout" 1, the default from 2^25 samples up; one time
  * block); by pixel in layout 0
- * (64 MiB blocks, sized to stay in the Infinity Cache).  Stored: per on-map
+ * (64 MiB blocks, sized to stay in the Infinity Cache).  Layout 2 (the
+ * default below 2^25 samples when it applies, else layout 0) keeps the
+ * samples in time order, in work items of one detector's consecutive
+ * samples (up to "plan_win_len") whose compact pixels span at most 2048
+ * values: from_map gathers from the item's window of the map held in LDS,
+ * to_map sums into it and a second pass adds the items' sums per pixel (no
+ * atomics, a fixed summation order); it is refused (layout 2 asked for) or
+ * skipped (automatic choice) when some 512 consecutive samples of a detector
+ * visit pixels that span more than 2048 compact values.  Stored: per on-map
  * sample the spin factors (the T factor once per detector when it is
  * constant over the detector, as in so3g's pointing matrices) and the tile
  * slot (tiled) or the sample index and run-start bits (layout 0); per
@@ -255,6 +263,12 @@ int so3g_pc_plan_create(const int32_t *pixel, int32_t k, const float *spin, int3
                         int64_t t_block_mb, so3g_pc_plan **out, void *stream);
 int so3g_pc_plan_to_map(const so3g_pc_plan *plan, double *map, const float *signal,
                         const float *det_weights, void *stream);
+/* map = P^T signal: so3g_pc_plan_to_map into a map that is first set to 0,
+ * as ProjEng_Precomp.to_map does when called with map=None
+ * (Projection.cxx:2094-2099 allocates a zero map); a layout-2 plan writes
+ * every pixel in its second pass, the other layouts zero the map first. */
+int so3g_pc_plan_to_map_set(const so3g_pc_plan *plan, double *map, const float *signal,
+                            const float *det_weights, void *stream);
 /* Float64-signal variants for map-making solvers (no counterpart in the
  * reference, whose signals are float32): products and sums in float64, so
  * that P^T N^-1 P stays symmetric to float64 rounding when N^-1 has a large
@@ -287,18 +301,20 @@ int so3g_pc_plan_from_map_tiles(const so3g_pc_plan *plan, const double *const *t
  * plan: map (nc, nc, ...) += sum of (pf_a * pf_b) * det_wt (float32) for
  * b >= a, the upper triangle as the reference fills it; summed per work item
  * in LDS (float64, another order than the reference's sequential sums).
- * Layout 0 plans are refused. */
+ * Tiled (1) and windowed (2) layouts; layout 0 plans are refused. */
 int so3g_pc_plan_to_weights(const so3g_pc_plan *plan, double *map, const float *det_weights,
                             void *stream);
 int so3g_pc_plan_to_weights_tiles(const so3g_pc_plan *plan, double *const *tiles,
                                   const float *det_weights, void *stream);
 int so3g_pc_plan_info(const so3g_pc_plan *plan, int64_t *n_on, int64_t *bytes);
-/* stats[11]: n_on, pixel runs (layout 0), T factor constant per detector
+/* stats[13]: n_on, pixel runs (layout 0), T factor constant per detector
  * (0/1), stored spin factors per sample, samples per time block, device
  * bytes, visited pixels, layout, stretch starts (counting each 64-position
- * chunk's first position) and work items (tiled layout), and the 512-sample
- * groups of the time-ordered copy whose compact pixels span more than 65535
- * values (stored as int32 instead of u16 offsets) */
+ * chunk's first position; tiled layout), work items (tiled and windowed
+ * layouts), the 512-sample groups of the time-ordered copy whose compact
+ * pixels span more than 65535 values (stored as int32 instead of u16
+ * offsets), and for the windowed layout its largest window and the windows'
+ * total size (partial sums per pass) */
 int so3g_pc_plan_stats(const so3g_pc_plan *plan, int64_t *stats);
 int so3g_pc_plan_destroy(so3g_pc_plan *plan);
 
@@ -510,8 +526,17 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *   "nmat_dchunk"   detectors per block of nmat_detvecs_apply's passes
  *                   (default 64, 1..65536)
  *   "plan_layout"   layout of plans created afterwards: 1 map tiles summed
- *                   in LDS, 0 pixel runs, -1 (default) tiles from 2^25
- *                   samples up
+ *                   in LDS, 0 pixel runs, 2 time-ordered windows, -1
+ *                   (default) tiles from 2^25 samples up, below that windows
+ *                   where they fit, else pixel runs
+ *   "plan_win_len"  samples per work item of the windowed layout at most
+ *                   (default 8192, 512..2^30)
+ *   "plan_win_block"  threads per work item of the windowed layout (1024
+ *                   default, or 256)
+ *   "plan_win_flush"  windowed to_map: 1 (default) float64 atomics from
+ *                   each window into the map; 0 window sums to a partial
+ *                   buffer and a second pass per pixel (no atomics, a fixed
+ *                   summation order, slower)
  *   "plan_tile_item"  sorted positions per work item of the tiled layout
  *                   (default 262144, 64..2^30)
  *   "plan_tile_rows", "plan_tile_cols"  (read only) the tiled plan's tile

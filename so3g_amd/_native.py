@@ -72,6 +72,7 @@ _SIGS = {
     'so3g_pc_plan_create': (c_int, [c_vp, c_i32, c_vp, c_i32, c_i64, c_i64, c_vp, c_i64,
                                     ctypes.POINTER(c_vp), c_vp]),
     'so3g_pc_plan_to_map': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    'so3g_pc_plan_to_map_set': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     'so3g_pc_plan_to_map_f64': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     'so3g_pc_plan_from_map': (c_int, [c_vp, c_vp, c_vp, c_int, c_vp]),
     'so3g_pc_plan_stats': (c_int, [c_vp, c_vp]),

@@ -23,6 +23,9 @@ extern int g_plan_to_lds;
 extern int g_plan_to_cpb;
 extern int g_plan_layout;
 extern int64_t g_plan_tile_item;
+extern int64_t g_plan_win_len;
+extern int g_plan_win_block;
+extern int g_plan_win_flush;
 extern int g_plan_time_copy;
 extern int g_nmat_dchunk;
 extern const int g_plan_tile_rows, g_plan_tile_cols;
@@ -999,8 +1002,8 @@ int so3g_set_option(const char *name, int64_t value) {
     else if (n == "plan_from_layout") so3g::g_plan_from_layout = (int)(value != 0);
     else if (n == "plan_to_lds") so3g::g_plan_to_lds = (int)(value != 0);
     else if (n == "plan_layout") {
-        if (value < -1 || value > 1)
-            return fail(SO3G_ERR_VALUE, "plan_layout must be -1 (auto), 0 or 1");
+        if (value < -1 || value > 2)
+            return fail(SO3G_ERR_VALUE, "plan_layout must be -1 (auto), 0, 1 or 2");
         so3g::g_plan_layout = (int)value;
     }
     else if (n == "plan_time_copy") so3g::g_plan_time_copy = (int)(value != 0);
@@ -1012,6 +1015,17 @@ int so3g_set_option(const char *name, int64_t value) {
         if (value < 64 || value > (1 << 30))
             return fail(SO3G_ERR_VALUE, "plan_tile_item must be 64..2^30");
         so3g::g_plan_tile_item = value;
+    }
+    else if (n == "plan_win_block") {
+        if (value != 256 && value != 1024)
+            return fail(SO3G_ERR_VALUE, "plan_win_block must be 256 or 1024");
+        so3g::g_plan_win_block = (int)value;
+    }
+    else if (n == "plan_win_flush") so3g::g_plan_win_flush = (int)(value != 0);
+    else if (n == "plan_win_len") {
+        if (value < 512 || value > (1 << 30))
+            return fail(SO3G_ERR_VALUE, "plan_win_len must be 512..2^30");
+        so3g::g_plan_win_len = value;
     }
     else if (n == "plan_to_cpb") {
         if (value < 1 || value > 64) return fail(SO3G_ERR_VALUE, "plan_to_cpb must be 1..64");
@@ -1039,6 +1053,9 @@ int so3g_get_option(const char *name, int64_t *value) {
     else if (n == "plan_to_cpb") *value = so3g::g_plan_to_cpb;
     else if (n == "plan_layout") *value = so3g::g_plan_layout;
     else if (n == "plan_tile_item") *value = so3g::g_plan_tile_item;
+    else if (n == "plan_win_len") *value = so3g::g_plan_win_len;
+    else if (n == "plan_win_block") *value = so3g::g_plan_win_block;
+    else if (n == "plan_win_flush") *value = so3g::g_plan_win_flush;
     else if (n == "plan_time_copy") *value = so3g::g_plan_time_copy;
     else if (n == "nmat_dchunk") *value = so3g::g_nmat_dchunk;
     else if (n == "plan_tile_rows") *value = so3g::g_plan_tile_rows;

@@ -116,6 +116,19 @@ struct so3g_pc_plan {
     int32_t *comp_tile;    // [n_vis] map tile of each visited pixel (comp2pix: offset in it)
     double **tiles_d;      // [n_tile_map] tile pointers of the current call
     std::vector<uint8_t> tile_used;   // map tiles the plan's samples touch
+    // Windowed layout (layout 2): samples in time order (tspin holds their
+    // stored factors), work items of one detector's consecutive samples.
+    uint16_t *wslot;       // [n] offset in the item's window, kTOff off the map
+    void *witems;          // [n_witems] WItem
+    int64_t n_witems;
+    int32_t win_max;       // largest window
+    int64_t n_wslots;      // windows summed: partial slots
+    double *wpart;         // [n_wslots * max(nc, nc (nc + 1) / 2)] per-item sums
+    uint32_t *wcsr;        // [n_vis + 1] first entry of each visited pixel
+    uint32_t *wse;         // [n_wslots] entry of each item slot (partials in entry order)
+    uint32_t *wbitmap;     // [npix / 32] visited pixels (overwriting to_map)
+    uint32_t *wwscan;      // [npix / 32] visited pixels before each word
+    int32_t wvec;          // n_t % 8 == 0: vector loads per 8 samples
 };
 
 namespace so3g {
@@ -143,13 +156,23 @@ int g_plan_from_layout = 1;
 int g_plan_to_lds = 1;
 int g_plan_to_cpb = 2;
 // Plan layout chosen at creation ("plan_layout"): 0 pixel-sorted runs, 1
-// map tiles in LDS, -1 (default) tiles from 2^25 samples up (below that a
-// plan has too few work items to fill the GPU: 56.8 vs 48.6 us for the
-// to_map + from_map pair of 4e6 samples); and the tiled layout's positions
-// per work item.
+// map tiles in LDS, 2 time-ordered windows, -1 (default) tiles from 2^25
+// samples up (below that a plan has too few tile work items to fill the
+// GPU: 56.8 vs 48.6 us for the to_map + from_map pair of 4e6 samples with
+// pixel runs), and below it windows where every window fits, else pixel
+// runs; the tiled layout's positions per work item; the windowed layout's
+// samples per work item ("plan_win_len").
 int g_plan_layout = -1;
 constexpr int64_t kPlanTiledMin = (int64_t)1 << 25;
 int64_t g_plan_tile_item = 262144;
+int64_t g_plan_win_len = 8192;
+// Windowed layout: threads per work item ("plan_win_block", 256 or 1024) and
+// how to_map adds the window sums ("plan_win_flush": 1 float64 atomics into
+// the map, 0 partial sums and a second pass per pixel -- no atomics and a
+// fixed summation order, but the second pass is latency-bound: 42.8 vs
+// 54.8 us for the cold from_map + overwriting to_map pair of 4e6 samples).
+int g_plan_win_block = 1024;
+int g_plan_win_flush = 1;
 // Tiled layout: also keep the time-ordered copy for from_map ("plan_time_copy",
 // 1: 2.18 ms gather vs 2.54 ms over the tiles; 0: a smaller plan).
 int g_plan_time_copy = 1;
@@ -1461,6 +1484,464 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_zero(const uint32_t *off, int
         sig[off[q]] = 0.f;
 }
 
+// ---------------------------------------------------------------------------
+// Windowed layout.  The samples stay in time order.  Each work item is a
+// stretch of one detector's consecutive samples whose compact pixels span at
+// most kWinMax values (the item's window); per sample the plan keeps the u16
+// offset into the window (kTOff off the map) and the stored factors.  For a
+// scan whose detectors sweep one strip of the map (a constant-elevation scan
+// in horizon coordinates puts each detector on one map row) a window covers
+// the pixels of the sweep, and:
+//   from_map  loads the item's window of the map into LDS and gathers from it;
+//   to_map    sums the item's samples into its window in LDS and writes the
+//             window's sums to a per-item partial buffer (plain stores), and a
+//             second kernel adds, per visited pixel, the partial sums of the
+//             items whose windows hold it (a fixed order: items in sample
+//             order) -- no global atomics, and the second kernel can write
+//             every map pixel, so an overwriting to_map needs no zeroing pass.
+// Items are formed on the host from per-512-sample window bounds: sub-chunks
+// of a detector are merged while the window fits and the item stays under
+// its length limit.  The layout is used only when every sub-chunk fits.
+// ---------------------------------------------------------------------------
+constexpr int kWSub = 512;       // samples per sub-chunk of the window bounds
+constexpr int kWinMax = 2048;    // largest window (LDS: 48 KB of TQU sums)
+
+struct WItem {
+    int64_t b;       // first sample (d * n_t + t)
+    int32_t n;       // samples
+    int32_t det;     // detector
+    int32_t lo;      // first compact pixel of the window
+    int32_t win;     // window size (0: no sample on the map)
+    int64_t pbase;   // first partial slot of the item
+};
+
+// Smallest and largest compact pixel of each sub-chunk of kWSub samples of a
+// detector (hi = -1 when no sample is on the map); one wave per sub-chunk.
+__global__ __launch_bounds__(kPlanBlock) void k_tw_bounds(const int32_t *pix, PixGeom pg,
+                                                          int64_t n_t, int64_t n_sub_det,
+                                                          int64_t n_sub, int64_t ny, int64_t nx,
+                                                          const uint32_t *bitmap,
+                                                          const uint32_t *wscan, int32_t *lo,
+                                                          int32_t *hi) {
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * (kPlanBlock / 64) + (threadIdx.x >> 6);
+    if (q >= n_sub) return;   // uniform per wave
+    const int64_t d = q / n_sub_det, t0 = (q % n_sub_det) * kWSub;
+    const int64_t t1 = min(n_t, t0 + (int64_t)kWSub);
+    int32_t mn = INT32_MAX, mx = -1;
+    for (int64_t t = t0 + lane; t < t1; t += 64) {
+        const int64_t p = flat_pixel(pix, pg, d * n_t + t, ny, nx);
+        if (p >= 0) {
+            const int32_t c = (int32_t)compact_of(p, bitmap, wscan);
+            mn = min(mn, c);
+            mx = max(mx, c);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = min(mn, __shfl_xor(mn, o));
+        mx = max(mx, __shfl_xor(mx, o));
+    }
+    if (lane == 0) { lo[q] = mn; hi[q] = mx; }
+}
+
+// Per sample: its window offset (kTOff off the map) and its stored factors
+// (spin components c0 .. nc-1), in time order.  sub_item: the item of each
+// sub-chunk; item_lo: each item's first compact pixel.
+__global__ __launch_bounds__(kPlanBlock) void k_tw_fill(const int32_t *pix, PixGeom pg,
+                                                        const float *spin, int nc, int c0,
+                                                        int64_t n, int64_t n_t,
+                                                        int64_t n_sub_det, int64_t ny,
+                                                        int64_t nx, const uint32_t *bitmap,
+                                                        const uint32_t *wscan,
+                                                        const int32_t *sub_item,
+                                                        const int32_t *item_lo, uint16_t *wslot,
+                                                        float *tspin, unsigned long long *n_on) {
+    const int ns = nc - c0;
+    uint32_t on = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kPlanBlock) {
+        const int64_t d = i / n_t, t = i - d * n_t;
+        const int64_t p = flat_pixel(pix, pg, i, ny, nx);
+        uint16_t sl = kTOff;
+        if (p >= 0) {
+            sl = (uint16_t)((int32_t)compact_of(p, bitmap, wscan) -
+                            item_lo[sub_item[d * n_sub_det + t / kWSub]]);
+            on++;
+        }
+        wslot[i] = sl;
+        for (int c = 0; c < ns; c++) tspin[i * ns + c] = spin[i * nc + c0 + c];
+    }
+    // on-map samples: summed over the block, one atomic per block
+    __shared__ uint32_t s_on;
+    if (threadIdx.x == 0) s_on = 0;
+    __syncthreads();
+    if (on) atomicAdd(&s_on, on);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_on) atomicAdd(n_on, (unsigned long long)s_on);
+}
+
+// The R = 8 consecutive samples of one lane starting at sample i0 (of an
+// item ending at sample e): window slots, stored factors and, for to_map and
+// accumulating from_map, the signal.  VEC: i0 % 8 == 0 (n_t % 8 == 0), so
+// whole groups load as vectors.
+template <int NS, bool VEC, typename ST>
+__device__ __forceinline__ void tw_load(int64_t i0, int64_t e, const uint16_t *__restrict__ wslot,
+                                        const float *__restrict__ tspin,
+                                        const ST *__restrict__ sig, bool with_sig,
+                                        uint32_t (&sl)[8], float (&sp)[8][NS > 0 ? NS : 1],
+                                        ST (&sg)[8]) {
+    if (VEC && i0 + 8 <= e) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(wslot + i0);
+        const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) { sl[2 * q] = w[q] & 0xFFFFu; sl[2 * q + 1] = w[q] >> 16; }
+        if constexpr (NS > 0) {
+#pragma unroll
+            for (int q = 0; q < 2 * NS; q++) {
+                const float4 f = reinterpret_cast<const float4 *>(tspin + i0 * NS)[q];
+                const float v[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) sp[(4 * q + k) / NS][(4 * q + k) % NS] = v[k];
+            }
+        }
+        if (with_sig) {
+            if constexpr (std::is_same<ST, float>::value) {
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const float4 s = reinterpret_cast<const float4 *>(sig + i0)[q];
+                    sg[4 * q] = s.x; sg[4 * q + 1] = s.y; sg[4 * q + 2] = s.z; sg[4 * q + 3] = s.w;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const double2 s = reinterpret_cast<const double2 *>(sig + i0)[q];
+                    sg[2 * q] = s.x; sg[2 * q + 1] = s.y;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; r++) sg[r] = (ST)0;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int64_t i = i0 + r;
+            const bool ok = i < e;
+            sl[r] = ok ? (uint32_t)wslot[i] : (uint32_t)kTOff;
+#pragma unroll
+            for (int c = 0; c < NS; c++) sp[r][c] = ok ? tspin[i * NS + c] : 0.f;
+            sg[r] = (ok && with_sig) ? sig[i] : (ST)0;
+        }
+    }
+}
+
+// Windowed to_map, first pass: one block of BS threads per item.  Each lane
+// sums its runs of equal slots among its 8 consecutive samples and adds each
+// run to the window in LDS (float64 LDS atomics).  The window's sums then go
+// to the partial buffer in entry order (wse: slot -> entry; ATOM = false) or
+// are added to the map with one float64 atomic per touched slot and
+// component (ATOM = true).  Arithmetic per sample as ProjEng_Precomp's
+// to_map (Projection.cxx:2100-2121): (w * sig) * pf in float32 (float64 for
+// a float64 signal), summed in float64; a zero detector weight skips the
+// detector.
+template <int NC, bool TC, typename ST, bool VEC, int BS, bool ATOM>
+__global__ __launch_bounds__(1024) void k_tw_to_map(const WItem *__restrict__ items,
+                                                    const uint16_t *__restrict__ wslot,
+                                                    const float *__restrict__ tspin,
+                                                    const float *__restrict__ tval,
+                                                    const ST *__restrict__ sig,
+                                                    const float *__restrict__ det_weights,
+                                                    const uint32_t *__restrict__ wse,
+                                                    double *__restrict__ part,
+                                                    const int32_t *__restrict__ comp2pix,
+                                                    int64_t plane, double *map) {
+    constexpr int NS = TC ? NC - 1 : NC;
+    extern __shared__ double wacc[];   // [NC][win]
+    const WItem it = items[blockIdx.x];
+    const int win = it.win;
+    const float w = det_weights != nullptr ? det_weights[it.det] : 1.f;
+    const float pt = TC ? tval[it.det] : 0.f;
+    const bool work = win > 0 && w != 0.f;   // a zero weight skips the detector (Projection.cxx:2104)
+    const int64_t e = it.b + it.n;
+    int64_t i0 = it.b + (int64_t)threadIdx.x * 8;
+    uint32_t sl[8];
+    float sp[8][NS > 0 ? NS : 1];
+    ST sg[8];
+    // the first 8 samples' loads are issued before the window is zeroed
+    if (work && i0 < e) tw_load<NS, VEC, ST>(i0, e, wslot, tspin, sig, true, sl, sp, sg);
+    for (int q = threadIdx.x; q < NC * win; q += BS) wacc[q] = 0.;
+    __syncthreads();
+    if (work) {
+        for (bool first = true; i0 < e; i0 += BS * 8, first = false) {
+            if (!first) tw_load<NS, VEC, ST>(i0, e, wslot, tspin, sig, true, sl, sp, sg);
+            uint32_t cur = kTOff;
+            double cv[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c++) cv[c] = 0.;
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                if (sl[r] == kTOff) continue;
+                float pf[NC];
+                if constexpr (TC) {
+                    pf[0] = pt;
+#pragma unroll
+                    for (int c = 1; c < NC; c++) pf[c] = sp[r][c - 1];
+                } else {
+#pragma unroll
+                    for (int c = 0; c < NC; c++) pf[c] = sp[r][c];
+                }
+                double x[NC];
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    if constexpr (std::is_same<ST, double>::value)
+                        x[c] = (double)w * sg[r] * (double)pf[c];
+                    else
+                        x[c] = (double)(w * sg[r] * pf[c]);
+                }
+                if (sl[r] != cur) {
+                    if (cur != kTOff) {
+#pragma unroll
+                        for (int c = 0; c < NC; c++) atomicAdd(wacc + c * win + cur, cv[c]);
+                    }
+                    cur = sl[r];
+#pragma unroll
+                    for (int c = 0; c < NC; c++) cv[c] = x[c];
+                } else {
+#pragma unroll
+                    for (int c = 0; c < NC; c++) cv[c] += x[c];
+                }
+            }
+            if (cur != kTOff) {
+#pragma unroll
+                for (int c = 0; c < NC; c++) atomicAdd(wacc + c * win + cur, cv[c]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < win; q += BS) {
+        if constexpr (ATOM) {
+            const int64_t p = comp2pix[it.lo + q];
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const double v = wacc[c * win + q];
+                if (v != 0.) unsafeAtomicAdd(map + c * plane + p, v);
+            }
+        } else {
+            double *pp = part + (int64_t)wse[it.pbase + q] * NC;
+#pragma unroll
+            for (int c = 0; c < NC; c++) pp[c] = wacc[c * win + q];
+        }
+    }
+}
+
+// Windowed to_weights, first pass: as k_tw_to_map with the NC (NC + 1) / 2
+// upper-triangle products (pf_a * pf_b) * w in float32 per sample
+// (Projection.cxx:2135-2154); partial sums in entry order.
+template <int NC, bool TC, bool VEC, int BS>
+__global__ __launch_bounds__(1024) void k_tw_to_weights(
+    const WItem *__restrict__ items, const uint16_t *__restrict__ wslot,
+    const float *__restrict__ tspin, const float *__restrict__ tval,
+    const float *__restrict__ det_weights, const uint32_t *__restrict__ wse,
+    double *__restrict__ part) {
+    constexpr int NS = TC ? NC - 1 : NC;
+    constexpr int NW = NC * (NC + 1) / 2;
+    extern __shared__ double wacc[];   // [NW][win]
+    const WItem it = items[blockIdx.x];
+    const int win = it.win;
+    for (int q = threadIdx.x; q < NW * win; q += BS) wacc[q] = 0.;
+    __syncthreads();
+    const float w = det_weights != nullptr ? det_weights[it.det] : 1.f;
+    const float pt = TC ? tval[it.det] : 0.f;
+    if (win > 0 && w != 0.f) {   // a zero weight skips the detector (Projection.cxx:2139)
+        const int64_t e = it.b + it.n;
+        for (int64_t i0 = it.b + (int64_t)threadIdx.x * 8; i0 < e; i0 += BS * 8) {
+            uint32_t sl[8];
+            float sp[8][NS > 0 ? NS : 1];
+            float sg[8];
+            tw_load<NS, VEC, float>(i0, e, wslot, tspin, nullptr, false, sl, sp, sg);
+            uint32_t cur = kTOff;
+            double cv[NW];
+#pragma unroll
+            for (int k = 0; k < NW; k++) cv[k] = 0.;
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                if (sl[r] == kTOff) continue;
+                float pf[NC];
+                if constexpr (TC) {
+                    pf[0] = pt;
+#pragma unroll
+                    for (int c = 1; c < NC; c++) pf[c] = sp[r][c - 1];
+                } else {
+#pragma unroll
+                    for (int c = 0; c < NC; c++) pf[c] = sp[r][c];
+                }
+                double x[NW];
+                int k = 0;
+#pragma unroll
+                for (int a = 0; a < NC; a++)
+#pragma unroll
+                    for (int b = a; b < NC; b++, k++) x[k] = (double)(pf[a] * pf[b] * w);
+                if (sl[r] != cur) {
+                    if (cur != kTOff) {
+#pragma unroll
+                        for (int q = 0; q < NW; q++) atomicAdd(wacc + q * win + cur, cv[q]);
+                    }
+                    cur = sl[r];
+#pragma unroll
+                    for (int q = 0; q < NW; q++) cv[q] = x[q];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NW; q++) cv[q] += x[q];
+                }
+            }
+            if (cur != kTOff) {
+#pragma unroll
+                for (int q = 0; q < NW; q++) atomicAdd(wacc + q * win + cur, cv[q]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < win; q += BS) {
+        double *pp = part + (int64_t)wse[it.pbase + q] * NW;
+#pragma unroll
+        for (int k = 0; k < NW; k++) pp[k] = wacc[k * win + q];
+    }
+}
+
+// Plane of each partial value in the output map, in units of the map plane
+// (to_map: component c; to_weights: entry (a, b), b >= a, at a * nc + b).
+struct WPlanes {
+    int32_t o[6];
+};
+
+// Sum of entries [e0, e1) of one component q of the partial buffer (NP
+// values per entry), four loads in flight.
+template <int NP>
+__device__ __forceinline__ double tw_sum(const double *__restrict__ part, uint32_t e0,
+                                         uint32_t e1, int q) {
+    double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+    uint32_t e = e0;
+    for (; e + 4 <= e1; e += 4) {
+        s0 += part[(int64_t)e * NP + q];
+        s1 += part[(int64_t)(e + 1) * NP + q];
+        s2 += part[(int64_t)(e + 2) * NP + q];
+        s3 += part[(int64_t)(e + 3) * NP + q];
+    }
+    for (; e < e1; e++) s0 += part[(int64_t)e * NP + q];
+    return (s0 + s1) + (s2 + s3);
+}
+
+// Windowed second pass, adding into the map: one thread per visited pixel
+// and value sums the pixel's entries (contiguous; items in sample order).
+template <int NP>
+__global__ __launch_bounds__(kPlanBlock) void k_tw_add(const uint32_t *__restrict__ wcsr,
+                                                       const double *__restrict__ part,
+                                                       const int32_t *__restrict__ comp2pix,
+                                                       int64_t n_vis, WPlanes pl, int64_t plane,
+                                                       double *map) {
+    for (int64_t j = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; j < n_vis * NP;
+         j += (int64_t)gridDim.x * kPlanBlock) {
+        const int64_t c = j / NP;
+        const int q = (int)(j - c * NP);
+        const double s = tw_sum<NP>(part, wcsr[c], wcsr[c + 1], q);
+        map[pl.o[q] * plane + comp2pix[c]] += s;
+    }
+}
+
+// Windowed second pass, overwriting the map: one thread per map pixel and
+// component writes its sum (0 for a pixel no sample visits), so no zeroing
+// pass is needed.
+template <int NP>
+__global__ __launch_bounds__(kPlanBlock) void k_tw_set(const uint32_t *__restrict__ wcsr,
+                                                       const double *__restrict__ part,
+                                                       const uint32_t *__restrict__ bitmap,
+                                                       const uint32_t *__restrict__ wscan,
+                                                       int64_t npix, double *map) {
+    for (int64_t j = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; j < npix * NP;
+         j += (int64_t)gridDim.x * kPlanBlock) {
+        const int q = (int)(j / npix);
+        const int64_t p = j - (int64_t)q * npix;
+        double s = 0.;
+        if ((bitmap[p >> 5] >> (p & 31)) & 1u) {
+            const uint32_t c = compact_of(p, bitmap, wscan);
+            s = tw_sum<NP>(part, wcsr[c], wcsr[c + 1], q);
+        }
+        map[j] = s;
+    }
+}
+
+// Windowed from_map: one block of BS threads per item; the item's window of
+// the map (NC values per slot, through comp2pix) is loaded into LDS, then
+// each lane takes 8 consecutive samples.  Arithmetic and off-map handling as
+// k_plan_from_map (ProjEng_Precomp's from_map, Projection.cxx:2060-2085).
+template <int NC, bool TC, bool VEC, int BS>
+__global__ __launch_bounds__(1024) void k_tw_from_map(const WItem *__restrict__ items,
+                                                      const uint16_t *__restrict__ wslot,
+                                                      const float *__restrict__ tspin,
+                                                      const float *__restrict__ tval,
+                                                      const int32_t *__restrict__ comp2pix,
+                                                      const double *__restrict__ map,
+                                                      int64_t plane, float *__restrict__ sig,
+                                                      int accumulate) {
+    constexpr int NS = TC ? NC - 1 : NC;
+    extern __shared__ double wmv[];   // [NC][win]
+    const WItem it = items[blockIdx.x];
+    const int win = it.win;
+    const float pt = TC ? tval[it.det] : 0.f;
+    const int64_t e = it.b + it.n;
+    int64_t i0 = it.b + (int64_t)threadIdx.x * 8;
+    uint32_t sl[8];
+    float sp[8][NS > 0 ? NS : 1];
+    float o[8];
+    // the first 8 samples' loads are issued before the window's
+    if (i0 < e) tw_load<NS, VEC, float>(i0, e, wslot, tspin, sig, accumulate != 0, sl, sp, o);
+    for (int q = threadIdx.x; q < win; q += BS) {
+        const int64_t p = comp2pix[it.lo + q];
+#pragma unroll
+        for (int c = 0; c < NC; c++) wmv[c * win + q] = map[c * plane + p];
+    }
+    __syncthreads();
+    for (bool first = true; i0 < e; i0 += BS * 8, first = false) {
+        if (!first) tw_load<NS, VEC, float>(i0, e, wslot, tspin, sig, accumulate != 0, sl, sp, o);
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            if (sl[r] == kTOff) continue;
+            float pf[NC];
+            if constexpr (TC) {
+                pf[0] = pt;
+#pragma unroll
+                for (int c = 1; c < NC; c++) pf[c] = sp[r][c - 1];
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; c++) pf[c] = sp[r][c];
+            }
+            float sr = 0;
+#pragma unroll
+            for (int c = 0; c < NC; c++) sr = sr + pf[c] * wmv[c * win + sl[r]];
+            o[r] = accumulate ? o[r] + sr : sr;
+        }
+        if (VEC && i0 + 8 <= e) {
+#pragma unroll
+            for (int q = 0; q < 2; q++)
+                reinterpret_cast<float4 *>(sig + i0)[q] =
+                    float4{o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]};
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; r++)
+                if (i0 + r < e) sig[i0 + r] = o[r];
+        }
+    }
+}
+
+// Plain fill of n doubles (the zeroing pass of an overwriting to_map on the
+// other layouts).
+__global__ __launch_bounds__(kPlanBlock) void k_plan_fill0(double *x, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kPlanBlock)
+        x[i] = 0.;
+}
+
 }  // namespace so3g
 
 namespace {
@@ -1480,6 +1961,9 @@ void free_plan_arrays(so3g_pc_plan *p) {
     (void)hipFree(p->items); (void)hipFree(p->off_idx);
     (void)hipFree(p->tdims_d); (void)hipFree(p->tbase_d); (void)hipFree(p->sbase_d);
     (void)hipFree(p->tile_map); (void)hipFree(p->comp_tile); (void)hipFree(p->tiles_d);
+    (void)hipFree(p->wslot); (void)hipFree(p->witems); (void)hipFree(p->wpart);
+    (void)hipFree(p->wcsr); (void)hipFree(p->wse); (void)hipFree(p->wbitmap);
+    (void)hipFree(p->wwscan);
 }
 
 // Time-ordered copy for from_map: spin factors, and the compact pixels as
@@ -1529,10 +2013,131 @@ int64_t tcopy_bytes(const so3g_pc_plan *p, int64_t n) {
            p->n_wide * (kTGroup * 4 + 4) + p->n_vis * 32;
 }
 
+// Windowed to_map: the items' window sums into the partial buffer, then
+// per pixel into the map (overwrite: every map pixel is written); or, with
+// "plan_win_flush" 1, added to the map by atomics (overwrite: zeroed first).
+template <typename ST>
+int plan_tw_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
+                   const float *det_weights, hipStream_t s, bool overwrite) {
+    const int64_t npix = p->ny * p->nx;
+    const bool vec = p->wvec && ((uintptr_t)signal % 16) == 0;
+    const bool atom = g_plan_win_flush != 0;
+    if (atom && overwrite) {
+        const int64_t n_map = (int64_t)p->nc * npix;
+        hipLaunchKernelGGL(k_plan_fill0, dim3(grid_of(n_map)), dim3(kPlanBlock), 0, s, map,
+                           n_map);
+    }
+    if (p->n_witems > 0) {
+        const size_t lds = (size_t)p->nc * std::max(1, p->win_max) * sizeof(double);
+        const dim3 gi((unsigned)p->n_witems);
+        const WItem *it = (const WItem *)p->witems;
+        const int bs = g_plan_win_block;
+#define TW_TO3(NC, TC, V, BS, A)                                                            \
+    hipLaunchKernelGGL((k_tw_to_map<NC, TC, ST, V, BS, A>), gi, dim3(BS), lds, s, it,       \
+                       p->wslot, p->tspin, p->tval, signal, det_weights, p->wse, p->wpart,  \
+                       p->comp2pix, npix, map)
+#define TW_TO2(NC, TC, V)                                                                   \
+    do {                                                                                    \
+        if (bs == 1024) {                                                                   \
+            if (atom) TW_TO3(NC, TC, V, 1024, true); else TW_TO3(NC, TC, V, 1024, false);   \
+        } else {                                                                            \
+            if (atom) TW_TO3(NC, TC, V, 256, true); else TW_TO3(NC, TC, V, 256, false);     \
+        }                                                                                   \
+    } while (0)
+#define TW_TO(NC, TC)                                                                       \
+    do {                                                                                    \
+        if (vec) TW_TO2(NC, TC, true); else TW_TO2(NC, TC, false);                          \
+    } while (0)
+        if (p->tconst) {
+            if (p->nc == 1) TW_TO(1, true);
+            else TW_TO(3, true);
+        } else {
+            if (p->nc == 1) TW_TO(1, false);
+            else if (p->nc == 2) TW_TO(2, false);
+            else TW_TO(3, false);
+        }
+#undef TW_TO
+#undef TW_TO2
+#undef TW_TO3
+        PLAN_TRY(hipGetLastError());
+    }
+    if (atom) return SO3G_OK;
+    if (overwrite) {
+        const dim3 g(grid_of(npix * p->nc));
+        switch (p->nc) {
+        case 1: hipLaunchKernelGGL(k_tw_set<1>, g, dim3(kPlanBlock), 0, s, p->wcsr, p->wpart, p->wbitmap, p->wwscan, npix, map); break;
+        case 2: hipLaunchKernelGGL(k_tw_set<2>, g, dim3(kPlanBlock), 0, s, p->wcsr, p->wpart, p->wbitmap, p->wwscan, npix, map); break;
+        default: hipLaunchKernelGGL(k_tw_set<3>, g, dim3(kPlanBlock), 0, s, p->wcsr, p->wpart, p->wbitmap, p->wwscan, npix, map); break;
+        }
+    } else if (p->n_vis > 0) {
+        const dim3 g(grid_of(p->n_vis * p->nc));
+        const WPlanes pl{{0, 1, 2, 0, 0, 0}};
+        switch (p->nc) {
+        case 1: hipLaunchKernelGGL(k_tw_add<1>, g, dim3(kPlanBlock), 0, s, p->wcsr, p->wpart, p->comp2pix, p->n_vis, pl, npix, map); break;
+        case 2: hipLaunchKernelGGL(k_tw_add<2>, g, dim3(kPlanBlock), 0, s, p->wcsr, p->wpart, p->comp2pix, p->n_vis, pl, npix, map); break;
+        default: hipLaunchKernelGGL(k_tw_add<3>, g, dim3(kPlanBlock), 0, s, p->wcsr, p->wpart, p->comp2pix, p->n_vis, pl, npix, map); break;
+        }
+    }
+    PLAN_TRY(hipGetLastError());
+    return SO3G_OK;
+}
+
+// Windowed to_weights: upper-triangle products per item window, then added
+// per pixel into the (nc, nc, ...) map.
+int plan_tw_to_weights(const so3g_pc_plan *p, double *map, const float *det_weights,
+                       hipStream_t s) {
+    if (p->n_witems == 0 || p->n_vis == 0) return SO3G_OK;
+    const int nc = p->nc, nw = nc * (nc + 1) / 2;
+    const size_t lds = (size_t)nw * std::max(1, p->win_max) * sizeof(double);
+    const dim3 gi((unsigned)p->n_witems);
+    const WItem *it = (const WItem *)p->witems;
+    const int bs = g_plan_win_block;
+#define TW_W2(NC, TC, V)                                                                    \
+    do {                                                                                    \
+        if (bs == 1024)                                                                     \
+            hipLaunchKernelGGL((k_tw_to_weights<NC, TC, V, 1024>), gi, dim3(1024), lds, s,  \
+                               it, p->wslot, p->tspin, p->tval, det_weights, p->wse,        \
+                               p->wpart);                                                   \
+        else                                                                                \
+            hipLaunchKernelGGL((k_tw_to_weights<NC, TC, V, 256>), gi, dim3(256), lds, s, it, \
+                               p->wslot, p->tspin, p->tval, det_weights, p->wse, p->wpart); \
+    } while (0)
+#define TW_W(NC, TC)                                                                        \
+    do {                                                                                    \
+        if (p->wvec) TW_W2(NC, TC, true); else TW_W2(NC, TC, false);                        \
+    } while (0)
+    if (p->tconst) {
+        if (nc == 1) TW_W(1, true);
+        else TW_W(3, true);
+    } else {
+        if (nc == 1) TW_W(1, false);
+        else if (nc == 2) TW_W(2, false);
+        else TW_W(3, false);
+    }
+#undef TW_W
+#undef TW_W2
+    PLAN_TRY(hipGetLastError());
+    WPlanes pl{{0, 0, 0, 0, 0, 0}};
+    int k = 0;
+    for (int a = 0; a < nc; a++)
+        for (int b = a; b < nc; b++) pl.o[k++] = a * nc + b;
+    const dim3 g(grid_of(p->n_vis * nw));
+    const int64_t npix = p->ny * p->nx;
+    switch (nw) {
+    case 1: hipLaunchKernelGGL(k_tw_add<1>, g, dim3(kPlanBlock), 0, s, p->wcsr, p->wpart, p->comp2pix, p->n_vis, pl, npix, map); break;
+    case 3: hipLaunchKernelGGL(k_tw_add<3>, g, dim3(kPlanBlock), 0, s, p->wcsr, p->wpart, p->comp2pix, p->n_vis, pl, npix, map); break;
+    default: hipLaunchKernelGGL(k_tw_add<6>, g, dim3(kPlanBlock), 0, s, p->wcsr, p->wpart, p->comp2pix, p->n_vis, pl, npix, map); break;
+    }
+    PLAN_TRY(hipGetLastError());
+    return SO3G_OK;
+}
+
 template <typename ST>
 int plan_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
                 const float *det_weights, void *stream, double *const *tiles = nullptr) {
     if (p == nullptr) return set_error(SO3G_ERR_VALUE, "plan is NULL");
+    if (p->layout == 2)
+        return plan_tw_to_map<ST>(p, map, signal, det_weights, (hipStream_t)stream, false);
     if (p->n_on == 0) return SO3G_OK;
     hipStream_t s = (hipStream_t)stream;
     const uint32_t plane = (uint32_t)(p->ny * p->nx);
@@ -1630,8 +2235,10 @@ __global__ __launch_bounds__(kPlanBlock) void k_pc_from_map_f64(int64_t n, int32
 int plan_to_weights(const so3g_pc_plan *p, double *map, const float *det_weights,
                     void *stream, double *const *tiles) {
     if (p == nullptr) return set_error(SO3G_ERR_VALUE, "plan is NULL");
+    if (p->layout == 2) return plan_tw_to_weights(p, map, det_weights, (hipStream_t)stream);
     if (p->layout != 1)
-        return set_error(SO3G_ERR_RUNTIME, "to_weights needs a tiled-layout plan (plan_layout 1)");
+        return set_error(SO3G_ERR_RUNTIME, "to_weights needs a tiled or windowed plan "
+                                           "(plan_layout 1 or 2)");
     if (p->n_on == 0 || p->n_items == 0) return SO3G_OK;
     hipStream_t s = (hipStream_t)stream;
     const uint32_t plane = (uint32_t)(p->ny * p->nx);
@@ -1769,6 +2376,158 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
         PLAN_STEP(hipGetLastError());
     }
     const int32_t n_vis = (int32_t)std::max<int64_t>(1, p->n_vis);
+
+    // Is the T factor constant per detector (bit for bit)?  Then it is
+    // stored once per detector (every so3g pointing matrix: T = the
+    // detector's T response).
+    p->tconst = 0;
+    if (nc != 2) {
+        PLAN_ALLOC(bad, n_det * sizeof(int));
+        PLAN_STEP(hipMemsetAsync(bad, 0, n_det * sizeof(int), s));
+        hipLaunchKernelGGL(k_plan_tcheck, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, spin, nc, n_t,
+                           n, bad);
+        PLAN_STEP(hipGetLastError());
+        std::vector<int> hbad(n_det);
+        PLAN_STEP(hipMemcpyAsync(hbad.data(), bad, n_det * sizeof(int), hipMemcpyDeviceToHost, s));
+        PLAN_STEP(hipStreamSynchronize(s));
+        p->tconst = std::all_of(hbad.begin(), hbad.end(), [](int b) { return b == 0; });
+    }
+    const int c0 = p->tconst ? 1 : 0;
+    p->ns = nc - c0;
+    if (p->tconst) {
+        PLAN_ALLOC(p->tval, n_det * sizeof(float));
+        hipLaunchKernelGGL(k_plan_tval, dim3(grid_of(n_det)), dim3(kPlanBlock), 0, s, spin, nc, n_t,
+                           n_det, p->tval);
+        PLAN_STEP(hipGetLastError());
+    }
+
+    // ---- windowed layout: time order, per-item windows ----
+    if (k != 3 && (g_plan_layout == 2 || (g_plan_layout < 0 && n < kPlanTiledMin))) {
+        const int64_t n_sub_det = (n_t + kWSub - 1) / kWSub;
+        const int64_t n_sub = n_det * n_sub_det;
+        PLAN_ALLOC(keys_in, n_sub * 4);
+        PLAN_ALLOC(keys_out, n_sub * 4);
+        hipLaunchKernelGGL(k_tw_bounds, dim3(grid_of(n_sub * 64, kPlanBlock, INT32_MAX)),
+                           dim3(kPlanBlock), 0, s, pixel, pg, n_t, n_sub_det, n_sub, ny, nx,
+                           bitmap, wscan, keys_in, keys_out);
+        PLAN_STEP(hipGetLastError());
+        std::vector<int32_t> hlo(n_sub), hhi(n_sub);
+        PLAN_STEP(hipMemcpyAsync(hlo.data(), keys_in, n_sub * 4, hipMemcpyDeviceToHost, s));
+        PLAN_STEP(hipMemcpyAsync(hhi.data(), keys_out, n_sub * 4, hipMemcpyDeviceToHost, s));
+        PLAN_STEP(hipStreamSynchronize(s));
+        (void)hipFree(keys_in); keys_in = nullptr;
+        (void)hipFree(keys_out); keys_out = nullptr;
+        bool fits = true;
+        for (int64_t q = 0; q < n_sub && fits; q++)
+            fits = hhi[q] < 0 || (int64_t)hhi[q] - hlo[q] + 1 <= kWinMax;
+        // Items: a detector's sub-chunks merged while the window fits and the
+        // item stays within plan_win_len samples.
+        std::vector<WItem> hit;
+        std::vector<int32_t> sub_item(fits ? n_sub : 0), item_lo;
+        int64_t n_slots = 0;
+        const int64_t len_max = std::max<int64_t>(kWSub, g_plan_win_len);
+        for (int64_t d = 0; d < n_det && fits; d++) {
+            int64_t lo = INT32_MAX, hi = -1, start = 0, len = 0;
+            auto emit = [&]() {
+                const int32_t win = hi >= 0 ? (int32_t)(hi - lo + 1) : 0;
+                hit.push_back(WItem{d * n_t + start, (int32_t)len, (int32_t)d,
+                                    hi >= 0 ? (int32_t)lo : 0, win, n_slots});
+                item_lo.push_back(hi >= 0 ? (int32_t)lo : 0);
+                n_slots += win;
+            };
+            for (int64_t j = 0; j < n_sub_det; j++) {
+                const int64_t q = d * n_sub_det + j;
+                const int64_t lj = std::min<int64_t>(n_t, (j + 1) * kWSub) - j * kWSub;
+                int64_t nlo = lo, nhi = hi;
+                if (hhi[q] >= 0) { nlo = std::min<int64_t>(lo, hlo[q]); nhi = std::max<int64_t>(hi, hhi[q]); }
+                if (len > 0 && ((nhi >= 0 && nhi - nlo + 1 > kWinMax) || len + lj > len_max)) {
+                    emit();
+                    start = j * kWSub;
+                    len = 0;
+                    lo = INT32_MAX; hi = -1;
+                    nlo = hhi[q] >= 0 ? hlo[q] : lo;
+                    nhi = hhi[q];
+                }
+                lo = nlo; hi = nhi; len += lj;
+                sub_item[q] = (int32_t)hit.size();
+            }
+            emit();
+        }
+        fits = fits && n_slots < ((int64_t)1 << 31) && (int64_t)hit.size() < INT32_MAX;
+        if (!fits && g_plan_layout == 2) {
+            cleanup();
+            free_plan_arrays(p);
+            delete p;
+            return set_error(SO3G_ERR_RUNTIME,
+                             "windowed plan (plan_layout 2): some detector's 512 consecutive "
+                             "samples visit pixels spanning more than 2048 visited pixels");
+        }
+        if (fits) {
+            p->layout = 2;
+            p->n_witems = (int64_t)hit.size();
+            p->n_wslots = n_slots;
+            p->win_max = 0;
+            for (const WItem &w : hit) p->win_max = std::max(p->win_max, w.win);
+            p->wvec = n_t % 8 == 0;
+            // entries of each visited pixel: the items whose windows hold it,
+            // in item order; ent: the entry of each item slot
+            std::vector<uint32_t> csr(p->n_vis + 1, 0), ent(std::max<int64_t>(1, n_slots));
+            {
+                std::vector<int64_t> diff(p->n_vis + 1, 0);
+                for (const WItem &w : hit)
+                    if (w.win > 0) { diff[w.lo] += 1; diff[w.lo + w.win] -= 1; }
+                int64_t run = 0, acc = 0;
+                for (int64_t c = 0; c < p->n_vis; c++) {
+                    run += diff[c];
+                    csr[c] = (uint32_t)acc;
+                    acc += run;
+                }
+                csr[p->n_vis] = (uint32_t)acc;
+                std::vector<uint32_t> pos(csr.begin(), csr.end() - 1);
+                for (const WItem &w : hit)
+                    for (int32_t q = 0; q < w.win; q++) ent[w.pbase + q] = pos[w.lo + q]++;
+            }
+            const int np = std::max(nc, nc * (nc + 1) / 2);
+            PLAN_ALLOC(p->witems, std::max<int64_t>(1, p->n_witems) * (int64_t)sizeof(WItem));
+            PLAN_ALLOC(p->wcsr, (p->n_vis + 1) * 4);
+            PLAN_ALLOC(p->wse, std::max<int64_t>(1, n_slots) * 4);
+            PLAN_ALLOC(p->wpart, std::max<int64_t>(1, n_slots) * np * (int64_t)sizeof(double));
+            PLAN_ALLOC(p->wslot, n * 2);
+            PLAN_ALLOC(p->tspin, std::max<int64_t>(1, n * p->ns) * 4);
+            PLAN_ALLOC(keys_in, n_sub * 4);
+            PLAN_ALLOC(keys_out, std::max<int64_t>(1, p->n_witems) * 4);
+            PLAN_STEP(hipMemcpyAsync(p->witems, hit.data(), p->n_witems * sizeof(WItem),
+                                     hipMemcpyHostToDevice, s));
+            PLAN_STEP(hipMemcpyAsync(p->wcsr, csr.data(), (p->n_vis + 1) * 4,
+                                     hipMemcpyHostToDevice, s));
+            PLAN_STEP(hipMemcpyAsync(p->wse, ent.data(), std::max<int64_t>(1, n_slots) * 4,
+                                     hipMemcpyHostToDevice, s));
+            PLAN_STEP(hipMemcpyAsync(keys_in, sub_item.data(), n_sub * 4, hipMemcpyHostToDevice,
+                                     s));
+            PLAN_STEP(hipMemcpyAsync(keys_out, item_lo.data(), p->n_witems * 4,
+                                     hipMemcpyHostToDevice, s));
+            PLAN_ALLOC(d_on, sizeof(unsigned long long));
+            PLAN_STEP(hipMemsetAsync(d_on, 0, sizeof(unsigned long long), s));
+            hipLaunchKernelGGL(k_tw_fill, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, pixel, pg,
+                               spin, nc, c0, n, n_t, n_sub_det, ny, nx, bitmap, wscan, keys_in,
+                               keys_out, p->wslot, p->tspin, d_on);
+            PLAN_STEP(hipGetLastError());
+            unsigned long long n_on = 0;
+            PLAN_STEP(hipMemcpyAsync(&n_on, d_on, sizeof(n_on), hipMemcpyDeviceToHost, s));
+            PLAN_STEP(hipStreamSynchronize(s));
+            p->n_on = (int64_t)n_on;
+            // the visited bitmap stays with the plan (overwriting to_map)
+            p->wbitmap = bitmap; bitmap = nullptr;
+            p->wwscan = wscan; wscan = nullptr;
+            cleanup();
+            p->bytes = n * (2 + 4 * p->ns) + p->n_witems * (int64_t)sizeof(WItem) +
+                       n_slots * (4 + 8 * np) + (p->n_vis + 1) * 4 + p->n_vis * 4 +
+                       n_words * 8 + (p->tconst ? n_det * 4 : 0);
+            *out = p;
+            return SO3G_OK;
+        }
+    }
+
     if (k == 3 || g_plan_layout == 1 || (g_plan_layout < 0 && n >= kPlanTiledMin)) {
         // ---- tiled layout ----
         p->layout = 1;
@@ -1844,28 +2603,6 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
         (void)hipFree(keys_in); keys_in = nullptr;
         (void)hipFree(vals_in); vals_in = nullptr;
         (void)hipFree(tmp); tmp = nullptr;
-        // T factor constant per detector?
-        p->tconst = 0;
-        if (nc != 2) {
-            PLAN_ALLOC(bad, n_det * sizeof(int));
-            PLAN_STEP(hipMemsetAsync(bad, 0, n_det * sizeof(int), s));
-            hipLaunchKernelGGL(k_plan_tcheck, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, spin, nc,
-                               n_t, n, bad);
-            PLAN_STEP(hipGetLastError());
-            std::vector<int> hbad(n_det);
-            PLAN_STEP(hipMemcpyAsync(hbad.data(), bad, n_det * sizeof(int),
-                                     hipMemcpyDeviceToHost, s));
-            PLAN_STEP(hipStreamSynchronize(s));
-            p->tconst = std::all_of(hbad.begin(), hbad.end(), [](int b) { return b == 0; });
-        }
-        const int c0 = p->tconst ? 1 : 0;
-        p->ns = nc - c0;
-        if (p->tconst) {
-            PLAN_ALLOC(p->tval, n_det * sizeof(float));
-            hipLaunchKernelGGL(k_plan_tval, dim3(grid_of(n_det)), dim3(kPlanBlock), 0, s, spin,
-                               nc, n_t, n_det, p->tval);
-            PLAN_STEP(hipGetLastError());
-        }
         // slots and stored factors per sorted position
         PLAN_ALLOC(p->tslot, p->n_on * 2);
         PLAN_ALLOC(p->sspin, p->n_on * p->ns * sizeof(float));
@@ -2002,28 +2739,6 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
     (void)hipFree(keys_in); keys_in = nullptr;
     (void)hipFree(vals_in); vals_in = nullptr;
     (void)hipFree(tmp); tmp = nullptr;
-
-    // 2. is the T factor constant per detector?
-    p->tconst = 0;
-    if (nc != 2) {
-        PLAN_ALLOC(bad, n_det * sizeof(int));
-        PLAN_STEP(hipMemsetAsync(bad, 0, n_det * sizeof(int), s));
-        hipLaunchKernelGGL(k_plan_tcheck, dim3(grid_of(n)), dim3(kPlanBlock), 0, s, spin, nc, n_t,
-                           n, bad);
-        PLAN_STEP(hipGetLastError());
-        std::vector<int> hbad(n_det);
-        PLAN_STEP(hipMemcpyAsync(hbad.data(), bad, n_det * sizeof(int), hipMemcpyDeviceToHost, s));
-        PLAN_STEP(hipStreamSynchronize(s));
-        p->tconst = std::all_of(hbad.begin(), hbad.end(), [](int b) { return b == 0; });
-    }
-    const int c0 = p->tconst ? 1 : 0;
-    p->ns = nc - c0;
-    if (p->tconst) {
-        PLAN_ALLOC(p->tval, n_det * sizeof(float));
-        hipLaunchKernelGGL(k_plan_tval, dim3(grid_of(n_det)), dim3(kPlanBlock), 0, s, spin, nc, n_t,
-                           n_det, p->tval);
-        PLAN_STEP(hipGetLastError());
-    }
 
     // 3. stored spin factors in sorted order
     PLAN_ALLOC(p->sspin, p->n_on * p->ns * sizeof(float));
@@ -2177,6 +2892,18 @@ int so3g_pc_plan_to_map(const so3g_pc_plan *p, double *map, const float *signal,
     return plan_to_map<float>(p, map, signal, det_weights, stream);
 }
 
+int so3g_pc_plan_to_map_set(const so3g_pc_plan *p, double *map, const float *signal,
+                            const float *det_weights, void *stream) {
+    if (int rc = plan_untiled(p)) return rc;
+    if (p->layout == 2)
+        return plan_tw_to_map<float>(p, map, signal, det_weights, (hipStream_t)stream, true);
+    const int64_t n_map = (int64_t)p->nc * p->ny * p->nx;
+    hipLaunchKernelGGL(k_plan_fill0, dim3(grid_of(n_map)), dim3(kPlanBlock), 0,
+                       (hipStream_t)stream, map, n_map);
+    PLAN_TRY(hipGetLastError());
+    return plan_to_map<float>(p, map, signal, det_weights, stream);
+}
+
 int so3g_pc_plan_to_map_f64(const so3g_pc_plan *p, double *map, const double *signal,
                             const float *det_weights, void *stream) {
     if (int rc = plan_untiled(p)) return rc;
@@ -2196,6 +2923,40 @@ static int plan_from_map_impl(const so3g_pc_plan *p, const double *map,
     hipStream_t s = (hipStream_t)stream;
     const int64_t plane = p->ny * p->nx;
     const int64_t n = p->n_det * p->n_t;
+    if (p->layout == 2) {
+        if (p->n_witems == 0) return SO3G_OK;
+        const size_t lds = (size_t)p->nc * std::max(1, p->win_max) * sizeof(double);
+        const dim3 gi((unsigned)p->n_witems);
+        const WItem *it = (const WItem *)p->witems;
+        const bool vec = p->wvec && ((uintptr_t)signal % 16) == 0;
+#define TW_FROM2(NC, TC, V)                                                                 \
+    do {                                                                                    \
+        if (g_plan_win_block == 1024)                                                       \
+            hipLaunchKernelGGL((k_tw_from_map<NC, TC, V, 1024>), gi, dim3(1024), lds, s, it, \
+                               p->wslot, p->tspin, p->tval, p->comp2pix, map, plane,        \
+                               signal, accumulate);                                         \
+        else                                                                                \
+            hipLaunchKernelGGL((k_tw_from_map<NC, TC, V, 256>), gi, dim3(256), lds, s, it,  \
+                               p->wslot, p->tspin, p->tval, p->comp2pix, map, plane,        \
+                               signal, accumulate);                                         \
+    } while (0)
+#define TW_FROM(NC, TC)                                                                     \
+    do {                                                                                    \
+        if (vec) TW_FROM2(NC, TC, true); else TW_FROM2(NC, TC, false);                      \
+    } while (0)
+        if (p->tconst) {
+            if (p->nc == 1) TW_FROM(1, true);
+            else TW_FROM(3, true);
+        } else {
+            if (p->nc == 1) TW_FROM(1, false);
+            else if (p->nc == 2) TW_FROM(2, false);
+            else TW_FROM(3, false);
+        }
+#undef TW_FROM2
+#undef TW_FROM
+        PLAN_TRY(hipGetLastError());
+        return SO3G_OK;
+    }
     if (p->layout == 1 && p->tpd == nullptr) {
         if (!accumulate && p->n_off > 0) {
             hipLaunchKernelGGL(k_tp_zero, dim3(grid_of(p->n_off)), dim3(kPlanBlock), 0, s,
@@ -2345,8 +3106,10 @@ int so3g_pc_plan_stats(const so3g_pc_plan *p, int64_t *stats) {
     stats[6] = p->n_vis;
     stats[7] = p->layout;
     stats[8] = p->n_heads;
-    stats[9] = p->n_items;
+    stats[9] = p->layout == 2 ? p->n_witems : p->n_items;
     stats[10] = p->n_wide;
+    stats[11] = p->win_max;
+    stats[12] = p->n_wslots;
     return SO3G_OK;
 }
 

@@ -804,7 +804,7 @@ class ProjEngPrecompBase:
         pix, spin, n_det, n_t, k, nc = self._pmat(call, pixel_index, spin_proj)
         if thread_intervals is None:
             plan = self._cached_plan(pixel_index, spin_proj, pix, spin, map)
-            if plan is not None and plan.info()['layout'] == 1:
+            if plan is not None and plan.info()['layout'] in (1, 2):
                 self._check_lead(map, 2, nc, k)
                 return plan.to_weights(map, weights)
         mp, sp, tp, dp = self._map(call, map, 2, k, writeback=True)
@@ -864,8 +864,13 @@ class PrecompPlan:
     layout=None chooses by size) sorts them by map
     tile, keeping sample order inside a tile, and sums each work item in the
     tile held in LDS; layout 0 sorts them by pixel and sums each pixel's run
-    with a wave-level segmented reduction.  from_map reads a compact
-    time-ordered copy.  to_map adds what ProjEng_Precomp_NonTiled.to_map adds
+    with a wave-level segmented reduction; layout 2 (the default below 2^25
+    samples where it applies, else layout 0) keeps time order in work items
+    of one detector's consecutive samples whose visited pixels span at most
+    2048 compact values, sums each item in that window in LDS and adds the
+    items' window sums per pixel in a second pass, and gathers from_map from
+    the window loaded into LDS.  Layouts 0 and 1 read a compact time-ordered
+    copy in from_map.  to_map adds what ProjEng_Precomp_NonTiled.to_map adds
     over every sample (float64 sums of the same float32 products, in a
     different order); from_map returns what ProjEng_Precomp_NonTiled.from_map
     returns, bit for bit.  t_block_mb: the time block of the sort (None: the
@@ -923,16 +928,20 @@ class PrecompPlan:
         """n_on (on-map samples), runs (pixel runs in sorted order),
         t_const (T factor stored once per detector), stored_spin (factors
         stored per sample), t_block (samples per time block), bytes, n_vis
-        (visited pixels), layout (0 pixel-sorted runs, 1 map tiles), and for
-        the tiled layout stretches (starts of runs of consecutive samples
-        within a tile, counting each 64-position chunk's first position) and
-        items (work items); wide_groups: 512-sample groups of the
-        time-ordered copy whose compact pixels span more than 65535 values."""
-        a = (ctypes.c_int64 * 11)()
+        (visited pixels), layout (0 pixel-sorted runs, 1 map tiles, 2
+        time-ordered windows), for the tiled layout stretches (starts of runs
+        of consecutive samples within a tile, counting each 64-position
+        chunk's first position), items (work items of the tiled and windowed
+        layouts); wide_groups: 512-sample groups of the time-ordered copy
+        whose compact pixels span more than 65535 values; window_max and
+        window_slots: the windowed layout's largest window and the windows'
+        total size."""
+        a = (ctypes.c_int64 * 13)()
         N.check(N.load().so3g_pc_plan_stats(self._h, a))
         return {'n_on': a[0], 'runs': a[1], 't_const': bool(a[2]), 'stored_spin': a[3],
                 't_block': a[4], 'bytes': a[5], 'n_vis': a[6], 'layout': a[7],
-                'stretches': a[8], 'items': a[9], 'wide_groups': a[10]}
+                'stretches': a[8], 'items': a[9], 'wide_groups': a[10],
+                'window_max': a[11], 'window_slots': a[12]}
 
     def _tiles(self, call, map, writeback):
         """Host array of the tile list's device pointers (0 for None)."""
@@ -947,9 +956,10 @@ class PrecompPlan:
         call.keep.append(ptrs)
         return ptrs.ctypes.data
 
-    def to_map(self, map, signal, det_weights=None):
+    def to_map(self, map, signal, det_weights=None, overwrite=False):
         """map += P^T signal; map is an (nc, ...) array, or a list of tiles
-        (None for inactive ones) for a k = 3 plan."""
+        (None for inactive ones) for a k = 3 plan.  overwrite=True (non-tiled
+        maps): map = P^T signal, whatever map held before."""
         call = _Call()
         sig = call.array(signal, 'float32', 'signal', spec=[self.n_det, self.n_t])
         dw = call.array(det_weights, 'float32', 'det_weights', spec=[self.n_det],
@@ -960,8 +970,9 @@ class PrecompPlan:
         else:
             m = call.array(map, 'float64', 'map', spec=[self.nc] + list(self.map_shape),
                            out=True)
-            N.check(N.load().so3g_pc_plan_to_map(self._h, m.data_ptr(), sig.data_ptr(),
-                                                 _ptr(dw), _stream()))
+            f = (N.load().so3g_pc_plan_to_map_set if overwrite else
+                 N.load().so3g_pc_plan_to_map)
+            N.check(f(self._h, m.data_ptr(), sig.data_ptr(), _ptr(dw), _stream()))
         call.finish()
         return map
 

@@ -187,10 +187,12 @@ def test_plan_options():
     and rejection of values outside their ranges."""
     defaults = {'plan_layout': -1, 'plan_tile_item': 262144, 'plan_time_copy': 1,
                 'plan_to_lds': 1, 'plan_to_cpb': 2, 'plan_from_layout': 1, 'plan_from_v': 4,
-                'nmat_dchunk': 64}
+                'nmat_dchunk': 64, 'plan_win_len': 8192, 'plan_win_block': 1024,
+                'plan_win_flush': 1}
     for name, v in defaults.items():
         assert _native.get_option(name) == v, name
-    for name, good, bad in (('plan_layout', 0, 2), ('plan_layout', 1, -2),
+    for name, good, bad in (('plan_layout', 2, 3), ('plan_layout', 1, -2),
+                            ('plan_win_len', 512, 511), ('plan_win_block', 256, 512),
                             ('plan_tile_item', 4096, 63), ('plan_to_cpb', 5, 0),
                             ('plan_from_v', 8, 5), ('nmat_dchunk', 128, 0)):
         old = _native.get_option(name)
