@@ -529,8 +529,11 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   in LDS, 0 pixel runs, 2 time-ordered windows, -1
  *                   (default) tiles from 2^25 samples up, below that windows
  *                   where they fit, else pixel runs
- *   "plan_win_len"  samples per work item of the windowed layout at most
- *                   (default 8192, 512..2^30)
+ *   "plan_win_len"  samples per work item of the windowed layout's to_map
+ *                   at most (default 65536, 512..2^30): a detector's samples
+ *                   go to ceil(n_t / plan_win_len) items of about equal
+ *                   length (more where the plan would have fewer than 64
+ *                   items); from_map takes them in pieces of 8192
  *   "plan_win_block"  threads per work item of the windowed layout (1024
  *                   default, or 256)
  *   "plan_win_flush"  windowed to_map: 1 (default) float64 atomics from

@@ -121,6 +121,8 @@ struct so3g_pc_plan {
     uint16_t *wslot;       // [n] offset in the item's window, kTOff off the map
     void *witems;          // [n_witems] WItem
     int64_t n_witems;
+    void *wfitems;         // [n_wfitems] from_map's pieces of the items (WItem)
+    int64_t n_wfitems;
     int32_t win_max;       // largest window
     int64_t n_wslots;      // windows summed: partial slots
     double *wpart;         // [n_wslots * max(nc, nc (nc + 1) / 2)] per-item sums
@@ -165,7 +167,7 @@ int g_plan_to_cpb = 2;
 int g_plan_layout = -1;
 constexpr int64_t kPlanTiledMin = (int64_t)1 << 25;
 int64_t g_plan_tile_item = 262144;
-int64_t g_plan_win_len = 8192;
+int64_t g_plan_win_len = 65536;
 // Windowed layout: threads per work item ("plan_win_block", 256 or 1024) and
 // how to_map adds the window sums ("plan_win_flush": 1 float64 atomics into
 // the map, 0 partial sums and a second pass per pixel -- no atomics and a
@@ -1505,6 +1507,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_tp_zero(const uint32_t *off, int
 // ---------------------------------------------------------------------------
 constexpr int kWSub = 512;       // samples per sub-chunk of the window bounds
 constexpr int kWinMax = 2048;    // largest window (LDS: 48 KB of TQU sums)
+constexpr int kWFrom = 8192;     // samples per from_map piece (8 per thread of 1024)
 
 struct WItem {
     int64_t b;       // first sample (d * n_t + t)
@@ -1645,7 +1648,7 @@ __device__ __forceinline__ void tw_load(int64_t i0, int64_t e, const uint16_t *_
 // to_map (Projection.cxx:2100-2121): (w * sig) * pf in float32 (float64 for
 // a float64 signal), summed in float64; a zero detector weight skips the
 // detector.
-template <int NC, bool TC, typename ST, bool VEC, int BS, bool ATOM>
+template <int NC, bool TC, typename ST, bool VEC, int BS, bool ATOM, int CP>
 __global__ __launch_bounds__(1024) void k_tw_to_map(const WItem *__restrict__ items,
                                                     const uint16_t *__restrict__ wslot,
                                                     const float *__restrict__ tspin,
@@ -1657,9 +1660,10 @@ __global__ __launch_bounds__(1024) void k_tw_to_map(const WItem *__restrict__ it
                                                     const int32_t *__restrict__ comp2pix,
                                                     int64_t plane, double *map) {
     constexpr int NS = TC ? NC - 1 : NC;
-    extern __shared__ double wacc[];   // [NC][win]
+    extern __shared__ double wsum[];   // [CP][NC][win]: lane parity picks the copy (CP = 2)
     const WItem it = items[blockIdx.x];
     const int win = it.win;
+    double *wacc = wsum + (CP > 1 ? (threadIdx.x & 1) * NC * win : 0);
     const float w = det_weights != nullptr ? det_weights[it.det] : 1.f;
     const float pt = TC ? tval[it.det] : 0.f;
     const bool work = win > 0 && w != 0.f;   // a zero weight skips the detector (Projection.cxx:2104)
@@ -1670,7 +1674,7 @@ __global__ __launch_bounds__(1024) void k_tw_to_map(const WItem *__restrict__ it
     ST sg[8];
     // the first 8 samples' loads are issued before the window is zeroed
     if (work && i0 < e) tw_load<NS, VEC, ST>(i0, e, wslot, tspin, sig, true, sl, sp, sg);
-    for (int q = threadIdx.x; q < NC * win; q += BS) wacc[q] = 0.;
+    for (int q = threadIdx.x; q < CP * NC * win; q += BS) wsum[q] = 0.;
     __syncthreads();
     if (work) {
         for (bool first = true; i0 < e; i0 += BS * 8, first = false) {
@@ -1720,17 +1724,21 @@ __global__ __launch_bounds__(1024) void k_tw_to_map(const WItem *__restrict__ it
     }
     __syncthreads();
     for (int q = threadIdx.x; q < win; q += BS) {
+        double v[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            v[c] = wsum[c * win + q];
+            if (CP > 1) v[c] += wsum[(NC + c) * win + q];
+        }
         if constexpr (ATOM) {
             const int64_t p = comp2pix[it.lo + q];
 #pragma unroll
-            for (int c = 0; c < NC; c++) {
-                const double v = wacc[c * win + q];
-                if (v != 0.) unsafeAtomicAdd(map + c * plane + p, v);
-            }
+            for (int c = 0; c < NC; c++)
+                if (v[c] != 0.) unsafeAtomicAdd(map + c * plane + p, v[c]);
         } else {
             double *pp = part + (int64_t)wse[it.pbase + q] * NC;
 #pragma unroll
-            for (int c = 0; c < NC; c++) pp[c] = wacc[c * win + q];
+            for (int c = 0; c < NC; c++) pp[c] = v[c];
         }
     }
 }
@@ -1937,9 +1945,16 @@ __global__ __launch_bounds__(1024) void k_tw_from_map(const WItem *__restrict__ 
 // Plain fill of n doubles (the zeroing pass of an overwriting to_map on the
 // other layouts).
 __global__ __launch_bounds__(kPlanBlock) void k_plan_fill0(double *x, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kPlanBlock)
-        x[i] = 0.;
+    if (n <= 0) return;
+    // 16-byte stores from the first 16-byte boundary; the ends one by one
+    const int64_t h = (int64_t)(((16 - ((uintptr_t)x & 15)) & 15) / 8);
+    const int64_t nv = (n - h) / 2;
+    const int64_t tid = (int64_t)blockIdx.x * kPlanBlock + threadIdx.x;
+    const int64_t nt = (int64_t)gridDim.x * kPlanBlock;
+    for (int64_t i = tid; i < nv; i += nt)
+        reinterpret_cast<double2 *>(x + h)[i] = double2{0., 0.};
+    if (tid < h) x[tid] = 0.;
+    if (tid == 0 && h + 2 * nv < n) x[n - 1] = 0.;
 }
 
 }  // namespace so3g
@@ -1962,6 +1977,7 @@ void free_plan_arrays(so3g_pc_plan *p) {
     (void)hipFree(p->tdims_d); (void)hipFree(p->tbase_d); (void)hipFree(p->sbase_d);
     (void)hipFree(p->tile_map); (void)hipFree(p->comp_tile); (void)hipFree(p->tiles_d);
     (void)hipFree(p->wslot); (void)hipFree(p->witems); (void)hipFree(p->wpart);
+    (void)hipFree(p->wfitems);
     (void)hipFree(p->wcsr); (void)hipFree(p->wse); (void)hipFree(p->wbitmap);
     (void)hipFree(p->wwscan);
 }
@@ -2024,18 +2040,26 @@ int plan_tw_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
     const bool atom = g_plan_win_flush != 0;
     if (atom && overwrite) {
         const int64_t n_map = (int64_t)p->nc * npix;
-        hipLaunchKernelGGL(k_plan_fill0, dim3(grid_of(n_map)), dim3(kPlanBlock), 0, s, map,
-                           n_map);
+        hipLaunchKernelGGL(k_plan_fill0, dim3(grid_of(n_map / 2 + 1)), dim3(kPlanBlock), 0, s,
+                           map, n_map);
     }
     if (p->n_witems > 0) {
-        const size_t lds = (size_t)p->nc * std::max(1, p->win_max) * sizeof(double);
+        // two copies of the window sums (by lane parity) where they leave room
+        // for two blocks per compute unit
+        const size_t lds1 = (size_t)p->nc * std::max(1, p->win_max) * sizeof(double);
+        const bool two = 2 * lds1 <= 65536;
+        const size_t lds = two ? 2 * lds1 : lds1;
         const dim3 gi((unsigned)p->n_witems);
         const WItem *it = (const WItem *)p->witems;
         const int bs = g_plan_win_block;
-#define TW_TO3(NC, TC, V, BS, A)                                                            \
-    hipLaunchKernelGGL((k_tw_to_map<NC, TC, ST, V, BS, A>), gi, dim3(BS), lds, s, it,       \
+#define TW_TO4(NC, TC, V, BS, A, CP)                                                        \
+    hipLaunchKernelGGL((k_tw_to_map<NC, TC, ST, V, BS, A, CP>), gi, dim3(BS), lds, s, it,   \
                        p->wslot, p->tspin, p->tval, signal, det_weights, p->wse, p->wpart,  \
                        p->comp2pix, npix, map)
+#define TW_TO3(NC, TC, V, BS, A)                                                            \
+    do {                                                                                    \
+        if (two) TW_TO4(NC, TC, V, BS, A, 2); else TW_TO4(NC, TC, V, BS, A, 1);             \
+    } while (0)
 #define TW_TO2(NC, TC, V)                                                                   \
     do {                                                                                    \
         if (bs == 1024) {                                                                   \
@@ -2059,6 +2083,7 @@ int plan_tw_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
 #undef TW_TO
 #undef TW_TO2
 #undef TW_TO3
+#undef TW_TO4
         PLAN_TRY(hipGetLastError());
     }
     if (atom) return SO3G_OK;
@@ -2425,7 +2450,16 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
         std::vector<WItem> hit;
         std::vector<int32_t> sub_item(fits ? n_sub : 0), item_lo;
         int64_t n_slots = 0;
-        const int64_t len_max = std::max<int64_t>(kWSub, g_plan_win_len);
+        // a detector's samples in k = ceil(n_t / plan_win_len) items of about
+        // equal length (whole sub-chunks), at least 64 items in all: the
+        // items' window flushes (one float64 atomic per touched slot and
+        // component) cost more than the parallelism of shorter items gains
+        // (4e6 samples: from_map + overwriting to_map 39.5 us cold with 100
+        // items, 41.2 with 200, 44.2 with 300)
+        const int64_t wl = std::max<int64_t>(kWSub, g_plan_win_len);
+        const int64_t k_items = std::min<int64_t>(
+            n_sub_det, std::max<int64_t>((n_t + wl - 1) / wl, (64 + n_det - 1) / n_det));
+        const int64_t len_max = ((n_sub_det + k_items - 1) / k_items) * kWSub;
         for (int64_t d = 0; d < n_det && fits; d++) {
             int64_t lo = INT32_MAX, hi = -1, start = 0, len = 0;
             auto emit = [&]() {
@@ -2487,6 +2521,17 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
                 for (const WItem &w : hit)
                     for (int32_t q = 0; q < w.win; q++) ent[w.pbase + q] = pos[w.lo + q]++;
             }
+            // from_map takes the items in pieces of at most kWFrom samples (one
+            // pass of a 1024-thread block), each with its item's window
+            std::vector<WItem> hfit;
+            for (const WItem &w : hit)
+                for (int32_t o = 0; o < w.n || (o == 0 && w.n == 0); o += kWFrom)
+                    hfit.push_back(WItem{w.b + o, std::min<int32_t>(kWFrom, w.n - o), w.det,
+                                         w.lo, w.win, w.pbase});
+            p->n_wfitems = (int64_t)hfit.size();
+            PLAN_ALLOC(p->wfitems, std::max<int64_t>(1, p->n_wfitems) * (int64_t)sizeof(WItem));
+            PLAN_STEP(hipMemcpyAsync(p->wfitems, hfit.data(), p->n_wfitems * sizeof(WItem),
+                                     hipMemcpyHostToDevice, s));
             const int np = std::max(nc, nc * (nc + 1) / 2);
             PLAN_ALLOC(p->witems, std::max<int64_t>(1, p->n_witems) * (int64_t)sizeof(WItem));
             PLAN_ALLOC(p->wcsr, (p->n_vis + 1) * 4);
@@ -2520,7 +2565,7 @@ int plan_create_impl(const int32_t *pixel, PixGeom pg, const float *spin, int32_
             p->wbitmap = bitmap; bitmap = nullptr;
             p->wwscan = wscan; wscan = nullptr;
             cleanup();
-            p->bytes = n * (2 + 4 * p->ns) + p->n_witems * (int64_t)sizeof(WItem) +
+            p->bytes = n * (2 + 4 * p->ns) + (p->n_witems + p->n_wfitems) * (int64_t)sizeof(WItem) +
                        n_slots * (4 + 8 * np) + (p->n_vis + 1) * 4 + p->n_vis * 4 +
                        n_words * 8 + (p->tconst ? n_det * 4 : 0);
             *out = p;
@@ -2898,7 +2943,7 @@ int so3g_pc_plan_to_map_set(const so3g_pc_plan *p, double *map, const float *sig
     if (p->layout == 2)
         return plan_tw_to_map<float>(p, map, signal, det_weights, (hipStream_t)stream, true);
     const int64_t n_map = (int64_t)p->nc * p->ny * p->nx;
-    hipLaunchKernelGGL(k_plan_fill0, dim3(grid_of(n_map)), dim3(kPlanBlock), 0,
+    hipLaunchKernelGGL(k_plan_fill0, dim3(grid_of(n_map / 2 + 1)), dim3(kPlanBlock), 0,
                        (hipStream_t)stream, map, n_map);
     PLAN_TRY(hipGetLastError());
     return plan_to_map<float>(p, map, signal, det_weights, stream);
@@ -2924,10 +2969,10 @@ static int plan_from_map_impl(const so3g_pc_plan *p, const double *map,
     const int64_t plane = p->ny * p->nx;
     const int64_t n = p->n_det * p->n_t;
     if (p->layout == 2) {
-        if (p->n_witems == 0) return SO3G_OK;
+        if (p->n_wfitems == 0) return SO3G_OK;
         const size_t lds = (size_t)p->nc * std::max(1, p->win_max) * sizeof(double);
-        const dim3 gi((unsigned)p->n_witems);
-        const WItem *it = (const WItem *)p->witems;
+        const dim3 gi((unsigned)p->n_wfitems);
+        const WItem *it = (const WItem *)p->wfitems;
         const bool vec = p->wvec && ((uintptr_t)signal % 16) == 0;
 #define TW_FROM2(NC, TC, V)                                                                 \
     do {                                                                                    \

@@ -25,9 +25,15 @@ def window_items(comp, win_len):
     """Work items of the windowed layout from the compact pixel of every
     sample (n_det, n_t; -1 off the map): 512-sample sub-chunks of a detector
     merged while the merged window spans at most WIN_MAX compact values and
-    the item holds at most win_len samples.  Returns [(lo, win)], or None
+    the item holds at most ceil(n_sub / k) sub-chunks, k = max(ceil(n_t /
+    win_len), ceil(64 / n_det)) capped at n_sub, the sub-chunks per
+    detector.  Returns [(lo, win)], or None
     when a sub-chunk alone spans more than WIN_MAX."""
     n_det, n_t = comp.shape
+    n_sub = -(-n_t // SUB)
+    # about equal items of whole sub-chunks, at least 64 items in all
+    k = min(n_sub, max(-(-n_t // max(SUB, win_len)), -(-64 // n_det)))
+    win_len = -(-n_sub // k) * SUB
     items = []
     for d in range(n_det):
         lo, hi, length = None, None, 0
@@ -80,7 +86,7 @@ def set_win_len(n):
     return old
 
 
-@pytest.mark.parametrize('win_len', [512, 8192])
+@pytest.mark.parametrize('win_len', [512, 8192, 20000])
 @pytest.mark.parametrize('n_det,n_samp,comps,tconst', [
     (30, 7000, 'TQU', True),       # n_t % 8 == 0: vector loads
     (21, 3001, 'TQU', True),       # odd sample count: per-sample loads
@@ -261,7 +267,9 @@ def test_windowed_plan_healpix(gpu, nest, nside):
     pix[2, 100:400, 0] = -1
     npix = 12 * nside * nside
     on = pix[..., 0] >= 0
-    items = window_items(compact(pix[..., 0].astype(np.int64), on), 8192)
+    from so3g_amd import _native
+    items = window_items(compact(pix[..., 0].astype(np.int64), on),
+                         _native.get_option('plan_win_len'))
     if items is None:
         with pytest.raises(RuntimeError):
             so3g_amd.PrecompPlan(pix, spin, (npix,), layout=2)

@@ -187,7 +187,7 @@ def test_plan_options():
     and rejection of values outside their ranges."""
     defaults = {'plan_layout': -1, 'plan_tile_item': 262144, 'plan_time_copy': 1,
                 'plan_to_lds': 1, 'plan_to_cpb': 2, 'plan_from_layout': 1, 'plan_from_v': 4,
-                'nmat_dchunk': 64, 'plan_win_len': 8192, 'plan_win_block': 1024,
+                'nmat_dchunk': 64, 'plan_win_len': 65536, 'plan_win_block': 1024,
                 'plan_win_flush': 1}
     for name, v in defaults.items():
         assert _native.get_option(name) == v, name
