@@ -301,7 +301,7 @@ int so3g_pc_plan_from_map_tiles(const so3g_pc_plan *plan, const double *const *t
  * plan: map (nc, nc, ...) += sum of (pf_a * pf_b) * det_wt (float32) for
  * b >= a, the upper triangle as the reference fills it; summed per work item
  * in LDS (float64, another order than the reference's sequential sums).
- * Tiled (1) and windowed (2) layouts; layout 0 plans are refused. */
+ * Every layout (pixel runs: per-run sums and atomics as to_map). */
 int so3g_pc_plan_to_weights(const so3g_pc_plan *plan, double *map, const float *det_weights,
                             void *stream);
 int so3g_pc_plan_to_weights_tiles(const so3g_pc_plan *plan, double *const *tiles,

@@ -512,6 +512,12 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_runs(const int32_t *key, in
     }
 }
 
+// Plane of each output value in the output map, in units of the map plane
+// (to_map: component c; to_weights: entry (a, b), b >= a, at a * nc + b).
+struct WPlanes {
+    int32_t o[6];
+};
+
 // Run sums and atomics of one lane's nr (<= kPlanR) consecutive sorted
 // samples, whose run-start bits are hb and whose first run index is
 // lane_base + (bit 0 of hb ? 0 : -1); xf(r, x) gives sample r's NC
@@ -522,7 +528,8 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_runs(const int32_t *key, in
 template <int NC, typename XF>
 __device__ __forceinline__ void plan_lane_scatter(uint32_t hb, uint32_t lane_base, int nr,
                                                   const uint32_t *__restrict__ run_pix,
-                                                  uint32_t plane, double *map, XF &&xf) {
+                                                  uint32_t plane, const WPlanes &pl,
+                                                  double *map, XF &&xf) {
     int64_t hk = -1, ck = -1;
     uint32_t cpix = 0;
     double hv[NC], cv[NC];
@@ -545,7 +552,7 @@ __device__ __forceinline__ void plan_lane_scatter(uint32_t hb, uint32_t lane_bas
                 } else {
 #pragma unroll
                     for (int c = 0; c < NC; c++)
-                        unsafeAtomicAdd(map + (int64_t)c * plane + cpix, cv[c]);
+                        unsafeAtomicAdd(map + (int64_t)pl.o[c] * plane + cpix, cv[c]);
                 }
             }
             ck = rid;
@@ -565,13 +572,13 @@ __device__ __forceinline__ void plan_lane_scatter(uint32_t hb, uint32_t lane_bas
     } else if (ck >= 0) {
 #pragma unroll
         for (int c = 0; c < NC; c++)
-            unsafeAtomicAdd(map + (int64_t)c * plane + cpix, cv[c]);
+            unsafeAtomicAdd(map + (int64_t)pl.o[c] * plane + cpix, cv[c]);
         hpix = hk >= 0 ? run_pix[hk] : 0u;
     }
     if (wave_runs<NC>(hk, hv) && hk >= 0) {
 #pragma unroll
         for (int c = 0; c < NC; c++)
-            unsafeAtomicAdd(map + (int64_t)c * plane + hpix, hv[c]);
+            unsafeAtomicAdd(map + (int64_t)pl.o[c] * plane + hpix, hv[c]);
     }
 }
 
@@ -631,7 +638,8 @@ __global__ __launch_bounds__(kPlanBlock, 6) void k_plan_to_map_c(
         const uint32_t hb = nr > 0 ? heads[base / 8] : 0u;
         const uint32_t wb = nr > 0 ? wbase[base / kGroup] : 0u;
         const uint32_t lane_base = wb + wave_excl_scan(__popc(hb));
-        plan_lane_scatter<NC>(hb, lane_base, nr, run_pix, plane, map, [&](int r, double *x) {
+        const WPlanes pl{{0, 1, 2, 0, 0, 0}};
+        plan_lane_scatter<NC>(hb, lane_base, nr, run_pix, plane, pl, map, [&](int r, double *x) {
             const uint32_t si = ii[r];
             uint32_t det = 0;
             if (TC || det_weights != nullptr) det = fast_div<DIV>(si, n_t);
@@ -663,6 +671,67 @@ __global__ __launch_bounds__(kPlanBlock, 6) void k_plan_to_map_c(
 #pragma unroll
                 for (int c = 0; c < NC; c++) x[c] = 0.;
             }
+        });
+    }
+}
+
+// Planned to_weights over the compact sorted stream (pixel-run layout):
+// per sample the NC (NC + 1) / 2 upper-triangle products (pf_a * pf_b) * w
+// in float32 (Projection.cxx:2135-2154), run sums and atomics as in
+// k_plan_to_map_c, into the (NC, NC, ...) map at planes a * NC + b.
+template <int NC, bool TC, int DIV>
+__global__ __launch_bounds__(kPlanBlock, 6) void k_plan_to_weights_c(
+    const uint32_t *__restrict__ idx, const float *__restrict__ sspin,
+    const float *__restrict__ tval, const uint8_t *__restrict__ heads,
+    const uint32_t *__restrict__ wbase, const uint32_t *__restrict__ run_pix, int64_t n_on,
+    FastDiv n_t, uint32_t plane, const float *__restrict__ det_weights, int32_t cpb,
+    double *map) {
+    constexpr int NS = TC ? NC - 1 : NC;
+    constexpr int NW = NC * (NC + 1) / 2;
+    constexpr int R = kPlanR;
+    WPlanes pl{{0, 0, 0, 0, 0, 0}};
+    {
+        int k = 0;
+#pragma unroll
+        for (int a = 0; a < NC; a++)
+#pragma unroll
+            for (int b = a; b < NC; b++) pl.o[k++] = a * NC + b;
+    }
+    const int64_t c0 = (int64_t)blockIdx.x * cpb;
+    for (int64_t blk = c0; blk < c0 + cpb && blk * (kPlanBlock * R) < n_on; blk++) {
+        const int64_t base = (blk * kPlanBlock + threadIdx.x) * R;
+        const int nr = (int)max((int64_t)0, min((int64_t)R, n_on - base));
+        uint32_t ii[R];
+        float sp[R * (NS > 0 ? NS : 1)];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            ii[r] = r < nr ? idx[base + r] : 0u;
+#pragma unroll
+            for (int c = 0; c < NS; c++) sp[r * NS + c] = r < nr ? sspin[(base + r) * NS + c] : 0.f;
+        }
+        const uint32_t hb = nr > 0 ? heads[base / 8] : 0u;
+        const uint32_t wb = nr > 0 ? wbase[base / kGroup] : 0u;
+        const uint32_t lane_base = wb + wave_excl_scan(__popc(hb));
+        plan_lane_scatter<NW>(hb, lane_base, nr, run_pix, plane, pl, map, [&](int r, double *x) {
+            const uint32_t si = ii[r];
+            uint32_t det = 0;
+            if (TC || det_weights != nullptr) det = fast_div<DIV>(si, n_t);
+            float pf[NC];
+            if constexpr (TC) {
+                pf[0] = tval[det];
+#pragma unroll
+                for (int c = 1; c < NC; c++) pf[c] = sp[r * NS + c - 1];
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; c++) pf[c] = sp[r * NS + c];
+            }
+            const float w = det_weights != nullptr ? det_weights[det] : 1.f;
+            int k = 0;
+#pragma unroll
+            for (int a = 0; a < NC; a++)
+#pragma unroll
+                for (int b = a; b < NC; b++, k++)
+                    x[k] = w != 0.f ? (double)(pf[a] * pf[b] * w) : 0.;   // zero weight: skipped
         });
     }
 }
@@ -1817,12 +1886,6 @@ __global__ __launch_bounds__(1024) void k_tw_to_weights(
     }
 }
 
-// Plane of each partial value in the output map, in units of the map plane
-// (to_map: component c; to_weights: entry (a, b), b >= a, at a * nc + b).
-struct WPlanes {
-    int32_t o[6];
-};
-
 // Sum of entries [e0, e1) of one component q of the partial buffer (NP
 // values per entry), four loads in flight.
 template <int NP>
@@ -2261,13 +2324,30 @@ int plan_to_weights(const so3g_pc_plan *p, double *map, const float *det_weights
                     void *stream, double *const *tiles) {
     if (p == nullptr) return set_error(SO3G_ERR_VALUE, "plan is NULL");
     if (p->layout == 2) return plan_tw_to_weights(p, map, det_weights, (hipStream_t)stream);
-    if (p->layout != 1)
-        return set_error(SO3G_ERR_RUNTIME, "to_weights needs a tiled or windowed plan "
-                                           "(plan_layout 1 or 2)");
-    if (p->n_on == 0 || p->n_items == 0) return SO3G_OK;
     hipStream_t s = (hipStream_t)stream;
     const uint32_t plane = (uint32_t)(p->ny * p->nx);
     const FastDiv nt{(uint32_t)p->n_t, 1.f / (float)p->n_t};
+    if (p->layout == 0) {
+        if (p->n_on == 0) return SO3G_OK;
+        const int32_t cpb = g_plan_to_cpb;
+        const dim3 g(grid_of(p->n_on, (int64_t)kPlanBlock * kPlanR * cpb, INT32_MAX));
+#define PW(NC, TC)                                                                          \
+    hipLaunchKernelGGL((k_plan_to_weights_c<NC, TC, 1>), g, dim3(kPlanBlock), 0, s, p->idx, \
+                       p->sspin, p->tval, p->heads, p->wbase, p->run_pix, p->n_on, nt,       \
+                       plane, det_weights, cpb, map)
+        if (p->tconst) {
+            if (p->nc == 1) PW(1, true);
+            else PW(3, true);
+        } else {
+            if (p->nc == 1) PW(1, false);
+            else if (p->nc == 2) PW(2, false);
+            else PW(3, false);
+        }
+#undef PW
+        PLAN_TRY(hipGetLastError());
+        return SO3G_OK;
+    }
+    if (p->n_on == 0 || p->n_items == 0) return SO3G_OK;
     const int tabs = (p->tconst ? 1 : 0) + (det_weights != nullptr ? 1 : 0);
     const bool ldt = g_plan_to_lds && tabs > 0 &&
                      p->n_det * tabs * (int64_t)sizeof(float) <= kPlanLdsBytes;

@@ -804,7 +804,7 @@ class ProjEngPrecompBase:
         pix, spin, n_det, n_t, k, nc = self._pmat(call, pixel_index, spin_proj)
         if thread_intervals is None:
             plan = self._cached_plan(pixel_index, spin_proj, pix, spin, map)
-            if plan is not None and plan.info()['layout'] in (1, 2):
+            if plan is not None:
                 self._check_lead(map, 2, nc, k)
                 return plan.to_weights(map, weights)
         mp, sp, tp, dp = self._map(call, map, 2, k, writeback=True)
@@ -979,7 +979,7 @@ class PrecompPlan:
     def to_weights(self, map, det_weights=None):
         """map += P^T diag(det_weights) P as ProjEng_Precomp.to_weight_map
         fills it: the upper triangle of an (nc, nc, ...) array, or of each
-        tile of a tile list (k = 3).  Tiled-layout plans only."""
+        tile of a tile list (k = 3)."""
         call = _Call()
         dw = call.array(det_weights, 'float32', 'det_weights', spec=[self.n_det],
                         optional=True)

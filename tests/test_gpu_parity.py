@@ -604,15 +604,11 @@ def test_precomp_plan(gpu, n_det, n_samp, comps, tconst, t_block_mb, layout, tco
     finally:
         for n, v in zip(names, old):
             _native.set_option(n, v)
-    if layout == 1:
-        # to_weights through the plan == Precomp to_weight_map (upper triangle)
-        for w in (None, dw):
-            g = plan.to_weights(np.zeros((nc, nc, 100, 300)), w)
-            r = oracle.precomp_to_weights(c.og, np.zeros((nc, nc, 100, 300)), pix, spin, w)
-            maps_close(g, r, f'plan to_weights {comps}', atol=1e-9)
-    else:
-        with pytest.raises(RuntimeError):
-            plan.to_weights(np.zeros((nc, nc, 100, 300)))
+    # to_weights through the plan == Precomp to_weight_map (upper triangle)
+    for w in (None, dw):
+        g = plan.to_weights(np.zeros((nc, nc, 100, 300)), w)
+        r = oracle.precomp_to_weights(c.og, np.zeros((nc, nc, 100, 300)), pix, spin, w)
+        maps_close(g, r, f'plan to_weights {comps} layout {layout}', atol=1e-9)
     m = c.map_like([nc])
     ref = oracle.precomp_from_map(c.og, m, pix, spin)
     s0 = np.array(c.signal)
@@ -679,6 +675,9 @@ def test_precomp_plan_edges(gpu, case, layout, tcopy):
     g = plan.to_map(m0.copy(), sig)
     r = oracle.precomp_to_map(og, m0.copy(), pix, spin, sig)
     maps_close(g, r, f'plan to_map {case}', atol=1e-9)
+    gw = plan.to_weights(np.zeros((3, 3, ny, nx)))
+    rw = oracle.precomp_to_weights(og, np.zeros((3, 3, ny, nx)), pix, spin)
+    maps_close(gw, rw, f'plan to_weights {case}', atol=1e-9)
     m = rng.standard_normal((3, ny, nx))
     assert_bit_equal(np.array(plan.from_map(m)), oracle.precomp_from_map(og, m, pix, spin),
                      f'plan from_map {case}')
@@ -856,10 +855,9 @@ def test_precomp_plan_healpix(gpu, nest, nside, t_block_mb, layout, tcopy):
         g = plan.to_map(np.zeros((3, npix)), c.signal, w)
         r = oracle.precomp_to_map(c.og, np.zeros((3, npix)), pix, spin, c.signal, w)
         maps_close(g, r, f'HP plan to_map nside={nside}', atol=1e-9)
-    if layout == 1:
-        g = plan.to_weights(np.zeros((3, 3, npix)), dw)
-        r = oracle.precomp_to_weights(c.og, np.zeros((3, 3, npix)), pix, spin, dw)
-        maps_close(g, r, f'HP plan to_weights nside={nside}', atol=1e-9)
+    g = plan.to_weights(np.zeros((3, 3, npix)), dw)
+    r = oracle.precomp_to_weights(c.og, np.zeros((3, 3, npix)), pix, spin, dw)
+    maps_close(g, r, f'HP plan to_weights nside={nside} layout {layout}', atol=1e-9)
     m = c.map_like([3])
     s0 = np.array(c.signal)
     assert_bit_equal(np.array(plan.from_map(m)), oracle.precomp_from_map(c.og, m, pix, spin),
