@@ -115,6 +115,8 @@ def parse():
                     help='skip the G3SuperTimestream decoding leg')
     ap.add_argument('--no-studies', action='store_true',
                     help='skip the precision (C3), batched (C4) and map-making (C5) legs')
+    ap.add_argument('--only-small', action='store_true',
+                    help='time only the small-observation pair and print its JSON')
     ap.add_argument('--cpu-full', action='store_true',
                     help='print the full CPU baseline table (oracle, median of 5) and exit')
     return ap.parse_args()
@@ -136,6 +138,10 @@ def main():
     torch.cuda.set_device(dev)
     N.init(0)
     lib = N.load()
+    if args.only_small:
+        stream = torch.cuda.current_stream().cuda_stream or None
+        print(json.dumps(small_config(torch, so3g_amd, N, workloads, dev, stream)), flush=True)
+        return
 
     t0 = time.time()
     obs = workloads.Observation(args.n_det, args.n_samp, 1.5, 0.5, seed=0)
