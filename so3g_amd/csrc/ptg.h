@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "cr_atan2.h"
 
 #pragma clang fp contract(off)
 
@@ -124,7 +125,9 @@ __device__ __forceinline__ void get_coords(const Q4 &qb, const Q4 &qd, const Tab
         } else if constexpr (PROJ == PROJ_CEA) {
             const double ct = a * a - b * b - c * c + d * d;
             const double h = 0.5 * sqrt(1 - ct * ct);
-            co[0] = atan2(c * d - a * b, c * a + d * b);
+            // the C library's atan2 in the reference (Projection.cxx:468);
+            // here correctly rounded (cr_atan2.h)
+            co[0] = cr_atan2(c * d - a * b, c * a + d * b);
             co[1] = ct;
             co[2] = (a * c - b * d) / h;
             co[3] = (c * d + a * b) / h;

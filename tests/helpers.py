@@ -109,3 +109,52 @@ def all_flat_cases():
             for interp in (0, 1):
                 out.append((proj, tiled, interp))
     return out
+
+
+def _atan_dec(x):
+    """atan(x) of a Decimal in the current context: argument halving
+    (atan x = 2 atan(x / (1 + sqrt(1 + x^2)))) to |x| < 0.05, then the
+    Taylor series."""
+    from decimal import Decimal
+    m = 0
+    while abs(x) > Decimal('0.05'):
+        x = x / (1 + (1 + x * x).sqrt())
+        m += 1
+    s, term, n, x2 = Decimal(0), x, 0, x * x
+    eps = Decimal(10) ** -60
+    while True:
+        t = term / (2 * n + 1)
+        if abs(t) < eps:
+            break
+        s += t if n % 2 == 0 else -t
+        term *= x2
+        n += 1
+    return s * (2 ** m)
+
+
+def cr_atan2(y, x):
+    """atan2 of two doubles, correctly rounded to double: the quadrant
+    arithmetic of C99 atan2 on a 70-digit decimal atan, then rounded once
+    (float(Decimal) rounds to nearest).  Finite, non-zero arguments."""
+    from decimal import Decimal, localcontext
+    with localcontext() as ctx:
+        ctx.prec = 70
+        yd, xd = Decimal(float(y)), Decimal(float(x))
+        pi = 4 * _atan_dec(Decimal(1))
+        if abs(yd) <= abs(xd):
+            a = _atan_dec(abs(yd) / abs(xd))
+        else:
+            a = pi / 2 - _atan_dec(abs(xd) / abs(yd))
+        if xd < 0:
+            a = pi - a
+        return float(-a if yd < 0 else a)
+
+
+def cea_atan2_args(bore, dets):
+    """The two atan2 arguments of so3g's CEA pointer (Projection.cxx:460-468):
+    y = c d - a b, x = c a + d b of the boresight x detector quaternion
+    (the oracle's product, bit-exact), in float64 without FMA."""
+    gq = oracle.geom('Quat', 4)
+    q = oracle.coords(gq, bore, dets)
+    a, b, c, d = (q[..., i] for i in range(4))
+    return c * d - a * b, c * a + d * b

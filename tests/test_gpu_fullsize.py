@@ -6,9 +6,11 @@ Every pixel index and every spin factor that get_pointing_matrix returns is
 compared bit for bit with the OpenMP oracle, in detector chunks (the host
 holds one chunk at a time), for the three maps of this configuration: CAR
 0.5' non-tiled, CAR 0.5' tiled 100 x 100, HEALPix NEST nside 4096.  CEA
-0.5' is compared as well: its longitude is the math library's atan2 (glibc
-on the CPU, the device library on the GPU), so the coordinates are counted
-ulp by ulp and the pixels must still agree everywhere.
+0.5' is compared as well: its longitude is the C library's atan2 in the
+oracle (glibc) and the correctly rounded atan2 on the device, so the
+longitudes are counted ulp by ulp (glibc's last bit differs in about 0.1 %
+of such arguments), the differing ones checked against a 70-digit decimal
+atan2, and the pixels must still agree everywhere.
 
 The device trig tables (built on the GPU host with its C library at
 so3g_init) are checked against the SHA-256 of the tables of the reference's
@@ -115,7 +117,8 @@ def test_full_size_cea(gpu, obs):
     eng = so3g_amd.ProjEng_CEA_TQU_NonTiled(args)
     og = oracle.geom_from_args('CEA', 'TQU', args)
     bore_t, dets_t, resp_t = _device_inputs(obs, gpu)
-    n_lon = n_other = max_ulp = 0
+    from helpers import cea_atan2_args, cr_atan2
+    n_lon = n_other = max_ulp = n_checked = 0
     for d0 in range(0, N_DET, CHUNK):
         d1 = min(d0 + CHUNK, N_DET)
         gc = eng.coords(bore_t, dets_t[d0:d1]).cpu().numpy()
@@ -124,11 +127,19 @@ def test_full_size_cea(gpu, obs):
         du = np.abs(gc[..., 0].view(np.int64) - rc[..., 0].view(np.int64))
         n_lon += int((du != 0).sum())
         max_ulp = max(max_ulp, int(du.max()))
+        # the first differing samples of each chunk hold the correctly
+        # rounded atan2
+        for dd, t in np.argwhere(du != 0)[:20]:
+            y, x = cea_atan2_args(obs.bore[t:t + 1], obs.dets[d0 + dd:d0 + dd + 1])
+            assert gc[dd, t, 0] == cr_atan2(y[0, 0], x[0, 0]), (d0 + dd, t)
+            n_checked += 1
         del gc, rc
-    print(f'CEA longitude (device atan2 vs glibc atan2): {n_lon} of {N_DET * N_SAMP} '
-          f'differ, by at most {max_ulp} ulp')
+    print(f'CEA longitude (device correctly rounded atan2 vs glibc atan2): {n_lon} of '
+          f'{N_DET * N_SAMP} differ, by at most {max_ulp} ulp; {n_checked} of them checked '
+          f'against a 70-digit atan2')
     assert n_other == 0
     assert max_ulp <= 1
+    assert n_lon <= 0.01 * N_DET * N_SAMP
     n, n_on = _compare_chunks(eng, og, obs, gpu)
     print(f'CEA 0.5\' {shape}: {n} samples, {n_on} on the map, pixels and spin bit-exact')
     assert n_on == n
