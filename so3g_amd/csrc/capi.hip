@@ -88,6 +88,14 @@ struct State {
     const int32_t *pc_order_pix = nullptr;
     int64_t pc_order_nt = -1, pc_order_ndet = -1;
     int32_t pc_order_k = 0;
+    // Detector groups of the row scatter (home row, up to G detectors
+    // each), cached like the order above.
+    int32_t *rg_det_d = nullptr, *rg_row_d = nullptr;
+    int64_t rg_cap = 0;
+    const void *rg_bore = nullptr, *rg_dets = nullptr;
+    int64_t rg_nt = -1, rg_ndet = -1;
+    int32_t rg_G = 0, rg_n = 0;
+    so3g_geom rg_geom{};
 };
 
 // Tunables (so3g_set_option): wave-level run aggregation of scatter-adds,
@@ -101,10 +109,12 @@ struct Options {
     // 0 never, 1 where measured faster than global atomics (all but tiled
     // flat to_map, the table per case), 2 always with the small table
     // (256 threads, 512 slots), 3 always with the large one (1024 threads,
-    // 112 KB).
+    // 112 KB), 4 map rows in LDS where the geometry allows it (non-tiled
+    // flat maps, nearest neighbour; else as 1).  Mode 1 takes the rows for
+    // to_weights where the geometry allows them.
     int scatter_hash = 1;
-    int hash_dets = 0;         // detectors per block in modes 2, 3 (0: 4 / 2)
-    int hash_span = 0;         // samples per block in modes 2, 3 (0: 256 / 32768)
+    int hash_dets = 0;         // detectors per block in modes 2-4 (0: 4 / 2 / 2)
+    int hash_span = 0;         // samples per block in modes 2-4 (0: 256 / 32768 / 16384)
 };
 Options g_opt;
 constexpr int kNumXcd = 8;
@@ -522,6 +532,95 @@ static int scatter_order(const so3g_geom *g, const OpArgs &a0, const int32_t **o
     return SO3G_OK;
 }
 
+// Detector groups of the row scatter: each detector's home row is its map
+// row at the middle sample (one k_coords launch over n_det samples, the
+// pixel formula of get_pixel on the host); detectors are sorted by home row,
+// then along it, and split into groups of at most G that share a row.  The
+// groups only steer which samples add into LDS rather than to the map, so a
+// stale cache entry costs speed, never correctness.
+static int row_groups(const so3g_geom *g, const OpArgs &a0, int G, RowWork &rw) {
+    const int64_t n_det = a0.in.n_det, n_t = a0.in.n_t;
+    if (!(g_state.rg_bore == a0.in.bore && g_state.rg_dets == a0.in.dets &&
+          g_state.rg_nt == n_t && g_state.rg_ndet == n_det && g_state.rg_G == G &&
+          memcmp(&g_state.rg_geom, g, sizeof(*g)) == 0)) {
+        if (n_det > g_state.order_cap) {
+            if (g_state.order_d) HIP_TRY(hipFree(g_state.order_d));
+            if (g_state.order_co_d) HIP_TRY(hipFree(g_state.order_co_d));
+            g_state.order_d = nullptr;
+            g_state.order_co_d = nullptr;
+            g_state.order_cap = 0;
+            g_state.order_bore = nullptr;
+            HIP_TRY(hipMalloc(&g_state.order_d, n_det * sizeof(int32_t)));
+            HIP_TRY(hipMalloc(&g_state.order_co_d, n_det * 4 * sizeof(double)));
+            g_state.order_cap = n_det;
+        }
+        OpArgs a = a0;
+        a.op = OP_COORDS;
+        a.in.bore = a0.in.bore + 4 * (n_t / 2);
+        a.in.n_t = 1;
+        a.in.chunks = nullptr;
+        a.coords_out = g_state.order_co_d;
+        int r = run_dispatch(g->proj, a);
+        if (r) return r;
+        std::vector<double> co(n_det * 4);
+        HIP_TRY(hipMemcpyAsync(co.data(), g_state.order_co_d, n_det * 4 * sizeof(double),
+                               hipMemcpyDeviceToHost, a.stream));
+        HIP_TRY(hipStreamSynchronize(a.stream));
+        std::vector<int32_t> row(n_det), idx(n_det);
+        for (int64_t d = 0; d < n_det; d++) {
+            const double x = co[4 * d] / g->cdelt[1] + g->crpix[1] - 1 + 0.5;
+            const double y = co[4 * d + 1] / g->cdelt[0] + g->crpix[0] - 1 + 0.5;
+            const bool on = x >= 0 && x < g->naxis[1] && y >= 0 && y < g->naxis[0];
+            row[d] = on ? (int32_t)y : -1;
+            idx[d] = (int32_t)d;
+        }
+        std::stable_sort(idx.begin(), idx.end(), [&](int32_t p, int32_t q) {
+            if (row[p] != row[q]) return row[p] < row[q];
+            return co[4 * p] < co[4 * q];
+        });
+        std::vector<int32_t> gdet, grow;
+        for (int64_t i = 0; i < n_det; i++) {
+            const int32_t d = idx[i];
+            const bool fresh = grow.empty() || grow.back() != row[d] ||
+                               gdet.size() % G == 0;
+            if (fresh) {
+                while (gdet.size() % G) gdet.push_back(-1);
+                grow.push_back(row[d]);
+            }
+            gdet.push_back(d);
+        }
+        while (gdet.size() % G) gdet.push_back(-1);
+        const int64_t ng = (int64_t)grow.size();
+        if (ng * G > g_state.rg_cap || ng > g_state.rg_cap) {
+            if (g_state.rg_det_d) HIP_TRY(hipFree(g_state.rg_det_d));
+            if (g_state.rg_row_d) HIP_TRY(hipFree(g_state.rg_row_d));
+            g_state.rg_det_d = g_state.rg_row_d = nullptr;
+            g_state.rg_cap = 0;
+            g_state.rg_bore = nullptr;
+            HIP_TRY(hipMalloc(&g_state.rg_det_d, ng * G * sizeof(int32_t)));
+            HIP_TRY(hipMalloc(&g_state.rg_row_d, ng * G * sizeof(int32_t)));
+            g_state.rg_cap = ng * G;
+        }
+        HIP_TRY(hipMemcpyAsync(g_state.rg_det_d, gdet.data(), ng * G * sizeof(int32_t),
+                               hipMemcpyHostToDevice, a.stream));
+        HIP_TRY(hipMemcpyAsync(g_state.rg_row_d, grow.data(), ng * sizeof(int32_t),
+                               hipMemcpyHostToDevice, a.stream));
+        HIP_TRY(hipStreamSynchronize(a.stream));
+        g_state.rg_bore = a0.in.bore;
+        g_state.rg_dets = a0.in.dets;
+        g_state.rg_nt = n_t;
+        g_state.rg_ndet = n_det;
+        g_state.rg_G = G;
+        g_state.rg_n = (int32_t)ng;
+        g_state.rg_geom = *g;
+    }
+    rw.gdet = g_state.rg_det_d;
+    rw.grow = g_state.rg_row_d;
+    rw.n_groups = g_state.rg_n;
+    rw.G = G;
+    return SO3G_OK;
+}
+
 static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
                          const int32_t *tile_dims, const double *bore, int64_t n_t,
                          const double *dets, int64_t n_det, const float *resp,
@@ -565,7 +664,28 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
     // pixels together, and keeps more blocks per compute unit than the
     // large one (1024 threads, 112 KB, a span of 32768 samples).
     int cls = 0, per_block = 0, span = 0;
-    if (g_opt.scatter_hash == 1) {
+    const int nv_row = op == OP_TO_MAP ? g->ncomp : g->ncomp * (g->ncomp + 1) / 2;
+    const bool row_ok = g->pixelizor == SO3G_PIX_FLAT && !g->tiled && g->interp == 0 &&
+                        prec == SO3G_PREC_REF && segs == nullptr &&
+                        (int64_t)nv_row * g->naxis[1] * 8 <= 160 * 1024;
+    // Map rows in LDS (k_scatter_row) for to_weights in mode 1 (1000 x 720000
+    // samples, CAR 0.5' TQU: 8.2 ms against 11.7 with the small tables;
+    // groups of 2 detectors over 16384 samples; 1, 3, 4, 6 detectors 8.4 -
+    // 8.5 ms, spans of 4096 / 8192 9.0 / 8.7 ms), and for to_map too in mode
+    // 4 (9.0 ms against 8.7 with the small tables, so not in mode 1).
+    if (row_ok && (g_opt.scatter_hash == 4 ||
+                   (g_opt.scatter_hash == 1 && op == OP_TO_WEIGHTS))) {
+        a.hash_cls = 3;
+        a.rw.span = g_opt.hash_span > 0 ? g_opt.hash_span : 16384;
+        r = row_groups(g, a, g_opt.hash_dets > 0 ? g_opt.hash_dets : 2, a.rw);
+        if (r) return r;
+        while (((n_t + a.rw.span - 1) / a.rw.span) * (int64_t)a.rw.n_groups > INT32_MAX)
+            a.rw.span *= 2;
+        r = run_dispatch(g->proj, a);
+        if (r) return r;
+        return SO3G_OK;
+    }
+    if (g_opt.scatter_hash == 1 || g_opt.scatter_hash == 4) {
         if (g->pixelizor == SO3G_PIX_HEALPIX) {
             if (op == OP_TO_MAP) cls = 1, per_block = 2, span = 32768;
             else cls = 2, per_block = 4, span = 256;
@@ -927,7 +1047,9 @@ static int pc_order(const int32_t *pixel, int32_t k, int64_t n_det, int64_t n_t,
 // 720000 samples into 100 x 100 tiles: 12.1 ms with global atomics, 12.6 ms
 // at best with the tables; to_weights 21.6 -> 16.8 ms).
 static bool pc_lds_applies(int32_t k, const PcMap &pm, const int32_t *segs, bool w) {
-    if (!(g_opt.scatter_hash == 1 || g_opt.scatter_hash == 2) || segs != nullptr) return false;
+    if (!(g_opt.scatter_hash == 1 || g_opt.scatter_hash == 2 || g_opt.scatter_hash == 4) ||
+        segs != nullptr)
+        return false;
     if (k == 3) return w || g_opt.scatter_hash == 2;
     return pm.ny * pm.nx < INT32_MAX;
 }
@@ -986,7 +1108,7 @@ int so3g_set_option(const char *name, int64_t value) {
     else if (n == "scatter_xcd") g_opt.scatter_xcd = (int)(value != 0);
     else if (n == "pc_from_dpb") g_opt.pc_from_dpb = (int)std::max<int64_t>(1, value);
     else if (n == "scatter_hash") {
-        if (value < 0 || value > 3) return fail(SO3G_ERR_VALUE, "scatter_hash must be 0..3");
+        if (value < 0 || value > 4) return fail(SO3G_ERR_VALUE, "scatter_hash must be 0..4");
         g_opt.scatter_hash = (int)value;
     } else if (n == "hash_dets") {
         if (value < 0 || value > 4096)

@@ -698,6 +698,112 @@ __global__ __launch_bounds__(NT) void k_scatter_lds(PtgIn in, Geo g, MapView<dou
 }
 
 // --------------------------------------------------------------------------
+// to_map / to_weights through a map row held in LDS (non-tiled flat maps).
+//
+// A constant-elevation scan in horizon coordinates keeps each detector on
+// one map row.  A block owns a group of up to G detectors that share a home
+// row (their row at mid-observation) over T consecutive samples, and keeps
+// that row's NV float64 sums per pixel in LDS (direct indexing: no keys, no
+// probing).  Each thread keeps its sample's boresight in registers and walks
+// the group's detectors; runs of equal pixels within a wave are summed first
+// (wave_runs) and each run's total is added to the row in LDS, or to the map
+// with float64 atomics when it lies on another row.  The row's touched
+// pixels are then added to the map, one atomic per pixel and value.  Sums are
+// float64 sums of the same float32 products as k_to_map / k_to_weights, in
+// another order.
+// --------------------------------------------------------------------------
+struct RowWork {
+    const int32_t *gdet;    // [n_groups * G] detectors of each group, -1 padded
+    const int32_t *grow;    // [n_groups] home row (-1: none)
+    int32_t n_groups, G, span;
+};
+
+template <int PROJ, int NC, bool W, int NT>
+__global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<double> m,
+                                                    const float *resp, const float *sig,
+                                                    const float *det_weights, RowWork rw) {
+    constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
+    extern __shared__ double racc[];   // [NV][nx]
+    const int nx = g.nx;
+    const int64_t ts = (int64_t)blockIdx.x / rw.n_groups;
+    const int gi = (int)((int64_t)blockIdx.x - ts * rw.n_groups);
+    const int row = rw.grow[gi];
+    for (int q = threadIdx.x; q < NV * nx; q += NT) racc[q] = 0.;
+    __syncthreads();
+    const int64_t t_beg = ts * rw.span;
+    const int64_t t_end = min(t_beg + (int64_t)rw.span, in.n_t);
+    const int32_t *gd = rw.gdet + (int64_t)gi * rw.G;
+    for (int64_t tb = t_beg; tb < t_end; tb += NT) {
+        const int64_t t = tb + threadIdx.x;
+        const bool valid = t < t_end;
+        const Q4 qb = valid ? load_q(in.bore + 4 * t) : Q4{1., 0., 0., 0.};
+        for (int j = 0; j < rw.G; j++) {
+            const int32_t d = gd[j];
+            if (d < 0) break;
+            float det_wt = 1.f;
+            if (det_weights != nullptr) {
+                det_wt = det_weights[d];
+                if (det_wt == 0.f) continue;   // skips the detector (Projection.cxx:1798)
+            }
+            int64_t key = -1;
+            int iy = -1, ix = 0;
+            double v[NV];
+#pragma unroll
+            for (int c = 0; c < NV; c++) v[c] = 0.;
+            if (valid) {
+                const Q4 qd = load_q(in.dets + 4 * d);
+                double co[4];
+                get_coords<PROJ>(qb, qd, in.tabs, co);
+                float pf[3];
+                spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
+                Pt p[4];
+                if (get_pixels<PK_FLAT, 0>(g, co, p) > 0) {
+                    iy = p[0].i0;
+                    ix = p[0].i1;
+                    key = (int64_t)iy * nx + ix;
+                    if constexpr (W) {
+                        int k = 0;
+#pragma unroll
+                        for (int a = 0; a < NC; a++)
+#pragma unroll
+                            for (int b = a; b < NC; b++, k++)
+                                v[k] = (double)(pf[a] * pf[b] * p[0].w * p[0].w * det_wt);
+                    } else {
+                        const float s = sig[d * in.n_t + t];
+#pragma unroll
+                        for (int c = 0; c < NC; c++)
+                            v[c] = (double)(s * pf[c] * p[0].w * det_wt);
+                    }
+                }
+            }
+            if (wave_runs<NV>(key, v) && key >= 0) {
+                if (iy == row) {
+#pragma unroll
+                    for (int c = 0; c < NV; c++) atomicAdd(racc + c * nx + ix, v[c]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < NV; c++) {
+                        const int lead = W ? hash_lead<NC>(c) : c;
+                        unsafeAtomicAdd(m.map + lead * g.npix + key, v[c]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (row < 0) return;
+    double *mrow = m.map + (int64_t)row * nx;
+    for (int x = threadIdx.x; x < nx; x += NT) {
+#pragma unroll
+        for (int c = 0; c < NV; c++) {
+            const double v = racc[c * nx + x];
+            const int lead = W ? hash_lead<NC>(c) : c;
+            if (v != 0.) unsafeAtomicAdd(mrow + lead * g.npix + x, v);
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
 // tile_hits and per-sample domains (thread-assignment helpers).
 // --------------------------------------------------------------------------
 // A wave's samples mostly fall in one or two tiles: the active lanes that

@@ -37,9 +37,11 @@ struct OpArgs {
     int32_t *dom_out;
     int32_t *hits;
     // LDS-privatised scatter (to_map / to_weights): hash_cls 0 = global
-    // atomics, 1 = large tables, 2 = small tables (launch_scatter_cls).
+    // atomics, 1 = large tables, 2 = small tables (launch_scatter_cls),
+    // 3 = map rows in LDS (launch_scatter_row).
     int hash_cls;
     HashWork hw;
+    RowWork rw;
     // batch ops
     int batch_slots = 0;       // LDS table of batch to_map: 0 none, 512, 1024
     const ObsDev *obs;
@@ -75,6 +77,21 @@ void launch_scatter_cls(const OpArgs &a) {
         launch_scatter_lds<PROJ, PK, INTERP, NC, W, 256, 512, true>(a);
     else
         launch_scatter_lds<PROJ, PK, INTERP, NC, W, 1024, hash_slots(NV, 112 * 1024), false>(a);
+}
+
+// Rows in LDS: 1024 threads, the row's NV x nx float64 sums as dynamic LDS
+// (non-tiled flat maps, nearest-neighbour pixels).
+template <int PROJ, int NC, bool W>
+void launch_scatter_row(const OpArgs &a) {
+    constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
+    constexpr int NT = 1024;
+    const size_t lds = (size_t)NV * a.g.nx * sizeof(double);
+    const dim3 grid((unsigned)(((a.in.n_t + a.rw.span - 1) / a.rw.span) * a.rw.n_groups));
+    MapView<double> m{(double *)a.map, nullptr, nullptr};
+    (void)hipFuncSetAttribute((const void *)k_scatter_row<PROJ, NC, W, NT>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_scatter_row<PROJ, NC, W, NT>), grid, dim3(NT), lds, a.stream, a.in,
+                       a.g, m, a.resp, a.sig_in, a.det_w, a.rw);
 }
 
 // Grid of the on-the-fly kernels: one block per chunk (segment mode), or
@@ -114,8 +131,16 @@ int launch_map_ops(const OpArgs &a) {
         }
         return 0;
     }
+    if constexpr (PK == PK_FLAT && INTERP == 0) {
+        if ((a.op == OP_TO_MAP || a.op == OP_TO_WEIGHTS) && a.prec == 0 && a.hash_cls == 3 &&
+            a.in.chunks == nullptr) {
+            if (a.op == OP_TO_MAP) launch_scatter_row<PROJ, NC, false>(a);
+            else launch_scatter_row<PROJ, NC, true>(a);
+            return 0;
+        }
+    }
     if ((a.op == OP_TO_MAP || a.op == OP_TO_WEIGHTS) && a.prec == 0 && a.hash_cls > 0 &&
-        a.in.chunks == nullptr) {
+        a.hash_cls < 3 && a.in.chunks == nullptr) {
         if (a.op == OP_TO_MAP) launch_scatter_cls<PROJ, PK, INTERP, NC, false>(a);
         else launch_scatter_cls<PROJ, PK, INTERP, NC, true>(a);
         return 0;

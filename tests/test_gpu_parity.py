@@ -383,7 +383,7 @@ def test_projectionist_torch_path(gpu):
 
 
 @pytest.mark.parametrize('agg,xcd,lds', [(0, 0, 0), (1, 1, 0), (0, 1, 1), (1, 0, 1), (1, 0, 2),
-                                         (1, 1, 2), (1, 0, 3), (0, 0, 3)])
+                                         (1, 1, 2), (1, 0, 3), (0, 0, 3), (1, 0, 4)])
 def test_scatter_variants(gpu, agg, xcd, lds):
     """Every scatter variant (wave-level run aggregation on/off, XCD-private
     partial maps on/off, LDS-privatised tables never / where they pay /
@@ -427,6 +427,63 @@ def test_scatter_variants(gpu, agg, xcd, lds):
     finally:
         for n, v in zip(names, old):
             _native.set_option(n, v)
+
+
+@pytest.mark.parametrize('proj,comps,clip', [('CAR', 'TQU', False), ('CAR', 'TQU', True),
+                                              ('CAR', 'QU', False), ('CAR', 'T', True),
+                                              ('ZEA', 'TQU', False), ('TAN', 'TQU', True)])
+@pytest.mark.parametrize('dets,span', [(8, 16384), (1, 64), (3, 1000), (5, 2500)])
+def test_row_scatter(gpu, proj, comps, clip, dets, span):
+    """On-the-fly to_map / to_weights with map rows in LDS (scatter_hash 4):
+    detector groups of 1, 3, 5 and 8 sharing a home row, spans of 64 to
+    16384 samples (partial groups and spans), samples off the map (clip) and
+    on other rows than their group's (the zenithal projections' curved
+    rows) added to the map directly, a detector of weight 0; maps equal the
+    oracle's."""
+    from so3g_amd import _native
+    names = ('scatter_hash', 'hash_dets', 'hash_span')
+    old = [_native.get_option(n) for n in names]
+    try:
+        for n, v in zip(names, (4, dets, span)):
+            _native.set_option(n, v)
+        c = Case(proj, comps=comps, n_det=23, n_samp=7001, ny=120, nx=400, clip=clip)
+        nc = len(comps)
+        dw = c.det_weights.copy()
+        dw[4] = 0.
+        for w in (None, dw):
+            g = c.engine.to_map(None, c.bore, c.dets, c.resps, c.signal, w)
+            r = oracle.to_map(c.og, c.zeros([nc]), c.bore, c.dets, c.resps, c.signal, w)
+            maps_close(g, r, f'row to_map {proj} {comps}')
+            g = c.engine.to_weight_map(None, c.bore, c.dets, c.resps, w)
+            r = oracle.to_weights(c.og, c.zeros([nc, nc]), c.bore, c.dets, c.resps, w)
+            maps_close(g, r, f'row to_weights {proj} {comps}')
+        # accumulating into a non-zero map
+        m0 = c.map_like([nc])
+        g = c.engine.to_map(copy_map(m0), c.bore, c.dets, c.resps, c.signal)
+        r = oracle.to_map(c.og, copy_map(m0), c.bore, c.dets, c.resps, c.signal)
+        maps_close(g, r, f'row to_map accumulate {proj} {comps}')
+    finally:
+        for n, v in zip(names, old):
+            _native.set_option(n, v)
+
+
+def test_row_scatter_wide_map(gpu):
+    """A map too wide for a row of to_weights sums in LDS (6 x 4000 float64
+    = 192 KB) takes the other scatter in mode 4; the maps equal the
+    oracle's."""
+    from so3g_amd import _native
+    old = _native.get_option('scatter_hash')
+    _native.set_option('scatter_hash', 4)
+    try:
+        c = Case('CAR', comps='TQU', n_det=7, n_samp=6000, ny=300, nx=4000)
+        g = c.engine.to_weight_map(None, c.bore, c.dets, c.resps)
+        r = oracle.to_weights(c.og, c.zeros([3, 3]), c.bore, c.dets, c.resps)
+        maps_close(g, r, 'wide to_weights')
+        g = c.engine.to_map(None, c.bore, c.dets, c.resps, c.signal)
+        r = oracle.to_map(c.og, c.zeros([3]), c.bore, c.dets, c.resps, c.signal)
+        maps_close(g, r, 'wide to_map (row fits: 3 x 4000 float64)')
+    finally:
+        _native.set_option('scatter_hash', old)
 
 
 @pytest.mark.parametrize('mode', [2, 3])
