@@ -511,12 +511,12 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   (1024 threads, 112 KB), 4 the map row of a group of
  *                   detectors held in LDS where the map allows it
  *                   (non-tiled flat, nearest neighbour, the row's sums
- *                   within 160 KB; else as 1).  Mode 1 uses the rows for
- *                   to_weights where the map allows them.  Precomp to_map /
+ *                   within 160 KB; else as 1).  Mode 1 uses the rows
+ *                   where the map allows them.  Precomp to_map /
  *                   to_weights use the small tables in modes 1, 2 and 4
  *                   (tiled maps: to_weights only in modes 1 and 4)
  *   "hash_dets", "hash_span"  detectors and samples per block of modes 2
- *                   to 4 (0, the default: 4 x 256, 2 x 32768 and 2 x 16384);
+ *                   to 4 (0, the default: 4 x 256, 2 x 32768 and 2 x 65536);
  *                   also the detectors per block of the Precomp tables (0:
  *                   measured per case, 2 to 8)
  *   "pc_from_dpb"   detectors per block of the Precomp gather (default 2)

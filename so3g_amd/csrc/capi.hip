@@ -110,11 +110,11 @@ struct Options {
     // flat to_map, the table per case), 2 always with the small table
     // (256 threads, 512 slots), 3 always with the large one (1024 threads,
     // 112 KB), 4 map rows in LDS where the geometry allows it (non-tiled
-    // flat maps, nearest neighbour; else as 1).  Mode 1 takes the rows for
-    // to_weights where the geometry allows them.
+    // flat maps, nearest neighbour; else as 1).  Mode 1 takes the rows
+    // where the geometry allows them.
     int scatter_hash = 1;
     int hash_dets = 0;         // detectors per block in modes 2-4 (0: 4 / 2 / 2)
-    int hash_span = 0;         // samples per block in modes 2-4 (0: 256 / 32768 / 16384)
+    int hash_span = 0;         // samples per block in modes 2-4 (0: 256 / 32768 / 65536)
 };
 Options g_opt;
 constexpr int kNumXcd = 8;
@@ -668,15 +668,14 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
     const bool row_ok = g->pixelizor == SO3G_PIX_FLAT && !g->tiled && g->interp == 0 &&
                         prec == SO3G_PREC_REF && segs == nullptr &&
                         (int64_t)nv_row * g->naxis[1] * 8 <= 160 * 1024;
-    // Map rows in LDS (k_scatter_row) for to_weights in mode 1 (1000 x 720000
-    // samples, CAR 0.5' TQU: 8.2 ms against 11.7 with the small tables;
-    // groups of 2 detectors over 16384 samples; 1, 3, 4, 6 detectors 8.4 -
-    // 8.5 ms, spans of 4096 / 8192 9.0 / 8.7 ms), and for to_map too in mode
-    // 4 (9.0 ms against 8.7 with the small tables, so not in mode 1).
-    if (row_ok && (g_opt.scatter_hash == 4 ||
-                   (g_opt.scatter_hash == 1 && op == OP_TO_WEIGHTS))) {
+    // Map rows in LDS (k_scatter_row) in modes 1 and 4 (1000 x 720000
+    // samples, CAR 0.5' TQU: to_map 7.2 ms against 8.7 with the small
+    // tables, to_weights 6.9 against 11.7; groups of 2 detectors over 65536
+    // samples: 4 / 8 detectors 7.4 / 7.7 ms, spans of 16384 / 32768 7.5 /
+    // 7.35 ms for to_map).
+    if (row_ok && (g_opt.scatter_hash == 4 || g_opt.scatter_hash == 1)) {
         a.hash_cls = 3;
-        a.rw.span = g_opt.hash_span > 0 ? g_opt.hash_span : 16384;
+        a.rw.span = g_opt.hash_span > 0 ? g_opt.hash_span : 65536;
         r = row_groups(g, a, g_opt.hash_dets > 0 ? g_opt.hash_dets : 2, a.rw);
         if (r) return r;
         while (((n_t + a.rw.span - 1) / a.rw.span) * (int64_t)a.rw.n_groups > INT32_MAX)

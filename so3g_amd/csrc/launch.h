@@ -79,8 +79,8 @@ void launch_scatter_cls(const OpArgs &a) {
         launch_scatter_lds<PROJ, PK, INTERP, NC, W, 1024, hash_slots(NV, 112 * 1024), false>(a);
 }
 
-// Rows in LDS: 1024 threads, the row's NV x nx float64 sums as dynamic LDS
-// (non-tiled flat maps, nearest-neighbour pixels).
+// Rows in LDS: 1024 threads of 4 samples each, the row's NV x nx float64
+// sums as dynamic LDS (non-tiled flat maps, nearest-neighbour pixels).
 template <int PROJ, int NC, bool W>
 void launch_scatter_row(const OpArgs &a) {
     constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
@@ -88,10 +88,10 @@ void launch_scatter_row(const OpArgs &a) {
     const size_t lds = (size_t)NV * a.g.nx * sizeof(double);
     const dim3 grid((unsigned)(((a.in.n_t + a.rw.span - 1) / a.rw.span) * a.rw.n_groups));
     MapView<double> m{(double *)a.map, nullptr, nullptr};
-    (void)hipFuncSetAttribute((const void *)k_scatter_row<PROJ, NC, W, NT>,
+    (void)hipFuncSetAttribute((const void *)k_scatter_row<PROJ, NC, W, NT, 4>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_scatter_row<PROJ, NC, W, NT>), grid, dim3(NT), lds, a.stream, a.in,
-                       a.g, m, a.resp, a.sig_in, a.det_w, a.rw);
+    hipLaunchKernelGGL((k_scatter_row<PROJ, NC, W, NT, 4>), grid, dim3(NT), lds, a.stream,
+                       a.in, a.g, m, a.resp, a.sig_in, a.det_w, a.rw);
 }
 
 // Grid of the on-the-fly kernels: one block per chunk (segment mode), or

@@ -432,14 +432,14 @@ def test_scatter_variants(gpu, agg, xcd, lds):
 @pytest.mark.parametrize('proj,comps,clip', [('CAR', 'TQU', False), ('CAR', 'TQU', True),
                                               ('CAR', 'QU', False), ('CAR', 'T', True),
                                               ('ZEA', 'TQU', False), ('TAN', 'TQU', True)])
-@pytest.mark.parametrize('dets,span', [(8, 16384), (1, 64), (3, 1000), (5, 2500)])
+@pytest.mark.parametrize('dets,span', [(2, 65536), (8, 16384), (1, 64), (3, 1000), (5, 2500)])
 def test_row_scatter(gpu, proj, comps, clip, dets, span):
     """On-the-fly to_map / to_weights with map rows in LDS (scatter_hash 4):
     detector groups of 1, 3, 5 and 8 sharing a home row, spans of 64 to
     16384 samples (partial groups and spans), samples off the map (clip) and
     on other rows than their group's (the zenithal projections' curved
-    rows) added to the map directly, a detector of weight 0; maps equal the
-    oracle's."""
+    rows) added to the map directly, a detector of weight 0, 7001 samples
+    (a partial last group of 4 samples per lane); maps equal the oracle's."""
     from so3g_amd import _native
     names = ('scatter_hash', 'hash_dets', 'hash_span')
     old = [_native.get_option(n) for n in names]
