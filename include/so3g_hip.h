@@ -510,11 +510,15 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   (256 threads, 512 slots), 3 always with large ones
  *                   (1024 threads, 112 KB), 4 the map row of a group of
  *                   detectors held in LDS where the map allows it
- *                   (non-tiled flat, nearest neighbour, the row's sums
- *                   within 160 KB; else as 1).  Mode 1 uses the rows
+ *                   (flat, tiled or not, nearest neighbour, the row's
+ *                   sums within 160 KB; else as 1).  Mode 1 uses the rows
  *                   where the map allows them.  Precomp to_map /
  *                   to_weights use the small tables in modes 1, 2 and 4
  *                   (tiled maps: to_weights only in modes 1 and 4)
+ *   "hash_r"        samples per lane of the hash tables in modes 2 and 3
+ *                   (1 default: runs summed across the wave; 4: four
+ *                   consecutive samples per lane, runs summed in the lane;
+ *                   mode 1 takes 4 for HEALPix to_map)
  *   "hash_dets", "hash_span"  detectors and samples per block of modes 2
  *                   to 4 (0, the default: 4 x 256, 2 x 32768 and 2 x 65536);
  *                   also the detectors per block of the Precomp tables (0:

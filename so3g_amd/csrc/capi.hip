@@ -115,6 +115,7 @@ struct Options {
     int scatter_hash = 1;
     int hash_dets = 0;         // detectors per block in modes 2-4 (0: 4 / 2 / 2)
     int hash_span = 0;         // samples per block in modes 2-4 (0: 256 / 32768 / 65536)
+    int hash_r = 1;            // hash tables: samples per lane (1: wave-level runs, 4)
 };
 Options g_opt;
 constexpr int kNumXcd = 8;
@@ -663,16 +664,18 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
     // four row-mates over one 256-sample chunk and flushes consecutive
     // pixels together, and keeps more blocks per compute unit than the
     // large one (1024 threads, 112 KB, a span of 32768 samples).
-    int cls = 0, per_block = 0, span = 0;
+    int cls = 0, per_block = 0, span = 0, lane_r = g_opt.hash_r;
     const int nv_row = op == OP_TO_MAP ? g->ncomp : g->ncomp * (g->ncomp + 1) / 2;
-    const bool row_ok = g->pixelizor == SO3G_PIX_FLAT && !g->tiled && g->interp == 0 &&
+    const bool row_ok = g->pixelizor == SO3G_PIX_FLAT && g->interp == 0 &&
                         prec == SO3G_PREC_REF && segs == nullptr &&
-                        (int64_t)nv_row * g->naxis[1] * 8 <= 160 * 1024;
+                        (int64_t)nv_row * g->naxis[1] * 8 + (g->tiled ? g->naxis[1] : 0) <=
+                            160 * 1024;
     // Map rows in LDS (k_scatter_row) in modes 1 and 4 (1000 x 720000
     // samples, CAR 0.5' TQU: to_map 7.2 ms against 8.7 with the small
     // tables, to_weights 6.9 against 11.7; groups of 2 detectors over 65536
     // samples: 4 / 8 detectors 7.4 / 7.7 ms, spans of 16384 / 32768 7.5 /
-    // 7.35 ms for to_map).
+    // 7.35 ms for to_map; in 100 x 100 tiles to_map 7.7 against 11.2,
+    // to_weights 7.6 against 12.4).
     if (row_ok && (g_opt.scatter_hash == 4 || g_opt.scatter_hash == 1)) {
         a.hash_cls = 3;
         a.rw.span = g_opt.hash_span > 0 ? g_opt.hash_span : 65536;
@@ -682,11 +685,15 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
             a.rw.span *= 2;
         r = run_dispatch(g->proj, a);
         if (r) return r;
-        return SO3G_OK;
+        return g->tiled ? check_tile_flag(a.stream) : SO3G_OK;
     }
     if (g_opt.scatter_hash == 1 || g_opt.scatter_hash == 4) {
         if (g->pixelizor == SO3G_PIX_HEALPIX) {
-            if (op == OP_TO_MAP) cls = 1, per_block = 2, span = 32768;
+            // to_map: 4 samples per lane with in-lane run sums (13.7 ms
+            // against 14.4 with the wave-level scan; with the small tables
+            // and to_weights 4 samples per lane overflow the 512 slots and
+            // lose)
+            if (op == OP_TO_MAP) cls = 1, per_block = 2, span = 32768, lane_r = 4;
             else cls = 2, per_block = 4, span = 256;
         } else if (op == OP_TO_WEIGHTS) {
             cls = 2, per_block = g->tiled ? 4 : 8, span = 256;
@@ -704,6 +711,7 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
         a.hash_cls = cls;
         a.hw.dets = per_block;
         a.hw.span = span;
+        a.hw.r = g->interp == 0 ? lane_r : 1;
         // the 1-D grid of (group, span) blocks stays below 2^31
         while (((n_t + a.hw.span - 1) / a.hw.span) * ((n_det + a.hw.dets - 1) / a.hw.dets) >
                INT32_MAX)
@@ -1109,6 +1117,9 @@ int so3g_set_option(const char *name, int64_t value) {
     else if (n == "scatter_hash") {
         if (value < 0 || value > 4) return fail(SO3G_ERR_VALUE, "scatter_hash must be 0..4");
         g_opt.scatter_hash = (int)value;
+    } else if (n == "hash_r") {
+        if (value != 1 && value != 4) return fail(SO3G_ERR_VALUE, "hash_r must be 1 or 4");
+        g_opt.hash_r = (int)value;
     } else if (n == "hash_dets") {
         if (value < 0 || value > 4096)
             return fail(SO3G_ERR_VALUE, "hash_dets must be 0 (default) or 1..4096");
@@ -1165,6 +1176,7 @@ int so3g_get_option(const char *name, int64_t *value) {
     else if (n == "scatter_xcd") *value = g_opt.scatter_xcd;
     else if (n == "pc_from_dpb") *value = g_opt.pc_from_dpb;
     else if (n == "scatter_hash") *value = g_opt.scatter_hash;
+    else if (n == "hash_r") *value = g_opt.hash_r;
     else if (n == "hash_dets") *value = g_opt.hash_dets;
     else if (n == "hash_span") *value = g_opt.hash_span;
     else if (n == "xcd_budget_mb") *value = g_opt.xcd_budget_mb;

@@ -488,6 +488,7 @@ struct HashWork {
     const int32_t *order;   // detector order, n_det entries
     int32_t dets;           // G: detectors per block
     int32_t span;           // T: samples per block
+    int32_t r;              // samples per lane: 1 (runs across the wave) or 4
 };
 
 constexpr int32_t kHashEmpty = -1;
@@ -648,6 +649,83 @@ __device__ __forceinline__ void lds_scatter_det(const PtgIn &in, const Geo &g,
     }
 }
 
+// One detector's R consecutive samples t0 .. t0 + R - 1 of this lane into
+// the block's table (nearest-neighbour pixels): the lane sums its own runs
+// of equal keys in registers and adds each run to the table (or to the map
+// when no slot is found), without a wave-level scan.
+template <int PROJ, int PK, int NC, bool W, int H, bool MOD, int R>
+__device__ __forceinline__ void lds_scatter_det_r(const PtgIn &in, const Geo &g,
+                                                  const MapView<double> &m, const float *resp,
+                                                  const float *sig, const float *det_weights,
+                                                  int *flags, int64_t d, int64_t t0,
+                                                  int64_t t_end, const Q4 (&qb)[R],
+                                                  int32_t *hk, double *hv) {
+    constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
+    float det_wt = 1.f;
+    if (det_weights != nullptr) {
+        det_wt = det_weights[d];
+        if (det_wt == 0.f) return;
+    }
+    const Q4 qd = load_q(in.dets + 4 * d);
+    const float r0 = resp[2 * d], r1 = resp[2 * d + 1];
+    int32_t ck = -1;
+    double cv[NV];
+#pragma unroll
+    for (int c = 0; c < NV; c++) cv[c] = 0.;
+    auto emit = [&](int32_t key, const double (&v)[NV]) {
+        if (key < 0) return;
+        if (!hash_add<NV, H, MOD>(hk, hv, key, v)) {
+            int64_t plane;
+            double *e = hash_elem<PK>(g, m, key, plane);
+#pragma unroll
+            for (int c = 0; c < NV; c++) {
+                const int lead = W ? hash_lead<NC>(c) : c;
+                unsafeAtomicAdd(e + lead * plane, v[c]);
+            }
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (t0 + r >= t_end) break;
+        double co[4];
+        get_coords<PROJ>(qb[r], qd, in.tabs, co);
+        float pf[3];
+        spin_factors(NC, co, r0, r1, pf);
+        Pt p[4];
+        if (get_pixels<PK, 0>(g, co, p) == 0) continue;
+        if constexpr (PK == PK_FLAT_TILED || PK == PK_HP_TILED) {
+            if (m.tiles[p[0].i0] == nullptr) {
+                raise_flag(flags, FLAG_TILE);
+                continue;
+            }
+        }
+        double v[NV];
+        if constexpr (W) {
+            int k = 0;
+#pragma unroll
+            for (int a = 0; a < NC; a++)
+#pragma unroll
+                for (int b = a; b < NC; b++, k++)
+                    v[k] = (double)(pf[a] * pf[b] * p[0].w * p[0].w * det_wt);
+        } else {
+            const float s = sig[d * in.n_t + t0 + r];
+#pragma unroll
+            for (int c = 0; c < NC; c++) v[c] = (double)(s * pf[c] * p[0].w * det_wt);
+        }
+        const int32_t key = hash_key<PK>(g, p[0]);
+        if (key != ck) {
+            emit(ck, cv);
+            ck = key;
+#pragma unroll
+            for (int c = 0; c < NV; c++) cv[c] = v[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < NV; c++) cv[c] += v[c];
+        }
+    }
+    emit(ck, cv);
+}
+
 // Adds the table's occupied slots to the map, one FP64 atomic per value.
 template <int PK, int NC, bool W, int H>
 __device__ __forceinline__ void lds_table_flush(const Geo &g, const MapView<double> &m,
@@ -666,7 +744,10 @@ __device__ __forceinline__ void lds_table_flush(const Geo &g, const MapView<doub
     }
 }
 
-template <int PROJ, int PK, int INTERP, int NC, bool W, int NT, int H, bool MOD>
+// R = 1: one sample per lane, runs summed across the wave (wave_runs);
+// R = 4 (nearest neighbour only): four consecutive samples per lane, runs
+// summed in the lane (lds_scatter_det_r).
+template <int PROJ, int PK, int INTERP, int NC, bool W, int NT, int H, bool MOD, int R>
 __global__ __launch_bounds__(NT) void k_scatter_lds(PtgIn in, Geo g, MapView<double> m,
                                                     const float *resp, const float *sig,
                                                     const float *det_weights, HashWork hw,
@@ -684,21 +765,36 @@ __global__ __launch_bounds__(NT) void k_scatter_lds(PtgIn in, Geo g, MapView<dou
     const int nd = (int)min((int64_t)hw.dets, in.n_det - g0);
     const int64_t t_beg = ts * hw.span;
     const int64_t t_end = min(t_beg + (int64_t)hw.span, in.n_t);
-    for (int64_t tb = t_beg; tb < t_end; tb += NT) {
-        const int64_t t = tb + threadIdx.x;
-        const bool valid = t < t_end;
-        const Q4 qb = valid ? load_q(in.bore + 4 * t) : Q4{1., 0., 0., 0.};
-        for (int j = 0; j < nd; j++)
-            lds_scatter_det<PROJ, PK, INTERP, NC, W, H, MOD>(in, g, m, resp, sig, det_weights,
-                                                             flags, hw.order[g0 + j], t, qb,
-                                                             valid, hk, hv);
+    if constexpr (R == 1 || INTERP != 0) {
+        for (int64_t tb = t_beg; tb < t_end; tb += NT) {
+            const int64_t t = tb + threadIdx.x;
+            const bool valid = t < t_end;
+            const Q4 qb = valid ? load_q(in.bore + 4 * t) : Q4{1., 0., 0., 0.};
+            for (int j = 0; j < nd; j++)
+                lds_scatter_det<PROJ, PK, INTERP, NC, W, H, MOD>(in, g, m, resp, sig,
+                                                                 det_weights, flags,
+                                                                 hw.order[g0 + j], t, qb,
+                                                                 valid, hk, hv);
+        }
+    } else {
+        for (int64_t tb = t_beg; tb < t_end; tb += (int64_t)NT * R) {
+            const int64_t t0 = tb + (int64_t)threadIdx.x * R;
+            Q4 qb[R];
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                qb[r] = t0 + r < t_end ? load_q(in.bore + 4 * (t0 + r)) : Q4{1., 0., 0., 0.};
+            for (int j = 0; j < nd; j++)
+                lds_scatter_det_r<PROJ, PK, NC, W, H, MOD, R>(in, g, m, resp, sig, det_weights,
+                                                              flags, hw.order[g0 + j], t0,
+                                                              t_end, qb, hk, hv);
+        }
     }
     __syncthreads();
     lds_table_flush<PK, NC, W, H>(g, m, hk, hv, NT);
 }
 
 // --------------------------------------------------------------------------
-// to_map / to_weights through a map row held in LDS (non-tiled flat maps).
+// to_map / to_weights through a map row held in LDS (flat maps).
 //
 // A constant-elevation scan in horizon coordinates keeps each detector on
 // one map row.  A block owns a group of up to G detectors that share a home
@@ -708,8 +804,9 @@ __global__ __launch_bounds__(NT) void k_scatter_lds(PtgIn in, Geo g, MapView<dou
 // walks the group's detectors; each run of equal pixels is added to the row
 // in LDS, or to the map with float64 atomics when it lies on another row.
 // The row's touched pixels are then added to the map, one atomic per pixel
-// and value.  Sums are float64 sums of the same float32 products as
-// k_to_map / k_to_weights, in another order.
+// and value (tiled maps: the row crosses the tiles of one tile row, the
+// samples' tiles are checked as in k_to_map).  Sums are float64 sums of the
+// same float32 products as k_to_map / k_to_weights, in another order.
 // --------------------------------------------------------------------------
 struct RowWork {
     const int32_t *gdet;    // [n_groups * G] detectors of each group, -1 padded
@@ -723,32 +820,66 @@ struct RowWork {
 // sample per lane (wave_runs) measured slower: to_map 9.0 vs 7.2 ms,
 // to_weights 8.2 vs 6.9 ms on 7.2e8 samples; R = 8 needs more than 128
 // registers and measured 7.5 / 7.7 ms.
-template <int PROJ, int NC, bool W, int NT, int R>
+template <int PROJ, int PK, int NC, bool W, int NT, int R>
 __global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<double> m,
-                                                      const float *resp, const float *sig,
-                                                      const float *det_weights, RowWork rw) {
+                                                    const float *resp, const float *sig,
+                                                    const float *det_weights, RowWork rw,
+                                                    int *flags) {
     constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
-    extern __shared__ double racc[];   // [NV][nx]
+    constexpr bool TILED = PK == PK_FLAT_TILED;
+    extern __shared__ double racc[];   // [NV][nx] sums, then (tiled) nx active flags
+    uint8_t *ract = reinterpret_cast<uint8_t *>(racc + NV * g.nx);
     const int nx = g.nx;
     const int64_t ts = (int64_t)blockIdx.x / rw.n_groups;
     const int gi = (int)((int64_t)blockIdx.x - ts * rw.n_groups);
     const int row = rw.grow[gi];
     for (int q = threadIdx.x; q < NV * nx; q += NT) racc[q] = 0.;
+    if constexpr (TILED) {
+        if (row >= 0) {
+            const int t0 = (row / g.ty) * g.n_tx;
+            for (int x = threadIdx.x; x < nx; x += NT)
+                ract[x] = m.tiles[t0 + x / g.tx] != nullptr;
+        }
+    }
     __syncthreads();
     const int64_t t_beg = ts * rw.span;
     const int64_t t_end = min(t_beg + (int64_t)rw.span, in.n_t);
     const int32_t *gd = rw.gdet + (int64_t)gi * rw.G;
-    // adds one run's sums: to the row in LDS, or to the map off the row
-    auto emit = [&](int64_t key, int iy, int ix, const double (&v)[NV]) {
-        if (key < 0) return;
+    // adds one run's sums (map row iy, column ix): to the row in LDS, or to
+    // the map off the row; a run in an inactive tile raises the tile flag
+    auto emit = [&](int iy, int ix, bool on, const double (&v)[NV]) {
+        if (!on) return;
         if (iy == row) {
+            if constexpr (TILED) {
+                if (!ract[ix]) {
+                    raise_flag(flags, FLAG_TILE);
+                    return;
+                }
+            }
 #pragma unroll
             for (int c = 0; c < NV; c++) atomicAdd(racc + c * nx + ix, v[c]);
         } else {
+            double *e;
+            int64_t plane;
+            if constexpr (TILED) {
+                const int sy = iy / g.ty, sx = ix / g.tx;
+                const int t = sy * g.n_tx + sx;
+                e = m.tiles[t];
+                if (e == nullptr) {
+                    raise_flag(flags, FLAG_TILE);
+                    return;
+                }
+                const int tw = m.tile_dims[2 * t + 1];
+                plane = (int64_t)m.tile_dims[2 * t] * tw;
+                e += (int64_t)(iy - sy * g.ty) * tw + (ix - sx * g.tx);
+            } else {
+                e = m.map + (int64_t)iy * nx + ix;
+                plane = g.npix;
+            }
 #pragma unroll
             for (int c = 0; c < NV; c++) {
                 const int lead = W ? hash_lead<NC>(c) : c;
-                unsafeAtomicAdd(m.map + lead * g.npix + key, v[c]);
+                unsafeAtomicAdd(e + lead * plane, v[c]);
             }
         }
     };
@@ -768,7 +899,7 @@ __global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<dou
             }
             const Q4 qd = load_q(in.dets + 4 * d);
             const float r0 = resp[2 * d], r1 = resp[2 * d + 1];
-            int64_t ck = -1;
+            bool con = false;
             int cy = -1, cx = 0;
             double cv[NV];
 #pragma unroll
@@ -780,8 +911,19 @@ __global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<dou
                 get_coords<PROJ>(qb[r], qd, in.tabs, co);
                 float pf[3];
                 spin_factors(NC, co, r0, r1, pf);
-                Pt p[4];
-                if (get_pixels<PK_FLAT, 0>(g, co, p) == 0) continue;
+                // the pixel of get_pixels<PK, 0> as (row, column) of the
+                // whole map; tiled maps truncate before the bounds test
+                const double fx = flat_x(g, co), fy = flat_y(g, co);
+                int gx, gy;
+                if constexpr (TILED) {
+                    gx = int(fx);
+                    gy = int(fy);
+                    if (gx < 0 || gx >= g.nx || gy < 0 || gy >= g.ny) continue;
+                } else {
+                    if (fx < 0 || fx >= g.nx || fy < 0 || fy >= g.ny) continue;
+                    gx = int(fx);
+                    gy = int(fy);
+                }
                 double v[NV];
                 if constexpr (W) {
                     int k = 0;
@@ -789,18 +931,17 @@ __global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<dou
                     for (int a = 0; a < NC; a++)
 #pragma unroll
                         for (int b = a; b < NC; b++, k++)
-                            v[k] = (double)(pf[a] * pf[b] * p[0].w * p[0].w * det_wt);
+                            v[k] = (double)(pf[a] * pf[b] * det_wt);
                 } else {
                     const float s = sig[d * in.n_t + t0 + r];
 #pragma unroll
-                    for (int c = 0; c < NC; c++) v[c] = (double)(s * pf[c] * p[0].w * det_wt);
+                    for (int c = 0; c < NC; c++) v[c] = (double)(s * pf[c] * det_wt);
                 }
-                const int64_t key = (int64_t)p[0].i0 * nx + p[0].i1;
-                if (key != ck) {
-                    emit(ck, cy, cx, cv);
-                    ck = key;
-                    cy = p[0].i0;
-                    cx = p[0].i1;
+                if (!(con && gy == cy && gx == cx)) {
+                    emit(cy, cx, con, cv);
+                    con = true;
+                    cy = gy;
+                    cx = gx;
 #pragma unroll
                     for (int c = 0; c < NV; c++) cv[c] = v[c];
                 } else {
@@ -808,18 +949,30 @@ __global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<dou
                     for (int c = 0; c < NV; c++) cv[c] += v[c];
                 }
             }
-            emit(ck, cy, cx, cv);
+            emit(cy, cx, con, cv);
         }
     }
     __syncthreads();
     if (row < 0) return;
-    double *mrow = m.map + (int64_t)row * nx;
     for (int x = threadIdx.x; x < nx; x += NT) {
+        double *e;
+        int64_t plane;
+        if constexpr (TILED) {
+            if (!ract[x]) continue;   // nothing was added there
+            const int sy = row / g.ty, sx = x / g.tx;
+            const int t = sy * g.n_tx + sx;
+            const int tw = m.tile_dims[2 * t + 1];
+            plane = (int64_t)m.tile_dims[2 * t] * tw;
+            e = m.tiles[t] + (int64_t)(row - sy * g.ty) * tw + (x - sx * g.tx);
+        } else {
+            e = m.map + (int64_t)row * nx + x;
+            plane = g.npix;
+        }
 #pragma unroll
         for (int c = 0; c < NV; c++) {
             const double v = racc[c * nx + x];
             const int lead = W ? hash_lead<NC>(c) : c;
-            if (v != 0.) unsafeAtomicAdd(mrow + lead * g.npix + x, v);
+            if (v != 0.) unsafeAtomicAdd(e + lead * plane, v);
         }
     }
 }

@@ -63,8 +63,14 @@ void launch_scatter_lds(const OpArgs &a) {
     const dim3 grid((unsigned)(((a.in.n_t + a.hw.span - 1) / a.hw.span) *
                                ((a.in.n_det + a.hw.dets - 1) / a.hw.dets)));
     MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
-    hipLaunchKernelGGL((k_scatter_lds<PROJ, PK, INTERP, NC, W, NT, H, MOD>), grid, dim3(NT), 0,
-                       a.stream, a.in, a.g, m, a.resp, a.sig_in, a.det_w, a.hw, a.flags);
+    if (INTERP == 0 && a.hw.r == 4)
+        hipLaunchKernelGGL((k_scatter_lds<PROJ, PK, INTERP, NC, W, NT, H, MOD, 4>), grid,
+                           dim3(NT), 0, a.stream, a.in, a.g, m, a.resp, a.sig_in, a.det_w, a.hw,
+                           a.flags);
+    else
+        hipLaunchKernelGGL((k_scatter_lds<PROJ, PK, INTERP, NC, W, NT, H, MOD, 1>), grid,
+                           dim3(NT), 0, a.stream, a.in, a.g, m, a.resp, a.sig_in, a.det_w, a.hw,
+                           a.flags);
 }
 
 // hash_cls 1: 1024 threads and a 112 KB table per block (one block per
@@ -80,18 +86,19 @@ void launch_scatter_cls(const OpArgs &a) {
 }
 
 // Rows in LDS: 1024 threads of 4 samples each, the row's NV x nx float64
-// sums as dynamic LDS (non-tiled flat maps, nearest-neighbour pixels).
-template <int PROJ, int NC, bool W>
+// sums as dynamic LDS (flat maps, nearest-neighbour pixels).
+template <int PROJ, int PK, int NC, bool W>
 void launch_scatter_row(const OpArgs &a) {
     constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
     constexpr int NT = 1024;
-    const size_t lds = (size_t)NV * a.g.nx * sizeof(double);
+    const size_t lds = (size_t)NV * a.g.nx * sizeof(double) +
+                       (PK == PK_FLAT_TILED ? (size_t)a.g.nx : 0);
     const dim3 grid((unsigned)(((a.in.n_t + a.rw.span - 1) / a.rw.span) * a.rw.n_groups));
-    MapView<double> m{(double *)a.map, nullptr, nullptr};
-    (void)hipFuncSetAttribute((const void *)k_scatter_row<PROJ, NC, W, NT, 4>,
+    MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
+    (void)hipFuncSetAttribute((const void *)k_scatter_row<PROJ, PK, NC, W, NT, 4>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_scatter_row<PROJ, NC, W, NT, 4>), grid, dim3(NT), lds, a.stream,
-                       a.in, a.g, m, a.resp, a.sig_in, a.det_w, a.rw);
+    hipLaunchKernelGGL((k_scatter_row<PROJ, PK, NC, W, NT, 4>), grid, dim3(NT), lds, a.stream,
+                       a.in, a.g, m, a.resp, a.sig_in, a.det_w, a.rw, a.flags);
 }
 
 // Grid of the on-the-fly kernels: one block per chunk (segment mode), or
@@ -131,11 +138,11 @@ int launch_map_ops(const OpArgs &a) {
         }
         return 0;
     }
-    if constexpr (PK == PK_FLAT && INTERP == 0) {
+    if constexpr ((PK == PK_FLAT || PK == PK_FLAT_TILED) && INTERP == 0) {
         if ((a.op == OP_TO_MAP || a.op == OP_TO_WEIGHTS) && a.prec == 0 && a.hash_cls == 3 &&
             a.in.chunks == nullptr) {
-            if (a.op == OP_TO_MAP) launch_scatter_row<PROJ, NC, false>(a);
-            else launch_scatter_row<PROJ, NC, true>(a);
+            if (a.op == OP_TO_MAP) launch_scatter_row<PROJ, PK, NC, false>(a);
+            else launch_scatter_row<PROJ, PK, NC, true>(a);
             return 0;
         }
     }

@@ -486,6 +486,50 @@ def test_row_scatter_wide_map(gpu):
         _native.set_option('scatter_hash', old)
 
 
+@pytest.mark.parametrize('proj', ['CAR', 'ZEA'])
+@pytest.mark.parametrize('tile_shape', [(37, 53), (8, 400), (150, 16)])
+def test_row_scatter_tiled(gpu, proj, tile_shape):
+    """Map rows in LDS on tiled flat maps (scatter_hash 4): a row crosses the
+    tiles of one tile row (ragged edge tiles, a tile as wide as the map,
+    tiles one row of samples tall or 16 columns wide), inactive tiles None;
+    to_map / to_weights equal the oracle's, and a hit tile passed as None
+    raises RuntimeError."""
+    import so3g_amd
+    from so3g_amd import _native
+    names = ('scatter_hash', 'hash_dets', 'hash_span')
+    old = [_native.get_option(n) for n in names]
+    try:
+        for dets, span in ((2, 65536), (3, 1000)):
+            for n, v in zip(names, (4, dets, span)):
+                _native.set_option(n, v)
+            c = Case(proj, comps='TQU', n_det=23, n_samp=7001, ny=150, nx=400)
+            ty, tx = tile_shape
+            n_tile = ((150 + ty - 1) // ty) * ((400 + tx - 1) // tx)
+            cls = getattr(so3g_amd, f'ProjEng_{proj}_TQU_Tiled')
+            probe = cls(c.pixargs[:6] + (ty, tx))
+            hits = np.asarray(probe.tile_hits(c.bore, c.dets))
+            active = [i for i in range(n_tile) if hits[i] > 0]
+            args = c.pixargs[:6] + (ty, tx, active)
+            eng = cls(args)
+            og = oracle.geom_from_args(proj, 'TQU', args)
+            dw = c.det_weights.copy()
+            dw[4] = 0.
+            for w in (None, dw):
+                g = eng.to_map(None, c.bore, c.dets, c.resps, c.signal, w)
+                r = oracle.to_map(og, eng.zeros((3,)), c.bore, c.dets, c.resps, c.signal, w)
+                maps_close(g, r, f'tiled row to_map {proj} {tile_shape} {dets}')
+                g = eng.to_weight_map(None, c.bore, c.dets, c.resps, w)
+                r = oracle.to_weights(og, eng.zeros((3, 3)), c.bore, c.dets, c.resps, w)
+                maps_close(g, r, f'tiled row to_weights {proj} {tile_shape} {dets}')
+            m = eng.zeros((3,))
+            m[active[len(active) // 2]] = None
+            with pytest.raises(RuntimeError):
+                eng.to_map(m, c.bore, c.dets, c.resps, c.signal)
+    finally:
+        for n, v in zip(names, old):
+            _native.set_option(n, v)
+
+
 @pytest.mark.parametrize('mode', [2, 3])
 @pytest.mark.parametrize('proj,tiled,interp', [('CAR', False, 0), ('CAR', False, 1),
                                                ('CAR', True, 0), ('HP', True, 0)])

@@ -188,13 +188,14 @@ def test_plan_options():
     defaults = {'plan_layout': -1, 'plan_tile_item': 262144, 'plan_time_copy': 1,
                 'plan_to_lds': 1, 'plan_to_cpb': 2, 'plan_from_layout': 1, 'plan_from_v': 4,
                 'nmat_dchunk': 64, 'plan_win_len': 65536, 'plan_win_block': 1024,
-                'plan_win_flush': 1}
+                'plan_win_flush': 1, 'scatter_hash': 1, 'hash_r': 1}
     for name, v in defaults.items():
         assert _native.get_option(name) == v, name
     for name, good, bad in (('plan_layout', 2, 3), ('plan_layout', 1, -2),
                             ('plan_win_len', 512, 511), ('plan_win_block', 256, 512),
                             ('plan_tile_item', 4096, 63), ('plan_to_cpb', 5, 0),
-                            ('plan_from_v', 8, 5), ('nmat_dchunk', 128, 0)):
+                            ('plan_from_v', 8, 5), ('nmat_dchunk', 128, 0),
+                            ('hash_r', 4, 2), ('scatter_hash', 0, 5)):
         old = _native.get_option(name)
         try:
             _native.set_option(name, good)
