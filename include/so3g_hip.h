@@ -332,11 +332,17 @@ int so3g_pc_plan_destroy(so3g_pc_plan *plan);
  * allocated while the batch is used.  to_map can sum each work item in an
  * LDS-privatised partial map (so3g_batch_set_table: 0 global atomics, 512
  * or 1024 slots; the detectors of an item taken in map-row order, computed
- * at creation).  so3g_batch_tune times every launch shape (dpb in
- * {1,2,4,8,16} x blocks per CU in {2,4,8,16, one per item}, and for to_map
- * x table slots in {0, 512, 1024}) and keeps the fastest for the op (0
- * from_map, 1 to_map); results holds (dpb, blocks per CU, slots, ms)
- * quadruples.  Tuning to_map adds into `map`: pass a scratch map. */
+ * at creation), or -w: segments of w columns (64 <= w <= the widest whose
+ * float64 sums fit 160 KB of LDS) of each group's home map row in LDS, with
+ * items of 65536 samples x 2 row-mate detectors of one observation, one
+ * block of 1024 lanes each (flat maps with nearest-neighbour pixels; the
+ * groups are computed at creation).  so3g_batch_tune times every launch
+ * shape (dpb in {1,2,4,8,16} x blocks per CU in {2,4,8,16, one per item},
+ * and for to_map x table slots in {0, 512, 1024}, then row segments of
+ * 1024, 2048, 4096 and the widest number of columns) and keeps the fastest
+ * for the op (0 from_map, 1 to_map); results holds (dpb, blocks per CU,
+ * slots, ms) quadruples (row segments: (2, 0, -w, ms)).  Tuning to_map adds
+ * into `map`: pass a scratch map. */
 typedef struct so3g_batch so3g_batch;
 typedef struct {
     const double *bore;        /* (n_t, 4) native-frame boresight */

@@ -109,14 +109,16 @@ class ObservationBatch:
 
     def set_table(self, slots):
         """to_map through an LDS table of `slots` slots per work item (512
-        or 1024), or global atomics (0)."""
+        or 1024), global atomics (0), or (flat maps, nearest neighbour)
+        segments of -slots columns of each detector group's home map row in
+        LDS."""
         N.check(N.load().so3g_batch_set_table(self._h, int(slots)))
 
     def tune(self, op, map, reps=1):
         """Time every launch shape on this batch and keep the fastest for op.
         Tuning to_map adds into `map` (pass a scratch map).  Returns a list
         of (dets per block, blocks per CU (0: one per item), LDS table slots
-        (0: global atomics), ms)."""
+        (0: global atomics; -w: row segments of w columns), ms)."""
         call, m, tp, dp = self._map_args(map, False)
         res = (ctypes.c_double * 400)()
         n = ctypes.c_int()

@@ -7,9 +7,9 @@
 // them with a block stride, so that a single launch -- one node of a
 // captured graph -- covers observations of any mix of sizes without a
 // host round trip between them.  Each item runs the same per-sample body
-// as the single-observation kernels (from_map_sample, to_map_sample, or
-// lds_scatter_det with a table per item), so results match one launch per
-// observation.
+// as the single-observation kernels (from_map_sample, to_map_sample,
+// lds_scatter_det with a table per item, or row_item over row-segment
+// items), so results match one launch per observation.
 #pragma once
 
 #include "kernels.h"
@@ -27,6 +27,12 @@ struct ObsDev {
     int64_t item0;             // first work item of this observation
     int64_t n_tt;              // time tiles, ceil(n_t / kBlock)
     const int32_t *order;      // detector order of the LDS-table scatter
+    // row scatter: groups of G detectors sharing a home row
+    const int32_t *rgdet;      // [n_groups * G], -1 padded
+    const int32_t *rgrow;      // [n_groups] home row (-1: none)
+    const int32_t *rgxc;       // [n_groups] home column
+    int64_t n_groups;
+    int64_t ritem0;            // first row item of this observation
 };
 
 // Calls f(obs, in, d, t, qbore, valid) for every detector of the block's
@@ -109,6 +115,36 @@ __global__ __launch_bounds__(kBlock) void k_batch_to_map_lds(const ObsDev *obs, 
         lds_table_flush<PK, NC, false, H>(g, m, hk, hv, kBlock);
         __syncthreads();
     }
+}
+
+// to_map of a batch through map-row segments in LDS: one block per item
+// (observation, sample span of `span`, group of G row-mate detectors; group
+// fastest within an observation), each summing into a segment of xw columns
+// of its group's home row (row_item), so the whole batch runs in one launch
+// with the single observation's row scatter.
+template <int PROJ, int PK, int NC, int NT, int R>
+__global__ __launch_bounds__(NT) void k_batch_to_map_row(const ObsDev *obs, int n_obs,
+                                                         Tabs tabs, Geo g, MapView<double> m,
+                                                         int G, int span, int xw, int *flags) {
+    extern __shared__ double racc[];
+    const int64_t it = blockIdx.x;
+    int lo = 0, hi = n_obs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (obs[mid].ritem0 <= it) lo = mid;
+        else hi = mid - 1;
+    }
+    const ObsDev &o = obs[lo];
+    const int64_t local = it - o.ritem0;
+    const int64_t ts = local / o.n_groups;
+    const int64_t gi = local - ts * o.n_groups;
+    const int64_t t_beg = ts * span;
+    const int64_t t_end = min(t_beg + (int64_t)span, o.n_t);
+    const PtgIn in{o.bore, o.dets, o.n_t, o.n_det, tabs, 1, 0, nullptr};
+    row_item<PROJ, PK, NC, false, NT, R>(in, g, m, o.resp, o.sig_in, o.det_w,
+                                         o.rgdet + gi * G, G, o.rgrow[gi],
+                                         row_x0(o.rgxc[gi], xw, g.nx), xw, t_beg, t_end, flags,
+                                         racc);
 }
 
 template <int PROJ, int PK, int INTERP, int NC>

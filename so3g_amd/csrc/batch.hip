@@ -20,6 +20,10 @@ int flags_zero(hipStream_t stream);
 int flags_check_tile(hipStream_t stream);
 void row_order(const so3g_geom *g, const double *co, int64_t n_det, int32_t *idx);
 bool hash_keys_fit(const so3g_geom *g);
+void make_row_groups(const so3g_geom *g, const double *co, int64_t n_det, int G,
+                     std::vector<int32_t> &gdet, std::vector<int32_t> &grow,
+                     std::vector<int32_t> &gxc);
+int row_width_max(const so3g_geom *g, int nv);
 }  // namespace so3g
 
 using namespace so3g;
@@ -37,6 +41,9 @@ namespace {
 constexpr int kDpbCand[] = {1, 2, 4, 8, 16};
 constexpr int kGridMult[] = {2, 4, 8, 16, 0};   // blocks per CU; 0 = one block per item
 constexpr int kSlots[] = {0, 512, 1024};         // LDS table of to_map; 0 = global atomics
+constexpr int kRowWidth[] = {1024, 2048, 4096, 0};   // row segments of to_map; 0 = widest
+constexpr int kRowG = 2;                         // detectors per row group
+constexpr int kRowSpan = 65536;                  // samples per row item
 
 struct GraphKey {
     int op = -1;
@@ -60,9 +67,14 @@ struct so3g_batch {
     int n_cu = 256;
     int dpb[2] = {4, 4};               // chosen shape, [0] from_map, [1] to_map
     int grid[2] = {0, 0};              // blocks (0: one per item)
-    int slots = 0;                     // LDS table of to_map (0: global atomics)
+    int slots = 0;                     // to_map: 0 global atomics, 512 / 1024 LDS
+                                       // table slots, -w row segments of w columns
     bool lds_ok = false;               // every map element has an int32 key
+    bool row_ok = false;               // flat map, nearest neighbour: row segments
     int32_t *order_d = nullptr;        // per observation: detectors in row order
+    int32_t *rg_d = nullptr;           // per observation: row groups, rows, columns
+    std::vector<int64_t> row_items;    // row items of each observation
+    int64_t n_row_items = 0;
     int64_t samples = 0;
     hipStream_t cap = nullptr;         // capture stream
     hipGraph_t graph = nullptr;
@@ -100,6 +112,28 @@ int enqueue(so3g_batch *b, int op, void *map, void *const *tiles, const int32_t 
     a.tiles = tiles;
     a.tile_dims = dims;
     a.accumulate = accumulate;
+    if (a.batch_slots < 0) {
+        // row segments: one block per (observation, span, group) item
+        a.rw.G = kRowG;
+        a.rw.span = kRowSpan;
+        a.rw.xw = (int32_t)std::min<int64_t>(-slots, b->g.naxis[1]);
+        if (!per_obs) {
+            if (b->n_row_items == 0) return SO3G_OK;
+            a.obs = b->table[dpb];
+            a.n_obs = (int)b->obs.size();
+            a.grid_blocks = (int)b->n_row_items;
+            return launch(b->g.proj, a);
+        }
+        for (size_t i = 0; i < b->obs.size(); i++) {
+            if (b->row_items[i] == 0) continue;
+            a.obs = b->single[dpb] + i;
+            a.n_obs = 1;
+            a.grid_blocks = (int)b->row_items[i];
+            r = launch(b->g.proj, a);
+            if (r) return r;
+        }
+        return SO3G_OK;
+    }
     if (!per_obs) {
         const int64_t n = b->n_items[dpb];
         if (n == 0) return SO3G_OK;
@@ -216,9 +250,15 @@ int so3g_batch_create(const so3g_geom *g, const so3g_obs *obs, int64_t n_obs,
         return cleanup(SO3G_ERR_HIP, "cannot create the capture stream");
     // Detector order of each observation for the LDS tables: the map row
     // (then column) of each detector at the observation's middle sample.
+    // Row groups of each observation for the row segments (flat maps,
+    // nearest neighbour), from the same coordinates.
     std::vector<int64_t> order_off(n_obs, 0);
     b->lds_ok = hash_keys_fit(g);
-    if (b->lds_ok) {
+    b->row_ok = g->pixelizor == SO3G_PIX_FLAT && g->interp == 0;
+    std::vector<int32_t> rg_host;
+    std::vector<int64_t> rg_off(n_obs, 0), rg_n(n_obs, 0);
+    b->row_items.assign(n_obs, 0);
+    if (b->lds_ok || b->row_ok) {
         int64_t total = 0, most = 1;
         for (int64_t i = 0; i < n_obs; i++) {
             order_off[i] = total;
@@ -251,6 +291,17 @@ int so3g_batch_create(const so3g_geom *g, const so3g_obs *obs, int64_t n_obs,
                                 hipMemcpyDeviceToHost) != hipSuccess)
                 r = set_error(SO3G_ERR_HIP, "cannot read detector coordinates");
             if (!r) row_order(g, co.data(), o.n_det, idx.data() + order_off[i]);
+            if (!r && b->row_ok) {
+                std::vector<int32_t> gdet, grow, gxc;
+                make_row_groups(g, co.data(), o.n_det, kRowG, gdet, grow, gxc);
+                rg_off[i] = (int64_t)rg_host.size();
+                rg_n[i] = (int64_t)grow.size();
+                rg_host.insert(rg_host.end(), gdet.begin(), gdet.end());
+                rg_host.insert(rg_host.end(), grow.begin(), grow.end());
+                rg_host.insert(rg_host.end(), gxc.begin(), gxc.end());
+                b->row_items[i] = rg_n[i] * ((o.n_t + kRowSpan - 1) / kRowSpan);
+                b->n_row_items += b->row_items[i];
+            }
         }
         (void)hipFree(co_d);
         if (r) {
@@ -260,11 +311,21 @@ int so3g_batch_create(const so3g_geom *g, const so3g_obs *obs, int64_t n_obs,
         if (total > 0 && hipMemcpy(b->order_d, idx.data(), total * sizeof(int32_t),
                                    hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(SO3G_ERR_HIP, "cannot upload the detector order");
+        if (b->n_row_items > INT32_MAX) b->row_ok = false;   // one block per item
+        if (b->row_ok) {
+            if (hipMalloc(&b->rg_d, std::max<size_t>(rg_host.size(), 1) * sizeof(int32_t)) !=
+                hipSuccess)
+                return cleanup(SO3G_ERR_HIP, "out of memory");
+            if (!rg_host.empty() &&
+                hipMemcpy(b->rg_d, rg_host.data(), rg_host.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice) != hipSuccess)
+                return cleanup(SO3G_ERR_HIP, "cannot upload the row groups");
+        }
     }
     for (int dpb : kDpbCand) {
         std::vector<ObsDev> tab(n_obs), one(n_obs);
         std::vector<int64_t> items(n_obs);
-        int64_t item0 = 0;
+        int64_t item0 = 0, ritem0 = 0;
         for (int64_t i = 0; i < n_obs; i++) {
             const so3g_obs &o = obs[i];
             ObsDev d{};
@@ -278,12 +339,22 @@ int so3g_batch_create(const so3g_geom *g, const so3g_obs *obs, int64_t n_obs,
             d.n_det = o.n_det;
             d.n_tt = (o.n_t + kBlock - 1) / kBlock;
             d.order = b->lds_ok ? b->order_d + order_off[i] : nullptr;
+            if (b->row_ok) {
+                const int64_t ng = rg_n[i];
+                d.rgdet = b->rg_d + rg_off[i];
+                d.rgrow = d.rgdet + ng * kRowG;
+                d.rgxc = d.rgrow + ng;
+                d.n_groups = ng;
+            }
             items[i] = d.n_tt * ((o.n_det + dpb - 1) / dpb);
             d.item0 = 0;
+            d.ritem0 = 0;
             one[i] = d;
             d.item0 = item0;
+            d.ritem0 = ritem0;
             tab[i] = d;
             item0 += items[i];
+            ritem0 += b->row_items[i];
         }
         ObsDev *t_d = nullptr, *o_d = nullptr;
         const size_t bytes = std::max<size_t>(1, n_obs) * sizeof(ObsDev);
@@ -329,8 +400,19 @@ int so3g_batch_set_shape(so3g_batch *b, int op, int dpb, int grid_mult) {
 
 int so3g_batch_set_table(so3g_batch *b, int slots) {
     if (b == nullptr) return set_error(SO3G_ERR_VALUE, "batch is NULL");
+    if (slots < 0) {
+        if (!b->row_ok)
+            return set_error(SO3G_ERR_VALUE,
+                             "row segments need a flat map with nearest-neighbour pixels");
+        const int wmax = row_width_max(&b->g, b->g.ncomp);
+        if (-slots < 64 || -slots > wmax)
+            return set_error(SO3G_ERR_VALUE, "row segment width must be 64.." +
+                                                 std::to_string(wmax) + " columns");
+        b->slots = slots;
+        return SO3G_OK;
+    }
     if (slots != 0 && slots != 512 && slots != 1024)
-        return set_error(SO3G_ERR_VALUE, "table slots must be 0, 512 or 1024");
+        return set_error(SO3G_ERR_VALUE, "table slots must be 0, 512 or 1024, or -w");
     if (slots > 0 && !b->lds_ok)
         return set_error(SO3G_ERR_VALUE, "map elements do not fit int32 table keys");
     b->slots = slots;
@@ -355,10 +437,25 @@ int so3g_batch_tune(so3g_batch *b, int op, double *map, double *const *tiles,
     double best = -1.;
     int n = 0;
     reps = std::max(reps, 1);
-    for (int slots : kSlots) {
+    // candidate to_map scatters: the LDS tables, then row segments of each
+    // width (their launch shape is fixed: one block of 1024 lanes per item)
+    std::vector<int> cand(std::begin(kSlots), std::end(kSlots));
+    if (op == 1 && b->row_ok) {
+        const int wmax = row_width_max(&b->g, b->g.ncomp);
+        std::vector<int> ws;
+        for (int w : kRowWidth) {
+            int x = w > 0 ? std::min(w, wmax) : wmax;
+            x = (int)std::min<int64_t>(x, b->g.naxis[1]);
+            x = std::max(x, 64);
+            if (std::find(ws.begin(), ws.end(), x) == ws.end()) ws.push_back(x);
+        }
+        for (int x : ws) cand.push_back(-x);
+    }
+    for (int slots : cand) {
         if (slots > 0 && (op == 0 || !b->lds_ok)) continue;
         for (int dpb : kDpbCand) {
             for (int gm : kGridMult) {
+                if (slots < 0 && (dpb != kDpbCand[0] || gm != kGridMult[0])) continue;
                 // warm-up, then reps timed launches
                 r = enqueue(b, op, map, (void *const *)tiles, tile_dims, 0, dpb, gm, slots, 0,
                             s);
@@ -377,8 +474,8 @@ int so3g_batch_tune(so3g_batch *b, int op, double *map, double *const *tiles,
                 }
                 const double t = ms / reps;
                 if (results != nullptr && n < max_results) {
-                    results[4 * n] = dpb;
-                    results[4 * n + 1] = gm;
+                    results[4 * n] = slots < 0 ? kRowG : dpb;
+                    results[4 * n + 1] = slots < 0 ? 0 : gm;
                     results[4 * n + 2] = slots;
                     results[4 * n + 3] = t;
                 }
@@ -418,6 +515,7 @@ int so3g_batch_destroy(so3g_batch *b) {
     for (auto &kv : b->table) (void)hipFree(kv.second);
     for (auto &kv : b->single) (void)hipFree(kv.second);
     if (b->order_d) (void)hipFree(b->order_d);
+    if (b->rg_d) (void)hipFree(b->rg_d);
     if (b->cap) (void)hipStreamDestroy(b->cap);
     delete b;
     return SO3G_OK;

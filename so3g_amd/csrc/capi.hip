@@ -90,7 +90,7 @@ struct State {
     int32_t pc_order_k = 0;
     // Detector groups of the row scatter (home row, up to G detectors
     // each), cached like the order above.
-    int32_t *rg_det_d = nullptr, *rg_row_d = nullptr;
+    int32_t *rg_det_d = nullptr, *rg_row_d = nullptr, *rg_xc_d = nullptr;
     int64_t rg_cap = 0;
     const void *rg_bore = nullptr, *rg_dets = nullptr;
     int64_t rg_nt = -1, rg_ndet = -1;
@@ -325,6 +325,57 @@ bool hash_keys_fit(const so3g_geom *g) {
     if (!g->tiled) return g->npix < INT32_MAX;
     return (int64_t)g->n_tile * g->tile_shape[0] * g->tile_shape[1] < INT32_MAX;
 }
+// Detector groups of the row scatter from each detector's coordinates at
+// one sample (co, n_det x 4): the home row and column are the pixel formula
+// of get_pixel on the host; detectors are sorted by home row, then along
+// it, and split into groups of at most G that share a row (gdet: G entries
+// per group, -1 padded; grow: the row, -1 for detectors off the map; gxc:
+// the mean home column).  The groups only steer which samples add into LDS
+// rather than to the map, so any grouping gives the same sums.
+void make_row_groups(const so3g_geom *g, const double *co, int64_t n_det, int G,
+                     std::vector<int32_t> &gdet, std::vector<int32_t> &grow,
+                     std::vector<int32_t> &gxc) {
+    std::vector<int32_t> row(n_det), col(n_det), idx(n_det);
+    for (int64_t d = 0; d < n_det; d++) {
+        const double x = co[4 * d] / g->cdelt[1] + g->crpix[1] - 1 + 0.5;
+        const double y = co[4 * d + 1] / g->cdelt[0] + g->crpix[0] - 1 + 0.5;
+        const bool on = x >= 0 && x < g->naxis[1] && y >= 0 && y < g->naxis[0];
+        row[d] = on ? (int32_t)y : -1;
+        col[d] = on ? (int32_t)x : 0;
+        idx[d] = (int32_t)d;
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t p, int32_t q) {
+        if (row[p] != row[q]) return row[p] < row[q];
+        return co[4 * p] < co[4 * q];
+    });
+    gdet.clear();
+    grow.clear();
+    gxc.clear();
+    int64_t csum = 0, cnt = 0;
+    auto close_group = [&]() {
+        while (gdet.size() % G) gdet.push_back(-1);
+        if (cnt) gxc.push_back((int32_t)(csum / cnt));
+        csum = cnt = 0;
+    };
+    for (int64_t i = 0; i < n_det; i++) {
+        const int32_t d = idx[i];
+        const bool fresh = grow.empty() || grow.back() != row[d] || gdet.size() % G == 0;
+        if (fresh) {
+            close_group();
+            grow.push_back(row[d]);
+        }
+        gdet.push_back(d);
+        csum += col[d];
+        cnt++;
+    }
+    close_group();
+}
+
+// Widest row segment whose sums (and tiled maps' active flags) fit in LDS.
+int row_width_max(const so3g_geom *g, int nv) {
+    return (int)((160 * 1024) / (nv * 8 + (g->tiled ? 1 : 0)));
+}
+
 }  // namespace so3g
 
 extern "C" {
@@ -533,12 +584,9 @@ static int scatter_order(const so3g_geom *g, const OpArgs &a0, const int32_t **o
     return SO3G_OK;
 }
 
-// Detector groups of the row scatter: each detector's home row is its map
-// row at the middle sample (one k_coords launch over n_det samples, the
-// pixel formula of get_pixel on the host); detectors are sorted by home row,
-// then along it, and split into groups of at most G that share a row.  The
-// groups only steer which samples add into LDS rather than to the map, so a
-// stale cache entry costs speed, never correctness.
+// Row groups of one observation (its middle sample; one k_coords launch over
+// n_det samples), cached on the device for repeated calls.  A stale cache
+// entry costs speed, never correctness.
 static int row_groups(const so3g_geom *g, const OpArgs &a0, int G, RowWork &rw) {
     const int64_t n_det = a0.in.n_det, n_t = a0.in.n_t;
     if (!(g_state.rg_bore == a0.in.bore && g_state.rg_dets == a0.in.dets &&
@@ -567,44 +615,26 @@ static int row_groups(const so3g_geom *g, const OpArgs &a0, int G, RowWork &rw) 
         HIP_TRY(hipMemcpyAsync(co.data(), g_state.order_co_d, n_det * 4 * sizeof(double),
                                hipMemcpyDeviceToHost, a.stream));
         HIP_TRY(hipStreamSynchronize(a.stream));
-        std::vector<int32_t> row(n_det), idx(n_det);
-        for (int64_t d = 0; d < n_det; d++) {
-            const double x = co[4 * d] / g->cdelt[1] + g->crpix[1] - 1 + 0.5;
-            const double y = co[4 * d + 1] / g->cdelt[0] + g->crpix[0] - 1 + 0.5;
-            const bool on = x >= 0 && x < g->naxis[1] && y >= 0 && y < g->naxis[0];
-            row[d] = on ? (int32_t)y : -1;
-            idx[d] = (int32_t)d;
-        }
-        std::stable_sort(idx.begin(), idx.end(), [&](int32_t p, int32_t q) {
-            if (row[p] != row[q]) return row[p] < row[q];
-            return co[4 * p] < co[4 * q];
-        });
-        std::vector<int32_t> gdet, grow;
-        for (int64_t i = 0; i < n_det; i++) {
-            const int32_t d = idx[i];
-            const bool fresh = grow.empty() || grow.back() != row[d] ||
-                               gdet.size() % G == 0;
-            if (fresh) {
-                while (gdet.size() % G) gdet.push_back(-1);
-                grow.push_back(row[d]);
-            }
-            gdet.push_back(d);
-        }
-        while (gdet.size() % G) gdet.push_back(-1);
+        std::vector<int32_t> gdet, grow, gxc;
+        make_row_groups(g, co.data(), n_det, G, gdet, grow, gxc);
         const int64_t ng = (int64_t)grow.size();
         if (ng * G > g_state.rg_cap || ng > g_state.rg_cap) {
             if (g_state.rg_det_d) HIP_TRY(hipFree(g_state.rg_det_d));
             if (g_state.rg_row_d) HIP_TRY(hipFree(g_state.rg_row_d));
-            g_state.rg_det_d = g_state.rg_row_d = nullptr;
+            if (g_state.rg_xc_d) HIP_TRY(hipFree(g_state.rg_xc_d));
+            g_state.rg_det_d = g_state.rg_row_d = g_state.rg_xc_d = nullptr;
             g_state.rg_cap = 0;
             g_state.rg_bore = nullptr;
             HIP_TRY(hipMalloc(&g_state.rg_det_d, ng * G * sizeof(int32_t)));
             HIP_TRY(hipMalloc(&g_state.rg_row_d, ng * G * sizeof(int32_t)));
+            HIP_TRY(hipMalloc(&g_state.rg_xc_d, ng * G * sizeof(int32_t)));
             g_state.rg_cap = ng * G;
         }
         HIP_TRY(hipMemcpyAsync(g_state.rg_det_d, gdet.data(), ng * G * sizeof(int32_t),
                                hipMemcpyHostToDevice, a.stream));
         HIP_TRY(hipMemcpyAsync(g_state.rg_row_d, grow.data(), ng * sizeof(int32_t),
+                               hipMemcpyHostToDevice, a.stream));
+        HIP_TRY(hipMemcpyAsync(g_state.rg_xc_d, gxc.data(), ng * sizeof(int32_t),
                                hipMemcpyHostToDevice, a.stream));
         HIP_TRY(hipStreamSynchronize(a.stream));
         g_state.rg_bore = a0.in.bore;
@@ -617,6 +647,7 @@ static int row_groups(const so3g_geom *g, const OpArgs &a0, int G, RowWork &rw) 
     }
     rw.gdet = g_state.rg_det_d;
     rw.grow = g_state.rg_row_d;
+    rw.gxc = g_state.rg_xc_d;
     rw.n_groups = g_state.rg_n;
     rw.G = G;
     return SO3G_OK;
@@ -667,9 +698,7 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
     int cls = 0, per_block = 0, span = 0, lane_r = g_opt.hash_r;
     const int nv_row = op == OP_TO_MAP ? g->ncomp : g->ncomp * (g->ncomp + 1) / 2;
     const bool row_ok = g->pixelizor == SO3G_PIX_FLAT && g->interp == 0 &&
-                        prec == SO3G_PREC_REF && segs == nullptr &&
-                        (int64_t)nv_row * g->naxis[1] * 8 + (g->tiled ? g->naxis[1] : 0) <=
-                            160 * 1024;
+                        prec == SO3G_PREC_REF && segs == nullptr;
     // Map rows in LDS (k_scatter_row) in modes 1 and 4 (1000 x 720000
     // samples, CAR 0.5' TQU: to_map 7.2 ms against 8.7 with the small
     // tables, to_weights 6.9 against 11.7; groups of 2 detectors over 65536
@@ -681,6 +710,7 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
         a.rw.span = g_opt.hash_span > 0 ? g_opt.hash_span : 65536;
         r = row_groups(g, a, g_opt.hash_dets > 0 ? g_opt.hash_dets : 2, a.rw);
         if (r) return r;
+        a.rw.xw = (int32_t)std::min<int64_t>(g->naxis[1], row_width_max(g, nv_row));
         while (((n_t + a.rw.span - 1) / a.rw.span) * (int64_t)a.rw.n_groups > INT32_MAX)
             a.rw.span *= 2;
         r = run_dispatch(g->proj, a);

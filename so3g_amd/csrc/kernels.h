@@ -799,65 +799,76 @@ __global__ __launch_bounds__(NT) void k_scatter_lds(PtgIn in, Geo g, MapView<dou
 // A constant-elevation scan in horizon coordinates keeps each detector on
 // one map row.  A block owns a group of up to G detectors that share a home
 // row (their row at mid-observation) over T consecutive samples, and keeps
-// that row's NV float64 sums per pixel in LDS (direct indexing: no keys, no
-// probing).  Each thread keeps its samples' boresights in registers and
-// walks the group's detectors; each run of equal pixels is added to the row
-// in LDS, or to the map with float64 atomics when it lies on another row.
-// The row's touched pixels are then added to the map, one atomic per pixel
-// and value (tiled maps: the row crosses the tiles of one tile row, the
-// samples' tiles are checked as in k_to_map).  Sums are float64 sums of the
-// same float32 products as k_to_map / k_to_weights, in another order.
+// the NV float64 sums per pixel of a segment of that row -- xw columns from
+// x0, the whole row where it fits in LDS -- in LDS (direct indexing: no
+// keys, no probing).  Each thread keeps its samples' boresights in
+// registers and walks the group's detectors; each run of equal pixels is
+// added to the segment in LDS, or to the map with float64 atomics when it
+// lies elsewhere.  The segment's touched pixels are then added to the map,
+// one atomic per pixel and value (tiled maps: the segment crosses the
+// tiles of one tile row, the samples' tiles are checked as in k_to_map).
+// Sums are float64 sums of the same float32 products as k_to_map /
+// k_to_weights, in another order.
 // --------------------------------------------------------------------------
 struct RowWork {
     const int32_t *gdet;    // [n_groups * G] detectors of each group, -1 padded
     const int32_t *grow;    // [n_groups] home row (-1: none)
+    const int32_t *gxc;     // [n_groups] home column (segment centre)
     int32_t n_groups, G, span;
+    int32_t xw;             // segment width in columns (<= nx)
 };
 
+// First column of a segment of xw columns centred on column xc.
+__device__ __forceinline__ int row_x0(int xc, int xw, int nx) {
+    return max(0, min(xc - xw / 2, nx - xw));
+}
+
+// LDS of one row segment: NV x xw float64 sums, then (tiled maps) xw
+// active-tile flags.
+template <int PK>
+constexpr size_t row_lds_bytes(int nv, int xw) {
+    return (size_t)nv * xw * sizeof(double) + (PK == PK_FLAT_TILED ? (size_t)xw : 0);
+}
+
+// One (group, sample span) item of the row scatter, by the whole block.
 // Each lane takes R consecutive samples (no wave-level scan): it sums its
 // own runs of equal pixels over them in registers and adds each run to the
-// row in LDS (or to the map off the row).  A wave-level scan over one
+// segment in LDS (or to the map off it).  A wave-level scan over one
 // sample per lane (wave_runs) measured slower: to_map 9.0 vs 7.2 ms,
 // to_weights 8.2 vs 6.9 ms on 7.2e8 samples; R = 8 needs more than 128
 // registers and measured 7.5 / 7.7 ms.
 template <int PROJ, int PK, int NC, bool W, int NT, int R>
-__global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<double> m,
-                                                    const float *resp, const float *sig,
-                                                    const float *det_weights, RowWork rw,
-                                                    int *flags) {
+__device__ __forceinline__ void row_item(const PtgIn &in, const Geo &g, const MapView<double> &m,
+                                         const float *resp, const float *sig,
+                                         const float *det_weights, const int32_t *gd, int G,
+                                         int row, int x0, int xw, int64_t t_beg,
+                                         int64_t t_end, int *flags, double *racc) {
     constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
     constexpr bool TILED = PK == PK_FLAT_TILED;
-    extern __shared__ double racc[];   // [NV][nx] sums, then (tiled) nx active flags
-    uint8_t *ract = reinterpret_cast<uint8_t *>(racc + NV * g.nx);
-    const int nx = g.nx;
-    const int64_t ts = (int64_t)blockIdx.x / rw.n_groups;
-    const int gi = (int)((int64_t)blockIdx.x - ts * rw.n_groups);
-    const int row = rw.grow[gi];
-    for (int q = threadIdx.x; q < NV * nx; q += NT) racc[q] = 0.;
+    uint8_t *ract = reinterpret_cast<uint8_t *>(racc + NV * xw);
+    for (int q = threadIdx.x; q < NV * xw; q += NT) racc[q] = 0.;
     if constexpr (TILED) {
         if (row >= 0) {
             const int t0 = (row / g.ty) * g.n_tx;
-            for (int x = threadIdx.x; x < nx; x += NT)
-                ract[x] = m.tiles[t0 + x / g.tx] != nullptr;
+            for (int x = threadIdx.x; x < xw; x += NT)
+                ract[x] = m.tiles[t0 + (x0 + x) / g.tx] != nullptr;
         }
     }
     __syncthreads();
-    const int64_t t_beg = ts * rw.span;
-    const int64_t t_end = min(t_beg + (int64_t)rw.span, in.n_t);
-    const int32_t *gd = rw.gdet + (int64_t)gi * rw.G;
-    // adds one run's sums (map row iy, column ix): to the row in LDS, or to
-    // the map off the row; a run in an inactive tile raises the tile flag
+    // adds one run's sums (map row iy, column ix): to the segment in LDS,
+    // or to the map off it; a run in an inactive tile raises the tile flag
     auto emit = [&](int iy, int ix, bool on, const double (&v)[NV]) {
         if (!on) return;
-        if (iy == row) {
+        const unsigned lx = (unsigned)(ix - x0);
+        if (iy == row && lx < (unsigned)xw) {
             if constexpr (TILED) {
-                if (!ract[ix]) {
+                if (!ract[lx]) {
                     raise_flag(flags, FLAG_TILE);
                     return;
                 }
             }
 #pragma unroll
-            for (int c = 0; c < NV; c++) atomicAdd(racc + c * nx + ix, v[c]);
+            for (int c = 0; c < NV; c++) atomicAdd(racc + c * xw + lx, v[c]);
         } else {
             double *e;
             int64_t plane;
@@ -873,7 +884,7 @@ __global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<dou
                 plane = (int64_t)m.tile_dims[2 * t] * tw;
                 e += (int64_t)(iy - sy * g.ty) * tw + (ix - sx * g.tx);
             } else {
-                e = m.map + (int64_t)iy * nx + ix;
+                e = m.map + (int64_t)iy * g.nx + ix;
                 plane = g.npix;
             }
 #pragma unroll
@@ -889,7 +900,7 @@ __global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<dou
 #pragma unroll
         for (int r = 0; r < R; r++)
             qb[r] = t0 + r < t_end ? load_q(in.bore + 4 * (t0 + r)) : Q4{1., 0., 0., 0.};
-        for (int j = 0; j < rw.G; j++) {
+        for (int j = 0; j < G; j++) {
             const int32_t d = gd[j];
             if (d < 0) break;
             float det_wt = 1.f;
@@ -954,27 +965,45 @@ __global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<dou
     }
     __syncthreads();
     if (row < 0) return;
-    for (int x = threadIdx.x; x < nx; x += NT) {
+    for (int lx = threadIdx.x; lx < xw; lx += NT) {
+        const int x = x0 + lx;
         double *e;
         int64_t plane;
         if constexpr (TILED) {
-            if (!ract[x]) continue;   // nothing was added there
+            if (!ract[lx]) continue;   // nothing was added there
             const int sy = row / g.ty, sx = x / g.tx;
             const int t = sy * g.n_tx + sx;
             const int tw = m.tile_dims[2 * t + 1];
             plane = (int64_t)m.tile_dims[2 * t] * tw;
             e = m.tiles[t] + (int64_t)(row - sy * g.ty) * tw + (x - sx * g.tx);
         } else {
-            e = m.map + (int64_t)row * nx + x;
+            e = m.map + (int64_t)row * g.nx + x;
             plane = g.npix;
         }
 #pragma unroll
         for (int c = 0; c < NV; c++) {
-            const double v = racc[c * nx + x];
+            const double v = racc[c * xw + lx];
             const int lead = W ? hash_lead<NC>(c) : c;
             if (v != 0.) unsafeAtomicAdd(e + lead * plane, v);
         }
     }
+}
+
+// One observation: block = (sample span, group), group fastest.
+template <int PROJ, int PK, int NC, bool W, int NT, int R>
+__global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<double> m,
+                                                    const float *resp, const float *sig,
+                                                    const float *det_weights, RowWork rw,
+                                                    int *flags) {
+    extern __shared__ double racc[];
+    const int64_t ts = (int64_t)blockIdx.x / rw.n_groups;
+    const int gi = (int)((int64_t)blockIdx.x - ts * rw.n_groups);
+    const int64_t t_beg = ts * rw.span;
+    const int64_t t_end = min(t_beg + (int64_t)rw.span, in.n_t);
+    row_item<PROJ, PK, NC, W, NT, R>(in, g, m, resp, sig, det_weights,
+                                     rw.gdet + (int64_t)gi * rw.G, rw.G, rw.grow[gi],
+                                     row_x0(rw.gxc[gi], rw.xw, g.nx), rw.xw, t_beg, t_end,
+                                     flags, racc);
 }
 
 // --------------------------------------------------------------------------

@@ -43,7 +43,8 @@ struct OpArgs {
     HashWork hw;
     RowWork rw;
     // batch ops
-    int batch_slots = 0;       // LDS table of batch to_map: 0 none, 512, 1024
+    int batch_slots = 0;       // batch to_map: 0 global atomics, 512 / 1024 LDS
+                               // table slots, < 0 row segments (width in rw.xw)
     const ObsDev *obs;
     int n_obs;
     int64_t n_items;
@@ -85,14 +86,13 @@ void launch_scatter_cls(const OpArgs &a) {
         launch_scatter_lds<PROJ, PK, INTERP, NC, W, 1024, hash_slots(NV, 112 * 1024), false>(a);
 }
 
-// Rows in LDS: 1024 threads of 4 samples each, the row's NV x nx float64
-// sums as dynamic LDS (flat maps, nearest-neighbour pixels).
+// Rows in LDS: 1024 threads of 4 samples each, a row segment's NV x xw
+// float64 sums as dynamic LDS (flat maps, nearest-neighbour pixels).
 template <int PROJ, int PK, int NC, bool W>
 void launch_scatter_row(const OpArgs &a) {
     constexpr int NV = W ? NC * (NC + 1) / 2 : NC;
     constexpr int NT = 1024;
-    const size_t lds = (size_t)NV * a.g.nx * sizeof(double) +
-                       (PK == PK_FLAT_TILED ? (size_t)a.g.nx : 0);
+    const size_t lds = row_lds_bytes<PK>(NV, a.rw.xw);
     const dim3 grid((unsigned)(((a.in.n_t + a.rw.span - 1) / a.rw.span) * a.rw.n_groups));
     MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
     (void)hipFuncSetAttribute((const void *)k_scatter_row<PROJ, PK, NC, W, NT, 4>,
@@ -114,9 +114,26 @@ template <int PROJ, int PK, int INTERP, int NC>
 int launch_map_ops(const OpArgs &a) {
     const dim3 block(kBlock);
     if (a.op == OP_BATCH_TO_MAP || a.op == OP_BATCH_FROM_MAP) {
+        if (PK == PK_FLAT_TILED || PK == PK_HP_TILED ? a.tiles == nullptr : a.map == nullptr)
+            return -1;   // no map to project from or into
         const dim3 bgrid((unsigned)a.grid_blocks);
         if (a.op == OP_BATCH_TO_MAP) {
             MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
+            if (a.batch_slots < 0) {
+                if constexpr ((PK == PK_FLAT || PK == PK_FLAT_TILED) && INTERP == 0) {
+                    constexpr int NT = 1024;
+                    const size_t lds = row_lds_bytes<PK>(NC, a.rw.xw);
+                    (void)hipFuncSetAttribute(
+                        (const void *)k_batch_to_map_row<PROJ, PK, NC, NT, 4>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                    hipLaunchKernelGGL((k_batch_to_map_row<PROJ, PK, NC, NT, 4>), bgrid,
+                                       dim3(NT), lds, a.stream, a.obs, a.n_obs, a.in.tabs,
+                                       a.g, m, a.rw.G, a.rw.span, a.rw.xw, a.flags);
+                    return 0;
+                } else {
+                    return -1;   // the host offers row segments for such maps only
+                }
+            }
             if (a.batch_slots == 512)
                 hipLaunchKernelGGL((k_batch_to_map_lds<PROJ, PK, INTERP, NC, 512>), bgrid,
                                    block, 0, a.stream, a.obs, a.n_obs, a.n_items, a.in.dpb,

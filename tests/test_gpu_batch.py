@@ -128,8 +128,9 @@ def test_batch_matches_per_observation(gpu, tiled):
                                it['det_weights'])
     np.testing.assert_allclose(ref_h, flat(om), rtol=0, atol=1e-9)
     # global atomics, then each work item summed in an LDS table of 512 or
-    # 1024 slots first
-    for slots in (0, 512, 1024):
+    # 1024 slots first, then row segments of 64 columns (most samples off the
+    # segment), 700 and 6000 (wider than the map: the whole row)
+    for slots in (0, 512, 1024, -64, -700, -6000):
         batch.set_table(slots)
         for mode in ['eager', 'per_obs', 'graph', 'graph', 'graph']:
             m = zero_like(sky)
@@ -159,14 +160,19 @@ def test_batch_tune(gpu):
     assert batch.info()['from_map_shape'] == best[:2]
     scratch = zero_like(sky)
     res = batch.tune('to_map', scratch)
-    # 25 launch shapes x (global atomics, 512- and 1024-slot LDS tables)
-    assert len(res) == 75 and sorted({r[2] for r in res}) == [0, 512, 1024]
+    # 25 launch shapes x (global atomics, 512- and 1024-slot LDS tables),
+    # then row segments of 1024 columns and of the whole row
+    nx = pixargs[1]
+    widths = sorted({min(1024, nx), nx})
+    assert len(res) == 75 + len(widths)
+    assert sorted({r[2] for r in res}) == sorted([-w for w in widths] + [0, 512, 1024])
+    assert all(r[:2] == (2, 0) for r in res if r[2] < 0)
     best = min(res, key=lambda r: r[3])
     assert batch.info()['to_map_shape'] == best[:2]
     assert batch.info()['to_map_slots'] == best[2]
     # every tuned shape gives the same result
     ref = None
-    for dpb, gm, slots in [(1, 2, 0), (16, 0, 512), (4, 8, 1024), (2, 0, 1024)]:
+    for dpb, gm, slots in [(1, 2, 0), (16, 0, 512), (4, 8, 1024), (2, 0, 1024), (1, 2, -100)]:
         batch.set_shape('to_map', dpb, gm)
         batch.set_table(slots)
         m = zero_like(sky)
@@ -179,6 +185,10 @@ def test_batch_tune(gpu):
         batch.set_shape('to_map', 3, 2)
     with pytest.raises(ValueError):
         batch.set_table(256)
+    with pytest.raises(ValueError):
+        batch.set_table(-63)
+    with pytest.raises(ValueError):
+        batch.set_table(-7000)   # 3 x 7000 float64 sums exceed 160 KB
 
 
 @pytest.mark.gpu
@@ -195,3 +205,7 @@ def test_batch_inactive_tile_raises(gpu):
         batch.from_map(sky, mode='eager')
     with pytest.raises(RuntimeError):
         batch.to_map(zero_like(sky), mode='graph')
+    batch.set_table(-300)
+    for mode in ('eager', 'per_obs', 'graph'):
+        with pytest.raises(RuntimeError):
+            batch.to_map(zero_like(sky), mode=mode)

@@ -467,21 +467,24 @@ def test_row_scatter(gpu, proj, comps, clip, dets, span):
             _native.set_option(n, v)
 
 
-def test_row_scatter_wide_map(gpu):
-    """A map too wide for a row of to_weights sums in LDS (6 x 4000 float64
-    = 192 KB) takes the other scatter in mode 4; the maps equal the
+@pytest.mark.parametrize('nx,tiled', [(4000, False), (12000, False), (12000, True)])
+def test_row_scatter_wide_map(gpu, nx, tiled):
+    """Maps too wide for a whole row of sums in LDS (to_weights: 6 x 4000
+    float64 = 192 KB; to_map: 3 x 12000) sum into a segment of the row
+    centred on each group's home column (3413 / 6826 columns, 3342 / 6553
+    when tiled), samples beyond it added to the map; the maps equal the
     oracle's."""
     from so3g_amd import _native
     old = _native.get_option('scatter_hash')
     _native.set_option('scatter_hash', 4)
     try:
-        c = Case('CAR', comps='TQU', n_det=7, n_samp=6000, ny=300, nx=4000)
+        c = Case('CAR', comps='TQU', tiled=tiled, n_det=7, n_samp=6000, ny=300, nx=nx)
         g = c.engine.to_weight_map(None, c.bore, c.dets, c.resps)
         r = oracle.to_weights(c.og, c.zeros([3, 3]), c.bore, c.dets, c.resps)
-        maps_close(g, r, 'wide to_weights')
+        maps_close(g, r, f'wide to_weights {nx}')
         g = c.engine.to_map(None, c.bore, c.dets, c.resps, c.signal)
         r = oracle.to_map(c.og, c.zeros([3]), c.bore, c.dets, c.resps, c.signal)
-        maps_close(g, r, 'wide to_map (row fits: 3 x 4000 float64)')
+        maps_close(g, r, f'wide to_map {nx}')
     finally:
         _native.set_option('scatter_hash', old)
 
