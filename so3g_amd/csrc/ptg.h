@@ -188,7 +188,10 @@ __device__ __forceinline__ int64_t hp_spread(int64_t v) {
 
 __device__ __forceinline__ int64_t hp_ang2pix(int64_t ns, bool nest, double theta,
                                               double phi) {
-    double z = cos(theta), s = sin(theta);
+    // sin and cos from one argument reduction: the same reduction and
+    // polynomials as separate sin() and cos() calls, so the same bits
+    double z, s;
+    sincos(theta, &s, &z);
     if (s < 0) { s = -s; phi += kPi; }
     // Continuous face coordinates.
     const double za = fabs(z);
