@@ -516,9 +516,11 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   (256 threads, 512 slots), 3 always with large ones
  *                   (1024 threads, 112 KB), 4 the map row of a group of
  *                   detectors held in LDS where the map allows it
- *                   (flat, tiled or not, nearest neighbour, the row's
- *                   sums within 160 KB; else as 1).  Mode 1 uses the rows
- *                   where the map allows them.  Precomp to_map /
+ *                   (flat, tiled or not, nearest neighbour; the whole
+ *                   row where its sums fit 160 KB, else a segment of the
+ *                   widest width that fits, centred on the group's home
+ *                   column; other maps as 1).  Mode 1 uses the rows where
+ *                   the map allows them.  Precomp to_map /
  *                   to_weights use the small tables in modes 1, 2 and 4
  *                   (tiled maps: to_weights only in modes 1 and 4)
  *   "hash_r"        samples per lane of the hash tables in modes 2 and 3
