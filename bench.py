@@ -117,6 +117,9 @@ def parse():
                     help='skip the precision (C3), batched (C4) and map-making (C5) legs')
     ap.add_argument('--only-small', action='store_true',
                     help='time only the small-observation pair and print its JSON')
+    ap.add_argument('--only-nmat', action='store_true',
+                    help='time only nmat_detvecs_apply and print its JSON, then stream the '
+                         'same array once (a scale and a sum) as a reference for profiles')
     ap.add_argument('--cpu-full', action='store_true',
                     help='print the full CPU baseline table (oracle, median of 5) and exit')
     return ap.parse_args()
@@ -141,6 +144,14 @@ def main():
     if args.only_small:
         stream = torch.cuda.current_stream().cuda_stream or None
         print(json.dumps(small_config(torch, so3g_amd, N, workloads, dev, stream)), flush=True)
+        return
+    if args.only_nmat:
+        print(json.dumps(nmat_timing(torch, so3g_amd, dev, args.n_det, args.n_samp)), flush=True)
+        ref = torch.ones((args.n_det, 2 * (args.n_samp // 2 + 1)), dtype=torch.float32,
+                         device=dev)
+        ref.mul_(1.5)
+        float(ref.sum())
+        torch.cuda.synchronize()
         return
 
     t0 = time.time()

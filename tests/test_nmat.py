@@ -65,7 +65,7 @@ def test_argument_errors_cpu():
 @pytest.mark.gpu
 @pytest.mark.parametrize('ndet,nfreq,nvec,nbin', [(50, 1000, 3, 4), (1000, 5000, 20, 7),
                                                   (3000, 800, 5, 3), (40, 333, 0, 2),
-                                                  (100, 2000, 64, 5)])
+                                                  (100, 2000, 64, 5), (67, 2501, 10, 6)])
 def test_nmat_gpu_against_oracle(gpu, ndet, nfreq, nvec, nbin):
     import so3g_amd
     rng = np.random.default_rng(ndet + nfreq)
@@ -89,6 +89,15 @@ def test_nmat_gpu_against_oracle(gpu, ndet, nfreq, nvec, nbin):
                                 torch.from_numpy(iV).cuda(), s, norm, dct_binning=True)
     ref1 = R.nmat_detvecs_apply(ft, bins, iD, iV, s, norm, dct_binning=True)
     assert np.abs(gt.cpu().numpy() - ref1).max() / np.abs(ref1).max() < 2e-6
+    # rows whose columns start 4 bytes past an 8-byte boundary (a view one
+    # float into a larger buffer): 4-byte accesses
+    buf = torch.zeros(ndet * nmode + 1, dtype=torch.float32, device='cuda')
+    gv = buf[1:].view(ndet, nmode)
+    gv.copy_(torch.from_numpy(ft))
+    so3g_amd.nmat_detvecs_apply(gv, bins, torch.from_numpy(iD).cuda(),
+                                torch.from_numpy(iV).cuda(), s, norm)
+    assert np.abs(gv.cpu().numpy() - ref).max() / np.abs(ref).max() < 2e-6
+    assert float(buf[0]) == 0.
 
 
 @pytest.mark.gpu
