@@ -1230,6 +1230,14 @@ constexpr int kTU = SO3G_TILE_U;
 #define SO3G_TILE_COPIES 2
 #endif
 constexpr int kTCopies = SO3G_TILE_COPIES;
+// Doubles between the copies: a copy holds NC x 2048 doubles, a multiple of
+// the LDS bank period (128 bytes), so without a gap a slot's two copies sit in
+// the same bank and neighbouring lanes adding to one pixel (consecutive
+// samples, alternate copies) conflict.  64 bytes put them 16 banks apart
+// (HEALPix nside 4096 scatter 2.55 -> 2.12 ms, CAR 2.05 -> 1.99 ms; LDS
+// bank-conflict cycles 4.8e8 -> 2.1e8; gaps of 0 / 4 / 8 / 12 / 16 / 24
+// doubles: 2.55 / 2.74 / 2.12 / 2.74 / 2.52 / 2.12 ms on HEALPix).
+constexpr int kTPad = 8;
 
 template <int NC, bool TC, typename ST, bool LDT>
 __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
@@ -1242,11 +1250,12 @@ __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
     double *const *__restrict__ tiles, const int32_t *__restrict__ tile_map,
     const int32_t *__restrict__ tdims) {
     constexpr int NS = TC ? NC - 1 : NC;
-    __shared__ double acc[kTCopies * NC * kTS];
+    constexpr int CS = NC * kTS + kTPad;   // doubles per copy and gap
+    __shared__ double acc[kTCopies * CS];
     extern __shared__ float s_tab[];   // LDT: [n_det] T factors (TC), [n_det] weights
     const int w_off = TC ? n_det : 0;
     const TItem it = items[blockIdx.x];
-    for (int q = threadIdx.x; q < kTCopies * NC * kTS; q += kTBlock) acc[q] = 0.;
+    for (int q = threadIdx.x; q < kTCopies * CS; q += kTBlock) acc[q] = 0.;
     if constexpr (LDT) {
         for (int d = threadIdx.x; d < n_det; d += kTBlock) {
             if (TC) s_tab[d] = tval[d];
@@ -1319,7 +1328,7 @@ __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
                     x = (double)w * sg[u] * (double)pf[q];
                 else
                     x = (double)(w * sg[u] * pf[q]);
-                atomicAdd(acc + ((lane & (kTCopies - 1)) * NC + q) * kTS + sl[u], x);
+                atomicAdd(acc + (lane & (kTCopies - 1)) * CS + q * kTS + sl[u], x);
             }
         }
     }
@@ -1339,7 +1348,7 @@ __global__ __launch_bounds__(kTBlock) void k_tp_to_map(
         for (int cc = 0; cc < NC; cc++) {
             double v = acc[cc * kTS + q];
 #pragma unroll
-            for (int cp = 1; cp < kTCopies; cp++) v += acc[(cp * NC + cc) * kTS + q];
+            for (int cp = 1; cp < kTCopies; cp++) v += acc[cp * CS + cc * kTS + q];
             if (v != 0.) unsafeAtomicAdd(mbase + cc * pl + px, v);
         }
     }
@@ -1717,6 +1726,9 @@ __device__ __forceinline__ void tw_load(int64_t i0, int64_t e, const uint16_t *_
 // to_map (Projection.cxx:2100-2121): (w * sig) * pf in float32 (float64 for
 // a float64 signal), summed in float64; a zero detector weight skips the
 // detector.
+// Doubles between the two copies of a window's sums (see kTPad).
+constexpr int kWPad = 8;
+
 template <int NC, bool TC, typename ST, bool VEC, int BS, bool ATOM, int CP>
 __global__ __launch_bounds__(1024) void k_tw_to_map(const WItem *__restrict__ items,
                                                     const uint16_t *__restrict__ wslot,
@@ -1729,10 +1741,13 @@ __global__ __launch_bounds__(1024) void k_tw_to_map(const WItem *__restrict__ it
                                                     const int32_t *__restrict__ comp2pix,
                                                     int64_t plane, double *map) {
     constexpr int NS = TC ? NC - 1 : NC;
-    extern __shared__ double wsum[];   // [CP][NC][win]: lane parity picks the copy (CP = 2)
+    // [CP][NC][win] with kWPad doubles between the copies: lane parity picks
+    // the copy (CP = 2)
+    extern __shared__ double wsum[];
     const WItem it = items[blockIdx.x];
     const int win = it.win;
-    double *wacc = wsum + (CP > 1 ? (threadIdx.x & 1) * NC * win : 0);
+    const int cs = NC * win + kWPad;   // doubles per copy and gap
+    double *wacc = wsum + (CP > 1 ? (threadIdx.x & 1) * cs : 0);
     const float w = det_weights != nullptr ? det_weights[it.det] : 1.f;
     const float pt = TC ? tval[it.det] : 0.f;
     const bool work = win > 0 && w != 0.f;   // a zero weight skips the detector (Projection.cxx:2104)
@@ -1743,7 +1758,7 @@ __global__ __launch_bounds__(1024) void k_tw_to_map(const WItem *__restrict__ it
     ST sg[8];
     // the first 8 samples' loads are issued before the window is zeroed
     if (work && i0 < e) tw_load<NS, VEC, ST>(i0, e, wslot, tspin, sig, true, sl, sp, sg);
-    for (int q = threadIdx.x; q < CP * NC * win; q += BS) wsum[q] = 0.;
+    for (int q = threadIdx.x; q < CP * cs; q += BS) wsum[q] = 0.;
     __syncthreads();
     if (work) {
         for (bool first = true; i0 < e; i0 += BS * 8, first = false) {
@@ -1797,7 +1812,7 @@ __global__ __launch_bounds__(1024) void k_tw_to_map(const WItem *__restrict__ it
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             v[c] = wsum[c * win + q];
-            if (CP > 1) v[c] += wsum[(NC + c) * win + q];
+            if (CP > 1) v[c] += wsum[cs + c * win + q];
         }
         if constexpr (ATOM) {
             const int64_t p = comp2pix[it.lo + q];
@@ -2109,7 +2124,7 @@ int plan_tw_to_map(const so3g_pc_plan *p, double *map, const ST *signal,
     if (p->n_witems > 0) {
         // two copies of the window sums (by lane parity) where they leave room
         // for two blocks per compute unit
-        const size_t lds1 = (size_t)p->nc * std::max(1, p->win_max) * sizeof(double);
+        const size_t lds1 = (size_t)(p->nc * std::max(1, p->win_max) + kWPad) * sizeof(double);
         const bool two = 2 * lds1 <= 65536;
         const size_t lds = two ? 2 * lds1 : lds1;
         const dim3 gi((unsigned)p->n_witems);
