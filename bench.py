@@ -117,6 +117,9 @@ def parse():
                     help='skip the precision (C3), batched (C4) and map-making (C5) legs')
     ap.add_argument('--only-small', action='store_true',
                     help='time only the small-observation pair and print its JSON')
+    ap.add_argument('--only-tiles', action='store_true',
+                    help='time only the tiled-plan to_map on the CAR 0.5 arcmin and HEALPix '
+                         'NEST 4096 maps of the benchmark observation and print their JSON')
     ap.add_argument('--only-nmat', action='store_true',
                     help='time only nmat_detvecs_apply and print its JSON, then stream the '
                          'same array once (a scale and a sum) as a reference for profiles')
@@ -144,6 +147,10 @@ def main():
     if args.only_small:
         stream = torch.cuda.current_stream().cuda_stream or None
         print(json.dumps(small_config(torch, so3g_amd, N, workloads, dev, stream)), flush=True)
+        return
+    if args.only_tiles:
+        print(json.dumps(tile_scatter_timing(torch, so3g_amd, workloads, dev, args.n_det,
+                                             args.n_samp)), flush=True)
         return
     if args.only_nmat:
         print(json.dumps(nmat_timing(torch, so3g_amd, dev, args.n_det, args.n_samp)), flush=True)
@@ -620,6 +627,40 @@ def small_config(torch, so3g_amd, N, workloads, dev, stream, reps=20):
            'median_of': reps}
     del flush_a, flush_b
     log(f'small config: warm {warm:.3f} ms, cold {cold:.3f} ms per pair')
+    return out
+
+
+def tile_scatter_timing(torch, so3g_amd, workloads, dev, n_det, n_t, reps=5):
+    """The tiled plan's to_map (k_tp_to_map: work items summed in a map tile
+    held in LDS) on the benchmark observation's CAR 0.5 arcmin map and on a
+    HEALPix NEST nside 4096 map, ms per call (the output map is not zeroed
+    in the timed calls)."""
+    obs = workloads.Observation(n_det, n_t, 1.5, 0.5, seed=0)
+    bore = torch.from_numpy(obs.bore).to(dev)
+    dets = torch.from_numpy(obs.dets).to(dev)
+    resp = torch.from_numpy(obs.resps).to(dev)
+    g = torch.Generator(device=dev).manual_seed(2)
+    sig = torch.randn((n_det, n_t), dtype=torch.float32, device=dev, generator=g)
+    out = {'workload': f'{n_det} det x {n_t} samples, tiled plans (layout 1)'}
+    car = so3g_amd.ProjEng_CAR_TQU_NonTiled(tuple(obs.projectionist()._get_pixelizor_args()))
+    hp = so3g_amd.ProjEng_HP_TQU_NonTiled((4096, True))
+    for name, eng, shape in (('car', car, tuple(obs.shape)), ('healpix_nest_4096', hp,
+                                                             (12 * 4096 ** 2,))):
+        pix, spin = eng.pointing_matrix(bore, dets, resp)
+        plan = so3g_amd.PrecompPlan(pix, spin, shape, layout=1)
+        del pix, spin
+        m = torch.zeros((3,) + shape, dtype=torch.float64, device=dev)
+        plan.to_map(m, sig)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            plan.to_map(m, sig)
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = {'ms_to_map': e0.elapsed_time(e1) / reps, 'items': plan.info()['items']}
+        del plan, m
+        torch.cuda.empty_cache()
     return out
 
 
