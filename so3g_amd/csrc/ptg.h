@@ -186,6 +186,16 @@ __device__ __forceinline__ int64_t hp_spread(int64_t v) {
     return r;
 }
 
+// The same interleave of a value below 2^16 in 32-bit arithmetic.
+__device__ __forceinline__ uint32_t hp_spread16(uint32_t v) {
+    uint32_t r = v & 0xffffu;
+    r = (r ^ (r << 8)) & 0x00ff00ffu;
+    r = (r ^ (r << 4)) & 0x0f0f0f0fu;
+    r = (r ^ (r << 2)) & 0x33333333u;
+    r = (r ^ (r << 1)) & 0x55555555u;
+    return r;
+}
+
 __device__ __forceinline__ int64_t hp_ang2pix(int64_t ns, bool nest, double theta,
                                               double phi) {
     // sin and cos from one argument reduction: the same reduction and
@@ -195,11 +205,13 @@ __device__ __forceinline__ int64_t hp_ang2pix(int64_t ns, bool nest, double thet
     if (s < 0) { s = -s; phi += kPi; }
     // Continuous face coordinates.
     const double za = fabs(z);
+    // phi is a longitude, so these truncations fit 32 bits (the reference
+    // truncates to 64 bits: the same values)
     double x = phi * (1. / (2. * kPi));
     if (x < 0.)
-        x += (int64_t)x + 1.;
+        x += (int)x + 1.;
     else if (x >= 1.)
-        x -= (int64_t)x;
+        x -= (int)x;
     const double tt = 4. * x;
     double fx, fy;
     int32_t face;
@@ -214,7 +226,7 @@ __device__ __forceinline__ int64_t hp_ang2pix(int64_t ns, bool nest, double thet
         fy = 1 + ifp - jp;
         face = (ifp == ifm) ? (ifp | 4) : ((ifp < ifm) ? ifp : (ifm + 8));
     } else {
-        int64_t ntt = (int64_t)tt;
+        int ntt = (int)tt;
         if (ntt >= 4) ntt = 3;
         const double tp = tt - ntt;
         const double tmp = s / sqrt((1. + za) * (1. / 3.));
@@ -225,9 +237,15 @@ __device__ __forceinline__ int64_t hp_ang2pix(int64_t ns, bool nest, double thet
         if (z >= 0) { fx = 1. - jm; fy = 1. - jp; face = (int32_t)ntt; }
         else { fx = jp; fy = jm; face = (int32_t)ntt + 8; }
     }
-    const int64_t ix = (int64_t)(fx * ns);
-    const int64_t iy = (int64_t)(fy * ns);
-    if (nest) return (face * ns * ns) + hp_spread(ix) + (hp_spread(iy) << 1);
+    // 0 <= fx, fy <= 1 and nside <= 2^29: 32-bit truncations
+    const int64_t ix = (int32_t)(fx * ns);
+    const int64_t iy = (int32_t)(fy * ns);
+    if (nest) {
+        if (ns <= 65536)   // face coordinates below 2^16: 32-bit interleave
+            return (face * ns * ns) +
+                   (int64_t)(hp_spread16((uint32_t)ix) | (hp_spread16((uint32_t)iy) << 1));
+        return (face * ns * ns) + hp_spread(ix) + (hp_spread(iy) << 1);
+    }
     // Ring index.  Face row jrll = {2,2,2,2,3,3,3,3,4,4,4,4} and face
     // column jpll = {1,3,5,7,0,2,4,6,1,3,5,7} (odd for the polar faces,
     // even for the equatorial ones).
