@@ -756,6 +756,15 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
         a.hw.dets = per_block;
         a.hw.span = span;
         a.hw.r = g->interp == 0 ? lane_r : 1;
+        if (cls == 1 && g_opt.hash_span == 0) {
+            // large tables: as for the rows, about 8 blocks per compute unit
+            // (spans of 4096 to 32768 samples, a multiple of 1024 lanes x 4)
+            const int64_t want = 8LL * g_state.n_cu;
+            const int64_t ng = (n_det + per_block - 1) / per_block;
+            int64_t sp = (n_t * ng + want - 1) / want;
+            sp = (sp + 4095) / 4096 * 4096;
+            a.hw.span = (int32_t)std::min<int64_t>(std::max<int64_t>(sp, 4096), span);
+        }
         // the 1-D grid of (group, span) blocks stays below 2^31
         while (((n_t + a.hw.span - 1) / a.hw.span) * ((n_det + a.hw.dets - 1) / a.hw.dets) >
                INT32_MAX)
