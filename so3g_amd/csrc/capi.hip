@@ -173,6 +173,8 @@ int geom_to_launch(const so3g_geom *g, Geo &geo, int &pk) {
         geo.nx = g->naxis[1];
         geo.cdelt_y = g->cdelt[0];
         geo.cdelt_x = g->cdelt[1];
+        geo.icdelt_y = 1. / g->cdelt[0];
+        geo.icdelt_x = 1. / g->cdelt[1];
         geo.crpix_y = g->crpix[0];
         geo.crpix_x = g->crpix[1];
         geo.npix = (int64_t)geo.ny * geo.nx;

@@ -168,16 +168,18 @@ __global__ __launch_bounds__(kBlock) void k_coords(PtgIn in, double *out) {
 // pixels() and pointing_matrix() in one kernel (spin == nullptr for
 // pixels()).  Off-map samples write slot 0 = -1 and raise FLAG_OFFMAP;
 // their other slots are filled afterwards by k_fill_stale.
-template <int PROJ, int PK, int NC>
+template <int PROJ, int PK, int NC, bool SPIN>
+// SPIN: pointing_matrix() (spin factors written) rather than pixels(); a
+// template flag, so that pixels() does not compute the spin terms.
 __global__ __launch_bounds__(kBlock) void k_pixels(PtgIn in, Geo g, const float *resp,
                                                    int32_t *pix, float *spin, int *flags) {
     constexpr int K = index_count<PK>();
     for_samples(in, [&](int64_t d, int64_t t, const Q4 &qb) {
         const Q4 qd = load_q(in.dets + 4 * d);
         double co[4];
-        get_coords<PROJ>(qb, qd, in.tabs, co);
+        get_coords<PROJ, SPIN>(qb, qd, in.tabs, co);
         int32_t pi[3] = {0, 0, 0};
-        const bool on = get_pixel<PK>(g, co, pi);
+        const bool on = get_pixel<PK, SPIN>(g, co, pi);
         int32_t *po = pix + (d * in.n_t + t) * K;
         if (on) {
 #pragma unroll
@@ -186,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_pixels(PtgIn in, Geo g, const float 
             po[0] = -1;
             raise_flag(flags, FLAG_OFFMAP);
         }
-        if (spin != nullptr) {
+        if constexpr (SPIN) {
             float pf[3];
             spin_factors(NC, co, resp[2 * d], resp[2 * d + 1], pf);
             float *so = spin + (d * in.n_t + t) * NC;

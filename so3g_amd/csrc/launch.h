@@ -244,19 +244,24 @@ int launch_pk(const OpArgs &a) {
     const dim3 block(kBlock);
     if (a.op == OP_PIXELS || a.op == OP_PMAT) {
         float *spin = (a.op == OP_PMAT) ? a.spin_out : nullptr;
-        switch (a.nc) {
-        case 1:
-            hipLaunchKernelGGL((k_pixels<PROJ, PK, 1>), grid, block, 0, a.stream, a.in, a.g,
-                               a.resp, a.pix_out, spin, a.flags);
-            break;
-        case 2:
-            hipLaunchKernelGGL((k_pixels<PROJ, PK, 2>), grid, block, 0, a.stream, a.in, a.g,
-                               a.resp, a.pix_out, spin, a.flags);
-            break;
-        default:
-            hipLaunchKernelGGL((k_pixels<PROJ, PK, 3>), grid, block, 0, a.stream, a.in, a.g,
-                               a.resp, a.pix_out, spin, a.flags);
-            break;
+        if (spin == nullptr) {
+            hipLaunchKernelGGL((k_pixels<PROJ, PK, 1, false>), grid, block, 0, a.stream, a.in,
+                               a.g, a.resp, a.pix_out, spin, a.flags);
+        } else {
+            switch (a.nc) {
+            case 1:
+                hipLaunchKernelGGL((k_pixels<PROJ, PK, 1, true>), grid, block, 0, a.stream,
+                                   a.in, a.g, a.resp, a.pix_out, spin, a.flags);
+                break;
+            case 2:
+                hipLaunchKernelGGL((k_pixels<PROJ, PK, 2, true>), grid, block, 0, a.stream,
+                                   a.in, a.g, a.resp, a.pix_out, spin, a.flags);
+                break;
+            default:
+                hipLaunchKernelGGL((k_pixels<PROJ, PK, 3, true>), grid, block, 0, a.stream,
+                                   a.in, a.g, a.resp, a.pix_out, spin, a.flags);
+                break;
+            }
         }
         constexpr int K = index_count<PK>();
         if constexpr (K > 1)
