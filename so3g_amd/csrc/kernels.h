@@ -838,8 +838,11 @@ constexpr size_t row_lds_bytes(int nv, int xw) {
 // segment in LDS (or to the map off it).  A wave-level scan over one
 // sample per lane (wave_runs) measured slower: to_map 9.0 vs 7.2 ms,
 // to_weights 8.2 vs 6.9 ms on 7.2e8 samples; R = 8 needs more than 128
-// registers and measured 7.5 / 7.7 ms.
-template <int PROJ, int PK, int NC, bool W, int NT, int R>
+// registers and measured 7.5 / 7.7 ms.  LEAN: the lane's R boresight
+// quaternions are reloaded for each detector (cache hits) instead of held
+// across the detector loop, and the sample loop is not unrolled, which
+// lets the kernel fit 64 registers (k_scatter_row_x2).
+template <int PROJ, int PK, int NC, bool W, int NT, int R, bool LEAN = false>
 __device__ __forceinline__ void row_item(const PtgIn &in, const Geo &g, const MapView<double> &m,
                                          const float *resp, const float *sig,
                                          const float *det_weights, const int32_t *gd, int G,
@@ -898,10 +901,12 @@ __device__ __forceinline__ void row_item(const PtgIn &in, const Geo &g, const Ma
     };
     for (int64_t tb = t_beg; tb < t_end; tb += (int64_t)NT * R) {
         const int64_t t0 = tb + (int64_t)threadIdx.x * R;
-        Q4 qb[R];
+        Q4 qb[LEAN ? 1 : R];
+        if constexpr (!LEAN) {
 #pragma unroll
-        for (int r = 0; r < R; r++)
-            qb[r] = t0 + r < t_end ? load_q(in.bore + 4 * (t0 + r)) : Q4{1., 0., 0., 0.};
+            for (int r = 0; r < R; r++)
+                qb[r] = t0 + r < t_end ? load_q(in.bore + 4 * (t0 + r)) : Q4{1., 0., 0., 0.};
+        }
         for (int j = 0; j < G; j++) {
             const int32_t d = gd[j];
             if (d < 0) break;
@@ -917,11 +922,10 @@ __device__ __forceinline__ void row_item(const PtgIn &in, const Geo &g, const Ma
             double cv[NV];
 #pragma unroll
             for (int c = 0; c < NV; c++) cv[c] = 0.;
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                if (t0 + r >= t_end) break;
+            // one sample: its pointing, then its run sums
+            auto sample = [&](int r, const Q4 &qbr) {
                 double co[4];
-                get_coords<PROJ>(qb[r], qd, in.tabs, co);
+                get_coords<PROJ>(qbr, qd, in.tabs, co);
                 float pf[3];
                 spin_factors(NC, co, r0, r1, pf);
                 // the pixel of get_pixels<PK, 0> as (row, column) of the
@@ -931,9 +935,9 @@ __device__ __forceinline__ void row_item(const PtgIn &in, const Geo &g, const Ma
                 if constexpr (TILED) {
                     gx = int(fx);
                     gy = int(fy);
-                    if (gx < 0 || gx >= g.nx || gy < 0 || gy >= g.ny) continue;
+                    if (gx < 0 || gx >= g.nx || gy < 0 || gy >= g.ny) return;
                 } else {
-                    if (fx < 0 || fx >= g.nx || fy < 0 || fy >= g.ny) continue;
+                    if (fx < 0 || fx >= g.nx || fy < 0 || fy >= g.ny) return;
                     gx = int(fx);
                     gy = int(fy);
                 }
@@ -960,6 +964,19 @@ __device__ __forceinline__ void row_item(const PtgIn &in, const Geo &g, const Ma
                 } else {
 #pragma unroll
                     for (int c = 0; c < NV; c++) cv[c] += v[c];
+                }
+            };
+            if constexpr (LEAN) {
+#pragma unroll 1
+                for (int r = 0; r < R; r++) {
+                    if (t0 + r >= t_end) break;
+                    sample(r, load_q(in.bore + 4 * (t0 + r)));
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if (t0 + r >= t_end) break;
+                    sample(r, qb[r]);
                 }
             }
             emit(cy, cx, con, cv);
@@ -992,20 +1009,43 @@ __device__ __forceinline__ void row_item(const PtgIn &in, const Geo &g, const Ma
 }
 
 // One observation: block = (sample span, group), group fastest.
+template <int PROJ, int PK, int NC, bool W, int NT, int R, bool LEAN>
+__device__ __forceinline__ void scatter_row_block(const PtgIn &in, const Geo &g,
+                                                  const MapView<double> &m, const float *resp,
+                                                  const float *sig, const float *det_weights,
+                                                  const RowWork &rw, int *flags,
+                                                  double *racc) {
+    const int64_t ts = (int64_t)blockIdx.x / rw.n_groups;
+    const int gi = (int)((int64_t)blockIdx.x - ts * rw.n_groups);
+    const int64_t t_beg = ts * rw.span;
+    const int64_t t_end = min(t_beg + (int64_t)rw.span, in.n_t);
+    row_item<PROJ, PK, NC, W, NT, R, LEAN>(in, g, m, resp, sig, det_weights,
+                                           rw.gdet + (int64_t)gi * rw.G, rw.G, rw.grow[gi],
+                                           row_x0(rw.gxc[gi], rw.xw, g.nx), rw.xw, t_beg,
+                                           t_end, flags, racc);
+}
+
 template <int PROJ, int PK, int NC, bool W, int NT, int R>
 __global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<double> m,
                                                     const float *resp, const float *sig,
                                                     const float *det_weights, RowWork rw,
                                                     int *flags) {
     extern __shared__ double racc[];
-    const int64_t ts = (int64_t)blockIdx.x / rw.n_groups;
-    const int gi = (int)((int64_t)blockIdx.x - ts * rw.n_groups);
-    const int64_t t_beg = ts * rw.span;
-    const int64_t t_end = min(t_beg + (int64_t)rw.span, in.n_t);
-    row_item<PROJ, PK, NC, W, NT, R>(in, g, m, resp, sig, det_weights,
-                                     rw.gdet + (int64_t)gi * rw.G, rw.G, rw.grow[gi],
-                                     row_x0(rw.gxc[gi], rw.xw, g.nx), rw.xw, t_beg, t_end,
-                                     flags, racc);
+    scatter_row_block<PROJ, PK, NC, W, NT, R, false>(in, g, m, resp, sig, det_weights, rw,
+                                                     flags, racc);
+}
+
+// The same with at most 64 registers per lane (8 waves per SIMD), so two
+// 1024-thread blocks share a compute unit where their row segments fit
+// its LDS together: CAR 0.5' TQU to_map on 1000 x 720000 samples 7.04 ->
+// 6.4 ms (held at one block per compute unit by 74 registers before).
+template <int PROJ, int PK, int NC, bool W, int NT, int R>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
+k_scatter_row_x2(PtgIn in, Geo g, MapView<double> m, const float *resp, const float *sig,
+                 const float *det_weights, RowWork rw, int *flags) {
+    extern __shared__ double racc[];
+    scatter_row_block<PROJ, PK, NC, W, NT, R, true>(in, g, m, resp, sig, det_weights, rw,
+                                                    flags, racc);
 }
 
 // --------------------------------------------------------------------------

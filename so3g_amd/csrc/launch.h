@@ -95,6 +95,19 @@ void launch_scatter_row(const OpArgs &a) {
     const size_t lds = row_lds_bytes<PK>(NV, a.rw.xw);
     const dim3 grid((unsigned)(((a.in.n_t + a.rw.span - 1) / a.rw.span) * a.rw.n_groups));
     MapView<double> m{(double *)a.map, (double *const *)a.tiles, a.tile_dims};
+    // two blocks per compute unit (160 KB of LDS) where the segments fit:
+    // the 64-register instance; the weights (6 sums per pixel) and CEA
+    // (its correctly rounded atan2) would spill under that cap
+    if constexpr (!W && PROJ != PROJ_CEA) {
+        if (2 * lds <= 160 * 1024) {
+            (void)hipFuncSetAttribute((const void *)k_scatter_row_x2<PROJ, PK, NC, W, NT, 4>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL((k_scatter_row_x2<PROJ, PK, NC, W, NT, 4>), grid, dim3(NT), lds,
+                               a.stream, a.in, a.g, m, a.resp, a.sig_in, a.det_w, a.rw,
+                               a.flags);
+            return;
+        }
+    }
     (void)hipFuncSetAttribute((const void *)k_scatter_row<PROJ, PK, NC, W, NT, 4>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL((k_scatter_row<PROJ, PK, NC, W, NT, 4>), grid, dim3(NT), lds, a.stream,

@@ -61,10 +61,14 @@ __host__ __device__ __forceinline__ double div_rn(double a, double b, double yb)
     double qq = fma(r, yb, q);
     if (!(fabs(a) >= 0x1p-900 && fabs(qq) < 0x1p1000)) {
 #ifdef __HIP_DEVICE_COMPILE__
-        // keeps the division in this rarely taken branch (the compiler
-        // would otherwise compute it for every call and select); not
-        // volatile, so an unused quotient is still removed
-        asm("" : "+v"(a));
+        // an identity lane move (DPP quad_perm [0,1,2,3]) of a: convergent,
+        // so the division that depends on it stays in this rarely taken
+        // branch (the compiler would otherwise compute it for every call
+        // and select), while an unused quotient is still removed
+        const uint64_t u = __builtin_bit_cast(uint64_t, a);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0xE4, 0xF, 0xF, false);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0xE4, 0xF, 0xF, false);
+        a = __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 #endif
         qq = a / b;
     }
