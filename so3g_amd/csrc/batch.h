@@ -122,11 +122,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_to_map_lds(const ObsDev *obs, 
 // fastest within an observation), each summing into a segment of xw columns
 // of its group's home row (row_item), so the whole batch runs in one launch
 // with the single observation's row scatter.
-template <int PROJ, int PK, int NC, int NT, int R>
-__global__ __launch_bounds__(NT) void k_batch_to_map_row(const ObsDev *obs, int n_obs,
-                                                         Tabs tabs, Geo g, MapView<double> m,
-                                                         int G, int span, int xw, int *flags) {
-    extern __shared__ double racc[];
+template <int PROJ, int PK, int NC, int NT, int R, bool LEAN>
+__device__ __forceinline__ void batch_row_block(const ObsDev *obs, int n_obs, const Tabs &tabs,
+                                                const Geo &g, const MapView<double> &m, int G,
+                                                int span, int xw, int *flags, double *racc) {
     const int64_t it = blockIdx.x;
     int lo = 0, hi = n_obs - 1;
     while (lo < hi) {
@@ -141,10 +140,30 @@ __global__ __launch_bounds__(NT) void k_batch_to_map_row(const ObsDev *obs, int 
     const int64_t t_beg = ts * span;
     const int64_t t_end = min(t_beg + (int64_t)span, o.n_t);
     const PtgIn in{o.bore, o.dets, o.n_t, o.n_det, tabs, 1, 0, nullptr};
-    row_item<PROJ, PK, NC, false, NT, R>(in, g, m, o.resp, o.sig_in, o.det_w,
-                                         o.rgdet + gi * G, G, o.rgrow[gi],
-                                         row_x0(o.rgxc[gi], xw, g.nx), xw, t_beg, t_end, flags,
-                                         racc);
+    row_item<PROJ, PK, NC, false, NT, R, LEAN>(in, g, m, o.resp, o.sig_in, o.det_w,
+                                               o.rgdet + gi * G, G, o.rgrow[gi],
+                                               row_x0(o.rgxc[gi], xw, g.nx), xw, t_beg, t_end,
+                                               flags, racc);
+}
+
+template <int PROJ, int PK, int NC, int NT, int R>
+__global__ __launch_bounds__(NT) void k_batch_to_map_row(const ObsDev *obs, int n_obs,
+                                                         Tabs tabs, Geo g, MapView<double> m,
+                                                         int G, int span, int xw, int *flags) {
+    extern __shared__ double racc[];
+    batch_row_block<PROJ, PK, NC, NT, R, false>(obs, n_obs, tabs, g, m, G, span, xw, flags,
+                                                racc);
+}
+
+// The same with at most 64 registers per lane, two blocks per compute unit
+// where their segments fit its LDS together (as k_scatter_row_x2).
+template <int PROJ, int PK, int NC, int NT, int R>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
+k_batch_to_map_row_x2(const ObsDev *obs, int n_obs, Tabs tabs, Geo g, MapView<double> m, int G,
+                      int span, int xw, int *flags) {
+    extern __shared__ double racc[];
+    batch_row_block<PROJ, PK, NC, NT, R, true>(obs, n_obs, tabs, g, m, G, span, xw, flags,
+                                               racc);
 }
 
 template <int PROJ, int PK, int INTERP, int NC>

@@ -136,6 +136,18 @@ int launch_map_ops(const OpArgs &a) {
                 if constexpr ((PK == PK_FLAT || PK == PK_FLAT_TILED) && INTERP == 0) {
                     constexpr int NT = 1024;
                     const size_t lds = row_lds_bytes<PK>(NC, a.rw.xw);
+                    if constexpr (PROJ != PROJ_CEA) {
+                        if (2 * lds <= 160 * 1024) {
+                            (void)hipFuncSetAttribute(
+                                (const void *)k_batch_to_map_row_x2<PROJ, PK, NC, NT, 4>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                            hipLaunchKernelGGL((k_batch_to_map_row_x2<PROJ, PK, NC, NT, 4>),
+                                               bgrid, dim3(NT), lds, a.stream, a.obs, a.n_obs,
+                                               a.in.tabs, a.g, m, a.rw.G, a.rw.span, a.rw.xw,
+                                               a.flags);
+                            return 0;
+                        }
+                    }
                     (void)hipFuncSetAttribute(
                         (const void *)k_batch_to_map_row<PROJ, PK, NC, NT, 4>,
                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
