@@ -41,7 +41,9 @@ namespace {
 constexpr int kDpbCand[] = {1, 2, 4, 8, 16};
 constexpr int kGridMult[] = {2, 4, 8, 16, 0};   // blocks per CU; 0 = one block per item
 constexpr int kSlots[] = {0, 512, 1024};         // LDS table of to_map; 0 = global atomics
-constexpr int kRowWidth[] = {1024, 2048, 4096, 0};   // row segments of to_map; 0 = widest
+// row segments of to_map; 0 = widest, -1 = the widest of which two fit the
+// LDS of a compute unit together (two blocks per compute unit)
+constexpr int kRowWidth[] = {1024, 2048, -1, 4096, 0};
 constexpr int kRowG = 2;                         // detectors per row group
 constexpr int kRowSpan = 65536;                  // samples per row item
 
@@ -443,8 +445,9 @@ int so3g_batch_tune(so3g_batch *b, int op, double *map, double *const *tiles,
     if (op == 1 && b->row_ok) {
         const int wmax = row_width_max(&b->g, b->g.ncomp);
         std::vector<int> ws;
+        const int w2 = (int)(80 * 1024 / (b->g.ncomp * 8 + (b->g.tiled ? 1 : 0))) / 64 * 64;
         for (int w : kRowWidth) {
-            int x = w > 0 ? std::min(w, wmax) : wmax;
+            int x = w > 0 ? std::min(w, wmax) : w < 0 ? std::min(w2, wmax) : wmax;
             x = (int)std::min<int64_t>(x, b->g.naxis[1]);
             x = std::max(x, 64);
             if (std::find(ws.begin(), ws.end(), x) == ws.end()) ws.push_back(x);
