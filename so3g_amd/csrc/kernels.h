@@ -1038,7 +1038,7 @@ __global__ __launch_bounds__(NT) void k_scatter_row(PtgIn in, Geo g, MapView<dou
 // The same with at most 64 registers per lane (8 waves per SIMD), so two
 // 1024-thread blocks share a compute unit where their row segments fit
 // its LDS together: CAR 0.5' TQU to_map on 1000 x 720000 samples 7.04 ->
-// 6.4 ms (held at one block per compute unit by 74 registers before).
+// 6.4 ms (held at one block per compute unit by 94 registers before).
 template <int PROJ, int PK, int NC, bool W, int NT, int R>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 k_scatter_row_x2(PtgIn in, Geo g, MapView<double> m, const float *resp, const float *sig,
