@@ -528,7 +528,7 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *                   consecutive samples per lane, runs summed in the lane;
  *                   mode 1 takes 4 for HEALPix to_map)
  *   "hash_dets", "hash_span"  detectors and samples per block of modes 2
- *                   to 4 (0, the default: 4 x 256, 2 x 32768, and for map
+ *                   to 4 (0, the default: 4 x 256, 2 x up to 131072, and for map
  *                   rows 2 x a span giving about 8 blocks per compute unit,
  *                   4096 to 131072 samples);
  *                   also the detectors per block of the Precomp tables (0:

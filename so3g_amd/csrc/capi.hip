@@ -115,7 +115,7 @@ struct Options {
     // where the geometry allows them.
     int scatter_hash = 1;
     int hash_dets = 0;         // detectors per block in modes 2-4 (0: 4 / 2 / 2)
-    int hash_span = 0;         // samples per block in modes 2-4 (0: 256 / 32768 / 65536)
+    int hash_span = 0;         // samples per block in modes 2-4 (0: 256 / 131072 / 65536)
     int hash_r = 1;            // hash tables: samples per lane (1: wave-level runs, 4)
 };
 Options g_opt;
@@ -699,7 +699,7 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
     // small table (256 threads, 512 slots, home slot = pixel mod 512) sums
     // four row-mates over one 256-sample chunk and flushes consecutive
     // pixels together, and keeps more blocks per compute unit than the
-    // large one (1024 threads, 112 KB, a span of 32768 samples).
+    // large one (1024 threads, 112 KB, a span of up to 131072 samples).
     int cls = 0, per_block = 0, span = 0, lane_r = g_opt.hash_r;
     const int nv_row = op == OP_TO_MAP ? g->ncomp : g->ncomp * (g->ncomp + 1) / 2;
     const bool row_ok = g->pixelizor == SO3G_PIX_FLAT && g->interp == 0 &&
@@ -738,8 +738,11 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
             // to_map: 4 samples per lane with in-lane run sums (13.7 ms
             // against 14.4 with the wave-level scan; with the small tables
             // and to_weights 4 samples per lane overflow the 512 slots and
-            // lose)
-            if (op == OP_TO_MAP) cls = 1, per_block = 2, span = 32768, lane_r = 4;
+            // lose); spans up to 131072 samples (1000 x 720000 on nside
+            // 4096 NEST, 2 detectors per block: 8192 / 16384 / 32768 /
+            // 65536 / 131072 samples 14.5 / 11.7 / 10.7 / 10.3 / 10.1 ms;
+            // 1 or 4 detectors slower at every span)
+            if (op == OP_TO_MAP) cls = 1, per_block = 2, span = 131072, lane_r = 4;
             else cls = 2, per_block = 4, span = 256;
         } else if (op == OP_TO_WEIGHTS) {
             cls = 2, per_block = g->tiled ? 4 : 8, span = 256;
@@ -749,7 +752,7 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
     } else if (g_opt.scatter_hash >= 2) {
         cls = g_opt.scatter_hash == 2 ? 2 : 1;
         per_block = g_opt.hash_dets > 0 ? g_opt.hash_dets : (cls == 2 ? 4 : 2);
-        span = g_opt.hash_span > 0 ? g_opt.hash_span : (cls == 2 ? 256 : 32768);
+        span = g_opt.hash_span > 0 ? g_opt.hash_span : (cls == 2 ? 256 : 131072);
     }
     if (cls > 0 && prec == SO3G_PREC_REF && segs == nullptr && hash_keys_fit(g)) {
         r = scatter_order(g, a, &a.hw.order);
@@ -760,7 +763,7 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
         a.hw.r = g->interp == 0 ? lane_r : 1;
         if (cls == 1 && g_opt.hash_span == 0) {
             // large tables: as for the rows, about 8 blocks per compute unit
-            // (spans of 4096 to 32768 samples, a multiple of 1024 lanes x 4)
+            // (spans of 4096 to 131072 samples, a multiple of 1024 lanes x 4)
             const int64_t want = 8LL * g_state.n_cu;
             const int64_t ng = (n_det + per_block - 1) / per_block;
             int64_t sp = (n_t * ng + want - 1) / want;
