@@ -742,8 +742,12 @@ static int to_map_common(const so3g_geom *g, void *map, void *const *tiles,
             // 4096 NEST, 2 detectors per block: 8192 / 16384 / 32768 /
             // 65536 / 131072 samples 14.5 / 11.7 / 10.7 / 10.3 / 10.1 ms;
             // 1 or 4 detectors slower at every span)
+            // to_weights: the large tables too, one detector per block (its
+            // 6 sums per pixel leave 2048 slots), 4 samples per lane: 13.8
+            // ms with the small tables (4 detectors x 256 samples) against
+            // 11.9; 2 / 4 detectors per block 13.6 / 15.4 ms
             if (op == OP_TO_MAP) cls = 1, per_block = 2, span = 131072, lane_r = 4;
-            else cls = 2, per_block = 4, span = 256;
+            else cls = 1, per_block = 1, span = 131072, lane_r = 4;
         } else if (op == OP_TO_WEIGHTS) {
             cls = 2, per_block = g->tiled ? 4 : 8, span = 256;
         } else if (!g->tiled) {
