@@ -526,7 +526,7 @@ int so3g_nn_to_bf16(const float *x, int64_t n, void *out, void *stream);
  *   "hash_r"        samples per lane of the hash tables in modes 2 and 3
  *                   (1 default: runs summed across the wave; 4: four
  *                   consecutive samples per lane, runs summed in the lane;
- *                   mode 1 takes 4 for HEALPix to_map)
+ *                   mode 1 takes 4 for HEALPix to_map and to_weights)
  *   "hash_dets", "hash_span"  detectors and samples per block of modes 2
  *                   to 4 (0, the default: 4 x 256, 2 x up to 131072, and for map
  *                   rows 2 x a span giving about 8 blocks per compute unit,
